@@ -1,0 +1,277 @@
+// C ABI of the handle API (include/lina_hip.h).  Every entry point catches
+// C++ exceptions and turns them into a negative return code plus a message
+// for lina_last_error(); nothing here exits the process.
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <stdexcept>
+
+#include "../../include/lina_hip.h"
+#include "lina_internal.hpp"
+#include "solver.hpp"
+
+using lina::Csr;
+using lina::Solver;
+
+struct lina_solver {
+    Solver s;
+};
+
+namespace {
+
+int fail(const std::exception& e, int code) {
+    lina::set_error("%s", e.what());
+    return code;
+}
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        return f();
+    } catch (const lina::HipError& e) {
+        return fail(e, LINA_EHIP);
+    } catch (const lina::NumericError& e) {
+        return fail(e, LINA_ENUMERIC);
+    } catch (const std::invalid_argument& e) {
+        return fail(e, LINA_EARG);
+    } catch (const std::out_of_range& e) {
+        return fail(e, LINA_EARG);
+    } catch (const std::runtime_error& e) {
+        return fail(e, LINA_ESETUP);
+    } catch (const std::exception& e) {
+        return fail(e, LINA_ESETUP);
+    }
+}
+
+Csr make_csr(int n, const int* rowptr, const int* col, const double* val) {
+    if (n < 0 || !rowptr || (n > 0 && (!col || !val))) throw std::invalid_argument("null CSR array");
+    Csr A;
+    A.n = n;
+    A.ncols = n;
+    A.rp.assign(rowptr, rowptr + n + 1);
+    if (A.rp[0] != 0) throw std::invalid_argument("rowptr[0] != 0");
+    for (int i = 0; i < n; i++) {
+        if (A.rp[i + 1] <= A.rp[i]) throw std::invalid_argument("every row needs at least one entry");
+    }
+    int nnz = A.rp[n];
+    A.ci.assign(col, col + nnz);
+    A.v.assign(val, val + nnz);
+    for (int i = 0; i < n; i++)
+        for (int p = A.rp[i]; p < A.rp[i + 1]; p++) {
+            if (A.ci[p] < 0 || A.ci[p] >= n) throw std::invalid_argument("column index out of range");
+            if (p > A.rp[i] && A.ci[p] <= A.ci[p - 1]) throw std::invalid_argument("columns not sorted/unique");
+        }
+    return A;
+}
+
+void check_params(const lina_amg_params* p) {
+    if (!p) throw std::invalid_argument("null params");
+    if (p->dof < 1 || p->depth < 2) throw std::invalid_argument("dof >= 1 and depth >= 2 required");
+    if (p->smoother < 0 || p->smoother > 5) throw std::invalid_argument("unknown smoother");
+    if (p->precision != LINA_PREC_FP64 && p->precision != LINA_PREC_MIXED)
+        throw std::invalid_argument("unknown precision");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lina_last_error(void) { return lina::last_error(); }
+
+int lina_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void lina_params_reference(lina_amg_params* p, int dof, int depth) {
+    std::memset(p, 0, sizeof(*p));
+    p->dof = dof;
+    p->depth = depth;
+    p->smoothing_iterations = 2;
+    p->aggr_smoothing_iterations = 3;
+    p->coarsening = LINA_COARSEN_AGGRESSIVE;
+    p->eps_minus = 0.3;
+    p->eps_plus = 0.5;
+    p->matrix_tol = 0.1;
+    p->smoother = LINA_SMOOTHER_SOR;
+    p->omega = 1.6;
+    p->amg_eps = 1e-2;
+    p->amg_max_iter = 100;
+    p->zero_init = 0;
+    p->precision = LINA_PREC_FP64;
+}
+
+void lina_params_vcycle(lina_amg_params* p, int dof, int depth) {
+    lina_params_reference(p, dof, depth);
+    p->smoothing_iterations = 1;
+    p->aggr_smoothing_iterations = 1;
+    p->smoother = LINA_SMOOTHER_MC_SSOR;
+    p->omega = 1.0;
+    p->amg_max_iter = 1;
+    p->zero_init = 1;
+}
+
+lina_solver* lina_amg_create(int n, const int* rowptr, const int* col, const double* val, const lina_amg_params* p,
+                             int upload) {
+    lina_solver* h = nullptr;
+    int rc = guard([&]() {
+        check_params(p);
+        Csr A = make_csr(n, rowptr, col, val);
+        h = new lina_solver();
+        h->s.setup_host(A, *p);
+        if (upload) {
+            if (lina_device_count() < 1) throw lina::HipError("no HIP device available");
+            h->s.upload();
+        }
+        return 0;
+    });
+    if (rc < 0) {
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+void lina_amg_destroy(lina_solver* s) { delete s; }
+
+int lina_amg_set_params(lina_solver* s, const lina_amg_params* p) {
+    return guard([&]() {
+        check_params(p);
+        const lina_amg_params& o = s->s.params();
+        if (p->dof != o.dof || p->depth != o.depth || p->coarsening != o.coarsening ||
+            p->eps_minus != o.eps_minus || p->eps_plus != o.eps_plus || p->matrix_tol != o.matrix_tol)
+            throw std::invalid_argument("setup parameters cannot change after AMG setup");
+        s->s.set_params(*p);
+        return 0;
+    });
+}
+
+int lina_amg_depth(const lina_solver* s) { return (int)s->s.host().lv.size(); }
+
+int lina_amg_level_info(const lina_solver* s, int level, int* fine, int* coarse, int* nnz_a, int* nnz_p, int* nnz_r) {
+    return guard([&]() {
+        const auto& L = s->s.host().lv.at(level);
+        if (fine) *fine = L.A.n;
+        if (coarse) *coarse = L.coarse;
+        if (nnz_a) *nnz_a = L.A.nnz();
+        if (nnz_p) *nnz_p = level > 0 ? L.P.nnz() : 0;
+        if (nnz_r) *nnz_r = level > 0 ? L.R.nnz() : 0;
+        return 0;
+    });
+}
+
+static const Csr& pick(const lina_solver* s, int level, char which) {
+    const auto& H = s->s.host();
+    const auto& L = H.lv.at(level);
+    switch (which) {
+        case 'A': return L.A;
+        case 'P': if (level > 0) return L.P; break;
+        case 'R': if (level > 0) return L.R; break;
+        default: break;
+    }
+    throw std::invalid_argument("unknown matrix selector");
+}
+
+int lina_amg_level_nnz(const lina_solver* s, int level, char which) {
+    return guard([&]() {
+        if (which == 'L' || which == 'U') {
+            size_t N = s->s.host().lu.dense_idx.size();
+            return (int)(N * N);
+        }
+        return pick(s, level, which).nnz();
+    });
+}
+
+int lina_amg_level_copy(const lina_solver* s, int level, char which, int* rowptr, int* col, double* val) {
+    return guard([&]() {
+        if (which == 'L' || which == 'U') {
+            // dense factors of the coarse LU, row-major over the dense rows
+            const auto& lu = s->s.host().lu;
+            const auto& M = which == 'L' ? lu.L : lu.U;
+            std::memcpy(val, M.data(), M.size() * sizeof(double));
+            for (size_t i = 0; i < lu.dense_idx.size(); i++) col[i] = lu.dense_idx[i];
+            rowptr[0] = (int)lu.dense_idx.size();
+            return 0;
+        }
+        const Csr& M = pick(s, level, which);
+        std::memcpy(rowptr, M.rp.data(), (M.n + 1) * sizeof(int));
+        std::memcpy(col, M.ci.data(), M.nnz() * sizeof(int));
+        std::memcpy(val, M.v.data(), M.nnz() * sizeof(double));
+        return 0;
+    });
+}
+
+int lina_amg_labels(const lina_solver* s, int level, int* labels) {
+    return guard([&]() {
+        const auto& L = s->s.host().lv.at(level);
+        if (level == 0) throw std::invalid_argument("level 0 has no labels");
+        std::memcpy(labels, L.labels.data(), L.labels.size() * sizeof(int));
+        return 0;
+    });
+}
+
+int lina_amg_colours(const lina_solver* s, int level, int* colours) {
+    return guard([&]() {
+        const auto& c = s->s.colours(level);
+        std::memcpy(colours, c.data(), c.size() * sizeof(int));
+        int m = 0;
+        for (int v : c) m = v + 1 > m ? v + 1 : m;
+        return m;
+    });
+}
+
+int lina_amg_recompute(lina_solver* s, const double* val) {
+    return guard([&]() {
+        int nnz = s->s.host().lv.back().A.nnz();
+        std::vector<double> v(val, val + nnz);
+        s->s.recompute(v);
+        return 0;
+    });
+}
+
+int lina_pcg(lina_solver* s, double* x, double* b, double tol, int max_iter, int stop, lina_stats* st) {
+    return guard([&]() {
+        if (!s->s.uploaded()) throw std::invalid_argument("solver was created without device upload");
+        return s->s.pcg_host(x, b, tol, max_iter, stop, st);
+    });
+}
+
+int lina_pcg_device(lina_solver* s, double* d_x, double* d_b, double tol, int max_iter, int stop, lina_stats* st) {
+    return guard([&]() {
+        if (!s->s.uploaded()) throw std::invalid_argument("solver was created without device upload");
+        return s->s.pcg_dev(d_x, d_b, tol, max_iter, stop, st);
+    });
+}
+
+int lina_amg_solve(lina_solver* s, const double* f, double* x) {
+    return guard([&]() {
+        if (!s->s.uploaded()) throw std::invalid_argument("solver was created without device upload");
+        return s->s.amg_solve_host(f, x);
+    });
+}
+
+int lina_spmv(lina_solver* s, int level, int precision, const double* x, double* y) {
+    return guard([&]() {
+        if (!s->s.uploaded()) throw std::invalid_argument("solver was created without device upload");
+        return s->s.spmv_host(level, precision, x, y);
+    });
+}
+
+double lina_time_spmv(lina_solver* s, int level, int precision, int reps) {
+    double out = -1.0;
+    int rc = guard([&]() {
+        if (!s->s.uploaded()) throw std::invalid_argument("solver was created without device upload");
+        out = s->s.time_spmv(level, precision, reps);
+        return 0;
+    });
+    return rc < 0 ? -1.0 : out;
+}
+
+double lina_spmv_bytes(const lina_solver* s, int level, int precision) {
+    return s->s.spmv_bytes(level, precision);
+}
+
+}  // extern "C"

@@ -1,0 +1,514 @@
+// Device kernels of liblina_hip (gfx950 / CDNA4, wave64).
+//
+// Every kernel that reproduces a reference loop keeps that loop's order of
+// floating-point operations inside a row: the CSR kernels first stage the
+// products a_ij*x_j of a block of rows in LDS (coalesced streaming of the
+// value and index arrays), then one lane per row adds them up in column
+// order, which is the order of the reference's `sum += A->Values[i][j]*x[...]`
+// loops (LinA/linear_algebra.c:134-184).  The library is compiled with
+// -ffp-contract=off, so no multiply-add is fused behind the reference's back.
+//
+// Reductions (dots, norms) are deterministic: fixed per-block trees, block
+// partials stored by logical block index, a single-block finalisation.
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace lina {
+namespace dev {
+
+constexpr int TPB = 256;           // threads per block for all row kernels
+constexpr int WAVE = 64;
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Remap so that XCD x processes one contiguous range of logical blocks:
+// neighbouring rows (and the x entries they gather) then share an L2.
+__device__ __forceinline__ int xcd_block(int bid, int nblk) {
+    constexpr int X = 8;
+    if (nblk < 2 * X) return bid;
+    int q = nblk / X, r = nblk % X;
+    int x = bid % X, k = bid / X;
+    return x * q + min(x, r) + k;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+// deterministic block sum (fixed shuffle tree, then waves summed in order)
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double ws[TPB / WAVE];
+    v = wave_sum(v);
+    int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    __syncthreads();
+    if (lane == 0) ws[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < TPB / WAVE; i++) s += ws[i];
+    return s;  // valid in thread 0
+}
+
+// ------------------------------------------------------------------ SpMV
+//
+// MODE_SET : y = A x                        (sparse_mult / PCG_mult)
+// MODE_RES : y = b + (-(A x))               (linear_map(R,-1,A,X) on R=b)
+// MODE_ADD : y = y + (A x)                  (linear_map(X, 1,P,c))
+// part != nullptr: block partial of sum_r y_r * w_r (w == nullptr: y_r^2)
+
+template <typename VT, typename XT, int MODE>
+__global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int* __restrict__ blk, const int* __restrict__ rp,
+                                              const int* __restrict__ ci, const VT* __restrict__ v,
+                                              const XT* __restrict__ x, const XT* __restrict__ b,
+                                              XT* __restrict__ y, double* __restrict__ part,
+                                              const XT* __restrict__ w) {
+    __shared__ XT prod[BLOCK_NNZ];
+    const int bb = xcd_block(blockIdx.x, nblk);
+    const int r0 = blk[bb], r1 = blk[bb + 1];
+    const int k0 = rp[r0], k1 = rp[r1];
+    double mine = 0.0;
+    if (k1 - k0 > BLOCK_NNZ) {
+        // one long row (the block builder isolates rows longer than a block):
+        // strided partial sums + tree, the only non-sequential row sum
+        XT s = 0;
+        for (int k = k0 + threadIdx.x; k < k1; k += TPB) s += (XT)v[k] * x[ci[k]];
+        double t = block_sum((double)s);
+        if (threadIdx.x == 0) {
+            XT sv = (XT)t;
+            XT out;
+            if (MODE == MODE_SET) out = sv;
+            else if (MODE == MODE_RES) out = b[r0] + (-sv);
+            else out = y[r0] + sv;
+            y[r0] = out;
+            if (part) mine = (double)out * (double)(w ? w[r0] : out);
+        }
+    } else {
+        for (int k = k0 + threadIdx.x; k < k1; k += TPB) prod[k - k0] = (XT)v[k] * x[ci[k]];
+        __syncthreads();
+        const int r = r0 + threadIdx.x;
+        if (r < r1) {
+            XT s = 0;
+            const int e = rp[r + 1] - k0;
+            for (int k = rp[r] - k0; k < e; k++) s += prod[k];
+            XT out;
+            if (MODE == MODE_SET) out = s;
+            else if (MODE == MODE_RES) out = b[r] + (-s);
+            else out = y[r] + s;
+            y[r] = out;
+            if (part) mine = (double)out * (double)(w ? w[r] : out);
+        }
+    }
+    if (part) {
+        double t = block_sum(mine);
+        if (threadIdx.x == 0) part[bb] = t;
+    }
+}
+
+// ------------------------------------------------------------ reductions
+
+template <typename XT>
+__global__ __launch_bounds__(TPB) void k_dot_part(int n, const XT* __restrict__ a, const XT* __restrict__ b,
+                                                  double* __restrict__ part) {
+    double s = 0.0;
+    for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) s += (double)a[i] * (double)b[i];
+    double t = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// out[slot] = sum(part); op: 0 plain, 1 sqrt, 2 shift (out[slot+1]=out[slot]) then plain
+__global__ __launch_bounds__(TPB) void k_finalize(int np, const double* __restrict__ part, double* __restrict__ scal,
+                                                  int slot, int op) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += TPB) s += part[i];
+    double t = block_sum(s);
+    if (threadIdx.x == 0) {
+        if (op == 2) scal[slot + 1] = scal[slot];
+        scal[slot] = (op == 1) ? sqrt(t) : t;
+    }
+}
+
+// ---------------------------------------------------------- vector ops
+
+template <typename T>
+__global__ void k_fill(int n, T* __restrict__ x, T val) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = val;
+}
+
+template <typename TI, typename TO>
+__global__ void k_convert(int n, const TI* __restrict__ a, TO* __restrict__ b) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (TO)a[i];
+}
+
+template <typename T>
+__global__ void k_copy(int n, const T* __restrict__ a, T* __restrict__ b) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = a[i];
+}
+
+// y = s .* x (sparse_multiplication with the diagonal Signs matrix: 0 + s*x)
+__global__ void k_signs(int n, const double* __restrict__ s, const double* __restrict__ x, double* __restrict__ y) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = 0.0 + s[i] * x[i];
+}
+
+// AMG_fix_isolated (Multigrid.c:1301-1319): Sol[i] = F[i]/a
+template <typename T>
+__global__ void k_fix_isolated(int m, const int* __restrict__ rows, const double* __restrict__ a,
+                               const T* __restrict__ f, T* __restrict__ x) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        int i = rows[k];
+        x[i] = (T)((double)f[i] / a[k]);
+    }
+}
+
+// scale by 1/h (pcg: scalar_mult((double)1/h, ...)), h read from scal[slot]
+// (h==0 -> 1); inverse==0 scales by h itself (the final unscaling)
+__global__ void k_scale_by(int n, double* __restrict__ x, const double* __restrict__ scal, int slot, int inverse) {
+    double h = scal[slot];
+    if (h == 0) h = 1;
+    double f = inverse ? (double)1 / h : h;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] *= f;
+}
+
+// pcg P update: i==0: P = Z; else P = (P*b) + Z with b = s/s_old
+__global__ void k_pcg_p(int n, double* __restrict__ P, const double* __restrict__ Z, const double* __restrict__ scal,
+                        int first) {
+    if (first) {
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) P[i] = Z[i];
+    } else {
+        double b = scal[S_S] / scal[S_SOLD];
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+            double p = P[i] * b;
+            P[i] = p + 1 * Z[i];
+        }
+    }
+}
+
+// pcg: a = s/(P.Q); X += a P; R += (-a) Q; partial |R|^2
+__global__ __launch_bounds__(TPB) void k_pcg_xr(int n, double* __restrict__ X, double* __restrict__ R,
+                                                const double* __restrict__ P, const double* __restrict__ Q,
+                                                const double* __restrict__ scal, double* __restrict__ part) {
+    double a = scal[S_S] / scal[S_PQ];
+    double na = -a;
+    double s = 0.0;
+    for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) {
+        X[i] += a * P[i];
+        double r = R[i] + na * Q[i];
+        R[i] = r;
+        s += r * r;
+    }
+    double t = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// generic x += f*y with f = sign * scal[slot_a] / scal[slot_b] (slot_b < 0: 1)
+__global__ void k_axpy_s(int n, double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ scal,
+                         int slot_a, int slot_b, double sign) {
+    double f = scal[slot_a];
+    if (slot_b >= 0) f = f / scal[slot_b];
+    f = sign * f;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += f * y[i];
+}
+
+// x = a*x + b*y with scalar constants (host-known)
+__global__ void k_axpby(int n, double a, double* __restrict__ x, double b, const double* __restrict__ y) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = a * x[i] + b * y[i];
+}
+
+// ------------------------------------------------------------- smoothers
+
+// Multicolour SOR, one colour: rows of the colour-permuted copy in block
+// range [blk0, blk0+nblk); row r of the copy is original row perm[r].
+// x_i = (1-w) x_i + w*(f_i - sum_{j!=i} a_ij x_j)/a_ii, columns in order
+// (the arithmetic of AMG_SOR_smoother, Multigrid.c:1345-1365).
+template <typename VT, typename XT>
+__global__ __launch_bounds__(TPB) void k_mcsor(int nblk, int blk0, const int* __restrict__ blk,
+                                               const int* __restrict__ rp, const int* __restrict__ ci,
+                                               const VT* __restrict__ v, const int* __restrict__ perm,
+                                               const XT* __restrict__ f, XT* __restrict__ x, XT omega) {
+    __shared__ XT prod[BLOCK_NNZ];
+    const int bb = blk0 + xcd_block(blockIdx.x, nblk);
+    const int r0 = blk[bb], r1 = blk[bb + 1];
+    const int k0 = rp[r0], k1 = rp[r1];
+    if (k1 - k0 > BLOCK_NNZ) {
+        // long row: one thread walks it (exact order)
+        if (threadIdx.x == 0) {
+            int i = perm[r0];
+            XT sum = f[i];
+            XT d = 0;
+            for (int k = k0; k < k1; k++) {
+                int j = ci[k];
+                if (j != i) sum -= (XT)v[k] * x[j];
+                else d = (XT)v[k];
+            }
+            x[i] = ((XT)1 - omega) * x[i] + omega * sum / d;
+        }
+        return;
+    }
+    for (int k = k0 + threadIdx.x; k < k1; k += TPB) prod[k - k0] = (XT)v[k] * x[ci[k]];
+    __syncthreads();
+    const int r = r0 + threadIdx.x;
+    if (r < r1) {
+        const int i = perm[r];
+        XT sum = f[i];
+        XT d = 0;
+        const int e = rp[r + 1];
+        for (int k = rp[r]; k < e; k++) {
+            if (ci[k] != i) sum -= prod[k - k0];
+            else d = (XT)v[k];
+        }
+        x[i] = ((XT)1 - omega) * x[i] + omega * sum / d;
+    }
+}
+
+// Exact lexicographic SOR (the reference's AMG_SOR_smoother) by level
+// scheduling inside one workgroup: rows of one dependency level are
+// independent; j<i reads the new x, j>i the snapshot xo of the sweep start.
+// backward != 0: the reversed sweep (rows n-1..0), j>i new, j<i snapshot.
+template <typename VT, typename XT>
+__global__ __launch_bounds__(1024) void k_sor_lex(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                  const VT* __restrict__ v, int nlv, const int* __restrict__ lv_off,
+                                                  const int* __restrict__ lv_rows, const XT* __restrict__ f,
+                                                  XT* __restrict__ x, XT* __restrict__ xo, XT omega, int sweeps,
+                                                  int backward) {
+    for (int s = 0; s < sweeps; s++) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) xo[i] = x[i];
+        __syncthreads();
+        for (int L = 0; L < nlv; L++) {
+            const int e = lv_off[L + 1];
+            for (int t = lv_off[L] + threadIdx.x; t < e; t += blockDim.x) {
+                const int i = lv_rows[t];
+                XT sum = f[i];
+                XT d = 0;
+                for (int k = rp[i]; k < rp[i + 1]; k++) {
+                    int j = ci[k];
+                    if (j == i) d = (XT)v[k];
+                    else if ((j < i) != (backward != 0)) sum -= (XT)v[k] * x[j];
+                    else sum -= (XT)v[k] * xo[j];
+                }
+                x[i] = ((XT)1 - omega) * x[i] + omega * sum / d;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Jacobi sweeps reading the snapshot xo:
+//   JAC_REF : x_i = (f_i - sum_{j!=i} a_ij xo_j)/a_ii for i < nlim
+//             (AMG_parallel_Jacobi_smoother, Multigrid.c:1367-1400)
+//   JAC_L1  : x_i = xo_i + (f_i - sum_j a_ij xo_j)/sum_j |a_ij|
+template <typename VT, typename XT, int KIND>
+__global__ __launch_bounds__(TPB) void k_jacobi(int nblk, const int* __restrict__ blk, const int* __restrict__ rp,
+                                                const int* __restrict__ ci, const VT* __restrict__ v,
+                                                const XT* __restrict__ f, const XT* __restrict__ xo,
+                                                XT* __restrict__ x, int nlim) {
+    __shared__ XT prod[BLOCK_NNZ];
+    const int bb = xcd_block(blockIdx.x, nblk);
+    const int r0 = blk[bb], r1 = blk[bb + 1];
+    const int k0 = rp[r0], k1 = rp[r1];
+    const bool lds = (k1 - k0 <= BLOCK_NNZ);
+    if (lds) {
+        for (int k = k0 + threadIdx.x; k < k1; k += TPB) prod[k - k0] = (XT)v[k] * xo[ci[k]];
+        __syncthreads();
+    }
+    for (int r = r0 + threadIdx.x; r < r1; r += TPB) {
+        if (KIND == JAC_REF && r >= nlim) continue;
+        if (!lds && threadIdx.x != 0) continue;
+        XT sum = f[r];
+        XT d = 0;
+        for (int k = rp[r]; k < rp[r + 1]; k++) {
+            XT p = lds ? prod[k - k0] : (XT)v[k] * xo[ci[k]];
+            if (KIND == JAC_REF) {
+                if (ci[k] != r) sum -= p;
+                else d = (XT)v[k];
+            } else {
+                sum -= p;
+                d += fabs((XT)v[k]);
+            }
+        }
+        x[r] = (KIND == JAC_REF) ? sum / d : xo[r] + sum / d;
+    }
+}
+
+// ------------------------------------------------------ coarse solve
+
+// x = inv * b over the dense rows (gathered through idx), trivial rows
+// x_i = b_i / diag_i.  One wave per dense row.
+template <typename XT>
+__global__ __launch_bounds__(TPB) void k_coarse(int nd, const int* __restrict__ didx, const double* __restrict__ inv,
+                                                int nt, const int* __restrict__ tidx, const double* __restrict__ tdiag,
+                                                const XT* __restrict__ b, XT* __restrict__ x) {
+    const int wave = (blockIdx.x * TPB + threadIdx.x) / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    const int nw = gridDim.x * TPB / WAVE;
+    for (int row = wave; row < nd; row += nw) {
+        const double* a = inv + (size_t)row * nd;
+        double s = 0.0;
+        for (int c = lane; c < nd; c += WAVE) s += a[c] * (double)b[didx[c]];
+        s = wave_sum(s);
+        if (lane == 0) x[didx[row]] = (XT)s;
+    }
+    for (int t = blockIdx.x * TPB + threadIdx.x; t < nt; t += gridDim.x * TPB) {
+        int i = tidx[t];
+        x[i] = (XT)((double)b[i] / tdiag[t]);
+    }
+}
+
+// ---------------------------------------------------------- launchers
+
+static inline int grid_for(int n, int cap = 4096) {
+    int g = (n + TPB - 1) / TPB;
+    if (g < 1) g = 1;
+    return g > cap ? cap : g;
+}
+
+template <typename VT, typename XT>
+void spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w) {
+    if (A.nblk == 0) return;
+    dim3 g(A.nblk), t(TPB);
+    switch (mode) {
+        case MODE_SET:
+            hipLaunchKernelGGL((k_spmv<VT, XT, MODE_SET>), g, t, 0, st, A.nblk, A.blk, A.rp, A.ci, A.v, x, b, y, part, w);
+            break;
+        case MODE_RES:
+            hipLaunchKernelGGL((k_spmv<VT, XT, MODE_RES>), g, t, 0, st, A.nblk, A.blk, A.rp, A.ci, A.v, x, b, y, part, w);
+            break;
+        default:
+            hipLaunchKernelGGL((k_spmv<VT, XT, MODE_ADD>), g, t, 0, st, A.nblk, A.blk, A.rp, A.ci, A.v, x, b, y, part, w);
+    }
+}
+template void spmv<double, double>(hipStream_t, const DMat<double>&, const double*, const double*, double*, int,
+                                   double*, const double*);
+template void spmv<float, float>(hipStream_t, const DMat<float>&, const float*, const float*, float*, int, double*,
+                                 const float*);
+
+template <typename XT>
+int dot_part(hipStream_t st, int n, const XT* a, const XT* b, double* part) {
+    int g = grid_for(n, 1024);
+    hipLaunchKernelGGL((k_dot_part<XT>), dim3(g), dim3(TPB), 0, st, n, a, b, part);
+    return g;
+}
+template int dot_part<double>(hipStream_t, int, const double*, const double*, double*);
+template int dot_part<float>(hipStream_t, int, const float*, const float*, double*);
+
+void finalize(hipStream_t st, int np, const double* part, double* scal, int slot, int op) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(TPB), 0, st, np, part, scal, slot, op);
+}
+
+template <typename T>
+void fill(hipStream_t st, int n, T* x, T val) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(n)), dim3(TPB), 0, st, n, x, val);
+}
+template void fill<double>(hipStream_t, int, double*, double);
+template void fill<float>(hipStream_t, int, float*, float);
+
+template <typename TI, typename TO>
+void convert(hipStream_t st, int n, const TI* a, TO* b) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_convert<TI, TO>), dim3(grid_for(n)), dim3(TPB), 0, st, n, a, b);
+}
+template void convert<double, float>(hipStream_t, int, const double*, float*);
+template void convert<float, double>(hipStream_t, int, const float*, double*);
+
+template <typename T>
+void copy(hipStream_t st, int n, const T* a, T* b) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_copy<T>), dim3(grid_for(n)), dim3(TPB), 0, st, n, a, b);
+}
+template void copy<double>(hipStream_t, int, const double*, double*);
+template void copy<float>(hipStream_t, int, const float*, float*);
+
+void signs(hipStream_t st, int n, const double* s, const double* x, double* y) {
+    hipLaunchKernelGGL(k_signs, dim3(grid_for(n)), dim3(TPB), 0, st, n, s, x, y);
+}
+
+template <typename T>
+void fix_isolated(hipStream_t st, int m, const int* rows, const double* a, const T* f, T* x) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL((k_fix_isolated<T>), dim3(grid_for(m)), dim3(TPB), 0, st, m, rows, a, f, x);
+}
+template void fix_isolated<double>(hipStream_t, int, const int*, const double*, const double*, double*);
+template void fix_isolated<float>(hipStream_t, int, const int*, const double*, const float*, float*);
+
+void scale_by(hipStream_t st, int n, double* x, const double* scal, int slot, int inverse) {
+    hipLaunchKernelGGL(k_scale_by, dim3(grid_for(n)), dim3(TPB), 0, st, n, x, scal, slot, inverse);
+}
+
+void pcg_p(hipStream_t st, int n, double* P, const double* Z, const double* scal, int first) {
+    hipLaunchKernelGGL(k_pcg_p, dim3(grid_for(n)), dim3(TPB), 0, st, n, P, Z, scal, first);
+}
+
+int pcg_xr(hipStream_t st, int n, double* X, double* R, const double* P, const double* Q, const double* scal,
+           double* part) {
+    int g = grid_for(n, 1024);
+    hipLaunchKernelGGL(k_pcg_xr, dim3(g), dim3(TPB), 0, st, n, X, R, P, Q, scal, part);
+    return g;
+}
+
+void axpy_s(hipStream_t st, int n, double* x, const double* y, const double* scal, int slot_a, int slot_b,
+            double sign) {
+    hipLaunchKernelGGL(k_axpy_s, dim3(grid_for(n)), dim3(TPB), 0, st, n, x, y, scal, slot_a, slot_b, sign);
+}
+
+void axpby(hipStream_t st, int n, double a, double* x, double b, const double* y) {
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(TPB), 0, st, n, a, x, b, y);
+}
+
+template <typename VT, typename XT>
+void mcsor_colour(hipStream_t st, const DMat<VT>& Ac, int blk0, int nblk, const int* perm, const XT* f, XT* x,
+                  XT omega) {
+    if (nblk <= 0) return;
+    hipLaunchKernelGGL((k_mcsor<VT, XT>), dim3(nblk), dim3(TPB), 0, st, nblk, blk0, Ac.blk, Ac.rp, Ac.ci, Ac.v, perm,
+                       f, x, omega);
+}
+template void mcsor_colour<double, double>(hipStream_t, const DMat<double>&, int, int, const int*, const double*,
+                                           double*, double);
+template void mcsor_colour<float, float>(hipStream_t, const DMat<float>&, int, int, const int*, const float*, float*,
+                                         float);
+
+template <typename VT, typename XT>
+void sor_lex(hipStream_t st, const DMat<VT>& A, int nlv, const int* lv_off, const int* lv_rows, const XT* f, XT* x,
+             XT* xo, XT omega, int sweeps, int backward) {
+    if (A.n == 0) return;
+    hipLaunchKernelGGL((k_sor_lex<VT, XT>), dim3(1), dim3(1024), 0, st, A.n, A.rp, A.ci, A.v, nlv, lv_off, lv_rows,
+                       f, x, xo, omega, sweeps, backward);
+}
+template void sor_lex<double, double>(hipStream_t, const DMat<double>&, int, const int*, const int*, const double*,
+                                      double*, double*, double, int, int);
+template void sor_lex<float, float>(hipStream_t, const DMat<float>&, int, const int*, const int*, const float*,
+                                    float*, float*, float, int, int);
+
+template <typename VT, typename XT>
+void jacobi(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, const XT* xo, XT* x, int nlim) {
+    if (A.nblk == 0) return;
+    if (kind == JAC_REF)
+        hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_REF>), dim3(A.nblk), dim3(TPB), 0, st, A.nblk, A.blk, A.rp, A.ci,
+                           A.v, f, xo, x, nlim);
+    else
+        hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_L1>), dim3(A.nblk), dim3(TPB), 0, st, A.nblk, A.blk, A.rp, A.ci,
+                           A.v, f, xo, x, nlim);
+}
+template void jacobi<double, double>(hipStream_t, const DMat<double>&, int, const double*, const double*, double*,
+                                     int);
+template void jacobi<float, float>(hipStream_t, const DMat<float>&, int, const float*, const float*, float*, int);
+
+template <typename XT>
+void coarse_solve(hipStream_t st, int nd, const int* didx, const double* inv, int nt, const int* tidx,
+                  const double* tdiag, const XT* b, XT* x) {
+    int rows_blocks = (nd * WAVE + TPB - 1) / TPB;
+    int tb = (nt + TPB - 1) / TPB;
+    int g = rows_blocks > tb ? rows_blocks : tb;
+    if (g < 1) g = 1;
+    if (g > 2048) g = 2048;
+    hipLaunchKernelGGL((k_coarse<XT>), dim3(g), dim3(TPB), 0, st, nd, didx, inv, nt, tidx, tdiag, b, x);
+}
+template void coarse_solve<double>(hipStream_t, int, const int*, const double*, int, const int*, const double*,
+                                   const double*, double*);
+template void coarse_solve<float>(hipStream_t, int, const int*, const double*, int, const int*, const double*,
+                                  const float*, float*);
+
+}  // namespace dev
+}  // namespace lina

@@ -1,0 +1,79 @@
+// Launch interface of the device kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace lina {
+namespace dev {
+
+// products of one row block staged in LDS: at most BLOCK_NNZ entries and
+// at most BLOCK_ROWS rows per block (one lane per row in the second phase)
+constexpr int BLOCK_NNZ = 2048;
+constexpr int BLOCK_ROWS = 256;
+
+enum { MODE_SET = 0, MODE_RES = 1, MODE_ADD = 2 };
+enum { JAC_REF = 0, JAC_L1 = 1 };
+
+// device scalar slots used by the Krylov drivers
+enum {
+    S_S = 0,      // s = R.Z       (S_SOLD = S_S + 1 by k_finalize's shift)
+    S_SOLD = 1,
+    S_PQ = 2,     // P.Q
+    S_R = 3,      // |R|
+    S_H = 4,      // |B|
+    S_R0 = 5,     // r0
+    S_T0 = 6,     // scratch slots
+    S_T1 = 7,
+    S_T2 = 8,
+    S_T3 = 9,
+    S_T4 = 10,
+    S_T5 = 11,
+    S_COUNT = 16
+};
+
+// device CSR with its LDS row-block schedule
+template <typename VT>
+struct DMat {
+    int n = 0, ncols = 0, nnz = 0;
+    int* rp = nullptr;
+    int* ci = nullptr;
+    VT* v = nullptr;
+    int nblk = 0;
+    int* blk = nullptr;  // nblk+1 row starts
+};
+
+template <typename VT, typename XT>
+void spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w);
+template <typename XT>
+int dot_part(hipStream_t st, int n, const XT* a, const XT* b, double* part);
+void finalize(hipStream_t st, int np, const double* part, double* scal, int slot, int op);
+template <typename T>
+void fill(hipStream_t st, int n, T* x, T val);
+template <typename TI, typename TO>
+void convert(hipStream_t st, int n, const TI* a, TO* b);
+template <typename T>
+void copy(hipStream_t st, int n, const T* a, T* b);
+void signs(hipStream_t st, int n, const double* s, const double* x, double* y);
+template <typename T>
+void fix_isolated(hipStream_t st, int m, const int* rows, const double* a, const T* f, T* x);
+void scale_by(hipStream_t st, int n, double* x, const double* scal, int slot, int inverse);
+void pcg_p(hipStream_t st, int n, double* P, const double* Z, const double* scal, int first);
+int pcg_xr(hipStream_t st, int n, double* X, double* R, const double* P, const double* Q, const double* scal,
+           double* part);
+void axpy_s(hipStream_t st, int n, double* x, const double* y, const double* scal, int slot_a, int slot_b,
+            double sign);
+void axpby(hipStream_t st, int n, double a, double* x, double b, const double* y);
+template <typename VT, typename XT>
+void mcsor_colour(hipStream_t st, const DMat<VT>& Ac, int blk0, int nblk, const int* perm, const XT* f, XT* x,
+                  XT omega);
+template <typename VT, typename XT>
+void sor_lex(hipStream_t st, const DMat<VT>& A, int nlv, const int* lv_off, const int* lv_rows, const XT* f, XT* x,
+             XT* xo, XT omega, int sweeps, int backward);
+template <typename VT, typename XT>
+void jacobi(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, const XT* xo, XT* x, int nlim);
+template <typename XT>
+void coarse_solve(hipStream_t st, int nd, const int* didx, const double* inv, int nt, const int* tidx,
+                  const double* tdiag, const XT* b, XT* x);
+
+}  // namespace dev
+}  // namespace lina

@@ -1,0 +1,622 @@
+// Device-resident AMG hierarchy and the Krylov drivers of liblina_hip.
+//
+// The host setup (amg_setup.cpp) produces the hierarchy; this file uploads it
+// once (CSR values in FP64 and an FP32 copy, LDS row-block schedules,
+// colour permutations, level schedules) and runs the reference's solve
+// algorithms with every vector operation on the device:
+//
+//   pcg        PCGS.c:163-233
+//   AMG_solve  Multigrid.c:1863-1931
+//   V-cycle    Multigrid.c:1536-1580
+//
+// The host only reads back the scalars that drive control flow (|R| per
+// Krylov iteration, |f-Ax| per V-cycle).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lina_hip.h"
+#include "kernels.hpp"
+#include "lina_internal.hpp"
+#include "solver.hpp"
+
+namespace lina {
+
+// ------------------------------------------------------------ errors
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+const char* last_error() { return g_err.c_str(); }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------ memory
+
+template <typename T>
+T* dalloc(size_t n, DevBuffers& owner) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+    owner.ptrs.push_back(p);
+    return (T*)p;
+}
+
+template <typename T>
+T* dupload(const std::vector<T>& h, DevBuffers& owner, hipStream_t st) {
+    T* d = dalloc<T>(h.size(), owner);
+    if (!h.empty()) hip_check(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st), "upload");
+    return d;
+}
+
+DevBuffers::~DevBuffers() {
+    for (void* p : ptrs) (void)hipFree(p);
+}
+
+// row-block schedule over rows [begin, end): at most BLOCK_NNZ products and
+// BLOCK_ROWS rows per block; a row longer than BLOCK_NNZ gets a block alone
+static void build_blocks(const Csr& A, int begin, int end, std::vector<int>& starts) {
+    int r = begin;
+    int nnz = 0, rows = 0;
+    if (begin >= end) return;
+    starts.push_back(begin);
+    while (r < end) {
+        int len = A.rp[r + 1] - A.rp[r];
+        if (rows > 0 && (nnz + len > dev::BLOCK_NNZ || rows == dev::BLOCK_ROWS)) {
+            starts.push_back(r);
+            nnz = 0;
+            rows = 0;
+        }
+        nnz += len;
+        rows++;
+        r++;
+    }
+}
+
+template <typename VT>
+static void upload_mat(const Csr& A, const std::vector<int>& starts_with_end, dev::DMat<VT>& D, DevBuffers& B,
+                       hipStream_t st, const int* shared_rp = nullptr, const int* shared_ci = nullptr,
+                       const int* shared_blk = nullptr) {
+    D.n = A.n;
+    D.ncols = A.ncols;
+    D.nnz = A.nnz();
+    D.rp = shared_rp ? (int*)shared_rp : dupload(A.rp, B, st);
+    D.ci = shared_ci ? (int*)shared_ci : dupload(A.ci, B, st);
+    std::vector<VT> v(A.v.begin(), A.v.end());
+    D.v = dupload(v, B, st);
+    D.nblk = (int)starts_with_end.size() - 1;
+    if (D.nblk < 0) D.nblk = 0;
+    D.blk = shared_blk ? (int*)shared_blk : dupload(starts_with_end, B, st);
+}
+
+// upload A as FP64 + FP32 copies sharing the index arrays
+static void upload_pair(const Csr& A, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st) {
+    std::vector<int> starts;
+    build_blocks(A, 0, A.n, starts);
+    starts.push_back(A.n);
+    if (A.n == 0) starts.assign(1, 0);
+    upload_mat(A, starts, D, B, st);
+    upload_mat(A, starts, F, B, st, D.rp, D.ci, D.blk);
+}
+
+// level schedule of the lexicographic sweep: forward, level(i) = 1 + max over
+// j<i with a_ij != 0; backward, over j>i
+static void level_schedule(const Csr& A, bool backward, std::vector<int>& off, std::vector<int>& rows) {
+    const int n = A.n;
+    std::vector<int> lev(n, 0);
+    int nl = 0;
+    for (int t = 0; t < n; t++) {
+        int i = backward ? n - 1 - t : t;
+        int l = 0;
+        for (int p = A.rp[i]; p < A.rp[i + 1]; p++) {
+            int j = A.ci[p];
+            if (A.v[p] == 0.0) continue;
+            if ((!backward && j < i) || (backward && j > i)) l = std::max(l, lev[j] + 1);
+        }
+        lev[i] = l;
+        nl = std::max(nl, l + 1);
+    }
+    off.assign(nl + 1, 0);
+    for (int i = 0; i < n; i++) off[lev[i] + 1]++;
+    for (int l = 0; l < nl; l++) off[l + 1] += off[l];
+    rows.assign(n, 0);
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int i = 0; i < n; i++) rows[fill[lev[i]]++] = i;
+}
+
+// ------------------------------------------------------------ Solver
+
+Solver::Solver() {}
+
+Solver::~Solver() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (host_scal_) (void)hipHostFree(host_scal_);
+    bufs_.reset();
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Solver::setup_host(const Csr& A, const lina_amg_params& p) {
+    prm_ = p;
+    SetupParams sp;
+    sp.dof = p.dof;
+    sp.depth = p.depth;
+    sp.smoothing_iterations = p.smoothing_iterations;
+    sp.aggr_smoothing_iterations = p.aggr_smoothing_iterations;
+    sp.coarsening = p.coarsening;
+    sp.eps_minus = p.eps_minus;
+    sp.eps_plus = p.eps_plus;
+    sp.matrix_tol = p.matrix_tol;
+    amg_setup(A, sp, H_);
+    n_ = A.n;
+    colours_.assign(H_.lv.size(), std::vector<int>());
+    ncolours_.assign(H_.lv.size(), 0);
+    for (size_t l = 1; l < H_.lv.size(); l++) colours_[l] = greedy_colouring(H_.lv[l].A, &ncolours_[l]);
+}
+
+void Solver::upload() {
+    if (!stream_) {
+        hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreate(&ev0_), "hipEventCreate");
+        hip_check(hipEventCreate(&ev1_), "hipEventCreate");
+        hip_check(hipHostMalloc((void**)&host_scal_, dev::S_COUNT * sizeof(double)), "hipHostMalloc");
+    }
+    bufs_.reset(new DevBuffers());
+    DevBuffers& B = *bufs_;
+    hipStream_t st = stream_;
+    const int depth = (int)H_.lv.size();
+    L_.assign(depth, DevLevel());
+    int maxn = 1;
+    for (int l = 0; l < depth; l++) {
+        const HostLevel& h = H_.lv[l];
+        DevLevel& d = L_[l];
+        d.n = h.A.n;
+        maxn = std::max(maxn, d.n);
+        upload_pair(h.A, d.A, d.Af, B, st);
+        d.x = dalloc<double>(d.n, B);
+        d.f = dalloc<double>(d.n, B);
+        d.r = dalloc<double>(d.n, B);
+        d.xo = dalloc<double>(d.n, B);
+        d.xf = dalloc<float>(d.n, B);
+        d.ff = dalloc<float>(d.n, B);
+        d.rf = dalloc<float>(d.n, B);
+        d.xof = dalloc<float>(d.n, B);
+        d.nlim = (d.n / std::max(1, prm_.dof)) * std::max(1, prm_.dof);
+        if (l == 0) continue;
+        upload_pair(h.P, d.P, d.Pf, B, st);
+        upload_pair(h.R, d.R, d.Rf, B, st);
+        // colour-permuted copy for the multicolour sweeps
+        const std::vector<int>& col = colours_[l];
+        int nc = ncolours_[l];
+        std::vector<int> perm(d.n);
+        {
+            std::vector<int> cnt(nc + 1, 0);
+            for (int i = 0; i < d.n; i++) cnt[col[i] + 1]++;
+            for (int c = 0; c < nc; c++) cnt[c + 1] += cnt[c];
+            std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+            for (int i = 0; i < d.n; i++) perm[fill[col[i]]++] = i;
+            d.colour_rows = cnt;
+        }
+        Csr Ac;
+        Ac.n = d.n;
+        Ac.ncols = d.n;
+        Ac.rp.assign(d.n + 1, 0);
+        Ac.ci.reserve(h.A.nnz());
+        Ac.v.reserve(h.A.nnz());
+        for (int r = 0; r < d.n; r++) {
+            int i = perm[r];
+            for (int p = h.A.rp[i]; p < h.A.rp[i + 1]; p++) {
+                Ac.ci.push_back(h.A.ci[p]);
+                Ac.v.push_back(h.A.v[p]);
+            }
+            Ac.rp[r + 1] = (int)Ac.ci.size();
+        }
+        std::vector<int> starts;
+        d.colour_blk.assign(nc + 1, 0);
+        for (int c = 0; c < nc; c++) {
+            d.colour_blk[c] = (int)starts.size();
+            build_blocks(Ac, d.colour_rows[c], d.colour_rows[c + 1], starts);
+        }
+        d.colour_blk[nc] = (int)starts.size();
+        starts.push_back(d.n);
+        upload_mat(Ac, starts, d.Ac, B, st);
+        upload_mat(Ac, starts, d.Acf, B, st, d.Ac.rp, d.Ac.ci, d.Ac.blk);
+        d.perm = dupload(perm, B, st);
+        d.ncolours = nc;
+        // level schedules of the exact lexicographic sweeps
+        std::vector<int> off, rows;
+        level_schedule(h.A, false, off, rows);
+        d.nlv_f = (int)off.size() - 1;
+        d.lvf_off = dupload(off, B, st);
+        d.lvf_rows = dupload(rows, B, st);
+        level_schedule(h.A, true, off, rows);
+        d.nlv_b = (int)off.size() - 1;
+        d.lvb_off = dupload(off, B, st);
+        d.lvb_rows = dupload(rows, B, st);
+    }
+    // finest-level extras
+    signs_ = dupload(H_.signs, B, st);
+    has_neg_signs_ = false;
+    for (double s : H_.signs)
+        if (s < 0) has_neg_signs_ = true;
+    iso_rows_ = dupload(H_.iso_rows, B, st);
+    iso_a_ = dupload(H_.iso_a, B, st);
+    niso_ = (int)H_.iso_rows.size();
+    // coarse LU inverse
+    const CoarseLU& lu = H_.lu;
+    nd_ = (int)lu.dense_idx.size();
+    nt_ = (int)lu.trivial_idx.size();
+    didx_ = dupload(lu.dense_idx, B, st);
+    inv_ = dupload(lu.inv, B, st);
+    tidx_ = dupload(lu.trivial_idx, B, st);
+    tdiag_ = dupload(lu.trivial_diag, B, st);
+    // Krylov vectors and scratch
+    const int n = n_;
+    ftop_ = dalloc<double>(n, B);
+    rtop_ = dalloc<double>(n, B);
+    kR_ = dalloc<double>(n, B);
+    kP_ = dalloc<double>(n, B);
+    kQ_ = dalloc<double>(n, B);
+    kZ_ = dalloc<double>(n, B);
+    kW_ = dalloc<double>(n, B);
+    kX_ = dalloc<double>(n, B);
+    kB_ = dalloc<double>(n, B);
+    int np = std::max(4096, (maxn + 255) / 256 + 16);
+    part_ = dalloc<double>(np, B);
+    scal_ = dalloc<double>(dev::S_COUNT, B);
+    dev::fill(st, dev::S_COUNT, scal_, 0.0);
+    hip_check(hipStreamSynchronize(st), "upload sync");
+}
+
+void Solver::recompute(const std::vector<double>& val) {
+    Csr A = H_.lv.back().A;  // pattern of the finest (sign-adjusted) matrix
+    if ((int)val.size() != A.nnz()) throw std::invalid_argument("recompute: value count differs from the pattern");
+    for (int i = 0; i < A.n; i++)
+        for (int p = A.rp[i]; p < A.rp[i + 1]; p++) A.v[p] = val[p] * (H_.signs[i] < 0 ? -1.0 : 1.0);
+    amg_recompute(H_, A);
+    for (size_t l = 1; l < H_.lv.size(); l++) colours_[l] = greedy_colouring(H_.lv[l].A, &ncolours_[l]);
+    upload();
+}
+
+double Solver::read_scalar(int slot) {
+    hip_check(hipMemcpyAsync(host_scal_ + slot, scal_ + slot, sizeof(double), hipMemcpyDeviceToHost, stream_), "d2h");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    return host_scal_[slot];
+}
+
+// ----------------------------------------------------------- smoothing
+
+template <typename T>
+static inline const dev::DMat<T>& pickA(const DevLevel& d);
+template <>
+inline const dev::DMat<double>& pickA<double>(const DevLevel& d) { return d.A; }
+template <>
+inline const dev::DMat<float>& pickA<float>(const DevLevel& d) { return d.Af; }
+template <typename T>
+static inline const dev::DMat<T>& pickAc(const DevLevel& d);
+template <>
+inline const dev::DMat<double>& pickAc<double>(const DevLevel& d) { return d.Ac; }
+template <>
+inline const dev::DMat<float>& pickAc<float>(const DevLevel& d) { return d.Acf; }
+template <typename T>
+static inline const dev::DMat<T>& pickP(const DevLevel& d);
+template <>
+inline const dev::DMat<double>& pickP<double>(const DevLevel& d) { return d.P; }
+template <>
+inline const dev::DMat<float>& pickP<float>(const DevLevel& d) { return d.Pf; }
+template <typename T>
+static inline const dev::DMat<T>& pickR(const DevLevel& d);
+template <>
+inline const dev::DMat<double>& pickR<double>(const DevLevel& d) { return d.R; }
+template <>
+inline const dev::DMat<float>& pickR<float>(const DevLevel& d) { return d.Rf; }
+
+template <typename T>
+struct LevelVecs;
+template <>
+struct LevelVecs<double> {
+    static double* x(DevLevel& d) { return d.x; }
+    static double* f(DevLevel& d) { return d.f; }
+    static double* r(DevLevel& d) { return d.r; }
+    static double* xo(DevLevel& d) { return d.xo; }
+};
+template <>
+struct LevelVecs<float> {
+    static float* x(DevLevel& d) { return d.xf; }
+    static float* f(DevLevel& d) { return d.ff; }
+    static float* r(DevLevel& d) { return d.rf; }
+    static float* xo(DevLevel& d) { return d.xof; }
+};
+
+template <typename T>
+void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post) {
+    DevLevel& d = L_[l];
+    hipStream_t st = stream_;
+    const T omega = (T)prm_.omega;
+    switch (prm_.smoother) {
+        case LINA_SMOOTHER_SOR:
+        case LINA_SMOOTHER_SSOR: {
+            bool back = (prm_.smoother == LINA_SMOOTHER_SSOR) && post;
+            if (back)
+                dev::sor_lex(st, pickA<T>(d), d.nlv_b, d.lvb_off, d.lvb_rows, f, x, LevelVecs<T>::xo(d), omega,
+                             sweeps, 1);
+            else
+                dev::sor_lex(st, pickA<T>(d), d.nlv_f, d.lvf_off, d.lvf_rows, f, x, LevelVecs<T>::xo(d), omega,
+                             sweeps, 0);
+            break;
+        }
+        case LINA_SMOOTHER_MC_SOR:
+        case LINA_SMOOTHER_MC_SSOR: {
+            bool back = (prm_.smoother == LINA_SMOOTHER_MC_SSOR) && post;
+            for (int s = 0; s < sweeps; s++)
+                for (int cc = 0; cc < d.ncolours; cc++) {
+                    int c = back ? d.ncolours - 1 - cc : cc;
+                    int b0 = d.colour_blk[c], b1 = d.colour_blk[c + 1];
+                    dev::mcsor_colour(st, pickAc<T>(d), b0, b1 - b0, d.perm, f, x, omega);
+                }
+            break;
+        }
+        case LINA_SMOOTHER_JACOBI:
+        case LINA_SMOOTHER_L1_JACOBI: {
+            int kind = prm_.smoother == LINA_SMOOTHER_JACOBI ? dev::JAC_REF : dev::JAC_L1;
+            for (int s = 0; s < sweeps; s++) {
+                dev::copy(st, d.n, (const T*)x, LevelVecs<T>::xo(d));
+                dev::jacobi(st, pickA<T>(d), kind, f, (const T*)LevelVecs<T>::xo(d), x, d.nlim);
+            }
+            break;
+        }
+        default:
+            throw std::invalid_argument("unknown smoother");
+    }
+}
+
+// AMG_V_cycle (Multigrid.c:1536-1580) with Iter[level] == 1
+template <typename T>
+void Solver::vcycle(int l, const T* f, T* x) {
+    hipStream_t st = stream_;
+    if (l == 0) {
+        dev::coarse_solve(st, nd_, didx_, inv_, nt_, tidx_, tdiag_, f, x);
+        return;
+    }
+    DevLevel& d = L_[l];
+    DevLevel& c = L_[l - 1];
+    const int depth = (int)L_.size();
+    int sw = (prm_.coarsening == AGGRESSIVE && l == depth - 1) ? prm_.aggr_smoothing_iterations
+                                                                : prm_.smoothing_iterations;
+    smooth<T>(l, f, x, sw, false);
+    T* r = LevelVecs<T>::r(d);
+    dev::spmv<T, T>(st, pickA<T>(d), x, f, r, dev::MODE_RES, nullptr, nullptr);
+    T* fc = LevelVecs<T>::f(c);
+    T* xc = LevelVecs<T>::x(c);
+    dev::spmv<T, T>(st, pickR<T>(d), r, nullptr, fc, dev::MODE_SET, nullptr, nullptr);
+    dev::fill<T>(st, c.n, xc, (T)0);
+    vcycle<T>(l - 1, fc, xc);
+    dev::spmv<T, T>(st, pickP<T>(d), xc, nullptr, x, dev::MODE_ADD, nullptr, nullptr);
+    smooth<T>(l, f, x, sw, true);
+}
+
+// |f - A x| on the finest level into scal[slot] (FP64 or the FP32 copy)
+template <typename T>
+double Solver::residual_norm(const T* f, const T* x, T* r, int slot) {
+    DevLevel& d = L_.back();
+    const auto& A = pickA<T>(d);
+    dev::spmv<T, T>(stream_, A, x, f, r, dev::MODE_RES, part_, nullptr);
+    dev::finalize(stream_, A.nblk, part_, scal_, slot, 1);
+    return read_scalar(slot);
+}
+
+// AMG_solve (Multigrid.c:1863-1931) with F, X device vectors of the finest
+// size.  The diagnostic ILU residuals of the reference (res0, res) do not
+// influence the result except through `res0 != 0`, which holds exactly when
+// the first residual r0 is nonzero; they are not computed.
+int Solver::amg_solve_dev(const double* F, double* X) {
+    hipStream_t st = stream_;
+    DevLevel& top = L_.back();
+    const int n = top.n;
+    const double* Ft = F;
+    if (has_neg_signs_) {
+        dev::signs(st, n, signs_, F, ftop_);
+        Ft = ftop_;
+    }
+    dev::fix_isolated<double>(st, niso_, iso_rows_, iso_a_, Ft, X);
+    int it = 0;
+    if (prm_.precision == LINA_PREC_FP64) {
+        double r0 = residual_norm<double>(Ft, X, rtop_, dev::S_T0);
+        if (r0 == 0.0) return 0;
+        double r = r0;
+        while (std::fabs(r / r0) > prm_.amg_eps && it < prm_.amg_max_iter) {
+            vcycle<double>((int)L_.size() - 1, Ft, X);
+            it++;
+            if (it < prm_.amg_max_iter) r = residual_norm<double>(Ft, X, rtop_, dev::S_T0);
+        }
+    } else {
+        float* ff = top.ff;
+        float* xf = top.xf;
+        dev::convert<double, float>(st, n, Ft, ff);
+        dev::convert<double, float>(st, n, X, xf);
+        double r0 = residual_norm<float>(ff, xf, top.rf, dev::S_T0);
+        if (r0 != 0.0) {
+            double r = r0;
+            while (std::fabs(r / r0) > prm_.amg_eps && it < prm_.amg_max_iter) {
+                vcycle<float>((int)L_.size() - 1, ff, xf);
+                it++;
+                if (it < prm_.amg_max_iter) r = residual_norm<float>(ff, xf, top.rf, dev::S_T0);
+            }
+            dev::convert<float, double>(st, n, xf, X);
+        }
+    }
+    vcycles_ += it;
+    precon_calls_++;
+    return it;
+}
+
+int Solver::precon(const double* R, double* Z) {
+    if (prm_.zero_init) dev::fill<double>(stream_, n_, Z, 0.0);
+    return amg_solve_dev(R, Z);
+}
+
+// pcg (PCGS.c:163-233) on device vectors X, B (length n)
+int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, lina_stats* stats) {
+    hipStream_t st = stream_;
+    const int n = n_;
+    DevLevel& top = L_.back();
+    vcycles_ = 0;
+    precon_calls_ = 0;
+    hip_check(hipEventRecord(ev0_, st), "event");
+    // h = |B| (h==0 -> 1), B and X scaled by 1/h
+    int np = dev::dot_part<double>(st, n, B, B, part_);
+    dev::finalize(st, np, part_, scal_, dev::S_H, 1);
+    dev::scale_by(st, n, B, scal_, dev::S_H, 1);
+    dev::scale_by(st, n, X, scal_, dev::S_H, 1);
+    double* R = kR_;
+    double* P = kP_;
+    double* Q = kQ_;
+    double* Z = kZ_;
+    double* W = kW_;
+    dev::fill<double>(st, n, W, 0.0);
+    dev::fill<double>(st, n, Z, 0.0);
+    // R = B + (-1)*(A X)
+    dev::spmv<double, double>(st, top.A, X, B, R, dev::MODE_RES, nullptr, nullptr);
+    precon(R, W);
+    np = dev::dot_part<double>(st, n, W, W, part_);
+    dev::finalize(st, np, part_, scal_, dev::S_R0, 1);
+    double r0 = read_scalar(dev::S_R0);
+    if (r0 == 0) {
+        // the reference returns here without undoing the 1/h scaling
+        if (stats) {
+            stats->iterations = 0;
+            stats->residual = 0;
+        }
+        return 0;
+    }
+    double r = r0;
+    int i = 0;
+    do {
+        precon(R, Z);
+        np = dev::dot_part<double>(st, n, R, Z, part_);
+        dev::finalize(st, np, part_, scal_, dev::S_S, 2);  // s_old = s; s = R.Z
+        dev::pcg_p(st, n, P, Z, scal_, i == 0);
+        dev::spmv<double, double>(st, top.A, P, nullptr, Q, dev::MODE_SET, part_, P);  // Q = A P, partial P.Q
+        dev::finalize(st, top.A.nblk, part_, scal_, dev::S_PQ, 0);
+        np = dev::pcg_xr(st, n, X, R, P, Q, scal_, part_);
+        dev::finalize(st, np, part_, scal_, dev::S_R, 1);
+        r = read_scalar(dev::S_R);
+        i++;
+        if (std::isnan(r)) break;
+    } while ((stop == LINA_STOP_RELRES ? r : r / r0) > tol && i < max_iter);
+    double relres = r;
+    precon(R, W);
+    np = dev::dot_part<double>(st, n, W, W, part_);
+    dev::finalize(st, np, part_, scal_, dev::S_T1, 1);
+    double rw = read_scalar(dev::S_T1);
+    dev::scale_by(st, n, B, scal_, dev::S_H, 0);
+    dev::scale_by(st, n, X, scal_, dev::S_H, 0);
+    hip_check(hipEventRecord(ev1_, st), "event");
+    hip_check(hipEventSynchronize(ev1_), "event sync");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "elapsed");
+    pcg_residuum_ = rw / r0;
+    if (stats) {
+        stats->iterations = i;
+        stats->precon_calls = precon_calls_;
+        stats->vcycles = vcycles_;
+        stats->residual = pcg_residuum_;
+        stats->relres = relres;
+        stats->t_solve_ms = ms;
+    }
+    if (std::isnan(rw / r0)) throw NumericError("pcg: NaN residual");
+    return i;
+}
+
+int Solver::pcg_host(double* x, double* b, double tol, int max_iter, int stop, lina_stats* stats) {
+    const size_t bytes = (size_t)n_ * sizeof(double);
+    hip_check(hipMemcpyAsync(kX_, x, bytes, hipMemcpyHostToDevice, stream_), "h2d");
+    hip_check(hipMemcpyAsync(kB_, b, bytes, hipMemcpyHostToDevice, stream_), "h2d");
+    int it = pcg_dev(kX_, kB_, tol, max_iter, stop, stats);
+    hip_check(hipMemcpyAsync(x, kX_, bytes, hipMemcpyDeviceToHost, stream_), "d2h");
+    hip_check(hipMemcpyAsync(b, kB_, bytes, hipMemcpyDeviceToHost, stream_), "d2h");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    return it;
+}
+
+int Solver::amg_solve_host(const double* f, double* x) {
+    const size_t bytes = (size_t)n_ * sizeof(double);
+    hip_check(hipMemcpyAsync(kB_, f, bytes, hipMemcpyHostToDevice, stream_), "h2d");
+    hip_check(hipMemcpyAsync(kX_, x, bytes, hipMemcpyHostToDevice, stream_), "h2d");
+    vcycles_ = 0;
+    precon_calls_ = 0;
+    int it = amg_solve_dev(kB_, kX_);
+    hip_check(hipMemcpyAsync(x, kX_, bytes, hipMemcpyDeviceToHost, stream_), "d2h");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    return it;
+}
+
+int Solver::spmv_host(int level, int precision, const double* x, double* y) {
+    DevLevel& d = L_.at(level);
+    const size_t bytes = (size_t)d.n * sizeof(double);
+    hip_check(hipMemcpyAsync(d.xo, x, bytes, hipMemcpyHostToDevice, stream_), "h2d");
+    if (precision == LINA_PREC_FP64) {
+        dev::spmv<double, double>(stream_, d.A, d.xo, nullptr, d.r, dev::MODE_SET, nullptr, nullptr);
+    } else {
+        dev::convert<double, float>(stream_, d.n, d.xo, d.xof);
+        dev::spmv<float, float>(stream_, d.Af, d.xof, nullptr, d.rf, dev::MODE_SET, nullptr, nullptr);
+        dev::convert<float, double>(stream_, d.n, d.rf, d.r);
+    }
+    hip_check(hipMemcpyAsync(y, d.r, bytes, hipMemcpyDeviceToHost, stream_), "d2h");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    return 0;
+}
+
+double Solver::time_spmv(int level, int precision, int reps) {
+    DevLevel& d = L_.at(level);
+    dev::fill<double>(stream_, d.n, d.xo, 1.0);
+    dev::fill<float>(stream_, d.n, d.xof, 1.0f);
+    auto launch = [&]() {
+        if (precision == LINA_PREC_FP64)
+            dev::spmv<double, double>(stream_, d.A, d.xo, nullptr, d.r, dev::MODE_SET, nullptr, nullptr);
+        else
+            dev::spmv<float, float>(stream_, d.Af, d.xof, nullptr, d.rf, dev::MODE_SET, nullptr, nullptr);
+    };
+    for (int i = 0; i < 3; i++) launch();
+    hip_check(hipEventRecord(ev0_, stream_), "event");
+    for (int i = 0; i < reps; i++) launch();
+    hip_check(hipEventRecord(ev1_, stream_), "event");
+    hip_check(hipEventSynchronize(ev1_), "event sync");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "elapsed");
+    return ms / reps;
+}
+
+double Solver::spmv_bytes(int level, int precision) const {
+    const Csr& A = H_.lv.at(level).A;
+    double nnz = A.nnz(), n = A.n;
+    if (precision == LINA_PREC_FP64) return 12.0 * nnz + 4.0 * (n + 1) + 16.0 * n;
+    return 8.0 * nnz + 4.0 * (n + 1) + 8.0 * n;
+}
+
+template void Solver::smooth<double>(int, const double*, double*, int, bool);
+template void Solver::smooth<float>(int, const float*, float*, int, bool);
+template void Solver::vcycle<double>(int, const double*, double*);
+template void Solver::vcycle<float>(int, const float*, float*);
+
+}  // namespace lina

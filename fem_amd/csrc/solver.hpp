@@ -1,0 +1,116 @@
+// Device-resident solver object behind the lina_solver handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lina_hip.h"
+#include "kernels.hpp"
+#include "lina_internal.hpp"
+
+namespace lina {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct NumericError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what);
+const char* last_error();
+
+struct DevBuffers {
+    std::vector<void*> ptrs;
+    ~DevBuffers();
+};
+
+struct DevLevel {
+    int n = 0;
+    dev::DMat<double> A, P, R, Ac;
+    dev::DMat<float> Af, Pf, Rf, Acf;
+    // multicolour sweeps: Ac holds the rows grouped by colour
+    int ncolours = 0;
+    int* perm = nullptr;
+    std::vector<int> colour_rows;  // ncolours+1 row offsets into Ac
+    std::vector<int> colour_blk;   // ncolours+1 block offsets into Ac.blk
+    // exact lexicographic sweeps
+    int nlv_f = 0, nlv_b = 0;
+    int *lvf_off = nullptr, *lvf_rows = nullptr, *lvb_off = nullptr, *lvb_rows = nullptr;
+    int nlim = 0;  // rows touched by AMG_parallel_Jacobi_smoother: (n/dof)*dof
+    // work vectors
+    double *x = nullptr, *f = nullptr, *r = nullptr, *xo = nullptr;
+    float *xf = nullptr, *ff = nullptr, *rf = nullptr, *xof = nullptr;
+};
+
+class Solver {
+  public:
+    Solver();
+    ~Solver();
+    void setup_host(const Csr& A, const lina_amg_params& p);
+    void upload();
+    void recompute(const std::vector<double>& val);
+    void set_params(const lina_amg_params& p) { prm_ = p; }
+    const lina_amg_params& params() const { return prm_; }
+    const HostHierarchy& host() const { return H_; }
+    const std::vector<int>& colours(int l) const { return colours_.at(l); }
+    bool uploaded() const { return bufs_ != nullptr; }
+    int n() const { return n_; }
+
+    int pcg_dev(double* X, double* B, double tol, int max_iter, int stop, lina_stats* st);
+    int pcg_host(double* x, double* b, double tol, int max_iter, int stop, lina_stats* st);
+    int amg_solve_dev(const double* F, double* X);
+    int amg_solve_host(const double* f, double* x);
+    int precon(const double* R, double* Z);
+    int spmv_host(int level, int precision, const double* x, double* y);
+    double time_spmv(int level, int precision, int reps);
+    double spmv_bytes(int level, int precision) const;
+    double pcg_residuum() const { return pcg_residuum_; }
+    hipStream_t stream() const { return stream_; }
+
+    template <typename T>
+    void smooth(int l, const T* f, T* x, int sweeps, bool post);
+    template <typename T>
+    void vcycle(int l, const T* f, T* x);
+    template <typename T>
+    double residual_norm(const T* f, const T* x, T* r, int slot);
+    double read_scalar(int slot);
+
+    // device buffers shared with other drivers (BiCGStab, GMRES, RN)
+    double* part() { return part_; }
+    double* scal() { return scal_; }
+    DevLevel& finest() { return L_.back(); }
+
+  private:
+    lina_amg_params prm_{};
+    HostHierarchy H_;
+    std::vector<std::vector<int>> colours_;
+    std::vector<int> ncolours_;
+    int n_ = 0;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    double* host_scal_ = nullptr;
+    std::unique_ptr<DevBuffers> bufs_;
+    std::vector<DevLevel> L_;
+    double* signs_ = nullptr;
+    bool has_neg_signs_ = false;
+    int* iso_rows_ = nullptr;
+    double* iso_a_ = nullptr;
+    int niso_ = 0;
+    int nd_ = 0, nt_ = 0;
+    int *didx_ = nullptr, *tidx_ = nullptr;
+    double *inv_ = nullptr, *tdiag_ = nullptr;
+    double *ftop_ = nullptr, *rtop_ = nullptr;
+    double *kR_ = nullptr, *kP_ = nullptr, *kQ_ = nullptr, *kZ_ = nullptr, *kW_ = nullptr, *kX_ = nullptr,
+           *kB_ = nullptr;
+    double* part_ = nullptr;
+    double* scal_ = nullptr;
+    int vcycles_ = 0, precon_calls_ = 0;
+    double pcg_residuum_ = 0;
+};
+
+}  // namespace lina

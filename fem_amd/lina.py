@@ -1,0 +1,229 @@
+"""ctypes binding of liblina_hip.so (the handle API of include/lina_hip.h).
+
+The library is built in-tree by ``fem_amd.build``; importing this module
+never falls back to anything else: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblina_hip.so")
+
+SMOOTHER = {"sor": 0, "jacobi": 1, "mc_sor": 2, "l1_jacobi": 3, "mc_ssor": 4, "ssor": 5}
+PREC = {"fp64": 0, "mixed": 1}
+STOP = {"reference": 0, "relres": 1}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Params(C.Structure):
+    _fields_ = [("dof", C.c_int), ("depth", C.c_int), ("smoothing_iterations", C.c_int),
+                ("aggr_smoothing_iterations", C.c_int), ("coarsening", C.c_int), ("eps_minus", C.c_double),
+                ("eps_plus", C.c_double), ("matrix_tol", C.c_double), ("smoother", C.c_int), ("omega", C.c_double),
+                ("amg_eps", C.c_double), ("amg_max_iter", C.c_int), ("zero_init", C.c_int),
+                ("precision", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("precon_calls", C.c_int), ("vcycles", C.c_int), ("residual", C.c_double),
+                ("relres", C.c_double), ("t_solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class LinaError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LinaError(f"{LIB_PATH} is not built: run python -m fem_amd.build")
+    L = C.CDLL(LIB_PATH)
+    L.lina_last_error.restype = C.c_char_p
+    L.lina_device_count.restype = C.c_int
+    L.lina_params_reference.argtypes = [C.POINTER(Params), C.c_int, C.c_int]
+    L.lina_params_vcycle.argtypes = [C.POINTER(Params), C.c_int, C.c_int]
+    L.lina_amg_create.argtypes = [C.c_int, _ip, _ip, _dp, C.POINTER(Params), C.c_int]
+    L.lina_amg_create.restype = C.c_void_p
+    L.lina_amg_destroy.argtypes = [C.c_void_p]
+    L.lina_amg_set_params.argtypes = [C.c_void_p, C.POINTER(Params)]
+    L.lina_amg_depth.argtypes = [C.c_void_p]
+    L.lina_amg_level_info.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, _ip]
+    L.lina_amg_level_nnz.argtypes = [C.c_void_p, C.c_int, C.c_char]
+    L.lina_amg_level_copy.argtypes = [C.c_void_p, C.c_int, C.c_char, _ip, _ip, _dp]
+    L.lina_amg_labels.argtypes = [C.c_void_p, C.c_int, _ip]
+    L.lina_amg_colours.argtypes = [C.c_void_p, C.c_int, _ip]
+    L.lina_amg_recompute.argtypes = [C.c_void_p, _dp]
+    L.lina_pcg.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, C.c_int, C.POINTER(Stats)]
+    L.lina_pcg_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int,
+                                  C.POINTER(Stats)]
+    L.lina_amg_solve.argtypes = [C.c_void_p, _dp, _dp]
+    L.lina_spmv.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp]
+    L.lina_time_spmv.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    L.lina_time_spmv.restype = C.c_double
+    L.lina_spmv_bytes.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.lina_spmv_bytes.restype = C.c_double
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc < 0:
+        raise LinaError(f"liblina_hip error {rc}: {lib().lina_last_error().decode()}")
+    return rc
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+def device_count() -> int:
+    return lib().lina_device_count()
+
+
+def params(kind: str = "reference", dof: int = 1, depth: int = 4, **kw) -> Params:
+    p = Params()
+    if kind == "reference":
+        lib().lina_params_reference(C.byref(p), dof, depth)
+    elif kind == "vcycle":
+        lib().lina_params_vcycle(C.byref(p), dof, depth)
+    else:
+        raise ValueError(kind)
+    for k, v in kw.items():
+        if k == "smoother" and isinstance(v, str):
+            v = SMOOTHER[v]
+        if k == "precision" and isinstance(v, str):
+            v = PREC[v]
+        setattr(p, k, v)
+    return p
+
+
+class AMG:
+    """A device-resident AMG hierarchy (lina_solver handle)."""
+
+    def __init__(self, rowptr, col, val, prm: Params, upload: bool = True):
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.n = len(self.rowptr) - 1
+        self.prm = prm
+        h = lib().lina_amg_create(self.n, _i(self.rowptr), _i(self.col), _d(self.val), C.byref(prm),
+                                  1 if upload else 0)
+        if not h:
+            raise LinaError(f"lina_amg_create failed: {lib().lina_last_error().decode()}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lina_amg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_params(self, prm: Params):
+        _check(lib().lina_amg_set_params(self.h, C.byref(prm)))
+        self.prm = prm
+
+    @property
+    def depth(self) -> int:
+        return lib().lina_amg_depth(self.h)
+
+    def level(self, l: int) -> dict:
+        fine, coarse, na, np_, nr = (C.c_int() for _ in range(5))
+        _check(lib().lina_amg_level_info(self.h, l, C.byref(fine), C.byref(coarse), C.byref(na), C.byref(np_),
+                                         C.byref(nr)))
+        out = {"fine": fine.value, "coarse": coarse.value}
+        for which, nnz, rows in (("A", na.value, fine.value), ("P", np_.value, fine.value),
+                                 ("R", nr.value, coarse.value)):
+            if which != "A" and l == 0:
+                continue
+            rp = np.zeros(rows + 1, np.int32)
+            ci = np.zeros(max(nnz, 1), np.int32)
+            v = np.zeros(max(nnz, 1), np.float64)
+            _check(lib().lina_amg_level_copy(self.h, l, which.encode(), _i(rp), _i(ci), _d(v)))
+            out[which] = (rp, ci[:nnz], v[:nnz])
+        if l > 0:
+            lab = np.zeros(fine.value, np.int32)
+            _check(lib().lina_amg_labels(self.h, l, _i(lab)))
+            out["labels"] = lab
+            col = np.zeros(fine.value, np.int32)
+            out["ncolours"] = _check(lib().lina_amg_colours(self.h, l, _i(col)))
+            out["colours"] = col
+        return out
+
+    def coarse_lu(self):
+        nn = _check(lib().lina_amg_level_nnz(self.h, 0, b"L"))
+        N = int(round(nn ** 0.5))
+        rp = np.zeros(1, np.int32)
+        idx = np.zeros(max(N, 1), np.int32)
+        Lm = np.zeros(max(nn, 1))
+        Um = np.zeros(max(nn, 1))
+        _check(lib().lina_amg_level_copy(self.h, 0, b"L", _i(rp), _i(idx), _d(Lm)))
+        _check(lib().lina_amg_level_copy(self.h, 0, b"U", _i(rp), _i(idx), _d(Um)))
+        return idx[:N], Lm[:nn].reshape(N, N), Um[:nn].reshape(N, N)
+
+    def recompute(self, val):
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        _check(lib().lina_amg_recompute(self.h, _d(v)))
+
+    def pcg(self, b, x0=None, tol=1e-10, max_iter=1000, stop="reference"):
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        B = np.array(b, dtype=np.float64)
+        st = Stats()
+        it = _check(lib().lina_pcg(self.h, _d(x), _d(B), tol, max_iter, STOP[stop], C.byref(st)))
+        d = st.as_dict()
+        d["iters"] = it
+        d["x"] = x
+        d["b_out"] = B
+        return d
+
+    def pcg_device(self, d_x: int, d_b: int, tol=1e-10, max_iter=1000, stop="reference"):
+        st = Stats()
+        it = _check(lib().lina_pcg_device(self.h, C.c_void_p(d_x), C.c_void_p(d_b), tol, max_iter, STOP[stop],
+                                          C.byref(st)))
+        d = st.as_dict()
+        d["iters"] = it
+        return d
+
+    def amg_solve(self, f, x0):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        x = np.array(x0, dtype=np.float64)
+        k = _check(lib().lina_amg_solve(self.h, _d(f), _d(x)))
+        return k, x
+
+    def spmv(self, x, level=None, precision="fp64"):
+        l = self.depth - 1 if level is None else level
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        _check(lib().lina_spmv(self.h, l, PREC[precision], _d(x), _d(y)))
+        return y
+
+    def time_spmv(self, level=None, precision="fp64", reps=50) -> float:
+        l = self.depth - 1 if level is None else level
+        ms = lib().lina_time_spmv(self.h, l, PREC[precision], reps)
+        if ms < 0:
+            raise LinaError(lib().lina_last_error().decode())
+        return ms
+
+    def spmv_bytes(self, level=None, precision="fp64") -> float:
+        l = self.depth - 1 if level is None else level
+        return lib().lina_spmv_bytes(self.h, l, PREC[precision])

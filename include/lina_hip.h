@@ -1,0 +1,136 @@
+/*
+ * lina_hip.h -- handle-based C ABI of liblina_hip, the MI355X (gfx950)
+ * implementation of the LinA sparse-solver path (MRadszuweit/FEM, LinA/).
+ *
+ * The LinA drop-in entry points (pcg, AMG_setup, AMG_solve, BiCGStab_solve,
+ * reflective_newton, ... with the reference's row-list sparse_matrix) are
+ * declared in the headers next to this one (linear_algebra.h, PCGS.h,
+ * Multigrid.h, BiCGStab.h, reflective_newton.h, gmres.h).  This header adds
+ * the handle API they are built on: plain pointers and sizes, no globals, so
+ * several systems can live on the device at once.
+ *
+ * Conventions: CSR with int32 indices, columns sorted inside each row and
+ * every row holding at least one entry (the reference's sparse_zero seeds
+ * the diagonal).  Functions return >= 0 on success and a negative LINA_E*
+ * code on failure; lina_last_error() returns the message.  Nothing calls
+ * exit(): where the reference prints and exits (e.g. NaN in pcg, PCGS.c:
+ * 220-223) these functions return LINA_ENUMERIC instead.
+ */
+#ifndef LINA_HIP_H
+#define LINA_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define LINA_OK 0
+#define LINA_EARG -1     /* invalid argument / shape */
+#define LINA_EHIP -2     /* HIP runtime failure */
+#define LINA_ESETUP -3   /* AMG setup failed (zero diagonal, no interpolation, ...) */
+#define LINA_ENUMERIC -4 /* NaN / breakdown */
+#define LINA_ENODEV -5   /* no device available */
+
+/* smoothers of the V-cycle (AMG_set_smoother, LinA/Multigrid.c:1967-1969) */
+#define LINA_SMOOTHER_SOR 0       /* AMG_SOR_smoother (Multigrid.c:1345-1365): lexicographic, exact */
+#define LINA_SMOOTHER_JACOBI 1    /* AMG_parallel_Jacobi_smoother (Multigrid.c:1367-1400) */
+#define LINA_SMOOTHER_MC_SOR 2    /* multicolour SOR, same sweep direction before and after */
+#define LINA_SMOOTHER_L1_JACOBI 3 /* l1-Jacobi x += (f-Ax)/sum_j|a_ij| */
+#define LINA_SMOOTHER_MC_SSOR 4   /* multicolour SOR, colours reversed on the post-smoothing call */
+#define LINA_SMOOTHER_SSOR 5      /* lexicographic SOR, rows reversed on the post-smoothing call */
+
+#define LINA_COARSEN_AGGRESSIVE 1 /* AGGRESSIVE (Multigrid.h:38) on the finest level */
+#define LINA_COARSEN_STANDARD 2   /* STANDARD (Multigrid.h:37) */
+
+#define LINA_PREC_FP64 0          /* hierarchy, smoothers and Krylov vectors in FP64 */
+#define LINA_PREC_MIXED 1         /* FP32 hierarchy + smoothers; FP64 outer Krylov,
+                                     dots/norms and coarsest solve */
+
+#define LINA_STOP_REFERENCE 0     /* pcg's test |R_k| / |M^-1 R_0| <= tol (PCGS.c:182,213) */
+#define LINA_STOP_RELRES 1        /* |R_k| <= tol with B scaled to unit norm, i.e. the
+                                     relative residual |b-Ax|/|b| of the CG recurrence */
+
+typedef struct lina_amg_params {
+    /* AMG_setup arguments (Multigrid.c:1613-1624) */
+    int dof;                       /* degrees of freedom per node, blocked ordering */
+    int depth;                     /* number of levels */
+    int smoothing_iterations;      /* sweeps per smoothing call */
+    int aggr_smoothing_iterations; /* sweeps on an aggressively coarsened finest level */
+    int coarsening;                /* LINA_COARSEN_* */
+    double eps_minus, eps_plus;    /* set_connection_thresholds (Multigrid.c:125-128) */
+    double matrix_tol;             /* AMG_set_matrix_tolerance: P truncation */
+    /* AMG_solve controls */
+    int smoother;                  /* LINA_SMOOTHER_* */
+    double omega;                  /* AMG_set_SOR_relaxation_coeff */
+    double amg_eps;                /* AMG_set_eps */
+    int amg_max_iter;              /* AMG_set_max_iter */
+    int zero_init;                 /* 1: the preconditioner starts every call from Z = 0 */
+    int precision;                 /* LINA_PREC_* */
+} lina_amg_params;
+
+typedef struct lina_stats {
+    int iterations;      /* Krylov iterations (pcg's return value) */
+    int precon_calls;    /* AMG_solve calls */
+    int vcycles;         /* V-cycles summed over all calls */
+    double residual;     /* get_pcg_residuum(): |M^-1 R_final| / |M^-1 R_0| */
+    double relres;       /* |R_final| of the recurrence with B scaled to 1 */
+    double t_solve_ms;   /* device time of the solve (HIP events on the library stream) */
+} lina_stats;
+
+typedef struct lina_solver lina_solver;
+
+const char* lina_last_error(void);
+int lina_device_count(void);
+
+/* parameters of PCG_set_AMG_preconditioner (PCGS.c:118-129): SOR(1.6), 2/3
+ * sweeps, aggressive first coarsening, AMG solve to 1e-2, <= 100 V-cycles */
+void lina_params_reference(lina_amg_params* p, int dof, int depth);
+/* one symmetric multicolour-SOR V(1,1)-cycle per Krylov iteration */
+void lina_params_vcycle(lina_amg_params* p, int dof, int depth);
+
+/* Build the hierarchy for A (host CSR).  upload=0 runs the host setup only
+ * (for inspection on machines without a GPU). */
+lina_solver* lina_amg_create(int n, const int* rowptr, const int* col, const double* val,
+                             const lina_amg_params* p, int upload);
+void lina_amg_destroy(lina_solver* s);
+int lina_amg_set_params(lina_solver* s, const lina_amg_params* p); /* smoother/eps/precision only */
+
+/* hierarchy inspection: depth; per level sizes; copies of A/P/R ('A','P','R'),
+ * labels, and the coarse LU factors ('L','U' on level 0) */
+int lina_amg_depth(const lina_solver* s);
+int lina_amg_level_info(const lina_solver* s, int level, int* fine, int* coarse, int* nnz_a, int* nnz_p, int* nnz_r);
+int lina_amg_level_nnz(const lina_solver* s, int level, char which);
+int lina_amg_level_copy(const lina_solver* s, int level, char which, int* rowptr, int* col, double* val);
+int lina_amg_labels(const lina_solver* s, int level, int* labels);
+int lina_amg_colours(const lina_solver* s, int level, int* colours);
+
+/* Values-only update with fixed P/R (AMG_recompute_coarse_matrices,
+ * Multigrid.c:1756-1780).  The pattern of A must be unchanged. */
+int lina_amg_recompute(lina_solver* s, const double* val);
+
+/* pcg (PCGS.c:163-233) preconditioned by this hierarchy; x (initial guess,
+ * overwritten) and b follow the reference's in/out semantics, including the
+ * in-place scaling of b by 1/|b| and back.  Host pointers. */
+int lina_pcg(lina_solver* s, double* x, double* b, double tol, int max_iter, int stop, lina_stats* st);
+/* the same on device-resident vectors (length n, FP64) */
+int lina_pcg_device(lina_solver* s, double* d_x, double* d_b, double tol, int max_iter, int stop, lina_stats* st);
+
+/* AMG_solve (Multigrid.c:1863-1931): V-cycles on A x = f from the initial
+ * guess x until |f-Ax|/|f-Ax0| <= amg_eps or amg_max_iter.  Host pointers;
+ * returns the number of V-cycles. */
+int lina_amg_solve(lina_solver* s, const double* f, double* x);
+
+/* single kernels on the finest level for verification and timing:
+ * y = A x (precision FP64 or FP32 copy of A), host pointers */
+int lina_spmv(lina_solver* s, int level, int precision, const double* x, double* y);
+/* average device time (ms) of one finest-level SpMV launch over reps
+ * launches, HIP events recorded on the library's stream */
+double lina_time_spmv(lina_solver* s, int level, int precision, int reps);
+/* per-launch byte count used for roofline reporting: 12 nnz + 4 (n+1) + 16 n
+ * for FP64 values (8 nnz + 4 (n+1) + 8 n for FP32) */
+double lina_spmv_bytes(const lina_solver* s, int level, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

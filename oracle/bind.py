@@ -1,0 +1,228 @@
+"""ctypes bindings for the test-side CPU libraries under oracle/.
+
+TEST INFRASTRUCTURE ONLY -- imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+
+* ``Ref``    -> oracle/_ref/libreflina.so, the reference LinA compiled from its
+               own sources (+ oracle/ref_harness.c glue).
+* ``Oracle`` -> oracle/liboracle.so, the C restatement in oracle/lina_oracle.c.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_SO = os.path.join(HERE, "_ref", "libreflina.so")
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+SMOOTHERS = {"sor": 0, "jacobi": 1, "mc_sor": 2, "l1_jacobi": 3, "mc_ssor": 4, "ssor": 5}
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+class CSR:
+    def __init__(self, rowptr, col, val, ncols=None):
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.n = len(self.rowptr) - 1
+        self.ncols = ncols
+
+    def dense(self, ncols=None):
+        nc = ncols or self.ncols or (int(self.col.max()) + 1 if len(self.col) else self.n)
+        D = np.zeros((self.n, nc))
+        for i in range(self.n):
+            for k in range(self.rowptr[i], self.rowptr[i + 1]):
+                D[i, self.col[k]] += self.val[k]
+        return D
+
+    def matvec(self, x):
+        y = np.zeros(self.n)
+        for i in range(self.n):
+            s = 0.0
+            for k in range(self.rowptr[i], self.rowptr[i + 1]):
+                s += self.val[k] * x[self.col[k]]
+            y[i] = s
+        return y
+
+
+class Ref:
+    """The compiled reference (oracle/_ref/libreflina.so)."""
+
+    def __init__(self, path: str = REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (make -C oracle ref)")
+        L = C.CDLL(path)
+        self.L = L
+        L.refh_init.argtypes = [C.c_int]
+        L.refh_from_csr.argtypes = [C.c_int, _ip, _ip, _dp]
+        L.refh_from_csr.restype = _vp
+        L.refh_free.argtypes = [_vp]
+        L.refh_size.argtypes = [_vp]
+        L.refh_nnz.argtypes = [_vp]
+        L.refh_to_csr.argtypes = [_vp, _ip, _ip, _dp]
+        L.refh_amg_setup.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.refh_amg_setup.restype = _vp
+        L.refh_amg_free.argtypes = [_vp]
+        L.refh_amg_sizes.argtypes = [_vp, _ip, _ip]
+        for f in ("refh_amg_A", "refh_amg_P", "refh_amg_R", "refh_amg_L", "refh_amg_U"):
+            getattr(L, f).argtypes = [_vp, C.c_int]
+            getattr(L, f).restype = _vp
+        L.refh_amg_signs.argtypes = [_vp]
+        L.refh_amg_signs.restype = _vp
+        L.refh_amg_labels.argtypes = [_vp, C.c_int, _ip]
+        L.refh_amg_recompute.argtypes = [_vp, _vp]
+        L.refh_amg_solve.argtypes = [_vp, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int]
+        L.refh_pcg_amg.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                   C.c_double, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _ip, C.c_int, _ip,
+                                   _dp, _dp, C.POINTER(_vp)]
+        L.refh_pcg_amg_default.argtypes = [_vp, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp, _dp]
+        L.refh_pcg_ilu.argtypes = [_vp, _dp, _dp, C.c_double, C.c_int, _dp]
+        L.refh_ilu.argtypes = [_vp, C.POINTER(_vp), C.POINTER(_vp)]
+        L.refh_L_invert.argtypes = [_vp, _dp]
+        L.refh_U_invert.argtypes = [_vp, _dp]
+        L.refh_sparse_LU.argtypes = [_vp, C.POINTER(_vp), C.POINTER(_vp)]
+        L.refh_product.argtypes = [_vp, _vp]
+        L.refh_product.restype = _vp
+        L.refh_colour_count.argtypes = [_vp, _ip]
+        L.refh_bicgstab_amg.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
+        L.refh_reflective_newton.argtypes = [_vp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                             C.c_double, C.c_double, _dp]
+        L.refh_init(1)
+
+    # -- matrices
+    def matrix(self, rowptr, col, val):
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        return self.L.refh_from_csr(len(rowptr) - 1, iptr(rowptr), iptr(col), dptr(val))
+
+    def to_csr(self, M) -> CSR:
+        n = self.L.refh_size(M)
+        nnz = self.L.refh_nnz(M)
+        rp = np.zeros(n + 1, np.int32)
+        c = np.zeros(max(nnz, 1), np.int32)
+        v = np.zeros(max(nnz, 1), np.float64)
+        self.L.refh_to_csr(M, iptr(rp), iptr(c), dptr(v))
+        return CSR(rp, c[:nnz], v[:nnz])
+
+    def free(self, M):
+        self.L.refh_free(M)
+
+    # -- AMG hierarchy
+    def amg_setup(self, M, dof=1, depth=4, smiter=2, aggr_smiter=3, coarsening=1, matrix_tol=None):
+        return self.L.refh_amg_setup(M, dof, depth, smiter, aggr_smiter, coarsening,
+                                     0 if matrix_tol is None else 1, matrix_tol or 0.0)
+
+    def amg_levels(self, I, depth):
+        fine = np.zeros(depth, np.int32)
+        coarse = np.zeros(depth, np.int32)
+        self.L.refh_amg_sizes(I, iptr(fine), iptr(coarse))
+        out = []
+        for l in range(depth):
+            d = {"fine": int(fine[l]), "coarse": int(coarse[l]), "A": self.to_csr(self.L.refh_amg_A(I, l))}
+            if l > 0:
+                d["P"] = self.to_csr(self.L.refh_amg_P(I, l))
+                d["R"] = self.to_csr(self.L.refh_amg_R(I, l))
+                lab = np.zeros(int(fine[l]), np.int32)
+                self.L.refh_amg_labels(I, l, iptr(lab))
+                d["labels"] = lab
+            else:
+                d["L"] = self.to_csr(self.L.refh_amg_L(I, 0))
+                d["U"] = self.to_csr(self.L.refh_amg_U(I, 0))
+            out.append(d)
+        return out
+
+    def amg_signs(self, I):
+        return self.to_csr(self.L.refh_amg_signs(I))
+
+    def amg_free(self, I):
+        self.L.refh_amg_free(I)
+
+    def amg_solve(self, I, F, X, smoother="sor", alpha=1.6, eps=1e-2, max_iter=100):
+        F = np.array(F, dtype=np.float64)
+        X = np.array(X, dtype=np.float64)
+        k = self.L.refh_amg_solve(I, dptr(F), dptr(X), SMOOTHERS[smoother], alpha, eps, max_iter)
+        return k, X
+
+    # -- solvers
+    def pcg_amg(self, M, b, x0=None, dof=1, depth=4, smiter=2, aggr_smiter=3, coarsening=1, smoother="sor",
+                alpha=1.6, amg_eps=1e-2, amg_max_iter=100, zero_init=False, tol=1e-10, max_iter=1000, keep=False):
+        n = self.L.refh_size(M)
+        X = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+        B = np.array(b, dtype=np.float64)
+        res = C.c_double()
+        cap = 100000
+        calls = np.zeros(cap, np.int32)
+        ncalls = C.c_int()
+        ts = C.c_double()
+        tv = C.c_double()
+        kept = _vp()
+        it = self.L.refh_pcg_amg(M, dof, depth, smiter, aggr_smiter, coarsening, SMOOTHERS[smoother], alpha,
+                                 amg_eps, amg_max_iter, int(zero_init), dptr(X), dptr(B), tol, max_iter, C.byref(res),
+                                 iptr(calls), cap, C.byref(ncalls), C.byref(ts), C.byref(tv),
+                                 C.byref(kept) if keep else None)
+        out = {"iters": it, "x": X, "b_out": B, "res": res.value, "vcycles": calls[:min(ncalls.value, cap)].copy(),
+               "t_setup": ts.value, "t_solve": tv.value}
+        if keep:
+            out["amg"] = kept.value
+        return out
+
+    def pcg_ilu(self, M, b, x0=None, tol=1e-10, max_iter=1000):
+        n = self.L.refh_size(M)
+        X = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+        B = np.array(b, dtype=np.float64)
+        res = C.c_double()
+        it = self.L.refh_pcg_ilu(M, dptr(X), dptr(B), tol, max_iter, C.byref(res))
+        return {"iters": it, "x": X, "b_out": B, "res": res.value}
+
+    def ilu(self, M):
+        Lp, Up = _vp(), _vp()
+        ok = self.L.refh_ilu(M, C.byref(Lp), C.byref(Up))
+        Lc, Uc = self.to_csr(Lp), self.to_csr(Up)
+        self.free(Lp)
+        self.free(Up)
+        return ok, Lc, Uc
+
+    def product(self, A, B):
+        P = self.L.refh_product(A, B)
+        out = self.to_csr(P)
+        self.free(P)
+        return out
+
+    def colours(self, M):
+        n = self.L.refh_size(M)
+        c = np.zeros(n, np.int32)
+        k = self.L.refh_colour_count(M, iptr(c))
+        return k, c
+
+    def bicgstab_amg(self, M, b, x0=None, dof=1, depth=4, smoother="sor", alpha=1.0, eps=1e-8, max_iter=1000):
+        n = self.L.refh_size(M)
+        X = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+        B = np.array(b, dtype=np.float64)
+        it = self.L.refh_bicgstab_amg(M, dof, depth, SMOOTHERS[smoother], alpha, eps, max_iter, dptr(X), dptr(B))
+        return {"iters": it, "x": X}
+
+    def reflective_newton(self, H, b, lower, upper, x0, solver=3, dof=1, depth=4, max_iter=50, eps=1e-8, delta=250.0):
+        X = np.array(x0, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        lo = np.ascontiguousarray(lower, dtype=np.float64)
+        up = np.ascontiguousarray(upper, dtype=np.float64)
+        q = C.c_double()
+        it = self.L.refh_reflective_newton(H, dptr(b), dptr(lo), dptr(up), dptr(X), solver, dof, depth, max_iter,
+                                           eps, delta, C.byref(q))
+        return {"iters": it, "x": X, "q": q.value}
