@@ -92,10 +92,48 @@ static void build_blocks(const Csr& A, int begin, int end, std::vector<int>& sta
     }
 }
 
+// every index a kernel will dereference must be inside its array
+static void validate(const Csr& A, const std::vector<int>& starts) {
+    if ((int)A.rp.size() != A.n + 1 || A.rp[0] != 0) throw std::logic_error("upload: bad row pointer");
+    for (int i = 0; i < A.n; i++)
+        if (A.rp[i + 1] < A.rp[i]) throw std::logic_error("upload: row pointer not monotone");
+    for (int c : A.ci)
+        if (c < 0 || c >= A.ncols) throw std::logic_error("upload: column index outside the matrix");
+    for (size_t b = 0; b + 1 < starts.size(); b++) {
+        int r0 = starts[b], r1 = starts[b + 1];
+        if (r0 >= r1 || r1 > A.n) throw std::logic_error("upload: bad row block");
+        if (r1 - r0 > dev::BLOCK_ROWS) throw std::logic_error("upload: row block too tall");
+        if (A.rp[r1] - A.rp[r0] > dev::BLOCK_NNZ && r1 - r0 != 1) throw std::logic_error("upload: row block too wide");
+    }
+}
+
+// Prolongation rows may carry sparse_zero's (i,0.0) seed at a column i that
+// is not a coarse index (rows left empty by the reference); a zero product
+// adds nothing, so such entries are dropped before upload.
+static Csr drop_outside_zeros(const Csr& A) {
+    Csr B;
+    B.n = A.n;
+    B.ncols = A.ncols;
+    B.rp.assign(A.n + 1, 0);
+    for (int i = 0; i < A.n; i++) {
+        for (int p = A.rp[i]; p < A.rp[i + 1]; p++) {
+            if (A.ci[p] >= A.ncols) {
+                if (A.v[p] != 0.0) throw std::logic_error("upload: nonzero entry outside the matrix");
+                continue;
+            }
+            B.ci.push_back(A.ci[p]);
+            B.v.push_back(A.v[p]);
+        }
+        B.rp[i + 1] = (int)B.ci.size();
+    }
+    return B;
+}
+
 template <typename VT>
 static void upload_mat(const Csr& A, const std::vector<int>& starts_with_end, dev::DMat<VT>& D, DevBuffers& B,
                        hipStream_t st, const int* shared_rp = nullptr, const int* shared_ci = nullptr,
                        const int* shared_blk = nullptr) {
+    validate(A, starts_with_end);
     D.n = A.n;
     D.ncols = A.ncols;
     D.nnz = A.nnz();
@@ -108,14 +146,16 @@ static void upload_mat(const Csr& A, const std::vector<int>& starts_with_end, de
     D.blk = shared_blk ? (int*)shared_blk : dupload(starts_with_end, B, st);
 }
 
-// upload A as FP64 + FP32 copies sharing the index arrays
-static void upload_pair(const Csr& A, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st) {
+// upload A as FP64 + FP32 copies sharing the index arrays; returns #blocks
+static int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st) {
+    Csr A = drop_outside_zeros(A0);
     std::vector<int> starts;
     build_blocks(A, 0, A.n, starts);
     starts.push_back(A.n);
     if (A.n == 0) starts.assign(1, 0);
     upload_mat(A, starts, D, B, st);
     upload_mat(A, starts, F, B, st, D.rp, D.ci, D.blk);
+    return D.nblk;
 }
 
 // level schedule of the lexicographic sweep: forward, level(i) = 1 + max over
@@ -186,13 +226,13 @@ void Solver::upload() {
     hipStream_t st = stream_;
     const int depth = (int)H_.lv.size();
     L_.assign(depth, DevLevel());
-    int maxn = 1;
+    int maxn = 1, maxblk = 1;
     for (int l = 0; l < depth; l++) {
         const HostLevel& h = H_.lv[l];
         DevLevel& d = L_[l];
         d.n = h.A.n;
         maxn = std::max(maxn, d.n);
-        upload_pair(h.A, d.A, d.Af, B, st);
+        maxblk = std::max(maxblk, upload_pair(h.A, d.A, d.Af, B, st));
         d.x = dalloc<double>(d.n, B);
         d.f = dalloc<double>(d.n, B);
         d.r = dalloc<double>(d.n, B);
@@ -203,8 +243,8 @@ void Solver::upload() {
         d.xof = dalloc<float>(d.n, B);
         d.nlim = (d.n / std::max(1, prm_.dof)) * std::max(1, prm_.dof);
         if (l == 0) continue;
-        upload_pair(h.P, d.P, d.Pf, B, st);
-        upload_pair(h.R, d.R, d.Rf, B, st);
+        maxblk = std::max(maxblk, upload_pair(h.P, d.P, d.Pf, B, st));
+        maxblk = std::max(maxblk, upload_pair(h.R, d.R, d.Rf, B, st));
         // colour-permuted copy for the multicolour sweeps
         const std::vector<int>& col = colours_[l];
         int nc = ncolours_[l];
@@ -281,7 +321,8 @@ void Solver::upload() {
     kW_ = dalloc<double>(n, B);
     kX_ = dalloc<double>(n, B);
     kB_ = dalloc<double>(n, B);
-    int np = std::max(4096, (maxn + 255) / 256 + 16);
+    // block partials: one per SpMV block (any matrix) or per reduction block
+    int np = std::max(4096, maxblk + 16);
     part_ = dalloc<double>(np, B);
     scal_ = dalloc<double>(dev::S_COUNT, B);
     dev::fill(st, dev::S_COUNT, scal_, 0.0);

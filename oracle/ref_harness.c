@@ -22,6 +22,8 @@
 #include "BiCGStab.h"
 #include "reflective_newton.h"
 
+void refh_sym_reset(void);
+
 /* ---------------------------------------------------------------- memory */
 
 void refh_init(int zero_fill) {
@@ -302,6 +304,7 @@ int refh_amg_solve(amg_system_info* I, double* F, double* X, int smoother, doubl
     AMG_set_eps(eps);
     AMG_set_max_iter(max_iter);
     AMG_print_info(NO);
+    refh_sym_reset();
     return AMG_solve(I, F, X, NULL);
 }
 
