@@ -13,11 +13,15 @@
 // Compiled with -ffp-contract=off: a*b+c is never fused, as in the reference.
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <stdexcept>
+
+#include <omp.h>
 
 #include "lina_internal.hpp"
 
@@ -116,18 +120,54 @@ static void delete_zeros_row(int row, std::vector<int>& cols, std::vector<double
 // C = A*B with the reference's traversal: for each A(i,k)!=0 in row order,
 // for each B(k,j)!=0 in row order, C(i,j) += A(i,k)*B(k,j); row i of C is
 // seeded with (i,0.0); then delete_zeros (linear_algebra.c:186-222).
-void sparse_product(const Csr& A, const Csr& B, int b_ncols, Csr& C) {
-    const int n = A.n;
-    int width = std::max(b_ncols, n);
-    RowAcc acc(width);
-    C.n = n;
-    C.ncols = b_ncols;
+// Rows are independent: chunks of rows are computed in parallel (one
+// accumulator per thread) and concatenated in row order.
+template <typename RowFn>
+static void rowwise(int n, int width, Csr& C, RowFn fn) {
+    const int nchunk = std::max(1, std::min(n, omp_get_max_threads() * 8));
+    std::vector<std::vector<int>> cc(nchunk), cl(nchunk);
+    std::vector<std::vector<double>> cv(nchunk);
+#pragma omp parallel
+    {
+        RowAcc acc(width);
+        std::vector<int> cols;
+        std::vector<double> vals;
+#pragma omp for schedule(dynamic, 1)
+        for (int c = 0; c < nchunk; c++) {
+            int r0 = (int)((long long)n * c / nchunk), r1 = (int)((long long)n * (c + 1) / nchunk);
+            for (int i = r0; i < r1; i++) {
+                fn(i, acc, cols, vals);
+                cc[c].insert(cc[c].end(), cols.begin(), cols.end());
+                cv[c].insert(cv[c].end(), vals.begin(), vals.end());
+                cl[c].push_back((int)cols.size());
+            }
+        }
+    }
     C.rp.assign(n + 1, 0);
+    size_t tot = 0;
+    for (int c = 0; c < nchunk; c++) tot += cc[c].size();
     C.ci.clear();
     C.v.clear();
-    std::vector<int> cols;
-    std::vector<double> vals;
-    for (int i = 0; i < n; i++) {
+    C.ci.reserve(tot);
+    C.v.reserve(tot);
+    int i = 0;
+    for (int c = 0; c < nchunk; c++) {
+        C.ci.insert(C.ci.end(), cc[c].begin(), cc[c].end());
+        C.v.insert(C.v.end(), cv[c].begin(), cv[c].end());
+        for (int len : cl[c]) {
+            C.rp[i + 1] = C.rp[i] + len;
+            i++;
+        }
+        std::vector<int>().swap(cc[c]);
+        std::vector<double>().swap(cv[c]);
+    }
+}
+
+void sparse_product(const Csr& A, const Csr& B, int b_ncols, Csr& C) {
+    const int n = A.n;
+    C.n = n;
+    C.ncols = b_ncols;
+    rowwise(n, std::max(b_ncols, n), C, [&](int i, RowAcc& acc, std::vector<int>& cols, std::vector<double>& vals) {
         acc.seed_zero(i);
         for (int p = A.rp[i]; p < A.rp[i + 1]; p++) {
             double a = A.v[p];
@@ -140,10 +180,7 @@ void sparse_product(const Csr& A, const Csr& B, int b_ncols, Csr& C) {
         }
         acc.drain(cols, vals);
         delete_zeros_row(i, cols, vals);
-        C.ci.insert(C.ci.end(), cols.begin(), cols.end());
-        C.v.insert(C.v.end(), vals.begin(), vals.end());
-        C.rp[i + 1] = (int)C.ci.size();
-    }
+    });
 }
 
 // get_restricted_matrix (Multigrid.c:475-484): R*(A*P) + square_matrix_clean
@@ -519,32 +556,23 @@ static void interpolation(const Csr& A, const int* label, const int* levels, con
         D0[i] = get_diag(A, i);
         if (D0[i] == 0.0) throw std::runtime_error("zero diagonal: standard interpolation not possible");
     }
-    // Q = A with rows divided by the diagonal; only FINE rows are modified, so
-    // the per-row state lives in a dense accumulator while a row is processed.
-    auto qval = [&](int p, int i) { return A.v[p] / D0[i]; };
-    RowAcc acc(n);
-    std::vector<int> snap_c, rc;
-    std::vector<double> snap_v, rv;
+    // rows with no coarse interpolation node: the reference `break`s there and
+    // leaves this and every later row of P at sparse_zero's (i,0.0)
+    std::vector<char> stops(n, 0);
     P.n = n;
     P.ncols = coarse_size;
-    P.rp.assign(n + 1, 0);
-    P.ci.clear();
-    P.v.clear();
-    bool broken = false;
-    std::vector<int> pc;
-    std::vector<double> pv;
-    for (int i = 0; i < n; i++) {
+    rowwise(n, n, P, [&](int i, RowAcc& acc, std::vector<int>& pc, std::vector<double>& pv) {
+        thread_local std::vector<int> snap_c, rc;
+        thread_local std::vector<double> snap_v, rv;
         pc.clear();
         pv.clear();
-        if (broken) {
-            // rows after the reference's `break` keep only sparse_zero's seed
-            pc.push_back(i);
-            pv.push_back(0.0);
-        } else if (label[i] == FINE) {
-            for (int p = A.rp[i]; p < A.rp[i + 1]; p++) acc.add(A.ci[p], qval(p, i));
+        if (label[i] == FINE) {
+            // Q row i = A row i / D0[i], then `passes` elimination passes over a
+            // snapshot of the row (clone_list/clone_vector)
+            for (int p = A.rp[i]; p < A.rp[i + 1]; p++) acc.add(A.ci[p], A.v[p] / D0[i]);
             for (int k = 0; k < passes; k++) {
-                acc.drain(snap_c, snap_v);  // snapshot (clone_list/clone_vector) ...
-                for (size_t t = 0; t < snap_c.size(); t++) acc.add(snap_c[t], snap_v[t]);  // ... and keep Q
+                acc.drain(snap_c, snap_v);
+                for (size_t t = 0; t < snap_c.size(); t++) acc.add(snap_c[t], snap_v[t]);
                 for (size_t t = 0; t < snap_c.size(); t++) {
                     int J = snap_c[t];
                     double q = snap_v[t];
@@ -590,26 +618,24 @@ static void interpolation(const Csr& A, const int* label, const int* levels, con
             if (c_m != 0) alpha = -s_m / c_m; else Di += s_m;
             if (c_p != 0) beta = -s_p / c_p; else Di += s_p;
             if (alpha == 0 && beta == 0) {
-                warn.push_back("interpolation: no coarse interpolation nodes at row " + std::to_string(i) +
-                               " (reference stops filling P here)");
-                broken = true;
+                stops[i] = 1;
                 pc.push_back(i);
                 pv.push_back(0.0);
-            } else {
-                alpha /= Di;
-                beta /= Di;
-                // P row: sparse_zero seed (i,0.0) merged with the inserts
-                acc.seed_zero(i);
-                for (size_t t = 0; t < rc.size(); t++) {
-                    int J = rc[t];
-                    double q = rv[t];
-                    if (J != i && label[J] == COARSE) {
-                        if (q < 0 && alpha != 0) acc.add(map[J], alpha * q);
-                        if (q > 0 && beta != 0) acc.add(map[J], beta * q);
-                    }
-                }
-                acc.drain(pc, pv);
+                return;
             }
+            alpha /= Di;
+            beta /= Di;
+            // P row: sparse_zero seed (i,0.0) merged with the inserts
+            acc.seed_zero(i);
+            for (size_t t = 0; t < rc.size(); t++) {
+                int J = rc[t];
+                double q = rv[t];
+                if (J != i && label[J] == COARSE) {
+                    if (q < 0 && alpha != 0) acc.add(map[J], alpha * q);
+                    if (q > 0 && beta != 0) acc.add(map[J], beta * q);
+                }
+            }
+            acc.drain(pc, pv);
         } else {
             acc.seed_zero(i);
             acc.add(map[i], 1.0);
@@ -637,9 +663,33 @@ static void interpolation(const Csr& A, const int* label, const int* levels, con
         for (size_t t = 0; t < pc.size(); t++) sum += pv[t];
         if (sum != 0)
             for (size_t t = 0; t < pc.size(); t++) pv[t] *= 1. / sum;
-        P.ci.insert(P.ci.end(), pc.begin(), pc.end());
-        P.v.insert(P.v.end(), pv.begin(), pv.end());
-        P.rp[i + 1] = (int)P.ci.size();
+    });
+    int first = -1;
+    for (int i = 0; i < n; i++)
+        if (stops[i]) {
+            first = i;
+            break;
+        }
+    if (first >= 0) {
+        warn.push_back("interpolation: no coarse interpolation nodes at row " + std::to_string(first) +
+                       " (reference stops filling P there)");
+        Csr Q;
+        Q.n = n;
+        Q.ncols = coarse_size;
+        Q.rp.assign(n + 1, 0);
+        for (int i = 0; i < n; i++) {
+            if (i < first) {
+                for (int p = P.rp[i]; p < P.rp[i + 1]; p++) {
+                    Q.ci.push_back(P.ci[p]);
+                    Q.v.push_back(P.v[p]);
+                }
+            } else {
+                Q.ci.push_back(i);
+                Q.v.push_back(0.0);
+            }
+            Q.rp[i + 1] = (int)Q.ci.size();
+        }
+        P = std::move(Q);
     }
 }
 
@@ -688,12 +738,14 @@ static void transpose_restriction(const Csr& P, int coarse_size, Csr& R) {
         }
 }
 
-// ------------------------------------------------------ coarsest LU
+// ------------------------------------------------------ coarsest level
 
-// sparse_LU_factorization (linear_algebra.c:3040-3062): rows with more than
-// one nonzero form a dense system factored by LU_factorzation (:2909-2942,
-// unit-lower L, bordering); the other rows get L=1, U=diag.  The dense
-// inverse U^-1 L^-1 is precomputed for the device-side coarse solve.
+// sparse_LU_factorization (linear_algebra.c:3040-3062) splits the coarsest
+// matrix into a dense system over the rows with more than one nonzero value
+// (rows and columns restricted to those indices, sub_convert_sparse_to_array
+// :2993-3023) and trivial rows solved as x_i = f_i / a_ii (L = 1, U = diag,
+// :3034-3037).  The dense system is extracted here; its inverse is formed on
+// the device (Gauss-Jordan, no pivoting, as LU_factorzation does not pivot).
 void coarse_lu(const Csr& A0, CoarseLU& lu) {
     const int n = A0.n;
     lu = CoarseLU();
@@ -712,66 +764,13 @@ void coarse_lu(const Csr& A0, CoarseLU& lu) {
         }
     }
     const int N = (int)lu.dense_idx.size();
-    std::vector<double> M((size_t)N * N, 0.0);
+    lu.dense.assign((size_t)N * N, 0.0);
     for (int I = 0; I < N; I++) {
         int i = lu.dense_idx[I];
         for (int p = A0.rp[i]; p < A0.rp[i + 1]; p++) {
             int J = map[A0.ci[p]];
-            if (J >= 0) M[(size_t)I * N + J] = A0.v[p];
+            if (J >= 0) lu.dense[(size_t)I * N + J] = A0.v[p];
         }
-    }
-    lu.L.assign((size_t)N * N, 0.0);
-    lu.U.assign((size_t)N * N, 0.0);
-    auto Lm = [&](int i, int j) -> double& { return lu.L[(size_t)i * N + j]; };
-    auto Um = [&](int i, int j) -> double& { return lu.U[(size_t)i * N + j]; };
-    if (N > 0) {
-        Lm(0, 0) = 1.0;
-        Um(0, 0) = M[0];
-    }
-    std::vector<double> u, l;
-    for (int s = 1; s < N; s++) {
-        // u = L^-1 a with a = A[0..s-1][s]  (Lower_triangular_invert)
-        u.assign(s, 0.0);
-        for (int i = 0; i < s; i++) u[i] = M[(size_t)i * N + s];
-        for (int i = 0; i < s; i++) {
-            for (int j = 0; j < i; j++) u[i] -= Lm(i, j) * u[j];
-            u[i] /= Lm(i, i);
-        }
-        // l = U^-T A[s][0..s-1]  (Upper_triangular_transpose_invert)
-        l.assign(s, 0.0);
-        for (int i = 0; i < s; i++) l[i] = M[(size_t)s * N + i];
-        for (int i = 0; i < s; i++) {
-            for (int j = 0; j < i; j++) l[i] -= Um(j, i) * l[j];
-            if (Um(i, i) == 0.0) throw std::runtime_error("coarse LU: zero pivot");
-            l[i] /= Um(i, i);
-        }
-        for (int i = 0; i < s; i++) {
-            Um(i, s) = u[i];
-            Lm(s, i) = l[i];
-        }
-        double diag = M[(size_t)s * N + s];
-        for (int i = 0; i < s; i++) diag -= u[i] * l[i];
-        Um(s, s) = diag;
-        Lm(s, s) = 1.0;
-    }
-    // inverse columns: x = U^-1 (L^-1 e_j), the same substitutions as
-    // L_triang_invert / U_triang_invert (linear_algebra.c:2008-2043)
-    lu.inv.assign((size_t)N * N, 0.0);
-    std::vector<double> x(N);
-    for (int j = 0; j < N; j++) {
-        std::fill(x.begin(), x.end(), 0.0);
-        x[j] = 1.0;
-        for (int i = j + 1; i < N; i++) {
-            double sum = 0.0;
-            for (int k = j; k < i; k++) sum -= Lm(i, k) * x[k];
-            x[i] = sum;  // L(i,i) == 1
-        }
-        for (int i = N - 1; i >= 0; i--) {
-            double sum = x[i];
-            for (int k = i + 1; k < N; k++) sum -= Um(i, k) * x[k];
-            x[i] = sum / Um(i, i);
-        }
-        for (int i = 0; i < N; i++) lu.inv[(size_t)i * N + j] = x[i];
     }
 }
 
@@ -902,8 +901,19 @@ int ilu0(const Csr& A, Csr& L, Csr& U) {
 
 // ------------------------------------------------------------ AMG_setup
 
+static bool verbose() {
+    static int v = -1;
+    if (v < 0) v = getenv("LINA_VERBOSE") ? 1 : 0;
+    return v == 1;
+}
+
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 void amg_setup(const Csr& Ain, const SetupParams& prm, HostHierarchy& H) {
     const int depth = prm.depth;
+    double t0 = now_s();
     if (depth < 2) throw std::runtime_error("AMG depth must be >= 2");
     H = HostHierarchy();
     H.prm = prm;
@@ -953,9 +963,17 @@ void amg_setup(const Csr& Ain, const SetupParams& prm, HostHierarchy& H) {
         int passes = (prm.coarsening == AGGRESSIVE && i == depth - 1) ? 2 : 1;
         interpolation(Af, L.labels.data(), levels.data(), map, coarse, passes, prm.matrix_tol, L.P, H.warnings);
         transpose_restriction(L.P, coarse, L.R);
+        double t1 = now_s();
         HostLevel& C = H.lv[i - 1];
         galerkin(Af, L.R, L.P, C.A);
         C.fine = coarse;
+        if (verbose()) {
+            int iso = 0;
+            for (int r = 0; r < L.fine; r++) iso += L.labels[r] == ISOLATED;
+            fprintf(stderr, "[lina] level %d: %d -> %d (isolated %d) nnz A %d P %d Ac %d  split+P %.2fs RAP %.2fs\n", i,
+                    L.fine, coarse, iso, Af.nnz(), L.P.nnz(), C.A.nnz(), t1 - t0, now_s() - t1);
+        }
+        t0 = now_s();
         levels.assign(new_levels.begin(), new_levels.begin() + coarse);
         if (i == depth - 1) {
             for (int r = 0; r < L.fine; r++)
@@ -973,6 +991,7 @@ void amg_setup(const Csr& Ain, const SetupParams& prm, HostHierarchy& H) {
     }
     H.lv[0].levels = levels;
     coarse_lu(H.lv[0].A, H.lu);
+    if (verbose()) fprintf(stderr, "[lina] coarsest: %d rows, dense %zu\n", H.lv[0].A.n, H.lu.dense_idx.size());
 }
 
 // AMG_recompute_coarse_matrices (Multigrid.c:1756-1780): new finest values,

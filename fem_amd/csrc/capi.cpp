@@ -176,26 +176,11 @@ static const Csr& pick(const lina_solver* s, int level, char which) {
 }
 
 int lina_amg_level_nnz(const lina_solver* s, int level, char which) {
-    return guard([&]() {
-        if (which == 'L' || which == 'U') {
-            size_t N = s->s.host().lu.dense_idx.size();
-            return (int)(N * N);
-        }
-        return pick(s, level, which).nnz();
-    });
+    return guard([&]() { return pick(s, level, which).nnz(); });
 }
 
 int lina_amg_level_copy(const lina_solver* s, int level, char which, int* rowptr, int* col, double* val) {
     return guard([&]() {
-        if (which == 'L' || which == 'U') {
-            // dense factors of the coarse LU, row-major over the dense rows
-            const auto& lu = s->s.host().lu;
-            const auto& M = which == 'L' ? lu.L : lu.U;
-            std::memcpy(val, M.data(), M.size() * sizeof(double));
-            for (size_t i = 0; i < lu.dense_idx.size(); i++) col[i] = lu.dense_idx[i];
-            rowptr[0] = (int)lu.dense_idx.size();
-            return 0;
-        }
         const Csr& M = pick(s, level, which);
         std::memcpy(rowptr, M.rp.data(), (M.n + 1) * sizeof(int));
         std::memcpy(col, M.ci.data(), M.nnz() * sizeof(int));
@@ -268,6 +253,39 @@ double lina_time_spmv(lina_solver* s, int level, int precision, int reps) {
         return 0;
     });
     return rc < 0 ? -1.0 : out;
+}
+
+double* lina_dev_alloc(long n) {
+    void* p = nullptr;
+    if (n < 1) n = 1;
+    if (hipMalloc(&p, (size_t)n * sizeof(double)) != hipSuccess) {
+        lina::set_error("hipMalloc of %ld doubles failed", n);
+        return nullptr;
+    }
+    return (double*)p;
+}
+
+int lina_dev_free(double* d) { return hipFree(d) == hipSuccess ? 0 : LINA_EHIP; }
+
+int lina_dev_copy(double* dst, const double* src, long n, int kind) {
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    if (n < 0 || kind < 0 || kind > 2) {
+        lina::set_error("lina_dev_copy: bad arguments");
+        return LINA_EARG;
+    }
+    if (hipMemcpy(dst, src, (size_t)n * sizeof(double), k) != hipSuccess) {
+        lina::set_error("hipMemcpy failed");
+        return LINA_EHIP;
+    }
+    return 0;
+}
+
+int lina_dev_fill(double* d, long n, double value) {
+    return guard([&]() {
+        lina::dev::fill<double>(nullptr, (int)n, d, value);
+        lina::hip_check(hipDeviceSynchronize(), "fill");
+        return 0;
+    });
 }
 
 double lina_spmv_bytes(const lina_solver* s, int level, int precision) {

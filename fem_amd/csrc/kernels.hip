@@ -12,6 +12,7 @@
 // partials stored by logical block index, a single-block finalisation.
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 
 #include "kernels.hpp"
@@ -356,7 +357,49 @@ __global__ __launch_bounds__(TPB) void k_coarse(int nd, const int* __restrict__ 
     }
 }
 
+// ------------------------------------------- coarse inverse (Gauss-Jordan)
+//
+// In-place Gauss-Jordan inversion of the dense SPD coarse matrix, no
+// pivoting (the reference's LU_factorzation does not pivot either).  Step k:
+//   a_ij -= a_ik a_kj / a_kk   (i,j != k)   -- k_gj_update, reads row/col k
+//   a_kj /= a_kk, a_ik *= -1/a_kk, a_kk = 1/a_kk -- k_gj_pivot
+__global__ __launch_bounds__(TPB) void k_gj_update(int N, int k, double* __restrict__ M, double* __restrict__ piv) {
+    const double p = 1.0 / M[(size_t)k * N + k];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *piv = p;
+    const double* rowk = M + (size_t)k * N;
+    for (int i = blockIdx.x; i < N; i += gridDim.x) {
+        if (i == k) continue;
+        double* row = M + (size_t)i * N;
+        const double f = row[k] * p;
+        for (int j = threadIdx.x; j < N; j += TPB)
+            if (j != k) row[j] -= f * rowk[j];
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_gj_pivot(int N, int k, double* __restrict__ M, const double* __restrict__ piv) {
+    const double p = *piv;
+    for (int t = blockIdx.x * TPB + threadIdx.x; t < N; t += gridDim.x * TPB) {
+        if (t == k) {
+            M[(size_t)k * N + k] = p;
+            continue;
+        }
+        M[(size_t)k * N + t] *= p;
+        M[(size_t)t * N + k] *= -p;
+    }
+}
+
 // ---------------------------------------------------------- launchers
+
+// piv: one device double of scratch
+void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* piv) {
+    if (N <= 0) return;
+    int g = std::min(N, 4096);
+    int gp = (N + TPB - 1) / TPB;
+    for (int k = 0; k < N; k++) {
+        hipLaunchKernelGGL(k_gj_update, dim3(g), dim3(TPB), 0, st, N, k, M, piv);
+        hipLaunchKernelGGL(k_gj_pivot, dim3(gp), dim3(TPB), 0, st, N, k, M, (const double*)piv);
+    }
+}
 
 static inline int grid_for(int n, int cap = 4096) {
     int g = (n + TPB - 1) / TPB;

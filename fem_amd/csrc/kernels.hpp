@@ -71,6 +71,7 @@ void sor_lex(hipStream_t st, const DMat<VT>& A, int nlv, const int* lv_off, cons
              XT* xo, XT omega, int sweeps, int backward);
 template <typename VT, typename XT>
 void jacobi(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, const XT* xo, XT* x, int nlim);
+void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* piv);
 template <typename XT>
 void coarse_solve(hipStream_t st, int nd, const int* didx, const double* inv, int nt, const int* tidx,
                   const double* tdiag, const XT* b, XT* x);

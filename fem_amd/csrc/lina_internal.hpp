@@ -39,8 +39,7 @@ struct CoarseLU {
     std::vector<int> dense_idx;      // rows/cols that enter the dense system
     std::vector<int> trivial_idx;    // rows with <= 1 nonzero (L=1, U=diag)
     std::vector<double> trivial_diag;
-    std::vector<double> L, U;        // dense factors, row-major, nd x nd
-    std::vector<double> inv;         // U^-1 L^-1, row-major, nd x nd
+    std::vector<double> dense;       // dense system, row-major, nd x nd
 };
 
 struct SetupParams {

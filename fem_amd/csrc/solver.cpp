@@ -307,7 +307,11 @@ void Solver::upload() {
     nd_ = (int)lu.dense_idx.size();
     nt_ = (int)lu.trivial_idx.size();
     didx_ = dupload(lu.dense_idx, B, st);
-    inv_ = dupload(lu.inv, B, st);
+    inv_ = dupload(lu.dense, B, st);
+    {
+        double* piv = dalloc<double>(1, B);
+        dev::gauss_jordan_inverse(st, nd_, inv_, piv);
+    }
     tidx_ = dupload(lu.trivial_idx, B, st);
     tdiag_ = dupload(lu.trivial_diag, B, st);
     // Krylov vectors and scratch

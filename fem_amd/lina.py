@@ -75,6 +75,11 @@ def lib():
     L.lina_time_spmv.restype = C.c_double
     L.lina_spmv_bytes.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.lina_spmv_bytes.restype = C.c_double
+    L.lina_dev_alloc.argtypes = [C.c_long]
+    L.lina_dev_alloc.restype = C.c_void_p
+    L.lina_dev_free.argtypes = [C.c_void_p]
+    L.lina_dev_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_long, C.c_int]
+    L.lina_dev_fill.argtypes = [C.c_void_p, C.c_long, C.c_double]
     _lib = L
     return L
 
@@ -170,17 +175,6 @@ class AMG:
             out["colours"] = col
         return out
 
-    def coarse_lu(self):
-        nn = _check(lib().lina_amg_level_nnz(self.h, 0, b"L"))
-        N = int(round(nn ** 0.5))
-        rp = np.zeros(1, np.int32)
-        idx = np.zeros(max(N, 1), np.int32)
-        Lm = np.zeros(max(nn, 1))
-        Um = np.zeros(max(nn, 1))
-        _check(lib().lina_amg_level_copy(self.h, 0, b"L", _i(rp), _i(idx), _d(Lm)))
-        _check(lib().lina_amg_level_copy(self.h, 0, b"U", _i(rp), _i(idx), _d(Um)))
-        return idx[:N], Lm[:nn].reshape(N, N), Um[:nn].reshape(N, N)
-
     def recompute(self, val):
         v = np.ascontiguousarray(val, dtype=np.float64)
         _check(lib().lina_amg_recompute(self.h, _d(v)))
@@ -227,3 +221,37 @@ class AMG:
     def spmv_bytes(self, level=None, precision="fp64") -> float:
         l = self.depth - 1 if level is None else level
         return lib().lina_spmv_bytes(self.h, l, PREC[precision])
+
+
+class DeviceVector:
+    """An FP64 vector in device memory allocated by liblina_hip."""
+
+    def __init__(self, n: int):
+        self.n = n
+        self.ptr = lib().lina_dev_alloc(n)
+        if not self.ptr:
+            raise LinaError(lib().lina_last_error().decode())
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == self.n
+        _check(lib().lina_dev_copy(self.ptr, a.ctypes.data, self.n, 0))
+
+    def download(self):
+        out = np.empty(self.n)
+        _check(lib().lina_dev_copy(out.ctypes.data, self.ptr, self.n, 1))
+        return out
+
+    def copy_from(self, other: "DeviceVector"):
+        _check(lib().lina_dev_copy(self.ptr, other.ptr, self.n, 2))
+
+    def fill(self, v: float):
+        _check(lib().lina_dev_fill(self.ptr, self.n, v))
+
+    def free(self):
+        if self.ptr:
+            lib().lina_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()

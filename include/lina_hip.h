@@ -120,6 +120,14 @@ int lina_pcg_device(lina_solver* s, double* d_x, double* d_b, double tol, int ma
  * returns the number of V-cycles. */
 int lina_amg_solve(lina_solver* s, const double* f, double* x);
 
+/* device vectors owned by the library (FP64): the solve entry points above
+ * take pointers from here when inputs should stay resident in HBM.
+ * kind: 0 host->device, 1 device->host, 2 device->device */
+double* lina_dev_alloc(long n);
+int lina_dev_free(double* d);
+int lina_dev_copy(double* dst, const double* src, long n, int kind);
+int lina_dev_fill(double* d, long n, double value);
+
 /* single kernels on the finest level for verification and timing:
  * y = A x (precision FP64 or FP32 copy of A), host pointers */
 int lina_spmv(lina_solver* s, int level, int precision, const double* x, double* y);

@@ -226,3 +226,99 @@ class Ref:
         it = self.L.refh_reflective_newton(H, dptr(b), dptr(lo), dptr(up), dptr(X), solver, dof, depth, max_iter,
                                            eps, delta, C.byref(q))
         return {"iters": it, "x": X, "q": q.value}
+
+
+class Oracle:
+    """The C restatement (oracle/liboracle.so)."""
+
+    def __init__(self, path: str = ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (make -C oracle oracle)")
+        L = C.CDLL(path)
+        self.L = L
+        L.orc_amg_setup.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_double, C.c_double, C.c_double]
+        L.orc_amg_setup.restype = _vp
+        L.orc_amg_free.argtypes = [_vp]
+        L.orc_amg_level.argtypes = [_vp, C.c_int, _ip, _ip, _ip, _ip, _ip]
+        L.orc_amg_copy.argtypes = [_vp, C.c_int, C.c_char, _ip, _ip, _dp]
+        L.orc_amg_labels.argtypes = [_vp, C.c_int, _ip]
+        L.orc_amg_recompute.argtypes = [_vp, _dp]
+        L.orc_amg_solve.argtypes = [_vp, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int]
+        L.orc_pcg_amg.argtypes = [_vp, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                  C.c_int, C.c_int, _dp, _ip]
+        L.orc_colouring.argtypes = [C.c_int, _ip, _ip, _dp, _ip]
+        L.orc_spmv.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp]
+
+    def amg_setup(self, rowptr, col, val, dof=1, depth=4, smiter=2, aggr_smiter=3, coarsening=1,
+                  eps_minus=0.3, eps_plus=0.5, matrix_tol=0.1):
+        rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+        ci = np.ascontiguousarray(col, dtype=np.int32)
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        h = self.L.orc_amg_setup(len(rp) - 1, iptr(rp), iptr(ci), dptr(v), dof, depth, smiter, aggr_smiter,
+                                 coarsening, eps_minus, eps_plus, matrix_tol)
+        if not h:
+            raise RuntimeError("oracle AMG setup failed")
+        return h
+
+    def amg_free(self, h):
+        self.L.orc_amg_free(h)
+
+    def amg_levels(self, h, depth):
+        out = []
+        for l in range(depth):
+            f, c, na, np_, nr = (C.c_int() for _ in range(5))
+            self.L.orc_amg_level(h, l, C.byref(f), C.byref(c), C.byref(na), C.byref(np_), C.byref(nr))
+            d = {"fine": f.value, "coarse": c.value}
+            for which, nnz, rows in (("A", na.value, f.value), ("P", np_.value, f.value), ("R", nr.value, c.value)):
+                if l == 0 and which != "A":
+                    continue
+                rp = np.zeros(rows + 1, np.int32)
+                ci = np.zeros(max(nnz, 1), np.int32)
+                v = np.zeros(max(nnz, 1), np.float64)
+                self.L.orc_amg_copy(h, l, which.encode(), iptr(rp), iptr(ci), dptr(v))
+                d[which] = CSR(rp, ci[:nnz], v[:nnz])
+            if l > 0:
+                lab = np.zeros(f.value, np.int32)
+                self.L.orc_amg_labels(h, l, iptr(lab))
+                d["labels"] = lab
+            out.append(d)
+        return out
+
+    def amg_solve(self, h, F, X, smoother="sor", alpha=1.6, eps=1e-2, max_iter=100):
+        F = np.array(F, dtype=np.float64)
+        X = np.array(X, dtype=np.float64)
+        k = self.L.orc_amg_solve(h, dptr(F), dptr(X), SMOOTHERS[smoother], alpha, eps, max_iter)
+        return k, X
+
+    def pcg_amg(self, rowptr, col, val, b, x0=None, dof=1, depth=4, smiter=2, aggr_smiter=3, coarsening=1,
+                smoother="sor", alpha=1.6, amg_eps=1e-2, amg_max_iter=100, zero_init=False, tol=1e-10,
+                max_iter=1000, stop="reference"):
+        h = self.amg_setup(rowptr, col, val, dof, depth, smiter, aggr_smiter, coarsening)
+        n = len(rowptr) - 1
+        X = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+        B = np.array(b, dtype=np.float64)
+        res = C.c_double()
+        vc = C.c_int()
+        it = self.L.orc_pcg_amg(h, dptr(X), dptr(B), tol, max_iter, SMOOTHERS[smoother], alpha, amg_eps,
+                                amg_max_iter, int(zero_init), 1 if stop == "relres" else 0, C.byref(res),
+                                C.byref(vc))
+        self.amg_free(h)
+        return {"iters": it, "x": X, "b_out": B, "res": res.value, "vcycles": vc.value}
+
+    def colouring(self, rowptr, col, val):
+        n = len(rowptr) - 1
+        c = np.zeros(n, np.int32)
+        k = self.L.orc_colouring(n, iptr(np.ascontiguousarray(rowptr, dtype=np.int32)),
+                                 iptr(np.ascontiguousarray(col, dtype=np.int32)),
+                                 dptr(np.ascontiguousarray(val, dtype=np.float64)), iptr(c))
+        return k, c
+
+    def spmv(self, rowptr, col, val, x):
+        n = len(rowptr) - 1
+        y = np.zeros(n)
+        self.L.orc_spmv(n, iptr(np.ascontiguousarray(rowptr, dtype=np.int32)),
+                        iptr(np.ascontiguousarray(col, dtype=np.int32)),
+                        dptr(np.ascontiguousarray(val, dtype=np.float64)),
+                        dptr(np.ascontiguousarray(x, dtype=np.float64)), dptr(y))
+        return y

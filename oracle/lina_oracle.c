@@ -1,0 +1,1368 @@
+/*
+ * lina_oracle.c -- CPU restatement of the LinA solver path, the parity
+ * oracle for liblina_hip.
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and
+ * bench.py through oracle/bind.py; never linked into the product.
+ *
+ * Each routine restates the reference routine cited above it, operation by
+ * operation (compiled with -ffp-contract=off: no fused multiply-adds), on
+ * CSR arrays instead of the reference's row-lists.  The reference's growth of
+ * rows by insert_sparse (linear_algebra.c:1124-1159) is replaced by a dense
+ * row accumulator that applies the same additions in the same order.  The
+ * red-black tree that orders coarsening candidates (red_black_tree.c) is
+ * replaced by a binary max-heap on (importance, insertion number): equal keys
+ * go to the right of existing ones in the tree, so the rightmost node taken
+ * by RBLargest is the most recently inserted among the largest keys.
+ *
+ * Pinning: tests/test_oracle_cpu.py checks every routine here against the
+ * compiled reference (oracle/_ref) and against the committed golden vectors
+ * in tests/golden/, plus closed-form checks (dense solves, CG in exact
+ * arithmetic on tiny systems, invariants of the AMG hierarchy).
+ */
+#include "lina_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+enum { UNDECIDED = 0, COARSE = 1, FINE = 2, ISOLATED = 3 };
+enum { AGGRESSIVE = 1, STANDARD = 2 };
+
+/* ------------------------------------------------------------ containers */
+
+typedef struct {
+    int* a;
+    int n, cap;
+} ivec;
+typedef struct {
+    double* a;
+    int n, cap;
+} dvec;
+
+static void ipush(ivec* v, int x) {
+    if (v->n == v->cap) {
+        v->cap = v->cap ? 2 * v->cap : 16;
+        v->a = (int*)realloc(v->a, v->cap * sizeof(int));
+    }
+    v->a[v->n++] = x;
+}
+static void dpush(dvec* v, double x) {
+    if (v->n == v->cap) {
+        v->cap = v->cap ? 2 * v->cap : 16;
+        v->a = (double*)realloc(v->a, v->cap * sizeof(double));
+    }
+    v->a[v->n++] = x;
+}
+
+typedef struct {
+    int n, ncols;
+    int* rp;
+    ivec ci;
+    dvec v;
+} Mat;
+
+static void mat_init(Mat* M, int n, int ncols) {
+    memset(M, 0, sizeof(*M));
+    M->n = n;
+    M->ncols = ncols;
+    M->rp = (int*)calloc(n + 1, sizeof(int));
+}
+static void mat_free(Mat* M) {
+    free(M->rp);
+    free(M->ci.a);
+    free(M->v.a);
+    memset(M, 0, sizeof(*M));
+}
+static int mat_nnz(const Mat* M) { return M->rp[M->n]; }
+static void mat_from(Mat* M, int n, const int* rp, const int* ci, const double* v) {
+    int k;
+    mat_init(M, n, n);
+    memcpy(M->rp, rp, (n + 1) * sizeof(int));
+    for (k = 0; k < rp[n]; k++) {
+        ipush(&M->ci, ci[k]);
+        dpush(&M->v, v[k]);
+    }
+}
+
+/* dense row accumulator: the value/presence of one row being built */
+typedef struct {
+    double* val;
+    char* has;
+    ivec touched;
+    int width;
+} Acc;
+
+static void acc_init(Acc* A, int width) {
+    A->val = (double*)calloc(width > 0 ? width : 1, sizeof(double));
+    A->has = (char*)calloc(width > 0 ? width : 1, 1);
+    A->touched.a = NULL;
+    A->touched.n = A->touched.cap = 0;
+    A->width = width;
+}
+static void acc_free(Acc* A) {
+    free(A->val);
+    free(A->has);
+    free(A->touched.a);
+}
+/* insert_sparse: add to an existing entry, create a new one otherwise */
+static void acc_add(Acc* A, int j, double a) {
+    if (A->has[j]) A->val[j] += a;
+    else {
+        A->has[j] = 1;
+        A->val[j] = a;
+        ipush(&A->touched, j);
+    }
+}
+/* sparse_zero's seed (j, 0.0) */
+static void acc_seed(Acc* A, int j) {
+    if (!A->has[j]) {
+        A->has[j] = 1;
+        A->val[j] = 0.0;
+        ipush(&A->touched, j);
+    }
+}
+/* remove_element (linear_algebra.c:1181-1185) */
+static void acc_remove(Acc* A, int j) {
+    if (A->has[j]) {
+        A->has[j] = 0;
+        A->val[j] = 0.0;
+    }
+}
+static int cmp_int(const void* a, const void* b) {
+    int x = *(const int*)a, y = *(const int*)b;
+    return (x > y) - (x < y);
+}
+/* move the live entries, sorted by column, into (c, v); reset */
+static void acc_take(Acc* A, ivec* c, dvec* v) {
+    int t, last = -1;
+    c->n = 0;
+    v->n = 0;
+    qsort(A->touched.a, A->touched.n, sizeof(int), cmp_int);
+    for (t = 0; t < A->touched.n; t++) {
+        int j = A->touched.a[t];
+        if (j == last) continue;
+        last = j;
+        if (A->has[j]) {
+            ipush(c, j);
+            dpush(v, A->val[j]);
+        }
+        A->has[j] = 0;
+        A->val[j] = 0.0;
+    }
+    A->touched.n = 0;
+}
+
+/* delete_zeros (linear_algebra.c:793-805) on one row seeded by sparse_zero:
+ * rows with more than one entry lose their exact zeros; an all-zero row ends
+ * as the single entry (row, 0.0) */
+static void row_delete_zeros(int row, ivec* c, dvec* v) {
+    int k, w = 0;
+    if (c->n <= 1) return;
+    for (k = 0; k < c->n; k++)
+        if (v->a[k] != 0.0) {
+            c->a[w] = c->a[k];
+            v->a[w] = v->a[k];
+            w++;
+        }
+    if (w == 0) {
+        c->a[0] = row;
+        v->a[0] = 0.0;
+        w = 1;
+    }
+    c->n = w;
+    v->n = w;
+}
+
+static void mat_append_row(Mat* M, int i, const ivec* c, const dvec* v) {
+    int k;
+    for (k = 0; k < c->n; k++) {
+        ipush(&M->ci, c->a[k]);
+        dpush(&M->v, v->a[k]);
+    }
+    M->rp[i + 1] = M->ci.n;
+}
+
+/* get_diag (linear_algebra.c:3434-3439) */
+static double diag_of(const Mat* A, int i) {
+    int p;
+    for (p = A->rp[i]; p < A->rp[i + 1]; p++)
+        if (A->ci.a[p] == i) return A->v.a[p];
+    return 0.0;
+}
+
+/* sparse_product (linear_algebra.c:186-222): C = A*B, row i seeded with
+ * (i,0.0), products added in A-row then B-row order, then delete_zeros */
+static void sparse_product(const Mat* A, const Mat* B, int b_ncols, Mat* C) {
+    int i, p, q, width = b_ncols > A->n ? b_ncols : A->n;
+    Acc acc;
+    ivec c = {0};
+    dvec v = {0};
+    acc_init(&acc, width);
+    mat_init(C, A->n, b_ncols);
+    for (i = 0; i < A->n; i++) {
+        acc_seed(&acc, i);
+        for (p = A->rp[i]; p < A->rp[i + 1]; p++) {
+            double a = A->v.a[p];
+            int k = A->ci.a[p];
+            if (a == 0) continue;
+            for (q = B->rp[k]; q < B->rp[k + 1]; q++) {
+                double b = B->v.a[q];
+                if (b != 0) acc_add(&acc, B->ci.a[q], a * b);
+            }
+        }
+        acc_take(&acc, &c, &v);
+        row_delete_zeros(i, &c, &v);
+        mat_append_row(C, i, &c, &v);
+    }
+    acc_free(&acc);
+    free(c.a);
+    free(v.a);
+}
+
+/* get_restricted_matrix (Multigrid.c:475-484) = R*(A*P), square_matrix_clean
+ * (linear_algebra.c:3100-3120) */
+static int galerkin(const Mat* A, const Mat* R, const Mat* P, Mat* out) {
+    Mat AP, RAP;
+    int i, p, n;
+    sparse_product(A, P, P->ncols, &AP);
+    sparse_product(R, &AP, P->ncols, &RAP);
+    mat_free(&AP);
+    n = RAP.n;
+    mat_init(out, n, n);
+    for (i = 0; i < n; i++) {
+        for (p = RAP.rp[i]; p < RAP.rp[i + 1]; p++) {
+            int j = RAP.ci.a[p];
+            double x = RAP.v.a[p];
+            if (j >= n) {
+                if (x == 0) continue;
+                mat_free(&RAP);
+                return 0;
+            }
+            if (x == 0 && j != i) continue;
+            ipush(&out->ci, j);
+            dpush(&out->v, x);
+        }
+        out->rp[i + 1] = out->ci.n;
+    }
+    mat_free(&RAP);
+    return 1;
+}
+
+/* ----------------------------------------------------- coarsening heap */
+
+typedef struct {
+    double key;
+    long seq;
+    int node;
+} hent;
+typedef struct {
+    hent* h;
+    int n, cap;
+    int* pos; /* heap index of a node, -1 when absent (Node_list[I]==NULL) */
+    long seq;
+} Heap;
+
+static int h_above(const hent* a, const hent* b) { return a->key > b->key || (a->key == b->key && a->seq > b->seq); }
+static void h_swap(Heap* H, int i, int j) {
+    hent t = H->h[i];
+    H->h[i] = H->h[j];
+    H->h[j] = t;
+    H->pos[H->h[i].node] = i;
+    H->pos[H->h[j].node] = j;
+}
+static void h_up(Heap* H, int i) {
+    while (i > 0) {
+        int p = (i - 1) / 2;
+        if (!h_above(&H->h[i], &H->h[p])) break;
+        h_swap(H, i, p);
+        i = p;
+    }
+}
+static void h_down(Heap* H, int i) {
+    for (;;) {
+        int l = 2 * i + 1, r = l + 1, m = i;
+        if (l < H->n && h_above(&H->h[l], &H->h[m])) m = l;
+        if (r < H->n && h_above(&H->h[r], &H->h[m])) m = r;
+        if (m == i) break;
+        h_swap(H, i, m);
+        i = m;
+    }
+}
+static void h_init(Heap* H, int nodes) {
+    int i;
+    H->n = 0;
+    H->cap = 64;
+    H->h = (hent*)malloc(H->cap * sizeof(hent));
+    H->pos = (int*)malloc((nodes > 0 ? nodes : 1) * sizeof(int));
+    for (i = 0; i < nodes; i++) H->pos[i] = -1;
+    H->seq = 0;
+}
+static void h_free(Heap* H) {
+    free(H->h);
+    free(H->pos);
+}
+static void h_insert(Heap* H, int node, double key) {
+    if (H->n == H->cap) {
+        H->cap *= 2;
+        H->h = (hent*)realloc(H->h, H->cap * sizeof(hent));
+    }
+    H->h[H->n].key = key;
+    H->h[H->n].seq = H->seq++;
+    H->h[H->n].node = node;
+    H->pos[node] = H->n;
+    H->n++;
+    h_up(H, H->n - 1);
+}
+static void h_remove(Heap* H, int node) {
+    int i = H->pos[node];
+    if (i < 0) return;
+    H->n--;
+    if (i != H->n) {
+        H->h[i] = H->h[H->n];
+        H->pos[H->h[i].node] = i;
+        h_down(H, i);
+        h_up(H, i);
+    }
+    H->pos[node] = -1;
+}
+
+/* ------------------------------------------------------------ coarsening */
+
+typedef struct {
+    int* ptr;
+    int* idx;
+} Lists;
+static void lists_free(Lists* L) {
+    free(L->ptr);
+    free(L->idx);
+}
+
+/* get_strong_connection_sets (Multigrid.c:240-258) with get_strong_connected
+ * (:156-184) and get_max_coupled (:86-97) */
+static void strong_sets(const Mat* A, const int* levels, double eps_minus, double eps_plus, Lists* S, Lists* T) {
+    int n = A->n, i, p, k;
+    ivec idx = {0};
+    int* cnt;
+    S->ptr = (int*)calloc(n + 1, sizeof(int));
+    for (i = 0; i < n; i++) {
+        double mn = 0;
+        int mi = -1;
+        for (p = A->rp[i]; p < A->rp[i + 1]; p++)
+            if (A->ci.a[p] != i && A->v.a[p] < mn) {
+                mn = A->v.a[p];
+                mi = p;
+            }
+        if (mi >= 0) {
+            double lo = -eps_minus * fabs(A->v.a[mi]);
+            double hi = eps_plus * fabs(A->v.a[mi]);
+            for (p = A->rp[i]; p < A->rp[i + 1]; p++) {
+                int J = A->ci.a[p];
+                double a;
+                if (J == i) continue;
+                a = levels[i] == levels[J] ? A->v.a[p] : 0;
+                if (a < lo || a > hi) ipush(&idx, J);
+            }
+        }
+        S->ptr[i + 1] = idx.n;
+    }
+    S->idx = idx.a;
+    T->ptr = (int*)calloc(n + 1, sizeof(int));
+    T->idx = (int*)malloc((idx.n > 0 ? idx.n : 1) * sizeof(int));
+    for (k = 0; k < idx.n; k++) T->ptr[idx.a[k] + 1]++;
+    for (i = 0; i < n; i++) T->ptr[i + 1] += T->ptr[i];
+    cnt = (int*)malloc((n + 1) * sizeof(int));
+    memcpy(cnt, T->ptr, (n + 1) * sizeof(int));
+    for (i = 0; i < n; i++)
+        for (p = S->ptr[i]; p < S->ptr[i + 1]; p++) T->idx[cnt[S->idx[p]]++] = i;
+    free(cnt);
+}
+
+/* AMG_get_lambda (Multigrid.c:364-375) */
+static double lambda_of(const int* label, const Lists* T, int i, int size) {
+    int p, undec = 0, fine = 0;
+    for (p = T->ptr[i]; p < T->ptr[i + 1]; p++) {
+        int J = T->idx[p];
+        if (label[J] == UNDECIDED) undec++;
+        if (label[J] == FINE) fine++;
+    }
+    return (double)i / size + undec + 2 * fine;
+}
+
+/* AMG_coarsener (Multigrid.c:377-473); `levels` is indexed with this
+ * matrix's own row numbers as in the reference */
+static void coarsener(const Mat* A, int* label, double* imp, const Lists* S, const Lists* T, Heap* H,
+                      const int* levels, int size, int seed) {
+    int level = levels[seed], next = seed, p, q, i;
+    do {
+        label[next] = COARSE;
+        for (p = A->rp[next]; p < A->rp[next + 1]; p++) {
+            int I = A->ci.a[p];
+            if (label[I] == UNDECIDED && A->v.a[p] != 0. && H->pos[I] < 0) h_insert(H, I, imp[I]);
+        }
+        for (q = T->ptr[next]; q < T->ptr[next + 1]; q++) {
+            int I = T->idx[q];
+            if (H->pos[I] >= 0) {
+                label[I] = FINE;
+                h_remove(H, I);
+            }
+        }
+        for (p = A->rp[next]; p < A->rp[next + 1]; p++) {
+            int I = A->ci.a[p];
+            if (A->v.a[p] != 0. && H->pos[I] >= 0) {
+                double lam = lambda_of(label, T, I, size);
+                if (lam != imp[I]) {
+                    h_remove(H, I);
+                    imp[I] = lam;
+                    h_insert(H, I, imp[I]);
+                }
+            }
+        }
+        for (p = A->rp[next]; p < A->rp[next + 1]; p++) {
+            int I = A->ci.a[p];
+            if (I == next) continue;
+            for (q = A->rp[I]; q < A->rp[I + 1]; q++) {
+                int J = A->ci.a[q];
+                if (label[J] == UNDECIDED && A->v.a[q] != 0. && H->pos[J] < 0) h_insert(H, J, imp[J]);
+            }
+            for (q = S->ptr[I]; q < S->ptr[I + 1]; q++) {
+                int J = S->idx[q];
+                if (label[J] == UNDECIDED) {
+                    double lam = lambda_of(label, T, J, size);
+                    if (lam != imp[J]) {
+                        h_remove(H, J);
+                        imp[J] = lam;
+                        h_insert(H, J, imp[J]);
+                    }
+                }
+            }
+        }
+        if (H->n > 0) {
+            next = H->h[0].node;
+            h_remove(H, next);
+        } else {
+            next = -1;
+            for (i = 0; i < size; i++)
+                if (label[i] == UNDECIDED && levels[i] == level) {
+                    next = i;
+                    break;
+                }
+        }
+    } while (next >= 0);
+}
+
+/* AMG_is_isolated (Multigrid.c:498-503) */
+static int isolated_row(const Mat* A, int i) {
+    int p, nz = 0;
+    for (p = A->rp[i]; p < A->rp[i + 1]; p++)
+        if (A->v.a[p] != 0) nz++;
+    return nz == 1;
+}
+
+/* standard_coarsening (Multigrid.c:506-548); premode: isolated not counted */
+static int standard_coarsening(const Mat* A, int* label, Lists* S, Lists* T, const int* levels, int dof,
+                               int premode, double em, double ep) {
+    int n = A->n, i, j, d, coarse = 0;
+    double* imp = (double*)calloc(n > 0 ? n : 1, sizeof(double));
+    Heap H;
+    strong_sets(A, levels, em, ep, S, T);
+    for (i = 0; i < n; i++) {
+        if (S->ptr[i + 1] == S->ptr[i]) label[i] = isolated_row(A, i) ? ISOLATED : COARSE;
+        else label[i] = UNDECIDED;
+    }
+    h_init(&H, n);
+    for (d = 0; d < dof; d++) {
+        int seed = -1;
+        for (j = 0; j < n; j++)
+            if (label[j] == UNDECIDED && levels[j] == d) {
+                seed = j;
+                break;
+            }
+        if (seed >= 0) coarsener(A, label, imp, S, T, &H, levels, n, seed);
+    }
+    h_free(&H);
+    for (i = 0; i < n; i++) {
+        if (label[i] == UNDECIDED) label[i] = FINE;
+        if (label[i] == COARSE) coarse++;
+    }
+    if (!premode)
+        for (i = 0; i < n; i++)
+            if (label[i] == ISOLATED) coarse++;
+    free(imp);
+    return coarse;
+}
+
+/* aggressive_coarsening (Multigrid.c:550-607) with get_2nd_max_coupled
+ * (:99-119), Get_2nd_strong_connected (:187-217) and
+ * get_2nd_strong_connection_sets (:260-292).  The uninitialised
+ * Importance_2nd of the reference is taken as zero. */
+static int aggressive_coarsening(const Mat* A, int* label, Lists* S, Lists* T, const int* levels, int dof,
+                                 double em, double ep) {
+    int n = A->n, i, p, q, I, d, k, coarse = 0;
+    int* pre = (int*)malloc((n > 0 ? n : 1) * sizeof(int));
+    int pre_size = standard_coarsening(A, pre, S, T, levels, dof, 1, em, ep);
+    int* map = (int*)malloc((n > 0 ? n : 1) * sizeof(int));
+    int* inv = (int*)malloc((pre_size > 0 ? pre_size : 1) * sizeof(int));
+    Mat PA;
+    Lists M2, T2;
+    ivec m2 = {0}, c = {0};
+    dvec v = {0};
+    Acc acc;
+    int *label2, *cnt;
+    double* imp2;
+    Heap H;
+    for (i = 0; i < n; i++) label[i] = pre[i];
+    k = 0;
+    for (i = 0; i < n; i++) {
+        map[i] = -1;
+        if (pre[i] == COARSE) {
+            map[i] = k;
+            inv[k] = i;
+            k++;
+        }
+    }
+    mat_init(&PA, pre_size, pre_size);
+    M2.ptr = (int*)calloc(pre_size + 1, sizeof(int));
+    acc_init(&acc, pre_size);
+    for (i = 0; i < n; i++) {
+        double mx = 0;
+        if (pre[i] != COARSE) continue;
+        I = map[i];
+        /* get_2nd_max_coupled */
+        for (p = A->rp[i]; p < A->rp[i + 1]; p++) {
+            double val = A->v.a[p];
+            int J = A->ci.a[p];
+            double dJ = diag_of(A, J);
+            if (J != i && val != 0)
+                for (q = A->rp[J]; q < A->rp[J + 1]; q++) {
+                    double val2 = A->v.a[q];
+                    double pp = fabs(val * val2 / dJ);
+                    int K = A->ci.a[q];
+                    if (K != J && K != i && val2 != 0 && pp > mx) mx = pp;
+                }
+        }
+        if (mx > 0) {
+            double lo = -em * mx, hi = ep * mx;
+            for (p = A->rp[i]; p < A->rp[i + 1]; p++) {
+                int J = A->ci.a[p];
+                double a = A->v.a[p], dJ = diag_of(A, J);
+                if (J != i && a != 0 && levels[i] == levels[J])
+                    for (q = A->rp[J]; q < A->rp[J + 1]; q++) {
+                        int K = A->ci.a[q];
+                        double b = A->v.a[q];
+                        if (pre[K] == COARSE && K != J && K != i && a * b / dJ > 0 && levels[K] == levels[i])
+                            if (-a * b / dJ < lo || a * b / dJ > hi) ipush(&m2, map[K]);
+                    }
+            }
+        }
+        M2.ptr[I + 1] = m2.n;
+        acc_seed(&acc, I);
+        for (p = A->rp[i]; p < A->rp[i + 1]; p++) {
+            int J = A->ci.a[p];
+            if (J != i && A->v.a[p] != 0)
+                for (q = A->rp[J]; q < A->rp[J + 1]; q++) {
+                    int K = A->ci.a[q];
+                    if (pre[K] == COARSE && K != J && K != i && A->v.a[q] != 0) acc_add(&acc, map[K], 1.);
+                }
+        }
+        acc_take(&acc, &c, &v);
+        mat_append_row(&PA, I, &c, &v);
+    }
+    M2.idx = m2.a;
+    T2.ptr = (int*)calloc(pre_size + 1, sizeof(int));
+    T2.idx = (int*)malloc((m2.n > 0 ? m2.n : 1) * sizeof(int));
+    for (k = 0; k < m2.n; k++) T2.ptr[m2.a[k] + 1]++;
+    for (I = 0; I < pre_size; I++) T2.ptr[I + 1] += T2.ptr[I];
+    cnt = (int*)malloc((pre_size + 1) * sizeof(int));
+    memcpy(cnt, T2.ptr, (pre_size + 1) * sizeof(int));
+    for (I = 0; I < pre_size; I++)
+        for (p = M2.ptr[I]; p < M2.ptr[I + 1]; p++) T2.idx[cnt[M2.idx[p]]++] = I;
+    free(cnt);
+    label2 = (int*)calloc(pre_size > 0 ? pre_size : 1, sizeof(int));
+    imp2 = (double*)calloc(pre_size > 0 ? pre_size : 1, sizeof(double));
+    h_init(&H, pre_size);
+    for (d = 0; d < dof; d++) {
+        int seed = -1;
+        for (I = 0; I < pre_size; I++)
+            if (levels[inv[I]] == d) {
+                seed = I;
+                break;
+            }
+        if (seed >= 0) coarsener(&PA, label2, imp2, &M2, &T2, &H, levels, pre_size, seed);
+    }
+    h_free(&H);
+    for (i = 0; i < n; i++) {
+        if (label[i] == COARSE) {
+            label[i] = label2[map[i]];
+            if (label[i] == COARSE) coarse++;
+        } else if (label[i] == ISOLATED)
+            coarse++;
+    }
+    acc_free(&acc);
+    mat_free(&PA);
+    lists_free(&M2);
+    lists_free(&T2);
+    free(label2);
+    free(imp2);
+    free(pre);
+    free(map);
+    free(inv);
+    free(c.a);
+    free(v.a);
+    return coarse;
+}
+
+/* AMG_simple_standard_interpolation (Multigrid.c:989-1079), then
+ * sparse_approximate (linear_algebra.c:1418-1442) and normalize_row_sum
+ * (:455-464) as in set_prolongation_map (Multigrid.c:1265-1286) */
+static int interpolation(const Mat* A, const int* label, const int* levels, const int* map, int coarse,
+                         int passes, double tol, Mat* P) {
+    int n = A->n, i, p, k, t, stop = 0;
+    double* D = (double*)malloc((n > 0 ? n : 1) * sizeof(double));
+    Acc acc;
+    ivec snc = {0}, rc = {0};
+    dvec snv = {0}, rv = {0};
+    for (i = 0; i < n; i++) {
+        D[i] = diag_of(A, i);
+        if (D[i] == 0) {
+            free(D);
+            return 0;
+        }
+    }
+    acc_init(&acc, n);
+    mat_init(P, n, coarse);
+    for (i = 0; i < n; i++) {
+        if (stop) {
+            acc_seed(&acc, i);
+        } else if (label[i] == FINE) {
+            double s_p = 0, c_p = 0, s_m = 0, c_m = 0, alpha = 0, beta = 0, Di = D[i];
+            int w = 0;
+            for (p = A->rp[i]; p < A->rp[i + 1]; p++) acc_add(&acc, A->ci.a[p], A->v.a[p] / D[i]);
+            for (k = 0; k < passes; k++) {
+                acc_take(&acc, &snc, &snv);
+                for (t = 0; t < snc.n; t++) acc_add(&acc, snc.a[t], snv.a[t]);
+                for (t = 0; t < snc.n; t++) {
+                    int J = snc.a[t];
+                    double q = snv.a[t];
+                    if (J != i && label[J] == FINE && q != 0) {
+                        acc_remove(&acc, J);
+                        for (p = A->rp[J]; p < A->rp[J + 1]; p++) {
+                            int L = A->ci.a[p];
+                            double a = A->v.a[p] / D[J];
+                            if (L != J && a != 0) acc_add(&acc, L, -a * q);
+                        }
+                    }
+                }
+            }
+            acc_take(&acc, &rc, &rv);
+            for (t = 0; t < rc.n; t++)
+                if (levels[rc.a[t]] == levels[i]) {
+                    rc.a[w] = rc.a[t];
+                    rv.a[w] = rv.a[t];
+                    w++;
+                }
+            rc.n = rv.n = w;
+            for (t = 0; t < rc.n; t++) {
+                int J = rc.a[t];
+                double q = rv.a[t];
+                if (J != i) {
+                    if (q < 0) {
+                        s_m += q;
+                        if (label[J] == COARSE) c_m += q;
+                    } else {
+                        s_p += q;
+                        if (label[J] == COARSE) c_p += q;
+                    }
+                } else
+                    Di = q;
+            }
+            if (c_m != 0) alpha = -s_m / c_m;
+            else Di += s_m;
+            if (c_p != 0) beta = -s_p / c_p;
+            else Di += s_p;
+            acc_seed(&acc, i);
+            if (alpha == 0 && beta == 0) {
+                stop = 1; /* the reference breaks out of its row loop here */
+            } else {
+                alpha /= Di;
+                beta /= Di;
+                for (t = 0; t < rc.n; t++) {
+                    int J = rc.a[t];
+                    double q = rv.a[t];
+                    if (J != i && label[J] == COARSE) {
+                        if (q < 0 && alpha != 0) acc_add(&acc, map[J], alpha * q);
+                        if (q > 0 && beta != 0) acc_add(&acc, map[J], beta * q);
+                    }
+                }
+            }
+        } else {
+            acc_seed(&acc, i);
+            acc_add(&acc, map[i], 1.);
+        }
+        acc_take(&acc, &rc, &rv);
+        row_delete_zeros(i, &rc, &rv);
+        {
+            double mx = 0, sum = 0;
+            int w = 0;
+            for (t = 0; t < rc.n; t++)
+                if (rc.a[t] != i && fabs(rv.a[t]) > mx) mx = fabs(rv.a[t]);
+            for (t = 0; t < rc.n; t++) {
+                if (rc.a[t] != i && fabs(rv.a[t]) < mx * tol) continue;
+                rc.a[w] = rc.a[t];
+                rv.a[w] = rv.a[t];
+                w++;
+            }
+            rc.n = rv.n = w;
+            for (t = 0; t < rc.n; t++) sum += rv.a[t];
+            if (sum != 0)
+                for (t = 0; t < rc.n; t++) rv.a[t] *= 1. / sum;
+        }
+        mat_append_row(P, i, &rc, &rv);
+    }
+    acc_free(&acc);
+    free(D);
+    free(snc.a);
+    free(snv.a);
+    free(rc.a);
+    free(rv.a);
+    return 1;
+}
+
+/* get_AMG_transposed (Multigrid.c:308-321): R = P^T, zeros of P skipped,
+ * each row k seeded with (k,0.0) by sparse_zero */
+static void restriction(const Mat* P, int coarse, Mat* R) {
+    int i, p, k;
+    Acc* rows = NULL;
+    ivec c = {0};
+    dvec v = {0};
+    /* build row by row with one accumulator per coarse row would be
+     * quadratic in memory; instead count, fill and merge the seed */
+    int* cnt = (int*)calloc(coarse + 1, sizeof(int));
+    int* fill;
+    int* ci;
+    double* cv;
+    (void)rows;
+    for (i = 0; i < P->n; i++)
+        for (p = P->rp[i]; p < P->rp[i + 1]; p++)
+            if (P->v.a[p] != 0) cnt[P->ci.a[p] + 1]++;
+    for (k = 0; k < coarse; k++) cnt[k + 1] += cnt[k];
+    ci = (int*)malloc((cnt[coarse] > 0 ? cnt[coarse] : 1) * sizeof(int));
+    cv = (double*)malloc((cnt[coarse] > 0 ? cnt[coarse] : 1) * sizeof(double));
+    fill = (int*)malloc((coarse + 1) * sizeof(int));
+    memcpy(fill, cnt, (coarse + 1) * sizeof(int));
+    for (i = 0; i < P->n; i++)
+        for (p = P->rp[i]; p < P->rp[i + 1]; p++)
+            if (P->v.a[p] != 0) {
+                k = P->ci.a[p];
+                ci[fill[k]] = i;
+                cv[fill[k]] = P->v.a[p];
+                fill[k]++;
+            }
+    mat_init(R, coarse, P->n);
+    for (k = 0; k < coarse; k++) {
+        int seeded = 0;
+        c.n = v.n = 0;
+        for (p = cnt[k]; p < cnt[k + 1]; p++) {
+            if (!seeded && ci[p] >= k) {
+                if (ci[p] > k) {
+                    ipush(&c, k);
+                    dpush(&v, 0.0);
+                }
+                seeded = 1;
+            }
+            ipush(&c, ci[p]);
+            dpush(&v, cv[p]);
+        }
+        if (!seeded) {
+            ipush(&c, k);
+            dpush(&v, 0.0);
+        }
+        mat_append_row(R, k, &c, &v);
+    }
+    free(cnt);
+    free(fill);
+    free(ci);
+    free(cv);
+    free(c.a);
+    free(v.a);
+}
+
+/* ------------------------------------------------------------- hierarchy */
+
+struct orc_amg {
+    int depth, dof, smiter, aggr_smiter, coarsening;
+    double em, ep, tol;
+    Mat* A;      /* Coarse_A[l] */
+    Mat* P;      /* Prolongation[l] */
+    Mat* R;      /* Restriction[l] */
+    int** label; /* Coarsening[l] */
+    int* coarse;
+    double* signs;
+    /* coarsest: sparse_LU_factorization (linear_algebra.c:3040-3062) */
+    int nd;
+    int* didx;
+    int* dmap;
+    double *L, *U; /* dense factors over the non-trivial rows, row-major */
+    /* colourings per level for the multicolour smoothers */
+    int** colour;
+    int* ncolour;
+};
+
+/* LU_factorzation (linear_algebra.c:2909-2942): unit-lower L, U by bordering,
+ * with Lower_triangular_invert (:2865-2878) and
+ * Upper_triangular_transpose_invert (:2894-2907) */
+static void dense_lu(const double* M, int N, double* L, double* U) {
+    int s, i, j;
+    double* u = (double*)malloc((N > 0 ? N : 1) * sizeof(double));
+    double* l = (double*)malloc((N > 0 ? N : 1) * sizeof(double));
+    memset(L, 0, (size_t)N * N * sizeof(double));
+    memset(U, 0, (size_t)N * N * sizeof(double));
+    if (N == 0) {
+        free(u);
+        free(l);
+        return;
+    }
+    L[0] = 1.;
+    U[0] = M[0];
+    for (s = 1; s < N; s++) {
+        for (i = 0; i < s; i++) u[i] = M[(size_t)i * N + s];
+        for (i = 0; i < s; i++) {
+            for (j = 0; j < i; j++) u[i] -= L[(size_t)i * N + j] * u[j];
+            u[i] /= L[(size_t)i * N + i];
+        }
+        for (i = 0; i < s; i++) l[i] = M[(size_t)s * N + i];
+        for (i = 0; i < s; i++) {
+            for (j = 0; j < i; j++) l[i] -= U[(size_t)j * N + i] * l[j];
+            l[i] /= U[(size_t)i * N + i];
+        }
+        for (i = 0; i < s; i++) {
+            U[(size_t)i * N + s] = u[i];
+            L[(size_t)s * N + i] = l[i];
+        }
+        {
+            double dg = M[(size_t)s * N + s];
+            for (i = 0; i < s; i++) dg -= u[i] * l[i];
+            U[(size_t)s * N + s] = dg;
+        }
+        L[(size_t)s * N + s] = 1.;
+    }
+    free(u);
+    free(l);
+}
+
+static void colour_level(orc_amg* h, int l) {
+    const Mat* A = &h->A[l];
+    h->colour[l] = (int*)malloc((A->n > 0 ? A->n : 1) * sizeof(int));
+    h->ncolour[l] = orc_colouring(A->n, A->rp, A->ci.a, A->v.a, h->colour[l]);
+}
+
+static void coarse_factor(orc_amg* h) {
+    const Mat* A0 = &h->A[0];
+    int n = A0->n, i, p, k = 0;
+    double* M;
+    h->dmap = (int*)malloc((n > 0 ? n : 1) * sizeof(int));
+    h->didx = (int*)malloc((n > 0 ? n : 1) * sizeof(int));
+    for (i = 0; i < n; i++) {
+        int nz = 0;
+        for (p = A0->rp[i]; p < A0->rp[i + 1]; p++)
+            if (A0->v.a[p] != 0) nz++;
+        h->dmap[i] = -1;
+        if (nz > 1) {
+            h->dmap[i] = k;
+            h->didx[k++] = i;
+        }
+    }
+    h->nd = k;
+    M = (double*)calloc((size_t)k * k + 1, sizeof(double));
+    for (i = 0; i < k; i++) {
+        int r = h->didx[i];
+        for (p = A0->rp[r]; p < A0->rp[r + 1]; p++) {
+            int J = h->dmap[A0->ci.a[p]];
+            if (J >= 0) M[(size_t)i * k + J] = A0->v.a[p];
+        }
+    }
+    h->L = (double*)malloc(((size_t)k * k + 1) * sizeof(double));
+    h->U = (double*)malloc(((size_t)k * k + 1) * sizeof(double));
+    dense_lu(M, k, h->L, h->U);
+    free(M);
+}
+
+static void coarse_free(orc_amg* h) {
+    free(h->dmap);
+    free(h->didx);
+    free(h->L);
+    free(h->U);
+    h->dmap = h->didx = NULL;
+    h->L = h->U = NULL;
+}
+
+static int build_levels(orc_amg* h, const int* finest_levels) {
+    int depth = h->depth, i, r;
+    int* levels = (int*)malloc((h->A[depth - 1].n > 0 ? h->A[depth - 1].n : 1) * sizeof(int));
+    memcpy(levels, finest_levels, h->A[depth - 1].n * sizeof(int));
+    for (i = depth - 1; i > 0; i--) {
+        Mat* A = &h->A[i];
+        Lists S, T;
+        int coarse, k = 0, *map, *nl, ok;
+        h->label[i] = (int*)calloc(A->n > 0 ? A->n : 1, sizeof(int));
+        if (h->coarsening == AGGRESSIVE && i == depth - 1)
+            coarse = aggressive_coarsening(A, h->label[i], &S, &T, levels, h->dof, h->em, h->ep);
+        else
+            coarse = standard_coarsening(A, h->label[i], &S, &T, levels, h->dof, 0, h->em, h->ep);
+        lists_free(&S);
+        lists_free(&T);
+        h->coarse[i] = coarse;
+        map = (int*)malloc((A->n > 0 ? A->n : 1) * sizeof(int));
+        for (r = 0; r < A->n; r++) map[r] = h->label[i][r] != FINE ? k++ : -1;
+        if (k != coarse) {
+            free(map);
+            free(levels);
+            return 0;
+        }
+        nl = (int*)malloc((coarse > 0 ? coarse : 1) * sizeof(int));
+        for (r = 0; r < A->n; r++)
+            if (map[r] >= 0) nl[map[r]] = levels[r];
+        ok = interpolation(A, h->label[i], levels, map, coarse,
+                           (h->coarsening == AGGRESSIVE && i == depth - 1) ? 2 : 1, h->tol, &h->P[i]);
+        free(map);
+        if (!ok) {
+            free(nl);
+            free(levels);
+            return 0;
+        }
+        restriction(&h->P[i], coarse, &h->R[i]);
+        if (!galerkin(A, &h->R[i], &h->P[i], &h->A[i - 1])) {
+            free(nl);
+            free(levels);
+            return 0;
+        }
+        free(levels);
+        levels = nl;
+    }
+    free(levels);
+    coarse_factor(h);
+    for (i = 1; i < depth; i++) colour_level(h, i);
+    return 1;
+}
+
+orc_amg* orc_amg_setup(int n, const int* rp, const int* ci, const double* v, int dof, int depth, int smiter,
+                       int aggr_smiter, int coarsening, double eps_minus, double eps_plus, double matrix_tol) {
+    orc_amg* h = (orc_amg*)calloc(1, sizeof(orc_amg));
+    int i, p, mesh;
+    int* levels;
+    h->depth = depth;
+    h->dof = dof;
+    h->smiter = smiter;
+    h->aggr_smiter = aggr_smiter;
+    h->coarsening = coarsening;
+    h->em = eps_minus;
+    h->ep = eps_plus;
+    h->tol = matrix_tol;
+    h->A = (Mat*)calloc(depth, sizeof(Mat));
+    h->P = (Mat*)calloc(depth, sizeof(Mat));
+    h->R = (Mat*)calloc(depth, sizeof(Mat));
+    h->label = (int**)calloc(depth, sizeof(int*));
+    h->coarse = (int*)calloc(depth, sizeof(int));
+    h->colour = (int**)calloc(depth, sizeof(int*));
+    h->ncolour = (int*)calloc(depth, sizeof(int));
+    /* clone + make_positive_diag (linear_algebra.c:2612-2620) */
+    mat_from(&h->A[depth - 1], n, rp, ci, v);
+    h->signs = (double*)malloc((n > 0 ? n : 1) * sizeof(double));
+    for (i = 0; i < n; i++) {
+        h->signs[i] = 1.;
+        if (diag_of(&h->A[depth - 1], i) < 0) {
+            for (p = rp[i]; p < rp[i + 1]; p++) h->A[depth - 1].v.a[p] *= -1.;
+            h->signs[i] = -1.;
+        }
+    }
+    /* AMG_init_finest_tupels (Multigrid.c:785-804) */
+    levels = (int*)calloc(n > 0 ? n : 1, sizeof(int));
+    mesh = n / dof;
+    for (i = 0; i < mesh; i++)
+        for (p = 1; p <= dof; p++) levels[(p - 1) * mesh + i] = p - 1;
+    if (!build_levels(h, levels)) {
+        free(levels);
+        orc_amg_free(h);
+        return NULL;
+    }
+    free(levels);
+    return h;
+}
+
+void orc_amg_free(orc_amg* h) {
+    int l;
+    if (!h) return;
+    for (l = 0; l < h->depth; l++) {
+        mat_free(&h->A[l]);
+        mat_free(&h->P[l]);
+        mat_free(&h->R[l]);
+        free(h->label[l]);
+        free(h->colour[l]);
+    }
+    free(h->A);
+    free(h->P);
+    free(h->R);
+    free(h->label);
+    free(h->coarse);
+    free(h->colour);
+    free(h->ncolour);
+    free(h->signs);
+    coarse_free(h);
+    free(h);
+}
+
+int orc_amg_level(const orc_amg* h, int l, int* fine, int* coarse, int* nnz_a, int* nnz_p, int* nnz_r) {
+    if (l < 0 || l >= h->depth) return -1;
+    *fine = h->A[l].n;
+    *coarse = h->coarse[l];
+    *nnz_a = mat_nnz(&h->A[l]);
+    *nnz_p = l > 0 ? mat_nnz(&h->P[l]) : 0;
+    *nnz_r = l > 0 ? mat_nnz(&h->R[l]) : 0;
+    return 0;
+}
+
+int orc_amg_copy(const orc_amg* h, int l, char which, int* rp, int* ci, double* v) {
+    const Mat* M;
+    if (l < 0 || l >= h->depth) return -1;
+    M = which == 'A' ? &h->A[l] : which == 'P' ? &h->P[l] : &h->R[l];
+    if (l == 0 && which != 'A') return -1;
+    memcpy(rp, M->rp, (M->n + 1) * sizeof(int));
+    memcpy(ci, M->ci.a, mat_nnz(M) * sizeof(int));
+    memcpy(v, M->v.a, mat_nnz(M) * sizeof(double));
+    return 0;
+}
+
+int orc_amg_labels(const orc_amg* h, int l, int* labels) {
+    if (l <= 0 || l >= h->depth) return -1;
+    memcpy(labels, h->label[l], h->A[l].n * sizeof(int));
+    return 0;
+}
+
+/* AMG_recompute_coarse_matrices (Multigrid.c:1756-1780): new finest values
+ * (clone(New_matrix), Signs unchanged), Galerkin products with the kept P/R */
+int orc_amg_recompute(orc_amg* h, const double* v) {
+    int l, k, d = h->depth - 1;
+    Mat* A = &h->A[d];
+    for (k = 0; k < mat_nnz(A); k++) A->v.a[k] = v[k];
+    for (l = d; l > 0; l--) {
+        mat_free(&h->A[l - 1]);
+        if (!galerkin(&h->A[l], &h->R[l], &h->P[l], &h->A[l - 1])) return -1;
+    }
+    coarse_free(h);
+    coarse_factor(h);
+    for (l = 1; l < h->depth; l++) {
+        free(h->colour[l]);
+        colour_level(h, l);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------- kernels */
+
+void orc_spmv(int n, const int* rp, const int* ci, const double* v, const double* x, double* y) {
+    int i, p;
+    for (i = 0; i < n; i++) {
+        double s = 0;
+        for (p = rp[i]; p < rp[i + 1]; p++) s += v[p] * x[ci[p]];
+        y[i] = s;
+    }
+}
+
+/* linear_map (linear_algebra.c:149-172): x += fac * A y */
+static void linear_map(double* x, double fac, const Mat* A, const double* y) {
+    int i, p;
+    for (i = 0; i < A->n; i++) {
+        double s = 0;
+        for (p = A->rp[i]; p < A->rp[i + 1]; p++) s += A->v.a[p] * y[A->ci.a[p]];
+        s *= fac;
+        x[i] += s;
+    }
+}
+
+static double dot(const double* a, const double* b, int n) {
+    int i;
+    double s = 0;
+    for (i = 0; i < n; i++) s = s + a[i] * b[i];
+    return s;
+}
+
+/* matrix_residuum (linear_algebra.c:2431-2438) */
+static double residual_norm(const Mat* A, const double* X, const double* F) {
+    int n = A->n;
+    double* R = (double*)malloc((n > 0 ? n : 1) * sizeof(double));
+    double r;
+    memcpy(R, F, n * sizeof(double));
+    linear_map(R, -1., A, X);
+    r = sqrt(dot(R, R, n));
+    free(R);
+    return r;
+}
+
+int orc_colouring(int n, const int* rp, const int* ci, const double* v, int* colour) {
+    int i, p, ncol = 0, cap = 64;
+    int* tp = (int*)calloc(n + 1, sizeof(int));
+    int *ta, *fill, *mark;
+    for (i = 0; i < n; i++)
+        for (p = rp[i]; p < rp[i + 1]; p++)
+            if (ci[p] != i && v[p] != 0) tp[ci[p] + 1]++;
+    for (i = 0; i < n; i++) tp[i + 1] += tp[i];
+    ta = (int*)malloc((tp[n] > 0 ? tp[n] : 1) * sizeof(int));
+    fill = (int*)malloc((n + 1) * sizeof(int));
+    memcpy(fill, tp, (n + 1) * sizeof(int));
+    for (i = 0; i < n; i++)
+        for (p = rp[i]; p < rp[i + 1]; p++)
+            if (ci[p] != i && v[p] != 0) ta[fill[ci[p]]++] = i;
+    mark = (int*)malloc(cap * sizeof(int));
+    for (p = 0; p < cap; p++) mark[p] = -1;
+    for (i = 0; i < n; i++) {
+        int c = 0;
+        for (p = rp[i]; p < rp[i + 1]; p++)
+            if (ci[p] < i && v[p] != 0) mark[colour[ci[p]]] = i;
+        for (p = tp[i]; p < tp[i + 1]; p++)
+            if (ta[p] < i) mark[colour[ta[p]]] = i;
+        while (c < cap && mark[c] == i) c++;
+        if (c == cap) {
+            int q;
+            mark = (int*)realloc(mark, 2 * cap * sizeof(int));
+            for (q = cap; q < 2 * cap; q++) mark[q] = -1;
+            cap *= 2;
+        }
+        colour[i] = c;
+        if (c + 1 > ncol) ncol = c + 1;
+    }
+    free(tp);
+    free(ta);
+    free(fill);
+    free(mark);
+    return ncol;
+}
+
+/* one SOR row update (AMG_SOR_smoother, Multigrid.c:1345-1365) */
+static void sor_row(const Mat* A, const double* F, double* X, int i, double w) {
+    int p, d = -1;
+    double sum = F[i];
+    for (p = A->rp[i]; p < A->rp[i + 1]; p++) {
+        int j = A->ci.a[p];
+        if (j != i) sum -= A->v.a[p] * X[j];
+        else d = p;
+    }
+    X[i] = (1. - w) * X[i] + w * sum / A->v.a[d];
+}
+
+/* smoother ids: 0 SOR, 1 parallel Jacobi, 2 MC-SOR, 3 l1-Jacobi, 4 MC-SSOR, 5 SSOR */
+static void smooth(orc_amg* h, int l, const double* F, double* X, int sweeps, double w, int sm, int post) {
+    const Mat* A = &h->A[l];
+    int n = A->n, k, i, c, p;
+    for (k = 0; k < sweeps; k++) {
+        if (sm == 0 || sm == 5) {
+            int back = sm == 5 && post;
+            for (i = 0; i < n; i++) sor_row(A, F, X, back ? n - 1 - i : i, w);
+        } else if (sm == 2 || sm == 4) {
+            int back = sm == 4 && post, nc = h->ncolour[l];
+            for (c = 0; c < nc; c++) {
+                int cc = back ? nc - 1 - c : c;
+                for (i = 0; i < n; i++)
+                    if (h->colour[l][i] == cc) sor_row(A, F, X, i, w);
+            }
+        } else {
+            /* AMG_parallel_Jacobi_smoother (Multigrid.c:1367-1400) or l1-Jacobi */
+            double* O = (double*)malloc((n > 0 ? n : 1) * sizeof(double));
+            int lim = sm == 1 ? (n / h->dof) * h->dof : n;
+            memcpy(O, X, n * sizeof(double));
+            for (i = 0; i < lim; i++) {
+                double sum = F[i], d = 0;
+                for (p = A->rp[i]; p < A->rp[i + 1]; p++) {
+                    int j = A->ci.a[p];
+                    if (sm == 1) {
+                        if (j != i) sum -= A->v.a[p] * O[j];
+                        else d = A->v.a[p];
+                    } else {
+                        sum -= A->v.a[p] * O[j];
+                        d += fabs(A->v.a[p]);
+                    }
+                }
+                X[i] = sm == 1 ? sum / d : O[i] + sum / d;
+            }
+            free(O);
+        }
+    }
+}
+
+/* AMG_Direct_solver (Multigrid.c:1450-1455): Sol = F, L_triang_invert,
+ * U_triang_invert (linear_algebra.c:2008-2043) with the factors of
+ * sparse_LU_factorization (trivial rows: L=1, U=diag) */
+static void coarse_solve(orc_amg* h, const double* F, double* X) {
+    const Mat* A0 = &h->A[0];
+    int n = A0->n, N = h->nd, i, j;
+    double* y = (double*)malloc((n > 0 ? n : 1) * sizeof(double));
+    memcpy(X, F, n * sizeof(double));
+    /* forward: rows in index order; dense rows use L, trivial rows L_ii = 1 */
+    for (i = 0; i < n; i++) {
+        int I = h->dmap[i];
+        if (I < 0) continue;
+        {
+            double s = X[i];
+            for (j = 0; j < I; j++) {
+                double l = h->L[(size_t)I * N + j];
+                if (l != 0) s -= l * X[h->didx[j]];
+            }
+            X[i] = s;
+        }
+    }
+    memcpy(y, X, n * sizeof(double));
+    for (i = n - 1; i >= 0; i--) {
+        int I = h->dmap[i];
+        if (I < 0) {
+            X[i] = y[i] / diag_of(A0, i);
+            continue;
+        }
+        {
+            double s = y[i];
+            for (j = I + 1; j < N; j++) {
+                double u = h->U[(size_t)I * N + j];
+                if (u != 0) s -= u * X[h->didx[j]];
+            }
+            X[i] = s / h->U[(size_t)I * N + I];
+        }
+    }
+    free(y);
+}
+
+/* AMG_V_cycle (Multigrid.c:1536-1580), Iter[level] = 1 */
+static void vcycle(orc_amg* h, int l, const double* F, double* X, int sm, double w) {
+    const Mat* A = &h->A[l];
+    int n = A->n, nc, sw;
+    double *R, *Fc, *Xc;
+    if (l == 0) {
+        coarse_solve(h, F, X);
+        return;
+    }
+    nc = h->A[l - 1].n;
+    sw = (h->coarsening == AGGRESSIVE && l == h->depth - 1) ? h->aggr_smiter : h->smiter;
+    smooth(h, l, F, X, sw, w, sm, 0);
+    R = (double*)malloc((n > 0 ? n : 1) * sizeof(double));
+    memcpy(R, F, n * sizeof(double));
+    linear_map(R, -1., A, X);
+    Fc = (double*)malloc((nc > 0 ? nc : 1) * sizeof(double));
+    Xc = (double*)calloc(nc > 0 ? nc : 1, sizeof(double));
+    {
+        /* sparse_mult (linear_algebra.c:134-147) of the restriction */
+        const Mat* Rm = &h->R[l];
+        int i, p;
+        for (i = 0; i < Rm->n; i++) {
+            double s = 0;
+            for (p = Rm->rp[i]; p < Rm->rp[i + 1]; p++) s += Rm->v.a[p] * R[Rm->ci.a[p]];
+            Fc[i] = s;
+        }
+    }
+    vcycle(h, l - 1, Fc, Xc, sm, w);
+    {
+        /* linear_map(X, 1, P, Xc); P's seed entries outside the coarse range are zero */
+        const Mat* P = &h->P[l];
+        int i, p;
+        for (i = 0; i < P->n; i++) {
+            double s = 0;
+            for (p = P->rp[i]; p < P->rp[i + 1]; p++)
+                if (P->ci.a[p] < nc) s += P->v.a[p] * Xc[P->ci.a[p]];
+            s *= 1.;
+            X[i] += s;
+        }
+    }
+    smooth(h, l, F, X, sw, w, sm, 1);
+    free(R);
+    free(Fc);
+    free(Xc);
+}
+
+/* AMG_solve (Multigrid.c:1863-1931), STAND_ALONE; the diagnostic ILU
+ * residuals decide only `res0 != 0`, i.e. r0 != 0 */
+int orc_amg_solve(orc_amg* h, const double* Fin, double* X, int sm, double w, double eps, int max_iter) {
+    int d = h->depth - 1, n = h->A[d].n, i, p, it = 0;
+    const Mat* A = &h->A[d];
+    double *F = (double*)malloc((n > 0 ? n : 1) * sizeof(double)), r0, r;
+    for (i = 0; i < n; i++) F[i] = 0 + h->signs[i] * Fin[i];
+    /* AMG_fix_isolated (Multigrid.c:1301-1319) */
+    for (i = 0; i < n; i++)
+        if (h->label[d][i] == ISOLATED)
+            for (p = A->rp[i]; p < A->rp[i + 1]; p++)
+                if (A->v.a[p] != 0) {
+                    X[i] = F[i] / A->v.a[p];
+                    break;
+                }
+    r0 = residual_norm(A, X, F);
+    r = r0;
+    if (r0 != 0)
+        while (fabs(r / r0) > eps && it < max_iter) {
+            vcycle(h, d, F, X, sm, w);
+            r = residual_norm(A, X, F);
+            it++;
+        }
+    free(F);
+    return it;
+}
+
+/* pcg (PCGS.c:163-233) with an AMG_solve preconditioner (PCG_AMG_precon) */
+int orc_pcg_amg(orc_amg* h, double* X, double* B, double tol, int max_iter, int sm, double w, double amg_eps,
+                int amg_max_iter, int zero_init, int stop, double* res, int* vcycles) {
+    int n = h->A[h->depth - 1].n, i, it = 0;
+    double hn, s = 0, s_old, a, b, r, r0;
+    double *R, *P, *W, *Q, *Z, *Y;
+    int vc = 0;
+#define PRECON(RR, ZZ)                                                       \
+    do {                                                                     \
+        if (zero_init) memset((ZZ), 0, n * sizeof(double));                  \
+        vc += orc_amg_solve(h, (RR), (ZZ), sm, w, amg_eps, amg_max_iter);    \
+    } while (0)
+    hn = sqrt(dot(B, B, n));
+    if (hn == 0) hn = 1;
+    for (i = 0; i < n; i++) B[i] *= (double)1 / hn;
+    for (i = 0; i < n; i++) X[i] *= (double)1 / hn;
+    W = (double*)calloc(n, sizeof(double));
+    Q = (double*)calloc(n, sizeof(double));
+    Z = (double*)calloc(n, sizeof(double));
+    P = (double*)calloc(n, sizeof(double));
+    R = (double*)malloc(n * sizeof(double));
+    Y = (double*)malloc(n * sizeof(double));
+    memcpy(R, B, n * sizeof(double));
+    orc_spmv(n, h->A[h->depth - 1].rp, h->A[h->depth - 1].ci.a, h->A[h->depth - 1].v.a, X, Q);
+    /* the finest Coarse_A has positive diagonal; PCG_mult uses the caller's
+     * matrix, equal to it up to the row signs */
+    for (i = 0; i < n; i++) Q[i] = h->signs[i] * Q[i];
+    for (i = 0; i < n; i++) R[i] += -1. * Q[i];
+    PRECON(R, W);
+    r0 = sqrt(dot(W, W, n));
+    if (r0 == 0) {
+        *res = 0;
+        *vcycles = vc;
+        free(W); free(Q); free(Z); free(P); free(R); free(Y);
+        return 0;
+    }
+    r = r0;
+    do {
+        PRECON(R, Z);
+        s_old = s;
+        s = dot(R, Z, n);
+        if (it > 0) {
+            b = s / s_old;
+            for (i = 0; i < n; i++) P[i] *= b;
+            for (i = 0; i < n; i++) P[i] += 1 * Z[i];
+        } else
+            memcpy(P, Z, n * sizeof(double));
+        orc_spmv(n, h->A[h->depth - 1].rp, h->A[h->depth - 1].ci.a, h->A[h->depth - 1].v.a, P, Q);
+        for (i = 0; i < n; i++) Q[i] = h->signs[i] * Q[i];
+        a = s / dot(P, Q, n);
+        for (i = 0; i < n; i++) X[i] += a * P[i];
+        for (i = 0; i < n; i++) R[i] += -a * Q[i];
+        r = sqrt(dot(R, R, n));
+        it++;
+    } while ((stop ? r : r / r0) > tol && it < max_iter);
+    PRECON(R, W);
+    r = sqrt(dot(W, W, n));
+    for (i = 0; i < n; i++) B[i] *= hn;
+    for (i = 0; i < n; i++) X[i] *= hn;
+    *res = r / r0;
+    *vcycles = vc;
+    free(W); free(Q); free(Z); free(P); free(R); free(Y);
+#undef PRECON
+    return it;
+}

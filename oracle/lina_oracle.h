@@ -1,0 +1,66 @@
+/*
+ * lina_oracle.h -- CPU restatement of the LinA solver path (MRadszuweit/FEM,
+ * LinA/) used as the parity oracle for liblina_hip.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Loaded by tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg through oracle/bind.py; the product library
+ * never links or calls it.
+ *
+ * Matrices are CSR (int32, sorted rows).  Every function restates one
+ * reference routine, cited at its definition in lina_oracle.c.
+ */
+#ifndef LINA_ORACLE_H
+#define LINA_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_amg orc_amg;
+
+/* AMG_setup (Multigrid.c:1613-1754) */
+orc_amg* orc_amg_setup(int n, const int* rp, const int* ci, const double* v, int dof, int depth, int smiter,
+                       int aggr_smiter, int coarsening, double eps_minus, double eps_plus, double matrix_tol);
+void orc_amg_free(orc_amg* h);
+int orc_amg_level(const orc_amg* h, int l, int* fine, int* coarse, int* nnz_a, int* nnz_p, int* nnz_r);
+int orc_amg_copy(const orc_amg* h, int l, char which, int* rp, int* ci, double* v);
+int orc_amg_labels(const orc_amg* h, int l, int* labels);
+/* AMG_recompute_coarse_matrices (Multigrid.c:1756-1780) with new finest values */
+int orc_amg_recompute(orc_amg* h, const double* v);
+
+/* AMG_solve (Multigrid.c:1863-1931); smoother ids as lina_hip.h */
+int orc_amg_solve(orc_amg* h, const double* F, double* X, int smoother, double omega, double eps, int max_iter);
+
+/* pcg (PCGS.c:163-233) preconditioned by AMG_solve; stop 0: reference test
+ * r/r0 <= tol, 1: |R| <= tol.  Returns iterations; *res = pcg residuum,
+ * *vcycles = V-cycles summed over all preconditioner calls. */
+int orc_pcg_amg(orc_amg* h, double* X, double* B, double tol, int max_iter, int smoother, double omega,
+                double amg_eps, int amg_max_iter, int zero_init, int stop, double* res, int* vcycles);
+
+/* greedy colouring of the symmetrised nonzero pattern (natural order) */
+int orc_colouring(int n, const int* rp, const int* ci, const double* v, int* colour);
+
+/* y = A x with sequential row sums (PCG_mult, PCGS.c:36-59) */
+void orc_spmv(int n, const int* rp, const int* ci, const double* v, const double* x, double* y);
+
+/* ILU(0) (linear_algebra.c:1952-1982): L lower with diagonal, U unit upper;
+ * output CSR arrays sized like A (pattern of A plus diagonals).  Returns the
+ * reference's flag (0 on a zero pivot). */
+int orc_ilu0(int n, const int* rp, const int* ci, const double* v, int* lrp, int* lci, double* lv, int* urp,
+             int* uci, double* uv);
+void orc_ilu_apply(int n, const int* lrp, const int* lci, const double* lv, const int* urp, const int* uci,
+                   const double* uv, const double* z, double* w);
+
+/* BiCGStab_solve (BiCGStab.c:115-198), right-preconditioned by AMG_solve */
+int orc_bicgstab_amg(orc_amg* h, double* X, const double* b, double eps, int max_iter, int smoother, double omega,
+                     double amg_eps, int amg_max_iter, double* res);
+
+/* restarted GMRES(m) with right ILU(0) preconditioning (Saad, Alg. 9.5):
+ * returns iterations, x holds the solution of A x = b */
+int orc_gmres_ilu(int n, const int* rp, const int* ci, const double* v, double* x, const double* b, int m,
+                  double tol, int max_iter, double* res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

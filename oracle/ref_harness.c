@@ -26,9 +26,13 @@ void refh_sym_reset(void);
 
 /* ---------------------------------------------------------------- memory */
 
-void refh_init(int zero_fill) {
-    if (zero_fill) mallopt(M_PERTURB, 0xFF);   /* malloc'd bytes = ~0xFF = 0 */
+/* 1: zero-fill every malloc (parity runs); 0: glibc defaults; 2: no
+ * perturbation but a fixed mmap threshold, so blocks >= 128 KiB are always
+ * fresh zero pages (timing runs in a fresh process) */
+void refh_init(int mode) {
+    if (mode == 1) mallopt(M_PERTURB, 0xFF);   /* malloc'd bytes = ~0xFF = 0 */
     else mallopt(M_PERTURB, 0);
+    if (mode == 2) mallopt(M_MMAP_THRESHOLD, 128 * 1024);
 }
 
 /* build a row-list matrix from CSR (rows must be sorted, >= 1 entry each) */
