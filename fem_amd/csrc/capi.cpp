@@ -70,6 +70,8 @@ void check_params(const lina_amg_params* p) {
     if (!p) throw std::invalid_argument("null params");
     if (p->dof < 1 || p->depth < 2) throw std::invalid_argument("dof >= 1 and depth >= 2 required");
     if (p->smoother < 0 || p->smoother > 5) throw std::invalid_argument("unknown smoother");
+    if (p->coarse_smoother < -1 || p->coarse_smoother > 5) throw std::invalid_argument("unknown coarse smoother");
+    if (p->coarse_smoother >= 0 && p->coarse_sweeps < 0) throw std::invalid_argument("negative sweeps");
     if (p->precision != LINA_PREC_FP64 && p->precision != LINA_PREC_MIXED)
         throw std::invalid_argument("unknown precision");
 }
@@ -102,6 +104,8 @@ void lina_params_reference(lina_amg_params* p, int dof, int depth) {
     p->amg_max_iter = 100;
     p->zero_init = 0;
     p->precision = LINA_PREC_FP64;
+    p->coarse_smoother = -1;
+    p->coarse_sweeps = 1;
 }
 
 void lina_params_vcycle(lina_amg_params* p, int dof, int depth) {
@@ -112,6 +116,8 @@ void lina_params_vcycle(lina_amg_params* p, int dof, int depth) {
     p->omega = 1.0;
     p->amg_max_iter = 1;
     p->zero_init = 1;
+    p->coarse_smoother = LINA_SMOOTHER_L1_JACOBI;
+    p->coarse_sweeps = 2;
 }
 
 lina_solver* lina_amg_create(int n, const int* rowptr, const int* col, const double* val, const lina_amg_params* p,

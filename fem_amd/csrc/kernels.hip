@@ -109,6 +109,105 @@ __global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int* __restrict__ 
     }
 }
 
+// ------------------------------------------------ vector (long-row) path
+//
+// Matrices whose rows average many entries (the Galerkin coarse levels: 50
+// to 200 per row) would leave a single lane with a long dependent chain.
+// There LPR lanes share a row: each lane walks the row with stride LPR, the
+// partial sums are combined by a fixed xor-shuffle tree (deterministic, but a
+// different association than the sequential sum).  Block b handles rows
+// [b*RPB, (b+1)*RPB) of the range, RPB = 2*(TPB/LPR).
+
+template <int LPR, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+    for (int o = LPR / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, LPR);
+    return v;
+}
+
+template <typename VT, typename XT, int MODE, int LPR>
+__global__ __launch_bounds__(TPB) void k_spmv_vec(int n, int nblk, const int* __restrict__ rp,
+                                                  const int* __restrict__ ci, const VT* __restrict__ v,
+                                                  const XT* __restrict__ x, const XT* __restrict__ b,
+                                                  XT* __restrict__ y, double* __restrict__ part,
+                                                  const XT* __restrict__ w) {
+    constexpr int G = TPB / LPR, RPB = 2 * G;
+    const int bb = xcd_block(blockIdx.x, nblk);
+    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+    double mine = 0.0;
+    for (int r = bb * RPB + g; r < min(n, (bb + 1) * RPB); r += G) {
+        XT s = 0;
+        for (int k = rp[r] + lane; k < rp[r + 1]; k += LPR) s += (XT)v[k] * x[ci[k]];
+        s = group_sum<LPR>(s);
+        if (lane == 0) {
+            XT out;
+            if (MODE == MODE_SET) out = s;
+            else if (MODE == MODE_RES) out = b[r] + (-s);
+            else out = y[r] + s;
+            y[r] = out;
+            if (part) mine += (double)out * (double)(w ? w[r] : out);
+        }
+    }
+    if (part) {
+        double t = block_sum(mine);
+        if (threadIdx.x == 0) part[bb] = t;
+    }
+}
+
+// multicolour SOR over rows [r0, r1) of the colour-permuted copy
+template <typename VT, typename XT, int LPR>
+__global__ __launch_bounds__(TPB) void k_mcsor_vec(int r0, int r1, const int* __restrict__ rp,
+                                                   const int* __restrict__ ci, const VT* __restrict__ v,
+                                                   const int* __restrict__ perm, const XT* __restrict__ f,
+                                                   XT* __restrict__ x, XT omega) {
+    constexpr int G = TPB / LPR, RPB = 2 * G;
+    const int nb = (r1 - r0 + RPB - 1) / RPB;
+    const int bb = xcd_block(blockIdx.x, nb);
+    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+    for (int r = r0 + bb * RPB + g; r < min(r1, r0 + (bb + 1) * RPB); r += G) {
+        const int i = perm[r];
+        XT s = 0, d = 0;
+        for (int k = rp[r] + lane; k < rp[r + 1]; k += LPR) {
+            int j = ci[k];
+            if (j != i) s += (XT)v[k] * x[j];
+            else d += (XT)v[k];
+        }
+        s = group_sum<LPR>(s);
+        d = group_sum<LPR>(d);
+        if (lane == 0) x[i] = ((XT)1 - omega) * x[i] + omega * (f[i] - s) / d;
+    }
+}
+
+// Jacobi sweeps x = J(xo) on the vector path (see k_jacobi)
+template <typename VT, typename XT, int KIND, int LPR>
+__global__ __launch_bounds__(TPB) void k_jacobi_vec(int n, int nblk, const int* __restrict__ rp,
+                                                    const int* __restrict__ ci, const VT* __restrict__ v,
+                                                    const XT* __restrict__ f, const XT* __restrict__ xo,
+                                                    XT* __restrict__ x, int nlim) {
+    constexpr int G = TPB / LPR, RPB = 2 * G;
+    const int bb = xcd_block(blockIdx.x, nblk);
+    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+    for (int r = bb * RPB + g; r < min(n, (bb + 1) * RPB); r += G) {
+        XT s = 0, d = 0;
+        for (int k = rp[r] + lane; k < rp[r + 1]; k += LPR) {
+            int j = ci[k];
+            XT a = (XT)v[k];
+            if (KIND == JAC_REF) {
+                if (j != r) s += a * xo[j];
+                else d += a;
+            } else {
+                s += a * xo[j];
+                d += fabs(a);
+            }
+        }
+        s = group_sum<LPR>(s);
+        d = group_sum<LPR>(d);
+        if (lane == 0) {
+            if (KIND == JAC_REF) x[r] = r < nlim ? (f[r] - s) / d : xo[r];
+            else x[r] = xo[r] + (f[r] - s) / d;
+        }
+    }
+}
+
 // ------------------------------------------------------------ reductions
 
 template <typename XT>
@@ -315,8 +414,11 @@ __global__ __launch_bounds__(TPB) void k_jacobi(int nblk, const int* __restrict_
         __syncthreads();
     }
     for (int r = r0 + threadIdx.x; r < r1; r += TPB) {
-        if (KIND == JAC_REF && r >= nlim) continue;
         if (!lds && threadIdx.x != 0) continue;
+        if (KIND == JAC_REF && r >= nlim) {
+            x[r] = xo[r];  // rows past (n/dof)*dof are not swept (Multigrid.c:1380-1383)
+            continue;
+        }
         XT sum = f[r];
         XT d = 0;
         for (int k = rp[r]; k < rp[r + 1]; k++) {
@@ -407,10 +509,33 @@ static inline int grid_for(int n, int cap = 4096) {
     return g > cap ? cap : g;
 }
 
+#define LINA_LPR_DISPATCH(LPR_VAL, ...)          \
+    switch (LPR_VAL) {                            \
+        case 8: { constexpr int L = 8; __VA_ARGS__; break; }   \
+        case 16: { constexpr int L = 16; __VA_ARGS__; break; } \
+        case 32: { constexpr int L = 32; __VA_ARGS__; break; } \
+        default: { constexpr int L = 64; __VA_ARGS__; break; } \
+    }
+
 template <typename VT, typename XT>
-void spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w) {
-    if (A.nblk == 0) return;
-    dim3 g(A.nblk), t(TPB);
+int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w) {
+    if (A.n == 0) return 0;
+    dim3 t(TPB);
+    if (A.lpr > 1) {
+        dim3 g(A.nvblk);
+        if (mode == MODE_SET) {
+            LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv_vec<VT, XT, MODE_SET, L>), g, t, 0, st, A.n, A.nvblk,
+                                                        A.rp, A.ci, A.v, x, b, y, part, w))
+        } else if (mode == MODE_RES) {
+            LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv_vec<VT, XT, MODE_RES, L>), g, t, 0, st, A.n, A.nvblk,
+                                                        A.rp, A.ci, A.v, x, b, y, part, w))
+        } else {
+            LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv_vec<VT, XT, MODE_ADD, L>), g, t, 0, st, A.n, A.nvblk,
+                                                        A.rp, A.ci, A.v, x, b, y, part, w))
+        }
+        return A.nvblk;
+    }
+    dim3 g(A.nblk);
     switch (mode) {
         case MODE_SET:
             hipLaunchKernelGGL((k_spmv<VT, XT, MODE_SET>), g, t, 0, st, A.nblk, A.blk, A.rp, A.ci, A.v, x, b, y, part, w);
@@ -421,11 +546,12 @@ void spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, in
         default:
             hipLaunchKernelGGL((k_spmv<VT, XT, MODE_ADD>), g, t, 0, st, A.nblk, A.blk, A.rp, A.ci, A.v, x, b, y, part, w);
     }
+    return A.nblk;
 }
-template void spmv<double, double>(hipStream_t, const DMat<double>&, const double*, const double*, double*, int,
-                                   double*, const double*);
-template void spmv<float, float>(hipStream_t, const DMat<float>&, const float*, const float*, float*, int, double*,
-                                 const float*);
+template int spmv<double, double>(hipStream_t, const DMat<double>&, const double*, const double*, double*, int,
+                                  double*, const double*);
+template int spmv<float, float>(hipStream_t, const DMat<float>&, const float*, const float*, float*, int, double*,
+                                const float*);
 
 template <typename XT>
 int dot_part(hipStream_t st, int n, const XT* a, const XT* b, double* part) {
@@ -501,16 +627,24 @@ void axpby(hipStream_t st, int n, double a, double* x, double b, const double* y
 }
 
 template <typename VT, typename XT>
-void mcsor_colour(hipStream_t st, const DMat<VT>& Ac, int blk0, int nblk, const int* perm, const XT* f, XT* x,
-                  XT omega) {
+void mcsor_colour(hipStream_t st, const DMat<VT>& Ac, int blk0, int nblk, int r0, int r1, const int* perm,
+                  const XT* f, XT* x, XT omega) {
+    if (r1 <= r0) return;
+    if (Ac.lpr > 1) {
+        const int rpb = 2 * (TPB / Ac.lpr);
+        dim3 g((r1 - r0 + rpb - 1) / rpb), t(TPB);
+        LINA_LPR_DISPATCH(Ac.lpr, hipLaunchKernelGGL((k_mcsor_vec<VT, XT, L>), g, t, 0, st, r0, r1, Ac.rp, Ac.ci,
+                                                     Ac.v, perm, f, x, omega))
+        return;
+    }
     if (nblk <= 0) return;
     hipLaunchKernelGGL((k_mcsor<VT, XT>), dim3(nblk), dim3(TPB), 0, st, nblk, blk0, Ac.blk, Ac.rp, Ac.ci, Ac.v, perm,
                        f, x, omega);
 }
-template void mcsor_colour<double, double>(hipStream_t, const DMat<double>&, int, int, const int*, const double*,
-                                           double*, double);
-template void mcsor_colour<float, float>(hipStream_t, const DMat<float>&, int, int, const int*, const float*, float*,
-                                         float);
+template void mcsor_colour<double, double>(hipStream_t, const DMat<double>&, int, int, int, int, const int*,
+                                           const double*, double*, double);
+template void mcsor_colour<float, float>(hipStream_t, const DMat<float>&, int, int, int, int, const int*,
+                                         const float*, float*, float);
 
 template <typename VT, typename XT>
 void sor_lex(hipStream_t st, const DMat<VT>& A, int nlv, const int* lv_off, const int* lv_rows, const XT* f, XT* x,
@@ -526,13 +660,25 @@ template void sor_lex<float, float>(hipStream_t, const DMat<float>&, int, const 
 
 template <typename VT, typename XT>
 void jacobi(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, const XT* xo, XT* x, int nlim) {
-    if (A.nblk == 0) return;
+    if (A.n == 0) return;
+    dim3 t(TPB);
+    if (A.lpr > 1) {
+        dim3 g(A.nvblk);
+        if (kind == JAC_REF) {
+            LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi_vec<VT, XT, JAC_REF, L>), g, t, 0, st, A.n,
+                                                        A.nvblk, A.rp, A.ci, A.v, f, xo, x, nlim))
+        } else {
+            LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi_vec<VT, XT, JAC_L1, L>), g, t, 0, st, A.n,
+                                                        A.nvblk, A.rp, A.ci, A.v, f, xo, x, nlim))
+        }
+        return;
+    }
     if (kind == JAC_REF)
-        hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_REF>), dim3(A.nblk), dim3(TPB), 0, st, A.nblk, A.blk, A.rp, A.ci,
-                           A.v, f, xo, x, nlim);
+        hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_REF>), dim3(A.nblk), t, 0, st, A.nblk, A.blk, A.rp, A.ci, A.v, f, xo,
+                           x, nlim);
     else
-        hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_L1>), dim3(A.nblk), dim3(TPB), 0, st, A.nblk, A.blk, A.rp, A.ci,
-                           A.v, f, xo, x, nlim);
+        hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_L1>), dim3(A.nblk), t, 0, st, A.nblk, A.blk, A.rp, A.ci, A.v, f, xo,
+                           x, nlim);
 }
 template void jacobi<double, double>(hipStream_t, const DMat<double>&, int, const double*, const double*, double*,
                                      int);

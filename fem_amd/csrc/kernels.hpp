@@ -40,10 +40,23 @@ struct DMat {
     VT* v = nullptr;
     int nblk = 0;
     int* blk = nullptr;  // nblk+1 row starts
+    int lpr = 1;         // lanes per row: 1 = LDS-staged sequential rows, else 8..64
+    int nvblk = 0;       // blocks of the multi-lane path: ceil(n / (2*TPB/lpr))
 };
 
+constexpr int TPB_ROWS = 256;
+// lanes per row for a matrix with the given mean row length
+inline int lanes_per_row(double mean_len) {
+    if (mean_len <= 24) return 1;
+    if (mean_len <= 48) return 8;
+    if (mean_len <= 96) return 16;
+    if (mean_len <= 192) return 32;
+    return 64;
+}
+
+// returns the number of block partials written to part (if part != nullptr)
 template <typename VT, typename XT>
-void spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w);
+int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w);
 template <typename XT>
 int dot_part(hipStream_t st, int n, const XT* a, const XT* b, double* part);
 void finalize(hipStream_t st, int np, const double* part, double* scal, int slot, int op);
@@ -64,8 +77,8 @@ void axpy_s(hipStream_t st, int n, double* x, const double* y, const double* sca
             double sign);
 void axpby(hipStream_t st, int n, double a, double* x, double b, const double* y);
 template <typename VT, typename XT>
-void mcsor_colour(hipStream_t st, const DMat<VT>& Ac, int blk0, int nblk, const int* perm, const XT* f, XT* x,
-                  XT omega);
+void mcsor_colour(hipStream_t st, const DMat<VT>& Ac, int blk0, int nblk, int r0, int r1, const int* perm,
+                  const XT* f, XT* x, XT omega);
 template <typename VT, typename XT>
 void sor_lex(hipStream_t st, const DMat<VT>& A, int nlv, const int* lv_off, const int* lv_rows, const XT* f, XT* x,
              XT* xo, XT omega, int sweeps, int backward);

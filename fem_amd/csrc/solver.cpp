@@ -144,6 +144,8 @@ static void upload_mat(const Csr& A, const std::vector<int>& starts_with_end, de
     D.nblk = (int)starts_with_end.size() - 1;
     if (D.nblk < 0) D.nblk = 0;
     D.blk = shared_blk ? (int*)shared_blk : dupload(starts_with_end, B, st);
+    D.lpr = dev::lanes_per_row(A.n > 0 ? (double)A.nnz() / A.n : 0.0);
+    D.nvblk = D.lpr > 1 ? (A.n + 2 * (dev::TPB_ROWS / D.lpr) - 1) / (2 * (dev::TPB_ROWS / D.lpr)) : 0;
 }
 
 // upload A as FP64 + FP32 copies sharing the index arrays; returns #blocks
@@ -155,7 +157,7 @@ static int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F,
     if (A.n == 0) starts.assign(1, 0);
     upload_mat(A, starts, D, B, st);
     upload_mat(A, starts, F, B, st, D.rp, D.ci, D.blk);
-    return D.nblk;
+    return std::max(D.nblk, D.nvblk);
 }
 
 // level schedule of the lexicographic sweep: forward, level(i) = 1 + max over
@@ -398,10 +400,15 @@ void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post) {
     DevLevel& d = L_[l];
     hipStream_t st = stream_;
     const T omega = (T)prm_.omega;
-    switch (prm_.smoother) {
+    int smoother = prm_.smoother;
+    if (l < (int)L_.size() - 1 && prm_.coarse_smoother >= 0) {
+        smoother = prm_.coarse_smoother;
+        sweeps = prm_.coarse_sweeps;
+    }
+    switch (smoother) {
         case LINA_SMOOTHER_SOR:
         case LINA_SMOOTHER_SSOR: {
-            bool back = (prm_.smoother == LINA_SMOOTHER_SSOR) && post;
+            bool back = (smoother == LINA_SMOOTHER_SSOR) && post;
             if (back)
                 dev::sor_lex(st, pickA<T>(d), d.nlv_b, d.lvb_off, d.lvb_rows, f, x, LevelVecs<T>::xo(d), omega,
                              sweeps, 1);
@@ -412,22 +419,28 @@ void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post) {
         }
         case LINA_SMOOTHER_MC_SOR:
         case LINA_SMOOTHER_MC_SSOR: {
-            bool back = (prm_.smoother == LINA_SMOOTHER_MC_SSOR) && post;
+            bool back = (smoother == LINA_SMOOTHER_MC_SSOR) && post;
             for (int s = 0; s < sweeps; s++)
                 for (int cc = 0; cc < d.ncolours; cc++) {
                     int c = back ? d.ncolours - 1 - cc : cc;
                     int b0 = d.colour_blk[c], b1 = d.colour_blk[c + 1];
-                    dev::mcsor_colour(st, pickAc<T>(d), b0, b1 - b0, d.perm, f, x, omega);
+                    dev::mcsor_colour(st, pickAc<T>(d), b0, b1 - b0, d.colour_rows[c], d.colour_rows[c + 1], d.perm,
+                                      f, x, omega);
                 }
             break;
         }
         case LINA_SMOOTHER_JACOBI:
         case LINA_SMOOTHER_L1_JACOBI: {
-            int kind = prm_.smoother == LINA_SMOOTHER_JACOBI ? dev::JAC_REF : dev::JAC_L1;
+            // ping-pong between x and the level's spare vector; an odd sweep
+            // count ends with one copy back
+            int kind = smoother == LINA_SMOOTHER_JACOBI ? dev::JAC_REF : dev::JAC_L1;
+            T* a = x;
+            T* b = LevelVecs<T>::xo(d);
             for (int s = 0; s < sweeps; s++) {
-                dev::copy(st, d.n, (const T*)x, LevelVecs<T>::xo(d));
-                dev::jacobi(st, pickA<T>(d), kind, f, (const T*)LevelVecs<T>::xo(d), x, d.nlim);
+                dev::jacobi(st, pickA<T>(d), kind, f, (const T*)a, b, d.nlim);
+                std::swap(a, b);
             }
+            if (a != x) dev::copy(st, d.n, (const T*)a, x);
             break;
         }
         default:
@@ -465,8 +478,8 @@ template <typename T>
 double Solver::residual_norm(const T* f, const T* x, T* r, int slot) {
     DevLevel& d = L_.back();
     const auto& A = pickA<T>(d);
-    dev::spmv<T, T>(stream_, A, x, f, r, dev::MODE_RES, part_, nullptr);
-    dev::finalize(stream_, A.nblk, part_, scal_, slot, 1);
+    int np = dev::spmv<T, T>(stream_, A, x, f, r, dev::MODE_RES, part_, nullptr);
+    dev::finalize(stream_, np, part_, scal_, slot, 1);
     return read_scalar(slot);
 }
 
@@ -561,8 +574,8 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
         np = dev::dot_part<double>(st, n, R, Z, part_);
         dev::finalize(st, np, part_, scal_, dev::S_S, 2);  // s_old = s; s = R.Z
         dev::pcg_p(st, n, P, Z, scal_, i == 0);
-        dev::spmv<double, double>(st, top.A, P, nullptr, Q, dev::MODE_SET, part_, P);  // Q = A P, partial P.Q
-        dev::finalize(st, top.A.nblk, part_, scal_, dev::S_PQ, 0);
+        np = dev::spmv<double, double>(st, top.A, P, nullptr, Q, dev::MODE_SET, part_, P);  // Q = A P, P.Q
+        dev::finalize(st, np, part_, scal_, dev::S_PQ, 0);
         np = dev::pcg_xr(st, n, X, R, P, Q, scal_, part_);
         dev::finalize(st, np, part_, scal_, dev::S_R, 1);
         r = read_scalar(dev::S_R);

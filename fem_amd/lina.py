@@ -26,7 +26,7 @@ class Params(C.Structure):
                 ("aggr_smoothing_iterations", C.c_int), ("coarsening", C.c_int), ("eps_minus", C.c_double),
                 ("eps_plus", C.c_double), ("matrix_tol", C.c_double), ("smoother", C.c_int), ("omega", C.c_double),
                 ("amg_eps", C.c_double), ("amg_max_iter", C.c_int), ("zero_init", C.c_int),
-                ("precision", C.c_int)]
+                ("precision", C.c_int), ("coarse_smoother", C.c_int), ("coarse_sweeps", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -113,7 +113,7 @@ def params(kind: str = "reference", dof: int = 1, depth: int = 4, **kw) -> Param
     else:
         raise ValueError(kind)
     for k, v in kw.items():
-        if k == "smoother" and isinstance(v, str):
+        if k in ("smoother", "coarse_smoother") and isinstance(v, str):
             v = SMOOTHER[v]
         if k == "precision" and isinstance(v, str):
             v = PREC[v]

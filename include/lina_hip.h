@@ -66,6 +66,8 @@ typedef struct lina_amg_params {
     int amg_max_iter;              /* AMG_set_max_iter */
     int zero_init;                 /* 1: the preconditioner starts every call from Z = 0 */
     int precision;                 /* LINA_PREC_* */
+    int coarse_smoother;           /* smoother of the levels below the finest, -1: `smoother` */
+    int coarse_sweeps;             /* sweeps per call there (when coarse_smoother >= 0) */
 } lina_amg_params;
 
 typedef struct lina_stats {
@@ -85,7 +87,9 @@ int lina_device_count(void);
 /* parameters of PCG_set_AMG_preconditioner (PCGS.c:118-129): SOR(1.6), 2/3
  * sweeps, aggressive first coarsening, AMG solve to 1e-2, <= 100 V-cycles */
 void lina_params_reference(lina_amg_params* p, int dof, int depth);
-/* one symmetric multicolour-SOR V(1,1)-cycle per Krylov iteration */
+/* one symmetric V-cycle per Krylov iteration: multicolour SOR (forward
+ * before, colours reversed after the coarse correction) on the finest level,
+ * two l1-Jacobi sweeps on the coarser levels */
 void lina_params_vcycle(lina_amg_params* p, int dof, int depth);
 
 /* Build the hierarchy for A (host CSR).  upload=0 runs the host setup only

@@ -21,7 +21,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _vp = C.c_void_p
 
-SMOOTHERS = {"sor": 0, "jacobi": 1, "mc_sor": 2, "l1_jacobi": 3, "mc_ssor": 4, "ssor": 5}
+SMOOTHERS = {"sor": 0, "jacobi": 1, "mc_sor": 2, "l1_jacobi": 3, "mc_ssor": 4, "ssor": 5, "hybrid": 6}
 
 
 def dptr(a: np.ndarray):
@@ -102,6 +102,7 @@ class Ref:
         L.refh_bicgstab_amg.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
         L.refh_reflective_newton.argtypes = [_vp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
                                              C.c_double, C.c_double, _dp]
+        L.refh_set_hybrid.argtypes = [C.c_int]
         L.refh_init(1)
 
     # -- matrices
@@ -153,16 +154,19 @@ class Ref:
     def amg_free(self, I):
         self.L.refh_amg_free(I)
 
-    def amg_solve(self, I, F, X, smoother="sor", alpha=1.6, eps=1e-2, max_iter=100):
+    def amg_solve(self, I, F, X, smoother="sor", alpha=1.6, eps=1e-2, max_iter=100, coarse_sweeps=2):
         F = np.array(F, dtype=np.float64)
         X = np.array(X, dtype=np.float64)
+        self.L.refh_set_hybrid(coarse_sweeps)
         k = self.L.refh_amg_solve(I, dptr(F), dptr(X), SMOOTHERS[smoother], alpha, eps, max_iter)
         return k, X
 
     # -- solvers
     def pcg_amg(self, M, b, x0=None, dof=1, depth=4, smiter=2, aggr_smiter=3, coarsening=1, smoother="sor",
-                alpha=1.6, amg_eps=1e-2, amg_max_iter=100, zero_init=False, tol=1e-10, max_iter=1000, keep=False):
+                alpha=1.6, amg_eps=1e-2, amg_max_iter=100, zero_init=False, tol=1e-10, max_iter=1000, keep=False,
+                coarse_sweeps=2):
         n = self.L.refh_size(M)
+        self.L.refh_set_hybrid(coarse_sweeps)
         X = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
         B = np.array(b, dtype=np.float64)
         res = C.c_double()

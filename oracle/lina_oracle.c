@@ -806,6 +806,8 @@ struct orc_amg {
     /* colourings per level for the multicolour smoothers */
     int** colour;
     int* ncolour;
+    /* smoother of the levels below the finest (-1: the same as the finest) */
+    int coarse_sm, coarse_sweeps;
 };
 
 /* LU_factorzation (linear_algebra.c:2909-2942): unit-lower L, U by bordering,
@@ -965,6 +967,8 @@ orc_amg* orc_amg_setup(int n, const int* rp, const int* ci, const double* v, int
     h->coarse = (int*)calloc(depth, sizeof(int));
     h->colour = (int**)calloc(depth, sizeof(int*));
     h->ncolour = (int*)calloc(depth, sizeof(int));
+    h->coarse_sm = -1;
+    h->coarse_sweeps = 1;
     /* clone + make_positive_diag (linear_algebra.c:2612-2620) */
     mat_from(&h->A[depth - 1], n, rp, ci, v);
     h->signs = (double*)malloc((n > 0 ? n : 1) * sizeof(double));
@@ -1239,6 +1243,10 @@ static void vcycle(orc_amg* h, int l, const double* F, double* X, int sm, double
     }
     nc = h->A[l - 1].n;
     sw = (h->coarsening == AGGRESSIVE && l == h->depth - 1) ? h->aggr_smiter : h->smiter;
+    if (l < h->depth - 1 && h->coarse_sm >= 0) {
+        sm = h->coarse_sm;
+        sw = h->coarse_sweeps;
+    }
     smooth(h, l, F, X, sw, w, sm, 0);
     R = (double*)malloc((n > 0 ? n : 1) * sizeof(double));
     memcpy(R, F, n * sizeof(double));
@@ -1365,4 +1373,10 @@ int orc_pcg_amg(orc_amg* h, double* X, double* B, double tol, int max_iter, int 
     free(W); free(Q); free(Z); free(P); free(R); free(Y);
 #undef PRECON
     return it;
+}
+
+/* smoother of the levels below the finest; sm < 0 restores the finest's */
+void orc_set_coarse_smoother(orc_amg* h, int sm, int sweeps) {
+    h->coarse_sm = sm;
+    h->coarse_sweeps = sweeps;
 }

@@ -28,6 +28,9 @@ int orc_amg_labels(const orc_amg* h, int l, int* labels);
 /* AMG_recompute_coarse_matrices (Multigrid.c:1756-1780) with new finest values */
 int orc_amg_recompute(orc_amg* h, const double* v);
 
+/* smoother on the levels below the finest (sm < 0: same as the finest) */
+void orc_set_coarse_smoother(orc_amg* h, int sm, int sweeps);
+
 /* AMG_solve (Multigrid.c:1863-1931); smoother ids as lina_hip.h */
 int orc_amg_solve(orc_amg* h, const double* F, double* X, int smoother, double omega, double eps, int max_iter);
 

@@ -244,6 +244,17 @@ void refh_ssor_smoother(sparse_matrix* A, double* F, double* Sol, int* Label, do
         }
 }
 
+/* Level-dependent smoother: multicolour SSOR on the finest level, l1-Jacobi
+ * with its own sweep count on every coarser level (symmetric V-cycle). */
+static sparse_matrix* hyb_finest = NULL;
+static int hyb_coarse_sweeps = 2;
+void refh_hybrid_smoother(sparse_matrix* A, double* F, double* Sol, int* Label, double alpha, int fix, int iter,
+                          int deg_freedom) {
+    if (A == hyb_finest) refh_mc_ssor_smoother(A, F, Sol, Label, alpha, fix, iter, deg_freedom);
+    else refh_l1_jacobi_smoother(A, F, Sol, Label, alpha, fix, hyb_coarse_sweeps, deg_freedom);
+}
+void refh_set_hybrid(int coarse_sweeps) { hyb_coarse_sweeps = coarse_sweeps; }
+
 int refh_colour_count(sparse_matrix* A, int* color_out) {
     int ncol;
     int* c = greedy_colouring(A, &ncol);
@@ -260,6 +271,7 @@ static void set_smoother(int id) {
         case 3: AMG_set_smoother(&refh_l1_jacobi_smoother); break;
         case 4: AMG_set_smoother(&refh_mc_ssor_smoother); break;
         case 5: AMG_set_smoother(&refh_ssor_smoother); break;
+        case 6: AMG_set_smoother(&refh_hybrid_smoother); break;
         default: AMG_set_smoother(&AMG_SOR_smoother); break;
     }
 }
@@ -304,6 +316,7 @@ void refh_amg_recompute(amg_system_info* I, sparse_matrix* M) { AMG_recompute_co
 int refh_amg_solve(amg_system_info* I, double* F, double* X, int smoother, double sor_alpha,
                    double eps, int max_iter) {
     set_smoother(smoother);
+    hyb_finest = I->Coarse_A[I->depth - 1];
     AMG_set_SOR_relaxation_coeff(sor_alpha);
     AMG_set_eps(eps);
     AMG_set_max_iter(max_iter);
@@ -349,6 +362,7 @@ int refh_pcg_amg(sparse_matrix* A, int dof, int depth, int smiter, int aggr_smit
     h_amg = AMG_setup(A, dof, depth, smiter, aggr_smiter, coarsening, STAND_ALONE, DIRECT, NULL, NULL);
     clock_gettime(CLOCK_MONOTONIC, &t1);
     set_smoother(smoother);
+    hyb_finest = h_amg->Coarse_A[h_amg->depth - 1];
     AMG_set_SOR_relaxation_coeff(sor_alpha);
     AMG_set_eps(amg_eps);
     AMG_set_max_iter(amg_max_iter);
