@@ -117,7 +117,7 @@ void lina_params_vcycle(lina_amg_params* p, int dof, int depth) {
     p->amg_max_iter = 1;
     p->zero_init = 1;
     p->coarse_smoother = LINA_SMOOTHER_L1_JACOBI;
-    p->coarse_sweeps = 2;
+    p->coarse_sweeps = 1;
 }
 
 lina_solver* lina_amg_create(int n, const int* rowptr, const int* col, const double* val, const lina_amg_params* p,

@@ -1,17 +1,26 @@
 // Device kernels of liblina_hip (gfx950 / CDNA4, wave64).
 //
-// Every kernel that reproduces a reference loop keeps that loop's order of
-// floating-point operations inside a row: the CSR kernels first stage the
-// products a_ij*x_j of a block of rows in LDS (coalesced streaming of the
-// value and index arrays), then one lane per row adds them up in column
-// order, which is the order of the reference's `sum += A->Values[i][j]*x[...]`
-// loops (LinA/linear_algebra.c:134-184).  The library is compiled with
-// -ffp-contract=off, so no multiply-add is fused behind the reference's back.
+// Row kernels (SpMV, multicolour SOR, Jacobi) share one structure:
 //
-// Reductions (dots, norms) are deterministic: fixed per-block trees, block
-// partials stored by logical block index, a single-block finalisation.
+//   phase 1  all 256 lanes of a workgroup stream the value and column arrays
+//            of a block of consecutive rows (<= BLOCK_NNZ entries, coalesced,
+//            nontemporal: each matrix entry is read once per sweep) and
+//            store the products a_ij * x_j in LDS;
+//   phase 2  rows are reduced from LDS by groups of LPR lanes.  LPR == 1
+//            adds a row's products one after another in column order, the
+//            order of the reference's `sum += A->Values[i][j]*x[...]` loops
+//            (LinA/linear_algebra.c:134-184), so the result is bit-identical;
+//            LPR > 1 (the dense Galerkin coarse levels, 50-200 entries a row)
+//            adds lane partials with a fixed xor-shuffle tree.
+//
+// A row longer than BLOCK_NNZ gets a block of its own and is reduced by the
+// whole workgroup.  The library is compiled with -ffp-contract=off, so no
+// multiply-add is fused.  Reductions (dots, norms) are deterministic: fixed
+// per-block trees, block partials stored by logical block index, a
+// single-block finalisation.
 
 #include <hip/hip_runtime.h>
+
 #include <algorithm>
 #include <cstdint>
 
@@ -20,7 +29,7 @@
 namespace lina {
 namespace dev {
 
-constexpr int TPB = 256;           // threads per block for all row kernels
+constexpr int TPB = 256;  // threads per block of the row kernels
 constexpr int WAVE = 64;
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
@@ -40,6 +49,12 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+template <int LPR, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+    for (int o = LPR / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, LPR);
+    return v;
+}
+
 // deterministic block sum (fixed shuffle tree, then waves summed in order)
 __device__ __forceinline__ double block_sum(double v) {
     __shared__ double ws[TPB / WAVE];
@@ -54,6 +69,18 @@ __device__ __forceinline__ double block_sum(double v) {
     return s;  // valid in thread 0
 }
 
+template <typename T>
+__device__ __forceinline__ T ldnt(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+
+// phase 1: prod[k-k0] = v[k] * x[ci[k]] for the block's entries
+template <typename VT, typename XT>
+__device__ __forceinline__ void stage_products(int k0, int k1, const int* __restrict__ ci, const VT* __restrict__ v,
+                                               const XT* __restrict__ x, XT* prod) {
+    for (int k = k0 + threadIdx.x; k < k1; k += TPB) prod[k - k0] = (XT)ldnt(v + k) * x[ldnt(ci + k)];
+}
+
 // ------------------------------------------------------------------ SpMV
 //
 // MODE_SET : y = A x                        (sparse_mult / PCG_mult)
@@ -61,7 +88,17 @@ __device__ __forceinline__ double block_sum(double v) {
 // MODE_ADD : y = y + (A x)                  (linear_map(X, 1,P,c))
 // part != nullptr: block partial of sum_r y_r * w_r (w == nullptr: y_r^2)
 
-template <typename VT, typename XT, int MODE>
+template <typename XT, int MODE>
+__device__ __forceinline__ XT spmv_out(XT s, int r, const XT* b, XT* y) {
+    XT out;
+    if (MODE == MODE_SET) out = s;
+    else if (MODE == MODE_RES) out = b[r] + (-s);
+    else out = y[r] + s;
+    y[r] = out;
+    return out;
+}
+
+template <typename VT, typename XT, int MODE, int LPR>
 __global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int* __restrict__ blk, const int* __restrict__ rp,
                                               const int* __restrict__ ci, const VT* __restrict__ v,
                                               const XT* __restrict__ x, const XT* __restrict__ b,
@@ -73,138 +110,33 @@ __global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int* __restrict__ 
     const int k0 = rp[r0], k1 = rp[r1];
     double mine = 0.0;
     if (k1 - k0 > BLOCK_NNZ) {
-        // one long row (the block builder isolates rows longer than a block):
-        // strided partial sums + tree, the only non-sequential row sum
+        // one long row: strided partial sums + block tree
         XT s = 0;
         for (int k = k0 + threadIdx.x; k < k1; k += TPB) s += (XT)v[k] * x[ci[k]];
         double t = block_sum((double)s);
         if (threadIdx.x == 0) {
-            XT sv = (XT)t;
-            XT out;
-            if (MODE == MODE_SET) out = sv;
-            else if (MODE == MODE_RES) out = b[r0] + (-sv);
-            else out = y[r0] + sv;
-            y[r0] = out;
+            XT out = spmv_out<XT, MODE>((XT)t, r0, b, y);
             if (part) mine = (double)out * (double)(w ? w[r0] : out);
         }
     } else {
-        for (int k = k0 + threadIdx.x; k < k1; k += TPB) prod[k - k0] = (XT)v[k] * x[ci[k]];
+        stage_products(k0, k1, ci, v, x, prod);
         __syncthreads();
-        const int r = r0 + threadIdx.x;
-        if (r < r1) {
+        constexpr int G = TPB / LPR;
+        const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+        for (int r = r0 + g; r < r1; r += G) {
             XT s = 0;
             const int e = rp[r + 1] - k0;
-            for (int k = rp[r] - k0; k < e; k++) s += prod[k];
-            XT out;
-            if (MODE == MODE_SET) out = s;
-            else if (MODE == MODE_RES) out = b[r] + (-s);
-            else out = y[r] + s;
-            y[r] = out;
-            if (part) mine = (double)out * (double)(w ? w[r] : out);
-        }
-    }
-    if (part) {
-        double t = block_sum(mine);
-        if (threadIdx.x == 0) part[bb] = t;
-    }
-}
-
-// ------------------------------------------------ vector (long-row) path
-//
-// Matrices whose rows average many entries (the Galerkin coarse levels: 50
-// to 200 per row) would leave a single lane with a long dependent chain.
-// There LPR lanes share a row: each lane walks the row with stride LPR, the
-// partial sums are combined by a fixed xor-shuffle tree (deterministic, but a
-// different association than the sequential sum).  Block b handles rows
-// [b*RPB, (b+1)*RPB) of the range, RPB = 2*(TPB/LPR).
-
-template <int LPR, typename T>
-__device__ __forceinline__ T group_sum(T v) {
-    for (int o = LPR / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, LPR);
-    return v;
-}
-
-template <typename VT, typename XT, int MODE, int LPR>
-__global__ __launch_bounds__(TPB) void k_spmv_vec(int n, int nblk, const int* __restrict__ rp,
-                                                  const int* __restrict__ ci, const VT* __restrict__ v,
-                                                  const XT* __restrict__ x, const XT* __restrict__ b,
-                                                  XT* __restrict__ y, double* __restrict__ part,
-                                                  const XT* __restrict__ w) {
-    constexpr int G = TPB / LPR, RPB = 2 * G;
-    const int bb = xcd_block(blockIdx.x, nblk);
-    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
-    double mine = 0.0;
-    for (int r = bb * RPB + g; r < min(n, (bb + 1) * RPB); r += G) {
-        XT s = 0;
-        for (int k = rp[r] + lane; k < rp[r + 1]; k += LPR) s += (XT)v[k] * x[ci[k]];
-        s = group_sum<LPR>(s);
-        if (lane == 0) {
-            XT out;
-            if (MODE == MODE_SET) out = s;
-            else if (MODE == MODE_RES) out = b[r] + (-s);
-            else out = y[r] + s;
-            y[r] = out;
-            if (part) mine += (double)out * (double)(w ? w[r] : out);
-        }
-    }
-    if (part) {
-        double t = block_sum(mine);
-        if (threadIdx.x == 0) part[bb] = t;
-    }
-}
-
-// multicolour SOR over rows [r0, r1) of the colour-permuted copy
-template <typename VT, typename XT, int LPR>
-__global__ __launch_bounds__(TPB) void k_mcsor_vec(int r0, int r1, const int* __restrict__ rp,
-                                                   const int* __restrict__ ci, const VT* __restrict__ v,
-                                                   const int* __restrict__ perm, const XT* __restrict__ f,
-                                                   XT* __restrict__ x, XT omega) {
-    constexpr int G = TPB / LPR, RPB = 2 * G;
-    const int nb = (r1 - r0 + RPB - 1) / RPB;
-    const int bb = xcd_block(blockIdx.x, nb);
-    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
-    for (int r = r0 + bb * RPB + g; r < min(r1, r0 + (bb + 1) * RPB); r += G) {
-        const int i = perm[r];
-        XT s = 0, d = 0;
-        for (int k = rp[r] + lane; k < rp[r + 1]; k += LPR) {
-            int j = ci[k];
-            if (j != i) s += (XT)v[k] * x[j];
-            else d += (XT)v[k];
-        }
-        s = group_sum<LPR>(s);
-        d = group_sum<LPR>(d);
-        if (lane == 0) x[i] = ((XT)1 - omega) * x[i] + omega * (f[i] - s) / d;
-    }
-}
-
-// Jacobi sweeps x = J(xo) on the vector path (see k_jacobi)
-template <typename VT, typename XT, int KIND, int LPR>
-__global__ __launch_bounds__(TPB) void k_jacobi_vec(int n, int nblk, const int* __restrict__ rp,
-                                                    const int* __restrict__ ci, const VT* __restrict__ v,
-                                                    const XT* __restrict__ f, const XT* __restrict__ xo,
-                                                    XT* __restrict__ x, int nlim) {
-    constexpr int G = TPB / LPR, RPB = 2 * G;
-    const int bb = xcd_block(blockIdx.x, nblk);
-    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
-    for (int r = bb * RPB + g; r < min(n, (bb + 1) * RPB); r += G) {
-        XT s = 0, d = 0;
-        for (int k = rp[r] + lane; k < rp[r + 1]; k += LPR) {
-            int j = ci[k];
-            XT a = (XT)v[k];
-            if (KIND == JAC_REF) {
-                if (j != r) s += a * xo[j];
-                else d += a;
-            } else {
-                s += a * xo[j];
-                d += fabs(a);
+            for (int k = rp[r] - k0 + lane; k < e; k += LPR) s += prod[k];
+            if (LPR > 1) s = group_sum<LPR>(s);
+            if (lane == 0) {
+                XT out = spmv_out<XT, MODE>(s, r, b, y);
+                if (part) mine += (double)out * (double)(w ? w[r] : out);
             }
         }
-        s = group_sum<LPR>(s);
-        d = group_sum<LPR>(d);
-        if (lane == 0) {
-            if (KIND == JAC_REF) x[r] = r < nlim ? (f[r] - s) / d : xo[r];
-            else x[r] = xo[r] + (f[r] - s) / d;
-        }
+    }
+    if (part) {
+        double t = block_sum(mine);
+        if (threadIdx.x == 0) part[bb] = t;
     }
 }
 
@@ -319,47 +251,59 @@ __global__ void k_axpby(int n, double a, double* __restrict__ x, double b, const
 
 // ------------------------------------------------------------- smoothers
 
-// Multicolour SOR, one colour: rows of the colour-permuted copy in block
-// range [blk0, blk0+nblk); row r of the copy is original row perm[r].
-// x_i = (1-w) x_i + w*(f_i - sum_{j!=i} a_ij x_j)/a_ii, columns in order
-// (the arithmetic of AMG_SOR_smoother, Multigrid.c:1345-1365).
-template <typename VT, typename XT>
+// Multicolour SOR, one colour: the rows of the colour-permuted copy in the
+// block range [blk0, blk0+nblk); row r of the copy is original row perm[r].
+// x_i = (1-w) x_i + w*(f_i - sum_{j!=i} a_ij x_j)/a_ii with the arithmetic of
+// AMG_SOR_smoother (Multigrid.c:1345-1365).  Rows of one colour are not
+// coupled, so the products read x before any row of the colour is updated.
+template <typename VT, typename XT, int LPR>
 __global__ __launch_bounds__(TPB) void k_mcsor(int nblk, int blk0, const int* __restrict__ blk,
                                                const int* __restrict__ rp, const int* __restrict__ ci,
-                                               const VT* __restrict__ v, const int* __restrict__ perm,
-                                               const XT* __restrict__ f, XT* __restrict__ x, XT omega) {
+                                               const VT* __restrict__ v, const int* __restrict__ dpos,
+                                               const int* __restrict__ perm, const XT* __restrict__ f,
+                                               XT* __restrict__ x, XT omega) {
     __shared__ XT prod[BLOCK_NNZ];
     const int bb = blk0 + xcd_block(blockIdx.x, nblk);
     const int r0 = blk[bb], r1 = blk[bb + 1];
     const int k0 = rp[r0], k1 = rp[r1];
     if (k1 - k0 > BLOCK_NNZ) {
-        // long row: one thread walks it (exact order)
+        // one long row: the whole workgroup reduces it (block tree)
+        const int i = perm[r0];
+        const int dp = dpos[r0];
+        double s = 0.0;
+        for (int k = k0 + threadIdx.x; k < k1; k += TPB)
+            if (k != dp) s += (double)((XT)v[k] * x[ci[k]]);
+        s = block_sum(s);
         if (threadIdx.x == 0) {
-            int i = perm[r0];
-            XT sum = f[i];
-            XT d = 0;
-            for (int k = k0; k < k1; k++) {
-                int j = ci[k];
-                if (j != i) sum -= (XT)v[k] * x[j];
-                else d = (XT)v[k];
-            }
-            x[i] = ((XT)1 - omega) * x[i] + omega * sum / d;
+            const XT d = dp >= 0 ? (XT)v[dp] : (XT)0;
+            x[i] = ((XT)1 - omega) * x[i] + omega * (f[i] - (XT)s) / d;
         }
         return;
     }
-    for (int k = k0 + threadIdx.x; k < k1; k += TPB) prod[k - k0] = (XT)v[k] * x[ci[k]];
+    stage_products(k0, k1, ci, v, x, prod);
     __syncthreads();
-    const int r = r0 + threadIdx.x;
-    if (r < r1) {
+    constexpr int G = TPB / LPR;
+    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+    for (int r = r0 + g; r < r1; r += G) {
         const int i = perm[r];
-        XT sum = f[i];
-        XT d = 0;
         const int e = rp[r + 1];
-        for (int k = rp[r]; k < e; k++) {
-            if (ci[k] != i) sum -= prod[k - k0];
-            else d = (XT)v[k];
+        const int dp = dpos[r];
+        if (LPR == 1) {
+            XT sum = f[i];
+            for (int k = rp[r]; k < e; k++)
+                if (k != dp) sum -= prod[k - k0];
+            const XT d = dp >= 0 ? (XT)v[dp] : (XT)0;
+            x[i] = ((XT)1 - omega) * x[i] + omega * sum / d;
+        } else {
+            XT s = 0;
+            for (int k = rp[r] + lane; k < e; k += LPR)
+                if (k != dp) s += prod[k - k0];
+            s = group_sum<LPR>(s);
+            if (lane == 0) {
+                const XT d = dp >= 0 ? (XT)v[dp] : (XT)0;
+                x[i] = ((XT)1 - omega) * x[i] + omega * (f[i] - s) / d;
+            }
         }
-        x[i] = ((XT)1 - omega) * x[i] + omega * sum / d;
     }
 }
 
@@ -395,43 +339,65 @@ __global__ __launch_bounds__(1024) void k_sor_lex(int n, const int* __restrict__
     }
 }
 
-// Jacobi sweeps reading the snapshot xo:
-//   JAC_REF : x_i = (f_i - sum_{j!=i} a_ij xo_j)/a_ii for i < nlim
-//             (AMG_parallel_Jacobi_smoother, Multigrid.c:1367-1400)
+// Jacobi sweep x = J(xo) (x and xo distinct):
+//   JAC_REF : x_i = (f_i - sum_{j!=i} a_ij xo_j)/a_ii for i < nlim, x_i = xo_i
+//             beyond (AMG_parallel_Jacobi_smoother, Multigrid.c:1367-1400)
 //   JAC_L1  : x_i = xo_i + (f_i - sum_j a_ij xo_j)/sum_j |a_ij|
-template <typename VT, typename XT, int KIND>
+template <typename VT, typename XT, int KIND, int LPR>
 __global__ __launch_bounds__(TPB) void k_jacobi(int nblk, const int* __restrict__ blk, const int* __restrict__ rp,
                                                 const int* __restrict__ ci, const VT* __restrict__ v,
+                                                const int* __restrict__ dpos, const VT* __restrict__ l1,
                                                 const XT* __restrict__ f, const XT* __restrict__ xo,
                                                 XT* __restrict__ x, int nlim) {
     __shared__ XT prod[BLOCK_NNZ];
     const int bb = xcd_block(blockIdx.x, nblk);
     const int r0 = blk[bb], r1 = blk[bb + 1];
     const int k0 = rp[r0], k1 = rp[r1];
-    const bool lds = (k1 - k0 <= BLOCK_NNZ);
-    if (lds) {
-        for (int k = k0 + threadIdx.x; k < k1; k += TPB) prod[k - k0] = (XT)v[k] * xo[ci[k]];
-        __syncthreads();
-    }
-    for (int r = r0 + threadIdx.x; r < r1; r += TPB) {
-        if (!lds && threadIdx.x != 0) continue;
+    if (k1 - k0 > BLOCK_NNZ) {
+        // one long row: the whole workgroup reduces it (block tree)
+        const int r = r0;
         if (KIND == JAC_REF && r >= nlim) {
-            x[r] = xo[r];  // rows past (n/dof)*dof are not swept (Multigrid.c:1380-1383)
+            if (threadIdx.x == 0) x[r] = xo[r];
+            return;
+        }
+        const int dp = KIND == JAC_REF ? dpos[r] : -1;
+        double s = 0.0;
+        for (int k = k0 + threadIdx.x; k < k1; k += TPB)
+            if (k != dp) s += (double)((XT)v[k] * xo[ci[k]]);
+        s = block_sum(s);
+        if (threadIdx.x == 0) {
+            const XT rr = f[r] - (XT)s;
+            if (KIND == JAC_REF) x[r] = rr / (dp >= 0 ? (XT)v[dp] : (XT)0);
+            else x[r] = xo[r] + rr / l1[r];
+        }
+        return;
+    }
+    stage_products(k0, k1, ci, v, xo, prod);
+    __syncthreads();
+    constexpr int G = TPB / LPR;
+    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+    for (int r = r0 + g; r < r1; r += G) {
+        const int e = rp[r + 1];
+        if (KIND == JAC_REF && r >= nlim) {
+            if (lane == 0) x[r] = xo[r];  // rows past (n/dof)*dof are not swept
             continue;
         }
-        XT sum = f[r];
-        XT d = 0;
-        for (int k = rp[r]; k < rp[r + 1]; k++) {
-            XT p = lds ? prod[k - k0] : (XT)v[k] * xo[ci[k]];
-            if (KIND == JAC_REF) {
-                if (ci[k] != r) sum -= p;
-                else d = (XT)v[k];
-            } else {
-                sum -= p;
-                d += fabs((XT)v[k]);
-            }
+        const int dp = KIND == JAC_REF ? dpos[r] : -1;
+        XT s;
+        if (LPR == 1) {
+            s = f[r];
+            for (int k = rp[r]; k < e; k++)
+                if (k != dp) s -= prod[k - k0];
+        } else {
+            s = 0;
+            for (int k = rp[r] + lane; k < e; k += LPR)
+                if (k != dp) s += prod[k - k0];
+            s = f[r] - group_sum<LPR>(s);
         }
-        x[r] = (KIND == JAC_REF) ? sum / d : xo[r] + sum / d;
+        if (lane == 0) {
+            if (KIND == JAC_REF) x[r] = s / (dp >= 0 ? (XT)v[dp] : (XT)0);
+            else x[r] = xo[r] + s / l1[r];
+        }
     }
 }
 
@@ -449,7 +415,7 @@ __global__ __launch_bounds__(TPB) void k_coarse(int nd, const int* __restrict__ 
     for (int row = wave; row < nd; row += nw) {
         const double* a = inv + (size_t)row * nd;
         double s = 0.0;
-        for (int c = lane; c < nd; c += WAVE) s += a[c] * (double)b[didx[c]];
+        for (int c = lane; c < nd; c += WAVE) s += ldnt(a + c) * (double)b[didx[c]];
         s = wave_sum(s);
         if (lane == 0) x[didx[row]] = (XT)s;
     }
@@ -492,59 +458,34 @@ __global__ __launch_bounds__(TPB) void k_gj_pivot(int N, int k, double* __restri
 
 // ---------------------------------------------------------- launchers
 
-// piv: one device double of scratch
-void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* piv) {
-    if (N <= 0) return;
-    int g = std::min(N, 4096);
-    int gp = (N + TPB - 1) / TPB;
-    for (int k = 0; k < N; k++) {
-        hipLaunchKernelGGL(k_gj_update, dim3(g), dim3(TPB), 0, st, N, k, M, piv);
-        hipLaunchKernelGGL(k_gj_pivot, dim3(gp), dim3(TPB), 0, st, N, k, M, (const double*)piv);
-    }
-}
-
 static inline int grid_for(int n, int cap = 4096) {
     int g = (n + TPB - 1) / TPB;
     if (g < 1) g = 1;
     return g > cap ? cap : g;
 }
 
-#define LINA_LPR_DISPATCH(LPR_VAL, ...)          \
-    switch (LPR_VAL) {                            \
+#define LINA_LPR_DISPATCH(LPR_VAL, ...)                    \
+    switch (LPR_VAL) {                                     \
+        case 1: { constexpr int L = 1; __VA_ARGS__; break; }   \
+        case 4: { constexpr int L = 4; __VA_ARGS__; break; }   \
         case 8: { constexpr int L = 8; __VA_ARGS__; break; }   \
         case 16: { constexpr int L = 16; __VA_ARGS__; break; } \
-        case 32: { constexpr int L = 32; __VA_ARGS__; break; } \
-        default: { constexpr int L = 64; __VA_ARGS__; break; } \
+        default: { constexpr int L = 32; __VA_ARGS__; break; } \
     }
 
 template <typename VT, typename XT>
 int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w) {
-    if (A.n == 0) return 0;
-    dim3 t(TPB);
-    if (A.lpr > 1) {
-        dim3 g(A.nvblk);
-        if (mode == MODE_SET) {
-            LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv_vec<VT, XT, MODE_SET, L>), g, t, 0, st, A.n, A.nvblk,
-                                                        A.rp, A.ci, A.v, x, b, y, part, w))
-        } else if (mode == MODE_RES) {
-            LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv_vec<VT, XT, MODE_RES, L>), g, t, 0, st, A.n, A.nvblk,
-                                                        A.rp, A.ci, A.v, x, b, y, part, w))
-        } else {
-            LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv_vec<VT, XT, MODE_ADD, L>), g, t, 0, st, A.n, A.nvblk,
-                                                        A.rp, A.ci, A.v, x, b, y, part, w))
-        }
-        return A.nvblk;
-    }
-    dim3 g(A.nblk);
-    switch (mode) {
-        case MODE_SET:
-            hipLaunchKernelGGL((k_spmv<VT, XT, MODE_SET>), g, t, 0, st, A.nblk, A.blk, A.rp, A.ci, A.v, x, b, y, part, w);
-            break;
-        case MODE_RES:
-            hipLaunchKernelGGL((k_spmv<VT, XT, MODE_RES>), g, t, 0, st, A.nblk, A.blk, A.rp, A.ci, A.v, x, b, y, part, w);
-            break;
-        default:
-            hipLaunchKernelGGL((k_spmv<VT, XT, MODE_ADD>), g, t, 0, st, A.nblk, A.blk, A.rp, A.ci, A.v, x, b, y, part, w);
+    if (A.nblk == 0) return 0;
+    dim3 g(A.nblk), t(TPB);
+    if (mode == MODE_SET) {
+        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, MODE_SET, L>), g, t, 0, st, A.nblk, A.blk, A.rp,
+                                                    A.ci, A.v, x, b, y, part, w))
+    } else if (mode == MODE_RES) {
+        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, MODE_RES, L>), g, t, 0, st, A.nblk, A.blk, A.rp,
+                                                    A.ci, A.v, x, b, y, part, w))
+    } else {
+        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, MODE_ADD, L>), g, t, 0, st, A.nblk, A.blk, A.rp,
+                                                    A.ci, A.v, x, b, y, part, w))
     }
     return A.nblk;
 }
@@ -627,24 +568,16 @@ void axpby(hipStream_t st, int n, double a, double* x, double b, const double* y
 }
 
 template <typename VT, typename XT>
-void mcsor_colour(hipStream_t st, const DMat<VT>& Ac, int blk0, int nblk, int r0, int r1, const int* perm,
-                  const XT* f, XT* x, XT omega) {
-    if (r1 <= r0) return;
-    if (Ac.lpr > 1) {
-        const int rpb = 2 * (TPB / Ac.lpr);
-        dim3 g((r1 - r0 + rpb - 1) / rpb), t(TPB);
-        LINA_LPR_DISPATCH(Ac.lpr, hipLaunchKernelGGL((k_mcsor_vec<VT, XT, L>), g, t, 0, st, r0, r1, Ac.rp, Ac.ci,
-                                                     Ac.v, perm, f, x, omega))
-        return;
-    }
+void mcsor_colour(hipStream_t st, const DMat<VT>& Ac, int blk0, int nblk, const int* perm, const XT* f, XT* x,
+                  XT omega) {
     if (nblk <= 0) return;
-    hipLaunchKernelGGL((k_mcsor<VT, XT>), dim3(nblk), dim3(TPB), 0, st, nblk, blk0, Ac.blk, Ac.rp, Ac.ci, Ac.v, perm,
-                       f, x, omega);
+    LINA_LPR_DISPATCH(Ac.lpr, hipLaunchKernelGGL((k_mcsor<VT, XT, L>), dim3(nblk), dim3(TPB), 0, st, nblk, blk0,
+                                                 Ac.blk, Ac.rp, Ac.ci, Ac.v, Ac.dpos, perm, f, x, omega))
 }
-template void mcsor_colour<double, double>(hipStream_t, const DMat<double>&, int, int, int, int, const int*,
-                                           const double*, double*, double);
-template void mcsor_colour<float, float>(hipStream_t, const DMat<float>&, int, int, int, int, const int*,
-                                         const float*, float*, float);
+template void mcsor_colour<double, double>(hipStream_t, const DMat<double>&, int, int, const int*, const double*,
+                                           double*, double);
+template void mcsor_colour<float, float>(hipStream_t, const DMat<float>&, int, int, const int*, const float*, float*,
+                                         float);
 
 template <typename VT, typename XT>
 void sor_lex(hipStream_t st, const DMat<VT>& A, int nlv, const int* lv_off, const int* lv_rows, const XT* f, XT* x,
@@ -660,25 +593,15 @@ template void sor_lex<float, float>(hipStream_t, const DMat<float>&, int, const 
 
 template <typename VT, typename XT>
 void jacobi(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, const XT* xo, XT* x, int nlim) {
-    if (A.n == 0) return;
-    dim3 t(TPB);
-    if (A.lpr > 1) {
-        dim3 g(A.nvblk);
-        if (kind == JAC_REF) {
-            LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi_vec<VT, XT, JAC_REF, L>), g, t, 0, st, A.n,
-                                                        A.nvblk, A.rp, A.ci, A.v, f, xo, x, nlim))
-        } else {
-            LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi_vec<VT, XT, JAC_L1, L>), g, t, 0, st, A.n,
-                                                        A.nvblk, A.rp, A.ci, A.v, f, xo, x, nlim))
-        }
-        return;
+    if (A.nblk == 0) return;
+    dim3 g(A.nblk), t(TPB);
+    if (kind == JAC_REF) {
+        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_REF, L>), g, t, 0, st, A.nblk, A.blk, A.rp,
+                                                    A.ci, A.v, A.dpos, A.l1, f, xo, x, nlim))
+    } else {
+        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_L1, L>), g, t, 0, st, A.nblk, A.blk, A.rp,
+                                                    A.ci, A.v, A.dpos, A.l1, f, xo, x, nlim))
     }
-    if (kind == JAC_REF)
-        hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_REF>), dim3(A.nblk), t, 0, st, A.nblk, A.blk, A.rp, A.ci, A.v, f, xo,
-                           x, nlim);
-    else
-        hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_L1>), dim3(A.nblk), t, 0, st, A.nblk, A.blk, A.rp, A.ci, A.v, f, xo,
-                           x, nlim);
 }
 template void jacobi<double, double>(hipStream_t, const DMat<double>&, int, const double*, const double*, double*,
                                      int);
@@ -698,6 +621,17 @@ template void coarse_solve<double>(hipStream_t, int, const int*, const double*, 
                                    const double*, double*);
 template void coarse_solve<float>(hipStream_t, int, const int*, const double*, int, const int*, const double*,
                                   const float*, float*);
+
+// piv: one device double of scratch
+void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* piv) {
+    if (N <= 0) return;
+    int g = std::min(N, 4096);
+    int gp = (N + TPB - 1) / TPB;
+    for (int k = 0; k < N; k++) {
+        hipLaunchKernelGGL(k_gj_update, dim3(g), dim3(TPB), 0, st, N, k, M, piv);
+        hipLaunchKernelGGL(k_gj_pivot, dim3(gp), dim3(TPB), 0, st, N, k, M, (const double*)piv);
+    }
+}
 
 }  // namespace dev
 }  // namespace lina

@@ -40,18 +40,18 @@ struct DMat {
     VT* v = nullptr;
     int nblk = 0;
     int* blk = nullptr;  // nblk+1 row starts
-    int lpr = 1;         // lanes per row: 1 = LDS-staged sequential rows, else 8..64
-    int nvblk = 0;       // blocks of the multi-lane path: ceil(n / (2*TPB/lpr))
+    int lpr = 1;         // lanes reducing one row from LDS: 1 (sequential), 4, 8, 16, 32
+    int* dpos = nullptr; // position of each row's diagonal entry in ci/v (-1: none)
+    VT* l1 = nullptr;    // sum_j |a_ij| of each row (l1-Jacobi), summed in column order
 };
 
-constexpr int TPB_ROWS = 256;
 // lanes per row for a matrix with the given mean row length
 inline int lanes_per_row(double mean_len) {
     if (mean_len <= 24) return 1;
-    if (mean_len <= 48) return 8;
-    if (mean_len <= 96) return 16;
-    if (mean_len <= 192) return 32;
-    return 64;
+    if (mean_len <= 48) return 4;
+    if (mean_len <= 96) return 8;
+    if (mean_len <= 192) return 16;
+    return 32;
 }
 
 // returns the number of block partials written to part (if part != nullptr)
@@ -77,8 +77,8 @@ void axpy_s(hipStream_t st, int n, double* x, const double* y, const double* sca
             double sign);
 void axpby(hipStream_t st, int n, double a, double* x, double b, const double* y);
 template <typename VT, typename XT>
-void mcsor_colour(hipStream_t st, const DMat<VT>& Ac, int blk0, int nblk, int r0, int r1, const int* perm,
-                  const XT* f, XT* x, XT omega);
+void mcsor_colour(hipStream_t st, const DMat<VT>& Ac, int blk0, int nblk, const int* perm, const XT* f, XT* x,
+                  XT omega);
 template <typename VT, typename XT>
 void sor_lex(hipStream_t st, const DMat<VT>& A, int nlv, const int* lv_off, const int* lv_rows, const XT* f, XT* x,
              XT* xo, XT omega, int sweeps, int backward);

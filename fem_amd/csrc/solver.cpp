@@ -132,7 +132,7 @@ static Csr drop_outside_zeros(const Csr& A) {
 template <typename VT>
 static void upload_mat(const Csr& A, const std::vector<int>& starts_with_end, dev::DMat<VT>& D, DevBuffers& B,
                        hipStream_t st, const int* shared_rp = nullptr, const int* shared_ci = nullptr,
-                       const int* shared_blk = nullptr) {
+                       const int* shared_blk = nullptr, const int* row_ids = nullptr) {
     validate(A, starts_with_end);
     D.n = A.n;
     D.ncols = A.ncols;
@@ -145,7 +145,20 @@ static void upload_mat(const Csr& A, const std::vector<int>& starts_with_end, de
     if (D.nblk < 0) D.nblk = 0;
     D.blk = shared_blk ? (int*)shared_blk : dupload(starts_with_end, B, st);
     D.lpr = dev::lanes_per_row(A.n > 0 ? (double)A.nnz() / A.n : 0.0);
-    D.nvblk = D.lpr > 1 ? (A.n + 2 * (dev::TPB_ROWS / D.lpr) - 1) / (2 * (dev::TPB_ROWS / D.lpr)) : 0;
+    // diagonal positions (row r's own index: row_ids == nullptr) and l1 norms
+    std::vector<int> dpos(A.n, -1);
+    std::vector<VT> l1(A.n, (VT)0);
+    for (int r = 0; r < A.n; r++) {
+        int i = row_ids ? row_ids[r] : r;
+        VT s = 0;
+        for (int p = A.rp[r]; p < A.rp[r + 1]; p++) {
+            if (A.ci[p] == i) dpos[r] = p;
+            s += std::fabs((VT)A.v[p]);
+        }
+        l1[r] = s;
+    }
+    D.dpos = dupload(dpos, B, st);
+    D.l1 = dupload(l1, B, st);
 }
 
 // upload A as FP64 + FP32 copies sharing the index arrays; returns #blocks
@@ -157,7 +170,7 @@ static int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F,
     if (A.n == 0) starts.assign(1, 0);
     upload_mat(A, starts, D, B, st);
     upload_mat(A, starts, F, B, st, D.rp, D.ci, D.blk);
-    return std::max(D.nblk, D.nvblk);
+    return D.nblk;
 }
 
 // level schedule of the lexicographic sweep: forward, level(i) = 1 + max over
@@ -281,8 +294,8 @@ void Solver::upload() {
         }
         d.colour_blk[nc] = (int)starts.size();
         starts.push_back(d.n);
-        upload_mat(Ac, starts, d.Ac, B, st);
-        upload_mat(Ac, starts, d.Acf, B, st, d.Ac.rp, d.Ac.ci, d.Ac.blk);
+        upload_mat(Ac, starts, d.Ac, B, st, nullptr, nullptr, nullptr, perm.data());
+        upload_mat(Ac, starts, d.Acf, B, st, d.Ac.rp, d.Ac.ci, d.Ac.blk, perm.data());
         d.perm = dupload(perm, B, st);
         d.ncolours = nc;
         // level schedules of the exact lexicographic sweeps
@@ -424,8 +437,7 @@ void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post) {
                 for (int cc = 0; cc < d.ncolours; cc++) {
                     int c = back ? d.ncolours - 1 - cc : cc;
                     int b0 = d.colour_blk[c], b1 = d.colour_blk[c + 1];
-                    dev::mcsor_colour(st, pickAc<T>(d), b0, b1 - b0, d.colour_rows[c], d.colour_rows[c + 1], d.perm,
-                                      f, x, omega);
+                    dev::mcsor_colour(st, pickAc<T>(d), b0, b1 - b0, d.perm, f, x, omega);
                 }
             break;
         }

@@ -89,7 +89,7 @@ int lina_device_count(void);
 void lina_params_reference(lina_amg_params* p, int dof, int depth);
 /* one symmetric V-cycle per Krylov iteration: multicolour SOR (forward
  * before, colours reversed after the coarse correction) on the finest level,
- * two l1-Jacobi sweeps on the coarser levels */
+ * one l1-Jacobi sweep on the coarser levels */
 void lina_params_vcycle(lina_amg_params* p, int dof, int depth);
 
 /* Build the hierarchy for A (host CSR).  upload=0 runs the host setup only

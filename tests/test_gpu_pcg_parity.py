@@ -83,13 +83,13 @@ def test_pcg_vcycle_mc_ssor(gpu, ref, kind, m):
 
 @pytest.mark.parametrize("kind,m", [("poisson", 65), ("elasticity", 33), ("elasticity", 65)])
 def test_pcg_vcycle_default(gpu, ref, kind, m):
-    """The default V-cycle: MC-SSOR on the finest level, two l1-Jacobi sweeps
-    below (vector multi-lane rows on the dense coarse levels)."""
+    """The default V-cycle: MC-SSOR on the finest level, one l1-Jacobi sweep
+    below (multi-lane rows on the dense coarse levels)."""
     s = P.poisson(m) if kind == "poisson" else P.degraded_elasticity(m)
     depth = 5
     say("pcg vcycle default", kind, m)
     r = _ref_solve(ref, s, depth, smoother="hybrid", alpha=1.0, smiter=1, aggr_smiter=1, amg_eps=1e-2,
-                   amg_max_iter=1, zero_init=True, tol=1e-8, max_iter=3000, coarse_sweeps=2)
+                   amg_max_iter=1, zero_init=True, tol=1e-8, max_iter=3000, coarse_sweeps=1)
     for prec in ("fp64", "mixed"):
         amg = gpu.AMG(s.rowptr, s.col, s.val, gpu.params("vcycle", s.dof, depth, precision=prec))
         g = amg.pcg(s.b, tol=1e-8, max_iter=3000)
