@@ -299,3 +299,75 @@ double lina_spmv_bytes(const lina_solver* s, int level, int precision) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------- ILU(0) systems
+
+#include "ilu.hpp"
+
+struct lina_ilu {
+    lina::IluSystem s;
+};
+
+extern "C" {
+
+lina_ilu* lina_ilu_create(int n, const int* rowptr, const int* col, const double* val, int upload) {
+    lina_ilu* h = nullptr;
+    int rc = guard([&]() {
+        Csr A = make_csr(n, rowptr, col, val);
+        h = new lina_ilu();
+        h->s.build(A, false);
+        if (upload) {
+            if (lina_device_count() < 1) throw lina::HipError("no HIP device available");
+            h->s.upload();
+        }
+        return 0;
+    });
+    if (rc < 0) {
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+void lina_ilu_destroy(lina_ilu* h) { delete h; }
+
+int lina_ilu_ok(const lina_ilu* h) { return h->s.ilu_ok() ? 1 : 0; }
+
+int lina_ilu_factor_nnz(const lina_ilu* h, char which) {
+    return which == 'L' ? h->s.L().nnz() : h->s.U().nnz();
+}
+
+int lina_ilu_factor_copy(const lina_ilu* h, char which, int* rowptr, int* col, double* val) {
+    const Csr& M = which == 'L' ? h->s.L() : h->s.U();
+    std::memcpy(rowptr, M.rp.data(), (M.n + 1) * sizeof(int));
+    std::memcpy(col, M.ci.data(), M.nnz() * sizeof(int));
+    std::memcpy(val, M.v.data(), M.nnz() * sizeof(double));
+    return 0;
+}
+
+int lina_ilu_apply(lina_ilu* h, const double* z, double* w) {
+    return guard([&]() {
+        h->s.apply_host(z, w);
+        return 0;
+    });
+}
+
+int lina_ilu_pcg(lina_ilu* h, double* x, double* b, double tol, int max_iter, int stop, double* residuum) {
+    return guard([&]() { return h->s.pcg_host(x, b, tol, max_iter, stop, residuum); });
+}
+
+int lina_ilu_gmres(lina_ilu* h, double* x, const double* b, int m, double tol, int max_iter, double* eta) {
+    return guard([&]() {
+        if (m < 1) throw std::invalid_argument("restart length m must be >= 1");
+        return h->s.gmres_host(x, b, m, tol, max_iter, eta);
+    });
+}
+
+int lina_bicgstab(lina_solver* s, double* x, const double* b, double eps, int max_iter, double* res) {
+    return guard([&]() {
+        if (!s->s.uploaded()) throw std::invalid_argument("solver was created without device upload");
+        return s->s.bicgstab_host(x, b, eps, max_iter, res);
+    });
+}
+
+}  // extern "C"

@@ -401,6 +401,72 @@ __global__ __launch_bounds__(TPB) void k_jacobi(int nblk, const int* __restrict_
     }
 }
 
+// ------------------------------------------ ILU(0) triangular solves
+//
+// PCG_preconl / PCG_preconr (LinA/PCGS.c:61-100) by level scheduling inside
+// one workgroup.  Forward: L rows hold the diagonal last,
+//   w_i = (z_i - sum_{j<last} l_ij w_j) / l_ii;
+// backward: U rows hold the (unit) diagonal first,
+//   w_i = (y_i - sum_{j>first} u_ij w_j) / u_ii.
+// Rows of one level depend only on rows of earlier levels.
+__global__ __launch_bounds__(1024) void k_trsv(const int* __restrict__ rp, const int* __restrict__ ci,
+                                               const double* __restrict__ v, int nlv, const int* __restrict__ lv_off,
+                                               const int* __restrict__ lv_rows, const double* __restrict__ z,
+                                               double* __restrict__ w, int upper) {
+    for (int L = 0; L < nlv; L++) {
+        const int e = lv_off[L + 1];
+        for (int t = lv_off[L] + threadIdx.x; t < e; t += blockDim.x) {
+            const int i = lv_rows[t];
+            const int b = rp[i], en = rp[i + 1];
+            double sum = z[i];
+            if (!upper) {
+                for (int k = b; k < en - 1; k++) sum -= v[k] * w[ci[k]];
+                sum /= v[en - 1];
+            } else {
+                const double d = v[b];
+                for (int k = b + 1; k < en; k++) sum -= v[k] * w[ci[k]];
+                sum /= d;
+            }
+            w[i] = sum;
+        }
+        __syncthreads();
+    }
+}
+
+// h_l = V_l . r for l < m (classical Gram-Schmidt dots), block partials
+// part[l * nblk + block]
+__global__ __launch_bounds__(TPB) void k_multidot(int n, int m, const double* __restrict__ V, size_t ld,
+                                                  const double* __restrict__ r, double* __restrict__ part) {
+    for (int l = 0; l < m; l++) {
+        const double* vl = V + (size_t)l * ld;
+        double s = 0.0;
+        for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) s += vl[i] * r[i];
+        double t = block_sum(s);
+        if (threadIdx.x == 0) part[(size_t)l * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+// out[l] = sum of the nblk partials of dot l (fixed order)
+__global__ __launch_bounds__(TPB) void k_multidot_final(int m, int nblk, const double* __restrict__ part,
+                                                        double* __restrict__ out) {
+    for (int l = 0; l < m; l++) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < nblk; i += TPB) s += part[(size_t)l * nblk + i];
+        double t = block_sum(s);
+        if (threadIdx.x == 0) out[l] = t;
+    }
+}
+
+// y[j] = base[j] + sign * sum_{l<m} V_l[j] * c[l]   (c on the device)
+__global__ void k_combine(int n, int m, const double* __restrict__ V, size_t ld, const double* __restrict__ c,
+                          const double* __restrict__ base, double sign, double* __restrict__ y) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int l = 0; l < m; l++) s += V[(size_t)l * ld + j] * c[l];
+        y[j] = base[j] + sign * s;
+    }
+}
+
 // ------------------------------------------------------ coarse solve
 
 // x = inv * b over the dense rows (gathered through idx), trivial rows
@@ -621,6 +687,23 @@ template void coarse_solve<double>(hipStream_t, int, const int*, const double*, 
                                    const double*, double*);
 template void coarse_solve<float>(hipStream_t, int, const int*, const double*, int, const int*, const double*,
                                   const float*, float*);
+
+void trsv(hipStream_t st, const int* rp, const int* ci, const double* v, int nlv, const int* lv_off,
+          const int* lv_rows, const double* z, double* w, int upper) {
+    hipLaunchKernelGGL(k_trsv, dim3(1), dim3(1024), 0, st, rp, ci, v, nlv, lv_off, lv_rows, z, w, upper);
+}
+
+int multidot(hipStream_t st, int n, int m, const double* V, size_t ld, const double* r, double* part, double* out) {
+    int g = grid_for(n, 512);
+    hipLaunchKernelGGL(k_multidot, dim3(g), dim3(TPB), 0, st, n, m, V, ld, r, part);
+    hipLaunchKernelGGL(k_multidot_final, dim3(1), dim3(TPB), 0, st, m, g, part, out);
+    return g;
+}
+
+void combine(hipStream_t st, int n, int m, const double* V, size_t ld, const double* c, const double* base,
+             double sign, double* y) {
+    hipLaunchKernelGGL(k_combine, dim3(grid_for(n)), dim3(TPB), 0, st, n, m, V, ld, c, base, sign, y);
+}
 
 // piv: one device double of scratch
 void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* piv) {

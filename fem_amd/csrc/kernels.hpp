@@ -85,6 +85,14 @@ void sor_lex(hipStream_t st, const DMat<VT>& A, int nlv, const int* lv_off, cons
 template <typename VT, typename XT>
 void jacobi(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, const XT* xo, XT* x, int nlim);
 void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* piv);
+// level-scheduled ILU(0) triangular solve (upper=0: L, diagonal last; 1: U, diagonal first)
+void trsv(hipStream_t st, const int* rp, const int* ci, const double* v, int nlv, const int* lv_off,
+          const int* lv_rows, const double* z, double* w, int upper);
+// out[l] = V_l . r, l < m (V column vectors with leading dimension ld); returns #partials per dot
+int multidot(hipStream_t st, int n, int m, const double* V, size_t ld, const double* r, double* part, double* out);
+// y = base + sign * sum_l V_l c_l
+void combine(hipStream_t st, int n, int m, const double* V, size_t ld, const double* c, const double* base,
+             double sign, double* y);
 template <typename XT>
 void coarse_solve(hipStream_t st, int nd, const int* didx, const double* inv, int nt, const int* tidx,
                   const double* tdiag, const XT* b, XT* x);

@@ -1,0 +1,415 @@
+// Krylov drivers beside AMG-PCG: ILU(0)-preconditioned pcg and GMRES(m),
+// and BiCGStab with the AMG preconditioner.  Vectors stay on the device;
+// the host reads the scalars that steer the iteration.
+//
+//   pcg with PCG_preconlr      PCGS.c:61-108, 163-233
+//   GMRES(m), right ILU(0)     the restarted GMRES that FEM.c drives
+//                              (FEM.c:730-745, CheckOnRestart): classical
+//                              Gram-Schmidt, Givens QR of the Hessenberg
+//                              matrix, convergence tested at restarts; own
+//                              implementation after Saad, Iterative Methods
+//                              for Sparse Linear Systems, Alg. 6.9 / 9.5
+//   BiCGStab_solve             BiCGStab.c:115-198
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "ilu.hpp"
+#include "kernels.hpp"
+#include "solver.hpp"
+
+namespace lina {
+
+template <typename T>
+static T* kalloc(size_t n, DevBuffers& owner) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+    owner.ptrs.push_back(p);
+    return (T*)p;
+}
+
+template <typename T>
+static T* kupload(const std::vector<T>& h, DevBuffers& owner, hipStream_t st) {
+    T* d = kalloc<T>(h.size(), owner);
+    if (!h.empty()) hip_check(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st), "upload");
+    return d;
+}
+
+// dependency levels of a triangular sweep over nonzero off-diagonal values
+static void tri_levels(const Csr& T, bool upper, std::vector<int>& off, std::vector<int>& rows) {
+    const int n = T.n;
+    std::vector<int> lev(n, 0);
+    int nl = 0;
+    for (int t = 0; t < n; t++) {
+        int i = upper ? n - 1 - t : t;
+        int l = 0;
+        for (int p = T.rp[i]; p < T.rp[i + 1]; p++) {
+            int j = T.ci[p];
+            if (T.v[p] == 0.0 || j == i) continue;
+            if ((!upper && j < i) || (upper && j > i)) l = std::max(l, lev[j] + 1);
+        }
+        lev[i] = l;
+        nl = std::max(nl, l + 1);
+    }
+    off.assign(nl + 1, 0);
+    for (int i = 0; i < n; i++) off[lev[i] + 1]++;
+    for (int l = 0; l < nl; l++) off[l + 1] += off[l];
+    rows.assign(n, 0);
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int i = 0; i < n; i++) rows[fill[lev[i]]++] = i;
+}
+
+static dev::DMat<double> upload_csr(const Csr& A, DevBuffers& B, hipStream_t st) {
+    dev::DMat<double> D;
+    D.n = A.n;
+    D.ncols = A.ncols;
+    D.nnz = A.nnz();
+    D.rp = kupload(A.rp, B, st);
+    D.ci = kupload(A.ci, B, st);
+    D.v = kupload(A.v, B, st);
+    // LDS row blocks for the SpMV (same builder rule as the AMG levels)
+    std::vector<int> starts;
+    int nnz = 0, rows = 0;
+    if (A.n > 0) starts.push_back(0);
+    for (int r = 0; r < A.n; r++) {
+        int len = A.rp[r + 1] - A.rp[r];
+        if (rows > 0 && (nnz + len > dev::BLOCK_NNZ || rows == dev::BLOCK_ROWS)) {
+            starts.push_back(r);
+            nnz = 0;
+            rows = 0;
+        }
+        nnz += len;
+        rows++;
+    }
+    starts.push_back(A.n);
+    D.nblk = (int)starts.size() - 1;
+    D.blk = kupload(starts, B, st);
+    D.lpr = dev::lanes_per_row(A.n > 0 ? (double)A.nnz() / A.n : 0.0);
+    return D;
+}
+
+// ------------------------------------------------------------- IluSystem
+
+IluSystem::IluSystem() {}
+
+IluSystem::~IluSystem() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (host_) (void)hipHostFree(host_);
+    bufs_.reset();
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void IluSystem::build(const Csr& A, bool upload) {
+    A_ = A;
+    Csr L, U;
+    ilu_ok_ = ilu0(A, L, U);
+    L_ = std::move(L);
+    U_ = std::move(U);
+    n_ = A.n;
+    if (upload) this->upload();
+}
+
+void IluSystem::upload() {
+    if (!stream_) {
+        hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipHostMalloc((void**)&host_, 64 * sizeof(double)), "hipHostMalloc");
+    }
+    bufs_.reset(new DevBuffers());
+    DevBuffers& B = *bufs_;
+    hipStream_t st = stream_;
+    dA_ = upload_csr(A_, B, st);
+    dL_ = upload_csr(L_, B, st);
+    dU_ = upload_csr(U_, B, st);
+    std::vector<int> off, rows;
+    tri_levels(L_, false, off, rows);
+    nlv_l_ = (int)off.size() - 1;
+    lvl_off_ = kupload(off, B, st);
+    lvl_rows_ = kupload(rows, B, st);
+    tri_levels(U_, true, off, rows);
+    nlv_u_ = (int)off.size() - 1;
+    lvu_off_ = kupload(off, B, st);
+    lvu_rows_ = kupload(rows, B, st);
+    const int n = n_;
+    for (double** p : {&vX_, &vB_, &vR_, &vP_, &vQ_, &vZ_, &vW_, &vY_, &vT_}) *p = kalloc<double>(n, B);
+    part_ = kalloc<double>(std::max(4096, dA_.nblk + 16) * 40, B);
+    scal_ = kalloc<double>(64, B);
+    dev::fill(st, 64, scal_, 0.0);
+    hip_check(hipStreamSynchronize(st), "upload sync");
+}
+
+void IluSystem::apply(const double* z, double* w) {
+    // PCG_preconlr: y = L^-1 z, w = U^-1 y
+    dev::trsv(stream_, dL_.rp, dL_.ci, dL_.v, nlv_l_, lvl_off_, lvl_rows_, z, vT_, 0);
+    dev::trsv(stream_, dU_.rp, dU_.ci, dU_.v, nlv_u_, lvu_off_, lvu_rows_, vT_, w, 1);
+}
+
+double IluSystem::dot(const double* a, const double* b) {
+    int np = dev::dot_part<double>(stream_, n_, a, b, part_);
+    dev::finalize(stream_, np, part_, scal_, 0, 0);
+    hip_check(hipMemcpyAsync(host_, scal_, sizeof(double), hipMemcpyDeviceToHost, stream_), "d2h");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    return host_[0];
+}
+
+void IluSystem::matvec(const double* x, double* y) {
+    dev::spmv<double, double>(stream_, dA_, x, nullptr, y, dev::MODE_SET, nullptr, nullptr);
+}
+
+// pcg (PCGS.c:163-233) with precon = PCG_preconlr and mat_mult = PCG_mult
+int IluSystem::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, double* residuum) {
+    if (!ilu_ok_) throw NumericError("ILU(0): zero pivot");
+    hipStream_t st = stream_;
+    const int n = n_;
+    double h = std::sqrt(dot(B, B));
+    if (h == 0) h = 1;
+    const double inv = (double)1 / h;
+    dev::axpby(st, n, inv, B, 0.0, B);
+    dev::axpby(st, n, inv, X, 0.0, X);
+    double *R = vR_, *P = vP_, *Q = vQ_, *Z = vZ_, *W = vW_;
+    dev::spmv<double, double>(st, dA_, X, B, R, dev::MODE_RES, nullptr, nullptr);
+    apply(R, W);
+    double r0 = std::sqrt(dot(W, W));
+    if (r0 == 0) return 0;
+    double r = r0, s = 0, s_old = 0;
+    int i = 0;
+    do {
+        apply(R, Z);
+        s_old = s;
+        s = dot(R, Z);
+        if (i > 0) {
+            double b = s / s_old;
+            dev::axpby(st, n, b, P, 1.0, Z);  // P*b + 1*Z, rounded as scalar_mult then vector_add
+        } else {
+            dev::copy(st, n, (const double*)Z, P);
+        }
+        matvec(P, Q);
+        double a = s / dot(P, Q);
+        dev::axpby(st, n, 1.0, X, a, P);
+        dev::axpby(st, n, 1.0, R, -a, Q);
+        r = std::sqrt(dot(R, R));
+        i++;
+    } while ((stop == LINA_STOP_RELRES ? r : r / r0) > tol && i < max_iter);
+    apply(R, W);
+    double rw = std::sqrt(dot(W, W));
+    dev::axpby(st, n, h, B, 0.0, B);
+    dev::axpby(st, n, h, X, 0.0, X);
+    hip_check(hipStreamSynchronize(st), "sync");
+    if (residuum) *residuum = rw / r0;
+    if (std::isnan(rw / r0)) throw NumericError("pcg: NaN residual");
+    return i;
+}
+
+// Givens QR of the (i+1) x i Hessenberg matrix, built column by column
+struct Givens {
+    std::vector<double> c, s;
+    void reset() {
+        c.clear();
+        s.clear();
+    }
+    // rotate a new column h[0..i] (in place) and append its rotation
+    void push(std::vector<double>& h, int i) {
+        for (int k = 0; k < i; k++) {
+            double t = c[k] * h[k] + s[k] * h[k + 1];
+            h[k + 1] = -s[k] * h[k] + c[k] * h[k + 1];
+            h[k] = t;
+        }
+        double a = h[i], b = h[i + 1];
+        double r = std::hypot(a, b);
+        double cc = r == 0 ? 1.0 : a / r, ss = r == 0 ? 0.0 : b / r;
+        c.push_back(cc);
+        s.push_back(ss);
+        h[i] = r;
+        h[i + 1] = 0.0;
+    }
+};
+
+// restarted GMRES(m) on A M^-1 y = b (M = ILU(0)), the caller's x = M^-1 y:
+// at each restart r = b - A M^-1 y, eta = |r|/|r_first|; a full cycle of m
+// Arnoldi steps (classical Gram-Schmidt) follows when eta > tol; the
+// least-squares update is formed at the cycle's end.  Returns the number of
+// Arnoldi steps; y is updated in place.
+int IluSystem::gmres_dev(double* Y, const double* Bv, int m, double tol, int max_iter, double* eta_out) {
+    if (!ilu_ok_) throw NumericError("ILU(0): zero pivot");
+    hipStream_t st = stream_;
+    const int n = n_;
+    if (mb_ < m + 2) {
+        // the Krylov basis as one (m+2) x n block: V_l = V + l*n
+        vbasis_ = kalloc<double>((size_t)(m + 2) * n, *bufs_);
+        mb_ = m + 2;
+    }
+    double* V = vbasis_;
+    const size_t ld = (size_t)n;
+    double* coef = scal_ + 8;  // device copy of the combination coefficients
+    double beta0 = 0, eta = 0;
+    int iter = 0;
+    std::vector<double> H((size_t)(m + 1) * m, 0.0), col(m + 2), g(m + 2), z(m + 1);
+    Givens rot;
+    for (;;) {
+        // restart: r = b - A M^-1 y, eta = |r| / |r of the first restart|
+        double* v0 = V;
+        apply(Y, v0);
+        matvec(v0, vW_);
+        dev::axpby(st, n, 0.0, vR_, 1.0, Bv);
+        dev::axpby(st, n, 1.0, vR_, -1.0, vW_);
+        double beta = std::sqrt(dot(vR_, vR_));
+        if (iter == 0) {
+            beta0 = beta;
+            eta = beta0 == 0.0 ? 0.0 : 1.0;
+        } else {
+            eta = beta / beta0;
+        }
+        if (eta > tol) {
+            dev::axpby(st, n, 0.0, v0, 1.0 / beta, vR_);
+            rot.reset();
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = beta;
+            for (int i = 1; i <= m; i++) {
+                double* vi = V + (size_t)(i - 1) * ld;
+                double* vn = V + (size_t)i * ld;
+                apply(vi, vn);
+                matvec(vn, vR_);
+                // classical Gram-Schmidt: all dots against the same r
+                dev::multidot(st, n, i, V, ld, vR_, part_, coef);
+                hip_check(hipMemcpyAsync(col.data(), coef, i * sizeof(double), hipMemcpyDeviceToHost, st), "d2h");
+                hip_check(hipStreamSynchronize(st), "sync");
+                dev::combine(st, n, i, V, ld, coef, vR_, -1.0, vn);
+                double nv = std::sqrt(dot(vn, vn));
+                dev::axpby(st, n, 0.0, vn, 1.0 / nv, vn);
+                col[i] = nv;
+                rot.push(col, i - 1);
+                for (int k = 0; k <= i - 1; k++) H[(size_t)(i - 1) * (m + 1) + k] = col[k];
+                double cc = rot.c[i - 1], ss = rot.s[i - 1];
+                double t = cc * g[i - 1] + ss * g[i];
+                g[i] = -ss * g[i - 1] + cc * g[i];
+                g[i - 1] = t;
+                iter++;
+            }
+            // least-squares update at the end of the cycle: R z = g, y += V z
+            for (int r = m - 1; r >= 0; r--) {
+                double s = g[r];
+                for (int c = r + 1; c < m; c++) s -= H[(size_t)c * (m + 1) + r] * z[c];
+                z[r] = s / H[(size_t)r * (m + 1) + r];
+            }
+            hip_check(hipMemcpyAsync(coef, z.data(), m * sizeof(double), hipMemcpyHostToDevice, st), "h2d");
+            dev::combine(st, n, m, V, ld, coef, Y, 1.0, vY_);
+            dev::copy(st, n, (const double*)vY_, Y);
+        }
+        if (!(eta > tol && iter < max_iter)) break;
+    }
+    hip_check(hipStreamSynchronize(st), "sync");
+    if (eta_out) *eta_out = eta;
+    return iter;
+}
+
+// FEM.c's solver(): H = x; gmres(H); x = precon(H)
+int IluSystem::gmres_host(double* x, const double* b, int m, double tol, int max_iter, double* eta) {
+    const size_t bytes = (size_t)n_ * sizeof(double);
+    hip_check(hipMemcpyAsync(vX_, x, bytes, hipMemcpyHostToDevice, stream_), "h2d");
+    hip_check(hipMemcpyAsync(vB_, b, bytes, hipMemcpyHostToDevice, stream_), "h2d");
+    int it = gmres_dev(vX_, vB_, m, tol, max_iter, eta);
+    apply(vX_, vZ_);
+    hip_check(hipMemcpyAsync(x, vZ_, bytes, hipMemcpyDeviceToHost, stream_), "d2h");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    return it;
+}
+
+int IluSystem::pcg_host(double* x, double* b, double tol, int max_iter, int stop, double* res) {
+    const size_t bytes = (size_t)n_ * sizeof(double);
+    hip_check(hipMemcpyAsync(vX_, x, bytes, hipMemcpyHostToDevice, stream_), "h2d");
+    hip_check(hipMemcpyAsync(vB_, b, bytes, hipMemcpyHostToDevice, stream_), "h2d");
+    int it = pcg_dev(vX_, vB_, tol, max_iter, stop, res);
+    hip_check(hipMemcpyAsync(x, vX_, bytes, hipMemcpyDeviceToHost, stream_), "d2h");
+    hip_check(hipMemcpyAsync(b, vB_, bytes, hipMemcpyDeviceToHost, stream_), "d2h");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    return it;
+}
+
+void IluSystem::apply_host(const double* z, double* w) {
+    const size_t bytes = (size_t)n_ * sizeof(double);
+    hip_check(hipMemcpyAsync(vZ_, z, bytes, hipMemcpyHostToDevice, stream_), "h2d");
+    apply(vZ_, vW_);
+    hip_check(hipMemcpyAsync(w, vW_, bytes, hipMemcpyDeviceToHost, stream_), "d2h");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+}
+
+// -------------------------------------------------- BiCGStab (AMG precon)
+
+double Solver::dot_host(const double* a, const double* b) {
+    int np = dev::dot_part<double>(stream_, n_, a, b, part_);
+    dev::finalize(stream_, np, part_, scal_, dev::S_T2, 0);
+    return read_scalar(dev::S_T2);
+}
+
+// y = A x with the caller's (unflipped) finest matrix
+void Solver::apply_A(const double* x, double* y) {
+    DevLevel& top = L_.back();
+    dev::spmv<double, double>(stream_, top.A, x, nullptr, y, dev::MODE_SET, nullptr, nullptr);
+    if (has_neg_signs_) dev::signs(stream_, n_, signs_, y, y);
+}
+
+// BiCGStab_solve (BiCGStab.c:115-198); preconditioner = AMG_solve with this
+// hierarchy, warm-started from the previous PP / PS as in the reference
+int Solver::bicgstab_dev(double* X, const double* b, double eps, int max_iter, double* res) {
+    hipStream_t st = stream_;
+    const int n = n_;
+    if (!bS_)
+        for (double** p : {&bS_, &bPS_, &bPP_, &bAPP_, &bAPS_, &bR_, &bR0_, &bP_}) *p = dalloc_public(n);
+    for (double* p : {bS_, bPS_, bPP_, bAPP_, bAPS_}) dev::fill<double>(st, n, p, 0.0);
+    vcycles_ = 0;
+    precon_calls_ = 0;
+    // R = -(A X - b)
+    apply_A(X, bR_);
+    dev::axpby(st, n, 1.0, bR_, -1.0, b);
+    dev::axpby(st, n, -1.0, bR_, 0.0, bR_);
+    dev::copy(st, n, (const double*)bR_, bR0_);
+    dev::copy(st, n, (const double*)bR_, bP_);
+    int iter = 0;
+    double r0 = std::sqrt(dot_host(bR_, bR_)), r = r0;
+    if (r0 != 0) {
+        do {
+            amg_solve_dev(bP_, bPP_);
+            apply_A(bPP_, bAPP_);
+            double h = dot_host(bR_, bR0_);
+            double t = dot_host(bAPP_, bR0_);
+            if (t == 0) break;
+            double alpha = h / t;
+            dev::copy(st, n, (const double*)bR_, bS_);
+            dev::axpby(st, n, 1.0, bS_, -alpha, bAPP_);
+            amg_solve_dev(bS_, bPS_);
+            apply_A(bPS_, bAPS_);
+            double s = dot_host(bAPS_, bAPS_);
+            if (s == 0) break;
+            double omega = dot_host(bAPS_, bS_) / s;
+            dev::axpby(st, n, 1.0, X, alpha, bPP_);
+            dev::axpby(st, n, 1.0, X, omega, bPS_);
+            dev::copy(st, n, (const double*)bS_, bR_);
+            dev::axpby(st, n, 1.0, bR_, -omega, bAPS_);
+            double beta = dot_host(bR_, bR0_) * alpha / (h * omega);
+            dev::axpby(st, n, 1.0, bP_, -omega, bAPP_);
+            dev::axpby(st, n, beta, bP_, 0.0, bP_);
+            dev::axpby(st, n, 1.0, bP_, 1.0, bR_);
+            r = std::sqrt(dot_host(bR_, bR_));
+            iter++;
+        } while (r / r0 > eps && iter <= max_iter);
+    }
+    if (res) *res = r0 == 0 ? 0.0 : std::fabs(r / r0);
+    return iter;
+}
+
+int Solver::bicgstab_host(double* x, const double* b, double eps, int max_iter, double* res) {
+    const size_t bytes = (size_t)n_ * sizeof(double);
+    hip_check(hipMemcpyAsync(kX_, x, bytes, hipMemcpyHostToDevice, stream_), "h2d");
+    hip_check(hipMemcpyAsync(kB_, b, bytes, hipMemcpyHostToDevice, stream_), "h2d");
+    int it = bicgstab_dev(kX_, kB_, eps, max_iter, res);
+    hip_check(hipMemcpyAsync(x, kX_, bytes, hipMemcpyDeviceToHost, stream_), "d2h");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    return it;
+}
+
+}  // namespace lina

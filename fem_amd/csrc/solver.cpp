@@ -358,6 +358,8 @@ void Solver::recompute(const std::vector<double>& val) {
     upload();
 }
 
+double* Solver::dalloc_public(size_t n) { return dalloc<double>(n, *bufs_); }
+
 double Solver::read_scalar(int slot) {
     hip_check(hipMemcpyAsync(host_scal_ + slot, scal_ + slot, sizeof(double), hipMemcpyDeviceToHost, stream_), "d2h");
     hip_check(hipStreamSynchronize(stream_), "sync");
@@ -565,8 +567,15 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
     double* W = kW_;
     dev::fill<double>(st, n, W, 0.0);
     dev::fill<double>(st, n, Z, 0.0);
-    // R = B + (-1)*(A X)
-    dev::spmv<double, double>(st, top.A, X, B, R, dev::MODE_RES, nullptr, nullptr);
+    // R = B + (-1)*(A X) with the caller's matrix (rows of the stored finest
+    // matrix carry the make_positive_diag signs)
+    if (has_neg_signs_) {
+        apply_A(X, Q);
+        dev::copy(st, n, (const double*)B, R);
+        dev::axpby(st, n, 1.0, R, -1.0, Q);
+    } else {
+        dev::spmv<double, double>(st, top.A, X, B, R, dev::MODE_RES, nullptr, nullptr);
+    }
     precon(R, W);
     np = dev::dot_part<double>(st, n, W, W, part_);
     dev::finalize(st, np, part_, scal_, dev::S_R0, 1);
@@ -586,7 +595,12 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
         np = dev::dot_part<double>(st, n, R, Z, part_);
         dev::finalize(st, np, part_, scal_, dev::S_S, 2);  // s_old = s; s = R.Z
         dev::pcg_p(st, n, P, Z, scal_, i == 0);
-        np = dev::spmv<double, double>(st, top.A, P, nullptr, Q, dev::MODE_SET, part_, P);  // Q = A P, P.Q
+        if (has_neg_signs_) {
+            apply_A(P, Q);
+            np = dev::dot_part<double>(st, n, P, Q, part_);
+        } else {
+            np = dev::spmv<double, double>(st, top.A, P, nullptr, Q, dev::MODE_SET, part_, P);  // Q = A P, P.Q
+        }
         dev::finalize(st, np, part_, scal_, dev::S_PQ, 0);
         np = dev::pcg_xr(st, n, X, R, P, Q, scal_, part_);
         dev::finalize(st, np, part_, scal_, dev::S_R, 1);

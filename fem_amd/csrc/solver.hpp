@@ -84,6 +84,12 @@ class Solver {
     double* part() { return part_; }
     double* scal() { return scal_; }
     DevLevel& finest() { return L_.back(); }
+    double* dalloc_public(size_t n);
+    double dot_host(const double* a, const double* b);
+    void apply_A(const double* x, double* y);
+    int bicgstab_dev(double* X, const double* b, double eps, int max_iter, double* res);
+    int bicgstab_host(double* x, const double* b, double eps, int max_iter, double* res);
+    int last_vcycles() const { return vcycles_; }
 
   private:
     lina_amg_params prm_{};
@@ -111,6 +117,8 @@ class Solver {
     double* scal_ = nullptr;
     int vcycles_ = 0, precon_calls_ = 0;
     double pcg_residuum_ = 0;
+    double *bS_ = nullptr, *bPS_ = nullptr, *bPP_ = nullptr, *bAPP_ = nullptr, *bAPS_ = nullptr, *bR_ = nullptr,
+           *bR0_ = nullptr, *bP_ = nullptr;
 };
 
 }  // namespace lina

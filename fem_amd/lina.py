@@ -255,3 +255,85 @@ class DeviceVector:
 
     def __del__(self):
         self.free()
+
+
+def _bind_krylov(L):
+    if getattr(L, "_krylov_bound", False):
+        return
+    L.lina_ilu_create.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int]
+    L.lina_ilu_create.restype = C.c_void_p
+    L.lina_ilu_destroy.argtypes = [C.c_void_p]
+    L.lina_ilu_ok.argtypes = [C.c_void_p]
+    L.lina_ilu_factor_nnz.argtypes = [C.c_void_p, C.c_char]
+    L.lina_ilu_factor_copy.argtypes = [C.c_void_p, C.c_char, _ip, _ip, _dp]
+    L.lina_ilu_apply.argtypes = [C.c_void_p, _dp, _dp]
+    L.lina_ilu_pcg.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, C.c_int, _dp]
+    L.lina_ilu_gmres.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_int, _dp]
+    L.lina_bicgstab.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, _dp]
+    L._krylov_bound = True
+
+
+class ILU:
+    """ILU(0) factors of A with device triangular solves (lina_ilu handle)."""
+
+    def __init__(self, rowptr, col, val, upload: bool = True):
+        L = lib()
+        _bind_krylov(L)
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.n = len(self.rowptr) - 1
+        h = L.lina_ilu_create(self.n, _i(self.rowptr), _i(self.col), _d(self.val), 1 if upload else 0)
+        if not h:
+            raise LinaError(f"lina_ilu_create failed: {L.lina_last_error().decode()}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lina_ilu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def ok(self) -> bool:
+        return bool(lib().lina_ilu_ok(self.h))
+
+    def factor(self, which: str):
+        nnz = lib().lina_ilu_factor_nnz(self.h, which.encode())
+        rp = np.zeros(self.n + 1, np.int32)
+        ci = np.zeros(max(nnz, 1), np.int32)
+        v = np.zeros(max(nnz, 1))
+        lib().lina_ilu_factor_copy(self.h, which.encode(), _i(rp), _i(ci), _d(v))
+        return rp, ci[:nnz], v[:nnz]
+
+    def apply(self, z):
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        w = np.zeros_like(z)
+        _check(lib().lina_ilu_apply(self.h, _d(z), _d(w)))
+        return w
+
+    def pcg(self, b, x0=None, tol=1e-10, max_iter=1000, stop="reference"):
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        B = np.array(b, dtype=np.float64)
+        res = C.c_double()
+        it = _check(lib().lina_ilu_pcg(self.h, _d(x), _d(B), tol, max_iter, STOP[stop], C.byref(res)))
+        return {"iters": it, "x": x, "b_out": B, "res": res.value}
+
+    def gmres(self, b, x0=None, m=30, tol=1e-8, max_iter=1000):
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        eta = C.c_double()
+        it = _check(lib().lina_ilu_gmres(self.h, _d(x), _d(b), m, tol, max_iter, C.byref(eta)))
+        return {"iters": it, "x": x, "eta": eta.value}
+
+
+def bicgstab(amg: "AMG", b, x0=None, eps=1e-8, max_iter=1000):
+    L = lib()
+    _bind_krylov(L)
+    x = np.zeros(amg.n) if x0 is None else np.array(x0, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    res = C.c_double()
+    it = _check(L.lina_bicgstab(amg.h, _d(x), _d(b), eps, max_iter, C.byref(res)))
+    return {"iters": it, "x": x, "res": res.value}

@@ -124,6 +124,28 @@ int lina_pcg_device(lina_solver* s, double* d_x, double* d_b, double tol, int ma
  * returns the number of V-cycles. */
 int lina_amg_solve(lina_solver* s, const double* f, double* x);
 
+/* BiCGStab_solve (BiCGStab.c:115-198) preconditioned by AMG_solve with this
+ * hierarchy (warm-started PP/PS as in the reference); stops at
+ * |R|/|R0| <= eps or after max_iter+1 iterations.  Host pointers. */
+int lina_bicgstab(lina_solver* s, double* x, const double* b, double eps, int max_iter, double* res);
+
+/* ILU(0) systems: incomplete_LU_factorization (linear_algebra.c:1952-1982)
+ * on the host, level-scheduled triangular solves on the device. */
+typedef struct lina_ilu lina_ilu;
+lina_ilu* lina_ilu_create(int n, const int* rowptr, const int* col, const double* val, int upload);
+void lina_ilu_destroy(lina_ilu* h);
+int lina_ilu_ok(const lina_ilu* h); /* 0: zero pivot, as the reference's return flag */
+int lina_ilu_factor_nnz(const lina_ilu* h, char which);
+int lina_ilu_factor_copy(const lina_ilu* h, char which, int* rowptr, int* col, double* val);
+/* w = U^-1 L^-1 z (PCG_preconlr, PCGS.c:61-108) */
+int lina_ilu_apply(lina_ilu* h, const double* z, double* w);
+/* pcg with PCG_default's ILU(0) preconditioner (PCGS.c:29-34, 163-233) */
+int lina_ilu_pcg(lina_ilu* h, double* x, double* b, double tol, int max_iter, int stop, double* residuum);
+/* restarted GMRES(m) with right ILU(0) preconditioning and the restart-only
+ * convergence test FEM.c uses (FEM.c:730-745); x in/out in the unknowns'
+ * space (x = M^-1 y); returns Arnoldi steps, *eta = final |r|/|r0| */
+int lina_ilu_gmres(lina_ilu* h, double* x, const double* b, int m, double tol, int max_iter, double* eta);
+
 /* device vectors owned by the library (FP64): the solve entry points above
  * take pointers from here when inputs should stay resident in HBM.
  * kind: 0 host->device, 1 device->host, 2 device->device */

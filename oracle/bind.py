@@ -103,6 +103,7 @@ class Ref:
         L.refh_reflective_newton.argtypes = [_vp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
                                              C.c_double, C.c_double, _dp]
         L.refh_set_hybrid.argtypes = [C.c_int]
+        L.refh_gmres_ilu.argtypes = [_vp, _dp, _dp, C.c_int, C.c_double, C.c_int, _dp]
         L.refh_init(1)
 
     # -- matrices
@@ -201,6 +202,14 @@ class Ref:
         self.free(Lp)
         self.free(Up)
         return ok, Lc, Uc
+
+    def gmres_ilu(self, M, b, x0=None, m=30, tol=1e-8, max_iter=1000):
+        n = self.L.refh_size(M)
+        X = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+        B = np.array(b, dtype=np.float64)
+        eta = C.c_double()
+        it = self.L.refh_gmres_ilu(M, dptr(X), dptr(B), m, tol, max_iter, C.byref(eta))
+        return {"iters": it, "x": X, "eta": eta.value}
 
     def product(self, A, B):
         P = self.L.refh_product(A, B)

@@ -524,3 +524,43 @@ int refh_reflective_newton(sparse_matrix* H, double* b, double* lower, double* u
     free(idx);
     return it;
 }
+
+/* ------------------------------------------------------------ GMRES(m) */
+#include "itlin.h"
+extern void gmres(int n, double* y, MATVEC* matvec, PRECON* preconr, PRECON* preconl, double* b,
+                  struct ITLIN_OPT* opt, struct ITLIN_INFO* info);
+extern void PCG_mult(int n, double* y, double* z);
+extern int PCG_preconlr(int n, double* z, double* w);
+static void h_gm_matvec(int n, double* y, double* z) { PCG_mult(n, y, z); }
+static void h_gm_precon(int n, double* z, double* w) { PCG_preconlr(n, z, w); }
+
+/* FEM.c's GMRES solver path (FEM.c:591-611, 730-745): ILU(0) factors of A,
+ * gmres(n, H, matvec, preconlr, NULL, F) with CheckOnRestart on H = x, then
+ * x = preconlr(H).  Returns the iteration count reported in info. */
+int refh_gmres_ilu(sparse_matrix* A, double* x, double* b, int m, double tol, int max_iter, double* eta_out) {
+    int n = A->size, i;
+    struct ITLIN_OPT opt;
+    struct ITLIN_INFO info;
+    sparse_matrix* L = sparse_zero(n);
+    sparse_matrix* U = sparse_zero(n);
+    double* H = (double*)malloc(n * sizeof(double));
+    incomplete_LU_factorization(A, L, U);
+    PCG_default(A, L, U);
+    memset(&opt, 0, sizeof(opt));
+    memset(&info, 0, sizeof(info));
+    opt.tol = tol;
+    opt.i_max = m;
+    opt.maxiter = max_iter;
+    opt.termcheck = CheckOnRestart;
+    opt.errorlevel = None;
+    opt.monitorlevel = None;
+    opt.datalevel = None;
+    for (i = 0; i < n; i++) H[i] = x[i];
+    gmres(n, H, &h_gm_matvec, &h_gm_precon, NULL, b, &opt, &info);
+    PCG_preconlr(n, H, x);
+    if (eta_out) *eta_out = info.precision;
+    free(H);
+    Free_sparse(L); free(L);
+    Free_sparse(U); free(U);
+    return info.iter;
+}

@@ -1,0 +1,36 @@
+"""ILU(0) on the host, checked without a GPU: the factors are the
+reference's bit for bit, and L*U reproduces A on A's pattern (the defining
+property of ILU(0))."""
+import numpy as np
+import pytest
+
+from fem_amd import lina
+from fem_amd import problems as P
+from oracle.bind import CSR
+
+
+@pytest.mark.parametrize("kind", ["poisson", "damage", "elasticity"])
+def test_ilu_factors_match_reference(ref, kind):
+    s = {"poisson": lambda: P.poisson(33), "damage": lambda: P.damage_system(33)[0],
+         "elasticity": lambda: P.degraded_elasticity(17)}[kind]()
+    M = ref.matrix(s.rowptr, s.col, s.val)
+    ok, Lr, Ur = ref.ilu(M)
+    ref.free(M)
+    il = lina.ILU(s.rowptr, s.col, s.val, upload=False)
+    assert il.ok == bool(ok)
+    for w, R in (("L", Lr), ("U", Ur)):
+        rp, ci, v = il.factor(w)
+        assert np.array_equal(rp, R.rowptr) and np.array_equal(ci, R.col) and np.array_equal(v, R.val)
+
+
+def test_ilu_reproduces_pattern_of_A():
+    s = P.damage_system(17)[0]
+    il = lina.ILU(s.rowptr, s.col, s.val, upload=False)
+    L = CSR(*il.factor("L")).dense(s.n)
+    U = CSR(*il.factor("U")).dense(s.n)
+    A = s.scipy().toarray()
+    LU = L @ U
+    mask = A != 0
+    assert np.allclose(LU[mask], A[mask], rtol=1e-12, atol=1e-12 * np.abs(A).max())
+    assert np.allclose(np.diag(U), 1.0)
+    assert np.all(np.triu(L, 1) == 0) and np.all(np.tril(U, -1) == 0)
