@@ -69,16 +69,53 @@ __device__ __forceinline__ double block_sum(double v) {
     return s;  // valid in thread 0
 }
 
-template <typename T>
-__device__ __forceinline__ T ldnt(const T* p) {
-    return __builtin_nontemporal_load(p);
+#ifndef LINA_NT_LOADS
+#define LINA_NT_LOADS 1
+#endif
+#ifndef LINA_STAGE_UNROLL
+#define LINA_STAGE_UNROLL 1
+#endif
+
+// Loads of streamed arrays.  NT: nontemporal (the FP64 matrices, whose
+// entries are read once per sweep and would otherwise displace the gathered
+// vector from L2); FP32 matrices measured faster with ordinary loads.
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+    if constexpr (NT && LINA_NT_LOADS) return __builtin_nontemporal_load(p);
+    else return *p;
 }
 
-// phase 1: prod[k-k0] = v[k] * x[ci[k]] for the block's entries
+// phase 1: prod[k-k0] = v[k] * x[ci[k]] for the block's entries (k1-k0 <=
+// BLOCK_NNZ).  Each lane first issues all of its column and value loads,
+// then all of its gathers, so that up to 3*BLOCK_NNZ/TPB loads per lane are
+// in flight at once.
 template <typename VT, typename XT>
 __device__ __forceinline__ void stage_products(int k0, int k1, const int* __restrict__ ci, const VT* __restrict__ v,
                                                const XT* __restrict__ x, XT* prod) {
-    for (int k = k0 + threadIdx.x; k < k1; k += TPB) prod[k - k0] = (XT)ldnt(v + k) * x[ldnt(ci + k)];
+    constexpr bool NT = sizeof(VT) == 8;
+#if LINA_STAGE_UNROLL
+    constexpr int U = BLOCK_NNZ / TPB;
+    const int base = k0 + threadIdx.x;
+    int c[U];
+    VT a[U];
+    XT xv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int k = base + u * TPB;
+        if (k < k1) {
+            c[u] = ld_stream<NT>(ci + k);
+            a[u] = ld_stream<NT>(v + k);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if (base + u * TPB < k1) xv[u] = x[c[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if (base + u * TPB < k1) prod[base + u * TPB - k0] = (XT)a[u] * xv[u];
+#else
+    for (int k = k0 + threadIdx.x; k < k1; k += TPB) prod[k - k0] = (XT)ld_stream<NT>(v + k) * x[ld_stream<NT>(ci + k)];
+#endif
 }
 
 // ------------------------------------------------------------------ SpMV
@@ -91,7 +128,7 @@ __device__ __forceinline__ void stage_products(int k0, int k1, const int* __rest
 template <typename XT, int MODE>
 __device__ __forceinline__ XT spmv_out(XT s, int r, const XT* b, XT* y) {
     XT out;
-    if (MODE == MODE_SET) out = s;
+    if (MODE == MODE_SET || MODE == MODE_OUTER) out = s;
     else if (MODE == MODE_RES) out = b[r] + (-s);
     else out = y[r] + s;
     y[r] = out;
@@ -192,6 +229,21 @@ __global__ void k_fix_isolated(int m, const int* __restrict__ rows, const double
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
         int i = rows[k];
         x[i] = (T)((double)f[i] / a[k]);
+    }
+}
+
+__global__ void k_zero_if(int n, double* __restrict__ x, const double* __restrict__ scal, int slot) {
+    if (scal[slot] != 0.0) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = 0.0;
+}
+
+__global__ void k_fix_isolated_if(int m, const int* __restrict__ rows, const double* __restrict__ a,
+                                  const double* __restrict__ f, double* __restrict__ x,
+                                  const double* __restrict__ scal, int slot) {
+    if (scal[slot] != 0.0) return;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        int i = rows[k];
+        x[i] = f[i] / a[k];
     }
 }
 
@@ -401,6 +453,28 @@ __global__ __launch_bounds__(TPB) void k_jacobi(int nblk, const int* __restrict_
     }
 }
 
+// First Jacobi sweep from x = 0 (the coarse levels of a V-cycle start from a
+// zero guess): every product vanishes, so the sweep is elementwise and reads
+// no matrix row, with the same result as k_jacobi on xo = 0:
+//   JAC_REF : x_i = f_i / a_ii for i < nlim, x_i = 0 beyond
+//   JAC_L1  : x_i = f_i / sum_j |a_ij|
+template <typename VT, typename XT, int KIND>
+__global__ void k_jacobi0(int n, const int* __restrict__ dpos, const VT* __restrict__ v, const VT* __restrict__ l1,
+                          const XT* __restrict__ f, XT* __restrict__ x, int nlim) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        if (KIND == JAC_REF) {
+            if (r >= nlim) {
+                x[r] = (XT)0;
+            } else {
+                const int dp = dpos[r];
+                x[r] = f[r] / (dp >= 0 ? (XT)v[dp] : (XT)0);
+            }
+        } else {
+            x[r] = f[r] / (XT)l1[r];
+        }
+    }
+}
+
 // ------------------------------------------ ILU(0) triangular solves
 //
 // PCG_preconl / PCG_preconr (LinA/PCGS.c:61-100) by level scheduling inside
@@ -481,7 +555,7 @@ __global__ __launch_bounds__(TPB) void k_coarse(int nd, const int* __restrict__ 
     for (int row = wave; row < nd; row += nw) {
         const double* a = inv + (size_t)row * nd;
         double s = 0.0;
-        for (int c = lane; c < nd; c += WAVE) s += ldnt(a + c) * (double)b[didx[c]];
+        for (int c = lane; c < nd; c += WAVE) s += ld_stream<true>(a + c) * (double)b[didx[c]];
         s = wave_sum(s);
         if (lane == 0) x[didx[row]] = (XT)s;
     }
@@ -546,6 +620,9 @@ int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int
     if (mode == MODE_SET) {
         LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, MODE_SET, L>), g, t, 0, st, A.nblk, A.blk, A.rp,
                                                     A.ci, A.v, x, b, y, part, w))
+    } else if (mode == MODE_OUTER) {
+        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, MODE_OUTER, L>), g, t, 0, st, A.nblk, A.blk,
+                                                    A.rp, A.ci, A.v, x, b, y, part, w))
     } else if (mode == MODE_RES) {
         LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, MODE_RES, L>), g, t, 0, st, A.nblk, A.blk, A.rp,
                                                     A.ci, A.v, x, b, y, part, w))
@@ -609,6 +686,13 @@ void fix_isolated(hipStream_t st, int m, const int* rows, const double* a, const
 template void fix_isolated<double>(hipStream_t, int, const int*, const double*, const double*, double*);
 template void fix_isolated<float>(hipStream_t, int, const int*, const double*, const float*, float*);
 
+void restore_if_zero(hipStream_t st, int n, double* x, const double* scal, int slot, int m, const int* rows,
+                     const double* a, const double* f) {
+    hipLaunchKernelGGL(k_zero_if, dim3(grid_for(n)), dim3(TPB), 0, st, n, x, scal, slot);
+    if (m > 0)
+        hipLaunchKernelGGL(k_fix_isolated_if, dim3(grid_for(m)), dim3(TPB), 0, st, m, rows, a, f, x, scal, slot);
+}
+
 void scale_by(hipStream_t st, int n, double* x, const double* scal, int slot, int inverse) {
     hipLaunchKernelGGL(k_scale_by, dim3(grid_for(n)), dim3(TPB), 0, st, n, x, scal, slot, inverse);
 }
@@ -671,6 +755,20 @@ void jacobi(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, const XT* 
 }
 template void jacobi<double, double>(hipStream_t, const DMat<double>&, int, const double*, const double*, double*,
                                      int);
+
+template <typename VT, typename XT>
+void jacobi0(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, XT* x, int nlim) {
+    if (A.n == 0) return;
+    const int g = std::min((A.n + TPB - 1) / TPB, 8192);
+    if (kind == JAC_REF)
+        hipLaunchKernelGGL((k_jacobi0<VT, XT, JAC_REF>), dim3(g), dim3(TPB), 0, st, A.n, A.dpos, A.v, A.l1, f, x,
+                           nlim);
+    else
+        hipLaunchKernelGGL((k_jacobi0<VT, XT, JAC_L1>), dim3(g), dim3(TPB), 0, st, A.n, A.dpos, A.v, A.l1, f, x,
+                           nlim);
+}
+template void jacobi0<double, double>(hipStream_t, const DMat<double>&, int, const double*, double*, int);
+template void jacobi0<float, float>(hipStream_t, const DMat<float>&, int, const float*, float*, int);
 template void jacobi<float, float>(hipStream_t, const DMat<float>&, int, const float*, const float*, float*, int);
 
 template <typename XT>

@@ -11,7 +11,9 @@ namespace dev {
 constexpr int BLOCK_NNZ = 2048;
 constexpr int BLOCK_ROWS = 256;
 
-enum { MODE_SET = 0, MODE_RES = 1, MODE_ADD = 2 };
+// MODE_OUTER is MODE_SET for the finest-level operator of the outer Krylov
+// loop; a separate instantiation so that profiles report it on its own
+enum { MODE_SET = 0, MODE_RES = 1, MODE_ADD = 2, MODE_OUTER = 3 };
 enum { JAC_REF = 0, JAC_L1 = 1 };
 
 // device scalar slots used by the Krylov drivers
@@ -28,6 +30,7 @@ enum {
     S_T3 = 9,
     S_T4 = 10,
     S_T5 = 11,
+    S_AMG0 = 12,  // sum of squares of the preconditioner's right-hand side
     S_COUNT = 16
 };
 
@@ -70,6 +73,10 @@ void signs(hipStream_t st, int n, const double* s, const double* x, double* y);
 template <typename T>
 void fix_isolated(hipStream_t st, int m, const int* rows, const double* a, const T* f, T* x);
 void scale_by(hipStream_t st, int n, double* x, const double* scal, int slot, int inverse);
+// when scal[slot] == 0: x = 0, then x_i = f_i / a_i on the isolated rows
+// (the guess AMG_solve returns untouched when its first residual is zero)
+void restore_if_zero(hipStream_t st, int n, double* x, const double* scal, int slot, int m, const int* rows,
+                     const double* a, const double* f);
 void pcg_p(hipStream_t st, int n, double* P, const double* Z, const double* scal, int first);
 int pcg_xr(hipStream_t st, int n, double* X, double* R, const double* P, const double* Q, const double* scal,
            double* part);
@@ -84,6 +91,9 @@ void sor_lex(hipStream_t st, const DMat<VT>& A, int nlv, const int* lv_off, cons
              XT* xo, XT omega, int sweeps, int backward);
 template <typename VT, typename XT>
 void jacobi(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, const XT* xo, XT* x, int nlim);
+// the first sweep from x = 0, elementwise (no matrix row is read)
+template <typename VT, typename XT>
+void jacobi0(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, XT* x, int nlim);
 void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* piv);
 // level-scheduled ILU(0) triangular solve (upper=0: L, diagonal last; 1: U, diagonal first)
 void trsv(hipStream_t st, const int* rp, const int* ci, const double* v, int nlv, const int* lv_off,

@@ -410,16 +410,27 @@ struct LevelVecs<float> {
     static float* xo(DevLevel& d) { return d.xof; }
 };
 
-template <typename T>
-void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post) {
-    DevLevel& d = L_[l];
-    hipStream_t st = stream_;
-    const T omega = (T)prm_.omega;
-    int smoother = prm_.smoother;
+// smoother and sweep count in effect on level l (coarse_smoother overrides
+// the levels below the finest)
+void Solver::level_smoother(int l, int& smoother, int& sweeps) const {
+    smoother = prm_.smoother;
     if (l < (int)L_.size() - 1 && prm_.coarse_smoother >= 0) {
         smoother = prm_.coarse_smoother;
         sweeps = prm_.coarse_sweeps;
     }
+}
+
+// x_zero: x is not initialised and stands for the zero vector (a coarse
+// level's guess in the V-cycle); the smoother then writes every entry
+template <typename T>
+void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post, bool x_zero) {
+    DevLevel& d = L_[l];
+    hipStream_t st = stream_;
+    const T omega = (T)prm_.omega;
+    int smoother;
+    level_smoother(l, smoother, sweeps);
+    const bool jac = smoother == LINA_SMOOTHER_JACOBI || smoother == LINA_SMOOTHER_L1_JACOBI;
+    if (x_zero && !(jac && sweeps > 0)) dev::fill<T>(st, d.n, x, (T)0);
     switch (smoother) {
         case LINA_SMOOTHER_SOR:
         case LINA_SMOOTHER_SSOR: {
@@ -446,11 +457,17 @@ void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post) {
         case LINA_SMOOTHER_JACOBI:
         case LINA_SMOOTHER_L1_JACOBI: {
             // ping-pong between x and the level's spare vector; an odd sweep
-            // count ends with one copy back
+            // count ends with one copy back.  From a zero guess the first
+            // sweep is elementwise.
             int kind = smoother == LINA_SMOOTHER_JACOBI ? dev::JAC_REF : dev::JAC_L1;
             T* a = x;
             T* b = LevelVecs<T>::xo(d);
-            for (int s = 0; s < sweeps; s++) {
+            int s = 0;
+            if (x_zero && sweeps > 0) {
+                dev::jacobi0(st, pickA<T>(d), kind, f, x, d.nlim);
+                s = 1;
+            }
+            for (; s < sweeps; s++) {
                 dev::jacobi(st, pickA<T>(d), kind, f, (const T*)a, b, d.nlim);
                 std::swap(a, b);
             }
@@ -462,11 +479,15 @@ void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post) {
     }
 }
 
-// AMG_V_cycle (Multigrid.c:1536-1580) with Iter[level] == 1
+// AMG_V_cycle (Multigrid.c:1536-1580) with Iter[level] == 1.  x_zero: x is
+// the zero vector, left for the pre-smoother to write (the reference zeroes
+// the coarse guess before the recursive call).
 template <typename T>
-void Solver::vcycle(int l, const T* f, T* x) {
+void Solver::vcycle(int l, const T* f, T* x, bool x_zero) {
     hipStream_t st = stream_;
     if (l == 0) {
+        // the dense-row/trivial-row split need not cover every row
+        if (x_zero) dev::fill<T>(st, L_[0].n, x, (T)0);
         dev::coarse_solve(st, nd_, didx_, inv_, nt_, tidx_, tdiag_, f, x);
         return;
     }
@@ -475,16 +496,15 @@ void Solver::vcycle(int l, const T* f, T* x) {
     const int depth = (int)L_.size();
     int sw = (prm_.coarsening == AGGRESSIVE && l == depth - 1) ? prm_.aggr_smoothing_iterations
                                                                 : prm_.smoothing_iterations;
-    smooth<T>(l, f, x, sw, false);
+    smooth<T>(l, f, x, sw, false, x_zero);
     T* r = LevelVecs<T>::r(d);
     dev::spmv<T, T>(st, pickA<T>(d), x, f, r, dev::MODE_RES, nullptr, nullptr);
     T* fc = LevelVecs<T>::f(c);
     T* xc = LevelVecs<T>::x(c);
     dev::spmv<T, T>(st, pickR<T>(d), r, nullptr, fc, dev::MODE_SET, nullptr, nullptr);
-    dev::fill<T>(st, c.n, xc, (T)0);
-    vcycle<T>(l - 1, fc, xc);
+    vcycle<T>(l - 1, fc, xc, true);
     dev::spmv<T, T>(st, pickP<T>(d), xc, nullptr, x, dev::MODE_ADD, nullptr, nullptr);
-    smooth<T>(l, f, x, sw, true);
+    smooth<T>(l, f, x, sw, true, false);
 }
 
 // |f - A x| on the finest level into scal[slot] (FP64 or the FP32 copy)
@@ -542,7 +562,47 @@ int Solver::amg_solve_dev(const double* F, double* X) {
     return it;
 }
 
+// AMG_solve from a zero guess with at most one V-cycle, without a host round
+// trip.  The reference's first residual is r0 = |F - A*0| = |F|; only
+// `r0 == 0` matters with one V-cycle (the loop test is r/r0 = 1 > eps
+// otherwise), and it holds exactly when every F_i^2 underflows, in whatever
+// order the squares are summed.  The V-cycle runs unconditionally and its
+// result is replaced by the untouched guess on the device when r0 == 0.
+int Solver::vcycle_once(const double* F, double* X) {
+    hipStream_t st = stream_;
+    DevLevel& top = L_.back();
+    const int n = top.n;
+    const int lt = (int)L_.size() - 1;
+    const double* Ft = F;
+    if (has_neg_signs_) {
+        dev::signs(st, n, signs_, F, ftop_);
+        Ft = ftop_;
+    }
+    int np = dev::dot_part<double>(st, n, Ft, Ft, part_);
+    dev::finalize(st, np, part_, scal_, dev::S_AMG0, 0);
+    // without isolated rows the guess is exactly zero and the pre-smoother
+    // writes it; otherwise AMG_fix_isolated sets those rows first
+    const bool xz = niso_ == 0;
+    if (!xz) {
+        dev::fill<double>(st, n, X, 0.0);
+        dev::fix_isolated<double>(st, niso_, iso_rows_, iso_a_, Ft, X);
+    }
+    if (prm_.precision == LINA_PREC_FP64) {
+        vcycle<double>(lt, Ft, X, xz);
+    } else {
+        dev::convert<double, float>(st, n, Ft, top.ff);
+        if (!xz) dev::convert<double, float>(st, n, X, top.xf);
+        vcycle<float>(lt, top.ff, top.xf, xz);
+        dev::convert<float, double>(st, n, top.xf, X);
+    }
+    dev::restore_if_zero(st, n, X, scal_, dev::S_AMG0, niso_, iso_rows_, iso_a_, Ft);
+    vcycles_ += 1;
+    precon_calls_++;
+    return 1;
+}
+
 int Solver::precon(const double* R, double* Z) {
+    if (prm_.zero_init && prm_.amg_max_iter == 1) return vcycle_once(R, Z);
     if (prm_.zero_init) dev::fill<double>(stream_, n_, Z, 0.0);
     return amg_solve_dev(R, Z);
 }
@@ -599,7 +659,7 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
             apply_A(P, Q);
             np = dev::dot_part<double>(st, n, P, Q, part_);
         } else {
-            np = dev::spmv<double, double>(st, top.A, P, nullptr, Q, dev::MODE_SET, part_, P);  // Q = A P, P.Q
+            np = dev::spmv<double, double>(st, top.A, P, nullptr, Q, dev::MODE_OUTER, part_, P);  // Q = A P, P.Q
         }
         dev::finalize(st, np, part_, scal_, dev::S_PQ, 0);
         np = dev::pcg_xr(st, n, X, R, P, Q, scal_, part_);
@@ -677,9 +737,9 @@ double Solver::time_spmv(int level, int precision, int reps) {
     dev::fill<float>(stream_, d.n, d.xof, 1.0f);
     auto launch = [&]() {
         if (precision == LINA_PREC_FP64)
-            dev::spmv<double, double>(stream_, d.A, d.xo, nullptr, d.r, dev::MODE_SET, nullptr, nullptr);
+            dev::spmv<double, double>(stream_, d.A, d.xo, nullptr, d.r, dev::MODE_OUTER, nullptr, nullptr);
         else
-            dev::spmv<float, float>(stream_, d.Af, d.xof, nullptr, d.rf, dev::MODE_SET, nullptr, nullptr);
+            dev::spmv<float, float>(stream_, d.Af, d.xof, nullptr, d.rf, dev::MODE_OUTER, nullptr, nullptr);
     };
     for (int i = 0; i < 3; i++) launch();
     hip_check(hipEventRecord(ev0_, stream_), "event");
@@ -698,9 +758,9 @@ double Solver::spmv_bytes(int level, int precision) const {
     return 8.0 * nnz + 4.0 * (n + 1) + 8.0 * n;
 }
 
-template void Solver::smooth<double>(int, const double*, double*, int, bool);
-template void Solver::smooth<float>(int, const float*, float*, int, bool);
-template void Solver::vcycle<double>(int, const double*, double*);
-template void Solver::vcycle<float>(int, const float*, float*);
+template void Solver::smooth<double>(int, const double*, double*, int, bool, bool);
+template void Solver::smooth<float>(int, const float*, float*, int, bool, bool);
+template void Solver::vcycle<double>(int, const double*, double*, bool);
+template void Solver::vcycle<float>(int, const float*, float*, bool);
 
 }  // namespace lina

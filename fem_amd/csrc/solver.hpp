@@ -66,16 +66,18 @@ class Solver {
     int amg_solve_dev(const double* F, double* X);
     int amg_solve_host(const double* f, double* x);
     int precon(const double* R, double* Z);
+    int vcycle_once(const double* F, double* X);
     int spmv_host(int level, int precision, const double* x, double* y);
     double time_spmv(int level, int precision, int reps);
     double spmv_bytes(int level, int precision) const;
     double pcg_residuum() const { return pcg_residuum_; }
     hipStream_t stream() const { return stream_; }
 
+    void level_smoother(int l, int& smoother, int& sweeps) const;
     template <typename T>
-    void smooth(int l, const T* f, T* x, int sweeps, bool post);
+    void smooth(int l, const T* f, T* x, int sweeps, bool post, bool x_zero = false);
     template <typename T>
-    void vcycle(int l, const T* f, T* x);
+    void vcycle(int l, const T* f, T* x, bool x_zero = false);
     template <typename T>
     double residual_norm(const T* f, const T* x, T* r, int slot);
     double read_scalar(int slot);

@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblina_hip.so")
+LIB_PATH = os.environ.get("LINA_LIB_PATH", os.path.join(HERE, "liblina_hip.so"))
 
 SMOOTHER = {"sor": 0, "jacobi": 1, "mc_sor": 2, "l1_jacobi": 3, "mc_ssor": 4, "ssor": 5}
 PREC = {"fp64": 0, "mixed": 1}
