@@ -65,6 +65,44 @@ print(json.dumps({{"n": s.n, "nnz": s.nnz, "iters": out["iters"], "t_setup": out
     return json.loads(lines[-1])
 
 
+def copy_gbs(lina, n=1 << 27, reps=5):
+    """hipMemcpy device-to-device of n doubles (read n*8 + write n*8 bytes):
+    the attainable streaming rate beside the 8 TB/s peak."""
+    a = lina.DeviceVector(n)
+    b = lina.DeviceVector(n)
+    a.fill(1.0)
+    b.copy_from(a)
+    t = time.perf_counter()
+    for _ in range(reps):
+        b.copy_from(a)
+    dt = (time.perf_counter() - t) / reps
+    return 16.0 * n / dt / 1e9
+
+
+def kernels_only(s, lina):
+    """The launches the counter runs read: one copy of known size (the
+    calibration of FETCH_SIZE / WRITE_SIZE) and the finest-level SpMVs, on a
+    device matrix handle (the same kernel and row blocks as the solver's)."""
+    gbs = copy_gbs(lina, reps=3)
+    m = lina.Matrix(s.rowptr, s.col, s.val)
+    ms64 = m.time_spmv("fp64", reps=5)
+    ms32 = m.time_spmv("mixed", reps=5)
+    print(json.dumps({"copy_bytes": 16 * (1 << 27), "copy_gbs": gbs, "spmv_fp64_bytes": m.spmv_bytes("fp64"),
+                      "spmv_fp64_ms": ms64, "spmv_fp32_bytes": m.spmv_bytes("mixed"), "spmv_fp32_ms": ms32}))
+
+
+def measured_traffic(path=os.path.join(ROOT, "profiles", "traffic_spmv_outer_elasticity1025.md")):
+    """HBM bytes per launch of the finest-level SpMV from the committed
+    counter summary (the line `traffic_bytes_fp64: <bytes>`), or None."""
+    try:
+        for line in open(path):
+            if line.strip().startswith("traffic_bytes_fp64:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,11 +114,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU baseline")
     ap.add_argument("--cpu-sample", type=int, default=129, help="side of the reference CPU sample")
     ap.add_argument("--coarse-sweeps", type=int, default=1, help="l1-Jacobi sweeps on the levels below the finest")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="no solves: a 1 GiB device copy and a few finest-level SpMVs, for counter runs")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded design is "
-              f"described in DESIGN.md and is not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); a row-block sharded solve "
+              f"is not built", file=sys.stderr)
         sys.exit(2)
 
     from fem_amd import lina, problems
@@ -88,6 +128,9 @@ def main():
     t0 = time.time()
     s = problems.degraded_elasticity(args.size)
     log(f"system n={s.n} nnz={s.nnz} built in {time.time() - t0:.1f}s")
+    if args.kernels_only:
+        kernels_only(s, lina)
+        return
     t0 = time.time()
     prm64 = lina.params("vcycle", 2, args.depth, coarse_sweeps=args.coarse_sweeps)
     amg = lina.AMG(s.rowptr, s.col, s.val, prm64)
@@ -122,25 +165,35 @@ def main():
         return float(np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref))
 
     res = {}
+    amg.kernel_timing(True)
     for name, prm in (("mixed", prm32), ("fp64", prm64)):
         for _ in range(args.warmup):
             solve(prm, args.tol)
         runs = [solve(prm, args.tol) for _ in range(args.steps)]
         x = dx.download()
         rr = float(np.linalg.norm(s.b - A @ x) / np.linalg.norm(s.b))
+        nk = sum(r["spmv_launches"] for r in runs)
         res[name] = {"ms": float(np.mean([r["wall_ms"] for r in runs])),
                      "device_ms": float(np.mean([r["t_solve_ms"] for r in runs])),
+                     "spmv_ms": float(sum(r["spmv_ms"] * r["spmv_launches"] for r in runs) / max(nk, 1)),
+                     "spmv_launches": nk,
                      "iters": runs[-1]["iters"], "vcycles": runs[-1]["vcycles"], "relres": rr, "err": err(x)}
         log(f"{name}: {res[name]}")
 
-    # finest-level SpMV against the HBM roofline (HIP events, library stream)
+    # finest-level SpMV against the HBM roofline: the outer-loop A*p launches
+    # of the timed mixed solves, each bracketed by HIP events on the library
+    # stream; beside it the same kernel back to back
     lvl = amg.depth - 1
-    spmv_ms = amg.time_spmv(lvl, "fp64", reps=100)
     spmv_bytes = amg.spmv_bytes(lvl, "fp64")
+    spmv_ms = res["mixed"]["spmv_ms"]
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    b2b_ms = amg.time_spmv(lvl, "fp64", reps=100)
     spmv32_ms = amg.time_spmv(lvl, "mixed", reps=100)
     spmv32_gbs = amg.spmv_bytes(lvl, "mixed") / (spmv32_ms * 1e-3) / 1e9
-    log(f"SpMV fp64 {spmv_ms * 1e3:.1f} us {spmv_gbs:.0f} GB/s; fp32 {spmv32_ms * 1e3:.1f} us {spmv32_gbs:.0f} GB/s")
+    log(f"SpMV fp64 in-solve {spmv_ms * 1e3:.1f} us {spmv_gbs:.0f} GB/s, back-to-back {b2b_ms * 1e3:.1f} us; "
+        f"fp32 back-to-back {spmv32_ms * 1e3:.1f} us {spmv32_gbs:.0f} GB/s")
+    cgbs = copy_gbs(lina)
+    log(f"device copy {cgbs:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
@@ -184,10 +237,12 @@ def main():
         "oracle": {"kind": "FP64 AMG-PCG to relres 1e-14", "iters": st_o["iters"], "true_relres": rr_o},
         "setup_s": t_setup,
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_spmv<double,double,SET> finest level",
-                     "bytes_per_launch": spmv_bytes, "avg_ms": spmv_ms},
-        "spmv_fp32": {"achieved": spmv32_gbs, "avg_ms": spmv32_ms},
+                     "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "kernel": "k_spmv<double, double, 3, 1> (finest-level A*p of the outer CG)",
+                     "bytes_per_launch": spmv_bytes, "avg_ms": spmv_ms, "launches": res["mixed"]["spmv_launches"],
+                     "back_to_back_ms": b2b_ms, "back_to_back_gbs": spmv_bytes / (b2b_ms * 1e-3) / 1e9,
+                     "copy_gbs": cgbs},
+        "spmv_fp32": {"achieved": spmv32_gbs, "avg_ms": spmv32_ms, "kernel": "k_spmv<float, float, 3, 1>"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -18,6 +18,10 @@ using lina::Solver;
 struct lina_solver {
     Solver s;
 };
+struct lina_matrix {
+    explicit lina_matrix(const Csr& A) : m(A) {}
+    lina::DevMatrix m;
+};
 
 namespace {
 
@@ -279,7 +283,8 @@ int lina_dev_copy(double* dst, const double* src, long n, int kind) {
         lina::set_error("lina_dev_copy: bad arguments");
         return LINA_EARG;
     }
-    if (hipMemcpy(dst, src, (size_t)n * sizeof(double), k) != hipSuccess) {
+    if (hipMemcpy(dst, src, (size_t)n * sizeof(double), k) != hipSuccess ||
+        (kind == 2 && hipDeviceSynchronize() != hipSuccess)) {
         lina::set_error("hipMemcpy failed");
         return LINA_EHIP;
     }
@@ -297,6 +302,55 @@ int lina_dev_fill(double* d, long n, double value) {
 double lina_spmv_bytes(const lina_solver* s, int level, int precision) {
     return s->s.spmv_bytes(level, precision);
 }
+
+int lina_amg_kernel_timing(lina_solver* s, int on) {
+    return guard([&]() {
+        if (!s) throw std::invalid_argument("null solver");
+        s->s.set_kernel_timing(on != 0);
+        return 0;
+    });
+}
+
+lina_matrix* lina_matrix_create(int n, const int* rowptr, const int* col, const double* val) {
+    lina_matrix* h = nullptr;
+    int rc = guard([&]() {
+        Csr A = make_csr(n, rowptr, col, val);
+        h = new lina_matrix(A);
+        return 0;
+    });
+    return rc == 0 ? h : nullptr;
+}
+
+void lina_matrix_destroy(lina_matrix* h) { delete h; }
+
+int lina_matrix_spmv(lina_matrix* h, int precision, const double* x, double* y) {
+    return guard([&]() {
+        if (!h || !x || !y) throw std::invalid_argument("null argument");
+        if (precision != LINA_PREC_FP64 && precision != LINA_PREC_MIXED) throw std::invalid_argument("precision");
+        h->m.spmv_host(precision, x, y);
+        return 0;
+    });
+}
+
+int lina_matrix_spmv_device(lina_matrix* h, const double* x, double* y) {
+    return guard([&]() {
+        if (!h || !x || !y) throw std::invalid_argument("null argument");
+        h->m.spmv_dev(x, y);
+        return 0;
+    });
+}
+
+double lina_matrix_time_spmv(lina_matrix* h, int precision, int reps) {
+    double out = -1.0;
+    int rc = guard([&]() {
+        if (!h || reps < 1) throw std::invalid_argument("bad arguments");
+        out = h->m.time_spmv(precision, reps);
+        return 0;
+    });
+    return rc == 0 ? out : -1.0;
+}
+
+double lina_matrix_spmv_bytes(const lina_matrix* h, int precision) { return h ? h->m.bytes(precision) : 0.0; }
 
 }  // extern "C"
 

@@ -162,7 +162,7 @@ static void upload_mat(const Csr& A, const std::vector<int>& starts_with_end, de
 }
 
 // upload A as FP64 + FP32 copies sharing the index arrays; returns #blocks
-static int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st) {
+int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st) {
     Csr A = drop_outside_zeros(A0);
     std::vector<int> starts;
     build_blocks(A, 0, A.n, starts);
@@ -206,6 +206,8 @@ Solver::~Solver() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
+    if (evk0_) (void)hipEventDestroy(evk0_);
+    if (evk1_) (void)hipEventDestroy(evk1_);
     if (host_scal_) (void)hipHostFree(host_scal_);
     bufs_.reset();
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -355,7 +357,15 @@ void Solver::recompute(const std::vector<double>& val) {
         for (int p = A.rp[i]; p < A.rp[i + 1]; p++) A.v[p] = val[p] * (H_.signs[i] < 0 ? -1.0 : 1.0);
     amg_recompute(H_, A);
     for (size_t l = 1; l < H_.lv.size(); l++) colours_[l] = greedy_colouring(H_.lv[l].A, &ncolours_[l]);
-    upload();
+    if (uploaded()) upload();
+}
+
+void Solver::set_kernel_timing(bool on) {
+    if (on && !evk0_) {
+        hip_check(hipEventCreate(&evk0_), "hipEventCreate");
+        hip_check(hipEventCreate(&evk1_), "hipEventCreate");
+    }
+    ktime_ = on;
 }
 
 double* Solver::dalloc_public(size_t n) { return dalloc<double>(n, *bufs_); }
@@ -650,6 +660,8 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
     }
     double r = r0;
     int i = 0;
+    double ksum = 0.0;
+    int kcount = 0;
     do {
         precon(R, Z);
         np = dev::dot_part<double>(st, n, R, Z, part_);
@@ -659,12 +671,20 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
             apply_A(P, Q);
             np = dev::dot_part<double>(st, n, P, Q, part_);
         } else {
+            if (ktime_) hip_check(hipEventRecord(evk0_, st), "event");
             np = dev::spmv<double, double>(st, top.A, P, nullptr, Q, dev::MODE_OUTER, part_, P);  // Q = A P, P.Q
+            if (ktime_) hip_check(hipEventRecord(evk1_, st), "event");
         }
         dev::finalize(st, np, part_, scal_, dev::S_PQ, 0);
         np = dev::pcg_xr(st, n, X, R, P, Q, scal_, part_);
         dev::finalize(st, np, part_, scal_, dev::S_R, 1);
         r = read_scalar(dev::S_R);
+        if (ktime_ && !has_neg_signs_) {
+            float kms = 0;
+            hip_check(hipEventElapsedTime(&kms, evk0_, evk1_), "elapsed");
+            ksum += kms;
+            kcount++;
+        }
         i++;
         if (std::isnan(r)) break;
     } while ((stop == LINA_STOP_RELRES ? r : r / r0) > tol && i < max_iter);
@@ -687,6 +707,8 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
         stats->residual = pcg_residuum_;
         stats->relres = relres;
         stats->t_solve_ms = ms;
+        stats->spmv_ms = kcount ? ksum / kcount : 0.0;
+        stats->spmv_launches = kcount;
     }
     if (std::isnan(rw / r0)) throw NumericError("pcg: NaN residual");
     return i;
@@ -751,12 +773,81 @@ double Solver::time_spmv(int level, int precision, int reps) {
     return ms / reps;
 }
 
-double Solver::spmv_bytes(int level, int precision) const {
-    const Csr& A = H_.lv.at(level).A;
-    double nnz = A.nnz(), n = A.n;
+double spmv_bytes(double n, double nnz, int precision) {
     if (precision == LINA_PREC_FP64) return 12.0 * nnz + 4.0 * (n + 1) + 16.0 * n;
     return 8.0 * nnz + 4.0 * (n + 1) + 8.0 * n;
 }
+
+double Solver::spmv_bytes(int level, int precision) const {
+    const Csr& A = H_.lv.at(level).A;
+    return lina::spmv_bytes(A.n, A.nnz(), precision);
+}
+
+// ------------------------------------------------------- DevMatrix
+
+DevMatrix::DevMatrix(const Csr& A) : n_(A.n), nnz_(A.nnz()), bufs_(new DevBuffers) {
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream");
+    hip_check(hipEventCreate(&ev0_), "event");
+    hip_check(hipEventCreate(&ev1_), "event");
+    upload_pair(A, A_, Af_, *bufs_, stream_);
+    x_ = dalloc<double>(n_, *bufs_);
+    y_ = dalloc<double>(n_, *bufs_);
+    xf_ = dalloc<float>(n_, *bufs_);
+    yf_ = dalloc<float>(n_, *bufs_);
+    hip_check(hipStreamSynchronize(stream_), "upload");
+}
+
+DevMatrix::~DevMatrix() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    bufs_.reset();
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void DevMatrix::apply(const double* x, double* y) {
+    dev::spmv<double, double>(stream_, A_, x, nullptr, y, dev::MODE_OUTER, nullptr, nullptr);
+}
+
+void DevMatrix::spmv_dev(const double* x, double* y) {
+    apply(x, y);
+    hip_check(hipStreamSynchronize(stream_), "spmv");
+}
+
+void DevMatrix::spmv_host(int precision, const double* x, double* y) {
+    const size_t bytes = (size_t)n_ * sizeof(double);
+    hip_check(hipMemcpyAsync(x_, x, bytes, hipMemcpyHostToDevice, stream_), "h2d");
+    if (precision == LINA_PREC_FP64) {
+        apply(x_, y_);
+    } else {
+        dev::convert<double, float>(stream_, n_, x_, xf_);
+        dev::spmv<float, float>(stream_, Af_, xf_, nullptr, yf_, dev::MODE_OUTER, nullptr, nullptr);
+        dev::convert<float, double>(stream_, n_, yf_, y_);
+    }
+    hip_check(hipMemcpyAsync(y, y_, bytes, hipMemcpyDeviceToHost, stream_), "d2h");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+}
+
+double DevMatrix::time_spmv(int precision, int reps) {
+    dev::fill<double>(stream_, n_, x_, 1.0);
+    dev::fill<float>(stream_, n_, xf_, 1.0f);
+    auto launch = [&]() {
+        if (precision == LINA_PREC_FP64)
+            dev::spmv<double, double>(stream_, A_, x_, nullptr, y_, dev::MODE_OUTER, nullptr, nullptr);
+        else
+            dev::spmv<float, float>(stream_, Af_, xf_, nullptr, yf_, dev::MODE_OUTER, nullptr, nullptr);
+    };
+    for (int i = 0; i < 3; i++) launch();
+    hip_check(hipEventRecord(ev0_, stream_), "event");
+    for (int i = 0; i < reps; i++) launch();
+    hip_check(hipEventRecord(ev1_, stream_), "event");
+    hip_check(hipEventSynchronize(ev1_), "event sync");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "elapsed");
+    return ms / reps;
+}
+
+double DevMatrix::bytes(int precision) const { return lina::spmv_bytes(n_, nnz_, precision); }
 
 template void Solver::smooth<double>(int, const double*, double*, int, bool, bool);
 template void Solver::smooth<float>(int, const float*, float*, int, bool, bool);

@@ -47,6 +47,37 @@ struct DevLevel {
     float *xf = nullptr, *ff = nullptr, *rf = nullptr, *xof = nullptr;
 };
 
+int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st);
+// algorithmic bytes of one SpMV: 12 nnz + 4 (n+1) + 16 n (FP64), 8 nnz + 4 (n+1) + 8 n (FP32)
+double spmv_bytes(double n, double nnz, int precision);
+
+// A device-resident matrix (FP64 and FP32 copies) for y = A x outside a
+// hierarchy: the library's own matrix-vector product of the Krylov drivers.
+class DevMatrix {
+  public:
+    explicit DevMatrix(const Csr& A);
+    ~DevMatrix();
+    int n() const { return n_; }
+    long nnz() const { return nnz_; }
+    void apply(const double* x, double* y);  // enqueue on stream(), FP64
+    void spmv_dev(const double* x, double* y);
+    void spmv_host(int precision, const double* x, double* y);
+    double time_spmv(int precision, int reps);
+    double bytes(int precision) const;
+    hipStream_t stream() const { return stream_; }
+
+  private:
+    int n_ = 0;
+    long nnz_ = 0;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    std::unique_ptr<DevBuffers> bufs_;
+    dev::DMat<double> A_;
+    dev::DMat<float> Af_;
+    double *x_ = nullptr, *y_ = nullptr;
+    float *xf_ = nullptr, *yf_ = nullptr;
+};
+
 class Solver {
   public:
     Solver();
@@ -92,6 +123,7 @@ class Solver {
     int bicgstab_dev(double* X, const double* b, double eps, int max_iter, double* res);
     int bicgstab_host(double* x, const double* b, double eps, int max_iter, double* res);
     int last_vcycles() const { return vcycles_; }
+    void set_kernel_timing(bool on);
 
   private:
     lina_amg_params prm_{};
@@ -119,6 +151,8 @@ class Solver {
     double* scal_ = nullptr;
     int vcycles_ = 0, precon_calls_ = 0;
     double pcg_residuum_ = 0;
+    bool ktime_ = false;
+    hipEvent_t evk0_ = nullptr, evk1_ = nullptr;
     double *bS_ = nullptr, *bPS_ = nullptr, *bPP_ = nullptr, *bAPP_ = nullptr, *bAPS_ = nullptr, *bR_ = nullptr,
            *bR0_ = nullptr, *bP_ = nullptr;
 };

@@ -31,7 +31,8 @@ class Params(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("precon_calls", C.c_int), ("vcycles", C.c_int), ("residual", C.c_double),
-                ("relres", C.c_double), ("t_solve_ms", C.c_double)]
+                ("relres", C.c_double), ("t_solve_ms", C.c_double), ("spmv_ms", C.c_double),
+                ("spmv_launches", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -75,11 +76,21 @@ def lib():
     L.lina_time_spmv.restype = C.c_double
     L.lina_spmv_bytes.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.lina_spmv_bytes.restype = C.c_double
+    L.lina_amg_kernel_timing.argtypes = [C.c_void_p, C.c_int]
     L.lina_dev_alloc.argtypes = [C.c_long]
     L.lina_dev_alloc.restype = C.c_void_p
     L.lina_dev_free.argtypes = [C.c_void_p]
     L.lina_dev_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_long, C.c_int]
     L.lina_dev_fill.argtypes = [C.c_void_p, C.c_long, C.c_double]
+    L.lina_matrix_create.argtypes = [C.c_int, _ip, _ip, _dp]
+    L.lina_matrix_create.restype = C.c_void_p
+    L.lina_matrix_destroy.argtypes = [C.c_void_p]
+    L.lina_matrix_spmv.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    L.lina_matrix_spmv_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.lina_matrix_time_spmv.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.lina_matrix_time_spmv.restype = C.c_double
+    L.lina_matrix_spmv_bytes.argtypes = [C.c_void_p, C.c_int]
+    L.lina_matrix_spmv_bytes.restype = C.c_double
     _lib = L
     return L
 
@@ -218,9 +229,52 @@ class AMG:
             raise LinaError(lib().lina_last_error().decode())
         return ms
 
+    def kernel_timing(self, on: bool = True):
+        _check(lib().lina_amg_kernel_timing(self.h, 1 if on else 0))
+
     def spmv_bytes(self, level=None, precision="fp64") -> float:
         l = self.depth - 1 if level is None else level
         return lib().lina_spmv_bytes(self.h, l, PREC[precision])
+
+
+class Matrix:
+    """Device-resident CSR matrix (lina_matrix handle): y = A x."""
+
+    def __init__(self, rowptr, col, val):
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.n = len(self.rowptr) - 1
+        h = lib().lina_matrix_create(self.n, _i(self.rowptr), _i(self.col), _d(self.val))
+        if not h:
+            raise LinaError(f"lina_matrix_create failed: {lib().lina_last_error().decode()}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lina_matrix_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def spmv(self, x, precision="fp64"):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        _check(lib().lina_matrix_spmv(self.h, PREC[precision], _d(x), _d(y)))
+        return y
+
+    def spmv_device(self, d_x: int, d_y: int):
+        _check(lib().lina_matrix_spmv_device(self.h, C.c_void_p(d_x), C.c_void_p(d_y)))
+
+    def time_spmv(self, precision="fp64", reps=50) -> float:
+        ms = lib().lina_matrix_time_spmv(self.h, PREC[precision], reps)
+        if ms < 0:
+            raise LinaError(lib().lina_last_error().decode())
+        return ms
+
+    def spmv_bytes(self, precision="fp64") -> float:
+        return lib().lina_matrix_spmv_bytes(self.h, PREC[precision])
 
 
 class DeviceVector:

@@ -77,6 +77,8 @@ typedef struct lina_stats {
     double residual;     /* get_pcg_residuum(): |M^-1 R_final| / |M^-1 R_0| */
     double relres;       /* |R_final| of the recurrence with B scaled to 1 */
     double t_solve_ms;   /* device time of the solve (HIP events on the library stream) */
+    double spmv_ms;      /* mean duration of the outer-loop A*p launch (kernel timing on), else 0 */
+    int spmv_launches;   /* launches averaged into spmv_ms */
 } lina_stats;
 
 typedef struct lina_solver lina_solver;
@@ -163,6 +165,23 @@ double lina_time_spmv(lina_solver* s, int level, int precision, int reps);
 /* per-launch byte count used for roofline reporting: 12 nnz + 4 (n+1) + 16 n
  * for FP64 values (8 nnz + 4 (n+1) + 8 n for FP32) */
 double lina_spmv_bytes(const lina_solver* s, int level, int precision);
+/* on != 0: pcg brackets each outer-loop A*p launch with HIP events and
+ * reports the mean launch duration in lina_stats.spmv_ms */
+int lina_amg_kernel_timing(lina_solver* s, int on);
+
+/* A device-resident CSR matrix outside any hierarchy: the library's own
+ * y = A x of sparse_mult (linear_algebra.c:134-147) and PCG_mult
+ * (PCGS.c:36-59), summed per row in column order like the reference.
+ * Rows need >= 1 entry with sorted unique columns.  NULL on error. */
+typedef struct lina_matrix lina_matrix;
+lina_matrix* lina_matrix_create(int n, const int* rowptr, const int* col, const double* val);
+void lina_matrix_destroy(lina_matrix* h);
+/* host pointers; precision LINA_PREC_MIXED uses the FP32 copy */
+int lina_matrix_spmv(lina_matrix* h, int precision, const double* x, double* y);
+/* device pointers (FP64); returns after the product is complete */
+int lina_matrix_spmv_device(lina_matrix* h, const double* x, double* y);
+double lina_matrix_time_spmv(lina_matrix* h, int precision, int reps);
+double lina_matrix_spmv_bytes(const lina_matrix* h, int precision);
 
 #ifdef __cplusplus
 }

@@ -732,16 +732,14 @@ static int interpolation(const Mat* A, const int* label, const int* levels, cons
  * each row k seeded with (k,0.0) by sparse_zero */
 static void restriction(const Mat* P, int coarse, Mat* R) {
     int i, p, k;
-    Acc* rows = NULL;
     ivec c = {0};
     dvec v = {0};
-    /* build row by row with one accumulator per coarse row would be
-     * quadratic in memory; instead count, fill and merge the seed */
+    /* counting sort of P's nonzeros by column, then each row k merged with
+     * its (k,0.0) seed */
     int* cnt = (int*)calloc(coarse + 1, sizeof(int));
     int* fill;
     int* ci;
     double* cv;
-    (void)rows;
     for (i = 0; i < P->n; i++)
         for (p = P->rp[i]; p < P->rp[i + 1]; p++)
             if (P->v.a[p] != 0) cnt[P->ci.a[p] + 1]++;

@@ -46,23 +46,6 @@ int orc_colouring(int n, const int* rp, const int* ci, const double* v, int* col
 /* y = A x with sequential row sums (PCG_mult, PCGS.c:36-59) */
 void orc_spmv(int n, const int* rp, const int* ci, const double* v, const double* x, double* y);
 
-/* ILU(0) (linear_algebra.c:1952-1982): L lower with diagonal, U unit upper;
- * output CSR arrays sized like A (pattern of A plus diagonals).  Returns the
- * reference's flag (0 on a zero pivot). */
-int orc_ilu0(int n, const int* rp, const int* ci, const double* v, int* lrp, int* lci, double* lv, int* urp,
-             int* uci, double* uv);
-void orc_ilu_apply(int n, const int* lrp, const int* lci, const double* lv, const int* urp, const int* uci,
-                   const double* uv, const double* z, double* w);
-
-/* BiCGStab_solve (BiCGStab.c:115-198), right-preconditioned by AMG_solve */
-int orc_bicgstab_amg(orc_amg* h, double* X, const double* b, double eps, int max_iter, int smoother, double omega,
-                     double amg_eps, int amg_max_iter, double* res);
-
-/* restarted GMRES(m) with right ILU(0) preconditioning (Saad, Alg. 9.5):
- * returns iterations, x holds the solution of A x = b */
-int orc_gmres_ilu(int n, const int* rp, const int* ci, const double* v, double* x, const double* b, int m,
-                  double tol, int max_iter, double* res);
-
 #ifdef __cplusplus
 }
 #endif

@@ -98,3 +98,43 @@ def test_library_exports_every_declared_symbol():
             if not hasattr(L, name):
                 missing.append((os.path.basename(header), name))
     assert not missing, missing
+
+
+def test_recompute_matches_reference_and_oracle(ref):
+    """AMG_recompute_coarse_matrices (Multigrid.c:1756-1780): new finest values
+    with P and R kept; every level's A is bit-identical to the compiled
+    reference and to the oracle."""
+    from oracle.bind import Oracle
+    s1 = P.damage_system(33, seed_h=11)[0]
+    s2 = P.damage_system(33, seed_h=12)[0]
+    assert np.array_equal(s1.rowptr, s2.rowptr) and np.array_equal(s1.col, s2.col)
+    depth = 4
+    M1 = ref.matrix(s1.rowptr, s1.col, s1.val)
+    M2 = ref.matrix(s2.rowptr, s2.col, s2.val)
+    I = ref.amg_setup(M1, dof=1, depth=depth)
+    ref.L.refh_amg_recompute(I, M2)
+    R = ref.amg_levels(I, depth)
+    orc = Oracle()
+    h = orc.amg_setup(s1.rowptr, s1.col, s1.val, dof=1, depth=depth)
+    orc.L.orc_amg_recompute(h, s2.val.ctypes.data_as(orc_dp()))
+    O = orc.amg_levels(h, depth)
+    amg = lina.AMG(s1.rowptr, s1.col, s1.val, lina.params("reference", 1, depth), upload=False)
+    amg.recompute(s2.val)
+    changed = False
+    for l in range(depth):
+        rp, ci, v = amg.level(l)["A"]
+        assert np.array_equal(ci, R[l]["A"].col) and np.array_equal(v, R[l]["A"].val), l
+        assert np.array_equal(ci, O[l]["A"].col) and np.array_equal(v, O[l]["A"].val), l
+        changed |= l < depth - 1 and not np.array_equal(v, lina.AMG(s1.rowptr, s1.col, s1.val, lina.params(
+            "reference", 1, depth), upload=False).level(l)["A"][2])
+    assert changed
+    amg.close()
+    orc.amg_free(h)
+    ref.amg_free(I)
+    ref.free(M1)
+    ref.free(M2)
+
+
+def orc_dp():
+    import ctypes
+    return ctypes.POINTER(ctypes.c_double)
