@@ -541,6 +541,82 @@ __global__ void k_combine(int n, int m, const double* __restrict__ V, size_t ld,
     }
 }
 
+// ------------------------------------------------- reflective Newton
+//
+// Box-constraint bookkeeping of reflective_newton.c over the bounded index
+// set idx[0..nb) with bounds lo/up (NaN: unbounded side).
+
+// RN_set_D_J (reflective_newton.c:227-248); D = 1, J = 0 beforehand
+__global__ void k_rn_dj(int nb, const int* __restrict__ idx, const double* __restrict__ lo,
+                        const double* __restrict__ up, const double* __restrict__ G, const double* __restrict__ X,
+                        double* __restrict__ D, double* __restrict__ J) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nb; t += gridDim.x * blockDim.x) {
+        const int i = idx[t];
+        if (G[i] < 0) {
+            if (!isnan(up[t])) {
+                D[i] = sqrt(fabs(X[i] - up[t]));
+                J[i] = 1.0;
+            }
+        } else if (!isnan(lo[t])) {
+            D[i] = sqrt(fabs(X[i] - lo[t]));
+            J[i] = 1.0;
+        }
+    }
+}
+
+// RN_reflect (reflective_newton.c:275-292): fold X back into the box
+__global__ void k_rn_reflect(int nb, const int* __restrict__ idx, const double* __restrict__ lo,
+                             const double* __restrict__ up, double* __restrict__ X) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nb; t += gridDim.x * blockDim.x) {
+        const int i = idx[t];
+        const double l = lo[t], u = up[t];
+        if (!isnan(l)) {
+            if (!isnan(u)) {
+                const double d = 2. * (u - l);
+                double whole;
+                const double w = modf(fabs(X[i] - l) / d, &whole) * d;
+                X[i] = l + (w < d - w ? w : d - w);
+            } else {
+                X[i] = l + fabs(X[i] - l);
+            }
+        } else if (X[i] > u) {
+            X[i] = 2. * u - X[i];
+        }
+    }
+}
+
+// delete_small_diff (reflective_newton.c:476-484)
+__global__ void k_rn_snap(int nb, const int* __restrict__ idx, const double* __restrict__ lo,
+                          const double* __restrict__ up, double* __restrict__ X, double eps) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nb; t += gridDim.x * blockDim.x) {
+        const int i = idx[t];
+        if (!isnan(up[t]) && fabs(X[i] - up[t]) < eps) X[i] = up[t];
+        if (!isnan(lo[t]) && fabs(X[i] - lo[t]) < eps) X[i] = lo[t];
+    }
+}
+
+// is_at_boundary (reflective_newton.c:498-513): block partial counts of
+// bounded entries within 1e-30 of a bound (the caller tests the sum > 0)
+__global__ __launch_bounds__(TPB) void k_rn_touch(int nb, const int* __restrict__ idx,
+                                                  const double* __restrict__ lo, const double* __restrict__ up,
+                                                  const double* __restrict__ X, double* __restrict__ part) {
+    double c = 0.0;
+    for (int t = blockIdx.x * TPB + threadIdx.x; t < nb; t += gridDim.x * TPB) {
+        const int i = idx[t];
+        if ((!isnan(up[t]) && fabs(X[i] - up[t]) < 1e-30) || (!isnan(lo[t]) && fabs(X[i] - lo[t]) < 1e-30)) c += 1.0;
+    }
+    double s = block_sum(c);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// x_i *= y_i (vector_pseudo_mult), x_i *= a (scalar_mult)
+__global__ void k_mul(int n, double* __restrict__ x, const double* __restrict__ y) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] *= y[i];
+}
+__global__ void k_scale(int n, double* __restrict__ x, double a) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] *= a;
+}
+
 // ------------------------------------------------------ coarse solve
 
 // x = inv * b over the dense rows (gathered through idx), trivial rows
@@ -691,6 +767,30 @@ void restore_if_zero(hipStream_t st, int n, double* x, const double* scal, int s
     hipLaunchKernelGGL(k_zero_if, dim3(grid_for(n)), dim3(TPB), 0, st, n, x, scal, slot);
     if (m > 0)
         hipLaunchKernelGGL(k_fix_isolated_if, dim3(grid_for(m)), dim3(TPB), 0, st, m, rows, a, f, x, scal, slot);
+}
+
+void rn_dj(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, const double* G,
+           const double* X, double* D, double* J) {
+    if (nb > 0) hipLaunchKernelGGL(k_rn_dj, dim3(grid_for(nb)), dim3(TPB), 0, st, nb, idx, lo, up, G, X, D, J);
+}
+void rn_reflect(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, double* X) {
+    if (nb > 0) hipLaunchKernelGGL(k_rn_reflect, dim3(grid_for(nb)), dim3(TPB), 0, st, nb, idx, lo, up, X);
+}
+void rn_snap(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, double* X, double eps) {
+    if (nb > 0) hipLaunchKernelGGL(k_rn_snap, dim3(grid_for(nb)), dim3(TPB), 0, st, nb, idx, lo, up, X, eps);
+}
+int rn_touch(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, const double* X,
+             double* part) {
+    if (nb <= 0) return 0;
+    const int g = std::min(grid_for(nb), 1024);
+    hipLaunchKernelGGL(k_rn_touch, dim3(g), dim3(TPB), 0, st, nb, idx, lo, up, X, part);
+    return g;
+}
+void mul(hipStream_t st, int n, double* x, const double* y) {
+    hipLaunchKernelGGL(k_mul, dim3(grid_for(n)), dim3(TPB), 0, st, n, x, y);
+}
+void scale(hipStream_t st, int n, double* x, double a) {
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(TPB), 0, st, n, x, a);
 }
 
 void scale_by(hipStream_t st, int n, double* x, const double* scal, int slot, int inverse) {

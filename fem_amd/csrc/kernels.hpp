@@ -73,6 +73,16 @@ void signs(hipStream_t st, int n, const double* s, const double* x, double* y);
 template <typename T>
 void fix_isolated(hipStream_t st, int m, const int* rows, const double* a, const T* f, T* x);
 void scale_by(hipStream_t st, int n, double* x, const double* scal, int slot, int inverse);
+// reflective Newton box bookkeeping over the bounded index set (NaN bound: unbounded)
+void rn_dj(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, const double* G,
+           const double* X, double* D, double* J);
+void rn_reflect(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, double* X);
+void rn_snap(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, double* X, double eps);
+// block partial counts of entries within 1e-30 of a bound; returns #partials
+int rn_touch(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, const double* X,
+             double* part);
+void mul(hipStream_t st, int n, double* x, const double* y);  // x_i *= y_i
+void scale(hipStream_t st, int n, double* x, double a);       // x_i *= a
 // when scal[slot] == 0: x = 0, then x_i = f_i / a_i on the isolated rows
 // (the guess AMG_solve returns untouched when its first residual is zero)
 void restore_if_zero(hipStream_t st, int n, double* x, const double* scal, int slot, int m, const int* rows,

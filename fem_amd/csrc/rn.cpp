@@ -1,0 +1,760 @@
+// Reflective Newton (LinA/reflective_newton.c) on the device.
+//
+// reflective_newton (:660-711) minimises q(x) = x.H.x/2 + x.b over a box by
+// trust-region steps in the scaled variables of Coleman-Li: D_i =
+// sqrt|x_i - bound_i| on bounded entries, M = D H D + diag(|g| J).  Each step
+// (RN_iterate, :536-627) solves M v = -D g with an inner Krylov method, takes
+// the Newton step if it is inside the trust region and otherwise the minimum
+// of q on the circle of radius Delta in span{D g, v} (subspace_minimum,
+// :394-450, roots of dq/dtheta by GMRES_Newton.c:353-439), reflects the
+// iterate into the box, bisects until the sigma_l/sigma_u tests hold and
+// updates the trust radius from the achieved/predicted reduction.
+//
+// Device: every length-n vector (gradient, scalings, subspace basis, steps,
+// iterates), the SpMVs with H and M, dots and the inner solvers (ILU(0)-PCG,
+// BiCGStab + AMG, AMG_solve).  Host: the scalars that steer the method and
+// the assembly of M with the reference's zero deletion, because the ILU(0)
+// pattern and the Galerkin products depend on it.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "../../include/lina_hip.h"
+#include "ilu.hpp"
+#include "kernels.hpp"
+#include "lina_internal.hpp"
+#include "solver.hpp"
+
+namespace lina {
+
+namespace {
+
+// ------------------------------------------------------------ host math
+
+// 1-D root search of subspace_minimum: the functions of GMRES_Newton.c with
+// the reference's constants.  `rand()` in the secant fallback is replaced by
+// a private linear congruential sequence (the reference's depends on the
+// process's earlier rand() calls).
+struct RootSearch {
+    double ax, ay, axy, bx, by, delta;
+    uint64_t lcg = 0x2545F4914F6CDD1DULL;
+
+    double uniform01() {
+        lcg = lcg * 6364136223846793005ULL + 1442695040888963407ULL;
+        return (double)(lcg >> 11) * (1.0 / 9007199254740992.0);
+    }
+    // RN_derivative (reflective_newton.c:372-374)
+    double F(double x) const {
+        return ((std::sin(2. * x) * (-ax + ay) + std::cos(2. * x) * axy) * delta - std::sin(x) * bx +
+                std::cos(x) * by) *
+               delta;
+    }
+    // quadratic_on_circle (:368-371)
+    double q(double x) const {
+        double c = std::cos(x) * delta;
+        double s = std::sin(x) * delta;
+        return c * (c * ax + s * axy + bx) + s * (s * ay + by);
+    }
+    // der_F / der_poly (GMRES_Newton.c:320-334): central differences, dx = 1e-6
+    double der_F(double x, int depth) const {
+        const double dx = 1e-6;
+        if (depth == 0) return F(x);
+        return (der_F(x + dx, depth - 1) - der_F(x - dx, depth - 1)) / (2. * dx);
+    }
+    static double root_poly(const std::vector<double>& r, double x) {
+        double res = 1.;
+        for (double v : r) res *= (x - v);
+        return res;
+    }
+    static double der_poly(const std::vector<double>& r, double x, int depth) {
+        const double dx = 1e-6;
+        if (depth == 0) return root_poly(r, x);
+        return (der_poly(r, x + dx, depth - 1) - der_poly(r, x - dx, depth - 1)) / (2. * dx);
+    }
+    // F_hospital (:336-351): F deflated by the roots found so far, by
+    // l'Hospital's rule where both vanish.  The depth is capped at 24 (the
+    // reference loops without bound on an identically vanishing F).
+    double hospital(const std::vector<double>& roots, double x) const {
+        const double eps = 1e-5;
+        int depth = 0;
+        double dp, df;
+        do {
+            dp = der_poly(roots, x, depth);
+            df = der_F(x, depth);
+            depth++;
+        } while (std::fabs(dp) < eps && std::fabs(df) < eps && depth < 24);
+        return df / dp;
+    }
+    // Newton_1D (:353-400): secant iterations with deflation until one fails
+    std::vector<double> newton(double mn, double mx, int max_iter, double eps) {
+        std::vector<double> roots;
+        int i;
+        do {
+            double local_eps = eps;
+            double x0 = mn, x1 = mx;
+            double f0 = hospital(roots, x0);
+            double f1 = hospital(roots, x1);
+            double F0 = f0 != 0 ? f0 : 1.;
+            double x = 0, f = 0;
+            i = 0;
+            do {
+                x = x1 - f1 * (x1 - x0) / (f1 - f0);
+                if (x < mn || x > mx) x = mn + (mx - mn) * uniform01();
+                f = hospital(roots, x);
+                x0 = x1;
+                x1 = x;
+                f0 = f1;
+                f1 = f;
+                i++;
+                if (i == max_iter / 2) local_eps *= 1e2;
+            } while (std::fabs(f / F0) > local_eps && i < max_iter);
+            if (i < max_iter) roots.push_back(x);
+        } while (i < max_iter);
+        return roots;
+    }
+    // Interval_Separation_Newton_1D (:402-439) on [0, 2 pi]
+    std::vector<double> roots() {
+        const int int_num = 1000, max_iter = 500;
+        const double eps = 1e-6, mn = 0, mx = 2. * M_PI;
+        const double d = (double)(mx - mn) / int_num;
+        std::vector<double> out;
+        for (int i = 0; i < int_num; i++) {
+            double l = i * d;
+            double r = l + d;
+            double fl = F(l), fr = F(r);
+            if (fl * fr < 0) {
+                std::vector<double> loc = newton(l, r, max_iter, eps);
+                // appended in reverse order of discovery, as the reference does
+                size_t base = out.size();
+                out.resize(base + loc.size());
+                for (size_t j = 0; j < loc.size(); j++) out[out.size() - j - 1] = loc[j];
+            } else {
+                if (fl == 0) out.push_back(fl);
+                if (fr == 0) out.push_back(fr);
+            }
+        }
+        return out;
+    }
+};
+
+uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+// RN_get_M_D's product D H D (linear_algebra.c:186-222 twice, then
+// delete_zeros) plus insert_sparse(M, |g_i| J_i, i, i) (reflective_newton.c:
+// 250-262), with the reference's rounding: M_ij = D_i (H_ij D_j), the
+// diagonal accumulated onto sparse_zero's 0.0 seed.
+Csr scaled_hessian(const Csr& H, const std::vector<double>& D, const std::vector<double>& G,
+                   const std::vector<double>& J) {
+    const int n = H.n;
+    Csr M;
+    M.n = n;
+    M.ncols = n;
+    M.rp.assign(n + 1, 0);
+    M.ci.reserve(H.ci.size());
+    M.v.reserve(H.v.size());
+    std::vector<int> hc;
+    std::vector<double> hv;
+    for (int i = 0; i < n; i++) {
+        // HD row i: seed (i, 0.0); a = H_ik != 0, b = D_k != 0 -> a*b
+        hc.clear();
+        hv.clear();
+        bool seeded = false;
+        for (int p = H.rp[i]; p < H.rp[i + 1]; p++) {
+            const int k = H.ci[p];
+            if (!seeded && k >= i) {
+                if (k > i) {
+                    hc.push_back(i);
+                    hv.push_back(0.0);
+                }
+                seeded = true;
+            }
+            const double a = H.v[p];
+            const double b = D[k];
+            double val = (a != 0 && b != 0) ? a * b : 0.0;
+            if (k == i) {
+                hc.push_back(i);
+                hv.push_back(0.0 + val);
+            } else if (a != 0 && b != 0) {
+                hc.push_back(k);
+                hv.push_back(val);
+            }
+        }
+        if (!seeded) {
+            hc.push_back(i);
+            hv.push_back(0.0);
+        }
+        // delete_zeros on HD's row (a row of zeros keeps (i, 0.0))
+        {
+            size_t w = 0;
+            if (hc.size() > 1) {
+                for (size_t t = 0; t < hc.size(); t++)
+                    if (hv[t] != 0) {
+                        hc[w] = hc[t];
+                        hv[w] = hv[t];
+                        w++;
+                    }
+                if (w == 0) {
+                    hc[0] = i;
+                    hv[0] = 0.0;
+                    w = 1;
+                }
+                hc.resize(w);
+                hv.resize(w);
+            }
+        }
+        // M row i = D_i * HD row i on the seed (i, 0.0), then delete_zeros
+        const int start = (int)M.ci.size();
+        const double di = D[i];
+        bool mseeded = false;
+        for (size_t t = 0; t < hc.size(); t++) {
+            const int j = hc[t];
+            if (!mseeded && j >= i) {
+                if (j > i) {
+                    M.ci.push_back(i);
+                    M.v.push_back(0.0);
+                }
+                mseeded = true;
+            }
+            const double b = hv[t];
+            if (j == i) {
+                M.ci.push_back(i);
+                M.v.push_back(di != 0 && b != 0 ? 0.0 + di * b : 0.0);
+            } else if (di != 0 && b != 0) {
+                M.ci.push_back(j);
+                M.v.push_back(di * b);
+            }
+        }
+        if (!mseeded) {
+            M.ci.push_back(i);
+            M.v.push_back(0.0);
+        }
+        int len = (int)M.ci.size() - start;
+        if (len > 1) {
+            int w = start;
+            for (int t = start; t < start + len; t++)
+                if (M.v[t] != 0) {
+                    M.ci[w] = M.ci[t];
+                    M.v[w] = M.v[t];
+                    w++;
+                }
+            if (w == start) {
+                M.ci[w] = i;
+                M.v[w] = 0.0;
+                w++;
+            }
+            M.ci.resize(w);
+            M.v.resize(w);
+        }
+        // insert_sparse(M, |g_i| J_i, i, i)
+        if (J[i] != 0) {
+            const double add = std::fabs(G[i]) * J[i];
+            int pos = start, end = (int)M.ci.size();
+            while (pos < end && M.ci[pos] < i) pos++;
+            if (pos < end && M.ci[pos] == i) {
+                M.v[pos] += add;
+            } else {
+                M.ci.insert(M.ci.begin() + pos, i);
+                M.v.insert(M.v.begin() + pos, add);
+            }
+        }
+        M.rp[i + 1] = (int)M.ci.size();
+    }
+    return M;
+}
+
+class ReflectiveNewton {
+  public:
+    ReflectiveNewton(const Csr& H, const double* b, int nb, const int* idx, const double* lo, const double* up,
+                     const lina_rn_params& p)
+        : H_(H), b_(b, b + H.n), p_(p), delta_(p.delta), seed_(p.seed) {
+        n_ = H.n;
+        idx_.assign(idx, idx + nb);
+        lo_.assign(lo, lo + nb);
+        up_.assign(up, up + nb);
+        for (int k : idx_)
+            if (k < 0 || k >= n_) throw std::invalid_argument("bound index out of range");
+        dH_.reset(new DevMatrix(H_));
+        st_ = dH_->stream();
+        B_.reset(new DevBuffers());
+        auto alloc = [&](size_t m) {
+            void* q = nullptr;
+            hip_check(hipMalloc(&q, std::max<size_t>(m, 1) * sizeof(double)), "hipMalloc");
+            B_->ptrs.push_back(q);
+            return (double*)q;
+        };
+        for (double** v : {&X_, &Xo_, &G_, &D_, &J_, &V1_, &V2_, &bs_, &Bv_, &S_, &T_, &T2_}) *v = alloc(n_);
+        part_ = alloc(std::max(8192, dH_->nblocks() + 16));
+        scal_ = alloc(16);
+        lod_ = alloc(nb);
+        upd_ = alloc(nb);
+        void* q = nullptr;
+        hip_check(hipMalloc(&q, std::max(nb, 1) * sizeof(int)), "hipMalloc");
+        B_->ptrs.push_back(q);
+        idxd_ = (int*)q;
+        hip_check(hipMemcpyAsync(idxd_, idx_.data(), nb * sizeof(int), hipMemcpyHostToDevice, st_), "h2d");
+        hip_check(hipMemcpyAsync(lod_, lo_.data(), nb * sizeof(double), hipMemcpyHostToDevice, st_), "h2d");
+        hip_check(hipMemcpyAsync(upd_, up_.data(), nb * sizeof(double), hipMemcpyHostToDevice, st_), "h2d");
+        hip_check(hipMemcpyAsync(bs_, b_.data(), n_ * sizeof(double), hipMemcpyHostToDevice, st_), "h2d");
+        hip_check(hipHostMalloc((void**)&host_, 16 * sizeof(double)), "hipHostMalloc");
+        bdev_ = alloc(n_);
+        hip_check(hipMemcpyAsync(bdev_, b_.data(), n_ * sizeof(double), hipMemcpyHostToDevice, st_), "h2d");
+        if (p_.solver == LINA_RN_BICGSTAB || p_.solver == LINA_RN_AMG) {
+            amg_.reset(new Solver());
+            amg_->setup_host(H_, p_.amg);
+            amg_->upload();
+        }
+        hip_check(hipStreamSynchronize(st_), "rn upload");
+    }
+
+    ~ReflectiveNewton() {
+        if (st_) (void)hipStreamSynchronize(st_);
+        if (host_) (void)hipHostFree(host_);
+    }
+
+    void set_delta(double d) { delta_ = d; }
+
+    int solve(double* x, lina_rn_stats* stats) {
+        const int n = n_;
+        stats_ = lina_rn_stats{};
+        hip_check(hipMemcpyAsync(X_, x, n * sizeof(double), hipMemcpyHostToDevice, st_), "h2d");
+        double q0 = functional(X_);
+        int iter = 0;
+        double q = q0, dq = 0;
+        std::vector<double> hD(n), hG(n), hJ(n);
+        do {
+            // G = b + H X (linear_map(G, 1, H, X) on a copy of b)
+            dev::copy(st_, n, (const double*)bdev_, G_);
+            dH_->launch(X_, nullptr, G_, dev::MODE_ADD, nullptr, nullptr);
+            dev::fill<double>(st_, n, D_, 1.0);
+            dev::fill<double>(st_, n, J_, 0.0);
+            dev::rn_dj(st_, (int)idx_.size(), idxd_, lod_, upd_, G_, X_, D_, J_);
+            download(D_, hD);
+            download(G_, hG);
+            download(J_, hJ);
+            Csr M = scaled_hessian(H_, hD, hG, hJ);
+            iterate(M);
+            q = functional(X_);
+            dq = std::fabs((q - q0) / q0);
+            iter++;
+            q0 = q;
+        } while (iter < p_.max_iter && dq > p_.eps);
+        dev::rn_snap(st_, (int)idx_.size(), idxd_, lod_, upd_, X_, 1e-3 * p_.eps);
+        hip_check(hipMemcpyAsync(x, X_, n * sizeof(double), hipMemcpyDeviceToHost, st_), "d2h");
+        hip_check(hipStreamSynchronize(st_), "sync");
+        stats_.iterations = iter;
+        stats_.dq = dq;
+        stats_.q = q;
+        stats_.delta = delta_;
+        if (stats) *stats = stats_;
+        return iter;
+    }
+
+  private:
+    // ------------------------------------------------------- device helpers
+    void download(const double* d, std::vector<double>& h) {
+        hip_check(hipMemcpyAsync(h.data(), d, n_ * sizeof(double), hipMemcpyDeviceToHost, st_), "d2h");
+        hip_check(hipStreamSynchronize(st_), "sync");
+    }
+    double read(int slot) {
+        hip_check(hipMemcpyAsync(host_, scal_ + slot, sizeof(double), hipMemcpyDeviceToHost, st_), "d2h");
+        hip_check(hipStreamSynchronize(st_), "sync");
+        return host_[0];
+    }
+    double dot(const double* a, const double* b) {
+        int np = dev::dot_part<double>(st_, n_, a, b, part_);
+        dev::finalize(st_, np, part_, scal_, 0, 0);
+        return read(0);
+    }
+    double norm(const double* a) { return std::sqrt(dot(a, a)); }
+    void axpy(double* x, double f, const double* y) { dev::axpby(st_, n_, 1.0, x, f, y); }  // vector_add
+    // sparse_bilinear(y, A, x) = y . (A x), A given as a DevMatrix
+    double bilinear(DevMatrix& A, const double* y, const double* x) {
+        sync_with(A);
+        int np = A.launch(x, nullptr, T2_, dev::MODE_SET, part_, y);
+        dev::finalize(A.stream(), np, part_, scal_, 1, 0);
+        hip_check(hipMemcpyAsync(host_, scal_ + 1, sizeof(double), hipMemcpyDeviceToHost, A.stream()), "d2h");
+        hip_check(hipStreamSynchronize(A.stream()), "sync");
+        return host_[0];
+    }
+    void sync_with(DevMatrix& A) {
+        if (A.stream() != st_) hip_check(hipStreamSynchronize(st_), "sync");
+    }
+    // RN_default_func: x.H.x/2 + b.x
+    double functional(const double* x) {
+        int np = dH_->launch(x, nullptr, T_, dev::MODE_SET, part_, x);
+        dev::finalize(st_, np, part_, scal_, 2, 0);
+        np = dev::dot_part<double>(st_, n_, bdev_, x, part_ + 4096);
+        dev::finalize(st_, np, part_ + 4096, scal_, 3, 0);
+        hip_check(hipMemcpyAsync(host_, scal_ + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st_), "d2h");
+        hip_check(hipStreamSynchronize(st_), "sync");
+        return host_[0] / 2. + host_[1];
+    }
+    void normalize(double* v) {
+        double a = norm(v);
+        dev::scale(st_, n_, v, (double)1 / a);
+    }
+
+    // ------------------------------------------------------ inner solvers
+    // RN_get_subspace (reflective_newton.c:294-366): V1 = D g, V2 solves
+    // M V2 = -V1 from the start V2 = -V1; returns POSITIVE_DEFINITE (1) or
+    // INDEFINITE (0)
+    int subspace(const Csr& M, DevMatrix& dM) {
+        const int n = n_;
+        dev::copy(st_, n, (const double*)G_, V1_);
+        dev::mul(st_, n, V1_, D_);
+        dev::copy(st_, n, (const double*)V1_, V2_);
+        dev::scale(st_, n, V2_, -1.);
+        dev::copy(st_, n, (const double*)V2_, bs_);
+        hip_check(hipStreamSynchronize(st_), "sync");
+        int status = 1, iter = 0;
+        switch (p_.solver) {
+            case LINA_RN_PCG: {
+                IluSystem sys;
+                sys.build(M, true);
+                if (!sys.ilu_ok()) {
+                    status = 0;
+                    break;
+                }
+                double res = 0;
+                iter = sys.pcg_dev(V2_, bs_, p_.eps, p_.pcg_iter, LINA_STOP_REFERENCE, &res);
+                status = iter < p_.pcg_iter ? 1 : 0;
+                break;
+            }
+            case LINA_RN_BICGSTAB: {
+                amg_->recompute_matrix(M);
+                double res = 0;
+                iter = amg_->bicgstab_dev(V2_, bs_, p_.bi_eps, p_.bi_max_iter, &res);
+                hip_check(hipStreamSynchronize(amg_->stream()), "sync");
+                status = iter < p_.bi_max_iter ? 1 : 0;
+                break;
+            }
+            case LINA_RN_AMG: {
+                amg_->recompute_matrix(M);
+                iter = amg_->amg_solve_dev(bs_, V2_);
+                hip_check(hipStreamSynchronize(amg_->stream()), "sync");
+                status = iter < p_.amg.amg_max_iter ? 1 : 0;
+                break;
+            }
+            default:
+                throw std::invalid_argument("unknown inner solver");
+        }
+        stats_.inner_iterations += iter;
+        if (status == 0) {
+            stats_.neg_curvature++;
+            neg_curvature(M, dM);
+        }
+        return status;
+    }
+
+    // power_method (linear_algebra.c:4802-4822) on A; the first eigenvalue
+    // estimate starts at 0 (the reference reads an uninitialised variable)
+    double power(DevMatrix& A, const Csr& Ah, double* v, int max_iter, double eps) {
+        double q = 0;
+        for (double a : Ah.v) q += std::fabs(a);
+        q = q / Ah.n;  // matrix_norm (:3441-3450)
+        double eig = 0, prev;
+        int i = 0;
+        double r;
+        do {
+            prev = eig;
+            dev::copy(st_, n_, (const double*)v, T_);
+            sync_with(A);
+            A.launch(T_, nullptr, v, dev::MODE_SET, nullptr, nullptr);
+            hip_check(hipStreamSynchronize(A.stream()), "sync");
+            eig = dot(v, T_);
+            double s = 1. / norm(v);
+            dev::scale(st_, n_, v, s);
+            r = std::fabs(eig - prev) / q;
+            i++;
+        } while (r > eps && i < max_iter);
+        return eig;
+    }
+    void random_unit(double* v) {
+        std::vector<double> h(n_);
+        for (int i = 0; i < n_; i++) h[i] = (double)(splitmix(seed_) >> 11) * (1.0 / 9007199254740992.0);
+        hip_check(hipMemcpyAsync(v, h.data(), n_ * sizeof(double), hipMemcpyHostToDevice, st_), "h2d");
+        hip_check(hipStreamSynchronize(st_), "sync");
+        normalize(v);
+    }
+    // get_neg_curvature_direction (linear_algebra.c:4849-4865): the largest
+    // eigenvalue s of M, then the dominant eigenvector of M - s I
+    void neg_curvature(const Csr& M, DevMatrix& dM) {
+        const int eig_iter = 10000;
+        const double eig_eps = 1e-6;
+        random_unit(V2_);
+        double shift = power(dM, M, V2_, 1000, eig_eps);
+        Csr Bm = M;
+        for (int i = 0; i < Bm.n; i++) {
+            int pos = Bm.rp[i];
+            while (pos < Bm.rp[i + 1] && Bm.ci[pos] < i) pos++;
+            if (pos < Bm.rp[i + 1] && Bm.ci[pos] == i) {
+                Bm.v[pos] += -shift;
+            } else {
+                Bm.ci.insert(Bm.ci.begin() + pos, i);
+                Bm.v.insert(Bm.v.begin() + pos, -shift);
+                for (int k = i + 1; k <= Bm.n; k++) Bm.rp[k]++;
+            }
+        }
+        DevMatrix dB(Bm);
+        random_unit(V2_);
+        double curv = power(dB, Bm, V2_, eig_iter, eig_eps) + shift;
+        last_curvature_ = curv;
+    }
+
+    // ---------------------------------------------------------- one step
+    // RN_min (reflective_newton.c:57-59)
+    static double rn_min(double a, double b) { return a < b ? a : b; }
+    // RN_test_sigma_l / _u (reflective_newton.c:460-470)
+    double bound_term(double alpha, double gs, double shs, double sigma) const {
+        return sigma * alpha * (gs + alpha * rn_min(shs, 0) / 2.);
+    }
+
+    void step_to(double alpha) {
+        dev::copy(st_, n_, (const double*)Xo_, X_);
+        axpy(X_, alpha, S_);
+        dev::rn_reflect(st_, (int)idx_.size(), idxd_, lod_, upd_, X_);
+    }
+
+    bool touches_bound() {
+        int np = dev::rn_touch(st_, (int)idx_.size(), idxd_, lod_, upd_, X_, part_);
+        if (np == 0) return false;
+        dev::finalize(st_, np, part_, scal_, 4, 0);
+        return read(4) > 0;
+    }
+
+    // RN_iterate (reflective_newton.c:536-627)
+    void iterate(const Csr& M) {
+        const int n = n_;
+        dev::copy(st_, n, (const double*)X_, Xo_);
+        DevMatrix dM(M);
+        int status = subspace(M, dM);
+        // B = D g
+        dev::copy(st_, n, (const double*)G_, Bv_);
+        dev::mul(st_, n, Bv_, D_);
+        const double nB = norm(Bv_);
+        if (nB == 0) return;
+        if (status == 1 && norm(V2_) < delta_) {
+            dev::copy(st_, n, (const double*)V2_, S_);
+        } else {
+            normalize(V1_);
+            double s = dot(V1_, V2_);
+            axpy(V2_, -s, V1_);
+            normalize(V2_);
+            subspace_minimum(dM);
+        }
+        dev::mul(st_, n, S_, D_);
+        // fixed quantities of the line search
+        const double fo = functional(Xo_);
+        const double gs = dot(G_, S_);
+        const double shs = bilinear(*dH_, S_, S_);
+        double alpha = 1.0;
+        axpy(X_, alpha, S_);
+        dev::rn_reflect(st_, (int)idx_.size(), idxd_, lod_, upd_, X_);
+        if (!(functional(X_) - fo < bound_term(alpha, gs, shs, p_.sigma_l))) {
+            stats_.line_searches++;
+            const int max_iter = 10;
+            int it = 0;
+            double al = 0, ar = 1.;
+            do {
+                alpha = (al + ar) / 2.;
+                step_to(alpha);
+                double dqv = functional(X_) - fo;
+                if (!(dqv < bound_term(alpha, gs, shs, p_.sigma_l))) {
+                    ar = alpha;
+                } else {
+                    if (!(dqv > bound_term(alpha, gs, shs, p_.sigma_u)) && !(alpha > p_.rho))
+                        al = alpha;
+                    else
+                        break;
+                }
+                it++;
+            } while (it < max_iter);
+            stats_.line_search_iter += it;
+        }
+        if (touches_bound()) {
+            alpha -= p_.xi * nB;
+            step_to(alpha);
+        }
+        const double d = functional(X_) - fo;
+        const double pred = gs + rn_min(shs, 0);  // predicted_reduction (:472-474)
+        const double contraction = d / pred;
+        if (contraction > p_.trust_upper && d < 0) delta_ = rn_min(p_.radius_max, 2. * delta_);
+        if (contraction < p_.trust_lower || d >= 0) delta_ = p_.radius_min > delta_ / 2. ? p_.radius_min : delta_ / 2.;
+    }
+
+    // subspace_minimum (reflective_newton.c:394-450) with b = B, basis V1, V2
+    void subspace_minimum(DevMatrix& dM) {
+        RootSearch rs;
+        rs.bx = dot(V1_, Bv_);
+        rs.by = dot(V2_, Bv_);
+        rs.ax = bilinear(dM, V1_, V1_);
+        rs.ay = bilinear(dM, V2_, V2_);
+        rs.axy = bilinear(dM, V1_, V2_) + bilinear(dM, V2_, V1_);
+        rs.delta = delta_;
+        std::vector<double> roots = rs.roots();
+        if (roots.empty()) throw NumericError("reflective Newton: no extremum on the trust-region circle");
+        size_t m = 0;
+        double mv = rs.q(roots[0]);
+        for (size_t i = 1; i < roots.size(); i++)
+            if (rs.q(roots[i]) < mv) {
+                m = i;
+                mv = rs.q(roots[i]);
+            }
+        dev::copy(st_, n_, (const double*)V1_, S_);
+        dev::scale(st_, n_, S_, delta_ * std::cos(roots[m]));
+        axpy(S_, delta_ * std::sin(roots[m]), V2_);
+    }
+
+    Csr H_;
+    std::vector<double> b_, lo_, up_;
+    std::vector<int> idx_;
+    lina_rn_params p_;
+    double delta_;
+    uint64_t seed_;
+    int n_ = 0;
+    std::unique_ptr<DevMatrix> dH_;
+    std::unique_ptr<Solver> amg_;
+    std::unique_ptr<DevBuffers> B_;
+    hipStream_t st_ = nullptr;
+    double *X_ = nullptr, *Xo_ = nullptr, *G_ = nullptr, *D_ = nullptr, *J_ = nullptr, *V1_ = nullptr,
+           *V2_ = nullptr, *bs_ = nullptr, *Bv_ = nullptr, *S_ = nullptr, *T_ = nullptr, *T2_ = nullptr;
+    double *part_ = nullptr, *scal_ = nullptr, *lod_ = nullptr, *upd_ = nullptr, *bdev_ = nullptr;
+    int* idxd_ = nullptr;
+    double* host_ = nullptr;
+    double last_curvature_ = 0;
+    lina_rn_stats stats_{};
+};
+
+}  // namespace lina
+
+// ------------------------------------------------------------------ C ABI
+
+struct lina_rn {
+    std::unique_ptr<lina::ReflectiveNewton> rn;
+};
+
+namespace {
+template <typename F>
+int rn_guard(F&& f) {
+    try {
+        return f();
+    } catch (const lina::HipError& e) {
+        lina::set_error("%s", e.what());
+        return LINA_EHIP;
+    } catch (const lina::NumericError& e) {
+        lina::set_error("%s", e.what());
+        return LINA_ENUMERIC;
+    } catch (const std::invalid_argument& e) {
+        lina::set_error("%s", e.what());
+        return LINA_EARG;
+    } catch (const std::exception& e) {
+        lina::set_error("%s", e.what());
+        return LINA_ESETUP;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+void lina_rn_params_default(lina_rn_params* p, int dof, int depth) {
+    p->max_iter = 50;
+    p->eps = 1e-8;
+    p->sigma_l = 0.1;
+    p->sigma_u = 0.9;
+    p->rho = 0.1;
+    p->delta = 250.;
+    p->xi = 1.;
+    p->trust_lower = 0.6;
+    p->trust_upper = 1.1;
+    p->radius_min = 1e-3;
+    p->radius_max = 1000.;
+    p->solver = LINA_RN_BICGSTAB;
+    p->pcg_iter = 100;
+    p->bi_eps = 1e-3;
+    p->bi_max_iter = 10;
+    lina_params_reference(&p->amg, dof, depth);
+    p->amg.omega = 1.0;
+    p->amg.amg_eps = 1e-4;
+    p->amg.amg_max_iter = 20;
+    p->seed = 0x5EED0007ULL;
+}
+
+lina_rn* lina_rn_create(int n, const int* rowptr, const int* col, const double* val, const double* b, int nb,
+                        const int* idx, const double* lower, const double* upper, const lina_rn_params* p) {
+    lina_rn* h = nullptr;
+    int rc = rn_guard([&]() {
+        if (n <= 0 || !rowptr || !col || !val || !b || !p || nb < 0 || (nb > 0 && (!idx || !lower || !upper)))
+            throw std::invalid_argument("lina_rn_create: bad arguments");
+        if (p->solver < LINA_RN_BICGSTAB || p->solver > LINA_RN_PCG) throw std::invalid_argument("unknown solver");
+        lina::Csr H;
+        H.n = n;
+        H.ncols = n;
+        H.rp.assign(rowptr, rowptr + n + 1);
+        if (H.rp[0] != 0) throw std::invalid_argument("rowptr[0] != 0");
+        for (int i = 0; i < n; i++)
+            if (H.rp[i + 1] <= H.rp[i]) throw std::invalid_argument("every row needs at least one entry");
+        H.ci.assign(col, col + H.rp[n]);
+        H.v.assign(val, val + H.rp[n]);
+        for (int i = 0; i < n; i++)
+            for (int q = H.rp[i]; q < H.rp[i + 1]; q++)
+                if (H.ci[q] < 0 || H.ci[q] >= n || (q > H.rp[i] && H.ci[q] <= H.ci[q - 1]))
+                    throw std::invalid_argument("columns out of range or not sorted");
+        h = new lina_rn();
+        h->rn.reset(new lina::ReflectiveNewton(H, b, nb, idx, lower, upper, *p));
+        return 0;
+    });
+    if (rc < 0) {
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+void lina_rn_destroy(lina_rn* h) { delete h; }
+
+int lina_rn_solve(lina_rn* h, double* x, lina_rn_stats* st) {
+    return rn_guard([&]() {
+        if (!h || !x) throw std::invalid_argument("null argument");
+        return h->rn->solve(x, st);
+    });
+}
+
+int lina_rn_scaled_hessian(int n, const int* rowptr, const int* col, const double* val, const double* D,
+                           const double* G, const double* J, int* mrp, int* mci, double* mv) {
+    return rn_guard([&]() {
+        lina::Csr H;
+        H.n = n;
+        H.ncols = n;
+        H.rp.assign(rowptr, rowptr + n + 1);
+        H.ci.assign(col, col + H.rp[n]);
+        H.v.assign(val, val + H.rp[n]);
+        std::vector<double> d(D, D + n), g(G, G + n), j(J, J + n);
+        lina::Csr M = lina::scaled_hessian(H, d, g, j);
+        std::copy(M.rp.begin(), M.rp.end(), mrp);
+        std::copy(M.ci.begin(), M.ci.end(), mci);
+        std::copy(M.v.begin(), M.v.end(), mv);
+        return M.nnz();
+    });
+}
+
+int lina_rn_set_trust_region(lina_rn* h, double delta) {
+    return rn_guard([&]() {
+        if (!h) throw std::invalid_argument("null handle");
+        h->rn->set_delta(delta);
+        return 0;
+    });
+}
+
+}  // extern "C"

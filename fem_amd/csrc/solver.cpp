@@ -360,6 +360,17 @@ void Solver::recompute(const std::vector<double>& val) {
     if (uploaded()) upload();
 }
 
+void Solver::recompute_matrix(const Csr& M) {
+    if (M.n != n_) throw std::invalid_argument("recompute: matrix size differs from the hierarchy");
+    Csr A = M;
+    for (int i = 0; i < A.n; i++)
+        if (H_.signs[i] < 0)
+            for (int p = A.rp[i]; p < A.rp[i + 1]; p++) A.v[p] = -A.v[p];
+    amg_recompute(H_, A);
+    for (size_t l = 1; l < H_.lv.size(); l++) colours_[l] = greedy_colouring(H_.lv[l].A, &ncolours_[l]);
+    if (uploaded()) upload();
+}
+
 void Solver::set_kernel_timing(bool on) {
     if (on && !evk0_) {
         hip_check(hipEventCreate(&evk0_), "hipEventCreate");
@@ -807,6 +818,10 @@ DevMatrix::~DevMatrix() {
 
 void DevMatrix::apply(const double* x, double* y) {
     dev::spmv<double, double>(stream_, A_, x, nullptr, y, dev::MODE_OUTER, nullptr, nullptr);
+}
+
+int DevMatrix::launch(const double* x, const double* b, double* y, int mode, double* part, const double* w) {
+    return dev::spmv<double, double>(stream_, A_, x, b, y, mode, part, w);
 }
 
 void DevMatrix::spmv_dev(const double* x, double* y) {

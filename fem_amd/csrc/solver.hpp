@@ -60,6 +60,10 @@ class DevMatrix {
     int n() const { return n_; }
     long nnz() const { return nnz_; }
     void apply(const double* x, double* y);  // enqueue on stream(), FP64
+    // y = A x / b - A x / y + A x (mode) on stream(); part != nullptr: block
+    // partials of sum_r y_r w_r, returns their number
+    int launch(const double* x, const double* b, double* y, int mode, double* part, const double* w);
+    int nblocks() const { return A_.nblk; }
     void spmv_dev(const double* x, double* y);
     void spmv_host(int precision, const double* x, double* y);
     double time_spmv(int precision, int reps);
@@ -85,6 +89,8 @@ class Solver {
     void setup_host(const Csr& A, const lina_amg_params& p);
     void upload();
     void recompute(const std::vector<double>& val);
+    // AMG_recompute_coarse_matrices with a new finest matrix of any pattern
+    void recompute_matrix(const Csr& M);
     void set_params(const lina_amg_params& p) { prm_ = p; }
     const lina_amg_params& params() const { return prm_; }
     const HostHierarchy& host() const { return H_; }

@@ -38,6 +38,26 @@ class Stats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class RNParams(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("eps", C.c_double), ("sigma_l", C.c_double), ("sigma_u", C.c_double),
+                ("rho", C.c_double), ("delta", C.c_double), ("xi", C.c_double), ("trust_lower", C.c_double),
+                ("trust_upper", C.c_double), ("radius_min", C.c_double), ("radius_max", C.c_double),
+                ("solver", C.c_int), ("pcg_iter", C.c_int), ("bi_eps", C.c_double), ("bi_max_iter", C.c_int),
+                ("amg", Params), ("seed", C.c_ulonglong)]
+
+
+class RNStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("line_searches", C.c_int), ("line_search_iter", C.c_int),
+                ("neg_curvature", C.c_int), ("inner_iterations", C.c_int), ("dq", C.c_double), ("q", C.c_double),
+                ("delta", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+RN_SOLVER = {"bicgstab": 1, "amg": 2, "pcg": 3}
+
+
 class LinaError(RuntimeError):
     pass
 
@@ -91,6 +111,13 @@ def lib():
     L.lina_matrix_time_spmv.restype = C.c_double
     L.lina_matrix_spmv_bytes.argtypes = [C.c_void_p, C.c_int]
     L.lina_matrix_spmv_bytes.restype = C.c_double
+    L.lina_rn_params_default.argtypes = [C.POINTER(RNParams), C.c_int, C.c_int]
+    L.lina_rn_create.argtypes = [C.c_int, _ip, _ip, _dp, _dp, C.c_int, _ip, _dp, _dp, C.POINTER(RNParams)]
+    L.lina_rn_create.restype = C.c_void_p
+    L.lina_rn_destroy.argtypes = [C.c_void_p]
+    L.lina_rn_solve.argtypes = [C.c_void_p, _dp, C.POINTER(RNStats)]
+    L.lina_rn_set_trust_region.argtypes = [C.c_void_p, C.c_double]
+    L.lina_rn_scaled_hessian.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _ip, _ip, _dp]
     _lib = L
     return L
 
@@ -391,3 +418,73 @@ def bicgstab(amg: "AMG", b, x0=None, eps=1e-8, max_iter=1000):
     res = C.c_double()
     it = _check(L.lina_bicgstab(amg.h, _d(x), _d(b), eps, max_iter, C.byref(res)))
     return {"iters": it, "x": x, "res": res.value}
+
+
+def rn_params(solver="bicgstab", dof=1, depth=6, **kw) -> RNParams:
+    p = RNParams()
+    lib().lina_rn_params_default(C.byref(p), dof, depth)
+    p.solver = RN_SOLVER[solver] if isinstance(solver, str) else solver
+    for k, v in kw.items():
+        if k == "amg":
+            for kk, vv in v.items():
+                setattr(p.amg, kk, vv)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class ReflectiveNewton:
+    """reflective_newton on q(x) = x.H.x/2 + x.b with box bounds (lina_rn handle).
+    idx: bounded indices (default all), lower/upper per bounded index (NaN:
+    unbounded side)."""
+
+    def __init__(self, rowptr, col, val, b, lower, upper, prm: RNParams, idx=None):
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.b = np.ascontiguousarray(b, dtype=np.float64)
+        self.n = len(self.rowptr) - 1
+        self.idx = np.arange(self.n, dtype=np.int32) if idx is None else np.ascontiguousarray(idx, dtype=np.int32)
+        self.lower = np.ascontiguousarray(lower, dtype=np.float64)
+        self.upper = np.ascontiguousarray(upper, dtype=np.float64)
+        h = lib().lina_rn_create(self.n, _i(self.rowptr), _i(self.col), _d(self.val), _d(self.b), len(self.idx),
+                                 _i(self.idx), _d(self.lower), _d(self.upper), C.byref(prm))
+        if not h:
+            raise LinaError(f"lina_rn_create failed: {lib().lina_last_error().decode()}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lina_rn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def solve(self, x0):
+        x = np.array(x0, dtype=np.float64)
+        st = RNStats()
+        it = _check(lib().lina_rn_solve(self.h, _d(x), C.byref(st)))
+        d = st.as_dict()
+        d["iters"] = it
+        d["x"] = x
+        return d
+
+    def set_trust_region(self, delta: float):
+        _check(lib().lina_rn_set_trust_region(self.h, delta))
+
+
+def rn_scaled_hessian(rowptr, col, val, D, G, J):
+    """M = D H D + diag(|G| J) as RN_get_M_D assembles it (host)."""
+    rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+    n = len(rp) - 1
+    cap = int(rp[-1]) + n
+    mrp = np.zeros(n + 1, np.int32)
+    mci = np.zeros(cap, np.int32)
+    mv = np.zeros(cap)
+    nnz = _check(lib().lina_rn_scaled_hessian(n, _i(rp), _i(np.ascontiguousarray(col, dtype=np.int32)),
+                                              _d(np.ascontiguousarray(val, dtype=np.float64)),
+                                              _d(np.ascontiguousarray(D, dtype=np.float64)),
+                                              _d(np.ascontiguousarray(G, dtype=np.float64)),
+                                              _d(np.ascontiguousarray(J, dtype=np.float64)), _i(mrp), _i(mci), _d(mv)))
+    return mrp, mci[:nnz], mv[:nnz]

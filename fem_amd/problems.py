@@ -286,11 +286,11 @@ def degraded_elasticity(m: int, seed: int = SEED_NOTCH, E: float = 210e3, nu: fl
 
 
 def damage_system(m: int, seed_h: int = SEED_DAMAGE_H, seed_d: int = SEED_DAMAGE_DOLD, Gc: float = 2.7,
-                  l0: float = 0.01):
-    """Damage subproblem Gc*l0*K + (Gc/l0 + 2H)*M with H ~ lognormal(0,1) per
-    element and bounds d_old <= d <= 1, d_old ~ U[0,0.2]."""
+                  l0: float = 0.01, h_scale: float = 1.0):
+    """Damage subproblem Gc*l0*K + (Gc/l0 + 2H)*M with H ~ h_scale *
+    lognormal(0,1) per element and bounds d_old <= d <= 1, d_old ~ U[0,0.2]."""
     mesh = Mesh(m, m)
-    H = np.exp(normal(seed_h, len(mesh.tri)))
+    H = h_scale * np.exp(normal(seed_h, len(mesh.tri)))
     K = p1_stiffness(mesh)
     M = p1_mass(mesh, Gc / l0 + 2.0 * H)
     A = (Gc * l0) * K + M

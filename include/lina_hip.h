@@ -183,6 +183,66 @@ int lina_matrix_spmv_device(lina_matrix* h, const double* x, double* y);
 double lina_matrix_time_spmv(lina_matrix* h, int precision, int reps);
 double lina_matrix_spmv_bytes(const lina_matrix* h, int precision);
 
+/* ------------------------------------------------------------------
+ * Reflective Newton (reflective_newton.c:660-711): minimise
+ *   q(x) = x.H.x/2 + x.b   subject to lower_k <= x_idx[k] <= upper_k
+ * (RN_default_func / RN_default_gradient, NaN bound = unbounded side) by the
+ * trust-region reflective Newton method with a two-dimensional subspace
+ * minimum (subspace_minimum) and the piecewise-linear line search of
+ * RN_iterate (:536-627).  Every length-n vector operation, SpMV and inner
+ * Krylov solve runs on the device; the scaled Hessian M = D H D + diag(|g| J)
+ * (RN_get_M_D, with sparse_product's zero deletion) is assembled on the host,
+ * as are its ILU(0) factors or Galerkin coarse matrices.
+ * ------------------------------------------------------------------ */
+#define LINA_RN_BICGSTAB 1 /* BiCGStab + AMG (RN_init_BiCGStab), eps bi_eps, <= bi_max_iter */
+#define LINA_RN_AMG 2      /* AMG_solve with the recomputed hierarchy */
+#define LINA_RN_PCG 3      /* pcg with ILU(0) of M, tol = eps, <= pcg_iter */
+
+typedef struct lina_rn_params {
+    int max_iter;        /* RN_max_iter (50) */
+    double eps;          /* RN_eps (1e-8): outer |dq/q| and the inner PCG tolerance */
+    double sigma_l, sigma_u, rho;  /* line search (0.1, 0.9, 0.1) */
+    double delta;        /* initial trust radius RN_Delta (250) */
+    double xi;           /* boundary shift RN_Xi (1) */
+    double trust_lower, trust_upper;   /* contraction thresholds (0.6, 1.1) */
+    double radius_min, radius_max;     /* (1e-3, 1000) */
+    int solver;          /* LINA_RN_* (reference default: BiCGStab) */
+    int pcg_iter;        /* RN_PCG_iter (100) */
+    double bi_eps;       /* BI_eps of RN_init_BiCGStab (1e-3) */
+    int bi_max_iter;     /* BI_max_iter (10) */
+    lina_amg_params amg; /* hierarchy of H for the BiCGStab / AMG inner solvers */
+    unsigned long long seed; /* start vectors of the negative-curvature power method */
+} lina_rn_params;
+
+typedef struct lina_rn_stats {
+    int iterations;      /* outer Newton steps (the return value) */
+    int line_searches;   /* steps that needed the bisection line search */
+    int line_search_iter;
+    int neg_curvature;   /* inner solves that reported an indefinite M */
+    int inner_iterations;/* summed inner solver iterations */
+    double dq;           /* last |(q - q0)/q0| */
+    double q;            /* final functional value */
+    double delta;        /* trust radius after the solve */
+} lina_rn_stats;
+
+typedef struct lina_rn lina_rn;
+void lina_rn_params_default(lina_rn_params* p, int dof, int depth);
+/* H (n x n CSR), b (n), nb bounded indices idx with lower/upper (NaN =
+ * unbounded); the arrays are copied.  NULL on error. */
+lina_rn* lina_rn_create(int n, const int* rowptr, const int* col, const double* val, const double* b, int nb,
+                        const int* idx, const double* lower, const double* upper, const lina_rn_params* p);
+void lina_rn_destroy(lina_rn* h);
+/* reflective_newton(X): X (host, n) in/out; the trust radius persists
+ * between calls like the reference's RN_Delta */
+int lina_rn_solve(lina_rn* h, double* x, lina_rn_stats* st);
+/* RN_set_trust_region */
+int lina_rn_set_trust_region(lina_rn* h, double delta);
+/* the scaled Hessian M = D H D + diag(|G| J) of RN_get_M_D
+ * (reflective_newton.c:250-262) with sparse_product's zero deletion, on the
+ * host; mci/mv need room for rowptr[n] + n entries; returns nnz(M) */
+int lina_rn_scaled_hessian(int n, const int* rowptr, const int* col, const double* val, const double* D,
+                           const double* G, const double* J, int* mrp, int* mci, double* mv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -230,6 +230,19 @@ class Ref:
         it = self.L.refh_bicgstab_amg(M, dof, depth, SMOOTHERS[smoother], alpha, eps, max_iter, dptr(X), dptr(B))
         return {"iters": it, "x": X}
 
+    def rn_get_M_D(self, H, G, X, lower, upper):
+        n = len(X)
+        D = np.zeros(n)
+        f = self.L.refh_rn_get_M_D
+        f.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
+        f.restype = _vp
+        M = f(H, dptr(np.ascontiguousarray(G, dtype=np.float64)), dptr(np.ascontiguousarray(X, dtype=np.float64)),
+              dptr(np.ascontiguousarray(lower, dtype=np.float64)), dptr(np.ascontiguousarray(upper, dtype=np.float64)),
+              dptr(D))
+        out = self.to_csr(M)
+        self.free(M)
+        return out, D
+
     def reflective_newton(self, H, b, lower, upper, x0, solver=3, dof=1, depth=4, max_iter=50, eps=1e-8, delta=250.0):
         X = np.array(x0, dtype=np.float64)
         b = np.ascontiguousarray(b, dtype=np.float64)

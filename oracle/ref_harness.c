@@ -513,6 +513,7 @@ int refh_reflective_newton(sparse_matrix* H, double* b, double* lower, double* u
         RN_set_AMG_data(I);
     }
     AMG_set_smoother(&AMG_SOR_smoother);
+    AMG_set_SOR_relaxation_coeff(1.0);
     it = reflective_newton(X);
     if (q_out) {
         double* Hx = sparse_mult(H, X);
@@ -523,6 +524,21 @@ int refh_reflective_newton(sparse_matrix* H, double* b, double* lower, double* u
     BI_set_AMG_setup_data(NULL);
     free(idx);
     return it;
+}
+
+/* RN_get_M_D (reflective_newton.c:250-262) for H, gradient G and iterate X
+ * with bounds on every index; D receives the scaling */
+extern sparse_matrix* RN_get_M_D(double* Gradient, double* X, double* D);
+sparse_matrix* refh_rn_get_M_D(sparse_matrix* H, double* G, double* X, double* lower, double* upper, double* D) {
+    int n = H->size;
+    int* idx = get_all_indices(n);
+    RN_print_info(NO);
+    set_RN_Hesse(H);
+    set_RN_constraints(idx, lower, upper, n);
+    sparse_matrix* M = RN_get_M_D(G, X, D);
+    set_RN_constraints(NULL, NULL, NULL, 0);
+    free(idx);
+    return M;
 }
 
 /* ------------------------------------------------------------ GMRES(m) */

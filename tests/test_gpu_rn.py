@@ -1,0 +1,70 @@
+"""GPU reflective Newton against the compiled reference (oracle/_ref):
+reflective_newton on the damage subproblem q(d) = d.A.d/2 + d.b with
+d_old <= d <= 1, for the three inner solvers the reference offers without
+UMFPACK (ILU(0)-PCG, BiCGStab + AMG, AMG_solve).  The driving force is
+scaled so that the solution is interior (h_scale 100), pressed onto the
+upper bound (1000) or left on the lower bound (1).  Newton iteration counts
+must agree exactly and the minimisers to 1e-9 relative; the KKT conditions
+(projected gradient) are checked independently of the reference."""
+import numpy as np
+import pytest
+
+from fem_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def kkt(s, x, lo, up):
+    g = s.scipy() @ x + s.b
+    pg = np.where(x <= lo + 1e-9, np.minimum(g, 0), np.where(x >= up - 1e-9, np.maximum(g, 0), g))
+    return float(np.linalg.norm(pg) / np.linalg.norm(g))
+
+
+@pytest.mark.parametrize("solver", ["pcg", "bicgstab", "amg"])
+@pytest.mark.parametrize("h_scale", [1.0, 100.0, 1000.0])
+def test_reflective_newton_matches_reference(gpu, ref, solver, h_scale):
+    s, dold, ones = P.damage_system(33, h_scale=h_scale)
+    say("rn", solver, h_scale)
+    delta = np.sqrt(s.n) / 2
+    H = ref.matrix(s.rowptr, s.col, s.val)
+    code = {"bicgstab": 1, "amg": 2, "pcg": 3}[solver]
+    r = ref.reflective_newton(H, s.b, dold, ones, dold.copy(), solver=code, dof=1, depth=4, max_iter=50, eps=1e-8,
+                              delta=delta)
+    ref.free(H)
+    prm = gpu.rn_params(solver, 1, 4, delta=delta)
+    rn = gpu.ReflectiveNewton(s.rowptr, s.col, s.val, s.b, dold, ones, prm)
+    g = rn.solve(dold)
+    rn.close()
+    rel = np.linalg.norm(g["x"] - r["x"]) / np.linalg.norm(r["x"])
+    say("  iters ref", r["iters"], "gpu", g["iters"], "rel", rel, "q", g["q"], r["q"], "kkt", kkt(s, g["x"], dold, ones))
+    assert g["iters"] == r["iters"]
+    assert rel < 1e-9
+    assert abs(g["q"] - r["q"]) <= 1e-10 * abs(r["q"])
+    assert np.all(g["x"] >= dold) and np.all(g["x"] <= 1.0)
+    if h_scale > 1:
+        assert kkt(s, g["x"], dold, ones) < 1e-4
+
+
+def test_reflective_newton_partial_bounds(gpu, ref):
+    """Bounds on a subset of the indices and one-sided bounds (NaN)."""
+    s, dold, ones = P.damage_system(33, h_scale=300.0)
+    n = s.n
+    rng = np.random.default_rng(5)
+    lo = dold.copy()
+    up = ones.copy()
+    up[rng.random(n) < 0.3] = np.nan
+    x0 = dold.copy()
+    H = ref.matrix(s.rowptr, s.col, s.val)
+    r = ref.reflective_newton(H, s.b, lo, up, x0.copy(), solver=3, dof=1, depth=4, max_iter=50, eps=1e-8,
+                              delta=np.sqrt(n) / 2)
+    ref.free(H)
+    prm = gpu.rn_params("pcg", 1, 4, delta=np.sqrt(n) / 2)
+    g = gpu.ReflectiveNewton(s.rowptr, s.col, s.val, s.b, lo, up, prm).solve(x0)
+    rel = np.linalg.norm(g["x"] - r["x"]) / np.linalg.norm(r["x"])
+    say("  iters ref", r["iters"], "gpu", g["iters"], "rel", rel)
+    assert g["iters"] == r["iters"]
+    assert rel < 1e-9
