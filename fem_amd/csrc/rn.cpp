@@ -77,7 +77,7 @@ struct RootSearch {
         return (der_poly(r, x + dx, depth - 1) - der_poly(r, x - dx, depth - 1)) / (2. * dx);
     }
     // F_hospital (:336-351): F deflated by the roots found so far, by
-    // l'Hospital's rule where both vanish.  The depth is capped at 24 (the
+    // l'Hospital's rule where both vanish.  The depth is capped at 12 (the
     // reference loops without bound on an identically vanishing F).
     double hospital(const std::vector<double>& roots, double x) const {
         const double eps = 1e-5;
@@ -87,7 +87,7 @@ struct RootSearch {
             dp = der_poly(roots, x, depth);
             df = der_F(x, depth);
             depth++;
-        } while (std::fabs(dp) < eps && std::fabs(df) < eps && depth < 24);
+        } while (std::fabs(dp) < eps && std::fabs(df) < eps && depth < 12);
         return df / dp;
     }
     // Newton_1D (:353-400): secant iterations with deflation until one fails
@@ -295,7 +295,8 @@ class ReflectiveNewton {
             return (double*)q;
         };
         for (double** v : {&X_, &Xo_, &G_, &D_, &J_, &V1_, &V2_, &bs_, &Bv_, &S_, &T_, &T2_}) *v = alloc(n_);
-        part_ = alloc(std::max(8192, dH_->nblocks() + 16));
+        // block partials: a row block holds >= 1 row, so n + 8192 covers H, M and the dots
+        part_ = alloc((size_t)n_ + 8192);
         scal_ = alloc(16);
         lod_ = alloc(nb);
         upd_ = alloc(nb);
@@ -430,6 +431,23 @@ class ReflectiveNewton {
                 double res = 0;
                 iter = sys.pcg_dev(V2_, bs_, p_.eps, p_.pcg_iter, LINA_STOP_REFERENCE, &res);
                 status = iter < p_.pcg_iter ? 1 : 0;
+                break;
+            }
+            case LINA_RN_GMRES: {
+                // RN_UMF's slot (reflective_newton.c:312-315): the local solver
+                // decides the status; FEM.c's convention y = V2, V2 = M^-1 y
+                IluSystem sys;
+                sys.build(M, true);
+                if (!sys.ilu_ok()) {
+                    status = 0;
+                    break;
+                }
+                double eta = 0;
+                iter = sys.gmres_dev(V2_, bs_, p_.gmres_m, p_.gmres_tol, p_.gmres_max_iter, &eta);
+                sys.apply(V2_, T_);
+                hip_check(hipStreamSynchronize(sys.stream()), "sync");
+                dev::copy(st_, n, (const double*)T_, V2_);
+                status = eta <= p_.gmres_tol ? 1 : 0;
                 break;
             }
             case LINA_RN_BICGSTAB: {
@@ -684,6 +702,9 @@ void lina_rn_params_default(lina_rn_params* p, int dof, int depth) {
     p->pcg_iter = 100;
     p->bi_eps = 1e-3;
     p->bi_max_iter = 10;
+    p->gmres_m = 30;
+    p->gmres_max_iter = 600;
+    p->gmres_tol = 1e-10;
     lina_params_reference(&p->amg, dof, depth);
     p->amg.omega = 1.0;
     p->amg.amg_eps = 1e-4;
@@ -697,7 +718,9 @@ lina_rn* lina_rn_create(int n, const int* rowptr, const int* col, const double* 
     int rc = rn_guard([&]() {
         if (n <= 0 || !rowptr || !col || !val || !b || !p || nb < 0 || (nb > 0 && (!idx || !lower || !upper)))
             throw std::invalid_argument("lina_rn_create: bad arguments");
-        if (p->solver < LINA_RN_BICGSTAB || p->solver > LINA_RN_PCG) throw std::invalid_argument("unknown solver");
+        if (p->solver < LINA_RN_BICGSTAB || p->solver > LINA_RN_GMRES) throw std::invalid_argument("unknown solver");
+        if (p->solver == LINA_RN_GMRES && (p->gmres_m < 1 || p->gmres_max_iter < 1))
+            throw std::invalid_argument("GMRES restart length and iteration limit must be positive");
         lina::Csr H;
         H.n = n;
         H.ncols = n;

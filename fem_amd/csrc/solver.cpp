@@ -238,7 +238,11 @@ void Solver::upload() {
         hip_check(hipEventCreate(&ev1_), "hipEventCreate");
         hip_check(hipHostMalloc((void**)&host_scal_, dev::S_COUNT * sizeof(double)), "hipHostMalloc");
     }
+    // a re-upload (recompute) frees every earlier buffer, including the
+    // BiCGStab work vectors allocated on first use
+    hip_check(hipStreamSynchronize(stream_), "sync");
     bufs_.reset(new DevBuffers());
+    for (double** p : {&bS_, &bPS_, &bPP_, &bAPP_, &bAPS_, &bR_, &bR0_, &bP_}) *p = nullptr;
     DevBuffers& B = *bufs_;
     hipStream_t st = stream_;
     const int depth = (int)H_.lv.size();

@@ -43,6 +43,7 @@ class RNParams(C.Structure):
                 ("rho", C.c_double), ("delta", C.c_double), ("xi", C.c_double), ("trust_lower", C.c_double),
                 ("trust_upper", C.c_double), ("radius_min", C.c_double), ("radius_max", C.c_double),
                 ("solver", C.c_int), ("pcg_iter", C.c_int), ("bi_eps", C.c_double), ("bi_max_iter", C.c_int),
+                ("gmres_m", C.c_int), ("gmres_max_iter", C.c_int), ("gmres_tol", C.c_double),
                 ("amg", Params), ("seed", C.c_ulonglong)]
 
 
@@ -55,7 +56,7 @@ class RNStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
-RN_SOLVER = {"bicgstab": 1, "amg": 2, "pcg": 3}
+RN_SOLVER = {"bicgstab": 1, "amg": 2, "pcg": 3, "gmres": 4}
 
 
 class LinaError(RuntimeError):

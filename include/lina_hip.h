@@ -197,6 +197,8 @@ double lina_matrix_spmv_bytes(const lina_matrix* h, int precision);
 #define LINA_RN_BICGSTAB 1 /* BiCGStab + AMG (RN_init_BiCGStab), eps bi_eps, <= bi_max_iter */
 #define LINA_RN_AMG 2      /* AMG_solve with the recomputed hierarchy */
 #define LINA_RN_PCG 3      /* pcg with ILU(0) of M, tol = eps, <= pcg_iter */
+#define LINA_RN_GMRES 4    /* local solver in the RN_UMF slot (RN_set_local_solver): GMRES(gmres_m) with right
+                            * ILU(0) as FEM.c drives it; M indefinite when it misses gmres_tol */
 
 typedef struct lina_rn_params {
     int max_iter;        /* RN_max_iter (50) */
@@ -210,6 +212,9 @@ typedef struct lina_rn_params {
     int pcg_iter;        /* RN_PCG_iter (100) */
     double bi_eps;       /* BI_eps of RN_init_BiCGStab (1e-3) */
     int bi_max_iter;     /* BI_max_iter (10) */
+    int gmres_m;         /* restart length (30) */
+    int gmres_max_iter;  /* Arnoldi steps (600) */
+    double gmres_tol;    /* |r|/|r0| of the preconditioned system (1e-10) */
     lina_amg_params amg; /* hierarchy of H for the BiCGStab / AMG inner solvers */
     unsigned long long seed; /* start vectors of the negative-curvature power method */
 } lina_rn_params;

@@ -2,10 +2,12 @@
 reflective_newton on the damage subproblem q(d) = d.A.d/2 + d.b with
 d_old <= d <= 1, for the three inner solvers the reference offers without
 UMFPACK (ILU(0)-PCG, BiCGStab + AMG, AMG_solve).  The driving force is
-scaled so that the solution is interior (h_scale 100), pressed onto the
+scaled so that the solution is interior (h_scale 100), pressed towards the
 upper bound (1000) or left on the lower bound (1).  Newton iteration counts
-must agree exactly and the minimisers to 1e-9 relative; the KKT conditions
-(projected gradient) are checked independently of the reference."""
+must agree exactly and the minimisers to 1e-9 relative; independently of the
+reference, the minimum value must not exceed the one L-BFGS-B finds (the
+reflective iterates approach active bounds only asymptotically, so a
+projected-gradient test is printed, not asserted)."""
 import numpy as np
 import pytest
 
@@ -22,6 +24,16 @@ def kkt(s, x, lo, up):
     g = s.scipy() @ x + s.b
     pg = np.where(x <= lo + 1e-9, np.minimum(g, 0), np.where(x >= up - 1e-9, np.maximum(g, 0), g))
     return float(np.linalg.norm(pg) / np.linalg.norm(g))
+
+
+def lbfgsb_min(s, lo, up):
+    """min q over the box by scipy's L-BFGS-B, independent of both RN codes"""
+    from scipy.optimize import minimize
+    A = s.scipy()
+    f = lambda x: (0.5 * x @ (A @ x) + s.b @ x, A @ x + s.b)
+    r = minimize(f, lo.copy(), jac=True, method="L-BFGS-B", bounds=list(zip(lo, up)),
+                 options={"maxiter": 20000, "ftol": 1e-15, "gtol": 1e-12})
+    return float(r.fun)
 
 
 @pytest.mark.parametrize("solver", ["pcg", "bicgstab", "amg"])
@@ -45,8 +57,9 @@ def test_reflective_newton_matches_reference(gpu, ref, solver, h_scale):
     assert rel < 1e-9
     assert abs(g["q"] - r["q"]) <= 1e-10 * abs(r["q"])
     assert np.all(g["x"] >= dold) and np.all(g["x"] <= 1.0)
-    if h_scale > 1:
-        assert kkt(s, g["x"], dold, ones) < 1e-4
+    # the minimum value itself, from an unrelated method
+    q_ref = lbfgsb_min(s, dold, ones)
+    assert g["q"] <= q_ref + 1e-7 * abs(q_ref)
 
 
 def test_reflective_newton_partial_bounds(gpu, ref):
@@ -68,3 +81,23 @@ def test_reflective_newton_partial_bounds(gpu, ref):
     say("  iters ref", r["iters"], "gpu", g["iters"], "rel", rel)
     assert g["iters"] == r["iters"]
     assert rel < 1e-9
+
+
+@pytest.mark.parametrize("h_scale", [100.0, 1000.0])
+def test_reflective_newton_gmres_local_solver(gpu, ref, h_scale):
+    """GMRES(30)+ILU(0) in the local-solver slot: no reference counterpart, so
+    the minimiser is compared with the reference's ILU-PCG variant and with
+    L-BFGS-B."""
+    s, dold, ones = P.damage_system(33, h_scale=h_scale)
+    delta = np.sqrt(s.n) / 2
+    H = ref.matrix(s.rowptr, s.col, s.val)
+    r = ref.reflective_newton(H, s.b, dold, ones, dold.copy(), solver=3, dof=1, depth=4, max_iter=50, eps=1e-8,
+                              delta=delta)
+    ref.free(H)
+    g = gpu.ReflectiveNewton(s.rowptr, s.col, s.val, s.b, dold, ones,
+                             gpu.rn_params("gmres", 1, 4, delta=delta)).solve(dold)
+    rel = np.linalg.norm(g["x"] - r["x"]) / np.linalg.norm(r["x"])
+    say("  gmres iters", g["iters"], "ref pcg", r["iters"], "rel", rel, "inner", g["inner_iterations"])
+    assert rel < 1e-6
+    q_ref = lbfgsb_min(s, dold, ones)
+    assert g["q"] <= q_ref + 1e-7 * abs(q_ref)
