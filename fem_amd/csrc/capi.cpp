@@ -352,6 +352,45 @@ double lina_matrix_time_spmv(lina_matrix* h, int precision, int reps) {
 
 double lina_matrix_spmv_bytes(const lina_matrix* h, int precision) { return h ? h->m.bytes(precision) : 0.0; }
 
+int lina_pcg_operator(lina_solver* s, lina_matrix* op, double* x, double* b, double tol, int max_iter, int stop,
+                      lina_stats* st) {
+    return guard([&]() {
+        if (!s || !op || !x || !b) throw std::invalid_argument("null argument");
+        if (!s->s.uploaded()) throw std::invalid_argument("solver was created without device upload");
+        s->s.set_operator(&op->m);
+        int it;
+        try {
+            it = s->s.pcg_host(x, b, tol, max_iter, stop, st);
+        } catch (...) {
+            s->s.set_operator(nullptr);
+            throw;
+        }
+        s->s.set_operator(nullptr);
+        return it;
+    });
+}
+
+int lina_elasticity_adjoint(lina_solver* s, int n_nodes, const double* xy, int n_elem, const int* tri, double nu,
+                            const double* elem_scale, int blocked, const double* u, const double* dLdu, double tol,
+                            int max_iter, double* lambda, double* grad, lina_stats* st) {
+    return guard([&]() {
+        if (!s || !xy || !tri || !u || !dLdu || !lambda || !grad || n_nodes <= 0 || n_elem < 0)
+            throw std::invalid_argument("null or empty argument");
+        if (!s->s.uploaded()) throw std::invalid_argument("solver was created without device upload");
+        const int n = 2 * n_nodes;
+        if (n != s->s.n()) throw std::invalid_argument("2 * n_nodes differs from the system size");
+        for (long k = 0; k < 3L * n_elem; k++)
+            if (tri[k] < 0 || tri[k] >= n_nodes) throw std::invalid_argument("element node index out of range");
+        if (!(nu > -1.0 && nu < 0.5)) throw std::invalid_argument("Poisson ratio outside (-1, 0.5)");
+        // adjoint: K lambda = dL/du (K symmetric: the forward hierarchy)
+        std::vector<double> lam((size_t)n, 0.0), rhs(dLdu, dLdu + n);
+        int it = s->s.pcg_host(lam.data(), rhs.data(), tol, max_iter, LINA_STOP_RELRES, st);
+        std::copy(lam.begin(), lam.end(), lambda);
+        s->s.elasticity_gradient(n_nodes, xy, n_elem, tri, nu, elem_scale, blocked, u, lam.data(), grad);
+        return it;
+    });
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------- ILU(0) systems

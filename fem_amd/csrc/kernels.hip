@@ -617,6 +617,46 @@ __global__ void k_scale(int n, double* __restrict__ x, double a) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] *= a;
 }
 
+// ------------------------------------------------- adjoint gradient
+//
+// dL/dtheta_e = -scale_e * lam_e^T K_e(E=1) u_e for P1 plane-strain elements
+// with Young's modulus theta_e (K linear in theta_e).  With the gradients
+// g_i of the barycentric coordinates and strains of u and lam,
+//   lam^T K_e u = area * [(l+2m)(ax cx + by dy) + l(ax dy + by cx) + m(ay + bx)(cy + dx)]
+// where (ax, ay) = grad lam_x, (bx, by) = grad lam_y, (cx, cy) = grad u_x,
+// (dx, dy) = grad u_y and l, m are the Lame constants of E = 1.  u and lam
+// are full vectors (constrained entries: prescribed values in u, zero in
+// lam); blocked != 0: [x of all nodes; y of all nodes], else interleaved.
+__global__ void k_elast_grad(int ne, const int* __restrict__ tri, const double* __restrict__ xy, int nn,
+                             const double* __restrict__ u, const double* __restrict__ lam, double lame_l,
+                             double lame_m, const double* __restrict__ scale, int blocked,
+                             double* __restrict__ grad) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
+        const int n0 = tri[3 * e], n1 = tri[3 * e + 1], n2 = tri[3 * e + 2];
+        const double x0 = xy[2 * n0], y0 = xy[2 * n0 + 1];
+        const double d1x = xy[2 * n1] - x0, d1y = xy[2 * n1 + 1] - y0;
+        const double d2x = xy[2 * n2] - x0, d2y = xy[2 * n2 + 1] - y0;
+        const double det = d1x * d2y - d1y * d2x;
+        const double area = 0.5 * fabs(det);
+        const double g1x = d2y / det, g1y = -d2x / det;
+        const double g2x = -d1y / det, g2y = d1x / det;
+        const double g0x = -(g1x + g2x), g0y = -(g1y + g2y);
+        const int ix0 = blocked ? n0 : 2 * n0, ix1 = blocked ? n1 : 2 * n1, ix2 = blocked ? n2 : 2 * n2;
+        const int iy0 = blocked ? n0 + nn : 2 * n0 + 1, iy1 = blocked ? n1 + nn : 2 * n1 + 1,
+                  iy2 = blocked ? n2 + nn : 2 * n2 + 1;
+        const double ux0 = u[ix0], ux1 = u[ix1], ux2 = u[ix2], uy0 = u[iy0], uy1 = u[iy1], uy2 = u[iy2];
+        const double lx0 = lam[ix0], lx1 = lam[ix1], lx2 = lam[ix2], ly0 = lam[iy0], ly1 = lam[iy1],
+                     ly2 = lam[iy2];
+        const double ax = lx0 * g0x + lx1 * g1x + lx2 * g2x, ay = lx0 * g0y + lx1 * g1y + lx2 * g2y;
+        const double bx = ly0 * g0x + ly1 * g1x + ly2 * g2x, by = ly0 * g0y + ly1 * g1y + ly2 * g2y;
+        const double cx = ux0 * g0x + ux1 * g1x + ux2 * g2x, cy = ux0 * g0y + ux1 * g1y + ux2 * g2y;
+        const double dx = uy0 * g0x + uy1 * g1x + uy2 * g2x, dy = uy0 * g0y + uy1 * g1y + uy2 * g2y;
+        const double form = (lame_l + 2.0 * lame_m) * (ax * cx + by * dy) + lame_l * (ax * dy + by * cx) +
+                            lame_m * (ay + bx) * (cy + dx);
+        grad[e] = -(scale ? scale[e] : 1.0) * area * form;
+    }
+}
+
 // ------------------------------------------------------ coarse solve
 
 // x = inv * b over the dense rows (gathered through idx), trivial rows
@@ -767,6 +807,15 @@ void restore_if_zero(hipStream_t st, int n, double* x, const double* scal, int s
     hipLaunchKernelGGL(k_zero_if, dim3(grid_for(n)), dim3(TPB), 0, st, n, x, scal, slot);
     if (m > 0)
         hipLaunchKernelGGL(k_fix_isolated_if, dim3(grid_for(m)), dim3(TPB), 0, st, m, rows, a, f, x, scal, slot);
+}
+
+void elast_grad(hipStream_t st, int ne, const int* tri, const double* xy, int nn, const double* u, const double* lam,
+                double nu, const double* scale, int blocked, double* grad) {
+    if (ne <= 0) return;
+    const double l = nu / ((1 + nu) * (1 - 2 * nu));
+    const double m = 1.0 / (2 * (1 + nu));
+    hipLaunchKernelGGL(k_elast_grad, dim3(grid_for(ne, 8192)), dim3(TPB), 0, st, ne, tri, xy, nn, u, lam, l, m, scale,
+                       blocked, grad);
 }
 
 void rn_dj(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, const double* G,

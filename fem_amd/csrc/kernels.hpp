@@ -81,6 +81,9 @@ void rn_snap(hipStream_t st, int nb, const int* idx, const double* lo, const dou
 // block partial counts of entries within 1e-30 of a bound; returns #partials
 int rn_touch(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, const double* X,
              double* part);
+// dL/dtheta_e = -scale_e lam_e^T K_e(E=1) u_e, P1 plane strain (Poisson ratio nu)
+void elast_grad(hipStream_t st, int ne, const int* tri, const double* xy, int nn, const double* u, const double* lam,
+                double nu, const double* scale, int blocked, double* grad);
 void mul(hipStream_t st, int n, double* x, const double* y);  // x_i *= y_i
 void scale(hipStream_t st, int n, double* x, double a);       // x_i *= a
 // when scal[slot] == 0: x = 0, then x_i = f_i / a_i on the isolated rows

@@ -349,6 +349,7 @@ void Solver::upload() {
     // block partials: one per SpMV block (any matrix) or per reduction block
     int np = std::max(4096, maxblk + 16);
     part_ = dalloc<double>(np, B);
+    part_cap_ = np;
     scal_ = dalloc<double>(dev::S_COUNT, B);
     dev::fill(st, dev::S_COUNT, scal_, 0.0);
     hip_check(hipStreamSynchronize(st), "upload sync");
@@ -375,6 +376,17 @@ void Solver::recompute_matrix(const Csr& M) {
     if (uploaded()) upload();
 }
 
+void Solver::set_operator(DevMatrix* op) {
+    if (op) {
+        if (op->n() != n_) throw std::invalid_argument("operator size differs from the hierarchy");
+        if (op->nblocks() + 16 > part_cap_) throw std::invalid_argument("operator has more row blocks than the "
+                                                                        "solver's partial-sum buffer");
+        // the operator's own stream must be idle: its kernels now run on ours
+        hip_check(hipStreamSynchronize(op->stream()), "sync");
+    }
+    op_ = op;
+}
+
 void Solver::set_kernel_timing(bool on) {
     if (on && !evk0_) {
         hip_check(hipEventCreate(&evk0_), "hipEventCreate");
@@ -384,6 +396,30 @@ void Solver::set_kernel_timing(bool on) {
 }
 
 double* Solver::dalloc_public(size_t n) { return dalloc<double>(n, *bufs_); }
+
+// dL/dtheta of P1 plane-strain elements (host arrays in and out)
+void Solver::elasticity_gradient(int n_nodes, const double* xy, int n_elem, const int* tri, double nu,
+                                 const double* scale, int blocked, const double* u, const double* lam,
+                                 double* grad) {
+    DevBuffers B;
+    auto up = [&](const void* h, size_t bytes) {
+        void* d = nullptr;
+        hip_check(hipMalloc(&d, std::max<size_t>(bytes, 8)), "hipMalloc");
+        B.ptrs.push_back(d);
+        if (h && bytes) hip_check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream_), "h2d");
+        return d;
+    };
+    const int n = 2 * n_nodes;
+    auto* dxy = (const double*)up(xy, 2 * (size_t)n_nodes * sizeof(double));
+    auto* dtri = (const int*)up(tri, 3 * (size_t)n_elem * sizeof(int));
+    auto* du = (const double*)up(u, (size_t)n * sizeof(double));
+    auto* dl = (const double*)up(lam, (size_t)n * sizeof(double));
+    auto* ds = scale ? (const double*)up(scale, (size_t)n_elem * sizeof(double)) : nullptr;
+    auto* dg = (double*)up(nullptr, (size_t)n_elem * sizeof(double));
+    dev::elast_grad(stream_, n_elem, dtri, dxy, n_nodes, du, dl, nu, ds, blocked, dg);
+    hip_check(hipMemcpyAsync(grad, dg, (size_t)n_elem * sizeof(double), hipMemcpyDeviceToHost, stream_), "d2h");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+}
 
 double Solver::read_scalar(int slot) {
     hip_check(hipMemcpyAsync(host_scal_ + slot, scal_ + slot, sizeof(double), hipMemcpyDeviceToHost, stream_), "d2h");
@@ -654,7 +690,9 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
     dev::fill<double>(st, n, Z, 0.0);
     // R = B + (-1)*(A X) with the caller's matrix (rows of the stored finest
     // matrix carry the make_positive_diag signs)
-    if (has_neg_signs_) {
+    if (op_) {
+        op_->launch_on(st, X, B, R, dev::MODE_RES, nullptr, nullptr);
+    } else if (has_neg_signs_) {
         apply_A(X, Q);
         dev::copy(st, n, (const double*)B, R);
         dev::axpby(st, n, 1.0, R, -1.0, Q);
@@ -682,7 +720,9 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
         np = dev::dot_part<double>(st, n, R, Z, part_);
         dev::finalize(st, np, part_, scal_, dev::S_S, 2);  // s_old = s; s = R.Z
         dev::pcg_p(st, n, P, Z, scal_, i == 0);
-        if (has_neg_signs_) {
+        if (op_) {
+            np = op_->launch_on(st, P, nullptr, Q, dev::MODE_OUTER, part_, P);
+        } else if (has_neg_signs_) {
             apply_A(P, Q);
             np = dev::dot_part<double>(st, n, P, Q, part_);
         } else {
@@ -694,7 +734,7 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
         np = dev::pcg_xr(st, n, X, R, P, Q, scal_, part_);
         dev::finalize(st, np, part_, scal_, dev::S_R, 1);
         r = read_scalar(dev::S_R);
-        if (ktime_ && !has_neg_signs_) {
+        if (ktime_ && !has_neg_signs_ && !op_) {
             float kms = 0;
             hip_check(hipEventElapsedTime(&kms, evk0_, evk1_), "elapsed");
             ksum += kms;
@@ -826,6 +866,11 @@ void DevMatrix::apply(const double* x, double* y) {
 
 int DevMatrix::launch(const double* x, const double* b, double* y, int mode, double* part, const double* w) {
     return dev::spmv<double, double>(stream_, A_, x, b, y, mode, part, w);
+}
+
+int DevMatrix::launch_on(hipStream_t st, const double* x, const double* b, double* y, int mode, double* part,
+                         const double* w) {
+    return dev::spmv<double, double>(st, A_, x, b, y, mode, part, w);
 }
 
 void DevMatrix::spmv_dev(const double* x, double* y) {

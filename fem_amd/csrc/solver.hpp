@@ -63,6 +63,8 @@ class DevMatrix {
     // y = A x / b - A x / y + A x (mode) on stream(); part != nullptr: block
     // partials of sum_r y_r w_r, returns their number
     int launch(const double* x, const double* b, double* y, int mode, double* part, const double* w);
+    int launch_on(hipStream_t st, const double* x, const double* b, double* y, int mode, double* part,
+                  const double* w);
     int nblocks() const { return A_.nblk; }
     void spmv_dev(const double* x, double* y);
     void spmv_host(int precision, const double* x, double* y);
@@ -130,6 +132,11 @@ class Solver {
     int bicgstab_host(double* x, const double* b, double eps, int max_iter, double* res);
     int last_vcycles() const { return vcycles_; }
     void set_kernel_timing(bool on);
+    // pcg's operator (PCG_set_mat_mult with a device matrix): the hierarchy
+    // stays the preconditioner; nullptr restores the hierarchy's own matrix
+    void set_operator(DevMatrix* op);
+    void elasticity_gradient(int n_nodes, const double* xy, int n_elem, const int* tri, double nu,
+                             const double* scale, int blocked, const double* u, const double* lam, double* grad);
 
   private:
     lina_amg_params prm_{};
@@ -158,6 +165,8 @@ class Solver {
     int vcycles_ = 0, precon_calls_ = 0;
     double pcg_residuum_ = 0;
     bool ktime_ = false;
+    DevMatrix* op_ = nullptr;
+    int part_cap_ = 0;
     hipEvent_t evk0_ = nullptr, evk1_ = nullptr;
     double *bS_ = nullptr, *bPS_ = nullptr, *bPP_ = nullptr, *bAPP_ = nullptr, *bAPS_ = nullptr, *bR_ = nullptr,
            *bR0_ = nullptr, *bP_ = nullptr;

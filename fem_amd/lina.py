@@ -112,6 +112,10 @@ def lib():
     L.lina_matrix_time_spmv.restype = C.c_double
     L.lina_matrix_spmv_bytes.argtypes = [C.c_void_p, C.c_int]
     L.lina_matrix_spmv_bytes.restype = C.c_double
+    L.lina_pcg_operator.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, C.c_double, C.c_int, C.c_int,
+                                    C.POINTER(Stats)]
+    L.lina_elasticity_adjoint.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip, C.c_double, _dp, C.c_int, _dp,
+                                          _dp, C.c_double, C.c_int, _dp, _dp, C.POINTER(Stats)]
     L.lina_rn_params_default.argtypes = [C.POINTER(RNParams), C.c_int, C.c_int]
     L.lina_rn_create.argtypes = [C.c_int, _ip, _ip, _dp, _dp, C.c_int, _ip, _dp, _dp, C.POINTER(RNParams)]
     L.lina_rn_create.restype = C.c_void_p
@@ -236,6 +240,32 @@ class AMG:
         d = st.as_dict()
         d["iters"] = it
         return d
+
+    def pcg_operator(self, op: "Matrix", b, x0=None, tol=1e-10, max_iter=1000, stop="relres"):
+        """op x = b by pcg with this hierarchy as the preconditioner"""
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        B = np.array(b, dtype=np.float64)
+        st = Stats()
+        it = _check(lib().lina_pcg_operator(self.h, op.h, _d(x), _d(B), tol, max_iter, STOP[stop], C.byref(st)))
+        d = st.as_dict()
+        d["iters"] = it
+        d["x"] = x
+        return d
+
+    def elasticity_adjoint(self, xy, tri, nu, u, dLdu, elem_scale=None, blocked=True, tol=1e-12, max_iter=5000):
+        """lambda and dL/dtheta per element (lina_elasticity_adjoint)"""
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+        tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1, 3)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        dLdu = np.ascontiguousarray(dLdu, dtype=np.float64)
+        lam = np.zeros(self.n)
+        grad = np.zeros(len(tri))
+        sc = None if elem_scale is None else _d(np.ascontiguousarray(elem_scale, dtype=np.float64))
+        st = Stats()
+        it = _check(lib().lina_elasticity_adjoint(self.h, len(xy), _d(xy), len(tri), _i(tri), nu, sc,
+                                                  1 if blocked else 0, _d(u), _d(dLdu), tol, max_iter, _d(lam),
+                                                  _d(grad), C.byref(st)))
+        return {"iters": it, "lambda": lam, "grad": grad, "stats": st.as_dict()}
 
     def amg_solve(self, f, x0):
         f = np.ascontiguousarray(f, dtype=np.float64)

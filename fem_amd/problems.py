@@ -302,3 +302,88 @@ def damage_system(m: int, seed_h: int = SEED_DAMAGE_H, seed_d: int = SEED_DAMAGE
     d_old = uniform(seed_d, mesh.n_nodes, 0.0, 0.2)
     sys_ = _to_system(A, b, 1, {"kind": "damage", "m": m})
     return sys_, d_old, np.ones(mesh.n_nodes)
+
+
+# ------------------------------------------------- adjoint (config 5) inputs
+
+SEED_GRF_OBS = 0x5EED0106
+SEED_OBS_NODES = 0x5EED0206
+
+
+def grf_cells(nc: int, seed: int, length: float = 0.1, var: float = 0.25) -> np.ndarray:
+    """Stationary Gaussian field with covariance var * exp(-r^2 / (2 length^2))
+    at the centres of the nc x nc cells of the unit square, by circulant
+    embedding on a (2 nc) x (2 nc) periodic grid (negative eigenvalues of the
+    embedding, if any, are set to zero).  Returns g[r, c] (r along y)."""
+    M = 2 * nc
+    h = 1.0 / nc
+    k = np.arange(M)
+    d = np.minimum(k, M - k) * h
+    c = var * np.exp(-(d[:, None] ** 2 + d[None, :] ** 2) / (2.0 * length ** 2))
+    lam = np.maximum(np.fft.fft2(c).real, 0.0)
+    z = normal(seed, 2 * M * M)
+    w = (z[: M * M] + 1j * z[M * M:]).reshape(M, M)
+    f = np.fft.fft2(np.sqrt(lam / (M * M)) * w)
+    return np.ascontiguousarray(f.real[:nc, :nc])
+
+
+def element_field(mesh: "Mesh", cells: np.ndarray) -> np.ndarray:
+    """Per-element values from per-cell values: both triangles of a cell (the
+    element order of Mesh.tri is all first triangles, then all second ones)."""
+    v = cells.ravel()
+    return np.concatenate([v, v])
+
+
+def elasticity_theta(mesh: "Mesh", theta: np.ndarray, nu: float = 0.3, top_disp: float = 1e-3):
+    """P1 plane strain with Young's modulus theta per element, bottom clamped,
+    top u_y = top_disp; returns the System and the constrained dofs."""
+    K = plane_strain_stiffness(mesh, theta, nu)
+    nn = mesh.n_nodes
+    bot = mesh.boundary("bottom")
+    top = mesh.boundary("top")
+    dofs = np.concatenate([bot, bot + nn, top + nn])
+    vals = np.concatenate([np.zeros(2 * len(bot)), np.full(len(top), top_disp)])
+    A, b = apply_dirichlet(K, np.zeros(2 * nn), dofs, vals)
+    return _to_system(A, b, 2, {"kind": "elasticity_theta", "m": mesh.nx}), dofs
+
+
+@dataclass
+class AdjointCase:
+    """Config-5 inputs: theta = exp(g) with g a seeded Gaussian random field,
+    observations of both displacement components at a seeded 5 % of the nodes,
+    taken from the solution for a second field theta_obs."""
+    mesh: "Mesh"
+    theta: np.ndarray
+    theta_obs: np.ndarray
+    obs_nodes: np.ndarray
+    nu: float = 0.3
+
+    @property
+    def obs_dofs(self) -> np.ndarray:
+        nn = self.mesh.n_nodes
+        return np.concatenate([self.obs_nodes, self.obs_nodes + nn])
+
+    def system(self, theta: np.ndarray):
+        return elasticity_theta(self.mesh, theta, self.nu)
+
+    def loss(self, u: np.ndarray, u_obs: np.ndarray) -> float:
+        r = u[self.obs_dofs] - u_obs[self.obs_dofs]
+        return 0.5 * float(r @ r)
+
+    def dloss(self, u: np.ndarray, u_obs: np.ndarray, constrained: np.ndarray) -> np.ndarray:
+        g = np.zeros_like(u)
+        d = self.obs_dofs
+        g[d] = u[d] - u_obs[d]
+        g[constrained] = 0.0
+        return g
+
+
+def adjoint_case(m: int, seed: int = SEED_GRF, seed_obs: int = SEED_GRF_OBS, seed_nodes: int = SEED_OBS_NODES,
+                 frac: float = 0.05, nu: float = 0.3) -> AdjointCase:
+    mesh = Mesh(m, m)
+    theta = np.exp(element_field(mesh, grf_cells(m - 1, seed)))
+    theta_obs = np.exp(element_field(mesh, grf_cells(m - 1, seed_obs)))
+    keys = uniform(seed_nodes, mesh.n_nodes)
+    k = max(1, int(round(frac * mesh.n_nodes)))
+    obs = np.sort(np.argsort(keys, kind="stable")[:k])
+    return AdjointCase(mesh, theta, theta_obs, obs, nu)

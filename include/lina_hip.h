@@ -182,6 +182,26 @@ int lina_matrix_spmv(lina_matrix* h, int precision, const double* x, double* y);
 int lina_matrix_spmv_device(lina_matrix* h, const double* x, double* y);
 double lina_matrix_time_spmv(lina_matrix* h, int precision, int reps);
 double lina_matrix_spmv_bytes(const lina_matrix* h, int precision);
+/* pcg (PCGS.c:163-233) with the operator op (PCG_set_mat_mult with a device
+ * matrix, PCGS.c:137-139) and this hierarchy as the preconditioner: solves
+ * op x = b; op has the hierarchy's size and at most as many row blocks as its
+ * finest matrix */
+int lina_pcg_operator(lina_solver* s, lina_matrix* op, double* x, double* b, double tol, int max_iter, int stop,
+                      lina_stats* st);
+
+/* Backward entry point of P1 plane-strain elasticity with per-element Young's
+ * moduli theta (K = sum_e elem_scale_e theta_e K_e(E=1)): the adjoint solve
+ * K lambda = dL/du with this hierarchy (K symmetric), then
+ *   grad_e = dL/dtheta_e = -elem_scale_e lambda_e^T K_e(E=1) u_e
+ * per element on the device.  No counterpart in the reference.  xy: node
+ * coordinates (x,y pairs), tri: 3 node indices per element, blocked != 0:
+ * dofs [x of all nodes; y of all nodes] (else interleaved), u: forward
+ * solution with the prescribed values on constrained dofs, dLdu: zero on
+ * constrained dofs; elem_scale may be NULL (1).  Host pointers; returns the
+ * adjoint pcg's iterations (relative-residual stop at tol). */
+int lina_elasticity_adjoint(lina_solver* s, int n_nodes, const double* xy, int n_elem, const int* tri, double nu,
+                            const double* elem_scale, int blocked, const double* u, const double* dLdu, double tol,
+                            int max_iter, double* lambda, double* grad, lina_stats* st);
 
 /* ------------------------------------------------------------------
  * Reflective Newton (reflective_newton.c:660-711): minimise
