@@ -902,9 +902,8 @@ int ilu0(const Csr& A, Csr& L, Csr& U) {
 // ------------------------------------------------------------ AMG_setup
 
 static bool verbose() {
-    static int v = -1;
-    if (v < 0) v = getenv("LINA_VERBOSE") ? 1 : 0;
-    return v == 1;
+    static const bool v = getenv("LINA_VERBOSE") != nullptr;  // thread-safe initialisation
+    return v;
 }
 
 static double now_s() {

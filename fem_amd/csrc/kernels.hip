@@ -657,6 +657,81 @@ __global__ void k_elast_grad(int ne, const int* __restrict__ tri, const double* 
     }
 }
 
+// -------------------------------------------------- batched Krylov
+//
+// A batch of independent systems stored as one block-diagonal system.  Rows
+// are split into chunks that never straddle two systems: chunk c covers rows
+// [crow[c], crow[c+1]) of system cinst[c]; system k owns chunks
+// [ichunk[k], ichunk[k+1]).  Per-system scalars live in device arrays.
+
+// part[c] = sum over chunk c of a_i b_i (fixed tree)
+__global__ __launch_bounds__(TPB) void k_seg_dot(int nchunk, const int* __restrict__ crow,
+                                                 const double* __restrict__ a, const double* __restrict__ b,
+                                                 double* __restrict__ part) {
+    for (int c = blockIdx.x; c < nchunk; c += gridDim.x) {
+        double s = 0.0;
+        for (int i = crow[c] + threadIdx.x; i < crow[c + 1]; i += TPB) s += a[i] * b[i];
+        double t = block_sum(s);
+        if (threadIdx.x == 0) part[c] = t;
+        __syncthreads();
+    }
+}
+
+// out[k] = sum of system k's chunk partials (shift != 0: old[k] = out[k] first)
+__global__ __launch_bounds__(TPB) void k_seg_sum(int ninst, const int* __restrict__ ichunk,
+                                                 const double* __restrict__ part, double* __restrict__ out,
+                                                 double* __restrict__ old) {
+    for (int k = blockIdx.x; k < ninst; k += gridDim.x) {
+        double s = 0.0;
+        for (int c = ichunk[k] + threadIdx.x; c < ichunk[k + 1]; c += TPB) s += part[c];
+        double t = block_sum(s);
+        if (threadIdx.x == 0) {
+            if (old) old[k] = out[k];
+            out[k] = t;
+        }
+        __syncthreads();
+    }
+}
+
+// P = Z + (s/s_old) P per system
+__global__ __launch_bounds__(TPB) void k_bpcg_p(int nchunk, const int* __restrict__ crow,
+                                                const int* __restrict__ cinst, double* __restrict__ P,
+                                                const double* __restrict__ Z, const double* __restrict__ s,
+                                                const double* __restrict__ sold) {
+    for (int c = blockIdx.x; c < nchunk; c += gridDim.x) {
+        const int k = cinst[c];
+        const double beta = s[k] / sold[k];
+        for (int i = crow[c] + threadIdx.x; i < crow[c + 1]; i += TPB) P[i] = Z[i] + beta * P[i];
+    }
+}
+
+// X += a P, R -= a Q with a = s/pq for active systems, 0 otherwise
+__global__ __launch_bounds__(TPB) void k_bpcg_xr(int nchunk, const int* __restrict__ crow,
+                                                 const int* __restrict__ cinst, double* __restrict__ X,
+                                                 double* __restrict__ R, const double* __restrict__ P,
+                                                 const double* __restrict__ Q, const double* __restrict__ s,
+                                                 const double* __restrict__ pq, const double* __restrict__ active) {
+    for (int c = blockIdx.x; c < nchunk; c += gridDim.x) {
+        const int k = cinst[c];
+        const double a = active[k] != 0.0 ? s[k] / pq[k] : 0.0;
+        for (int i = crow[c] + threadIdx.x; i < crow[c + 1]; i += TPB) {
+            X[i] += a * P[i];
+            R[i] -= a * Q[i];
+        }
+    }
+}
+
+// after an iteration: count it for active systems, then stay active while
+// |R| > tol |B| and the iteration limit is not reached
+__global__ void k_bpcg_check(int ninst, const double* __restrict__ rr, const double* __restrict__ hb, double tol,
+                             int max_iter, double* __restrict__ active, double* __restrict__ iters, int count) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < ninst; k += gridDim.x * blockDim.x) {
+        if (active[k] == 0.0) continue;
+        if (count) iters[k] += 1.0;
+        active[k] = (sqrt(rr[k]) > tol * hb[k] && iters[k] < (double)max_iter) ? 1.0 : 0.0;
+    }
+}
+
 // ------------------------------------------------------ coarse solve
 
 // x = inv * b over the dense rows (gathered through idx), trivial rows
@@ -678,6 +753,79 @@ __global__ __launch_bounds__(TPB) void k_coarse(int nd, const int* __restrict__ 
     for (int t = blockIdx.x * TPB + threadIdx.x; t < nt; t += gridDim.x * TPB) {
         int i = tidx[t];
         x[i] = (XT)((double)b[i] / tdiag[t]);
+    }
+}
+
+// Block-diagonal coarse solve of a batch: dense row g belongs to block
+// rblk[g]; block b holds rows [off[b], off[b+1]) of didx and the inverse at
+// inv + moff[b] (row-major, nd_b x nd_b).  One wave per dense row.
+template <typename XT>
+__global__ __launch_bounds__(TPB) void k_bcoarse(int nd, const int* __restrict__ rblk, const int* __restrict__ off,
+                                                 const size_t* __restrict__ moff, const int* __restrict__ didx,
+                                                 const double* __restrict__ inv, int nt, const int* __restrict__ tidx,
+                                                 const double* __restrict__ tdiag, const XT* __restrict__ b,
+                                                 XT* __restrict__ x) {
+    const int wave = (blockIdx.x * TPB + threadIdx.x) / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    const int nw = gridDim.x * TPB / WAVE;
+    for (int g = wave; g < nd; g += nw) {
+        const int bb = rblk[g];
+        const int o = off[bb], m = off[bb + 1] - o;
+        const double* a = inv + moff[bb] + (size_t)(g - o) * m;
+        double s = 0.0;
+        for (int c = lane; c < m; c += WAVE) s += ld_stream<true>(a + c) * (double)b[didx[o + c]];
+        s = wave_sum(s);
+        if (lane == 0) x[didx[g]] = (XT)s;
+    }
+    for (int t = blockIdx.x * TPB + threadIdx.x; t < nt; t += gridDim.x * TPB) {
+        int i = tidx[t];
+        x[i] = (XT)((double)b[i] / tdiag[t]);
+    }
+}
+
+// Gauss-Jordan step k on every block with more than k rows (one wave per
+// row); the wave of row k stores the block's pivot reciprocal
+__global__ __launch_bounds__(TPB) void k_bgj_update(int nd, int k, const int* __restrict__ rblk,
+                                                    const int* __restrict__ off, const size_t* __restrict__ moff,
+                                                    double* __restrict__ M, double* __restrict__ piv) {
+    const int wave = (blockIdx.x * TPB + threadIdx.x) / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    const int nw = gridDim.x * TPB / WAVE;
+    for (int g = wave; g < nd; g += nw) {
+        const int bb = rblk[g];
+        const int o = off[bb], m = off[bb + 1] - o;
+        if (k >= m) continue;
+        double* A = M + moff[bb];
+        const int i = g - o;
+        const double p = 1.0 / A[(size_t)k * m + k];
+        if (i == k) {
+            if (lane == 0) piv[bb] = p;
+            continue;
+        }
+        double* row = A + (size_t)i * m;
+        const double* rowk = A + (size_t)k * m;
+        const double f = row[k] * p;
+        for (int j = lane; j < m; j += WAVE)
+            if (j != k) row[j] -= f * rowk[j];
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_bgj_pivot(int nd, int k, const int* __restrict__ rblk,
+                                                   const int* __restrict__ off, const size_t* __restrict__ moff,
+                                                   double* __restrict__ M, const double* __restrict__ piv) {
+    for (int g = blockIdx.x * TPB + threadIdx.x; g < nd; g += gridDim.x * TPB) {
+        const int bb = rblk[g];
+        const int o = off[bb], m = off[bb + 1] - o;
+        if (k >= m) continue;
+        double* A = M + moff[bb];
+        const int t = g - o;
+        const double p = piv[bb];
+        if (t == k) {
+            A[(size_t)k * m + k] = p;
+        } else {
+            A[(size_t)k * m + t] *= p;
+            A[(size_t)t * m + k] *= -p;
+        }
     }
 }
 
@@ -818,6 +966,29 @@ void elast_grad(hipStream_t st, int ne, const int* tri, const double* xy, int nn
                        blocked, grad);
 }
 
+void seg_dot(hipStream_t st, int nchunk, const int* crow, const double* a, const double* b, double* part) {
+    if (nchunk > 0) hipLaunchKernelGGL(k_seg_dot, dim3(std::min(nchunk, 65535)), dim3(TPB), 0, st, nchunk, crow, a, b, part);
+}
+void seg_sum(hipStream_t st, int ninst, const int* ichunk, const double* part, double* out, double* old) {
+    if (ninst > 0) hipLaunchKernelGGL(k_seg_sum, dim3(std::min(ninst, 65535)), dim3(TPB), 0, st, ninst, ichunk, part, out, old);
+}
+void bpcg_p(hipStream_t st, int nchunk, const int* crow, const int* cinst, double* P, const double* Z, const double* s,
+            const double* sold) {
+    if (nchunk > 0) hipLaunchKernelGGL(k_bpcg_p, dim3(std::min(nchunk, 65535)), dim3(TPB), 0, st, nchunk, crow, cinst, P, Z, s, sold);
+}
+void bpcg_xr(hipStream_t st, int nchunk, const int* crow, const int* cinst, double* X, double* R, const double* P,
+             const double* Q, const double* s, const double* pq, const double* active) {
+    if (nchunk > 0)
+        hipLaunchKernelGGL(k_bpcg_xr, dim3(std::min(nchunk, 65535)), dim3(TPB), 0, st, nchunk, crow, cinst, X, R, P, Q, s, pq,
+                           active);
+}
+void bpcg_check(hipStream_t st, int ninst, const double* rr, const double* hb, double tol, int max_iter,
+                double* active, double* iters, int count) {
+    if (ninst > 0)
+        hipLaunchKernelGGL(k_bpcg_check, dim3(grid_for(ninst)), dim3(TPB), 0, st, ninst, rr, hb, tol, max_iter, active,
+                           iters, count);
+}
+
 void rn_dj(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, const double* G,
            const double* X, double* D, double* J) {
     if (nb > 0) hipLaunchKernelGGL(k_rn_dj, dim3(grid_for(nb)), dim3(TPB), 0, st, nb, idx, lo, up, G, X, D, J);
@@ -953,6 +1124,31 @@ void combine(hipStream_t st, int n, int m, const double* V, size_t ld, const dou
 }
 
 // piv: one device double of scratch
+template <typename XT>
+void bcoarse_solve(hipStream_t st, int nd, const int* rblk, const int* off, const size_t* moff, const int* didx,
+                   const double* inv, int nt, const int* tidx, const double* tdiag, const XT* b, XT* x) {
+    const int waves_per_block = TPB / WAVE;
+    int g = std::max(1, std::min((nd + waves_per_block - 1) / waves_per_block, 8192));
+    g = std::max(g, std::min((nt + TPB - 1) / TPB, 4096));
+    hipLaunchKernelGGL((k_bcoarse<XT>), dim3(g), dim3(TPB), 0, st, nd, rblk, off, moff, didx, inv, nt, tidx, tdiag,
+                       b, x);
+}
+template void bcoarse_solve<double>(hipStream_t, int, const int*, const int*, const size_t*, const int*,
+                                    const double*, int, const int*, const double*, const double*, double*);
+template void bcoarse_solve<float>(hipStream_t, int, const int*, const int*, const size_t*, const int*,
+                                   const double*, int, const int*, const double*, const float*, float*);
+
+void batched_gauss_jordan_inverse(hipStream_t st, int nd, int max_block, const int* rblk, const int* off,
+                                  const size_t* moff, double* M, double* piv) {
+    if (nd <= 0) return;
+    const int gw = std::max(1, std::min((nd + TPB / WAVE - 1) / (TPB / WAVE), 8192));
+    const int gp = std::max(1, std::min((nd + TPB - 1) / TPB, 4096));
+    for (int k = 0; k < max_block; k++) {
+        hipLaunchKernelGGL(k_bgj_update, dim3(gw), dim3(TPB), 0, st, nd, k, rblk, off, moff, M, piv);
+        hipLaunchKernelGGL(k_bgj_pivot, dim3(gp), dim3(TPB), 0, st, nd, k, rblk, off, moff, M, (const double*)piv);
+    }
+}
+
 void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* piv) {
     if (N <= 0) return;
     int g = std::min(N, 4096);

@@ -8,8 +8,14 @@ namespace dev {
 
 // products of one row block staged in LDS: at most BLOCK_NNZ entries and
 // at most BLOCK_ROWS rows per block (one lane per row in the second phase)
-constexpr int BLOCK_NNZ = 2048;
-constexpr int BLOCK_ROWS = 256;
+#ifndef LINA_BLOCK_NNZ
+#define LINA_BLOCK_NNZ 2048
+#endif
+#ifndef LINA_BLOCK_ROWS
+#define LINA_BLOCK_ROWS 256
+#endif
+constexpr int BLOCK_NNZ = LINA_BLOCK_NNZ;
+constexpr int BLOCK_ROWS = LINA_BLOCK_ROWS;
 
 // MODE_OUTER is MODE_SET for the finest-level operator of the outer Krylov
 // loop; a separate instantiation so that profiles report it on its own
@@ -73,6 +79,17 @@ void signs(hipStream_t st, int n, const double* s, const double* x, double* y);
 template <typename T>
 void fix_isolated(hipStream_t st, int m, const int* rows, const double* a, const T* f, T* x);
 void scale_by(hipStream_t st, int n, double* x, const double* scal, int slot, int inverse);
+// batched Krylov over a block-diagonal system: chunks of rows that never
+// straddle two systems (crow: nchunk+1 row starts, cinst: system of a chunk,
+// ichunk: ninst+1 chunk starts), per-system scalars in device arrays
+void seg_dot(hipStream_t st, int nchunk, const int* crow, const double* a, const double* b, double* part);
+void seg_sum(hipStream_t st, int ninst, const int* ichunk, const double* part, double* out, double* old);
+void bpcg_p(hipStream_t st, int nchunk, const int* crow, const int* cinst, double* P, const double* Z, const double* s,
+            const double* sold);
+void bpcg_xr(hipStream_t st, int nchunk, const int* crow, const int* cinst, double* X, double* R, const double* P,
+             const double* Q, const double* s, const double* pq, const double* active);
+void bpcg_check(hipStream_t st, int ninst, const double* rr, const double* hb, double tol, int max_iter,
+                double* active, double* iters, int count);
 // reflective Newton box bookkeeping over the bounded index set (NaN bound: unbounded)
 void rn_dj(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, const double* G,
            const double* X, double* D, double* J);
@@ -108,6 +125,13 @@ void jacobi(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, const XT* 
 template <typename VT, typename XT>
 void jacobi0(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, XT* x, int nlim);
 void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* piv);
+// block-diagonal dense inverse of a batch (block of dense row g: rblk[g];
+// block b: rows [off[b], off[b+1]), matrix at M + moff[b]); piv: one per block
+void batched_gauss_jordan_inverse(hipStream_t st, int nd, int max_block, const int* rblk, const int* off,
+                                  const size_t* moff, double* M, double* piv);
+template <typename XT>
+void bcoarse_solve(hipStream_t st, int nd, const int* rblk, const int* off, const size_t* moff, const int* didx,
+                   const double* inv, int nt, const int* tidx, const double* tdiag, const XT* b, XT* x);
 // level-scheduled ILU(0) triangular solve (upper=0: L, diagonal last; 1: U, diagonal first)
 void trsv(hipStream_t st, const int* rp, const int* ci, const double* v, int nlv, const int* lv_off,
           const int* lv_rows, const double* z, double* w, int upper);

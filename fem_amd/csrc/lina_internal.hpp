@@ -40,6 +40,11 @@ struct CoarseLU {
     std::vector<int> trivial_idx;    // rows with <= 1 nonzero (L=1, U=diag)
     std::vector<double> trivial_diag;
     std::vector<double> dense;       // dense system, row-major, nd x nd
+    // block-diagonal coarsest level (a batch of independent systems): block b
+    // owns dense_idx[blk_off[b] .. blk_off[b+1]) and the row-major square
+    // matrix dense[mat_off[b] ..); empty: one block
+    std::vector<int> blk_off;
+    std::vector<size_t> mat_off;
 };
 
 struct SetupParams {

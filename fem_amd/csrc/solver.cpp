@@ -231,6 +231,98 @@ void Solver::setup_host(const Csr& A, const lina_amg_params& p) {
     for (size_t l = 1; l < H_.lv.size(); l++) colours_[l] = greedy_colouring(H_.lv[l].A, &ncolours_[l]);
 }
 
+// ------------------------------------------------------- autotuning
+
+namespace {
+constexpr int kLprCandidates[] = {1, 4, 8, 16, 32};
+}
+
+template <typename VT, typename XT>
+int Solver::tune_spmv(dev::DMat<VT>& M, const XT* x, XT* y) {
+    if (M.nblk == 0) return M.lpr;
+    float best = 1e30f;
+    int best_lpr = M.lpr;
+    for (int lpr : kLprCandidates) {
+        M.lpr = lpr;
+        dev::spmv<VT, XT>(stream_, M, x, nullptr, y, dev::MODE_SET, nullptr, nullptr);
+        hip_check(hipEventRecord(ev0_, stream_), "event");
+        for (int k = 0; k < 3; k++) dev::spmv<VT, XT>(stream_, M, x, nullptr, y, dev::MODE_SET, nullptr, nullptr);
+        hip_check(hipEventRecord(ev1_, stream_), "event");
+        hip_check(hipEventSynchronize(ev1_), "event sync");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "elapsed");
+        if (ms < best) {
+            best = ms;
+            best_lpr = lpr;
+        }
+    }
+    M.lpr = best_lpr;
+    return best_lpr;
+}
+
+template <typename VT, typename XT>
+int Solver::tune_mcsor(DevLevel& d, dev::DMat<VT>& Ac, const XT* f, XT* x) {
+    if (Ac.nblk == 0 || d.ncolours == 0) return Ac.lpr;
+    float best = 1e30f;
+    int best_lpr = Ac.lpr;
+    auto sweep = [&]() {
+        for (int c = 0; c < d.ncolours; c++)
+            dev::mcsor_colour(stream_, Ac, d.colour_blk[c], d.colour_blk[c + 1] - d.colour_blk[c], d.perm, f, x,
+                              (XT)1);
+    };
+    for (int lpr : kLprCandidates) {
+        Ac.lpr = lpr;
+        sweep();
+        hip_check(hipEventRecord(ev0_, stream_), "event");
+        for (int k = 0; k < 2; k++) sweep();
+        hip_check(hipEventRecord(ev1_, stream_), "event");
+        hip_check(hipEventSynchronize(ev1_), "event sync");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "elapsed");
+        if (ms < best) {
+            best = ms;
+            best_lpr = lpr;
+        }
+    }
+    Ac.lpr = best_lpr;
+    return best_lpr;
+}
+
+// The row partition (lanes per row in the LDS reduction phase) of every
+// matrix of the hierarchy, chosen by timing each candidate on the device:
+// the SpMV-shaped kernels (residual, restriction, prolongation, Jacobi) take
+// their matrix's choice, the multicolour sweeps the colour-permuted copy's.
+void Solver::autotune() {
+    hipStream_t st = stream_;
+    tuned_.clear();
+    for (size_t l = 0; l < L_.size(); l++) {
+        DevLevel& d = L_[l];
+        for (double* v : {d.x, d.f, d.r, d.xo}) dev::fill<double>(st, d.n, v, 1.0);
+        for (float* v : {d.xf, d.ff, d.rf, d.xof}) dev::fill<float>(st, d.n, v, 1.0f);
+        tuned_.push_back({(int)l, 'A', tune_spmv<double, double>(d.A, d.x, d.r), tune_spmv<float, float>(d.Af, d.xf, d.rf)});
+        if (l == 0) continue;
+        DevLevel& c = L_[l - 1];
+        for (double* v : {c.x, c.f}) dev::fill<double>(st, c.n, v, 1.0);
+        for (float* v : {c.xf, c.ff}) dev::fill<float>(st, c.n, v, 1.0f);
+        tuned_.push_back({(int)l, 'P', tune_spmv<double, double>(d.P, c.x, d.r), tune_spmv<float, float>(d.Pf, c.xf, d.rf)});
+        tuned_.push_back({(int)l, 'R', tune_spmv<double, double>(d.R, d.r, c.f), tune_spmv<float, float>(d.Rf, d.rf, c.ff)});
+        dev::fill<double>(st, d.n, d.f, 0.0);
+        dev::fill<float>(st, d.n, d.ff, 0.0f);
+        tuned_.push_back({(int)l, 'C', tune_mcsor<double, double>(d, d.Ac, d.f, d.x),
+                          tune_mcsor<float, float>(d, d.Acf, d.ff, d.xf)});
+    }
+    hip_check(hipStreamSynchronize(st), "autotune");
+}
+
+void Solver::set_hierarchy(HostHierarchy&& H, const lina_amg_params& p) {
+    prm_ = p;
+    H_ = std::move(H);
+    n_ = H_.lv.back().A.n;
+    colours_.assign(H_.lv.size(), std::vector<int>());
+    ncolours_.assign(H_.lv.size(), 0);
+    for (size_t l = 1; l < H_.lv.size(); l++) colours_[l] = greedy_colouring(H_.lv[l].A, &ncolours_[l]);
+}
+
 void Solver::upload() {
     if (!stream_) {
         hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
@@ -329,7 +421,22 @@ void Solver::upload() {
     nt_ = (int)lu.trivial_idx.size();
     didx_ = dupload(lu.dense_idx, B, st);
     inv_ = dupload(lu.dense, B, st);
-    {
+    cbatched_ = lu.blk_off.size() > 2;
+    if (cbatched_) {
+        // block-diagonal coarsest level of a batch: one inverse per block
+        const int nb = (int)lu.blk_off.size() - 1;
+        std::vector<int> rblk(nd_);
+        int maxb = 0;
+        for (int b = 0; b < nb; b++) {
+            maxb = std::max(maxb, lu.blk_off[b + 1] - lu.blk_off[b]);
+            for (int g = lu.blk_off[b]; g < lu.blk_off[b + 1]; g++) rblk[g] = b;
+        }
+        crblk_ = dupload(rblk, B, st);
+        coff_ = dupload(lu.blk_off, B, st);
+        cmoff_ = dupload(lu.mat_off, B, st);
+        double* piv = dalloc<double>(nb, B);
+        dev::batched_gauss_jordan_inverse(st, nd_, maxb, crblk_, coff_, cmoff_, inv_, piv);
+    } else {
         double* piv = dalloc<double>(1, B);
         dev::gauss_jordan_inverse(st, nd_, inv_, piv);
     }
@@ -353,6 +460,7 @@ void Solver::upload() {
     scal_ = dalloc<double>(dev::S_COUNT, B);
     dev::fill(st, dev::S_COUNT, scal_, 0.0);
     hip_check(hipStreamSynchronize(st), "upload sync");
+    if (prm_.autotune) autotune();
 }
 
 void Solver::recompute(const std::vector<double>& val) {
@@ -549,7 +657,10 @@ void Solver::vcycle(int l, const T* f, T* x, bool x_zero) {
     if (l == 0) {
         // the dense-row/trivial-row split need not cover every row
         if (x_zero) dev::fill<T>(st, L_[0].n, x, (T)0);
-        dev::coarse_solve(st, nd_, didx_, inv_, nt_, tidx_, tdiag_, f, x);
+        if (cbatched_)
+            dev::bcoarse_solve(st, nd_, crblk_, coff_, cmoff_, didx_, inv_, nt_, tidx_, tdiag_, f, x);
+        else
+            dev::coarse_solve(st, nd_, didx_, inv_, nt_, tidx_, tdiag_, f, x);
         return;
     }
     DevLevel& d = L_[l];

@@ -89,6 +89,16 @@ class Solver {
     Solver();
     ~Solver();
     void setup_host(const Csr& A, const lina_amg_params& p);
+    // a hierarchy built elsewhere (the block-diagonal union of a batch)
+    void set_hierarchy(HostHierarchy&& H, const lina_amg_params& p);
+    // row-partition autotuning (run by upload when params.autotune != 0)
+    void autotune();
+    struct Tuned {
+        int level;
+        char which;  // A, P, R, C (colour-permuted copy of A)
+        int lpr64, lpr32;
+    };
+    const std::vector<Tuned>& tuned() const { return tuned_; }
     void upload();
     void recompute(const std::vector<double>& val);
     // AMG_recompute_coarse_matrices with a new finest matrix of any pattern
@@ -139,6 +149,11 @@ class Solver {
                              const double* scale, int blocked, const double* u, const double* lam, double* grad);
 
   private:
+    template <typename VT, typename XT>
+    int tune_spmv(dev::DMat<VT>& M, const XT* x, XT* y);
+    template <typename VT, typename XT>
+    int tune_mcsor(DevLevel& d, dev::DMat<VT>& Ac, const XT* f, XT* x);
+    std::vector<Tuned> tuned_;
     lina_amg_params prm_{};
     HostHierarchy H_;
     std::vector<std::vector<int>> colours_;
@@ -157,6 +172,9 @@ class Solver {
     int nd_ = 0, nt_ = 0;
     int *didx_ = nullptr, *tidx_ = nullptr;
     double *inv_ = nullptr, *tdiag_ = nullptr;
+    bool cbatched_ = false;
+    int *crblk_ = nullptr, *coff_ = nullptr;
+    size_t* cmoff_ = nullptr;
     double *ftop_ = nullptr, *rtop_ = nullptr;
     double *kR_ = nullptr, *kP_ = nullptr, *kQ_ = nullptr, *kZ_ = nullptr, *kW_ = nullptr, *kX_ = nullptr,
            *kB_ = nullptr;

@@ -26,7 +26,8 @@ class Params(C.Structure):
                 ("aggr_smoothing_iterations", C.c_int), ("coarsening", C.c_int), ("eps_minus", C.c_double),
                 ("eps_plus", C.c_double), ("matrix_tol", C.c_double), ("smoother", C.c_int), ("omega", C.c_double),
                 ("amg_eps", C.c_double), ("amg_max_iter", C.c_int), ("zero_init", C.c_int),
-                ("precision", C.c_int), ("coarse_smoother", C.c_int), ("coarse_sweeps", C.c_int)]
+                ("precision", C.c_int), ("coarse_smoother", C.c_int), ("coarse_sweeps", C.c_int),
+                ("autotune", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -57,6 +58,13 @@ class RNStats(C.Structure):
 
 
 RN_SOLVER = {"bicgstab": 1, "amg": 2, "pcg": 3, "gmres": 4}
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("iterations_launched", C.c_int), ("active", C.c_int), ("t_solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class LinaError(RuntimeError):
@@ -123,6 +131,13 @@ def lib():
     L.lina_rn_solve.argtypes = [C.c_void_p, _dp, C.POINTER(RNStats)]
     L.lina_rn_set_trust_region.argtypes = [C.c_void_p, C.c_double]
     L.lina_rn_scaled_hessian.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _ip, _ip, _dp]
+    L.lina_batch_create.argtypes = [C.c_int, _ip, _ip, _ip, _dp, C.POINTER(Params)]
+    L.lina_batch_create.restype = C.c_void_p
+    L.lina_batch_destroy.argtypes = [C.c_void_p]
+    L.lina_batch_pcg.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, C.c_int, _dp, _dp, C.POINTER(BatchStats)]
+    L.lina_batch_pcg_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int, _dp, _dp,
+                                        C.POINTER(BatchStats)]
+    L.lina_batch_tuned.argtypes = [C.c_void_p, _ip, C.c_char_p, _ip, _ip, C.c_int]
     _lib = L
     return L
 
@@ -519,3 +534,57 @@ def rn_scaled_hessian(rowptr, col, val, D, G, J):
                                               _d(np.ascontiguousarray(G, dtype=np.float64)),
                                               _d(np.ascontiguousarray(J, dtype=np.float64)), _i(mrp), _i(mci), _d(mv)))
     return mrp, mci[:nnz], mv[:nnz]
+
+
+class Batch:
+    """Batched AMG-PCG over independent systems (lina_batch handle): one
+    block-diagonal union (row_off, rowptr, col, val) as from
+    problems.block_diagonal."""
+
+    def __init__(self, row_off, rowptr, col, val, prm: Params):
+        self.row_off = np.ascontiguousarray(row_off, dtype=np.int32)
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.nsys = len(self.row_off) - 1
+        self.n = int(self.row_off[-1])
+        h = lib().lina_batch_create(self.nsys, _i(self.row_off), _i(self.rowptr), _i(self.col), _d(self.val),
+                                    C.byref(prm))
+        if not h:
+            raise LinaError(f"lina_batch_create failed: {lib().lina_last_error().decode()}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lina_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def pcg(self, b, x0=None, tol=1e-8, max_iter=5000, graph_iters=0):
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        it = np.zeros(self.nsys)
+        rr = np.zeros(self.nsys)
+        st = BatchStats()
+        launched = _check(lib().lina_batch_pcg(self.h, _d(x), _d(b), tol, max_iter, graph_iters, _d(it), _d(rr),
+                                               C.byref(st)))
+        return {"x": x, "iters": it.astype(int), "relres": rr, "launched": launched, **st.as_dict()}
+
+    def pcg_device(self, d_x: int, d_b: int, tol=1e-8, max_iter=5000, graph_iters=0):
+        it = np.zeros(self.nsys)
+        rr = np.zeros(self.nsys)
+        st = BatchStats()
+        launched = _check(lib().lina_batch_pcg_device(self.h, C.c_void_p(d_x), C.c_void_p(d_b), tol, max_iter,
+                                                      graph_iters, _d(it), _d(rr), C.byref(st)))
+        return {"iters": it.astype(int), "relres": rr, "launched": launched, **st.as_dict()}
+
+    def tuned(self):
+        cap = 256
+        lv = np.zeros(cap, np.int32)
+        wh = C.create_string_buffer(cap)
+        l64 = np.zeros(cap, np.int32)
+        l32 = np.zeros(cap, np.int32)
+        m = _check(lib().lina_batch_tuned(self.h, _i(lv), wh, _i(l64), _i(l32), cap))
+        return [(int(lv[i]), wh.raw[i:i + 1].decode(), int(l64[i]), int(l32[i])) for i in range(min(m, cap))]

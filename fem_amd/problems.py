@@ -387,3 +387,47 @@ def adjoint_case(m: int, seed: int = SEED_GRF, seed_obs: int = SEED_GRF_OBS, see
     k = max(1, int(round(frac * mesh.n_nodes)))
     obs = np.sort(np.argsort(keys, kind="stable")[:k])
     return AdjointCase(mesh, theta, theta_obs, obs, nu)
+
+
+# ------------------------------------------------- batched specimens (config 4)
+
+def batch_specimens(count: int = 512, seed: int = SEED_BATCH, side_min: int = 32, side_max: int = 180,
+                    E: float = 210e3, nu: float = 0.3) -> list:
+    """Independent degraded-elasticity specimens: structured triangulated
+    rectangles with nx, ny ~ U{side_min..side_max} nodes (spacing 1/(nx-1),
+    width 1), each with its own seeded notch (height and angle) and
+    elementwise E = E * lognormal(0, 0.5).  Specimen k uses the streams of
+    seed + 16 k."""
+    out = []
+    for k in range(count):
+        sk = seed + 16 * k
+        u = uniform(sk, 2)
+        span = side_max - side_min + 1
+        nx = side_min + min(int(u[0] * span), span - 1)
+        ny = side_min + min(int(u[1] * span), span - 1)
+        lx = 1.0
+        ly = (ny - 1) / (nx - 1)
+        notch = notch_segment(sk + 1, 0.3 * lx, lx, ly)
+        ne = 2 * (nx - 1) * (ny - 1)
+        E_elem = E * np.exp(0.5 * normal(sk + 2, ne))
+        s = degraded_elasticity(ny, seed=sk + 1, nx=nx, lx=lx, ly=ly, E_elem=E_elem, notch=notch, nu=nu)
+        s.meta.update({"nx": nx, "ny": ny, "specimen": k})
+        out.append(s)
+    return out
+
+
+def block_diagonal(systems: list):
+    """The union CSR of independent systems: (row_off, rowptr, col, val, b)."""
+    row_off = np.zeros(len(systems) + 1, dtype=np.int32)
+    for k, s in enumerate(systems):
+        row_off[k + 1] = row_off[k] + s.n
+    nnz_off = np.cumsum([0] + [s.nnz for s in systems])
+    rowptr = np.zeros(row_off[-1] + 1, dtype=np.int32)
+    col = np.empty(nnz_off[-1], dtype=np.int32)
+    val = np.empty(nnz_off[-1])
+    for k, s in enumerate(systems):
+        rowptr[row_off[k] + 1:row_off[k + 1] + 1] = s.rowptr[1:] + nnz_off[k]
+        col[nnz_off[k]:nnz_off[k + 1]] = s.col + row_off[k]
+        val[nnz_off[k]:nnz_off[k + 1]] = s.val
+    b = np.concatenate([s.b for s in systems])
+    return row_off, rowptr, col, val, b

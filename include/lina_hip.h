@@ -68,6 +68,8 @@ typedef struct lina_amg_params {
     int precision;                 /* LINA_PREC_* */
     int coarse_smoother;           /* smoother of the levels below the finest, -1: `smoother` */
     int coarse_sweeps;             /* sweeps per call there (when coarse_smoother >= 0) */
+    int autotune;                  /* 1: time the row-partition candidates of every level's kernels at upload
+                                    * and keep the fastest (changes summation order of multi-lane rows) */
 } lina_amg_params;
 
 typedef struct lina_stats {
@@ -202,6 +204,38 @@ int lina_pcg_operator(lina_solver* s, lina_matrix* op, double* x, double* b, dou
 int lina_elasticity_adjoint(lina_solver* s, int n_nodes, const double* xy, int n_elem, const int* tri, double nu,
                             const double* elem_scale, int blocked, const double* u, const double* dLdu, double tol,
                             int max_iter, double* lambda, double* grad, lina_stats* st);
+
+/* ------------------------------------------------------------------
+ * Batched AMG-PCG (no counterpart in the reference, whose API holds one
+ * system in globals): nsys independent SPD systems given as one
+ * block-diagonal CSR whose system k owns rows [row_off[k], row_off[k+1]).
+ * Each system gets its own AMG_setup hierarchy (params p, equal depth); the
+ * hierarchies are joined into one block-diagonal hierarchy so that every
+ * kernel of the V-cycle serves all systems.  pcg per system with the
+ * relative-residual stop |b_k - A_k x_k| <= tol |b_k|; graph_iters > 0
+ * captures that many iterations into a hipGraph and replays it.
+ * p->autotune = 1 picks each level's row partition by timing.
+ * ------------------------------------------------------------------ */
+typedef struct lina_batch_stats {
+    int iterations_launched; /* iterations run (graph replays x graph_iters) */
+    int active;              /* systems not converged at the end */
+    double t_solve_ms;       /* device time of the solve */
+} lina_batch_stats;
+typedef struct lina_batch lina_batch;
+lina_batch* lina_batch_create(int nsys, const int* row_off, const int* rowptr, const int* col, const double* val,
+                              const lina_amg_params* p);
+void lina_batch_destroy(lina_batch* h);
+/* host arrays: x (guess in, solution out) and b of the union; iters[k] and
+ * relres[k] per system (may be NULL); returns the iterations launched */
+int lina_batch_pcg(lina_batch* h, double* x, const double* b, double tol, int max_iter, int graph_iters,
+                   double* iters, double* relres, lina_batch_stats* st);
+/* the same on device vectors (lina_dev_alloc) */
+int lina_batch_pcg_device(lina_batch* h, double* x, const double* b, double tol, int max_iter, int graph_iters,
+                          double* iters, double* relres, lina_batch_stats* st);
+/* the autotuner's choices: per tuned matrix its level, kind (A, P, R, C =
+ * colour-permuted A) and lanes per row for the FP64 / FP32 copies; returns
+ * the number of entries (arrays of size cap) */
+int lina_batch_tuned(lina_batch* h, int* level, char* which, int* lpr64, int* lpr32, int cap);
 
 /* ------------------------------------------------------------------
  * Reflective Newton (reflective_newton.c:660-711): minimise
