@@ -103,6 +103,211 @@ def measured_traffic(path=os.path.join(ROOT, "profiles", "traffic_spmv_outer_ela
     return None
 
 
+def _cpu_run(code: str, timeout: int = 900) -> dict:
+    """run `code` in a fresh single-threaded interpreter, last JSON line out"""
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=timeout)
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        raise RuntimeError(f"cpu run failed ({r.returncode}): {r.stderr[-1500:]}")
+    return json.loads(lines[-1])
+
+
+def workload_damage(args, lina, problems):
+    """Reflective Newton on the 1025^2 damage subproblem (1.05M DOF, d_old <= d
+    <= 1) with each inner solver; the reference's RN (ILU-PCG inner) on a
+    129^2 member of the family on one host core beside it."""
+    m = 1025
+    res = {}
+    os.environ.setdefault("LINA_RN_TIMING", "1")  # one progress line per Newton step
+    for h_scale in (1.0,):
+        s, dold, ones = problems.damage_system(m, h_scale=h_scale)
+        log(f"damage h_scale={h_scale}: n={s.n} nnz={s.nnz}")
+        for solver, prec in (("pcg", "fp64"), ("gmres", "fp64"), ("bicgstab", "mixed")):
+            prm = lina.rn_params(solver, 1, 6, delta=float(np.sqrt(s.n) / 2), amg={"precision": 1 if prec == "mixed" else 0})
+            t0 = time.perf_counter()
+            rn = lina.ReflectiveNewton(s.rowptr, s.col, s.val, s.b, dold, ones, prm)
+            t1 = time.perf_counter()
+            g = rn.solve(dold)
+            t2 = time.perf_counter()
+            rn.close()
+            key = f"h{h_scale:g}_{solver}_{prec}"
+            res[key] = {"setup_s": t1 - t0, "solve_s": t2 - t1, "newton_iters": g["iters"],
+                        "inner_iters": g["inner_iterations"], "line_searches": g["line_searches"],
+                        "neg_curvature": g["neg_curvature"], "q": g["q"], "dq": g["dq"],
+                        "converged": bool(g["dq"] <= 1e-8)}
+            log(key, res[key])
+    cpu = None
+    try:
+        code = f"""
+import json, sys, time
+sys.path.insert(0, {ROOT!r})
+import numpy as np
+from fem_amd import problems as P
+from oracle.bind import Ref
+ref = Ref(); ref.L.refh_init(2)
+s, dold, ones = P.damage_system(129, h_scale=1.0)
+H = ref.matrix(s.rowptr, s.col, s.val)
+t = time.time()
+r = ref.reflective_newton(H, s.b, dold, ones, dold.copy(), solver=3, dof=1, depth=6, max_iter=50, eps=1e-8,
+                          delta=float(np.sqrt(s.n) / 2))
+print(json.dumps({{"n": s.n, "iters": r["iters"], "t": time.time() - t}}))
+"""
+        c = _cpu_run(code)
+        gi = res["h1_pcg_fp64"]
+        per = c["t"] / max(c["iters"], 1) / c["n"]
+        cpu = {"value": per * m * m * gi["newton_iters"], "unit": "s per RN solve (scaled)", "cores": 1,
+               "kind": "reference",
+               "sample": f"reference reflective_newton with RN_PCG (ILU(0)-PCG inner), 129x129 "
+                         f"({c['n']} DOF): {c['iters']} Newton steps in {c['t']:.2f}s; scaled per Newton step "
+                         f"and DOF to the 1025x1025 solve's step count"}
+    except Exception as e:
+        log(f"cpu baseline skipped: {e}")
+    best = res["h1_pcg_fp64"]["solve_s"]
+    return {"metric": "reflective-Newton damage solve time @1.05M DOF", "value": best, "unit": "s",
+            "n_gpus": 1, "higher_is_better": False, "dtype": "f64 (mixed variant: f32 AMG)",
+            "data": "synthetic (seeded lognormal H, uniform d_old)",
+            "config": {"workload": "damage subproblem P1 1025x1025, H ~ lognormal(0,1), d_old <= d <= 1, RN eps "
+                                   "1e-8; value = the ILU(0)-PCG inner solver"},
+            "results": res, "cpu_baseline": cpu}
+
+
+def workload_batch(args, lina, problems):
+    """512 batched degraded-elasticity specimens (sides U{32..180}), FP32
+    hierarchy / FP64 pcg to relative residual 1e-8, hipGraph replay."""
+    from scipy.sparse.linalg import splu
+    count = 512
+    t0 = time.time()
+    sp = problems.batch_specimens(count)
+    row_off, rp, ci, v, b = problems.block_diagonal(sp)
+    log(f"{count} specimens, {row_off[-1]} DOF, {len(ci)} nnz, built in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    bt = lina.Batch(row_off, rp, ci, v, lina.params("vcycle", 2, 5, precision="mixed", autotune=1))
+    t_setup = time.time() - t0
+    log(f"setup {t_setup:.1f}s, tuned {bt.tuned()}")
+    runs = []
+    for k in range(args.warmup + args.steps):
+        out = bt.pcg(b, tol=1e-8, max_iter=3000, graph_iters=10)
+        if k >= args.warmup:
+            runs.append(out)
+        log(f"solve {k}: {out['t_solve_ms']:.1f} ms, launched {out['launched']}, active {out['active']}")
+    ms = float(np.mean([r["t_solve_ms"] for r in runs]))
+    out = runs[-1]
+    # oracle: direct FP64 solves of every 32nd specimen and the largest
+    idx = sorted(set(list(range(0, count, 32)) + [int(np.argmax([s.n for s in sp]))]))
+    errs = []
+    for k in idx:
+        s = sp[k]
+        xd = splu(s.scipy().tocsc()).solve(s.b)
+        x = out["x"][row_off[k]:row_off[k + 1]]
+        errs.append(float(np.linalg.norm(x - xd) / np.linalg.norm(xd)))
+    cpu = None
+    try:
+        ks = idx[:4]
+        code = f"""
+import json, sys, time
+sys.path.insert(0, {ROOT!r})
+from fem_amd import problems as P
+from oracle.bind import Ref
+ref = Ref(); ref.L.refh_init(2)
+sp = P.batch_specimens({count})
+tot = 0.0; n = 0
+for k in {ks!r}:
+    s = sp[k]
+    M = ref.matrix(s.rowptr, s.col, s.val)
+    o = ref.pcg_amg(M, s.b, dof=2, depth=5, smiter=1, aggr_smiter=1, smoother="ssor", alpha=1.0, amg_eps=1e-2,
+                    amg_max_iter=1, zero_init=True, tol=1e-8, max_iter=5000)
+    tot += o["t_solve"]; n += s.n
+print(json.dumps({{"t": tot, "n": n}}))
+"""
+        c = _cpu_run(code)
+        per_dof = c["t"] / c["n"]
+        cpu = {"value": 1.0 / (per_dof * row_off[-1] / count), "unit": "solves/s", "cores": 1, "kind": "reference",
+               "sample": f"reference pcg + AMG_solve (one symmetric SOR V(1,1), relres 1e-8) on specimens {ks} "
+                         f"({c['n']} DOF, {c['t']:.2f}s); solves/s of the whole batch at that time per DOF"}
+    except Exception as e:
+        log(f"cpu baseline skipped: {e}")
+    return {"metric": "batched AMG-PCG solves/s (512 specimens, relres 1e-8)", "value": count / (ms * 1e-3),
+            "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": True, "dtype": "f32 hierarchy / f64 outer", "data": "synthetic (seeded specimens)",
+            "config": {"workload": "512 degraded-elasticity specimens, sides U{32..180}, lognormal(0,0.5) E",
+                       "dof": int(row_off[-1]), "nnz": int(len(ci)), "depth": 5, "graph_iters": 10},
+            "setup_s": t_setup, "iters": {"min": int(out["iters"].min()), "mean": float(out["iters"].mean()),
+                                          "max": int(out["iters"].max())},
+            "max_relres": float(out["relres"].max()),
+            "oracle_error": {"specimens": idx, "max": max(errs), "median": float(np.median(errs))},
+            "cpu_baseline": cpu}
+
+
+def workload_adjoint(args, lina, problems):
+    """Adjoint gradient on 513^2 plane strain with theta = exp(GRF): forward
+    AMG-PCG, adjoint solve, element gradient kernel; 20 central-difference
+    directions (h = 1e-6) with device solves; FP64 CPU reference gradient."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from scipy.sparse.linalg import splu
+    m = 513
+    case = problems.adjoint_case(m)
+    s, dofs = case.system(case.theta)
+    so, _ = case.system(case.theta_obs)
+    log(f"adjoint case n={s.n} elements={len(case.theta)}")
+    t0 = time.time()
+    amg = lina.AMG(s.rowptr, s.col, s.val, lina.params("vcycle", 2, 7, precision="mixed"))
+    t_setup = time.time() - t0
+    op_obs = lina.Matrix(so.rowptr, so.col, so.val)
+    u_obs = amg.pcg_operator(op_obs, so.b, tol=1e-12, max_iter=20000)["x"]
+    op_obs.close()
+    times = []
+    for k in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        fw = amg.pcg(s.b, tol=1e-12, max_iter=20000, stop="relres")
+        u = fw["x"]
+        bw = amg.elasticity_adjoint(case.mesh.xy, case.mesh.tri, case.nu, u, case.dloss(u, u_obs, dofs), tol=1e-12,
+                                    max_iter=20000)
+        if k >= args.warmup:
+            times.append(time.perf_counter() - t0)
+    grad = bw["grad"]
+    # central differences with device solves (cancellation-free form)
+    import adjoint_ref as R
+    fd_err = []
+    h = 1e-6
+    for v in R.directions(len(case.theta), 20):
+        r0, rv = R.fd_pieces(case, u, v)
+        d = []
+        for sg in (1, -1):
+            sy, _ = case.system(case.theta + sg * h * v)
+            op = lina.Matrix(sy.rowptr, sy.col, sy.val)
+            d.append(amg.pcg_operator(op, r0 + sg * h * rv, tol=1e-12, max_iter=20000)["x"])
+            op.close()
+        fd = R.fd_quotient(case, d[0], d[1], u, u_obs, h)
+        fd_err.append(abs(fd - grad @ v) / abs(grad @ v))
+        log(f"FD direction {len(fd_err)}: rel err {fd_err[-1]:.2e}")
+    log(f"FD max rel err {max(fd_err):.2e}; CPU reference gradient (SuperLU) ...")
+    # FP64 CPU reference gradient (direct solves)
+    t0 = time.time()
+    lu = splu(s.scipy().tocsc())
+    u_ref = lu.solve(s.b)
+    uo_ref = splu(so.scipy().tocsc()).solve(so.b)
+    lam_ref = lu.solve(case.dloss(u_ref, uo_ref, dofs))
+    Ke = R.element_matrices(case.mesh, case.nu)
+    ed = R.element_dofs(case.mesh)
+    g_ref = -np.einsum("ei,eij,ej->e", lam_ref[ed], Ke, u_ref[ed])
+    t_cpu = time.time() - t0
+    gerr = float(np.linalg.norm(grad - g_ref) / np.linalg.norm(g_ref))
+    ms = float(np.mean(times)) * 1e3
+    return {"metric": "adjoint gradient time (forward + adjoint + dL/dtheta) @526k DOF", "value": ms / 1e3,
+            "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": False, "dtype": "f32 hierarchy / f64 outer",
+            "data": "synthetic (seeded GRF moduli and observations)",
+            "config": {"workload": "P1 plane strain 513x513, theta = exp(GRF length 0.1 var 0.25), 5% observed nodes",
+                       "dof": s.n, "elements": len(case.theta), "depth": 7, "tol": 1e-12},
+            "setup_s": t_setup, "forward_iters": fw["iters"], "adjoint_iters": bw["iters"],
+            "fd_check": {"directions": 20, "h": h, "max_rel_err": max(fd_err)},
+            "cpu_reference_gradient": {"rel_err": gerr, "time_s": t_cpu,
+                                       "kind": "scipy splu FP64 forward/adjoint + B^T C B element matrices"},
+            "cpu_baseline": {"value": t_cpu, "unit": "s", "cores": 1, "kind": "port",
+                             "sample": "FP64 CPU reference gradient (SuperLU direct solves) at full size"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +321,9 @@ def main():
     ap.add_argument("--coarse-sweeps", type=int, default=1, help="l1-Jacobi sweeps on the levels below the finest")
     ap.add_argument("--kernels-only", action="store_true",
                     help="no solves: a 1 GiB device copy and a few finest-level SpMVs, for counter runs")
+    ap.add_argument("--workload", default="elasticity", choices=["elasticity", "damage", "batch", "adjoint"],
+                    help="elasticity: the headline 2.1M-DOF AMG-PCG (default); damage: reflective Newton on the "
+                         "1025^2 damage subproblem; batch: 512 batched specimens; adjoint: 513^2 adjoint gradient")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -124,6 +332,12 @@ def main():
         sys.exit(2)
 
     from fem_amd import lina, problems
+
+    if args.workload != "elasticity":
+        out = {"damage": workload_damage, "batch": workload_batch, "adjoint": workload_adjoint}[args.workload](
+            args, lina, problems)
+        print(json.dumps(out))
+        return
 
     t0 = time.time()
     s = problems.degraded_elasticity(args.size)

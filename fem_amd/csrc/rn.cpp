@@ -18,8 +18,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <memory>
 #include <stdexcept>
 #include <vector>
@@ -141,6 +144,16 @@ struct RootSearch {
         return out;
     }
 };
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// LINA_RN_TIMING set: one line per Newton step with the time of its phases
+bool rn_timing() {
+    static const bool v = getenv("LINA_RN_TIMING") != nullptr;
+    return v;
+}
 
 uint64_t splitmix(uint64_t& s) {
     uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
@@ -344,8 +357,13 @@ class ReflectiveNewton {
             download(D_, hD);
             download(G_, hG);
             download(J_, hJ);
+            const double t0 = now_s();
             Csr M = scaled_hessian(H_, hD, hG, hJ);
+            t_m_ = now_s() - t0;
             iterate(M);
+            if (rn_timing())
+                fprintf(stderr, "[rn] step %d: M %.3fs (nnz %d), inner %.3fs (build %.3fs, %d its), rest %.3fs\n",
+                        iter, t_m_, M.nnz(), t_inner_, t_build_, last_inner_, now_s() - t0 - t_m_ - t_inner_);
             q = functional(X_);
             dq = std::fabs((q - q0) / q0);
             iter++;
@@ -419,11 +437,14 @@ class ReflectiveNewton {
         dev::scale(st_, n, V2_, -1.);
         dev::copy(st_, n, (const double*)V2_, bs_);
         hip_check(hipStreamSynchronize(st_), "sync");
+        const double t_in = now_s();
         int status = 1, iter = 0;
         switch (p_.solver) {
             case LINA_RN_PCG: {
                 IluSystem sys;
+                const double tb = now_s();
                 sys.build(M, true);
+                t_build_ = now_s() - tb;
                 if (!sys.ilu_ok()) {
                     status = 0;
                     break;
@@ -469,6 +490,8 @@ class ReflectiveNewton {
                 throw std::invalid_argument("unknown inner solver");
         }
         stats_.inner_iterations += iter;
+        t_inner_ = now_s() - t_in;
+        last_inner_ = iter;
         if (status == 0) {
             stats_.neg_curvature++;
             neg_curvature(M, dM);
@@ -652,6 +675,8 @@ class ReflectiveNewton {
     int* idxd_ = nullptr;
     double* host_ = nullptr;
     double last_curvature_ = 0;
+    double t_m_ = 0, t_inner_ = 0, t_build_ = 0;
+    int last_inner_ = 0;
     lina_rn_stats stats_{};
 };
 
