@@ -18,6 +18,10 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblina_hip.so")
+# the LinA drop-in: C host code with LinA's entry points over liblina_hip's C ABI
+HOST = os.path.join(HERE, "host")
+DROPIN_LIB = os.path.join(HERE, "liblinalg_hip.so")
+DROPIN_SOURCES = ["linalg.c", "pcgs.c", "multigrid.c", "bicgstab.c", "reflective_newton.c"]
 
 SOURCES = ["amg_setup.cpp", "solver.cpp", "capi.cpp", "kernels.hip", "krylov.cpp", "rn.cpp", "batch.cpp",
            "dropin.cpp"]
@@ -54,7 +58,28 @@ def build(verbose: bool = True, out: str = LIB, build_dir: str = BUILD, extra=()
     os.replace(tmp, out)
     if verbose:
         print(f"built {out}")
+    if out == LIB:
+        build_dropin(verbose)
     return out
+
+
+def build_dropin(verbose: bool = True) -> str:
+    """gcc: fem_amd/host/*.c -> liblinalg_hip.so linked to liblina_hip.so"""
+    cc = os.environ.get("CC", shutil.which("gcc") or "cc")
+    srcs = [os.path.join(HOST, s) for s in DROPIN_SOURCES]
+    tmp = DROPIN_LIB + ".tmp"
+    cmd = [cc, "-O2", "-fPIC", "-shared", "-std=c99", "-D_POSIX_C_SOURCE=200809L", "-Wall", "-Wno-unused-function",
+           "-ffp-contract=off", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "include", "linalg"),
+           *srcs, "-o", tmp, "-L", HERE, "-llina_hip", "-Wl,-rpath,$ORIGIN", "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"drop-in build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if r.stderr.strip() and verbose:
+        print(r.stderr)
+    os.replace(tmp, DROPIN_LIB)
+    if verbose:
+        print(f"built {DROPIN_LIB}")
+    return DROPIN_LIB
 
 
 if __name__ == "__main__":

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -227,6 +229,15 @@ int lina_amg_recompute(lina_solver* s, const double* val) {
     });
 }
 
+int lina_amg_recompute_csr(lina_solver* s, int n, const int* rowptr, const int* col, const double* val) {
+    return guard([&]() {
+        if (!s) throw std::invalid_argument("null solver");
+        Csr M = make_csr(n, rowptr, col, val);
+        s->s.recompute_matrix(M);
+        return 0;
+    });
+}
+
 int lina_pcg(lina_solver* s, double* x, double* b, double tol, int max_iter, int stop, lina_stats* st) {
     return guard([&]() {
         if (!s->s.uploaded()) throw std::invalid_argument("solver was created without device upload");
@@ -245,6 +256,15 @@ int lina_amg_solve(lina_solver* s, const double* f, double* x) {
     return guard([&]() {
         if (!s->s.uploaded()) throw std::invalid_argument("solver was created without device upload");
         return s->s.amg_solve_host(f, x);
+    });
+}
+
+int lina_amg_solve_device(lina_solver* s, const double* f, double* x) {
+    return guard([&]() {
+        if (!s->s.uploaded()) throw std::invalid_argument("solver was created without device upload");
+        int it = s->s.amg_solve_dev(f, x);
+        lina::hip_check(hipStreamSynchronize(s->s.stream()), "sync");
+        return it;
     });
 }
 
@@ -307,6 +327,60 @@ int lina_amg_kernel_timing(lina_solver* s, int on) {
     return guard([&]() {
         if (!s) throw std::invalid_argument("null solver");
         s->s.set_kernel_timing(on != 0);
+        return 0;
+    });
+}
+
+// ---- device vector primitives on the library's primitive stream
+namespace {
+struct Prim {
+    hipStream_t st = nullptr;
+    double* part = nullptr;
+    double* scal = nullptr;
+    double* host = nullptr;
+    void init() {
+        if (st) return;
+        lina::hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        lina::hip_check(hipMalloc((void**)&part, 4096 * sizeof(double)), "hipMalloc");
+        lina::hip_check(hipMalloc((void**)&scal, 8 * sizeof(double)), "hipMalloc");
+        lina::hip_check(hipHostMalloc((void**)&host, 8 * sizeof(double)), "hipHostMalloc");
+    }
+};
+Prim g_prim;
+}  // namespace
+
+double lina_dev_dot(const double* a, const double* b, long n) {
+    double out = NAN;
+    int rc = guard([&]() {
+        if (n < 0 || n > INT32_MAX || (n > 0 && (!a || !b))) throw std::invalid_argument("bad arguments");
+        g_prim.init();
+        int np = lina::dev::dot_part<double>(g_prim.st, (int)n, a, b, g_prim.part);
+        lina::dev::finalize(g_prim.st, np, g_prim.part, g_prim.scal, 0, 0);
+        lina::hip_check(hipMemcpyAsync(g_prim.host, g_prim.scal, sizeof(double), hipMemcpyDeviceToHost, g_prim.st),
+                        "d2h");
+        lina::hip_check(hipStreamSynchronize(g_prim.st), "sync");
+        out = g_prim.host[0];
+        return 0;
+    });
+    return rc == 0 ? out : NAN;
+}
+
+int lina_dev_axpby(long n, double a, double* x, double b, const double* y) {
+    return guard([&]() {
+        if (n < 0 || n > INT32_MAX || (n > 0 && (!x || !y))) throw std::invalid_argument("bad arguments");
+        g_prim.init();
+        lina::dev::axpby(g_prim.st, (int)n, a, x, b, y);
+        lina::hip_check(hipStreamSynchronize(g_prim.st), "sync");
+        return 0;
+    });
+}
+
+int lina_dev_scale(long n, double* x, double a) {
+    return guard([&]() {
+        if (n < 0 || n > INT32_MAX || (n > 0 && !x)) throw std::invalid_argument("bad arguments");
+        g_prim.init();
+        lina::dev::scale(g_prim.st, (int)n, x, a);
+        lina::hip_check(hipStreamSynchronize(g_prim.st), "sync");
         return 0;
     });
 }
@@ -436,6 +510,46 @@ int lina_ilu_factor_copy(const lina_ilu* h, char which, int* rowptr, int* col, d
     std::memcpy(col, M.ci.data(), M.nnz() * sizeof(int));
     std::memcpy(val, M.v.data(), M.nnz() * sizeof(double));
     return 0;
+}
+
+lina_ilu* lina_ilu_create_factors(int n, const int* lrp, const int* lci, const double* lv, const int* urp,
+                                  const int* uci, const double* uv) {
+    lina_ilu* h = nullptr;
+    int rc = guard([&]() {
+        auto mk = [&](const int* rp, const int* ci, const double* v, bool diag_last) {
+            Csr T;
+            T.n = T.ncols = n;
+            T.rp.assign(rp, rp + n + 1);
+            if (T.rp[0] != 0) throw std::invalid_argument("rowptr[0] != 0");
+            for (int i = 0; i < n; i++) {
+                if (T.rp[i + 1] <= T.rp[i]) throw std::invalid_argument("every factor row needs its diagonal");
+                const int d = diag_last ? T.rp[i + 1] - 1 : T.rp[i];
+                if (ci[d] != i) throw std::invalid_argument("factor diagonal not in place");
+            }
+            T.ci.assign(ci, ci + T.rp[n]);
+            T.v.assign(v, v + T.rp[n]);
+            for (int c : T.ci)
+                if (c < 0 || c >= n) throw std::invalid_argument("factor column out of range");
+            return T;
+        };
+        Csr L = mk(lrp, lci, lv, true), U = mk(urp, uci, uv, false);
+        h = new lina_ilu();
+        h->s.set_factors(L, U, true);
+        return 0;
+    });
+    if (rc < 0) {
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+int lina_ilu_apply_device(lina_ilu* h, const double* z, double* w) {
+    return guard([&]() {
+        h->s.apply(z, w);
+        lina::hip_check(hipStreamSynchronize(h->s.stream()), "sync");
+        return 0;
+    });
 }
 
 int lina_ilu_apply(lina_ilu* h, const double* z, double* w) {

@@ -18,6 +18,9 @@ class IluSystem {
     ~IluSystem();
     // incomplete_LU_factorization on the host (bit-identical factors), then upload
     void build(const Csr& A, bool upload = true);
+    // PCG_default(A, L, U) with factors computed elsewhere (L: diagonal last
+    // in each row, U: diagonal first); only the preconditioner is available
+    void set_factors(const Csr& L, const Csr& U, bool upload = true);
     void upload();
     bool ilu_ok() const { return ilu_ok_ != 0; }
     const Csr& L() const { return L_; }

@@ -115,6 +115,16 @@ void IluSystem::build(const Csr& A, bool upload) {
     if (upload) this->upload();
 }
 
+void IluSystem::set_factors(const Csr& L, const Csr& U, bool up) {
+    if (L.n != U.n) throw std::invalid_argument("L and U sizes differ");
+    A_ = Csr();
+    L_ = L;
+    U_ = U;
+    ilu_ok_ = 1;
+    n_ = L.n;
+    if (up) upload();
+}
+
 void IluSystem::upload() {
     if (!stream_) {
         hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
@@ -123,7 +133,7 @@ void IluSystem::upload() {
     bufs_.reset(new DevBuffers());
     DevBuffers& B = *bufs_;
     hipStream_t st = stream_;
-    dA_ = upload_csr(A_, B, st);
+    if (A_.n > 0) dA_ = upload_csr(A_, B, st);
     dL_ = upload_csr(L_, B, st);
     dU_ = upload_csr(U_, B, st);
     std::vector<int> off, rows;

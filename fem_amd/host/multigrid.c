@@ -1,0 +1,182 @@
+/* AMG entry points of the LinA drop-in (LinA/Multigrid.c): the hierarchy is
+ * built by liblina_hip (bit-identical to AMG_setup) and kept in HBM;
+ * AMG_solve runs on the GPU with the settings of the global AMG_set_*
+ * calls in effect at the time of the solve, as in the reference. */
+#include <string.h>
+
+#include "../../include/linalg/Multigrid.h"
+#include "dropin.h"
+
+/* the reference's global defaults (Multigrid.c:35-59) */
+static double g_eps_minus = 0.3, g_eps_plus = 0.5, g_matrix_tol = 0.1;
+static int g_max_iter = 20;
+static double g_eps = 1e-8;
+static double g_alpha = 1.0;
+static int g_info = YES;
+static int g_smoother = LINA_SMOOTHER_SOR;
+static int g_dof = 1;
+
+void AMG_SOR_smoother(sparse_matrix* A, double* F, double* Sol, int* Label, double alpha, int fix, int iter,
+                      int deg_freedom) {
+    (void)A; (void)F; (void)Sol; (void)Label; (void)alpha; (void)fix; (void)iter; (void)deg_freedom;
+    fprintf(stderr, "liblinalg_hip: AMG_SOR_smoother runs inside AMG_solve on the GPU; select it with "
+                    "AMG_set_smoother\n");
+}
+
+void AMG_parallel_Jacobi_smoother(sparse_matrix* A, double* F, double* Sol, int* Label, double alpha, int fix,
+                                  int iter, int deg_freedom) {
+    (void)A; (void)F; (void)Sol; (void)Label; (void)alpha; (void)fix; (void)iter; (void)deg_freedom;
+    fprintf(stderr, "liblinalg_hip: AMG_parallel_Jacobi_smoother runs inside AMG_solve on the GPU; select it "
+                    "with AMG_set_smoother\n");
+}
+
+void AMG_set_smoother(void (*Smoother)(sparse_matrix*, double*, double*, int*, double, int, int, int)) {
+    if (Smoother == NULL || Smoother == AMG_SOR_smoother) g_smoother = LINA_SMOOTHER_SOR;
+    else if (Smoother == AMG_parallel_Jacobi_smoother) g_smoother = LINA_SMOOTHER_JACOBI;
+    else fprintf(stderr, "liblinalg_hip: a caller-defined AMG smoother is not run on the GPU; keeping SOR\n");
+}
+
+void AMG_print_info(int s) { g_info = s; }
+int AMG_info_set(void) { return g_info; }
+void AMG_set_SOR_relaxation_coeff(double alpha) { g_alpha = alpha; }
+void set_connection_thresholds(double negative, double positive) {
+    g_eps_minus = negative;
+    g_eps_plus = positive;
+}
+void AMG_set_eps(double eps) { g_eps = eps; }
+void AMG_set_max_iter(int iter) { g_max_iter = iter; }
+int AMG_get_max_iter(void) { return g_max_iter; }
+void AMG_set_deg_of_freedom(int n) { g_dof = n; }
+void AMG_set_matrix_tolerance(double tol) { g_matrix_tol = tol; }
+
+int* AMG_default_iterations(int depth) {
+    int i;
+    int* it = (int*)malloc((depth > 0 ? depth : 1) * sizeof(int));
+    for (i = 0; i < depth; i++) it[i] = 1;
+    return it;
+}
+
+/* params of a hierarchy, refreshed from the globals before each solve */
+typedef struct {
+    lina_amg_params p;
+} amg_side;
+
+void dropin_amg_sync(void* info) {
+    amg_system_info* I = (amg_system_info*)info;
+    amg_side* sd = (amg_side*)I->settings;
+    sd->p.smoother = g_smoother;
+    sd->p.omega = g_alpha;
+    sd->p.amg_eps = g_eps;
+    sd->p.amg_max_iter = g_max_iter;
+    dropin_check(lina_amg_set_params((lina_solver*)I->device, &sd->p), "AMG settings");
+}
+
+amg_system_info* AMG_setup(sparse_matrix* A, int dof, int depth, int smoothing_iterations,
+                           int aggr_smoothing_iterations, int coarsening, int cycle_structure,
+                           int prolongation_method,
+                           void (*coarsest_solver)(sparse_matrix*, sparse_matrix*, double*, double*, int),
+                           int* (*Get_iter)(int)) {
+    amg_system_info* I;
+    amg_side* sd;
+    dropin_csr c;
+    int l;
+    if (cycle_structure != STAND_ALONE)
+        fprintf(stderr, "liblinalg_hip: only the STAND_ALONE V-cycle is built; using it\n");
+    if (coarsest_solver || Get_iter)
+        fprintf(stderr, "liblinalg_hip: caller-defined coarse solvers and cycle counts are not used\n");
+    sd = (amg_side*)calloc(1, sizeof(amg_side));
+    lina_params_reference(&sd->p, dof, depth);
+    sd->p.smoothing_iterations = smoothing_iterations;
+    sd->p.aggr_smoothing_iterations = aggr_smoothing_iterations;
+    sd->p.coarsening = coarsening;
+    sd->p.eps_minus = g_eps_minus;
+    sd->p.eps_plus = g_eps_plus;
+    sd->p.matrix_tol = g_matrix_tol;
+    sd->p.smoother = g_smoother;
+    sd->p.omega = g_alpha;
+    sd->p.amg_eps = g_eps;
+    sd->p.amg_max_iter = g_max_iter;
+    c = dropin_to_csr(A);
+    I = (amg_system_info*)calloc(1, sizeof(amg_system_info));
+    I->device = lina_amg_create(c.n, c.rp, c.ci, c.v, &sd->p, 1);
+    dropin_free_csr(&c);
+    if (!I->device) {
+        dropin_check(-1, "AMG_setup");
+        free(sd);
+        free(I);
+        return NULL;
+    }
+    I->settings = sd;
+    I->depth = lina_amg_depth((lina_solver*)I->device);
+    I->Fine_sizes = (int*)calloc(I->depth, sizeof(int));
+    I->Coarse_sizes = (int*)calloc(I->depth, sizeof(int));
+    for (l = 0; l < I->depth; l++) {
+        int f, co, na, np, nr;
+        lina_amg_level_info((lina_solver*)I->device, l, &f, &co, &na, &np, &nr);
+        I->Fine_sizes[l] = f;
+        I->Coarse_sizes[l] = co;
+    }
+    I->AMG_cycle_structure = STAND_ALONE;
+    I->AMG_prolongation_method = prolongation_method;
+    I->AMG_smoothing_iterations = smoothing_iterations;
+    I->AMG_aggr_smoothing_iterations = aggr_smoothing_iterations;
+    I->AMG_coarsening = coarsening;
+    I->AMG_info = g_info;
+    I->AMG_deg_of_freedom = dof;
+    I->Get_iter = AMG_default_iterations;
+    I->main_size = A->size;
+    return I;
+}
+
+int AMG_solve(amg_system_info* Data, double* F, double* Fine_Sol, amg_statistics* Stat) {
+    int it;
+    dropin_amg_sync(Data);
+    it = lina_amg_solve((lina_solver*)Data->device, F, Fine_Sol);
+    dropin_check(it, "AMG_solve");
+    if (Stat) AMG_add_stat(Stat, it, 0.0);
+    return it;
+}
+
+void AMG_recompute_coarse_matrices(amg_system_info* Info, sparse_matrix* New_matrix) {
+    dropin_csr c = dropin_to_csr(New_matrix);
+    dropin_check(lina_amg_recompute_csr((lina_solver*)Info->device, c.n, c.rp, c.ci, c.v),
+                 "AMG_recompute_coarse_matrices");
+    dropin_free_csr(&c);
+}
+
+void AMG_free_data(amg_system_info* Data) {
+    if (!Data) return;
+    lina_amg_destroy((lina_solver*)Data->device);
+    free(Data->settings);
+    free(Data->Fine_sizes);
+    free(Data->Coarse_sizes);
+    Data->device = NULL;
+    Data->settings = NULL;
+    Data->Fine_sizes = Data->Coarse_sizes = NULL;
+}
+
+/* statistics (Multigrid.c AMG_init_stat / AMG_add_stat / AMG_print_stat) */
+amg_statistics AMG_init_stat(const char* Out) {
+    amg_statistics s;
+    memset(&s, 0, sizeof(s));
+    if (Out) strncpy(s.File_name, Out, sizeof(s.File_name) - 1);
+    return s;
+}
+
+void AMG_add_stat(amg_statistics* Stat, int iter, double res) {
+    Stat->Iterations = (int*)realloc(Stat->Iterations, (Stat->calls + 1) * sizeof(int));
+    Stat->Residuals = (double*)realloc(Stat->Residuals, (Stat->calls + 1) * sizeof(double));
+    Stat->Iterations[Stat->calls] = iter;
+    Stat->Residuals[Stat->calls] = res;
+    Stat->calls++;
+    if (iter > Stat->max_iter) Stat->max_iter = iter;
+    if (res > Stat->max_res) Stat->max_res = res;
+}
+
+void AMG_print_stat(amg_statistics* Stat) {
+    FILE* f = Stat->File_name[0] ? fopen(Stat->File_name, "w") : NULL;
+    int i;
+    if (!f) return;
+    for (i = 0; i < Stat->calls; i++) fprintf(f, "%d\t%e\n", Stat->Iterations[i], Stat->Residuals[i]);
+    fclose(f);
+}

@@ -115,6 +115,8 @@ int lina_amg_colours(const lina_solver* s, int level, int* colours);
 /* Values-only update with fixed P/R (AMG_recompute_coarse_matrices,
  * Multigrid.c:1756-1780).  The pattern of A must be unchanged. */
 int lina_amg_recompute(lina_solver* s, const double* val);
+/* the same with a new finest matrix of any pattern (P and R kept) */
+int lina_amg_recompute_csr(lina_solver* s, int n, const int* rowptr, const int* col, const double* val);
 
 /* pcg (PCGS.c:163-233) preconditioned by this hierarchy; x (initial guess,
  * overwritten) and b follow the reference's in/out semantics, including the
@@ -127,6 +129,8 @@ int lina_pcg_device(lina_solver* s, double* d_x, double* d_b, double tol, int ma
  * guess x until |f-Ax|/|f-Ax0| <= amg_eps or amg_max_iter.  Host pointers;
  * returns the number of V-cycles. */
 int lina_amg_solve(lina_solver* s, const double* f, double* x);
+/* the same on device vectors */
+int lina_amg_solve_device(lina_solver* s, const double* f, double* x);
 
 /* BiCGStab_solve (BiCGStab.c:115-198) preconditioned by AMG_solve with this
  * hierarchy (warm-started PP/PS as in the reference); stops at
@@ -141,6 +145,12 @@ void lina_ilu_destroy(lina_ilu* h);
 int lina_ilu_ok(const lina_ilu* h); /* 0: zero pivot, as the reference's return flag */
 int lina_ilu_factor_nnz(const lina_ilu* h, char which);
 int lina_ilu_factor_copy(const lina_ilu* h, char which, int* rowptr, int* col, double* val);
+/* PCG_default(A, L, U) (PCGS.c:29-34) with factors given by the caller (L:
+ * diagonal last in each row, U: diagonal first); NULL on error */
+lina_ilu* lina_ilu_create_factors(int n, const int* lrp, const int* lci, const double* lv, const int* urp,
+                                  const int* uci, const double* uv);
+/* w = U^-1 L^-1 z on device vectors */
+int lina_ilu_apply_device(lina_ilu* h, const double* z, double* w);
 /* w = U^-1 L^-1 z (PCG_preconlr, PCGS.c:61-108) */
 int lina_ilu_apply(lina_ilu* h, const double* z, double* w);
 /* pcg with PCG_default's ILU(0) preconditioner (PCGS.c:29-34, 163-233) */
@@ -157,6 +167,14 @@ double* lina_dev_alloc(long n);
 int lina_dev_free(double* d);
 int lina_dev_copy(double* dst, const double* src, long n, int kind);
 int lina_dev_fill(double* d, long n, double value);
+/* device vector primitives for host code that drives its own Krylov loop
+ * (the LinA drop-in layer): each returns after completion.
+ * scalar (linear_algebra.c:256-263) on device vectors, deterministic tree order */
+double lina_dev_dot(const double* a, const double* b, long n);
+/* x = a*x + b*y  (vector_add with a = 1, linear_algebra.c:311-316) */
+int lina_dev_axpby(long n, double a, double* x, double b, const double* y);
+/* x = a*x  (scalar_mult, linear_algebra.c:299-304) */
+int lina_dev_scale(long n, double* x, double a);
 
 /* single kernels on the finest level for verification and timing:
  * y = A x (precision FP64 or FP32 copy of A), host pointers */

@@ -1,0 +1,141 @@
+"""The LinA drop-in on the GPU against the compiled reference: the same
+caller code (a C program compiled once against include/linalg and linked
+against either library, and ctypes calls through the LinA names) gives the
+same iteration counts and iterates."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fem_amd import problems as P
+from tests.dropin_bind import DROPIN_SO, LinA, ROOT, dptr
+
+pytestmark = pytest.mark.gpu
+
+from oracle.bind import REF_SO  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def libs(gpu, ref):
+    return LinA(DROPIN_SO), LinA(ref.L)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_c_caller_links_against_either_library(tmp_path):
+    obj = tmp_path / "example.o"
+    inc = os.path.join(ROOT, "include", "linalg")
+    subprocess.run(["gcc", "-std=c99", "-O2", "-c", "-I", inc, "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "dropin_example.c"), "-o", str(obj)], check=True)
+    out = {}
+    for name, lib in (("dropin", DROPIN_SO), ("reference", REF_SO)):
+        exe = tmp_path / f"example_{name}"
+        d = os.path.dirname(lib)
+        subprocess.run(["gcc", str(obj), "-o", str(exe), lib, f"-Wl,-rpath,{d}", "-lm", "-fopenmp"], check=True)
+        r = subprocess.run([str(exe), "65"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        line = [l for l in r.stdout.splitlines() if l.startswith("iterations")][-1]
+        out[name] = dict(zip(line.split()[0::2], map(float, line.split()[1::2])))
+        say(name, out[name])
+    assert abs(out["dropin"]["iterations"] - out["reference"]["iterations"]) <= 1
+    assert abs(out["dropin"]["relres"] / out["reference"]["relres"] - 1) < 1e-6
+    assert abs(out["dropin"]["residuum"] / out["reference"]["residuum"] - 1) < 1e-9
+    assert abs(out["dropin"]["x0"] / out["reference"]["x0"] - 1) < 1e-12
+
+
+def _solve_both(libs, build, solve):
+    res = []
+    for lib in libs:
+        ctx = build(lib)
+        res.append(solve(lib, ctx))
+    return res
+
+
+def test_pcg_amg_matches_reference(libs):
+    s = P.degraded_elasticity(17)
+
+    def run(lib):
+        A = lib.from_csr(s.rowptr, s.col, s.val)
+        lib.L.PCG_default(A, None, None)
+        lib.L.PCG_set_AMG_preconditioner(A, 2, 4)
+        x = np.zeros(s.n)
+        b = s.b.copy()
+        it = lib.L.pcg(dptr(x), dptr(b), s.n, 1e-10, 1000)
+        return it, x, lib.L.get_pcg_residuum()
+
+    (i1, x1, r1), (i2, x2, r2) = run(libs[0]), run(libs[1])
+    say("pcg amg iters", i1, i2, "rel", rel(x1, x2))
+    assert abs(i1 - i2) <= 1 and rel(x1, x2) < 1e-12 and abs(r1 / r2 - 1) < 1e-6
+
+
+def test_pcg_ilu_matches_reference(libs):
+    s = P.damage_system(33)[0]
+
+    def run(lib):
+        A = lib.from_csr(s.rowptr, s.col, s.val)
+        L = lib.L.sparse_zero(s.n)
+        U = lib.L.sparse_zero(s.n)
+        lib.L.incomplete_LU_factorization(A, L, U)
+        lib.L.PCG_default(A, L, U)
+        x = np.zeros(s.n)
+        b = s.b.copy()
+        it = lib.L.pcg(dptr(x), dptr(b), s.n, 1e-10, 1000)
+        return it, x
+
+    (i1, x1), (i2, x2) = run(libs[0]), run(libs[1])
+    say("pcg ilu iters", i1, i2, "rel", rel(x1, x2))
+    assert abs(i1 - i2) <= 1 and rel(x1, x2) < 1e-12
+
+
+def test_pcg_host_callback_preconditioner(libs):
+    """a caller-defined preconditioner (Jacobi) registered with PCG_set_precon"""
+    s = P.poisson(33)
+    d = s.scipy().diagonal().copy()
+    PREC = C.CFUNCTYPE(C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+    @PREC
+    def jacobi(n, r, z):
+        for i in range(n):
+            z[i] = r[i] / d[i]
+        return 1
+
+    def run(lib):
+        A = lib.from_csr(s.rowptr, s.col, s.val)
+        lib.L.PCG_default(A, None, None)
+        lib.PCG_set_precon(C.cast(jacobi, C.c_void_p))
+        x = np.zeros(s.n)
+        b = s.b.copy()
+        return lib.L.pcg(dptr(x), dptr(b), s.n, 1e-10, 2000), x
+
+    (i1, x1), (i2, x2) = run(libs[0]), run(libs[1])
+    say("pcg callback iters", i1, i2, "rel", rel(x1, x2))
+    assert abs(i1 - i2) <= 1 and rel(x1, x2) < 1e-11
+
+
+def test_bicgstab_amg_matches_reference(libs):
+    s = P.poisson(33)
+
+    def run(lib):
+        A = lib.from_csr(s.rowptr, s.col, s.val)
+        lib.L.RN_set_default_matrix(A)
+        lib.BI_set_mat_mult(C.cast(lib.RN_default_mat_mult, C.c_void_p))
+        lib.L.AMG_set_SOR_relaxation_coeff(1.0)
+        lib.L.BI_set_AMG_preconditioner(A, 1, 4)
+        lib.L.BI_print_info(0)
+        lib.L.BI_set_eps(1e-13)
+        lib.L.BI_set_max_iter(200)
+        x = np.zeros(s.n)
+        b = s.b.copy()
+        return lib.L.BiCGStab_solve(dptr(x), dptr(b), s.n), x
+
+    (i1, x1), (i2, x2) = run(libs[0]), run(libs[1])
+    say("bicgstab iters", i1, i2, "rel", rel(x1, x2))
+    assert abs(i1 - i2) <= 1 and rel(x1, x2) < 1e-10
