@@ -139,3 +139,41 @@ def test_bicgstab_amg_matches_reference(libs):
     (i1, x1), (i2, x2) = run(libs[0]), run(libs[1])
     say("bicgstab iters", i1, i2, "rel", rel(x1, x2))
     assert abs(i1 - i2) <= 1 and rel(x1, x2) < 1e-10
+
+
+def test_reflective_newton_matches_reference(libs):
+    """set_RN_Hesse / set_RN_linear / set_RN_constraints / RN_set_params /
+    RN_set_solver(RN_PCG) / reflective_newton through the LinA names"""
+    s, dold, ones = P.damage_system(33, h_scale=100.0)
+    n = s.n
+    idx = np.arange(n, dtype=np.int32)
+
+    def run(lib):
+        L = lib.L
+        H = lib.from_csr(s.rowptr, s.col, s.val)
+        b = s.b.copy()
+        lo, up = dold.copy(), ones.copy()
+        L.RN_print_info(0)
+        L.set_RN_Hesse.argtypes = [C.c_void_p]
+        L.set_RN_Hesse(C.cast(H, C.c_void_p))
+        L.set_RN_linear.argtypes = [C.POINTER(C.c_double)]
+        L.set_RN_linear(dptr(b))
+        L.set_RN_functional.argtypes = [C.c_void_p]
+        L.set_RN_functional(None)
+        L.RN_set_gradient.argtypes = [C.c_void_p]
+        L.RN_set_gradient(None)
+        L.set_RN_constraints.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+        L.set_RN_constraints(idx.ctypes.data_as(C.POINTER(C.c_int)), dptr(lo), dptr(up), n)
+        L.RN_set_params.argtypes = [C.c_int] + [C.c_double] * 10
+        L.RN_set_params(50, 1e-8, 0.1, 0.9, 0.1, float(np.sqrt(n) / 2), 1.0, 0.6, 1.1, 1e-3, 1000.0)
+        L.RN_set_local_solver.argtypes = [C.c_void_p]
+        L.RN_set_local_solver(None)
+        L.RN_set_solver(3)
+        x = dold.copy()
+        L.reflective_newton.argtypes = [C.POINTER(C.c_double)]
+        it = L.reflective_newton(dptr(x))
+        return it, x
+
+    (i1, x1), (i2, x2) = run(libs[0]), run(libs[1])
+    say("reflective newton iters", i1, i2, "rel", rel(x1, x2))
+    assert i1 == i2 and rel(x1, x2) < 1e-9
