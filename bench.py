@@ -346,12 +346,15 @@ def main():
         kernels_only(s, lina)
         return
     t0 = time.time()
-    prm64 = lina.params("vcycle", 2, args.depth, coarse_sweeps=args.coarse_sweeps)
+    prm64 = lina.params("vcycle", 2, args.depth, coarse_sweeps=args.coarse_sweeps, autotune=1)
     amg = lina.AMG(s.rowptr, s.col, s.val, prm64)
     t_setup = time.time() - t0
     sizes = [amg.level(l)["fine"] for l in range(amg.depth)] if amg.depth < 12 else []
-    log(f"AMG setup + upload {t_setup:.1f}s, level sizes {sizes}")
-    prm32 = lina.params("vcycle", 2, args.depth, precision="mixed", coarse_sweeps=args.coarse_sweeps)
+    tuned = amg.tuned()
+    lpr_top = [t for t in tuned if t[0] == amg.depth - 1 and t[1] == "A"]
+    lpr64 = lpr_top[0][2] if lpr_top else 1
+    log(f"AMG setup + upload {t_setup:.1f}s, level sizes {sizes}, autotuned row partitions {tuned}")
+    prm32 = lina.params("vcycle", 2, args.depth, precision="mixed", coarse_sweeps=args.coarse_sweeps, autotune=1)
 
     dx = lina.DeviceVector(s.n)
     db = lina.DeviceVector(s.n)
@@ -452,7 +455,7 @@ def main():
         "setup_s": t_setup,
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "k_spmv<double, double, 3, 1> (finest-level A*p of the outer CG)",
+                     "kernel": f"k_spmv<double, double, 3, {lpr64}> (finest-level A*p of the outer CG)",
                      "bytes_per_launch": spmv_bytes, "avg_ms": spmv_ms, "launches": res["mixed"]["spmv_launches"],
                      "back_to_back_ms": b2b_ms, "back_to_back_gbs": spmv_bytes / (b2b_ms * 1e-3) / 1e9,
                      "copy_gbs": cgbs},

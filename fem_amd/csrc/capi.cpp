@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -321,6 +322,21 @@ int lina_dev_fill(double* d, long n, double value) {
 
 double lina_spmv_bytes(const lina_solver* s, int level, int precision) {
     return s->s.spmv_bytes(level, precision);
+}
+
+int lina_amg_tuned(lina_solver* s, int* level, char* which, int* lpr64, int* lpr32, int cap) {
+    return guard([&]() {
+        if (!s) throw std::invalid_argument("null solver");
+        const auto& t = s->s.tuned();
+        int m = std::min(cap, (int)t.size());
+        for (int i = 0; i < m; i++) {
+            level[i] = t[i].level;
+            which[i] = t[i].which;
+            lpr64[i] = t[i].lpr64;
+            lpr32[i] = t[i].lpr32;
+        }
+        return (int)t.size();
+    });
 }
 
 int lina_amg_kernel_timing(lina_solver* s, int on) {

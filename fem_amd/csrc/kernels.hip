@@ -89,10 +89,13 @@ __device__ __forceinline__ T ld_stream(const T* p) {
 // BLOCK_NNZ).  Each lane first issues all of its column and value loads,
 // then all of its gathers, so that up to 3*BLOCK_NNZ/TPB loads per lane are
 // in flight at once.
+#ifndef LINA_NT_FP32
+#define LINA_NT_FP32 0
+#endif
 template <typename VT, typename XT>
 __device__ __forceinline__ void stage_products(int k0, int k1, const int* __restrict__ ci, const VT* __restrict__ v,
                                                const XT* __restrict__ x, XT* prod) {
-    constexpr bool NT = sizeof(VT) == 8;
+    constexpr bool NT = sizeof(VT) == 8 || LINA_NT_FP32;
 #if LINA_STAGE_UNROLL
     constexpr int U = BLOCK_NNZ / TPB;
     const int base = k0 + threadIdx.x;
@@ -135,6 +138,21 @@ __device__ __forceinline__ XT spmv_out(XT s, int r, const XT* b, XT* y) {
     return out;
 }
 
+// Logical row block of iteration k of workgroup b.  A launch with one
+// workgroup per block (G >= nblk) takes xcd_block's remap; a persistent grid
+// (G < nblk, G a multiple of 8) gives XCD x the contiguous block range
+// [x*chunk, (x+1)*chunk) and walks it with the XCD's G/8 workgroups.
+__device__ __forceinline__ int grid_block(int b, int G, int k, int nblk) {
+    if (G >= nblk) return k == 0 ? xcd_block(b, nblk) : -1;
+    constexpr int X = 8;
+    const int per = G / X, x = b % X, j = b / X;
+    const int chunk = (nblk + X - 1) / X;
+    const int local = j + k * per;
+    if (local >= chunk) return -1;
+    const int idx = x * chunk + local;
+    return idx < nblk ? idx : -1;
+}
+
 template <typename VT, typename XT, int MODE, int LPR>
 __global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int* __restrict__ blk, const int* __restrict__ rp,
                                               const int* __restrict__ ci, const VT* __restrict__ v,
@@ -142,38 +160,42 @@ __global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int* __restrict__ 
                                               XT* __restrict__ y, double* __restrict__ part,
                                               const XT* __restrict__ w) {
     __shared__ XT prod[BLOCK_NNZ];
-    const int bb = xcd_block(blockIdx.x, nblk);
-    const int r0 = blk[bb], r1 = blk[bb + 1];
-    const int k0 = rp[r0], k1 = rp[r1];
-    double mine = 0.0;
-    if (k1 - k0 > BLOCK_NNZ) {
-        // one long row: strided partial sums + block tree
-        XT s = 0;
-        for (int k = k0 + threadIdx.x; k < k1; k += TPB) s += (XT)v[k] * x[ci[k]];
-        double t = block_sum((double)s);
-        if (threadIdx.x == 0) {
-            XT out = spmv_out<XT, MODE>((XT)t, r0, b, y);
-            if (part) mine = (double)out * (double)(w ? w[r0] : out);
-        }
-    } else {
-        stage_products(k0, k1, ci, v, x, prod);
-        __syncthreads();
-        constexpr int G = TPB / LPR;
-        const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
-        for (int r = r0 + g; r < r1; r += G) {
+    for (int it = 0;; it++) {
+        const int bb = grid_block(blockIdx.x, gridDim.x, it, nblk);
+        if (bb < 0) break;
+        const int r0 = blk[bb], r1 = blk[bb + 1];
+        const int k0 = rp[r0], k1 = rp[r1];
+        double mine = 0.0;
+        if (k1 - k0 > BLOCK_NNZ) {
+            // one long row: strided partial sums + block tree
             XT s = 0;
-            const int e = rp[r + 1] - k0;
-            for (int k = rp[r] - k0 + lane; k < e; k += LPR) s += prod[k];
-            if (LPR > 1) s = group_sum<LPR>(s);
-            if (lane == 0) {
-                XT out = spmv_out<XT, MODE>(s, r, b, y);
-                if (part) mine += (double)out * (double)(w ? w[r] : out);
+            for (int k = k0 + threadIdx.x; k < k1; k += TPB) s += (XT)v[k] * x[ci[k]];
+            double t = block_sum((double)s);
+            if (threadIdx.x == 0) {
+                XT out = spmv_out<XT, MODE>((XT)t, r0, b, y);
+                if (part) mine = (double)out * (double)(w ? w[r0] : out);
+            }
+        } else {
+            stage_products(k0, k1, ci, v, x, prod);
+            __syncthreads();
+            constexpr int G = TPB / LPR;
+            const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+            for (int r = r0 + g; r < r1; r += G) {
+                XT s = 0;
+                const int e = rp[r + 1] - k0;
+                for (int k = rp[r] - k0 + lane; k < e; k += LPR) s += prod[k];
+                if (LPR > 1) s = group_sum<LPR>(s);
+                if (lane == 0) {
+                    XT out = spmv_out<XT, MODE>(s, r, b, y);
+                    if (part) mine += (double)out * (double)(w ? w[r] : out);
+                }
             }
         }
-    }
-    if (part) {
-        double t = block_sum(mine);
-        if (threadIdx.x == 0) part[bb] = t;
+        if (part) {
+            double t = block_sum(mine);
+            if (threadIdx.x == 0) part[bb] = t;
+        }
+        __syncthreads();  // prod is restaged by the next iteration
     }
 }
 
@@ -737,17 +759,38 @@ __global__ void k_bpcg_check(int ninst, const double* __restrict__ rr, const dou
 // x = inv * b over the dense rows (gathered through idx), trivial rows
 // x_i = b_i / diag_i.  One wave per dense row.
 template <typename XT>
+__global__ void k_coarse_gather(int nd, const int* __restrict__ didx, const XT* __restrict__ b,
+                                double* __restrict__ bd) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nd; c += gridDim.x * blockDim.x) bd[c] = (double)b[didx[c]];
+}
+
+// x = inv * bd over the dense rows (bd: the right-hand side gathered to the
+// dense numbering), trivial rows x_i = b_i / diag_i.  One wave per dense
+// row; the row is streamed with nontemporal 16-byte loads (nd even) into
+// two accumulators per lane, summed in a fixed order.
+template <typename XT>
 __global__ __launch_bounds__(TPB) void k_coarse(int nd, const int* __restrict__ didx, const double* __restrict__ inv,
                                                 int nt, const int* __restrict__ tidx, const double* __restrict__ tdiag,
-                                                const XT* __restrict__ b, XT* __restrict__ x) {
+                                                const XT* __restrict__ b, XT* __restrict__ x,
+                                                const double* __restrict__ bd) {
     const int wave = (blockIdx.x * TPB + threadIdx.x) / WAVE;
     const int lane = threadIdx.x % WAVE;
     const int nw = gridDim.x * TPB / WAVE;
     for (int row = wave; row < nd; row += nw) {
         const double* a = inv + (size_t)row * nd;
-        double s = 0.0;
-        for (int c = lane; c < nd; c += WAVE) s += ld_stream<true>(a + c) * (double)b[didx[c]];
-        s = wave_sum(s);
+        double s0 = 0.0, s1 = 0.0;
+        if ((nd & 1) == 0) {
+            const int h = nd / 2;
+#pragma unroll 4
+            for (int c = lane; c < h; c += WAVE) {
+                s0 += ld_stream<true>(a + 2 * c) * bd[2 * c];
+                s1 += ld_stream<true>(a + 2 * c + 1) * bd[2 * c + 1];
+            }
+        } else {
+#pragma unroll 4
+            for (int c = lane; c < nd; c += WAVE) s0 += ld_stream<true>(a + c) * bd[c];
+        }
+        double s = wave_sum(s0 + s1);
         if (lane == 0) x[didx[row]] = (XT)s;
     }
     for (int t = blockIdx.x * TPB + threadIdx.x; t < nt; t += gridDim.x * TPB) {
@@ -877,10 +920,20 @@ static inline int grid_for(int n, int cap = 4096) {
         default: { constexpr int L = 32; __VA_ARGS__; break; } \
     }
 
+#ifndef LINA_SPMV_GRID
+#define LINA_SPMV_GRID 0
+#endif
+// workgroups of an SpMV launch: one per row block, or a persistent grid of
+// LINA_SPMV_GRID workgroups (a multiple of 8) when set and smaller
+static inline int spmv_grid(int nblk) {
+    const int cap = (LINA_SPMV_GRID / 8) * 8;
+    return (cap > 0 && nblk > cap) ? cap : nblk;
+}
+
 template <typename VT, typename XT>
 int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w) {
     if (A.nblk == 0) return 0;
-    dim3 g(A.nblk), t(TPB);
+    dim3 g(spmv_grid(A.nblk)), t(TPB);
     if (mode == MODE_SET) {
         LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, MODE_SET, L>), g, t, 0, st, A.nblk, A.blk, A.rp,
                                                     A.ci, A.v, x, b, y, part, w))
@@ -1093,18 +1146,22 @@ template void jacobi<float, float>(hipStream_t, const DMat<float>&, int, const f
 
 template <typename XT>
 void coarse_solve(hipStream_t st, int nd, const int* didx, const double* inv, int nt, const int* tidx,
-                  const double* tdiag, const XT* b, XT* x) {
+                  const double* tdiag, const XT* b, XT* x, double* bd) {
+    if (nd > 0)
+        hipLaunchKernelGGL((k_coarse_gather<XT>), dim3(std::min((nd + TPB - 1) / TPB, 1024)), dim3(TPB), 0, st, nd,
+                           didx, b, bd);
     int rows_blocks = (nd * WAVE + TPB - 1) / TPB;
     int tb = (nt + TPB - 1) / TPB;
     int g = rows_blocks > tb ? rows_blocks : tb;
     if (g < 1) g = 1;
     if (g > 2048) g = 2048;
-    hipLaunchKernelGGL((k_coarse<XT>), dim3(g), dim3(TPB), 0, st, nd, didx, inv, nt, tidx, tdiag, b, x);
+    hipLaunchKernelGGL((k_coarse<XT>), dim3(g), dim3(TPB), 0, st, nd, didx, inv, nt, tidx, tdiag, b, x,
+                       (const double*)bd);
 }
 template void coarse_solve<double>(hipStream_t, int, const int*, const double*, int, const int*, const double*,
-                                   const double*, double*);
+                                   const double*, double*, double*);
 template void coarse_solve<float>(hipStream_t, int, const int*, const double*, int, const int*, const double*,
-                                  const float*, float*);
+                                  const float*, float*, double*);
 
 void trsv(hipStream_t st, const int* rp, const int* ci, const double* v, int nlv, const int* lv_off,
           const int* lv_rows, const double* z, double* w, int upper) {

@@ -140,9 +140,10 @@ int multidot(hipStream_t st, int n, int m, const double* V, size_t ld, const dou
 // y = base + sign * sum_l V_l c_l
 void combine(hipStream_t st, int n, int m, const double* V, size_t ld, const double* c, const double* base,
              double sign, double* y);
+// bd: nd doubles of scratch for the gathered right-hand side
 template <typename XT>
 void coarse_solve(hipStream_t st, int nd, const int* didx, const double* inv, int nt, const int* tidx,
-                  const double* tdiag, const XT* b, XT* x);
+                  const double* tdiag, const XT* b, XT* x, double* bd);
 
 }  // namespace dev
 }  // namespace lina

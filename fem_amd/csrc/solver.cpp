@@ -440,6 +440,7 @@ void Solver::upload() {
         double* piv = dalloc<double>(1, B);
         dev::gauss_jordan_inverse(st, nd_, inv_, piv);
     }
+    cbd_ = dalloc<double>(std::max(nd_, 1), B);
     tidx_ = dupload(lu.trivial_idx, B, st);
     tdiag_ = dupload(lu.trivial_diag, B, st);
     // Krylov vectors and scratch
@@ -660,7 +661,7 @@ void Solver::vcycle(int l, const T* f, T* x, bool x_zero) {
         if (cbatched_)
             dev::bcoarse_solve(st, nd_, crblk_, coff_, cmoff_, didx_, inv_, nt_, tidx_, tdiag_, f, x);
         else
-            dev::coarse_solve(st, nd_, didx_, inv_, nt_, tidx_, tdiag_, f, x);
+            dev::coarse_solve(st, nd_, didx_, inv_, nt_, tidx_, tdiag_, f, x, cbd_);
         return;
     }
     DevLevel& d = L_[l];

@@ -173,6 +173,7 @@ class Solver {
     int *didx_ = nullptr, *tidx_ = nullptr;
     double *inv_ = nullptr, *tdiag_ = nullptr;
     bool cbatched_ = false;
+    double* cbd_ = nullptr;
     int *crblk_ = nullptr, *coff_ = nullptr;
     size_t* cmoff_ = nullptr;
     double *ftop_ = nullptr, *rtop_ = nullptr;

@@ -106,6 +106,7 @@ def lib():
     L.lina_spmv_bytes.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.lina_spmv_bytes.restype = C.c_double
     L.lina_amg_kernel_timing.argtypes = [C.c_void_p, C.c_int]
+    L.lina_amg_tuned.argtypes = [C.c_void_p, _ip, C.c_char_p, _ip, _ip, C.c_int]
     L.lina_dev_alloc.argtypes = [C.c_long]
     L.lina_dev_alloc.restype = C.c_void_p
     L.lina_dev_free.argtypes = [C.c_void_p]
@@ -301,6 +302,16 @@ class AMG:
         if ms < 0:
             raise LinaError(lib().lina_last_error().decode())
         return ms
+
+    def tuned(self):
+        """[(level, kind, lanes FP64, lanes FP32)] chosen by the autotuner"""
+        cap = 256
+        lv = np.zeros(cap, np.int32)
+        wh = C.create_string_buffer(cap)
+        l64 = np.zeros(cap, np.int32)
+        l32 = np.zeros(cap, np.int32)
+        m = _check(lib().lina_amg_tuned(self.h, _i(lv), wh, _i(l64), _i(l32), cap))
+        return [(int(lv[i]), wh.raw[i:i + 1].decode(), int(l64[i]), int(l32[i])) for i in range(min(m, cap))]
 
     def kernel_timing(self, on: bool = True):
         _check(lib().lina_amg_kernel_timing(self.h, 1 if on else 0))

@@ -188,6 +188,10 @@ double lina_spmv_bytes(const lina_solver* s, int level, int precision);
 /* on != 0: pcg brackets each outer-loop A*p launch with HIP events and
  * reports the mean launch duration in lina_stats.spmv_ms */
 int lina_amg_kernel_timing(lina_solver* s, int on);
+/* the autotuner's choices (params.autotune): per tuned matrix its level, kind
+ * (A, P, R, C = colour-permuted A) and lanes per row for the FP64 / FP32
+ * copies; returns the number of entries (arrays of size cap) */
+int lina_amg_tuned(lina_solver* s, int* level, char* which, int* lpr64, int* lpr32, int cap);
 
 /* A device-resident CSR matrix outside any hierarchy: the library's own
  * y = A x of sparse_mult (linear_algebra.c:134-147) and PCG_mult
