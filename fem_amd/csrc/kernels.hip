@@ -126,15 +126,36 @@ __device__ __forceinline__ void stage_products(int k0, int k1, const int* __rest
 // MODE_SET : y = A x                        (sparse_mult / PCG_mult)
 // MODE_RES : y = b + (-(A x))               (linear_map(R,-1,A,X) on R=b)
 // MODE_ADD : y = y + (A x)                  (linear_map(X, 1,P,c))
+// MODE_ADDB: y = b + (A x)                  (the same sum written beside b)
+// MODE_SETX: y = A x, then the Epi epilogue on row r
 // part != nullptr: block partial of sum_r y_r * w_r (w == nullptr: y_r^2)
 
-template <typename XT, int MODE>
-__device__ __forceinline__ XT spmv_out(XT s, int r, const XT* b, XT* y) {
+template <typename VT, typename XT, int MODE>
+__device__ __forceinline__ XT spmv_out(XT s, int r, const XT* b, XT* y, const Epi<VT, XT>& e) {
     XT out;
-    if (MODE == MODE_SET || MODE == MODE_OUTER) out = s;
+    if (MODE == MODE_SET || MODE == MODE_OUTER || MODE == MODE_SETX) out = s;
     else if (MODE == MODE_RES) out = b[r] + (-s);
+    else if (MODE == MODE_ADDB) out = b[r] + s;
     else out = y[r] + s;
     y[r] = out;
+    if (MODE == MODE_SETX) {
+        if (e.x0) {
+            if (e.kind == JAC_REF) {
+                if (r >= e.nlim) {
+                    e.x0[r] = (XT)0;
+                } else {
+                    const int dp = e.dpos[r];
+                    e.x0[r] = out / (dp >= 0 ? (XT)e.dv[dp] : (XT)0);
+                }
+            } else {
+                e.x0[r] = out / (XT)e.l1[r];
+            }
+        }
+        if (e.bd) {
+            const int g = e.dinv[r];
+            if (g >= 0) e.bd[g] = (double)out;
+        }
+    }
     return out;
 }
 
@@ -158,7 +179,7 @@ __global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int* __restrict__ 
                                               const int* __restrict__ ci, const VT* __restrict__ v,
                                               const XT* __restrict__ x, const XT* __restrict__ b,
                                               XT* __restrict__ y, double* __restrict__ part,
-                                              const XT* __restrict__ w) {
+                                              const XT* __restrict__ w, const Epi<VT, XT> epi) {
     __shared__ XT prod[BLOCK_NNZ];
     for (int it = 0;; it++) {
         const int bb = grid_block(blockIdx.x, gridDim.x, it, nblk);
@@ -172,7 +193,7 @@ __global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int* __restrict__ 
             for (int k = k0 + threadIdx.x; k < k1; k += TPB) s += (XT)v[k] * x[ci[k]];
             double t = block_sum((double)s);
             if (threadIdx.x == 0) {
-                XT out = spmv_out<XT, MODE>((XT)t, r0, b, y);
+                XT out = spmv_out<VT, XT, MODE>((XT)t, r0, b, y, epi);
                 if (part) mine = (double)out * (double)(w ? w[r0] : out);
             }
         } else {
@@ -186,7 +207,7 @@ __global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int* __restrict__ 
                 for (int k = rp[r] - k0 + lane; k < e; k += LPR) s += prod[k];
                 if (LPR > 1) s = group_sum<LPR>(s);
                 if (lane == 0) {
-                    XT out = spmv_out<XT, MODE>(s, r, b, y);
+                    XT out = spmv_out<VT, XT, MODE>(s, r, b, y, epi);
                     if (part) mine += (double)out * (double)(w ? w[r] : out);
                 }
             }
@@ -219,6 +240,75 @@ __global__ __launch_bounds__(TPB) void k_finalize(int np, const double* __restri
     if (threadIdx.x == 0) {
         if (op == 2) scal[slot + 1] = scal[slot];
         scal[slot] = (op == 1) ? sqrt(t) : t;
+    }
+}
+
+// pcg's |R| and stop test (the loop condition of PCGS.c:224 evaluated on the
+// device): nothing happens once scal[S_STOP] is set
+__global__ __launch_bounds__(TPB) void k_finalize_stop(int np, const double* __restrict__ part,
+                                                       double* __restrict__ scal, double tol, int relres,
+                                                       int max_iter) {
+    if (scal[S_STOP] != 0.0) return;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += TPB) s += part[i];
+    double t = block_sum(s);
+    if (threadIdx.x == 0) {
+        const double r = sqrt(t);
+        scal[S_R] = r;
+        const double it = scal[S_IT] + 1.0;
+        scal[S_IT] = it;
+        const double crit = relres ? r : r / scal[S_R0];
+        if (!(crit > tol) || r != r || it >= (double)max_iter) scal[S_STOP] = 1.0;
+    }
+}
+
+// zero-guess preconditioner entry (see precon_in); the F.F partials follow
+// k_dot_part's loop and tree exactly
+template <typename XT>
+__global__ __launch_bounds__(TPB) void k_precon_in(int n, const double* __restrict__ F, double* __restrict__ part,
+                                                   double* __restrict__ X, XT* __restrict__ xf, XT* __restrict__ ff,
+                                                   const double* __restrict__ isa) {
+    double s = 0.0;
+    for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) {
+        const double f = F[i];
+        s += f * f;
+        if (X || xf) {
+            double x = 0.0;
+            if (isa && isa[i] != 0.0) x = f / isa[i];
+            if (X) X[i] = x;
+            if (xf) xf[i] = (XT)x;
+        }
+        if (ff) ff[i] = (XT)f;
+    }
+    double t = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// zero-guess preconditioner exit (see precon_out)
+template <typename XT>
+__global__ __launch_bounds__(TPB) void k_precon_out(int n, const XT* __restrict__ xs, double* __restrict__ Z,
+                                                    const double* __restrict__ F, const double* __restrict__ isa,
+                                                    const double* __restrict__ scal, int slot,
+                                                    const double* __restrict__ w, double* __restrict__ part) {
+    const bool restore = scal[slot] == 0.0;
+    double s = 0.0;
+    for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) {
+        double z;
+        if (restore) {
+            z = 0.0;
+            if (isa && isa[i] != 0.0) z = F[i] / isa[i];
+            Z[i] = z;
+        } else if (xs) {
+            z = (double)xs[i];
+            Z[i] = z;
+        } else {
+            z = Z[i];
+        }
+        if (w) s += w[i] * z;
+    }
+    if (w) {
+        double t = block_sum(s);
+        if (threadIdx.x == 0) part[blockIdx.x] = t;
     }
 }
 
@@ -296,6 +386,7 @@ __global__ void k_pcg_p(int n, double* __restrict__ P, const double* __restrict_
 __global__ __launch_bounds__(TPB) void k_pcg_xr(int n, double* __restrict__ X, double* __restrict__ R,
                                                 const double* __restrict__ P, const double* __restrict__ Q,
                                                 const double* __restrict__ scal, double* __restrict__ part) {
+    if (scal[S_STOP] != 0.0) return;
     double a = scal[S_S] / scal[S_PQ];
     double na = -a;
     double s = 0.0;
@@ -799,6 +890,50 @@ __global__ __launch_bounds__(TPB) void k_coarse(int nd, const int* __restrict__ 
     }
 }
 
+// x = inv * bd with an FP32 copy of the inverse (rows padded to ld, a
+// multiple of 4, with zeros; bd padded likewise): one wave per dense row,
+// nontemporal 16-byte loads of 4 entries, products and sums in FP64 (four
+// accumulators per lane, combined in a fixed order); trivial rows as k_coarse
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename XT>
+__global__ __launch_bounds__(TPB) void k_coarse_f32(int nd, int ld, const int* __restrict__ didx,
+                                                    const float* __restrict__ inv, int nt,
+                                                    const int* __restrict__ tidx, const double* __restrict__ tdiag,
+                                                    const XT* __restrict__ b, XT* __restrict__ x,
+                                                    const double* __restrict__ bd) {
+    const int wave = (blockIdx.x * TPB + threadIdx.x) / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    const int nw = gridDim.x * TPB / WAVE;
+    const int q = ld / 4;
+    for (int row = wave; row < nd; row += nw) {
+        const f32x4* a = reinterpret_cast<const f32x4*>(inv + (size_t)row * ld);
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll 4
+        for (int c = lane; c < q; c += WAVE) {
+            const f32x4 v = __builtin_nontemporal_load(a + c);
+            s0 += (double)v.x * bd[4 * c];
+            s1 += (double)v.y * bd[4 * c + 1];
+            s2 += (double)v.z * bd[4 * c + 2];
+            s3 += (double)v.w * bd[4 * c + 3];
+        }
+        double s = wave_sum((s0 + s1) + (s2 + s3));
+        if (lane == 0) x[didx[row]] = (XT)s;
+    }
+    for (int t = blockIdx.x * TPB + threadIdx.x; t < nt; t += gridDim.x * TPB) {
+        int i = tidx[t];
+        x[i] = (XT)((double)b[i] / tdiag[t]);
+    }
+}
+
+__global__ void k_inv_f32(int nd, int ld, const double* __restrict__ inv, float* __restrict__ out) {
+    const size_t total = (size_t)nd * ld;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(k / ld), c = (int)(k % ld);
+        out[k] = c < nd ? (float)inv[(size_t)r * nd + c] : 0.0f;
+    }
+}
+
 // Block-diagonal coarse solve of a batch: dense row g belongs to block
 // rblk[g]; block b holds rows [off[b], off[b+1]) of didx and the inverse at
 // inv + moff[b] (row-major, nd_b x nd_b).  One wave per dense row.
@@ -931,28 +1066,30 @@ static inline int spmv_grid(int nblk) {
 }
 
 template <typename VT, typename XT>
-int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w) {
+int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w,
+         const Epi<VT, XT>* epi) {
     if (A.nblk == 0) return 0;
     dim3 g(spmv_grid(A.nblk)), t(TPB);
-    if (mode == MODE_SET) {
-        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, MODE_SET, L>), g, t, 0, st, A.nblk, A.blk, A.rp,
-                                                    A.ci, A.v, x, b, y, part, w))
-    } else if (mode == MODE_OUTER) {
-        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, MODE_OUTER, L>), g, t, 0, st, A.nblk, A.blk,
-                                                    A.rp, A.ci, A.v, x, b, y, part, w))
-    } else if (mode == MODE_RES) {
-        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, MODE_RES, L>), g, t, 0, st, A.nblk, A.blk, A.rp,
-                                                    A.ci, A.v, x, b, y, part, w))
-    } else {
-        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, MODE_ADD, L>), g, t, 0, st, A.nblk, A.blk, A.rp,
-                                                    A.ci, A.v, x, b, y, part, w))
+    const Epi<VT, XT> e = epi ? *epi : Epi<VT, XT>{};
+#define LINA_SPMV_MODE(M)                                                                                       \
+    LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, M, L>), g, t, 0, st, A.nblk, A.blk, A.rp, A.ci, \
+                                                A.v, x, b, y, part, w, e))
+    switch (mode) {
+        case MODE_SET: LINA_SPMV_MODE(MODE_SET); break;
+        case MODE_OUTER: LINA_SPMV_MODE(MODE_OUTER); break;
+        case MODE_RES: LINA_SPMV_MODE(MODE_RES); break;
+        case MODE_ADD: LINA_SPMV_MODE(MODE_ADD); break;
+        case MODE_ADDB: LINA_SPMV_MODE(MODE_ADDB); break;
+        case MODE_SETX: LINA_SPMV_MODE(MODE_SETX); break;
+        default: return -1;
     }
+#undef LINA_SPMV_MODE
     return A.nblk;
 }
 template int spmv<double, double>(hipStream_t, const DMat<double>&, const double*, const double*, double*, int,
-                                  double*, const double*);
+                                  double*, const double*, const Epi<double, double>*);
 template int spmv<float, float>(hipStream_t, const DMat<float>&, const float*, const float*, float*, int, double*,
-                                const float*);
+                                const float*, const Epi<float, float>*);
 
 template <typename XT>
 int dot_part(hipStream_t st, int n, const XT* a, const XT* b, double* part) {
@@ -966,6 +1103,31 @@ template int dot_part<float>(hipStream_t, int, const float*, const float*, doubl
 void finalize(hipStream_t st, int np, const double* part, double* scal, int slot, int op) {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(TPB), 0, st, np, part, scal, slot, op);
 }
+
+void finalize_stop(hipStream_t st, int np, const double* part, double* scal, double tol, int relres, int max_iter) {
+    hipLaunchKernelGGL(k_finalize_stop, dim3(1), dim3(TPB), 0, st, np, part, scal, tol, relres, max_iter);
+}
+
+template <typename XT>
+int precon_in(hipStream_t st, int n, const double* F, double* part, double* X, XT* xf, XT* ff, const double* isa) {
+    int g = grid_for(n, 1024);
+    hipLaunchKernelGGL((k_precon_in<XT>), dim3(g), dim3(TPB), 0, st, n, F, part, X, xf, ff, isa);
+    return g;
+}
+template int precon_in<double>(hipStream_t, int, const double*, double*, double*, double*, double*, const double*);
+template int precon_in<float>(hipStream_t, int, const double*, double*, double*, float*, float*, const double*);
+
+template <typename XT>
+int precon_out(hipStream_t st, int n, const XT* xs, double* Z, const double* F, const double* isa,
+               const double* scal, int slot, const double* w, double* part) {
+    int g = grid_for(n, 1024);
+    hipLaunchKernelGGL((k_precon_out<XT>), dim3(g), dim3(TPB), 0, st, n, xs, Z, F, isa, scal, slot, w, part);
+    return g;
+}
+template int precon_out<double>(hipStream_t, int, const double*, double*, const double*, const double*,
+                                const double*, int, const double*, double*);
+template int precon_out<float>(hipStream_t, int, const float*, double*, const double*, const double*,
+                               const double*, int, const double*, double*);
 
 template <typename T>
 void fill(hipStream_t st, int n, T* x, T val) {
@@ -1146,8 +1308,8 @@ template void jacobi<float, float>(hipStream_t, const DMat<float>&, int, const f
 
 template <typename XT>
 void coarse_solve(hipStream_t st, int nd, const int* didx, const double* inv, int nt, const int* tidx,
-                  const double* tdiag, const XT* b, XT* x, double* bd) {
-    if (nd > 0)
+                  const double* tdiag, const XT* b, XT* x, double* bd, bool gathered) {
+    if (nd > 0 && !gathered)
         hipLaunchKernelGGL((k_coarse_gather<XT>), dim3(std::min((nd + TPB - 1) / TPB, 1024)), dim3(TPB), 0, st, nd,
                            didx, b, bd);
     int rows_blocks = (nd * WAVE + TPB - 1) / TPB;
@@ -1159,9 +1321,33 @@ void coarse_solve(hipStream_t st, int nd, const int* didx, const double* inv, in
                        (const double*)bd);
 }
 template void coarse_solve<double>(hipStream_t, int, const int*, const double*, int, const int*, const double*,
-                                   const double*, double*, double*);
+                                   const double*, double*, double*, bool);
 template void coarse_solve<float>(hipStream_t, int, const int*, const double*, int, const int*, const double*,
-                                  const float*, float*, double*);
+                                  const float*, float*, double*, bool);
+
+template <typename XT>
+void coarse_solve_f32(hipStream_t st, int nd, int ld, const int* didx, const float* inv, int nt, const int* tidx,
+                      const double* tdiag, const XT* b, XT* x, double* bd, bool gathered) {
+    if (nd > 0 && !gathered)
+        hipLaunchKernelGGL((k_coarse_gather<XT>), dim3(std::min((nd + TPB - 1) / TPB, 1024)), dim3(TPB), 0, st, nd,
+                           didx, b, bd);
+    int rows_blocks = (nd * WAVE + TPB - 1) / TPB;
+    int tb = (nt + TPB - 1) / TPB;
+    int g = rows_blocks > tb ? rows_blocks : tb;
+    if (g < 1) g = 1;
+    if (g > 2048) g = 2048;
+    hipLaunchKernelGGL((k_coarse_f32<XT>), dim3(g), dim3(TPB), 0, st, nd, ld, didx, inv, nt, tidx, tdiag, b, x,
+                       (const double*)bd);
+}
+template void coarse_solve_f32<double>(hipStream_t, int, int, const int*, const float*, int, const int*,
+                                       const double*, const double*, double*, double*, bool);
+template void coarse_solve_f32<float>(hipStream_t, int, int, const int*, const float*, int, const int*,
+                                      const double*, const float*, float*, double*, bool);
+
+void inverse_to_f32(hipStream_t st, int nd, int ld, const double* inv, float* out) {
+    if (nd <= 0) return;
+    hipLaunchKernelGGL(k_inv_f32, dim3(4096), dim3(TPB), 0, st, nd, ld, inv, out);
+}
 
 void trsv(hipStream_t st, const int* rp, const int* ci, const double* v, int nlv, const int* lv_off,
           const int* lv_rows, const double* z, double* w, int upper) {

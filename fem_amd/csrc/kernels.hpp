@@ -18,8 +18,10 @@ constexpr int BLOCK_NNZ = LINA_BLOCK_NNZ;
 constexpr int BLOCK_ROWS = LINA_BLOCK_ROWS;
 
 // MODE_OUTER is MODE_SET for the finest-level operator of the outer Krylov
-// loop; a separate instantiation so that profiles report it on its own
-enum { MODE_SET = 0, MODE_RES = 1, MODE_ADD = 2, MODE_OUTER = 3 };
+// loop; a separate instantiation so that profiles report it on its own.
+// MODE_ADDB: y = b + A x (a prolongation written beside its base vector);
+// MODE_SETX: y = A x followed by the epilogue of an Epi (below).
+enum { MODE_SET = 0, MODE_RES = 1, MODE_ADD = 2, MODE_OUTER = 3, MODE_ADDB = 4, MODE_SETX = 5 };
 enum { JAC_REF = 0, JAC_L1 = 1 };
 
 // device scalar slots used by the Krylov drivers
@@ -37,6 +39,8 @@ enum {
     S_T4 = 10,
     S_T5 = 11,
     S_AMG0 = 12,  // sum of squares of the preconditioner's right-hand side
+    S_STOP = 13,  // pcg's device-side stop flag (nonzero: converged, NaN or out of iterations)
+    S_IT = 14,    // pcg iterations completed on the device
     S_COUNT = 16
 };
 
@@ -54,6 +58,24 @@ struct DMat {
     VT* l1 = nullptr;    // sum_j |a_ij| of each row (l1-Jacobi), summed in column order
 };
 
+// Epilogue of a MODE_SETX product y = A x (a restriction), per output row r:
+//   x0 != nullptr: the output level's first Jacobi sweep from a zero guess,
+//     JAC_REF: x0_r = r < nlim ? y_r / a_rr : 0 (a_rr = dv[dpos[r]], 0 without
+//     a diagonal), JAC_L1: x0_r = y_r / l1_r -- k_jacobi0's arithmetic;
+//   bd != nullptr: bd[dinv[r]] = y_r for rows with dinv[r] >= 0 (the dense
+//     rows of the coarse direct solve, gathered to the dense numbering).
+template <typename VT, typename XT>
+struct Epi {
+    XT* x0 = nullptr;
+    const int* dpos = nullptr;
+    const VT* dv = nullptr;
+    const VT* l1 = nullptr;
+    int nlim = 0;
+    int kind = 0;
+    double* bd = nullptr;
+    const int* dinv = nullptr;
+};
+
 // lanes per row for a matrix with the given mean row length
 inline int lanes_per_row(double mean_len) {
     if (mean_len <= 24) return 1;
@@ -65,10 +87,29 @@ inline int lanes_per_row(double mean_len) {
 
 // returns the number of block partials written to part (if part != nullptr)
 template <typename VT, typename XT>
-int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w);
+int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w,
+         const Epi<VT, XT>* epi = nullptr);
 template <typename XT>
 int dot_part(hipStream_t st, int n, const XT* a, const XT* b, double* part);
+// scal[slot] = sum(part); op 0 plain, 1 sqrt, 2 shift (scal[slot+1] = scal[slot]) then plain
 void finalize(hipStream_t st, int np, const double* part, double* scal, int slot, int op);
+// pcg's residual norm and stop test: unless scal[S_STOP] is already set,
+// scal[S_R] = sqrt(sum(part)), scal[S_IT] += 1 and scal[S_STOP] = 1 when
+// !((relres ? r : r / scal[S_R0]) > tol), r is NaN or S_IT reached max_iter
+void finalize_stop(hipStream_t st, int np, const double* part, double* scal, double tol, int relres, int max_iter);
+// Zero-guess preconditioner entry: block partials of F.F (k_dot_part's grid
+// and order; returns their number); X (if given) = the initial guess, 0 and
+// F_i / isa_i where isa_i != 0 (isa: the isolated rows' diagonals, nullptr:
+// none); xf (if given) = (XT) X; ff (if given) = (XT) F.
+template <typename XT>
+int precon_in(hipStream_t st, int n, const double* F, double* part, double* X, XT* xf, XT* ff, const double* isa);
+// Zero-guess preconditioner exit: Z = the initial guess of precon_in when
+// scal[slot] == 0, else the V-cycle's result (xs, or Z itself when xs is
+// nullptr); w != nullptr: block partials of w.Z (k_dot_part's grid and order),
+// returns their number
+template <typename XT>
+int precon_out(hipStream_t st, int n, const XT* xs, double* Z, const double* F, const double* isa,
+               const double* scal, int slot, const double* w, double* part);
 template <typename T>
 void fill(hipStream_t st, int n, T* x, T val);
 template <typename TI, typename TO>
@@ -108,6 +149,7 @@ void scale(hipStream_t st, int n, double* x, double a);       // x_i *= a
 void restore_if_zero(hipStream_t st, int n, double* x, const double* scal, int slot, int m, const int* rows,
                      const double* a, const double* f);
 void pcg_p(hipStream_t st, int n, double* P, const double* Z, const double* scal, int first);
+// (skips every update once scal[S_STOP] is set)
 int pcg_xr(hipStream_t st, int n, double* X, double* R, const double* P, const double* Q, const double* scal,
            double* part);
 void axpy_s(hipStream_t st, int n, double* x, const double* y, const double* scal, int slot_a, int slot_b,
@@ -140,10 +182,18 @@ int multidot(hipStream_t st, int n, int m, const double* V, size_t ld, const dou
 // y = base + sign * sum_l V_l c_l
 void combine(hipStream_t st, int n, int m, const double* V, size_t ld, const double* c, const double* base,
              double sign, double* y);
-// bd: nd doubles of scratch for the gathered right-hand side
+// bd: nd doubles of scratch for the gathered right-hand side; gathered:
+// bd already holds it (a MODE_SETX restriction wrote it)
 template <typename XT>
 void coarse_solve(hipStream_t st, int nd, const int* didx, const double* inv, int nt, const int* tidx,
-                  const double* tdiag, const XT* b, XT* x, double* bd);
+                  const double* tdiag, const XT* b, XT* x, double* bd, bool gathered = false);
+// the same with an FP32 copy of the inverse, rows padded to ld (a multiple of
+// 4) with zeros; bd padded likewise; FP64 accumulation
+template <typename XT>
+void coarse_solve_f32(hipStream_t st, int nd, int ld, const int* didx, const float* inv, int nt, const int* tidx,
+                      const double* tdiag, const XT* b, XT* x, double* bd, bool gathered);
+// out (nd x ld, zero padded) = (float) inv (nd x nd)
+void inverse_to_f32(hipStream_t st, int nd, int ld, const double* inv, float* out);
 
 }  // namespace dev
 }  // namespace lina

@@ -22,6 +22,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/lina_hip.h"
@@ -206,9 +207,13 @@ Solver::~Solver() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
-    if (evk0_) (void)hipEventDestroy(evk0_);
-    if (evk1_) (void)hipEventDestroy(evk1_);
+    for (auto& pr : evk_)
+        for (hipEvent_t e : pr)
+            if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : evr_)
+        if (e) (void)hipEventDestroy(e);
     if (host_scal_) (void)hipHostFree(host_scal_);
+    if (host_ring_) (void)hipHostFree(host_ring_);
     bufs_.reset();
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -329,6 +334,8 @@ void Solver::upload() {
         hip_check(hipEventCreate(&ev0_), "hipEventCreate");
         hip_check(hipEventCreate(&ev1_), "hipEventCreate");
         hip_check(hipHostMalloc((void**)&host_scal_, dev::S_COUNT * sizeof(double)), "hipHostMalloc");
+        hip_check(hipHostMalloc((void**)&host_ring_, 2 * dev::S_COUNT * sizeof(double)), "hipHostMalloc");
+        for (hipEvent_t& e : evr_) hip_check(hipEventCreate(&e), "hipEventCreate");
     }
     // a re-upload (recompute) frees every earlier buffer, including the
     // BiCGStab work vectors allocated on first use
@@ -415,6 +422,18 @@ void Solver::upload() {
     iso_rows_ = dupload(H_.iso_rows, B, st);
     iso_a_ = dupload(H_.iso_a, B, st);
     niso_ = (int)H_.iso_rows.size();
+    // the isolated rows' diagonals by row for the fused preconditioner entry
+    // and exit (a zero diagonal keeps the separate kernels)
+    isa_ = nullptr;
+    isa_ok_ = true;
+    if (niso_ > 0) {
+        std::vector<double> isa(n_, 0.0);
+        for (int k = 0; k < niso_; k++) {
+            if (H_.iso_a[k] == 0.0) isa_ok_ = false;
+            isa[H_.iso_rows[k]] = H_.iso_a[k];
+        }
+        if (isa_ok_) isa_ = dupload(isa, B, st);
+    }
     // coarse LU inverse
     const CoarseLU& lu = H_.lu;
     nd_ = (int)lu.dense_idx.size();
@@ -440,7 +459,33 @@ void Solver::upload() {
         double* piv = dalloc<double>(1, B);
         dev::gauss_jordan_inverse(st, nd_, inv_, piv);
     }
-    cbd_ = dalloc<double>(std::max(nd_, 1), B);
+    // gathered right-hand side, padded with zeros for the FP32 inverse's rows
+    cbd_ = dalloc<double>(std::max((nd_ + 3) / 4 * 4, 4), B);
+    dev::fill<double>(st, std::max((nd_ + 3) / 4 * 4, 4), cbd_, 0.0);
+    // dense index of each coarsest row (restriction epilogue) and whether the
+    // dense and trivial rows cover the level (no zero fill needed)
+    dinv_ = nullptr;
+    ccover_ = false;
+    if (!cbatched_) {
+        const int n0 = L_[0].n;
+        std::vector<int> dinv(n0, -1), seen(n0, 0);
+        bool ok = true;
+        for (int g = 0; g < nd_ && ok; g++) {
+            const int r = lu.dense_idx[g];
+            if (r < 0 || r >= n0 || seen[r]++) ok = false;
+            else dinv[r] = g;
+        }
+        for (int t = 0; t < nt_ && ok; t++) {
+            const int r = lu.trivial_idx[t];
+            if (r < 0 || r >= n0 || seen[r]++) ok = false;
+        }
+        if (ok) {
+            dinv_ = dupload(dinv, B, st);
+            ccover_ = nd_ + nt_ == n0;
+        }
+    }
+    inv32_ = nullptr;
+    ld32_ = 0;
     tidx_ = dupload(lu.trivial_idx, B, st);
     tdiag_ = dupload(lu.trivial_diag, B, st);
     // Krylov vectors and scratch
@@ -461,7 +506,23 @@ void Solver::upload() {
     scal_ = dalloc<double>(dev::S_COUNT, B);
     dev::fill(st, dev::S_COUNT, scal_, 0.0);
     hip_check(hipStreamSynchronize(st), "upload sync");
+    ensure_inv32();
     if (prm_.autotune) autotune();
+}
+
+void Solver::set_params(const lina_amg_params& p) {
+    prm_ = p;
+    ensure_inv32();
+}
+
+// FP32 copy of the coarse inverse for mixed-precision V-cycles (made when the
+// hierarchy is uploaded or the precision changes, never inside a solve)
+void Solver::ensure_inv32() {
+    if (inv32_ || cbatched_ || nd_ == 0 || !uploaded() || prm_.precision != LINA_PREC_MIXED) return;
+    ld32_ = (nd_ + 3) / 4 * 4;
+    inv32_ = dalloc<float>((size_t)nd_ * ld32_, *bufs_);
+    dev::inverse_to_f32(stream_, nd_, ld32_, inv_, inv32_);
+    hip_check(hipStreamSynchronize(stream_), "sync");
 }
 
 void Solver::recompute(const std::vector<double>& val) {
@@ -497,10 +558,9 @@ void Solver::set_operator(DevMatrix* op) {
 }
 
 void Solver::set_kernel_timing(bool on) {
-    if (on && !evk0_) {
-        hip_check(hipEventCreate(&evk0_), "hipEventCreate");
-        hip_check(hipEventCreate(&evk1_), "hipEventCreate");
-    }
+    if (on && !evk_[0][0])
+        for (auto& pr : evk_)
+            for (hipEvent_t& e : pr) hip_check(hipEventCreate(&e), "hipEventCreate");
     ktime_ = on;
 }
 
@@ -590,17 +650,36 @@ void Solver::level_smoother(int l, int& smoother, int& sweeps) const {
     }
 }
 
-// x_zero: x is not initialised and stands for the zero vector (a coarse
-// level's guess in the V-cycle); the smoother then writes every entry
+// smoother kind and sweep count of level l's pre- and post-smoothing in the
+// V-cycle (AMG_V_cycle's Iter[level] settings)
+void Solver::vcycle_smoother(int l, int& smoother, int& sweeps) const {
+    const int depth = (int)L_.size();
+    sweeps = (prm_.coarsening == AGGRESSIVE && l == depth - 1) ? prm_.aggr_smoothing_iterations
+                                                                : prm_.smoothing_iterations;
+    level_smoother(l, smoother, sweeps);
+}
+
+static bool is_jacobi(int smoother) {
+    return smoother == LINA_SMOOTHER_JACOBI || smoother == LINA_SMOOTHER_L1_JACOBI;
+}
+
+// xinit (Solver::XInit): X_GUESS: x holds the iterate; X_ZERO: x stands for
+// the zero vector and is not initialised (a coarse level's guess in the
+// V-cycle), the smoother writes every entry; X_J0: x holds the first Jacobi
+// sweep from zero (written by the restriction's epilogue); X_SPARE: the
+// iterate is in the level's spare vector (a prolongation written beside x).
+// X_J0 and X_SPARE are for Jacobi smoothers only.
 template <typename T>
-void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post, bool x_zero) {
+void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post, int xinit) {
     DevLevel& d = L_[l];
     hipStream_t st = stream_;
     const T omega = (T)prm_.omega;
     int smoother;
     level_smoother(l, smoother, sweeps);
-    const bool jac = smoother == LINA_SMOOTHER_JACOBI || smoother == LINA_SMOOTHER_L1_JACOBI;
-    if (x_zero && !(jac && sweeps > 0)) dev::fill<T>(st, d.n, x, (T)0);
+    const bool jac = is_jacobi(smoother);
+    if ((xinit == X_J0 || xinit == X_SPARE) && !(jac && sweeps > 0))
+        throw std::logic_error("smooth: a fused first step without a Jacobi sweep");
+    if (xinit == X_ZERO && !(jac && sweeps > 0)) dev::fill<T>(st, d.n, x, (T)0);
     switch (smoother) {
         case LINA_SMOOTHER_SOR:
         case LINA_SMOOTHER_SSOR: {
@@ -626,16 +705,21 @@ void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post, bool x_zero)
         }
         case LINA_SMOOTHER_JACOBI:
         case LINA_SMOOTHER_L1_JACOBI: {
-            // ping-pong between x and the level's spare vector; an odd sweep
-            // count ends with one copy back.  From a zero guess the first
-            // sweep is elementwise.
+            // ping-pong between x and the level's spare vector, ending in x
+            // when the iterate starts where the caller put it for that
+            // (otherwise one copy back).  From a zero guess the first sweep
+            // is elementwise.
             int kind = smoother == LINA_SMOOTHER_JACOBI ? dev::JAC_REF : dev::JAC_L1;
             T* a = x;
             T* b = LevelVecs<T>::xo(d);
             int s = 0;
-            if (x_zero && sweeps > 0) {
+            if (xinit == X_ZERO && sweeps > 0) {
                 dev::jacobi0(st, pickA<T>(d), kind, f, x, d.nlim);
                 s = 1;
+            } else if (xinit == X_J0) {
+                s = 1;
+            } else if (xinit == X_SPARE) {
+                std::swap(a, b);
             }
             for (; s < sweeps; s++) {
                 dev::jacobi(st, pickA<T>(d), kind, f, (const T*)a, b, d.nlim);
@@ -649,35 +733,73 @@ void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post, bool x_zero)
     }
 }
 
-// AMG_V_cycle (Multigrid.c:1536-1580) with Iter[level] == 1.  x_zero: x is
-// the zero vector, left for the pre-smoother to write (the reference zeroes
-// the coarse guess before the recursive call).
+// the coarsest level's dense right-hand side is gathered by the restriction
+bool Solver::fused_gather() const { return !cbatched_ && nd_ > 0 && dinv_ != nullptr && L_.size() > 1; }
+
+// AMG_V_cycle (Multigrid.c:1536-1580) with Iter[level] == 1.  xinit as for
+// smooth: X_ZERO stands for the reference zeroing the coarse guess before the
+// recursive call.  Two steps are fused into neighbouring kernels where the
+// arithmetic allows: a coarse level's first Jacobi sweep from zero (and the
+// coarsest level's gather) into the restriction's epilogue, and a Jacobi
+// post-smoother with an odd sweep count reads the prolongation from the
+// spare vector, so its last sweep lands in x without a copy.
 template <typename T>
-void Solver::vcycle(int l, const T* f, T* x, bool x_zero) {
+void Solver::vcycle(int l, const T* f, T* x, int xinit) {
     hipStream_t st = stream_;
     if (l == 0) {
         // the dense-row/trivial-row split need not cover every row
-        if (x_zero) dev::fill<T>(st, L_[0].n, x, (T)0);
+        if (xinit == X_ZERO && !ccover_) dev::fill<T>(st, L_[0].n, x, (T)0);
         if (cbatched_)
             dev::bcoarse_solve(st, nd_, crblk_, coff_, cmoff_, didx_, inv_, nt_, tidx_, tdiag_, f, x);
+        else if (std::is_same<T, float>::value && inv32_)
+            dev::coarse_solve_f32(st, nd_, ld32_, didx_, inv32_, nt_, tidx_, tdiag_, f, x, cbd_, fused_gather());
         else
-            dev::coarse_solve(st, nd_, didx_, inv_, nt_, tidx_, tdiag_, f, x, cbd_);
+            dev::coarse_solve(st, nd_, didx_, inv_, nt_, tidx_, tdiag_, f, x, cbd_, fused_gather());
         return;
     }
     DevLevel& d = L_[l];
     DevLevel& c = L_[l - 1];
-    const int depth = (int)L_.size();
-    int sw = (prm_.coarsening == AGGRESSIVE && l == depth - 1) ? prm_.aggr_smoothing_iterations
-                                                                : prm_.smoothing_iterations;
-    smooth<T>(l, f, x, sw, false, x_zero);
+    int sm, sw;
+    vcycle_smoother(l, sm, sw);
+    smooth<T>(l, f, x, sw, false, xinit);
     T* r = LevelVecs<T>::r(d);
     dev::spmv<T, T>(st, pickA<T>(d), x, f, r, dev::MODE_RES, nullptr, nullptr);
     T* fc = LevelVecs<T>::f(c);
     T* xc = LevelVecs<T>::x(c);
-    dev::spmv<T, T>(st, pickR<T>(d), r, nullptr, fc, dev::MODE_SET, nullptr, nullptr);
-    vcycle<T>(l - 1, fc, xc, true);
-    dev::spmv<T, T>(st, pickP<T>(d), xc, nullptr, x, dev::MODE_ADD, nullptr, nullptr);
-    smooth<T>(l, f, x, sw, true, false);
+    dev::Epi<T, T> e;
+    bool epi = false;
+    int cinit = X_ZERO;
+    if (l - 1 == 0) {
+        if (fused_gather()) {
+            e.bd = cbd_;
+            e.dinv = dinv_;
+            epi = true;
+        }
+    } else {
+        int csm, csw;
+        vcycle_smoother(l - 1, csm, csw);
+        if (is_jacobi(csm) && csw > 0) {
+            const auto& Ac = pickA<T>(c);
+            e.x0 = xc;
+            e.dpos = Ac.dpos;
+            e.dv = Ac.v;
+            e.l1 = Ac.l1;
+            e.nlim = c.nlim;
+            e.kind = csm == LINA_SMOOTHER_JACOBI ? dev::JAC_REF : dev::JAC_L1;
+            epi = true;
+            cinit = X_J0;
+        }
+    }
+    dev::spmv<T, T>(st, pickR<T>(d), r, nullptr, fc, epi ? dev::MODE_SETX : dev::MODE_SET, nullptr, nullptr,
+                    epi ? &e : nullptr);
+    vcycle<T>(l - 1, fc, xc, cinit);
+    if (is_jacobi(sm) && sw % 2 == 1) {
+        dev::spmv<T, T>(st, pickP<T>(d), xc, x, LevelVecs<T>::xo(d), dev::MODE_ADDB, nullptr, nullptr);
+        smooth<T>(l, f, x, sw, true, X_SPARE);
+    } else {
+        dev::spmv<T, T>(st, pickP<T>(d), xc, nullptr, x, dev::MODE_ADD, nullptr, nullptr);
+        smooth<T>(l, f, x, sw, true, X_GUESS);
+    }
 }
 
 // |f - A x| on the finest level into scal[slot] (FP64 or the FP32 copy)
@@ -741,7 +863,9 @@ int Solver::amg_solve_dev(const double* F, double* X) {
 // otherwise), and it holds exactly when every F_i^2 underflows, in whatever
 // order the squares are summed.  The V-cycle runs unconditionally and its
 // result is replaced by the untouched guess on the device when r0 == 0.
-int Solver::vcycle_once(const double* F, double* X) {
+// w != nullptr: also the block partials of w.X into part_ (their number in
+// *np), the dot k_dot_part(w, X) would give.
+int Solver::vcycle_once(const double* F, double* X, const double* w, int* np) {
     hipStream_t st = stream_;
     DevLevel& top = L_.back();
     const int n = top.n;
@@ -751,36 +875,65 @@ int Solver::vcycle_once(const double* F, double* X) {
         dev::signs(st, n, signs_, F, ftop_);
         Ft = ftop_;
     }
-    int np = dev::dot_part<double>(st, n, Ft, Ft, part_);
-    dev::finalize(st, np, part_, scal_, dev::S_AMG0, 0);
     // without isolated rows the guess is exactly zero and the pre-smoother
     // writes it; otherwise AMG_fix_isolated sets those rows first
     const bool xz = niso_ == 0;
-    if (!xz) {
-        dev::fill<double>(st, n, X, 0.0);
-        dev::fix_isolated<double>(st, niso_, iso_rows_, iso_a_, Ft, X);
-    }
-    if (prm_.precision == LINA_PREC_FP64) {
-        vcycle<double>(lt, Ft, X, xz);
+    const bool fp64 = prm_.precision == LINA_PREC_FP64;
+    int npd = 0;
+    if (isa_ok_) {
+        // entry and exit fused into one elementwise kernel each
+        int n0;
+        if (fp64) n0 = dev::precon_in<double>(st, n, Ft, part_, xz ? nullptr : X, nullptr, nullptr, isa_);
+        else n0 = dev::precon_in<float>(st, n, Ft, part_, nullptr, xz ? nullptr : top.xf, top.ff, isa_);
+        dev::finalize(st, n0, part_, scal_, dev::S_AMG0, 0);
+        if (fp64) {
+            vcycle<double>(lt, Ft, X, xz ? X_ZERO : X_GUESS);
+            npd = dev::precon_out<double>(st, n, nullptr, X, Ft, isa_, scal_, dev::S_AMG0, w, part_);
+        } else {
+            vcycle<float>(lt, top.ff, top.xf, xz ? X_ZERO : X_GUESS);
+            npd = dev::precon_out<float>(st, n, top.xf, X, Ft, isa_, scal_, dev::S_AMG0, w, part_);
+        }
     } else {
-        dev::convert<double, float>(st, n, Ft, top.ff);
-        if (!xz) dev::convert<double, float>(st, n, X, top.xf);
-        vcycle<float>(lt, top.ff, top.xf, xz);
-        dev::convert<float, double>(st, n, top.xf, X);
+        int n0 = dev::dot_part<double>(st, n, Ft, Ft, part_);
+        dev::finalize(st, n0, part_, scal_, dev::S_AMG0, 0);
+        if (!xz) {
+            dev::fill<double>(st, n, X, 0.0);
+            dev::fix_isolated<double>(st, niso_, iso_rows_, iso_a_, Ft, X);
+        }
+        if (fp64) {
+            vcycle<double>(lt, Ft, X, xz ? X_ZERO : X_GUESS);
+        } else {
+            dev::convert<double, float>(st, n, Ft, top.ff);
+            if (!xz) dev::convert<double, float>(st, n, X, top.xf);
+            vcycle<float>(lt, top.ff, top.xf, xz ? X_ZERO : X_GUESS);
+            dev::convert<float, double>(st, n, top.xf, X);
+        }
+        dev::restore_if_zero(st, n, X, scal_, dev::S_AMG0, niso_, iso_rows_, iso_a_, Ft);
+        if (w) npd = dev::dot_part<double>(st, n, w, X, part_);
     }
-    dev::restore_if_zero(st, n, X, scal_, dev::S_AMG0, niso_, iso_rows_, iso_a_, Ft);
+    if (np) *np = npd;
     vcycles_ += 1;
     precon_calls_++;
     return 1;
 }
 
-int Solver::precon(const double* R, double* Z) {
-    if (prm_.zero_init && prm_.amg_max_iter == 1) return vcycle_once(R, Z);
+// the preconditioner runs without host round trips (pcg's iterations can then
+// be queued ahead of the stop test)
+bool Solver::precon_async() const { return prm_.zero_init && prm_.amg_max_iter == 1; }
+
+int Solver::precon(const double* R, double* Z, const double* w, int* np) {
+    if (precon_async()) return vcycle_once(R, Z, w, np);
     if (prm_.zero_init) dev::fill<double>(stream_, n_, Z, 0.0);
-    return amg_solve_dev(R, Z);
+    int it = amg_solve_dev(R, Z);
+    if (w) *np = dev::dot_part<double>(stream_, n_, w, Z, part_);
+    return it;
 }
 
-// pcg (PCGS.c:163-233) on device vectors X, B (length n)
+// pcg (PCGS.c:163-233) on device vectors X, B (length n).  The loop test is
+// evaluated on the device (k_finalize_stop); with a preconditioner that needs
+// no host round trip, iteration k+1 is queued before the host waits for
+// iteration k's test, and an iteration queued after the stop leaves X, R and
+// the scalars untouched.
 int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, lina_stats* stats) {
     hipStream_t st = stream_;
     const int n = n_;
@@ -788,6 +941,7 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
     vcycles_ = 0;
     precon_calls_ = 0;
     hip_check(hipEventRecord(ev0_, st), "event");
+    dev::fill<double>(st, 2, scal_ + dev::S_STOP, 0.0);  // S_STOP, S_IT
     // h = |B| (h==0 -> 1), B and X scaled by 1/h
     int np = dev::dot_part<double>(st, n, B, B, part_);
     dev::finalize(st, np, part_, scal_, dev::S_H, 1);
@@ -811,8 +965,7 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
     } else {
         dev::spmv<double, double>(st, top.A, X, B, R, dev::MODE_RES, nullptr, nullptr);
     }
-    precon(R, W);
-    np = dev::dot_part<double>(st, n, W, W, part_);
+    precon(R, W, W, &np);
     dev::finalize(st, np, part_, scal_, dev::S_R0, 1);
     double r0 = read_scalar(dev::S_R0);
     if (r0 == 0) {
@@ -823,41 +976,62 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
         }
         return 0;
     }
-    double r = r0;
-    int i = 0;
-    double ksum = 0.0;
-    int kcount = 0;
-    do {
-        precon(R, Z);
-        np = dev::dot_part<double>(st, n, R, Z, part_);
-        dev::finalize(st, np, part_, scal_, dev::S_S, 2);  // s_old = s; s = R.Z
-        dev::pcg_p(st, n, P, Z, scal_, i == 0);
+    const int relres = stop == LINA_STOP_RELRES;
+    const int cap = std::max(max_iter, 1);  // the loop body runs at least once
+    // one iteration: Z = M^-1 R with R.Z, P, Q = A P with P.Q, X and R with
+    // |R| and the stop test; its state (S_R..S_IT) copied to ring slot k % 2
+    auto enqueue = [&](int k) {
+        int m = 0;
+        precon(R, Z, R, &m);
+        dev::finalize(st, m, part_, scal_, dev::S_S, 2);  // s_old = s; s = R.Z
+        dev::pcg_p(st, n, P, Z, scal_, k == 0);
         if (op_) {
-            np = op_->launch_on(st, P, nullptr, Q, dev::MODE_OUTER, part_, P);
+            m = op_->launch_on(st, P, nullptr, Q, dev::MODE_OUTER, part_, P);
         } else if (has_neg_signs_) {
             apply_A(P, Q);
-            np = dev::dot_part<double>(st, n, P, Q, part_);
+            m = dev::dot_part<double>(st, n, P, Q, part_);
         } else {
-            if (ktime_) hip_check(hipEventRecord(evk0_, st), "event");
-            np = dev::spmv<double, double>(st, top.A, P, nullptr, Q, dev::MODE_OUTER, part_, P);  // Q = A P, P.Q
-            if (ktime_) hip_check(hipEventRecord(evk1_, st), "event");
+            if (ktime_) hip_check(hipEventRecord(evk_[k % 2][0], st), "event");
+            m = dev::spmv<double, double>(st, top.A, P, nullptr, Q, dev::MODE_OUTER, part_, P);  // Q = A P, P.Q
+            if (ktime_) hip_check(hipEventRecord(evk_[k % 2][1], st), "event");
         }
-        dev::finalize(st, np, part_, scal_, dev::S_PQ, 0);
-        np = dev::pcg_xr(st, n, X, R, P, Q, scal_, part_);
-        dev::finalize(st, np, part_, scal_, dev::S_R, 1);
-        r = read_scalar(dev::S_R);
-        if (ktime_ && !has_neg_signs_ && !op_) {
+        dev::finalize(st, m, part_, scal_, dev::S_PQ, 0);
+        m = dev::pcg_xr(st, n, X, R, P, Q, scal_, part_);
+        dev::finalize_stop(st, m, part_, scal_, tol, relres, cap);
+        double* h = host_ring_ + (k % 2) * dev::S_COUNT;
+        hip_check(hipMemcpyAsync(h + dev::S_R, scal_ + dev::S_R, (dev::S_IT - dev::S_R + 1) * sizeof(double),
+                                 hipMemcpyDeviceToHost, st),
+                  "d2h");
+        hip_check(hipEventRecord(evr_[k % 2], st), "event");
+    };
+    const bool timed = ktime_ && !has_neg_signs_ && !op_;
+    const int ahead = precon_async() ? 2 : 1;
+    double ksum = 0.0;
+    int kcount = 0;
+    int enq = 0, done = 0;
+    double r = 0.0;
+    int i = 0;
+    for (;;) {
+        while (enq < cap && enq - done < ahead) enqueue(enq++);
+        hip_check(hipEventSynchronize(evr_[done % 2]), "event sync");
+        const double* h = host_ring_ + (done % 2) * dev::S_COUNT;
+        const int it_dev = (int)h[dev::S_IT];
+        if (timed && it_dev == done + 1) {
             float kms = 0;
-            hip_check(hipEventElapsedTime(&kms, evk0_, evk1_), "elapsed");
+            hip_check(hipEventElapsedTime(&kms, evk_[done % 2][0], evk_[done % 2][1]), "elapsed");
             ksum += kms;
             kcount++;
         }
-        i++;
-        if (std::isnan(r)) break;
-    } while ((stop == LINA_STOP_RELRES ? r : r / r0) > tol && i < max_iter);
-    double relres = r;
-    precon(R, W);
-    np = dev::dot_part<double>(st, n, W, W, part_);
+        r = h[dev::S_R];
+        i = it_dev;
+        done++;
+        if (h[dev::S_STOP] != 0.0 || (done == enq && enq >= cap)) break;
+    }
+    // an iteration queued after the stop ran its V-cycle and nothing else
+    vcycles_ -= enq - i;
+    precon_calls_ -= enq - i;
+    double relres_out = r;
+    precon(R, W, W, &np);
     dev::finalize(st, np, part_, scal_, dev::S_T1, 1);
     double rw = read_scalar(dev::S_T1);
     dev::scale_by(st, n, B, scal_, dev::S_H, 0);
@@ -872,7 +1046,7 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
         stats->precon_calls = precon_calls_;
         stats->vcycles = vcycles_;
         stats->residual = pcg_residuum_;
-        stats->relres = relres;
+        stats->relres = relres_out;
         stats->t_solve_ms = ms;
         stats->spmv_ms = kcount ? ksum / kcount : 0.0;
         stats->spmv_launches = kcount;
@@ -1025,9 +1199,9 @@ double DevMatrix::time_spmv(int precision, int reps) {
 
 double DevMatrix::bytes(int precision) const { return lina::spmv_bytes(n_, nnz_, precision); }
 
-template void Solver::smooth<double>(int, const double*, double*, int, bool, bool);
-template void Solver::smooth<float>(int, const float*, float*, int, bool, bool);
-template void Solver::vcycle<double>(int, const double*, double*, bool);
-template void Solver::vcycle<float>(int, const float*, float*, bool);
+template void Solver::smooth<double>(int, const double*, double*, int, bool, int);
+template void Solver::smooth<float>(int, const float*, float*, int, bool, int);
+template void Solver::vcycle<double>(int, const double*, double*, int);
+template void Solver::vcycle<float>(int, const float*, float*, int);
 
 }  // namespace lina

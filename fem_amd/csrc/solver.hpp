@@ -103,7 +103,7 @@ class Solver {
     void recompute(const std::vector<double>& val);
     // AMG_recompute_coarse_matrices with a new finest matrix of any pattern
     void recompute_matrix(const Csr& M);
-    void set_params(const lina_amg_params& p) { prm_ = p; }
+    void set_params(const lina_amg_params& p);
     const lina_amg_params& params() const { return prm_; }
     const HostHierarchy& host() const { return H_; }
     const std::vector<int>& colours(int l) const { return colours_.at(l); }
@@ -114,19 +114,24 @@ class Solver {
     int pcg_host(double* x, double* b, double tol, int max_iter, int stop, lina_stats* st);
     int amg_solve_dev(const double* F, double* X);
     int amg_solve_host(const double* f, double* x);
-    int precon(const double* R, double* Z);
-    int vcycle_once(const double* F, double* X);
+    // Z = M^-1 R; w != nullptr: block partials of w.Z in part(), their number in *np
+    int precon(const double* R, double* Z, const double* w = nullptr, int* np = nullptr);
+    int vcycle_once(const double* F, double* X, const double* w = nullptr, int* np = nullptr);
+    bool precon_async() const;
     int spmv_host(int level, int precision, const double* x, double* y);
     double time_spmv(int level, int precision, int reps);
     double spmv_bytes(int level, int precision) const;
     double pcg_residuum() const { return pcg_residuum_; }
     hipStream_t stream() const { return stream_; }
 
+    // state of a level's iterate on entry to smooth / vcycle
+    enum XInit { X_GUESS = 0, X_ZERO = 1, X_J0 = 2, X_SPARE = 3 };
     void level_smoother(int l, int& smoother, int& sweeps) const;
+    void vcycle_smoother(int l, int& smoother, int& sweeps) const;
     template <typename T>
-    void smooth(int l, const T* f, T* x, int sweeps, bool post, bool x_zero = false);
+    void smooth(int l, const T* f, T* x, int sweeps, bool post, int xinit = X_GUESS);
     template <typename T>
-    void vcycle(int l, const T* f, T* x, bool x_zero = false);
+    void vcycle(int l, const T* f, T* x, int xinit = X_GUESS);
     template <typename T>
     double residual_norm(const T* f, const T* x, T* r, int slot);
     double read_scalar(int slot);
@@ -149,6 +154,8 @@ class Solver {
                              const double* scale, int blocked, const double* u, const double* lam, double* grad);
 
   private:
+    bool fused_gather() const;
+    void ensure_inv32();
     template <typename VT, typename XT>
     int tune_spmv(dev::DMat<VT>& M, const XT* x, XT* y);
     template <typename VT, typename XT>
@@ -174,6 +181,12 @@ class Solver {
     double *inv_ = nullptr, *tdiag_ = nullptr;
     bool cbatched_ = false;
     double* cbd_ = nullptr;
+    int* dinv_ = nullptr;     // dense index of each coarsest-level row (-1: not dense)
+    bool ccover_ = false;     // dense and trivial rows cover the coarsest level
+    float* inv32_ = nullptr;  // FP32 copy of the coarse inverse (mixed precision), rows padded to ld32_
+    int ld32_ = 0;
+    double* isa_ = nullptr;   // isolated rows' diagonals by row (0 elsewhere); nullptr without isolated rows
+    bool isa_ok_ = true;      // no isolated row has a zero diagonal (fused preconditioner entry/exit)
     int *crblk_ = nullptr, *coff_ = nullptr;
     size_t* cmoff_ = nullptr;
     double *ftop_ = nullptr, *rtop_ = nullptr;
@@ -186,7 +199,9 @@ class Solver {
     bool ktime_ = false;
     DevMatrix* op_ = nullptr;
     int part_cap_ = 0;
-    hipEvent_t evk0_ = nullptr, evk1_ = nullptr;
+    hipEvent_t evk_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // outer SpMV timing, per ring slot
+    hipEvent_t evr_[2] = {nullptr, nullptr};                            // pcg iteration done, per ring slot
+    double* host_ring_ = nullptr;                                       // pinned: 2 x S_COUNT scalars
     double *bS_ = nullptr, *bPS_ = nullptr, *bPP_ = nullptr, *bAPP_ = nullptr, *bAPS_ = nullptr, *bR_ = nullptr,
            *bR0_ = nullptr, *bP_ = nullptr;
 };
