@@ -3,9 +3,12 @@
 // Row kernels (SpMV, multicolour SOR, Jacobi) share one structure:
 //
 //   phase 1  all 256 lanes of a workgroup stream the value and column arrays
-//            of a block of consecutive rows (<= BLOCK_NNZ entries, coalesced,
-//            nontemporal: each matrix entry is read once per sweep) and
-//            store the products a_ij * x_j in LDS;
+//            of a block of consecutive rows (<= 2048 entries, coalesced, each
+//            matrix entry read once per sweep) and store the products
+//            a_ij * x_j in LDS.  A block's
+//            bounds come as one (first row, first entry) pair, and each lane
+//            group's row data is loaded together with the streamed entries,
+//            so a block waits for two memory round trips, not five;
 //   phase 2  rows are reduced from LDS by groups of LPR lanes.  LPR == 1
 //            adds a row's products one after another in column order, the
 //            order of the reference's `sum += A->Values[i][j]*x[...]` loops
@@ -13,7 +16,7 @@
 //            LPR > 1 (the dense Galerkin coarse levels, 50-200 entries a row)
 //            adds lane partials with a fixed xor-shuffle tree.
 //
-// A row longer than BLOCK_NNZ gets a block of its own and is reduced by the
+// A row longer than a block gets a block of its own and is reduced by the
 // whole workgroup.  The library is compiled with -ffp-contract=off, so no
 // multiply-add is fused.  Reductions (dots, norms) are deterministic: fixed
 // per-block trees, block partials stored by logical block index, a
@@ -72,9 +75,6 @@ __device__ __forceinline__ double block_sum(double v) {
 #ifndef LINA_NT_LOADS
 #define LINA_NT_LOADS 1
 #endif
-#ifndef LINA_STAGE_UNROLL
-#define LINA_STAGE_UNROLL 1
-#endif
 
 // Loads of streamed arrays.  NT: nontemporal (the FP64 matrices, whose
 // entries are read once per sweep and would otherwise displace the gathered
@@ -85,41 +85,41 @@ __device__ __forceinline__ T ld_stream(const T* p) {
     else return *p;
 }
 
-// phase 1: prod[k-k0] = v[k] * x[ci[k]] for the block's entries (k1-k0 <=
-// BLOCK_NNZ).  Each lane first issues all of its column and value loads,
-// then all of its gathers, so that up to 3*BLOCK_NNZ/TPB loads per lane are
-// in flight at once.
+// phase 1, split in two so that a kernel can issue its own per-row loads
+// between them: load() issues every column and value load of the lane's
+// entries k = k0 + lane + u*TPB (U = block_nnz/TPB of them), finish() issues
+// the gathers x[ci[k]] and stores the products v[k] * x[ci[k]] in LDS at
+// k - k0.  Up to 3U loads of a lane are in flight at once.
 #ifndef LINA_NT_FP32
 #define LINA_NT_FP32 0
 #endif
 template <typename VT, typename XT>
-__device__ __forceinline__ void stage_products(int k0, int k1, const int* __restrict__ ci, const VT* __restrict__ v,
-                                               const XT* __restrict__ x, XT* prod) {
-    constexpr bool NT = sizeof(VT) == 8 || LINA_NT_FP32;
-#if LINA_STAGE_UNROLL
-    constexpr int U = BLOCK_NNZ / TPB;
-    const int base = k0 + threadIdx.x;
+struct Stage {
+    static constexpr int U = block_nnz<VT>() / TPB;
+    static constexpr bool NT = sizeof(VT) == 8 || LINA_NT_FP32;
     int c[U];
     VT a[U];
-    XT xv[U];
+    // entries past k1 load entry k1-1 again (in bounds, same cache line) so
+    // that every load is issued unconditionally; k1 > k0
+    __device__ __forceinline__ void load(int k0, int k1, const int* __restrict__ ci, const VT* __restrict__ v) {
+        const int base = k0 + threadIdx.x;
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-        const int k = base + u * TPB;
-        if (k < k1) {
+        for (int u = 0; u < U; u++) {
+            const int k = min(base + u * TPB, k1 - 1);
             c[u] = ld_stream<NT>(ci + k);
             a[u] = ld_stream<NT>(v + k);
         }
     }
+    __device__ __forceinline__ void finish(int k0, int k1, const XT* __restrict__ x, XT* prod) {
+        const int base = k0 + threadIdx.x;
+        XT xv[U];
 #pragma unroll
-    for (int u = 0; u < U; u++)
-        if (base + u * TPB < k1) xv[u] = x[c[u]];
+        for (int u = 0; u < U; u++) xv[u] = x[c[u]];
 #pragma unroll
-    for (int u = 0; u < U; u++)
-        if (base + u * TPB < k1) prod[base + u * TPB - k0] = (XT)a[u] * xv[u];
-#else
-    for (int k = k0 + threadIdx.x; k < k1; k += TPB) prod[k - k0] = (XT)ld_stream<NT>(v + k) * x[ld_stream<NT>(ci + k)];
-#endif
-}
+        for (int u = 0; u < U; u++)
+            if (base + u * TPB < k1) prod[base + u * TPB - k0] = (XT)a[u] * xv[u];
+    }
+};
 
 // ------------------------------------------------------------------ SpMV
 //
@@ -130,13 +130,20 @@ __device__ __forceinline__ void stage_products(int k0, int k1, const int* __rest
 // MODE_SETX: y = A x, then the Epi epilogue on row r
 // part != nullptr: block partial of sum_r y_r * w_r (w == nullptr: y_r^2)
 
+// the vector entry a row's result is combined with (b_r or y_r), if any
+template <int MODE, typename XT>
+__device__ __forceinline__ XT spmv_base(int r, const XT* b, const XT* y) {
+    if (MODE == MODE_RES || MODE == MODE_ADDB) return b[r];
+    if (MODE == MODE_ADD) return y[r];
+    return (XT)0;
+}
+
 template <typename VT, typename XT, int MODE>
-__device__ __forceinline__ XT spmv_out(XT s, int r, const XT* b, XT* y, const Epi<VT, XT>& e) {
+__device__ __forceinline__ XT spmv_out(XT s, XT base, int r, XT* y, const Epi<VT, XT>& e) {
     XT out;
     if (MODE == MODE_SET || MODE == MODE_OUTER || MODE == MODE_SETX) out = s;
-    else if (MODE == MODE_RES) out = b[r] + (-s);
-    else if (MODE == MODE_ADDB) out = b[r] + s;
-    else out = y[r] + s;
+    else if (MODE == MODE_RES) out = base + (-s);
+    else out = base + s;
     y[r] = out;
     if (MODE == MODE_SETX) {
         if (e.x0) {
@@ -159,64 +166,61 @@ __device__ __forceinline__ XT spmv_out(XT s, int r, const XT* b, XT* y, const Ep
     return out;
 }
 
-// Logical row block of iteration k of workgroup b.  A launch with one
-// workgroup per block (G >= nblk) takes xcd_block's remap; a persistent grid
-// (G < nblk, G a multiple of 8) gives XCD x the contiguous block range
-// [x*chunk, (x+1)*chunk) and walks it with the XCD's G/8 workgroups.
-__device__ __forceinline__ int grid_block(int b, int G, int k, int nblk) {
-    if (G >= nblk) return k == 0 ? xcd_block(b, nblk) : -1;
-    constexpr int X = 8;
-    const int per = G / X, x = b % X, j = b / X;
-    const int chunk = (nblk + X - 1) / X;
-    const int local = j + k * per;
-    if (local >= chunk) return -1;
-    const int idx = x * chunk + local;
-    return idx < nblk ? idx : -1;
-}
-
+// One workgroup per row block (block bb of the XCD remap).  The first row of
+// each lane group's reduction (all rows of a block when it has at most
+// TPB/LPR rows) has its row bounds and base entry loaded together with the
+// streamed entries, off the critical path.
 template <typename VT, typename XT, int MODE, int LPR>
-__global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int* __restrict__ blk, const int* __restrict__ rp,
+__global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int2* __restrict__ bk, const int* __restrict__ rp,
                                               const int* __restrict__ ci, const VT* __restrict__ v,
                                               const XT* __restrict__ x, const XT* __restrict__ b,
                                               XT* __restrict__ y, double* __restrict__ part,
                                               const XT* __restrict__ w, const Epi<VT, XT> epi) {
-    __shared__ XT prod[BLOCK_NNZ];
-    for (int it = 0;; it++) {
-        const int bb = grid_block(blockIdx.x, gridDim.x, it, nblk);
-        if (bb < 0) break;
-        const int r0 = blk[bb], r1 = blk[bb + 1];
-        const int k0 = rp[r0], k1 = rp[r1];
-        double mine = 0.0;
-        if (k1 - k0 > BLOCK_NNZ) {
-            // one long row: strided partial sums + block tree
+    constexpr int BN = block_nnz<VT>();
+    __shared__ XT prod[BN];
+    const int bb = xcd_block(blockIdx.x, nblk);
+    const int2 s0 = bk[bb], s1 = bk[bb + 1];
+    const int r0 = s0.x, r1 = s1.x, k0 = s0.y, k1 = s1.y;
+    double mine = 0.0;
+    if (k1 - k0 > BN) {
+        // one long row: strided partial sums + block tree
+        XT s = 0;
+        for (int k = k0 + threadIdx.x; k < k1; k += TPB) s += (XT)v[k] * x[ci[k]];
+        double t = block_sum((double)s);
+        if (threadIdx.x == 0) {
+            XT out = spmv_out<VT, XT, MODE>((XT)t, spmv_base<MODE>(r0, b, y), r0, y, epi);
+            if (part) mine = (double)out * (double)(w ? w[r0] : out);
+        }
+    } else {
+        constexpr int G = TPB / LPR;
+        const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+        const int rf = r0 + g;
+        Stage<VT, XT> st;
+        if (k1 > k0) st.load(k0, k1, ci, v);
+        int pb = 0, pe = 0;
+        XT base0 = (XT)0;
+        if (rf < r1) {
+            pb = rp[rf];
+            pe = rp[rf + 1];
+            base0 = spmv_base<MODE>(rf, b, y);
+        }
+        if (k1 > k0) st.finish(k0, k1, x, prod);
+        __syncthreads();
+        for (int r = rf; r < r1; r += G) {
+            const bool first = r == rf;
             XT s = 0;
-            for (int k = k0 + threadIdx.x; k < k1; k += TPB) s += (XT)v[k] * x[ci[k]];
-            double t = block_sum((double)s);
-            if (threadIdx.x == 0) {
-                XT out = spmv_out<VT, XT, MODE>((XT)t, r0, b, y, epi);
-                if (part) mine = (double)out * (double)(w ? w[r0] : out);
-            }
-        } else {
-            stage_products(k0, k1, ci, v, x, prod);
-            __syncthreads();
-            constexpr int G = TPB / LPR;
-            const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
-            for (int r = r0 + g; r < r1; r += G) {
-                XT s = 0;
-                const int e = rp[r + 1] - k0;
-                for (int k = rp[r] - k0 + lane; k < e; k += LPR) s += prod[k];
-                if (LPR > 1) s = group_sum<LPR>(s);
-                if (lane == 0) {
-                    XT out = spmv_out<VT, XT, MODE>(s, r, b, y, epi);
-                    if (part) mine += (double)out * (double)(w ? w[r] : out);
-                }
+            const int e = (first ? pe : rp[r + 1]) - k0;
+            for (int k = (first ? pb : rp[r]) - k0 + lane; k < e; k += LPR) s += prod[k];
+            if (LPR > 1) s = group_sum<LPR>(s);
+            if (lane == 0) {
+                XT out = spmv_out<VT, XT, MODE>(s, first ? base0 : spmv_base<MODE>(r, b, y), r, y, epi);
+                if (part) mine += (double)out * (double)(w ? w[r] : out);
             }
         }
-        if (part) {
-            double t = block_sum(mine);
-            if (threadIdx.x == 0) part[bb] = t;
-        }
-        __syncthreads();  // prod is restaged by the next iteration
+    }
+    if (part) {
+        double t = block_sum(mine);
+        if (threadIdx.x == 0) part[bb] = t;
     }
 }
 
@@ -422,16 +426,17 @@ __global__ void k_axpby(int n, double a, double* __restrict__ x, double b, const
 // AMG_SOR_smoother (Multigrid.c:1345-1365).  Rows of one colour are not
 // coupled, so the products read x before any row of the colour is updated.
 template <typename VT, typename XT, int LPR>
-__global__ __launch_bounds__(TPB) void k_mcsor(int nblk, int blk0, const int* __restrict__ blk,
+__global__ __launch_bounds__(TPB) void k_mcsor(int nblk, int blk0, const int2* __restrict__ bk,
                                                const int* __restrict__ rp, const int* __restrict__ ci,
                                                const VT* __restrict__ v, const int* __restrict__ dpos,
                                                const int* __restrict__ perm, const XT* __restrict__ f,
                                                XT* __restrict__ x, XT omega) {
-    __shared__ XT prod[BLOCK_NNZ];
+    constexpr int BN = block_nnz<VT>();
+    __shared__ XT prod[BN];
     const int bb = blk0 + xcd_block(blockIdx.x, nblk);
-    const int r0 = blk[bb], r1 = blk[bb + 1];
-    const int k0 = rp[r0], k1 = rp[r1];
-    if (k1 - k0 > BLOCK_NNZ) {
+    const int2 s0 = bk[bb], s1 = bk[bb + 1];
+    const int r0 = s0.x, r1 = s1.x, k0 = s0.y, k1 = s1.y;
+    if (k1 - k0 > BN) {
         // one long row: the whole workgroup reduces it (block tree)
         const int i = perm[r0];
         const int dp = dpos[r0];
@@ -445,28 +450,53 @@ __global__ __launch_bounds__(TPB) void k_mcsor(int nblk, int blk0, const int* __
         }
         return;
     }
-    stage_products(k0, k1, ci, v, x, prod);
-    __syncthreads();
     constexpr int G = TPB / LPR;
     const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
-    for (int r = r0 + g; r < r1; r += G) {
-        const int i = perm[r];
-        const int e = rp[r + 1];
-        const int dp = dpos[r];
+    const bool leader = LPR == 1 || lane == 0;
+    // the group's first row: permutation, bounds and diagonal position first,
+    // then the streamed entries, then f_i, x_i and a_ii (x_i is written only
+    // by this lane, after every gather of the block)
+    const int rf = r0 + g;
+    int pi = 0, pb = 0, pe = 0, pdp = -1;
+    if (rf < r1) {
+        pi = perm[rf];
+        pb = rp[rf];
+        pe = rp[rf + 1];
+        pdp = dpos[rf];
+    }
+    Stage<VT, XT> st;
+    if (k1 > k0) st.load(k0, k1, ci, v);
+    XT pf = 0, px = 0, pd = 0;
+    if (rf < r1 && leader) {
+        pf = f[pi];
+        px = x[pi];
+        pd = pdp >= 0 ? (XT)v[pdp] : (XT)0;
+    }
+    if (k1 > k0) st.finish(k0, k1, x, prod);
+    __syncthreads();
+    for (int r = rf; r < r1; r += G) {
+        const bool first = r == rf;
+        const int i = first ? pi : perm[r];
+        const int kb = first ? pb : rp[r];
+        const int e = first ? pe : rp[r + 1];
+        const int dp = first ? pdp : dpos[r];
         if (LPR == 1) {
-            XT sum = f[i];
-            for (int k = rp[r]; k < e; k++)
+            XT sum = first ? pf : f[i];
+            for (int k = kb; k < e; k++)
                 if (k != dp) sum -= prod[k - k0];
-            const XT d = dp >= 0 ? (XT)v[dp] : (XT)0;
-            x[i] = ((XT)1 - omega) * x[i] + omega * sum / d;
+            const XT d = first ? pd : (dp >= 0 ? (XT)v[dp] : (XT)0);
+            const XT xi = first ? px : x[i];
+            x[i] = ((XT)1 - omega) * xi + omega * sum / d;
         } else {
             XT s = 0;
-            for (int k = rp[r] + lane; k < e; k += LPR)
+            for (int k = kb + lane; k < e; k += LPR)
                 if (k != dp) s += prod[k - k0];
             s = group_sum<LPR>(s);
             if (lane == 0) {
-                const XT d = dp >= 0 ? (XT)v[dp] : (XT)0;
-                x[i] = ((XT)1 - omega) * x[i] + omega * (f[i] - s) / d;
+                const XT d = first ? pd : (dp >= 0 ? (XT)v[dp] : (XT)0);
+                const XT fi = first ? pf : f[i];
+                const XT xi = first ? px : x[i];
+                x[i] = ((XT)1 - omega) * xi + omega * (fi - s) / d;
             }
         }
     }
@@ -509,16 +539,17 @@ __global__ __launch_bounds__(1024) void k_sor_lex(int n, const int* __restrict__
 //             beyond (AMG_parallel_Jacobi_smoother, Multigrid.c:1367-1400)
 //   JAC_L1  : x_i = xo_i + (f_i - sum_j a_ij xo_j)/sum_j |a_ij|
 template <typename VT, typename XT, int KIND, int LPR>
-__global__ __launch_bounds__(TPB) void k_jacobi(int nblk, const int* __restrict__ blk, const int* __restrict__ rp,
+__global__ __launch_bounds__(TPB) void k_jacobi(int nblk, const int2* __restrict__ bk, const int* __restrict__ rp,
                                                 const int* __restrict__ ci, const VT* __restrict__ v,
                                                 const int* __restrict__ dpos, const VT* __restrict__ l1,
                                                 const XT* __restrict__ f, const XT* __restrict__ xo,
                                                 XT* __restrict__ x, int nlim) {
-    __shared__ XT prod[BLOCK_NNZ];
+    constexpr int BN = block_nnz<VT>();
+    __shared__ XT prod[BN];
     const int bb = xcd_block(blockIdx.x, nblk);
-    const int r0 = blk[bb], r1 = blk[bb + 1];
-    const int k0 = rp[r0], k1 = rp[r1];
-    if (k1 - k0 > BLOCK_NNZ) {
+    const int2 s0 = bk[bb], s1 = bk[bb + 1];
+    const int r0 = s0.x, r1 = s1.x, k0 = s0.y, k1 = s1.y;
+    if (k1 - k0 > BN) {
         // one long row: the whole workgroup reduces it (block tree)
         const int r = r0;
         if (KIND == JAC_REF && r >= nlim) {
@@ -537,31 +568,52 @@ __global__ __launch_bounds__(TPB) void k_jacobi(int nblk, const int* __restrict_
         }
         return;
     }
-    stage_products(k0, k1, ci, v, xo, prod);
-    __syncthreads();
     constexpr int G = TPB / LPR;
     const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
-    for (int r = r0 + g; r < r1; r += G) {
-        const int e = rp[r + 1];
+    const bool leader = LPR == 1 || lane == 0;
+    // the group's first row: bounds and (l1-Jacobi) f_r, xo_r, l1_r loaded
+    // with the streamed entries
+    const int rf = r0 + g;
+    int pb = 0, pe = 0;
+    XT pf = 0, pxo = 0;
+    VT pl1 = 0;
+    Stage<VT, XT> st;
+    if (k1 > k0) st.load(k0, k1, ci, v);
+    if (rf < r1) {
+        pb = rp[rf];
+        pe = rp[rf + 1];
+        if (KIND == JAC_L1 && leader) {
+            pf = f[rf];
+            pxo = xo[rf];
+            pl1 = l1[rf];
+        }
+    }
+    if (k1 > k0) st.finish(k0, k1, xo, prod);
+    __syncthreads();
+    for (int r = rf; r < r1; r += G) {
+        const bool first = r == rf;
+        const int kb = first ? pb : rp[r];
+        const int e = first ? pe : rp[r + 1];
         if (KIND == JAC_REF && r >= nlim) {
             if (lane == 0) x[r] = xo[r];  // rows past (n/dof)*dof are not swept
             continue;
         }
         const int dp = KIND == JAC_REF ? dpos[r] : -1;
+        const XT fr = (KIND == JAC_L1 && first) ? pf : f[r];
         XT s;
         if (LPR == 1) {
-            s = f[r];
-            for (int k = rp[r]; k < e; k++)
+            s = fr;
+            for (int k = kb; k < e; k++)
                 if (k != dp) s -= prod[k - k0];
         } else {
             s = 0;
-            for (int k = rp[r] + lane; k < e; k += LPR)
+            for (int k = kb + lane; k < e; k += LPR)
                 if (k != dp) s += prod[k - k0];
-            s = f[r] - group_sum<LPR>(s);
+            s = fr - group_sum<LPR>(s);
         }
         if (lane == 0) {
             if (KIND == JAC_REF) x[r] = s / (dp >= 0 ? (XT)v[dp] : (XT)0);
-            else x[r] = xo[r] + s / l1[r];
+            else x[r] = (first ? pxo : xo[r]) + s / (first ? pl1 : l1[r]);
         }
     }
 }
@@ -1055,24 +1107,14 @@ static inline int grid_for(int n, int cap = 4096) {
         default: { constexpr int L = 32; __VA_ARGS__; break; } \
     }
 
-#ifndef LINA_SPMV_GRID
-#define LINA_SPMV_GRID 0
-#endif
-// workgroups of an SpMV launch: one per row block, or a persistent grid of
-// LINA_SPMV_GRID workgroups (a multiple of 8) when set and smaller
-static inline int spmv_grid(int nblk) {
-    const int cap = (LINA_SPMV_GRID / 8) * 8;
-    return (cap > 0 && nblk > cap) ? cap : nblk;
-}
-
 template <typename VT, typename XT>
 int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w,
          const Epi<VT, XT>* epi) {
     if (A.nblk == 0) return 0;
-    dim3 g(spmv_grid(A.nblk)), t(TPB);
+    dim3 g(A.nblk), t(TPB);
     const Epi<VT, XT> e = epi ? *epi : Epi<VT, XT>{};
 #define LINA_SPMV_MODE(M)                                                                                       \
-    LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, M, L>), g, t, 0, st, A.nblk, A.blk, A.rp, A.ci, \
+    LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, M, L>), g, t, 0, st, A.nblk, A.bk, A.rp, A.ci, \
                                                 A.v, x, b, y, part, w, e))
     switch (mode) {
         case MODE_SET: LINA_SPMV_MODE(MODE_SET); break;
@@ -1257,7 +1299,7 @@ void mcsor_colour(hipStream_t st, const DMat<VT>& Ac, int blk0, int nblk, const 
                   XT omega) {
     if (nblk <= 0) return;
     LINA_LPR_DISPATCH(Ac.lpr, hipLaunchKernelGGL((k_mcsor<VT, XT, L>), dim3(nblk), dim3(TPB), 0, st, nblk, blk0,
-                                                 Ac.blk, Ac.rp, Ac.ci, Ac.v, Ac.dpos, perm, f, x, omega))
+                                                 Ac.bk, Ac.rp, Ac.ci, Ac.v, Ac.dpos, perm, f, x, omega))
 }
 template void mcsor_colour<double, double>(hipStream_t, const DMat<double>&, int, int, const int*, const double*,
                                            double*, double);
@@ -1281,10 +1323,10 @@ void jacobi(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, const XT* 
     if (A.nblk == 0) return;
     dim3 g(A.nblk), t(TPB);
     if (kind == JAC_REF) {
-        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_REF, L>), g, t, 0, st, A.nblk, A.blk, A.rp,
+        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_REF, L>), g, t, 0, st, A.nblk, A.bk, A.rp,
                                                     A.ci, A.v, A.dpos, A.l1, f, xo, x, nlim))
     } else {
-        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_L1, L>), g, t, 0, st, A.nblk, A.blk, A.rp,
+        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_L1, L>), g, t, 0, st, A.nblk, A.bk, A.rp,
                                                     A.ci, A.v, A.dpos, A.l1, f, xo, x, nlim))
     }
 }

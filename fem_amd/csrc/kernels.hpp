@@ -6,16 +6,25 @@
 namespace lina {
 namespace dev {
 
-// products of one row block staged in LDS: at most BLOCK_NNZ entries and
-// at most BLOCK_ROWS rows per block (one lane per row in the second phase)
+// products of one row block staged in LDS: at most block_nnz<VT>() entries
+// (2048 for both value types; 4096-entry FP32 blocks measured slower) and at
+// most BLOCK_ROWS rows per block (one lane per row in the second phase)
 #ifndef LINA_BLOCK_NNZ
 #define LINA_BLOCK_NNZ 2048
+#endif
+#ifndef LINA_BLOCK_NNZ32
+#define LINA_BLOCK_NNZ32 2048
 #endif
 #ifndef LINA_BLOCK_ROWS
 #define LINA_BLOCK_ROWS 256
 #endif
 constexpr int BLOCK_NNZ = LINA_BLOCK_NNZ;
+constexpr int BLOCK_NNZ32 = LINA_BLOCK_NNZ32;
 constexpr int BLOCK_ROWS = LINA_BLOCK_ROWS;
+template <typename VT>
+constexpr int block_nnz() {
+    return sizeof(VT) == 8 ? BLOCK_NNZ : BLOCK_NNZ32;
+}
 
 // MODE_OUTER is MODE_SET for the finest-level operator of the outer Krylov
 // loop; a separate instantiation so that profiles report it on its own.
@@ -52,7 +61,7 @@ struct DMat {
     int* ci = nullptr;
     VT* v = nullptr;
     int nblk = 0;
-    int* blk = nullptr;  // nblk+1 row starts
+    int2* bk = nullptr;  // nblk+1 (first row, first entry) pairs of the row blocks
     int lpr = 1;         // lanes reducing one row from LDS: 1 (sequential), 4, 8, 16, 32
     int* dpos = nullptr; // position of each row's diagonal entry in ci/v (-1: none)
     VT* l1 = nullptr;    // sum_j |a_ij| of each row (l1-Jacobi), summed in column order

@@ -75,21 +75,11 @@ static dev::DMat<double> upload_csr(const Csr& A, DevBuffers& B, hipStream_t st)
     D.v = kupload(A.v, B, st);
     // LDS row blocks for the SpMV (same builder rule as the AMG levels)
     std::vector<int> starts;
-    int nnz = 0, rows = 0;
-    if (A.n > 0) starts.push_back(0);
-    for (int r = 0; r < A.n; r++) {
-        int len = A.rp[r + 1] - A.rp[r];
-        if (rows > 0 && (nnz + len > dev::BLOCK_NNZ || rows == dev::BLOCK_ROWS)) {
-            starts.push_back(r);
-            nnz = 0;
-            rows = 0;
-        }
-        nnz += len;
-        rows++;
-    }
+    build_blocks(A, 0, A.n, dev::block_nnz<double>(), starts);
     starts.push_back(A.n);
+    if (A.n == 0) starts.assign(1, 0);
     D.nblk = (int)starts.size() - 1;
-    D.blk = kupload(starts, B, st);
+    D.bk = kupload(block_pairs(A, starts), B, st);
     D.lpr = dev::lanes_per_row(A.n > 0 ? (double)A.nnz() / A.n : 0.0);
     return D;
 }

@@ -73,16 +73,17 @@ DevBuffers::~DevBuffers() {
     for (void* p : ptrs) (void)hipFree(p);
 }
 
-// row-block schedule over rows [begin, end): at most BLOCK_NNZ products and
-// BLOCK_ROWS rows per block; a row longer than BLOCK_NNZ gets a block alone
-static void build_blocks(const Csr& A, int begin, int end, std::vector<int>& starts) {
+// row-block schedule over rows [begin, end): at most max_nnz products and
+// BLOCK_ROWS rows per block; a row longer than max_nnz gets a block alone.
+// Appends the blocks' first rows to starts.
+void build_blocks(const Csr& A, int begin, int end, int max_nnz, std::vector<int>& starts) {
     int r = begin;
     int nnz = 0, rows = 0;
     if (begin >= end) return;
     starts.push_back(begin);
     while (r < end) {
         int len = A.rp[r + 1] - A.rp[r];
-        if (rows > 0 && (nnz + len > dev::BLOCK_NNZ || rows == dev::BLOCK_ROWS)) {
+        if (rows > 0 && (nnz + len > max_nnz || rows == dev::BLOCK_ROWS)) {
             starts.push_back(r);
             nnz = 0;
             rows = 0;
@@ -93,8 +94,18 @@ static void build_blocks(const Csr& A, int begin, int end, std::vector<int>& sta
     }
 }
 
+// (first row, first entry) of each block start (the list ends with A.n)
+std::vector<int2> block_pairs(const Csr& A, const std::vector<int>& starts_with_end) {
+    std::vector<int2> bk(starts_with_end.size());
+    for (size_t b = 0; b < starts_with_end.size(); b++) {
+        bk[b].x = starts_with_end[b];
+        bk[b].y = A.rp[starts_with_end[b]];
+    }
+    return bk;
+}
+
 // every index a kernel will dereference must be inside its array
-static void validate(const Csr& A, const std::vector<int>& starts) {
+static void validate(const Csr& A, const std::vector<int>& starts, int max_nnz) {
     if ((int)A.rp.size() != A.n + 1 || A.rp[0] != 0) throw std::logic_error("upload: bad row pointer");
     for (int i = 0; i < A.n; i++)
         if (A.rp[i + 1] < A.rp[i]) throw std::logic_error("upload: row pointer not monotone");
@@ -104,7 +115,7 @@ static void validate(const Csr& A, const std::vector<int>& starts) {
         int r0 = starts[b], r1 = starts[b + 1];
         if (r0 >= r1 || r1 > A.n) throw std::logic_error("upload: bad row block");
         if (r1 - r0 > dev::BLOCK_ROWS) throw std::logic_error("upload: row block too tall");
-        if (A.rp[r1] - A.rp[r0] > dev::BLOCK_NNZ && r1 - r0 != 1) throw std::logic_error("upload: row block too wide");
+        if (A.rp[r1] - A.rp[r0] > max_nnz && r1 - r0 != 1) throw std::logic_error("upload: row block too wide");
     }
 }
 
@@ -133,8 +144,8 @@ static Csr drop_outside_zeros(const Csr& A) {
 template <typename VT>
 static void upload_mat(const Csr& A, const std::vector<int>& starts_with_end, dev::DMat<VT>& D, DevBuffers& B,
                        hipStream_t st, const int* shared_rp = nullptr, const int* shared_ci = nullptr,
-                       const int* shared_blk = nullptr, const int* row_ids = nullptr) {
-    validate(A, starts_with_end);
+                       const int* row_ids = nullptr) {
+    validate(A, starts_with_end, dev::block_nnz<VT>());
     D.n = A.n;
     D.ncols = A.ncols;
     D.nnz = A.nnz();
@@ -144,7 +155,7 @@ static void upload_mat(const Csr& A, const std::vector<int>& starts_with_end, de
     D.v = dupload(v, B, st);
     D.nblk = (int)starts_with_end.size() - 1;
     if (D.nblk < 0) D.nblk = 0;
-    D.blk = shared_blk ? (int*)shared_blk : dupload(starts_with_end, B, st);
+    D.bk = dupload(block_pairs(A, starts_with_end), B, st);
     D.lpr = dev::lanes_per_row(A.n > 0 ? (double)A.nnz() / A.n : 0.0);
     // diagonal positions (row r's own index: row_ids == nullptr) and l1 norms
     std::vector<int> dpos(A.n, -1);
@@ -162,16 +173,23 @@ static void upload_mat(const Csr& A, const std::vector<int>& starts_with_end, de
     D.l1 = dupload(l1, B, st);
 }
 
-// upload A as FP64 + FP32 copies sharing the index arrays; returns #blocks
-int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st) {
-    Csr A = drop_outside_zeros(A0);
+// block schedule of [0, A.n) for value type VT, ending with A.n
+template <typename VT>
+static std::vector<int> schedule(const Csr& A) {
     std::vector<int> starts;
-    build_blocks(A, 0, A.n, starts);
+    build_blocks(A, 0, A.n, dev::block_nnz<VT>(), starts);
     starts.push_back(A.n);
     if (A.n == 0) starts.assign(1, 0);
-    upload_mat(A, starts, D, B, st);
-    upload_mat(A, starts, F, B, st, D.rp, D.ci, D.blk);
-    return D.nblk;
+    return starts;
+}
+
+// upload A as FP64 + FP32 copies sharing the index arrays (each with the row
+// blocks of its own value size); returns the larger block count
+int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st) {
+    Csr A = drop_outside_zeros(A0);
+    upload_mat(A, schedule<double>(A), D, B, st);
+    upload_mat(A, schedule<float>(A), F, B, st, D.rp, D.ci);
+    return std::max(D.nblk, F.nblk);
 }
 
 // level schedule of the lexicographic sweep: forward, level(i) = 1 + max over
@@ -270,10 +288,10 @@ int Solver::tune_mcsor(DevLevel& d, dev::DMat<VT>& Ac, const XT* f, XT* x) {
     if (Ac.nblk == 0 || d.ncolours == 0) return Ac.lpr;
     float best = 1e30f;
     int best_lpr = Ac.lpr;
+    const std::vector<int>& cb = sizeof(VT) == 8 ? d.colour_blk : d.colour_blk32;
     auto sweep = [&]() {
         for (int c = 0; c < d.ncolours; c++)
-            dev::mcsor_colour(stream_, Ac, d.colour_blk[c], d.colour_blk[c + 1] - d.colour_blk[c], d.perm, f, x,
-                              (XT)1);
+            dev::mcsor_colour(stream_, Ac, cb[c], cb[c + 1] - cb[c], d.perm, f, x, (XT)1);
     };
     for (int lpr : kLprCandidates) {
         Ac.lpr = lpr;
@@ -391,16 +409,22 @@ void Solver::upload() {
             }
             Ac.rp[r + 1] = (int)Ac.ci.size();
         }
-        std::vector<int> starts;
-        d.colour_blk.assign(nc + 1, 0);
-        for (int c = 0; c < nc; c++) {
-            d.colour_blk[c] = (int)starts.size();
-            build_blocks(Ac, d.colour_rows[c], d.colour_rows[c + 1], starts);
-        }
-        d.colour_blk[nc] = (int)starts.size();
-        starts.push_back(d.n);
-        upload_mat(Ac, starts, d.Ac, B, st, nullptr, nullptr, nullptr, perm.data());
-        upload_mat(Ac, starts, d.Acf, B, st, d.Ac.rp, d.Ac.ci, d.Ac.blk, perm.data());
+        // colour c's rows form the blocks [colour_blk[c], colour_blk[c+1])
+        auto colour_schedule = [&](int max_nnz, std::vector<int>& cb) {
+            std::vector<int> starts;
+            cb.assign(nc + 1, 0);
+            for (int c = 0; c < nc; c++) {
+                cb[c] = (int)starts.size();
+                build_blocks(Ac, d.colour_rows[c], d.colour_rows[c + 1], max_nnz, starts);
+            }
+            cb[nc] = (int)starts.size();
+            starts.push_back(d.n);
+            return starts;
+        };
+        upload_mat(Ac, colour_schedule(dev::block_nnz<double>(), d.colour_blk), d.Ac, B, st, nullptr, nullptr,
+                   perm.data());
+        upload_mat(Ac, colour_schedule(dev::block_nnz<float>(), d.colour_blk32), d.Acf, B, st, d.Ac.rp, d.Ac.ci,
+                   perm.data());
         d.perm = dupload(perm, B, st);
         d.ncolours = nc;
         // level schedules of the exact lexicographic sweeps
@@ -695,10 +719,11 @@ void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post, int xinit) {
         case LINA_SMOOTHER_MC_SOR:
         case LINA_SMOOTHER_MC_SSOR: {
             bool back = (smoother == LINA_SMOOTHER_MC_SSOR) && post;
+            const std::vector<int>& cb = sizeof(T) == 8 ? d.colour_blk : d.colour_blk32;
             for (int s = 0; s < sweeps; s++)
                 for (int cc = 0; cc < d.ncolours; cc++) {
                     int c = back ? d.ncolours - 1 - cc : cc;
-                    int b0 = d.colour_blk[c], b1 = d.colour_blk[c + 1];
+                    int b0 = cb[c], b1 = cb[c + 1];
                     dev::mcsor_colour(st, pickAc<T>(d), b0, b1 - b0, d.perm, f, x, omega);
                 }
             break;

@@ -37,7 +37,8 @@ struct DevLevel {
     int ncolours = 0;
     int* perm = nullptr;
     std::vector<int> colour_rows;  // ncolours+1 row offsets into Ac
-    std::vector<int> colour_blk;   // ncolours+1 block offsets into Ac.blk
+    std::vector<int> colour_blk;   // ncolours+1 block offsets into Ac's blocks
+    std::vector<int> colour_blk32; // the same for Acf (FP32 blocks hold more entries)
     // exact lexicographic sweeps
     int nlv_f = 0, nlv_b = 0;
     int *lvf_off = nullptr, *lvf_rows = nullptr, *lvb_off = nullptr, *lvb_rows = nullptr;
@@ -48,6 +49,8 @@ struct DevLevel {
 };
 
 int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st);
+void build_blocks(const Csr& A, int begin, int end, int max_nnz, std::vector<int>& starts);
+std::vector<int2> block_pairs(const Csr& A, const std::vector<int>& starts_with_end);
 // algorithmic bytes of one SpMV: 12 nnz + 4 (n+1) + 16 n (FP64), 8 nnz + 4 (n+1) + 8 n (FP32)
 double spmv_bytes(double n, double nnz, int precision);
 
