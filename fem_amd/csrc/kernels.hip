@@ -5,10 +5,10 @@
 //   phase 1  all 256 lanes of a workgroup stream the value and column arrays
 //            of a block of consecutive rows (<= 2048 entries, coalesced, each
 //            matrix entry read once per sweep) and store the products
-//            a_ij * x_j in LDS.  A block's
-//            bounds come as one (first row, first entry) pair, and each lane
-//            group's row data is loaded together with the streamed entries,
-//            so a block waits for two memory round trips, not five;
+//            a_ij * x_j in LDS.  A block's bounds come as one (first row,
+//            first entry) pair, and each lane group's row data is loaded
+//            together with the streamed entries, so a block waits for two
+//            memory round trips, not five;
 //   phase 2  rows are reduced from LDS by groups of LPR lanes.  LPR == 1
 //            adds a row's products one after another in column order, the
 //            order of the reference's `sum += A->Values[i][j]*x[...]` loops
@@ -100,14 +100,26 @@ struct Stage {
     int c[U];
     VT a[U];
     // entries past k1 load entry k1-1 again (in bounds, same cache line) so
-    // that every load is issued unconditionally; k1 > k0
-    __device__ __forceinline__ void load(int k0, int k1, const int* __restrict__ ci, const VT* __restrict__ v) {
+    // that every load is issued unconditionally; k1 > k0.  cb.x >= 0: the
+    // block's 16-bit column codes (see DMat::cs) with bases cb.
+    __device__ __forceinline__ void load(int k0, int k1, const int* __restrict__ ci,
+                                         const uint16_t* __restrict__ cs, int2 cb, const VT* __restrict__ v) {
         const int base = k0 + threadIdx.x;
+        if (cs && cb.x >= 0) {
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int k = min(base + u * TPB, k1 - 1);
-            c[u] = ld_stream<NT>(ci + k);
-            a[u] = ld_stream<NT>(v + k);
+            for (int u = 0; u < U; u++) {
+                const int k = min(base + u * TPB, k1 - 1);
+                const int o = (int)ld_stream<NT>(cs + k);
+                c[u] = ((o & 0x8000) ? cb.y : cb.x) + (o & 0x7fff);
+                a[u] = ld_stream<NT>(v + k);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = min(base + u * TPB, k1 - 1);
+                c[u] = ld_stream<NT>(ci + k);
+                a[u] = ld_stream<NT>(v + k);
+            }
         }
     }
     __device__ __forceinline__ void finish(int k0, int k1, const XT* __restrict__ x, XT* prod) {
@@ -172,7 +184,8 @@ __device__ __forceinline__ XT spmv_out(XT s, XT base, int r, XT* y, const Epi<VT
 // streamed entries, off the critical path.
 template <typename VT, typename XT, int MODE, int LPR>
 __global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int2* __restrict__ bk, const int* __restrict__ rp,
-                                              const int* __restrict__ ci, const VT* __restrict__ v,
+                                              const int* __restrict__ ci, const uint16_t* __restrict__ cs,
+                                              const int2* __restrict__ cbase, const VT* __restrict__ v,
                                               const XT* __restrict__ x, const XT* __restrict__ b,
                                               XT* __restrict__ y, double* __restrict__ part,
                                               const XT* __restrict__ w, const Epi<VT, XT> epi) {
@@ -196,7 +209,7 @@ __global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int2* __restrict__
         const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
         const int rf = r0 + g;
         Stage<VT, XT> st;
-        if (k1 > k0) st.load(k0, k1, ci, v);
+        if (k1 > k0) st.load(k0, k1, ci, cs, cs ? cbase[bb] : make_int2(-1, -1), v);
         int pb = 0, pe = 0;
         XT base0 = (XT)0;
         if (rf < r1) {
@@ -235,12 +248,34 @@ __global__ __launch_bounds__(TPB) void k_dot_part(int n, const XT* __restrict__ 
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// Sum of np block partials by one workgroup of FTPB threads: four
+// independent accumulators per thread (so that four loads are in flight),
+// then the wave trees, then the waves in order.  Deterministic.
+constexpr int FTPB = 1024;
+__device__ __forceinline__ double partial_sum(int np, const double* __restrict__ part) {
+    __shared__ double ws[FTPB / WAVE];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 3 * FTPB < np; i += 4 * FTPB) {
+        s0 += part[i];
+        s1 += part[i + FTPB];
+        s2 += part[i + 2 * FTPB];
+        s3 += part[i + 3 * FTPB];
+    }
+    for (; i < np; i += FTPB) s0 += part[i];
+    double v = wave_sum((s0 + s1) + (s2 + s3));
+    if (threadIdx.x % WAVE == 0) ws[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < FTPB / WAVE; w++) t += ws[w];
+    return t;  // valid in thread 0
+}
+
 // out[slot] = sum(part); op: 0 plain, 1 sqrt, 2 shift (out[slot+1]=out[slot]) then plain
-__global__ __launch_bounds__(TPB) void k_finalize(int np, const double* __restrict__ part, double* __restrict__ scal,
-                                                  int slot, int op) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < np; i += TPB) s += part[i];
-    double t = block_sum(s);
+__global__ __launch_bounds__(FTPB) void k_finalize(int np, const double* __restrict__ part, double* __restrict__ scal,
+                                                   int slot, int op) {
+    double t = partial_sum(np, part);
     if (threadIdx.x == 0) {
         if (op == 2) scal[slot + 1] = scal[slot];
         scal[slot] = (op == 1) ? sqrt(t) : t;
@@ -249,13 +284,11 @@ __global__ __launch_bounds__(TPB) void k_finalize(int np, const double* __restri
 
 // pcg's |R| and stop test (the loop condition of PCGS.c:224 evaluated on the
 // device): nothing happens once scal[S_STOP] is set
-__global__ __launch_bounds__(TPB) void k_finalize_stop(int np, const double* __restrict__ part,
-                                                       double* __restrict__ scal, double tol, int relres,
-                                                       int max_iter) {
+__global__ __launch_bounds__(FTPB) void k_finalize_stop(int np, const double* __restrict__ part,
+                                                        double* __restrict__ scal, double tol, int relres,
+                                                        int max_iter) {
     if (scal[S_STOP] != 0.0) return;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < np; i += TPB) s += part[i];
-    double t = block_sum(s);
+    double t = partial_sum(np, part);
     if (threadIdx.x == 0) {
         const double r = sqrt(t);
         scal[S_R] = r;
@@ -428,6 +461,7 @@ __global__ void k_axpby(int n, double a, double* __restrict__ x, double b, const
 template <typename VT, typename XT, int LPR>
 __global__ __launch_bounds__(TPB) void k_mcsor(int nblk, int blk0, const int2* __restrict__ bk,
                                                const int* __restrict__ rp, const int* __restrict__ ci,
+                                               const uint16_t* __restrict__ cs, const int2* __restrict__ cbase,
                                                const VT* __restrict__ v, const int* __restrict__ dpos,
                                                const int* __restrict__ perm, const XT* __restrict__ f,
                                                XT* __restrict__ x, XT omega) {
@@ -465,7 +499,7 @@ __global__ __launch_bounds__(TPB) void k_mcsor(int nblk, int blk0, const int2* _
         pdp = dpos[rf];
     }
     Stage<VT, XT> st;
-    if (k1 > k0) st.load(k0, k1, ci, v);
+    if (k1 > k0) st.load(k0, k1, ci, cs, cs ? cbase[bb] : make_int2(-1, -1), v);
     XT pf = 0, px = 0, pd = 0;
     if (rf < r1 && leader) {
         pf = f[pi];
@@ -540,7 +574,8 @@ __global__ __launch_bounds__(1024) void k_sor_lex(int n, const int* __restrict__
 //   JAC_L1  : x_i = xo_i + (f_i - sum_j a_ij xo_j)/sum_j |a_ij|
 template <typename VT, typename XT, int KIND, int LPR>
 __global__ __launch_bounds__(TPB) void k_jacobi(int nblk, const int2* __restrict__ bk, const int* __restrict__ rp,
-                                                const int* __restrict__ ci, const VT* __restrict__ v,
+                                                const int* __restrict__ ci, const uint16_t* __restrict__ cs,
+                                                const int2* __restrict__ cbase, const VT* __restrict__ v,
                                                 const int* __restrict__ dpos, const VT* __restrict__ l1,
                                                 const XT* __restrict__ f, const XT* __restrict__ xo,
                                                 XT* __restrict__ x, int nlim) {
@@ -578,7 +613,7 @@ __global__ __launch_bounds__(TPB) void k_jacobi(int nblk, const int2* __restrict
     XT pf = 0, pxo = 0;
     VT pl1 = 0;
     Stage<VT, XT> st;
-    if (k1 > k0) st.load(k0, k1, ci, v);
+    if (k1 > k0) st.load(k0, k1, ci, cs, cs ? cbase[bb] : make_int2(-1, -1), v);
     if (rf < r1) {
         pb = rp[rf];
         pe = rp[rf + 1];
@@ -1115,7 +1150,7 @@ int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int
     const Epi<VT, XT> e = epi ? *epi : Epi<VT, XT>{};
 #define LINA_SPMV_MODE(M)                                                                                       \
     LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, M, L>), g, t, 0, st, A.nblk, A.bk, A.rp, A.ci, \
-                                                A.v, x, b, y, part, w, e))
+                                                A.cs, A.cb, A.v, x, b, y, part, w, e))
     switch (mode) {
         case MODE_SET: LINA_SPMV_MODE(MODE_SET); break;
         case MODE_OUTER: LINA_SPMV_MODE(MODE_OUTER); break;
@@ -1143,11 +1178,11 @@ template int dot_part<double>(hipStream_t, int, const double*, const double*, do
 template int dot_part<float>(hipStream_t, int, const float*, const float*, double*);
 
 void finalize(hipStream_t st, int np, const double* part, double* scal, int slot, int op) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(TPB), 0, st, np, part, scal, slot, op);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FTPB), 0, st, np, part, scal, slot, op);
 }
 
 void finalize_stop(hipStream_t st, int np, const double* part, double* scal, double tol, int relres, int max_iter) {
-    hipLaunchKernelGGL(k_finalize_stop, dim3(1), dim3(TPB), 0, st, np, part, scal, tol, relres, max_iter);
+    hipLaunchKernelGGL(k_finalize_stop, dim3(1), dim3(FTPB), 0, st, np, part, scal, tol, relres, max_iter);
 }
 
 template <typename XT>
@@ -1299,7 +1334,7 @@ void mcsor_colour(hipStream_t st, const DMat<VT>& Ac, int blk0, int nblk, const 
                   XT omega) {
     if (nblk <= 0) return;
     LINA_LPR_DISPATCH(Ac.lpr, hipLaunchKernelGGL((k_mcsor<VT, XT, L>), dim3(nblk), dim3(TPB), 0, st, nblk, blk0,
-                                                 Ac.bk, Ac.rp, Ac.ci, Ac.v, Ac.dpos, perm, f, x, omega))
+                                                 Ac.bk, Ac.rp, Ac.ci, Ac.cs, Ac.cb, Ac.v, Ac.dpos, perm, f, x, omega))
 }
 template void mcsor_colour<double, double>(hipStream_t, const DMat<double>&, int, int, const int*, const double*,
                                            double*, double);
@@ -1324,10 +1359,10 @@ void jacobi(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, const XT* 
     dim3 g(A.nblk), t(TPB);
     if (kind == JAC_REF) {
         LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_REF, L>), g, t, 0, st, A.nblk, A.bk, A.rp,
-                                                    A.ci, A.v, A.dpos, A.l1, f, xo, x, nlim))
+                                                    A.ci, A.cs, A.cb, A.v, A.dpos, A.l1, f, xo, x, nlim))
     } else {
         LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_L1, L>), g, t, 0, st, A.nblk, A.bk, A.rp,
-                                                    A.ci, A.v, A.dpos, A.l1, f, xo, x, nlim))
+                                                    A.ci, A.cs, A.cb, A.v, A.dpos, A.l1, f, xo, x, nlim))
     }
 }
 template void jacobi<double, double>(hipStream_t, const DMat<double>&, int, const double*, const double*, double*,

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 namespace lina {
 namespace dev {
 
@@ -62,6 +64,13 @@ struct DMat {
     VT* v = nullptr;
     int nblk = 0;
     int2* bk = nullptr;  // nblk+1 (first row, first entry) pairs of the row blocks
+    // 16-bit column codes: a block's columns form at most two clusters, each
+    // spanning < 32768, with base columns cb[block] = (b0, b1); an entry's
+    // column is (code & 0x8000 ? b1 : b0) + (code & 0x7fff).  b0 < 0: the
+    // block's columns are read from ci.  cs == nullptr: ci everywhere.
+    uint16_t* cs = nullptr;
+    int2* cb = nullptr;
+    double idx_bytes = 0;  // column index bytes the row kernels read (host-side accounting)
     int lpr = 1;         // lanes reducing one row from LDS: 1 (sequential), 4, 8, 16, 32
     int* dpos = nullptr; // position of each row's diagonal entry in ci/v (-1: none)
     VT* l1 = nullptr;    // sum_j |a_ij| of each row (l1-Jacobi), summed in column order

@@ -141,21 +141,76 @@ static Csr drop_outside_zeros(const Csr& A) {
     return B;
 }
 
-template <typename VT>
+// 16-bit column codes of a schedule (see dev::DMat::cs): per block, the
+// sorted columns are split at their largest gap into at most two clusters;
+// when each spans < 32768 the block gets bases (b0, b1) and codes
+// sel << 15 | (column - base_sel), otherwise b0 = -1 (32-bit columns).
+// Returns the number of entries coded in 16 bits.
+static long column_codes(const Csr& A, const std::vector<int>& starts_with_end, std::vector<uint16_t>& cs,
+                         std::vector<int2>& cb) {
+    const int nblk = (int)starts_with_end.size() - 1;
+    cb.assign(std::max(nblk, 1), make_int2(-1, -1));
+    cs.assign(A.ci.size(), 0);
+    long coded = 0;
+    std::vector<int> c;
+    for (int b = 0; b < nblk; b++) {
+        const int k0 = A.rp[starts_with_end[b]], k1 = A.rp[starts_with_end[b + 1]];
+        if (k1 <= k0) continue;
+        c.assign(A.ci.begin() + k0, A.ci.begin() + k1);
+        std::sort(c.begin(), c.end());
+        size_t g = 0;  // clusters [c0, c[g]] and [c[g+1], c.back()]
+        for (size_t i = 1; i < c.size(); i++)
+            if (c[i] - c[i - 1] > c[g + 1] - c[g]) g = i - 1;
+        int b0 = c.front(), b1 = c.front();
+        if (c.back() - c.front() > 32767) {
+            if (c[g] - c.front() > 32767 || c.back() - c[g + 1] > 32767) continue;
+            b1 = c[g + 1];
+        }
+        cb[b] = make_int2(b0, b1);
+        for (int k = k0; k < k1; k++) {
+            const int col = A.ci[k];
+            cs[k] = (b1 != b0 && col >= b1) ? (uint16_t)(0x8000 | (col - b1)) : (uint16_t)(col - b0);
+        }
+        coded += k1 - k0;
+    }
+    return coded;
+}
+
+// share: a copy of A with other values uploaded before; its row pointers and
+// column indices are reused, and its 16-bit offsets too when its schedule
+// (share_starts) is the same
+template <typename VT, typename OT = double>
 static void upload_mat(const Csr& A, const std::vector<int>& starts_with_end, dev::DMat<VT>& D, DevBuffers& B,
-                       hipStream_t st, const int* shared_rp = nullptr, const int* shared_ci = nullptr,
-                       const int* row_ids = nullptr) {
+                       hipStream_t st, const dev::DMat<OT>* share = nullptr,
+                       const std::vector<int>* share_starts = nullptr, const int* row_ids = nullptr) {
     validate(A, starts_with_end, dev::block_nnz<VT>());
     D.n = A.n;
     D.ncols = A.ncols;
     D.nnz = A.nnz();
-    D.rp = shared_rp ? (int*)shared_rp : dupload(A.rp, B, st);
-    D.ci = shared_ci ? (int*)shared_ci : dupload(A.ci, B, st);
+    D.rp = share ? share->rp : dupload(A.rp, B, st);
+    D.ci = share ? share->ci : dupload(A.ci, B, st);
     std::vector<VT> v(A.v.begin(), A.v.end());
     D.v = dupload(v, B, st);
     D.nblk = (int)starts_with_end.size() - 1;
     if (D.nblk < 0) D.nblk = 0;
     D.bk = dupload(block_pairs(A, starts_with_end), B, st);
+    if (share && share_starts && *share_starts == starts_with_end) {
+        D.cs = share->cs;
+        D.cb = share->cb;
+        D.idx_bytes = share->idx_bytes;
+    } else {
+        std::vector<uint16_t> cs;
+        std::vector<int2> cb;
+        const long coded = A.nnz() > 0 ? column_codes(A, starts_with_end, cs, cb) : 0;
+        D.cs = nullptr;
+        D.cb = nullptr;
+        D.idx_bytes = 4.0 * A.nnz();
+        if (coded > 0) {
+            D.cs = dupload(cs, B, st);
+            D.cb = dupload(cb, B, st);
+            D.idx_bytes = 2.0 * coded + 4.0 * (A.nnz() - coded) + 8.0 * D.nblk;
+        }
+    }
     D.lpr = dev::lanes_per_row(A.n > 0 ? (double)A.nnz() / A.n : 0.0);
     // diagonal positions (row r's own index: row_ids == nullptr) and l1 norms
     std::vector<int> dpos(A.n, -1);
@@ -173,11 +228,20 @@ static void upload_mat(const Csr& A, const std::vector<int>& starts_with_end, de
     D.l1 = dupload(l1, B, st);
 }
 
+// entries per row block of a matrix with nnz entries: the kernels' maximum,
+// halved (down to 256) until the matrix has about 2048 blocks -- 8 per CU, so
+// that the small coarse levels and transfers are not left to a few CUs
+static int block_cap(long nnz, int cap) {
+    int m = cap;
+    while (m > 256 && nnz / m < 2048) m /= 2;
+    return m;
+}
+
 // block schedule of [0, A.n) for value type VT, ending with A.n
 template <typename VT>
 static std::vector<int> schedule(const Csr& A) {
     std::vector<int> starts;
-    build_blocks(A, 0, A.n, dev::block_nnz<VT>(), starts);
+    build_blocks(A, 0, A.n, block_cap(A.nnz(), dev::block_nnz<VT>()), starts);
     starts.push_back(A.n);
     if (A.n == 0) starts.assign(1, 0);
     return starts;
@@ -187,8 +251,9 @@ static std::vector<int> schedule(const Csr& A) {
 // blocks of its own value size); returns the larger block count
 int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st) {
     Csr A = drop_outside_zeros(A0);
-    upload_mat(A, schedule<double>(A), D, B, st);
-    upload_mat(A, schedule<float>(A), F, B, st, D.rp, D.ci);
+    const std::vector<int> s64 = schedule<double>(A), s32 = schedule<float>(A);
+    upload_mat(A, s64, D, B, st);
+    upload_mat(A, s32, F, B, st, &D, &s64);
     return std::max(D.nblk, F.nblk);
 }
 
@@ -421,10 +486,10 @@ void Solver::upload() {
             starts.push_back(d.n);
             return starts;
         };
-        upload_mat(Ac, colour_schedule(dev::block_nnz<double>(), d.colour_blk), d.Ac, B, st, nullptr, nullptr,
-                   perm.data());
-        upload_mat(Ac, colour_schedule(dev::block_nnz<float>(), d.colour_blk32), d.Acf, B, st, d.Ac.rp, d.Ac.ci,
-                   perm.data());
+        const std::vector<int> c64 = colour_schedule(dev::block_nnz<double>(), d.colour_blk);
+        const std::vector<int> c32 = colour_schedule(dev::block_nnz<float>(), d.colour_blk32);
+        upload_mat<double, double>(Ac, c64, d.Ac, B, st, nullptr, nullptr, perm.data());
+        upload_mat(Ac, c32, d.Acf, B, st, &d.Ac, &c64, perm.data());
         d.perm = dupload(perm, B, st);
         d.ncolours = nc;
         // level schedules of the exact lexicographic sweeps
@@ -1005,6 +1070,9 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
     const int cap = std::max(max_iter, 1);  // the loop body runs at least once
     // one iteration: Z = M^-1 R with R.Z, P, Q = A P with P.Q, X and R with
     // |R| and the stop test; its state (S_R..S_IT) copied to ring slot k % 2
+    // kernel timing samples every fourth iteration's outer SpMV (the events
+    // themselves cost a few microseconds)
+    auto sampled = [&](int k) { return ktime_ && k % 4 == 0; };
     auto enqueue = [&](int k) {
         int m = 0;
         precon(R, Z, R, &m);
@@ -1016,9 +1084,9 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
             apply_A(P, Q);
             m = dev::dot_part<double>(st, n, P, Q, part_);
         } else {
-            if (ktime_) hip_check(hipEventRecord(evk_[k % 2][0], st), "event");
+            if (sampled(k)) hip_check(hipEventRecord(evk_[k % 2][0], st), "event");
             m = dev::spmv<double, double>(st, top.A, P, nullptr, Q, dev::MODE_OUTER, part_, P);  // Q = A P, P.Q
-            if (ktime_) hip_check(hipEventRecord(evk_[k % 2][1], st), "event");
+            if (sampled(k)) hip_check(hipEventRecord(evk_[k % 2][1], st), "event");
         }
         dev::finalize(st, m, part_, scal_, dev::S_PQ, 0);
         m = dev::pcg_xr(st, n, X, R, P, Q, scal_, part_);
@@ -1041,7 +1109,7 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
         hip_check(hipEventSynchronize(evr_[done % 2]), "event sync");
         const double* h = host_ring_ + (done % 2) * dev::S_COUNT;
         const int it_dev = (int)h[dev::S_IT];
-        if (timed && it_dev == done + 1) {
+        if (timed && sampled(done) && it_dev == done + 1) {
             float kms = 0;
             hip_check(hipEventElapsedTime(&kms, evk_[done % 2][0], evk_[done % 2][1]), "elapsed");
             ksum += kms;
@@ -1144,7 +1212,18 @@ double spmv_bytes(double n, double nnz, int precision) {
     return 8.0 * nnz + 4.0 * (n + 1) + 8.0 * n;
 }
 
+// algorithmic bytes of one y = A x with the uploaded arrays: values, column
+// indices (2-byte codes + 8 bytes of bases per block, or 4 bytes), row
+// pointers, block pairs, x gathered once, y written once
+template <typename VT>
+static double mat_bytes(const dev::DMat<VT>& A) {
+    return (double)sizeof(VT) * A.nnz + A.idx_bytes + 4.0 * (A.n + 1) + 8.0 * (A.nblk + 1) +
+           (double)sizeof(VT) * (A.ncols + A.n);
+}
+
 double Solver::spmv_bytes(int level, int precision) const {
+    if (uploaded() && level >= 0 && level < (int)L_.size())
+        return precision == LINA_PREC_FP64 ? mat_bytes(L_[level].A) : mat_bytes(L_[level].Af);
     const Csr& A = H_.lv.at(level).A;
     return lina::spmv_bytes(A.n, A.nnz(), precision);
 }
@@ -1222,7 +1301,9 @@ double DevMatrix::time_spmv(int precision, int reps) {
     return ms / reps;
 }
 
-double DevMatrix::bytes(int precision) const { return lina::spmv_bytes(n_, nnz_, precision); }
+double DevMatrix::bytes(int precision) const {
+    return precision == LINA_PREC_FP64 ? mat_bytes(A_) : mat_bytes(Af_);
+}
 
 template void Solver::smooth<double>(int, const double*, double*, int, bool, int);
 template void Solver::smooth<float>(int, const float*, float*, int, bool, int);

@@ -51,7 +51,8 @@ struct DevLevel {
 int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st);
 void build_blocks(const Csr& A, int begin, int end, int max_nnz, std::vector<int>& starts);
 std::vector<int2> block_pairs(const Csr& A, const std::vector<int>& starts_with_end);
-// algorithmic bytes of one SpMV: 12 nnz + 4 (n+1) + 16 n (FP64), 8 nnz + 4 (n+1) + 8 n (FP32)
+// algorithmic bytes of one SpMV with 32-bit indices (before upload): 12 nnz + 4 (n+1) + 16 n (FP64),
+// 8 nnz + 4 (n+1) + 8 n (FP32)
 double spmv_bytes(double n, double nnz, int precision);
 
 // A device-resident matrix (FP64 and FP32 copies) for y = A x outside a

@@ -80,7 +80,7 @@ typedef struct lina_stats {
     double relres;       /* |R_final| of the recurrence with B scaled to 1 */
     double t_solve_ms;   /* device time of the solve (HIP events on the library stream) */
     double spmv_ms;      /* mean duration of the outer-loop A*p launch (kernel timing on), else 0 */
-    int spmv_launches;   /* launches averaged into spmv_ms */
+    int spmv_launches;   /* launches averaged into spmv_ms (every fourth iteration's) */
 } lina_stats;
 
 typedef struct lina_solver lina_solver;
@@ -182,11 +182,13 @@ int lina_spmv(lina_solver* s, int level, int precision, const double* x, double*
 /* average device time (ms) of one finest-level SpMV launch over reps
  * launches, HIP events recorded on the library's stream */
 double lina_time_spmv(lina_solver* s, int level, int precision, int reps);
-/* per-launch byte count used for roofline reporting: 12 nnz + 4 (n+1) + 16 n
- * for FP64 values (8 nnz + 4 (n+1) + 8 n for FP32) */
+/* per-launch algorithmic byte count used for roofline reporting, from the
+ * uploaded arrays: (value size + index size) nnz + 4 (n+1) row pointers +
+ * 8 (blocks+1) block bounds (+ 4 per block of base columns) + x and y once;
+ * the index size is 2 where every row block's columns span < 65536, else 4 */
 double lina_spmv_bytes(const lina_solver* s, int level, int precision);
-/* on != 0: pcg brackets each outer-loop A*p launch with HIP events and
- * reports the mean launch duration in lina_stats.spmv_ms */
+/* on != 0: pcg brackets the outer-loop A*p launch of every fourth iteration
+ * with HIP events and reports the mean launch duration in lina_stats.spmv_ms */
 int lina_amg_kernel_timing(lina_solver* s, int on);
 /* the autotuner's choices (params.autotune): per tuned matrix its level, kind
  * (A, P, R, C = colour-permuted A) and lanes per row for the FP64 / FP32
@@ -205,6 +207,7 @@ int lina_matrix_spmv(lina_matrix* h, int precision, const double* x, double* y);
 /* device pointers (FP64); returns after the product is complete */
 int lina_matrix_spmv_device(lina_matrix* h, const double* x, double* y);
 double lina_matrix_time_spmv(lina_matrix* h, int precision, int reps);
+/* the algorithmic bytes of one product, counted as for lina_spmv_bytes */
 double lina_matrix_spmv_bytes(const lina_matrix* h, int precision);
 /* pcg (PCGS.c:163-233) with the operator op (PCG_set_mat_mult with a device
  * matrix, PCGS.c:137-139) and this hierarchy as the preconditioner: solves

@@ -55,19 +55,40 @@ def _ragged(n, lens, seed):
     return np.array(rp, np.int32), np.array(cols, np.int32), np.array(vals)
 
 
-@pytest.mark.parametrize("case", ["elast65", "dense_rows", "long_rows", "one_row"])
+def _clusters(n, seed):
+    """CSR whose row blocks see one, two or three column clusters: row i
+    couples to i, i+1, i+n/2 and i+n/2+7 (mod n), and every 997th row also to
+    a far column, so the 16-bit column codes take the single-base, the
+    two-base and the 32-bit fallback path."""
+    rng = np.random.default_rng(seed)
+    rp = [0]
+    cols = []
+    for i in range(n):
+        c = {i, (i + 1) % n, (i + n // 2) % n, (i + n // 2 + 7) % n}
+        if i % 997 == 0:
+            c.add((i * 7919 + n // 4) % n)
+        cols += sorted(c)
+        rp.append(len(cols))
+    return np.array(rp, np.int32), np.array(cols, np.int32), rng.uniform(-1, 1, len(cols))
+
+
+@pytest.mark.parametrize("case", ["elast65", "elast257", "clusters", "dense_rows", "long_rows", "one_row"])
 def test_matrix_handle_spmv(gpu, case):
     """lina_matrix: bit-identical to sequential row sums where one lane sums a
     row (mean row length <= 24); otherwise within the componentwise bound of
     a reordered sum, |y - y_seq| <= 64 eps sum_j |a_ij x_j|.  Covers blocks
     of many short rows, 16/32-lane rows, rows longer than one LDS block (the
-    workgroup-wide path) and a 1x1 matrix."""
+    workgroup-wide path), a 1x1 matrix, and column layouts that take each
+    path of the 16-bit column codes (one base, two bases, 32-bit fallback:
+    elast257's blocked [ux; uy] ordering and the synthetic clusters)."""
     from oracle.bind import Oracle
     say("matrix spmv", case)
     rng = np.random.default_rng(7)
-    if case == "elast65":
-        s = P.degraded_elasticity(65)
+    if case in ("elast65", "elast257"):
+        s = P.degraded_elasticity(int(case[5:]))
         rp, ci, v = s.rowptr, s.col, s.val
+    elif case == "clusters":
+        rp, ci, v = _clusters(120000, 3)
     elif case == "dense_rows":
         rp, ci, v = _ragged(3000, rng.integers(60, 400, 3000), 1)
     elif case == "long_rows":
