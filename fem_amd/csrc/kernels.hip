@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 
 #include "kernels.hpp"
 
@@ -178,20 +179,33 @@ __device__ __forceinline__ XT spmv_out(XT s, XT base, int r, XT* y, const Epi<VT
     return out;
 }
 
-// One workgroup per row block (block bb of the XCD remap).  The first row of
-// each lane group's reduction (all rows of a block when it has at most
-// TPB/LPR rows) has its row bounds and base entry loaded together with the
-// streamed entries, off the critical path.
+// Logical row block of iteration k of workgroup b.  A launch with one
+// workgroup per block (G >= nblk) takes xcd_block's remap; a persistent grid
+// (G < nblk, G a multiple of 8) gives XCD x the contiguous block range
+// [x*chunk, (x+1)*chunk) and walks it with the XCD's G/8 workgroups, so that
+// workgroups stay resident instead of being dispatched once per block.
+__device__ __forceinline__ int grid_block(int b, int G, int k, int nblk) {
+    if (G >= nblk) return k == 0 ? xcd_block(b, nblk) : -1;
+    constexpr int X = 8;
+    const int per = G / X, x = b % X, j = b / X;
+    const int chunk = (nblk + X - 1) / X;
+    const int local = j + k * per;
+    if (local >= chunk) return -1;
+    const int idx = x * chunk + local;
+    return idx < nblk ? idx : -1;
+}
+
+// One row block bb.  The first row of each lane group's reduction (all rows
+// of a block when it has at most TPB/LPR rows) has its row bounds and base
+// entry loaded together with the streamed entries, off the critical path.
 template <typename VT, typename XT, int MODE, int LPR>
-__global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int2* __restrict__ bk, const int* __restrict__ rp,
-                                              const int* __restrict__ ci, const uint16_t* __restrict__ cs,
-                                              const int2* __restrict__ cbase, const VT* __restrict__ v,
-                                              const XT* __restrict__ x, const XT* __restrict__ b,
-                                              XT* __restrict__ y, double* __restrict__ part,
-                                              const XT* __restrict__ w, const Epi<VT, XT> epi) {
+__device__ __forceinline__ void spmv_block(int bb, const int2* __restrict__ bk, const int* __restrict__ rp,
+                                           const int* __restrict__ ci, const uint16_t* __restrict__ cs,
+                                           const int2* __restrict__ cbase, const VT* __restrict__ v,
+                                           const XT* __restrict__ x, const XT* __restrict__ b, XT* __restrict__ y,
+                                           double* __restrict__ part, const XT* __restrict__ w,
+                                           const Epi<VT, XT>& epi, XT* prod) {
     constexpr int BN = block_nnz<VT>();
-    __shared__ XT prod[BN];
-    const int bb = xcd_block(blockIdx.x, nblk);
     const int2 s0 = bk[bb], s1 = bk[bb + 1];
     const int r0 = s0.x, r1 = s1.x, k0 = s0.y, k1 = s1.y;
     double mine = 0.0;
@@ -234,6 +248,22 @@ __global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int2* __restrict__
     if (part) {
         double t = block_sum(mine);
         if (threadIdx.x == 0) part[bb] = t;
+    }
+}
+
+template <typename VT, typename XT, int MODE, int LPR>
+__global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int2* __restrict__ bk, const int* __restrict__ rp,
+                                              const int* __restrict__ ci, const uint16_t* __restrict__ cs,
+                                              const int2* __restrict__ cbase, const VT* __restrict__ v,
+                                              const XT* __restrict__ x, const XT* __restrict__ b,
+                                              XT* __restrict__ y, double* __restrict__ part,
+                                              const XT* __restrict__ w, const Epi<VT, XT> epi) {
+    __shared__ XT prod[block_nnz<VT>()];
+    for (int it = 0;; it++) {
+        const int bb = grid_block(blockIdx.x, gridDim.x, it, nblk);
+        if (bb < 0) break;
+        spmv_block<VT, XT, MODE, LPR>(bb, bk, rp, ci, cs, cbase, v, x, b, y, part, w, epi, prod);
+        __syncthreads();  // prod is restaged by the next block
     }
 }
 
@@ -292,6 +322,7 @@ __global__ __launch_bounds__(FTPB) void k_finalize_stop(int np, const double* __
     if (threadIdx.x == 0) {
         const double r = sqrt(t);
         scal[S_R] = r;
+        scal[S_AMG0] = t;  // the next zero-guess preconditioner's r0 test when pcg_xr prepared its entry
         const double it = scal[S_IT] + 1.0;
         scal[S_IT] = it;
         const double crit = relres ? r : r / scal[S_R0];
@@ -299,19 +330,24 @@ __global__ __launch_bounds__(FTPB) void k_finalize_stop(int np, const double* __
     }
 }
 
+// row i is isolated (bit i of the mask)
+__device__ __forceinline__ bool isolated(const uint32_t* __restrict__ isb, int i) {
+    return isb && ((isb[i >> 5] >> (i & 31)) & 1u);
+}
+
 // zero-guess preconditioner entry (see precon_in); the F.F partials follow
 // k_dot_part's loop and tree exactly
 template <typename XT>
 __global__ __launch_bounds__(TPB) void k_precon_in(int n, const double* __restrict__ F, double* __restrict__ part,
                                                    double* __restrict__ X, XT* __restrict__ xf, XT* __restrict__ ff,
-                                                   const double* __restrict__ isa) {
+                                                   const uint32_t* __restrict__ isb, const double* __restrict__ isa) {
     double s = 0.0;
     for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) {
         const double f = F[i];
         s += f * f;
         if (X || xf) {
             double x = 0.0;
-            if (isa && isa[i] != 0.0) x = f / isa[i];
+            if (isolated(isb, i)) x = f / isa[i];
             if (X) X[i] = x;
             if (xf) xf[i] = (XT)x;
         }
@@ -324,7 +360,8 @@ __global__ __launch_bounds__(TPB) void k_precon_in(int n, const double* __restri
 // zero-guess preconditioner exit (see precon_out)
 template <typename XT>
 __global__ __launch_bounds__(TPB) void k_precon_out(int n, const XT* __restrict__ xs, double* __restrict__ Z,
-                                                    const double* __restrict__ F, const double* __restrict__ isa,
+                                                    const double* __restrict__ F, const uint32_t* __restrict__ isb,
+                                                    const double* __restrict__ isa,
                                                     const double* __restrict__ scal, int slot,
                                                     const double* __restrict__ w, double* __restrict__ part) {
     const bool restore = scal[slot] == 0.0;
@@ -333,7 +370,7 @@ __global__ __launch_bounds__(TPB) void k_precon_out(int n, const XT* __restrict_
         double z;
         if (restore) {
             z = 0.0;
-            if (isa && isa[i] != 0.0) z = F[i] / isa[i];
+            if (isolated(isb, i)) z = F[i] / isa[i];
             Z[i] = z;
         } else if (xs) {
             z = (double)xs[i];
@@ -419,10 +456,19 @@ __global__ void k_pcg_p(int n, double* __restrict__ P, const double* __restrict_
     }
 }
 
-// pcg: a = s/(P.Q); X += a P; R += (-a) Q; partial |R|^2
+// pcg: a = s/(P.Q); X += a P; R += (-a) Q; partial |R|^2.  Also the next
+// zero-guess preconditioner entry on the new R where asked (precon_in's
+// outputs, with the signed right-hand side ft = 0 + sg_i r_i when sg is
+// given): ff = (XT) ft when ff is given, xf = the initial guess (0; ft / isa_i
+// on the rows marked in isb) when xf is given.  Sum of squares of ft = that
+// of r.
+template <typename XT>
 __global__ __launch_bounds__(TPB) void k_pcg_xr(int n, double* __restrict__ X, double* __restrict__ R,
                                                 const double* __restrict__ P, const double* __restrict__ Q,
-                                                const double* __restrict__ scal, double* __restrict__ part) {
+                                                const double* __restrict__ scal, double* __restrict__ part,
+                                                XT* __restrict__ ff, XT* __restrict__ xf,
+                                                const double* __restrict__ sg, const uint32_t* __restrict__ isb,
+                                                const double* __restrict__ isa) {
     if (scal[S_STOP] != 0.0) return;
     double a = scal[S_S] / scal[S_PQ];
     double na = -a;
@@ -432,6 +478,15 @@ __global__ __launch_bounds__(TPB) void k_pcg_xr(int n, double* __restrict__ X, d
         double r = R[i] + na * Q[i];
         R[i] = r;
         s += r * r;
+        if (ff || xf) {
+            const double ft = sg ? 0.0 + sg[i] * r : r;
+            if (ff) ff[i] = (XT)ft;
+            if (xf) {
+                double x0 = 0.0;
+                if (isolated(isb, i)) x0 = ft / isa[i];
+                xf[i] = (XT)x0;
+            }
+        }
     }
     double t = block_sum(s);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
@@ -977,35 +1032,68 @@ __global__ __launch_bounds__(TPB) void k_coarse(int nd, const int* __restrict__ 
     }
 }
 
-// x = inv * bd with an FP32 copy of the inverse (rows padded to ld, a
-// multiple of 4, with zeros; bd padded likewise): one wave per dense row,
-// nontemporal 16-byte loads of 4 entries, products and sums in FP64 (four
-// accumulators per lane, combined in a fixed order); trivial rows as k_coarse
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// Coarse solve with the symmetric inverse held as its upper triangle of
+// CT x CT FP32 tiles (tile t = (ti[t], tj[t]), ti <= tj, row-major, zero
+// padded), products and sums in FP64.  k_csym_tiles: one workgroup per stored
+// tile (I, J) writes the tile's row sums against bd_J into part[I][J] and,
+// off the diagonal, its column sums against bd_I into part[J][I];
+// k_csym_reduce sums part[I][0..nct) in a fixed order into the dense rows of
+// block I.  Every part[I][K] has exactly one writer.
+__global__ __launch_bounds__(TPB) void k_csym_tiles(int nct, const int* __restrict__ ti, const int* __restrict__ tj,
+                                                    const float* __restrict__ tiles, const double* __restrict__ bd,
+                                                    double* __restrict__ part) {
+    __shared__ float T[CT][CT + 1];
+    __shared__ double bi[CT], bj[CT];
+    const int t = blockIdx.x;
+    const int I = ti[t], J = tj[t];
+    const f32x4* src = reinterpret_cast<const f32x4*>(tiles + (size_t)t * CT * CT);
+#pragma unroll
+    for (int u = 0; u < CT * CT / 4 / TPB; u++) {
+        const int q = threadIdx.x + u * TPB;
+        const f32x4 v = __builtin_nontemporal_load(src + q);
+        const int r = (4 * q) / CT, c = (4 * q) % CT;
+        T[r][c] = v.x;
+        T[r][c + 1] = v.y;
+        T[r][c + 2] = v.z;
+        T[r][c + 3] = v.w;
+    }
+    if (threadIdx.x < CT) bi[threadIdx.x] = bd[I * CT + threadIdx.x];
+    else if (threadIdx.x < 2 * CT) bj[threadIdx.x - CT] = bd[J * CT + threadIdx.x - CT];
+    __syncthreads();
+    if (threadIdx.x < CT) {
+        const int r = threadIdx.x;
+        double s = 0.0;
+        for (int c = 0; c < CT; c++) s += (double)T[r][c] * bj[c];
+        part[((size_t)I * nct + J) * CT + r] = s;
+    } else if (threadIdx.x < 2 * CT && J != I) {
+        const int c = threadIdx.x - CT;
+        double s = 0.0;
+        for (int r = 0; r < CT; r++) s += (double)T[r][c] * bi[r];
+        part[((size_t)J * nct + I) * CT + c] = s;
+    }
+}
+
 template <typename XT>
-__global__ __launch_bounds__(TPB) void k_coarse_f32(int nd, int ld, const int* __restrict__ didx,
-                                                    const float* __restrict__ inv, int nt,
-                                                    const int* __restrict__ tidx, const double* __restrict__ tdiag,
-                                                    const XT* __restrict__ b, XT* __restrict__ x,
-                                                    const double* __restrict__ bd) {
-    const int wave = (blockIdx.x * TPB + threadIdx.x) / WAVE;
-    const int lane = threadIdx.x % WAVE;
-    const int nw = gridDim.x * TPB / WAVE;
-    const int q = ld / 4;
-    for (int row = wave; row < nd; row += nw) {
-        const f32x4* a = reinterpret_cast<const f32x4*>(inv + (size_t)row * ld);
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-#pragma unroll 4
-        for (int c = lane; c < q; c += WAVE) {
-            const f32x4 v = __builtin_nontemporal_load(a + c);
-            s0 += (double)v.x * bd[4 * c];
-            s1 += (double)v.y * bd[4 * c + 1];
-            s2 += (double)v.z * bd[4 * c + 2];
-            s3 += (double)v.w * bd[4 * c + 3];
+__global__ __launch_bounds__(TPB) void k_csym_reduce(int nd, int nct, const double* __restrict__ part,
+                                                     const int* __restrict__ didx, int nt, const int* __restrict__ tidx,
+                                                     const double* __restrict__ tdiag, const XT* __restrict__ b,
+                                                     XT* __restrict__ x) {
+    constexpr int Q = TPB / CT;  // lanes per dense row, each summing every Q-th tile column
+    __shared__ double acc[Q][CT];
+    if (blockIdx.x < nct) {
+        const int I = blockIdx.x, r = threadIdx.x % CT, q = threadIdx.x / CT;
+        double s = 0.0;
+        for (int K = q; K < nct; K += Q) s += part[((size_t)I * nct + K) * CT + r];
+        acc[q][r] = s;
+        __syncthreads();
+        const int row = I * CT + r;
+        if (q == 0 && row < nd) {
+            double y = acc[0][r];
+            for (int k = 1; k < Q; k++) y += acc[k][r];
+            x[didx[row]] = (XT)y;
         }
-        double s = wave_sum((s0 + s1) + (s2 + s3));
-        if (lane == 0) x[didx[row]] = (XT)s;
     }
     for (int t = blockIdx.x * TPB + threadIdx.x; t < nt; t += gridDim.x * TPB) {
         int i = tidx[t];
@@ -1013,11 +1101,17 @@ __global__ __launch_bounds__(TPB) void k_coarse_f32(int nd, int ld, const int* _
     }
 }
 
-__global__ void k_inv_f32(int nd, int ld, const double* __restrict__ inv, float* __restrict__ out) {
-    const size_t total = (size_t)nd * ld;
+// tiles of the upper triangle: entry (row, col) of tile t takes
+// inv[min(row, col)][max(row, col)], so that the FP32 copy is exactly symmetric
+__global__ void k_inv_tiles(int nd, int ntiles, const int* __restrict__ ti, const int* __restrict__ tj,
+                            const double* __restrict__ inv, float* __restrict__ tiles) {
+    const size_t total = (size_t)ntiles * CT * CT;
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (size_t)gridDim.x * blockDim.x) {
-        const int r = (int)(k / ld), c = (int)(k % ld);
-        out[k] = c < nd ? (float)inv[(size_t)r * nd + c] : 0.0f;
+        const int t = (int)(k / (CT * CT)), e = (int)(k % (CT * CT));
+        const int row = ti[t] * CT + e / CT, col = tj[t] * CT + e % CT;
+        float v = 0.0f;
+        if (row < nd && col < nd) v = (float)inv[(size_t)std::min(row, col) * nd + std::max(row, col)];
+        tiles[k] = v;
     }
 }
 
@@ -1142,11 +1236,23 @@ static inline int grid_for(int n, int cap = 4096) {
         default: { constexpr int L = 32; __VA_ARGS__; break; } \
     }
 
+// workgroups of a row-kernel launch: one per row block, or a persistent grid
+// of at most LINA_SPMV_GRID workgroups (rounded down to a multiple of 8; read
+// from the environment once, 0: off)
+static int row_grid(int nblk) {
+    static const int cap = [] {
+        const char* e = std::getenv("LINA_SPMV_GRID");
+        const int v = e ? std::atoi(e) : 0;
+        return v > 0 ? (v / 8) * 8 : 0;
+    }();
+    return (cap > 0 && nblk > cap) ? cap : nblk;
+}
+
 template <typename VT, typename XT>
 int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w,
          const Epi<VT, XT>* epi) {
     if (A.nblk == 0) return 0;
-    dim3 g(A.nblk), t(TPB);
+    dim3 g(row_grid(A.nblk)), t(TPB);
     const Epi<VT, XT> e = epi ? *epi : Epi<VT, XT>{};
 #define LINA_SPMV_MODE(M)                                                                                       \
     LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, M, L>), g, t, 0, st, A.nblk, A.bk, A.rp, A.ci, \
@@ -1186,25 +1292,28 @@ void finalize_stop(hipStream_t st, int np, const double* part, double* scal, dou
 }
 
 template <typename XT>
-int precon_in(hipStream_t st, int n, const double* F, double* part, double* X, XT* xf, XT* ff, const double* isa) {
+int precon_in(hipStream_t st, int n, const double* F, double* part, double* X, XT* xf, XT* ff, const uint32_t* isb,
+              const double* isa) {
     int g = grid_for(n, 1024);
-    hipLaunchKernelGGL((k_precon_in<XT>), dim3(g), dim3(TPB), 0, st, n, F, part, X, xf, ff, isa);
+    hipLaunchKernelGGL((k_precon_in<XT>), dim3(g), dim3(TPB), 0, st, n, F, part, X, xf, ff, isb, isa);
     return g;
 }
-template int precon_in<double>(hipStream_t, int, const double*, double*, double*, double*, double*, const double*);
-template int precon_in<float>(hipStream_t, int, const double*, double*, double*, float*, float*, const double*);
+template int precon_in<double>(hipStream_t, int, const double*, double*, double*, double*, double*, const uint32_t*,
+                               const double*);
+template int precon_in<float>(hipStream_t, int, const double*, double*, double*, float*, float*, const uint32_t*,
+                              const double*);
 
 template <typename XT>
-int precon_out(hipStream_t st, int n, const XT* xs, double* Z, const double* F, const double* isa,
-               const double* scal, int slot, const double* w, double* part) {
+int precon_out(hipStream_t st, int n, const XT* xs, double* Z, const double* F, const uint32_t* isb,
+               const double* isa, const double* scal, int slot, const double* w, double* part) {
     int g = grid_for(n, 1024);
-    hipLaunchKernelGGL((k_precon_out<XT>), dim3(g), dim3(TPB), 0, st, n, xs, Z, F, isa, scal, slot, w, part);
+    hipLaunchKernelGGL((k_precon_out<XT>), dim3(g), dim3(TPB), 0, st, n, xs, Z, F, isb, isa, scal, slot, w, part);
     return g;
 }
-template int precon_out<double>(hipStream_t, int, const double*, double*, const double*, const double*,
-                                const double*, int, const double*, double*);
-template int precon_out<float>(hipStream_t, int, const float*, double*, const double*, const double*,
-                               const double*, int, const double*, double*);
+template int precon_out<double>(hipStream_t, int, const double*, double*, const double*, const uint32_t*,
+                                const double*, const double*, int, const double*, double*);
+template int precon_out<float>(hipStream_t, int, const float*, double*, const double*, const uint32_t*,
+                               const double*, const double*, int, const double*, double*);
 
 template <typename T>
 void fill(hipStream_t st, int n, T* x, T val) {
@@ -1313,12 +1422,17 @@ void pcg_p(hipStream_t st, int n, double* P, const double* Z, const double* scal
     hipLaunchKernelGGL(k_pcg_p, dim3(grid_for(n)), dim3(TPB), 0, st, n, P, Z, scal, first);
 }
 
+template <typename XT>
 int pcg_xr(hipStream_t st, int n, double* X, double* R, const double* P, const double* Q, const double* scal,
-           double* part) {
+           double* part, XT* ff, XT* xf, const double* sg, const uint32_t* isb, const double* isa) {
     int g = grid_for(n, 1024);
-    hipLaunchKernelGGL(k_pcg_xr, dim3(g), dim3(TPB), 0, st, n, X, R, P, Q, scal, part);
+    hipLaunchKernelGGL((k_pcg_xr<XT>), dim3(g), dim3(TPB), 0, st, n, X, R, P, Q, scal, part, ff, xf, sg, isb, isa);
     return g;
 }
+template int pcg_xr<double>(hipStream_t, int, double*, double*, const double*, const double*, const double*, double*,
+                            double*, double*, const double*, const uint32_t*, const double*);
+template int pcg_xr<float>(hipStream_t, int, double*, double*, const double*, const double*, const double*, double*,
+                           float*, float*, const double*, const uint32_t*, const double*);
 
 void axpy_s(hipStream_t st, int n, double* x, const double* y, const double* scal, int slot_a, int slot_b,
             double sign) {
@@ -1403,27 +1517,27 @@ template void coarse_solve<float>(hipStream_t, int, const int*, const double*, i
                                   const float*, float*, double*, bool);
 
 template <typename XT>
-void coarse_solve_f32(hipStream_t st, int nd, int ld, const int* didx, const float* inv, int nt, const int* tidx,
-                      const double* tdiag, const XT* b, XT* x, double* bd, bool gathered) {
+void coarse_solve_sym(hipStream_t st, int nd, int nct, int ntiles, const int* ti, const int* tj, const float* tiles,
+                      double* part, const int* didx, int nt, const int* tidx, const double* tdiag, const XT* b, XT* x,
+                      double* bd, bool gathered) {
     if (nd > 0 && !gathered)
         hipLaunchKernelGGL((k_coarse_gather<XT>), dim3(std::min((nd + TPB - 1) / TPB, 1024)), dim3(TPB), 0, st, nd,
                            didx, b, bd);
-    int rows_blocks = (nd * WAVE + TPB - 1) / TPB;
-    int tb = (nt + TPB - 1) / TPB;
-    int g = rows_blocks > tb ? rows_blocks : tb;
-    if (g < 1) g = 1;
-    if (g > 2048) g = 2048;
-    hipLaunchKernelGGL((k_coarse_f32<XT>), dim3(g), dim3(TPB), 0, st, nd, ld, didx, inv, nt, tidx, tdiag, b, x,
-                       (const double*)bd);
+    if (ntiles > 0) hipLaunchKernelGGL(k_csym_tiles, dim3(ntiles), dim3(TPB), 0, st, nct, ti, tj, tiles, bd, part);
+    const int g = std::max(std::max(nct, (nt + TPB - 1) / TPB), 1);
+    hipLaunchKernelGGL((k_csym_reduce<XT>), dim3(g), dim3(TPB), 0, st, nd, nct, part, didx, nt, tidx, tdiag, b, x);
 }
-template void coarse_solve_f32<double>(hipStream_t, int, int, const int*, const float*, int, const int*,
-                                       const double*, const double*, double*, double*, bool);
-template void coarse_solve_f32<float>(hipStream_t, int, int, const int*, const float*, int, const int*,
-                                      const double*, const float*, float*, double*, bool);
+template void coarse_solve_sym<double>(hipStream_t, int, int, int, const int*, const int*, const float*, double*,
+                                       const int*, int, const int*, const double*, const double*, double*, double*,
+                                       bool);
+template void coarse_solve_sym<float>(hipStream_t, int, int, int, const int*, const int*, const float*, double*,
+                                      const int*, int, const int*, const double*, const float*, float*, double*,
+                                      bool);
 
-void inverse_to_f32(hipStream_t st, int nd, int ld, const double* inv, float* out) {
-    if (nd <= 0) return;
-    hipLaunchKernelGGL(k_inv_f32, dim3(4096), dim3(TPB), 0, st, nd, ld, inv, out);
+void inverse_to_tiles(hipStream_t st, int nd, int ntiles, const int* ti, const int* tj, const double* inv,
+                      float* tiles) {
+    if (ntiles <= 0) return;
+    hipLaunchKernelGGL(k_inv_tiles, dim3(4096), dim3(TPB), 0, st, nd, ntiles, ti, tj, inv, tiles);
 }
 
 void trsv(hipStream_t st, const int* rp, const int* ci, const double* v, int nlv, const int* lv_off,

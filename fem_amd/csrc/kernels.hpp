@@ -112,22 +112,24 @@ int dot_part(hipStream_t st, int n, const XT* a, const XT* b, double* part);
 // scal[slot] = sum(part); op 0 plain, 1 sqrt, 2 shift (scal[slot+1] = scal[slot]) then plain
 void finalize(hipStream_t st, int np, const double* part, double* scal, int slot, int op);
 // pcg's residual norm and stop test: unless scal[S_STOP] is already set,
-// scal[S_R] = sqrt(sum(part)), scal[S_IT] += 1 and scal[S_STOP] = 1 when
+// scal[S_R] = sqrt(sum(part)), scal[S_AMG0] = sum(part), scal[S_IT] += 1 and
+// scal[S_STOP] = 1 when
 // !((relres ? r : r / scal[S_R0]) > tol), r is NaN or S_IT reached max_iter
 void finalize_stop(hipStream_t st, int np, const double* part, double* scal, double tol, int relres, int max_iter);
 // Zero-guess preconditioner entry: block partials of F.F (k_dot_part's grid
 // and order; returns their number); X (if given) = the initial guess, 0 and
-// F_i / isa_i where isa_i != 0 (isa: the isolated rows' diagonals, nullptr:
-// none); xf (if given) = (XT) X; ff (if given) = (XT) F.
+// F_i / isa_i on the isolated rows (bit i of isb set; isb, isa: nullptr
+// without isolated rows); xf (if given) = (XT) X; ff (if given) = (XT) F.
 template <typename XT>
-int precon_in(hipStream_t st, int n, const double* F, double* part, double* X, XT* xf, XT* ff, const double* isa);
+int precon_in(hipStream_t st, int n, const double* F, double* part, double* X, XT* xf, XT* ff,
+              const uint32_t* isb, const double* isa);
 // Zero-guess preconditioner exit: Z = the initial guess of precon_in when
 // scal[slot] == 0, else the V-cycle's result (xs, or Z itself when xs is
 // nullptr); w != nullptr: block partials of w.Z (k_dot_part's grid and order),
 // returns their number
 template <typename XT>
-int precon_out(hipStream_t st, int n, const XT* xs, double* Z, const double* F, const double* isa,
-               const double* scal, int slot, const double* w, double* part);
+int precon_out(hipStream_t st, int n, const XT* xs, double* Z, const double* F, const uint32_t* isb,
+               const double* isa, const double* scal, int slot, const double* w, double* part);
 template <typename T>
 void fill(hipStream_t st, int n, T* x, T val);
 template <typename TI, typename TO>
@@ -167,9 +169,12 @@ void scale(hipStream_t st, int n, double* x, double a);       // x_i *= a
 void restore_if_zero(hipStream_t st, int n, double* x, const double* scal, int slot, int m, const int* rows,
                      const double* a, const double* f);
 void pcg_p(hipStream_t st, int n, double* P, const double* Z, const double* scal, int first);
-// (skips every update once scal[S_STOP] is set)
+// (skips every update once scal[S_STOP] is set).  ff, xf: also the next
+// zero-guess preconditioner entry on the new R, see k_pcg_xr
+template <typename XT = double>
 int pcg_xr(hipStream_t st, int n, double* X, double* R, const double* P, const double* Q, const double* scal,
-           double* part);
+           double* part, XT* ff = nullptr, XT* xf = nullptr, const double* sg = nullptr, const uint32_t* isb = nullptr,
+           const double* isa = nullptr);
 void axpy_s(hipStream_t st, int n, double* x, const double* y, const double* scal, int slot_a, int slot_b,
             double sign);
 void axpby(hipStream_t st, int n, double a, double* x, double b, const double* y);
@@ -205,13 +210,18 @@ void combine(hipStream_t st, int n, int m, const double* V, size_t ld, const dou
 template <typename XT>
 void coarse_solve(hipStream_t st, int nd, const int* didx, const double* inv, int nt, const int* tidx,
                   const double* tdiag, const XT* b, XT* x, double* bd, bool gathered = false);
-// the same with an FP32 copy of the inverse, rows padded to ld (a multiple of
-// 4) with zeros; bd padded likewise; FP64 accumulation
+// the same with the symmetric inverse's upper triangle as FP32 tiles of CT x
+// CT (tile t = (ti[t], tj[t]), ti[t] <= tj[t], nct tiles a side, zero
+// padded), FP64 products and sums; part: nct * nct * CT doubles of scratch,
+// bd: nct * CT doubles, zero beyond nd
+constexpr int CT = 64;
 template <typename XT>
-void coarse_solve_f32(hipStream_t st, int nd, int ld, const int* didx, const float* inv, int nt, const int* tidx,
-                      const double* tdiag, const XT* b, XT* x, double* bd, bool gathered);
-// out (nd x ld, zero padded) = (float) inv (nd x nd)
-void inverse_to_f32(hipStream_t st, int nd, int ld, const double* inv, float* out);
+void coarse_solve_sym(hipStream_t st, int nd, int nct, int ntiles, const int* ti, const int* tj, const float* tiles,
+                      double* part, const int* didx, int nt, const int* tidx, const double* tdiag, const XT* b, XT* x,
+                      double* bd, bool gathered);
+// tiles = the upper triangle of inv (nd x nd, FP64) in FP32, symmetrised
+void inverse_to_tiles(hipStream_t st, int nd, int ntiles, const int* ti, const int* tj, const double* inv,
+                      float* tiles);
 
 }  // namespace dev
 }  // namespace lina

@@ -74,16 +74,18 @@ DevBuffers::~DevBuffers() {
 }
 
 // row-block schedule over rows [begin, end): at most max_nnz products and
-// BLOCK_ROWS rows per block; a row longer than max_nnz gets a block alone.
+// max_rows rows per block; a row longer than max_nnz gets a block alone.
+// (Blocks of up to 1024 rows for the prolongations, a few entries a row,
+// measured no faster.)
 // Appends the blocks' first rows to starts.
-void build_blocks(const Csr& A, int begin, int end, int max_nnz, std::vector<int>& starts) {
+void build_blocks(const Csr& A, int begin, int end, int max_nnz, std::vector<int>& starts, int max_rows) {
     int r = begin;
     int nnz = 0, rows = 0;
     if (begin >= end) return;
     starts.push_back(begin);
     while (r < end) {
         int len = A.rp[r + 1] - A.rp[r];
-        if (rows > 0 && (nnz + len > max_nnz || rows == dev::BLOCK_ROWS)) {
+        if (rows > 0 && (nnz + len > max_nnz || rows == max_rows)) {
             starts.push_back(r);
             nnz = 0;
             rows = 0;
@@ -514,14 +516,21 @@ void Solver::upload() {
     // the isolated rows' diagonals by row for the fused preconditioner entry
     // and exit (a zero diagonal keeps the separate kernels)
     isa_ = nullptr;
+    isb_ = nullptr;
     isa_ok_ = true;
     if (niso_ > 0) {
         std::vector<double> isa(n_, 0.0);
+        std::vector<uint32_t> isb((n_ + 31) / 32, 0u);
         for (int k = 0; k < niso_; k++) {
+            const int i = H_.iso_rows[k];
             if (H_.iso_a[k] == 0.0) isa_ok_ = false;
-            isa[H_.iso_rows[k]] = H_.iso_a[k];
+            isa[i] = H_.iso_a[k];
+            isb[i >> 5] |= 1u << (i & 31);
         }
-        if (isa_ok_) isa_ = dupload(isa, B, st);
+        if (isa_ok_) {
+            isa_ = dupload(isa, B, st);
+            isb_ = dupload(isb, B, st);
+        }
     }
     // coarse LU inverse
     const CoarseLU& lu = H_.lu;
@@ -548,9 +557,10 @@ void Solver::upload() {
         double* piv = dalloc<double>(1, B);
         dev::gauss_jordan_inverse(st, nd_, inv_, piv);
     }
-    // gathered right-hand side, padded with zeros for the FP32 inverse's rows
-    cbd_ = dalloc<double>(std::max((nd_ + 3) / 4 * 4, 4), B);
-    dev::fill<double>(st, std::max((nd_ + 3) / 4 * 4, 4), cbd_, 0.0);
+    // gathered right-hand side, padded with zeros to whole tiles of the FP32 inverse
+    const int nbd = std::max((nd_ + dev::CT - 1) / dev::CT * dev::CT, dev::CT);
+    cbd_ = dalloc<double>(nbd, B);
+    dev::fill<double>(st, nbd, cbd_, 0.0);
     // dense index of each coarsest row (restriction epilogue) and whether the
     // dense and trivial rows cover the level (no zero fill needed)
     dinv_ = nullptr;
@@ -574,7 +584,7 @@ void Solver::upload() {
         }
     }
     inv32_ = nullptr;
-    ld32_ = 0;
+    nct_ = ntiles_ = 0;
     tidx_ = dupload(lu.trivial_idx, B, st);
     tdiag_ = dupload(lu.trivial_diag, B, st);
     // Krylov vectors and scratch
@@ -604,13 +614,24 @@ void Solver::set_params(const lina_amg_params& p) {
     ensure_inv32();
 }
 
-// FP32 copy of the coarse inverse for mixed-precision V-cycles (made when the
-// hierarchy is uploaded or the precision changes, never inside a solve)
+// FP32 copy of the coarse inverse for mixed-precision V-cycles: its upper
+// triangle in CT x CT tiles (made when the hierarchy is uploaded or the
+// precision changes, never inside a solve)
 void Solver::ensure_inv32() {
     if (inv32_ || cbatched_ || nd_ == 0 || !uploaded() || prm_.precision != LINA_PREC_MIXED) return;
-    ld32_ = (nd_ + 3) / 4 * 4;
-    inv32_ = dalloc<float>((size_t)nd_ * ld32_, *bufs_);
-    dev::inverse_to_f32(stream_, nd_, ld32_, inv_, inv32_);
+    nct_ = (nd_ + dev::CT - 1) / dev::CT;
+    std::vector<int> ti, tj;
+    for (int I = 0; I < nct_; I++)
+        for (int J = I; J < nct_; J++) {
+            ti.push_back(I);
+            tj.push_back(J);
+        }
+    ntiles_ = (int)ti.size();
+    cti_ = dupload(ti, *bufs_, stream_);
+    ctj_ = dupload(tj, *bufs_, stream_);
+    inv32_ = dalloc<float>((size_t)ntiles_ * dev::CT * dev::CT, *bufs_);
+    cpart_ = dalloc<double>((size_t)nct_ * nct_ * dev::CT, *bufs_);
+    dev::inverse_to_tiles(stream_, nd_, ntiles_, cti_, ctj_, inv_, inv32_);
     hip_check(hipStreamSynchronize(stream_), "sync");
 }
 
@@ -842,7 +863,8 @@ void Solver::vcycle(int l, const T* f, T* x, int xinit) {
         if (cbatched_)
             dev::bcoarse_solve(st, nd_, crblk_, coff_, cmoff_, didx_, inv_, nt_, tidx_, tdiag_, f, x);
         else if (std::is_same<T, float>::value && inv32_)
-            dev::coarse_solve_f32(st, nd_, ld32_, didx_, inv32_, nt_, tidx_, tdiag_, f, x, cbd_, fused_gather());
+            dev::coarse_solve_sym(st, nd_, nct_, ntiles_, cti_, ctj_, inv32_, cpart_, didx_, nt_, tidx_, tdiag_, f, x,
+                                  cbd_, fused_gather());
         else
             dev::coarse_solve(st, nd_, didx_, inv_, nt_, tidx_, tdiag_, f, x, cbd_, fused_gather());
         return;
@@ -955,14 +977,17 @@ int Solver::amg_solve_dev(const double* F, double* X) {
 // result is replaced by the untouched guess on the device when r0 == 0.
 // w != nullptr: also the block partials of w.X into part_ (their number in
 // *np), the dot k_dot_part(w, X) would give.
-int Solver::vcycle_once(const double* F, double* X, const double* w, int* np) {
+// prepared: pcg_xr already wrote this entry's outputs for F (the signed
+// right-hand side, its FP32 copy, the initial guess, scal[S_AMG0]); see
+// entry_prepared.
+int Solver::vcycle_once(const double* F, double* X, const double* w, int* np, bool prepared) {
     hipStream_t st = stream_;
     DevLevel& top = L_.back();
     const int n = top.n;
     const int lt = (int)L_.size() - 1;
     const double* Ft = F;
     if (has_neg_signs_) {
-        dev::signs(st, n, signs_, F, ftop_);
+        if (!prepared) dev::signs(st, n, signs_, F, ftop_);
         Ft = ftop_;
     }
     // without isolated rows the guess is exactly zero and the pre-smoother
@@ -971,19 +996,23 @@ int Solver::vcycle_once(const double* F, double* X, const double* w, int* np) {
     const bool fp64 = prm_.precision == LINA_PREC_FP64;
     int npd = 0;
     if (isa_ok_) {
-        // entry and exit fused into one elementwise kernel each
-        int n0;
-        if (fp64) n0 = dev::precon_in<double>(st, n, Ft, part_, xz ? nullptr : X, nullptr, nullptr, isa_);
-        else n0 = dev::precon_in<float>(st, n, Ft, part_, nullptr, xz ? nullptr : top.xf, top.ff, isa_);
-        dev::finalize(st, n0, part_, scal_, dev::S_AMG0, 0);
+        // entry and exit fused into one elementwise kernel each (the entry
+        // into pcg_xr when prepared)
+        if (!prepared) {
+            int n0;
+            if (fp64) n0 = dev::precon_in<double>(st, n, Ft, part_, xz ? nullptr : X, nullptr, nullptr, isb_, isa_);
+            else n0 = dev::precon_in<float>(st, n, Ft, part_, nullptr, xz ? nullptr : top.xf, top.ff, isb_, isa_);
+            dev::finalize(st, n0, part_, scal_, dev::S_AMG0, 0);
+        }
         if (fp64) {
             vcycle<double>(lt, Ft, X, xz ? X_ZERO : X_GUESS);
-            npd = dev::precon_out<double>(st, n, nullptr, X, Ft, isa_, scal_, dev::S_AMG0, w, part_);
+            npd = dev::precon_out<double>(st, n, nullptr, X, Ft, isb_, isa_, scal_, dev::S_AMG0, w, part_);
         } else {
             vcycle<float>(lt, top.ff, top.xf, xz ? X_ZERO : X_GUESS);
-            npd = dev::precon_out<float>(st, n, top.xf, X, Ft, isa_, scal_, dev::S_AMG0, w, part_);
+            npd = dev::precon_out<float>(st, n, top.xf, X, Ft, isb_, isa_, scal_, dev::S_AMG0, w, part_);
         }
     } else {
+        if (prepared) throw std::logic_error("vcycle_once: prepared entry without the fused path");
         int n0 = dev::dot_part<double>(st, n, Ft, Ft, part_);
         dev::finalize(st, n0, part_, scal_, dev::S_AMG0, 0);
         if (!xz) {
@@ -1011,8 +1040,14 @@ int Solver::vcycle_once(const double* F, double* X, const double* w, int* np) {
 // be queued ahead of the stop test)
 bool Solver::precon_async() const { return prm_.zero_init && prm_.amg_max_iter == 1; }
 
-int Solver::precon(const double* R, double* Z, const double* w, int* np) {
-    if (precon_async()) return vcycle_once(R, Z, w, np);
+// pcg_xr can write the next preconditioner entry (vcycle_once's fused path;
+// not with make_positive_diag signs, whose signed right-hand side precon_out
+// would also need)
+bool Solver::entry_prepared_ok() const { return precon_async() && isa_ok_ && !has_neg_signs_; }
+
+int Solver::precon(const double* R, double* Z, const double* w, int* np, bool prepared) {
+    if (precon_async()) return vcycle_once(R, Z, w, np, prepared);
+    if (prepared) throw std::logic_error("precon: prepared entry without a one-V-cycle preconditioner");
     if (prm_.zero_init) dev::fill<double>(stream_, n_, Z, 0.0);
     int it = amg_solve_dev(R, Z);
     if (w) *np = dev::dot_part<double>(stream_, n_, w, Z, part_);
@@ -1073,9 +1108,12 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
     // kernel timing samples every fourth iteration's outer SpMV (the events
     // themselves cost a few microseconds)
     auto sampled = [&](int k) { return ktime_ && k % 4 == 0; };
+    // from the second iteration on, pcg_xr writes the preconditioner's entry
+    // on the new R (its FP32 copy, initial guess and r0 test)
+    const bool prep = entry_prepared_ok();
     auto enqueue = [&](int k) {
         int m = 0;
-        precon(R, Z, R, &m);
+        precon(R, Z, R, &m, prep && k > 0);
         dev::finalize(st, m, part_, scal_, dev::S_S, 2);  // s_old = s; s = R.Z
         dev::pcg_p(st, n, P, Z, scal_, k == 0);
         if (op_) {
@@ -1089,7 +1127,15 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
             if (sampled(k)) hip_check(hipEventRecord(evk_[k % 2][1], st), "event");
         }
         dev::finalize(st, m, part_, scal_, dev::S_PQ, 0);
-        m = dev::pcg_xr(st, n, X, R, P, Q, scal_, part_);
+        if (!prep) {
+            m = dev::pcg_xr(st, n, X, R, P, Q, scal_, part_);
+        } else if (prm_.precision == LINA_PREC_FP64) {
+            m = dev::pcg_xr<double>(st, n, X, R, P, Q, scal_, part_, nullptr, niso_ == 0 ? nullptr : Z, nullptr,
+                                    isb_, isa_);
+        } else {
+            m = dev::pcg_xr<float>(st, n, X, R, P, Q, scal_, part_, top.ff, niso_ == 0 ? nullptr : top.xf, nullptr,
+                                   isb_, isa_);
+        }
         dev::finalize_stop(st, m, part_, scal_, tol, relres, cap);
         double* h = host_ring_ + (k % 2) * dev::S_COUNT;
         hip_check(hipMemcpyAsync(h + dev::S_R, scal_ + dev::S_R, (dev::S_IT - dev::S_R + 1) * sizeof(double),

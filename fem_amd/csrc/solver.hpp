@@ -49,7 +49,8 @@ struct DevLevel {
 };
 
 int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st);
-void build_blocks(const Csr& A, int begin, int end, int max_nnz, std::vector<int>& starts);
+void build_blocks(const Csr& A, int begin, int end, int max_nnz, std::vector<int>& starts,
+                  int max_rows = dev::BLOCK_ROWS);
 std::vector<int2> block_pairs(const Csr& A, const std::vector<int>& starts_with_end);
 // algorithmic bytes of one SpMV with 32-bit indices (before upload): 12 nnz + 4 (n+1) + 16 n (FP64),
 // 8 nnz + 4 (n+1) + 8 n (FP32)
@@ -119,9 +120,10 @@ class Solver {
     int amg_solve_dev(const double* F, double* X);
     int amg_solve_host(const double* f, double* x);
     // Z = M^-1 R; w != nullptr: block partials of w.Z in part(), their number in *np
-    int precon(const double* R, double* Z, const double* w = nullptr, int* np = nullptr);
-    int vcycle_once(const double* F, double* X, const double* w = nullptr, int* np = nullptr);
+    int precon(const double* R, double* Z, const double* w = nullptr, int* np = nullptr, bool prepared = false);
+    int vcycle_once(const double* F, double* X, const double* w = nullptr, int* np = nullptr, bool prepared = false);
     bool precon_async() const;
+    bool entry_prepared_ok() const;
     int spmv_host(int level, int precision, const double* x, double* y);
     double time_spmv(int level, int precision, int reps);
     double spmv_bytes(int level, int precision) const;
@@ -187,9 +189,12 @@ class Solver {
     double* cbd_ = nullptr;
     int* dinv_ = nullptr;     // dense index of each coarsest-level row (-1: not dense)
     bool ccover_ = false;     // dense and trivial rows cover the coarsest level
-    float* inv32_ = nullptr;  // FP32 copy of the coarse inverse (mixed precision), rows padded to ld32_
-    int ld32_ = 0;
+    float* inv32_ = nullptr;  // FP32 upper-triangle tiles of the coarse inverse (mixed precision)
+    int nct_ = 0, ntiles_ = 0;  // tiles a side, stored tiles
+    int *cti_ = nullptr, *ctj_ = nullptr;  // tile coordinates
+    double* cpart_ = nullptr;   // nct_ x nct_ x CT tile partials
     double* isa_ = nullptr;   // isolated rows' diagonals by row (0 elsewhere); nullptr without isolated rows
+    uint32_t* isb_ = nullptr; // bit i: row i is isolated (read instead of isa_ on every row)
     bool isa_ok_ = true;      // no isolated row has a zero diagonal (fused preconditioner entry/exit)
     int *crblk_ = nullptr, *coff_ = nullptr;
     size_t* cmoff_ = nullptr;
