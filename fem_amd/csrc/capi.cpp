@@ -516,12 +516,23 @@ void lina_ilu_destroy(lina_ilu* h) { delete h; }
 
 int lina_ilu_ok(const lina_ilu* h) { return h->s.ilu_ok() ? 1 : 0; }
 
+int lina_ilu_refactor(lina_ilu* h, int n, const int* rowptr, const int* col, const double* val) {
+    return guard([&]() {
+        if (!h) throw std::invalid_argument("null handle");
+        Csr A = make_csr(n, rowptr, col, val);
+        h->s.refactor(A);
+        return 0;
+    });
+}
+
 int lina_ilu_factor_nnz(const lina_ilu* h, char which) {
-    return which == 'L' ? h->s.L().nnz() : h->s.U().nnz();
+    lina::IluSystem& s = const_cast<lina_ilu*>(h)->s;  // downloading device factors changes no result
+    return which == 'L' ? s.L().nnz() : s.U().nnz();
 }
 
 int lina_ilu_factor_copy(const lina_ilu* h, char which, int* rowptr, int* col, double* val) {
-    const Csr& M = which == 'L' ? h->s.L() : h->s.U();
+    lina::IluSystem& s = const_cast<lina_ilu*>(h)->s;
+    const Csr& M = which == 'L' ? s.L() : s.U();
     std::memcpy(rowptr, M.rp.data(), (M.n + 1) * sizeof(int));
     std::memcpy(col, M.ci.data(), M.nnz() * sizeof(int));
     std::memcpy(val, M.v.data(), M.nnz() * sizeof(double));

@@ -18,13 +18,20 @@ class IluSystem {
     ~IluSystem();
     // incomplete_LU_factorization on the host (bit-identical factors), then upload
     void build(const Csr& A, bool upload = true);
+    // the same factorisation on the device (bit-identical factors) for a
+    // matrix whose rows are sorted by column: the symbolic structure (L/U
+    // patterns, level schedules, device buffers) is kept while A's pattern
+    // stays the same, so a refactorisation uploads A's values and runs the
+    // numeric kernel only.  Unsorted rows fall back to build().
+    void refactor(const Csr& A);
     // PCG_default(A, L, U) with factors computed elsewhere (L: diagonal last
     // in each row, U: diagonal first); only the preconditioner is available
     void set_factors(const Csr& L, const Csr& U, bool upload = true);
-    void upload();
+    void upload(bool pattern_levels = false);
     bool ilu_ok() const { return ilu_ok_ != 0; }
-    const Csr& L() const { return L_; }
-    const Csr& U() const { return U_; }
+    // the factors (downloaded first when they were computed on the device)
+    const Csr& L() { sync_factors(); return L_; }
+    const Csr& U() { sync_factors(); return U_; }
     int n() const { return n_; }
 
     void apply(const double* z, double* w);  // w = U^-1 L^-1 z (device)
@@ -40,8 +47,12 @@ class IluSystem {
     int levels_u() const { return nlv_u_; }
 
   private:
+    void sync_factors();
     Csr A_, L_, U_;
     int ilu_ok_ = 0;
+    bool device_ready_ = false;    // refactor's symbolic structure is uploaded for A_'s pattern
+    bool host_factors_ = true;     // L_, U_ hold the current values
+    int* fail_ = nullptr;          // zero-pivot flag of the device factorisation
     int n_ = 0;
     hipStream_t stream_ = nullptr;
     double* host_ = nullptr;

@@ -762,6 +762,73 @@ __global__ __launch_bounds__(1024) void k_trsv(const int* __restrict__ rp, const
     }
 }
 
+// ILU(0) numeric factorisation (incomplete_LU_factorization,
+// linear_algebra.c:1952-1982, with row_times_row :1795-1819) of A with rows
+// sorted by column, into the fixed patterns L (row i: the lower columns of A's
+// row i, then i) and U (row i: i, then the upper columns).  Row i (one
+// thread; the rows of a level in parallel, levels in order) walks its A
+// entries in column order:
+//   s = sum, k ascending over the L entries (i, k) computed so far (k < j,
+//       and k < i), of L(i,k) U(k,j) where U has (k, j)
+//   j < i: L(i,j) = a_ij - s;  j == i: L(i,i) = 0 + (a_ii - s)
+//   j > i: U(i,j) = (a_ij - s) / L(i,i)   (a zero pivot sets *fail)
+// U(i,i) = 1.  The same operations in the same order as the host ilu0, so
+// the factors are bit-identical.
+__device__ __forceinline__ double ilu_dot(int q0, int q1, const int* __restrict__ ciL, const double* vL,
+                                          const int* __restrict__ rpU, const int* __restrict__ ciU,
+                                          const double* vU, int j) {
+    double s = 0.0;
+    for (int q = q0; q < q1; q++) {
+        const int k = ciL[q];
+        int lo = rpU[k] + 1, hi = rpU[k + 1] - 1;  // U row k: diagonal first, then ascending
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            const int c = ciU[mid];
+            if (c == j) {
+                s += vL[q] * vU[mid];
+                break;
+            }
+            if (c < j) lo = mid + 1;
+            else hi = mid - 1;
+        }
+    }
+    return s;
+}
+
+__global__ __launch_bounds__(1024) void k_ilu0(int nlv, const int* __restrict__ lv_off,
+                                               const int* __restrict__ lv_rows, const int* __restrict__ rpA,
+                                               const int* __restrict__ ciA, const double* __restrict__ vA,
+                                               const int* __restrict__ rpL, const int* __restrict__ ciL, double* vL,
+                                               const int* __restrict__ rpU, const int* __restrict__ ciU, double* vU,
+                                               int* __restrict__ fail) {
+    for (int L = 0; L < nlv; L++) {
+        const int e = lv_off[L + 1];
+        for (int t = lv_off[L] + threadIdx.x; t < e; t += blockDim.x) {
+            const int i = lv_rows[t];
+            const int l0 = rpL[i], ld = rpL[i + 1] - 1;
+            int pl = l0, pu = rpU[i] + 1;
+            vU[rpU[i]] = 1.0;
+            vL[ld] = 0.0;
+            for (int p = rpA[i]; p < rpA[i + 1]; p++) {
+                const int j = ciA[p];
+                const double a = vA[p];
+                const double s = ilu_dot(l0, pl, ciL, vL, rpU, ciU, vU, j);
+                if (j < i) {
+                    vL[pl++] = a - s;
+                } else if (j == i) {
+                    vL[ld] = 0.0 + (a - s);
+                } else {
+                    const double d = vL[ld];
+                    if (d != 0.0) vU[pu] = (a - s) / d;
+                    else *fail = 1;
+                    pu++;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // h_l = V_l . r for l < m (classical Gram-Schmidt dots), block partials
 // part[l * nblk + block]
 __global__ __launch_bounds__(TPB) void k_multidot(int n, int m, const double* __restrict__ V, size_t ld,
@@ -1543,6 +1610,13 @@ void inverse_to_tiles(hipStream_t st, int nd, int ntiles, const int* ti, const i
 void trsv(hipStream_t st, const int* rp, const int* ci, const double* v, int nlv, const int* lv_off,
           const int* lv_rows, const double* z, double* w, int upper) {
     hipLaunchKernelGGL(k_trsv, dim3(1), dim3(1024), 0, st, rp, ci, v, nlv, lv_off, lv_rows, z, w, upper);
+}
+
+void ilu0_numeric(hipStream_t st, int nlv, const int* lv_off, const int* lv_rows, const int* rpA, const int* ciA,
+                  const double* vA, const int* rpL, const int* ciL, double* vL, const int* rpU, const int* ciU,
+                  double* vU, int* fail) {
+    hipLaunchKernelGGL(k_ilu0, dim3(1), dim3(1024), 0, st, nlv, lv_off, lv_rows, rpA, ciA, vA, rpL, ciL, vL, rpU, ciU,
+                       vU, fail);
 }
 
 int multidot(hipStream_t st, int n, int m, const double* V, size_t ld, const double* r, double* part, double* out) {

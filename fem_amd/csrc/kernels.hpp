@@ -200,6 +200,11 @@ void bcoarse_solve(hipStream_t st, int nd, const int* rblk, const int* off, cons
 // level-scheduled ILU(0) triangular solve (upper=0: L, diagonal last; 1: U, diagonal first)
 void trsv(hipStream_t st, const int* rp, const int* ci, const double* v, int nlv, const int* lv_off,
           const int* lv_rows, const double* z, double* w, int upper);
+// level-scheduled ILU(0) numeric factorisation into fixed L/U patterns (see
+// k_ilu0); A's rows sorted by column; *fail = 1 on a zero pivot
+void ilu0_numeric(hipStream_t st, int nlv, const int* lv_off, const int* lv_rows, const int* rpA, const int* ciA,
+                  const double* vA, const int* rpL, const int* ciL, double* vL, const int* rpU, const int* ciU,
+                  double* vU, int* fail);
 // out[l] = V_l . r, l < m (V column vectors with leading dimension ld); returns #partials per dot
 int multidot(hipStream_t st, int n, int m, const double* V, size_t ld, const double* r, double* part, double* out);
 // y = base + sign * sum_l V_l c_l

@@ -42,7 +42,11 @@ static T* kupload(const std::vector<T>& h, DevBuffers& owner, hipStream_t st) {
 }
 
 // dependency levels of a triangular sweep over nonzero off-diagonal values
-static void tri_levels(const Csr& T, bool upper, std::vector<int>& off, std::vector<int>& rows) {
+// level schedule of a triangular factor: level(i) = 1 + max level(j) over
+// its off-diagonal entries (entries holding 0.0 skipped unless
+// pattern_only, for factors whose values are not known yet)
+static void tri_levels(const Csr& T, bool upper, std::vector<int>& off, std::vector<int>& rows,
+                       bool pattern_only = false) {
     const int n = T.n;
     std::vector<int> lev(n, 0);
     int nl = 0;
@@ -51,7 +55,7 @@ static void tri_levels(const Csr& T, bool upper, std::vector<int>& off, std::vec
         int l = 0;
         for (int p = T.rp[i]; p < T.rp[i + 1]; p++) {
             int j = T.ci[p];
-            if (T.v[p] == 0.0 || j == i) continue;
+            if ((!pattern_only && T.v[p] == 0.0) || j == i) continue;
             if ((!upper && j < i) || (upper && j > i)) l = std::max(l, lev[j] + 1);
         }
         lev[i] = l;
@@ -97,6 +101,8 @@ IluSystem::~IluSystem() {
 
 void IluSystem::build(const Csr& A, bool upload) {
     A_ = A;
+    device_ready_ = false;
+    host_factors_ = true;
     Csr L, U;
     ilu_ok_ = ilu0(A, L, U);
     L_ = std::move(L);
@@ -105,9 +111,81 @@ void IluSystem::build(const Csr& A, bool upload) {
     if (upload) this->upload();
 }
 
+// the patterns ilu0 produces for A (rows strictly ascending): L row i = the
+// lower columns then i, U row i = i then the upper columns; false when a row
+// is not sorted
+static bool ilu_patterns(const Csr& A, Csr& L, Csr& U) {
+    const int n = A.n;
+    L = Csr();
+    U = Csr();
+    L.n = L.ncols = U.n = U.ncols = n;
+    L.rp.assign(n + 1, 0);
+    U.rp.assign(n + 1, 0);
+    for (int i = 0; i < n; i++) {
+        for (int p = A.rp[i]; p < A.rp[i + 1]; p++) {
+            const int j = A.ci[p];
+            if (p > A.rp[i] && j <= A.ci[p - 1]) return false;
+            if (j < i) L.ci.push_back(j);
+        }
+        L.ci.push_back(i);
+        L.rp[i + 1] = (int)L.ci.size();
+        U.ci.push_back(i);
+        for (int p = A.rp[i]; p < A.rp[i + 1]; p++)
+            if (A.ci[p] > i) U.ci.push_back(A.ci[p]);
+        U.rp[i + 1] = (int)U.ci.size();
+    }
+    L.v.assign(L.ci.size(), 0.0);
+    U.v.assign(U.ci.size(), 0.0);
+    return true;
+}
+
+void IluSystem::refactor(const Csr& A) {
+    const bool same = device_ready_ && A.n == A_.n && A.rp == A_.rp && A.ci == A_.ci;
+    if (!same) {
+        Csr L, U;
+        if (!ilu_patterns(A, L, U)) {
+            device_ready_ = false;
+            build(A, true);
+            return;
+        }
+        A_ = A;
+        L_ = std::move(L);
+        U_ = std::move(U);
+        n_ = A.n;
+        upload(true);
+        device_ready_ = true;
+    } else {
+        // A_ keeps the pattern; its values are not refreshed (only dA_'s are)
+        hip_check(hipMemcpyAsync(dA_.v, A.v.data(), A.v.size() * sizeof(double), hipMemcpyHostToDevice, stream_),
+                  "h2d");
+    }
+    host_[0] = 0.0;
+    hip_check(hipMemcpyAsync(fail_, host_, sizeof(int), hipMemcpyHostToDevice, stream_), "h2d");
+    dev::ilu0_numeric(stream_, nlv_l_, lvl_off_, lvl_rows_, dA_.rp, dA_.ci, dA_.v, dL_.rp, dL_.ci, dL_.v, dU_.rp,
+                      dU_.ci, dU_.v, fail_);
+    int fail = 0;
+    hip_check(hipMemcpyAsync(&host_[1], fail_, sizeof(int), hipMemcpyDeviceToHost, stream_), "d2h");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    std::memcpy(&fail, &host_[1], sizeof(int));
+    ilu_ok_ = fail ? 0 : 1;
+    host_factors_ = false;
+}
+
+void IluSystem::sync_factors() {
+    if (host_factors_ || !stream_) return;
+    hip_check(hipMemcpyAsync(L_.v.data(), dL_.v, L_.v.size() * sizeof(double), hipMemcpyDeviceToHost, stream_),
+              "d2h");
+    hip_check(hipMemcpyAsync(U_.v.data(), dU_.v, U_.v.size() * sizeof(double), hipMemcpyDeviceToHost, stream_),
+              "d2h");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    host_factors_ = true;
+}
+
 void IluSystem::set_factors(const Csr& L, const Csr& U, bool up) {
     if (L.n != U.n) throw std::invalid_argument("L and U sizes differ");
     A_ = Csr();
+    device_ready_ = false;
+    host_factors_ = true;
     L_ = L;
     U_ = U;
     ilu_ok_ = 1;
@@ -115,23 +193,31 @@ void IluSystem::set_factors(const Csr& L, const Csr& U, bool up) {
     if (up) upload();
 }
 
-void IluSystem::upload() {
+// pattern_levels: level schedules from the factors' patterns (their values
+// are computed on the device afterwards)
+void IluSystem::upload(bool pattern_levels) {
     if (!stream_) {
         hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipHostMalloc((void**)&host_, 64 * sizeof(double)), "hipHostMalloc");
     }
+    // a re-upload frees every earlier buffer, the GMRES basis allocated on
+    // first use included: nothing may still run on them, and no pointer to
+    // them may survive
+    hip_check(hipStreamSynchronize(stream_), "sync");
     bufs_.reset(new DevBuffers());
+    vbasis_ = nullptr;
+    mb_ = 0;
     DevBuffers& B = *bufs_;
     hipStream_t st = stream_;
     if (A_.n > 0) dA_ = upload_csr(A_, B, st);
     dL_ = upload_csr(L_, B, st);
     dU_ = upload_csr(U_, B, st);
     std::vector<int> off, rows;
-    tri_levels(L_, false, off, rows);
+    tri_levels(L_, false, off, rows, pattern_levels);
     nlv_l_ = (int)off.size() - 1;
     lvl_off_ = kupload(off, B, st);
     lvl_rows_ = kupload(rows, B, st);
-    tri_levels(U_, true, off, rows);
+    tri_levels(U_, true, off, rows, pattern_levels);
     nlv_u_ = (int)off.size() - 1;
     lvu_off_ = kupload(off, B, st);
     lvu_rows_ = kupload(rows, B, st);
@@ -139,6 +225,7 @@ void IluSystem::upload() {
     for (double** p : {&vX_, &vB_, &vR_, &vP_, &vQ_, &vZ_, &vW_, &vY_, &vT_}) *p = kalloc<double>(n, B);
     part_ = kalloc<double>(std::max(4096, dA_.nblk + 16) * 40, B);
     scal_ = kalloc<double>(64, B);
+    fail_ = kalloc<int>(1, B);
     dev::fill(st, 64, scal_, 0.0);
     hip_check(hipStreamSynchronize(st), "upload sync");
 }

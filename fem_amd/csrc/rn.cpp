@@ -441,32 +441,34 @@ class ReflectiveNewton {
         int status = 1, iter = 0;
         switch (p_.solver) {
             case LINA_RN_PCG: {
-                IluSystem sys;
+                // one ILU system for every Newton step: while M's pattern
+                // stays the same only the numeric factorisation runs (device)
                 const double tb = now_s();
-                sys.build(M, true);
+                ilu_.refactor(M);
                 t_build_ = now_s() - tb;
-                if (!sys.ilu_ok()) {
+                if (!ilu_.ilu_ok()) {
                     status = 0;
                     break;
                 }
                 double res = 0;
-                iter = sys.pcg_dev(V2_, bs_, p_.eps, p_.pcg_iter, LINA_STOP_REFERENCE, &res);
+                iter = ilu_.pcg_dev(V2_, bs_, p_.eps, p_.pcg_iter, LINA_STOP_REFERENCE, &res);
                 status = iter < p_.pcg_iter ? 1 : 0;
                 break;
             }
             case LINA_RN_GMRES: {
                 // RN_UMF's slot (reflective_newton.c:312-315): the local solver
                 // decides the status; FEM.c's convention y = V2, V2 = M^-1 y
-                IluSystem sys;
-                sys.build(M, true);
-                if (!sys.ilu_ok()) {
+                const double tb = now_s();
+                ilu_.refactor(M);
+                t_build_ = now_s() - tb;
+                if (!ilu_.ilu_ok()) {
                     status = 0;
                     break;
                 }
                 double eta = 0;
-                iter = sys.gmres_dev(V2_, bs_, p_.gmres_m, p_.gmres_tol, p_.gmres_max_iter, &eta);
-                sys.apply(V2_, T_);
-                hip_check(hipStreamSynchronize(sys.stream()), "sync");
+                iter = ilu_.gmres_dev(V2_, bs_, p_.gmres_m, p_.gmres_tol, p_.gmres_max_iter, &eta);
+                ilu_.apply(V2_, T_);
+                hip_check(hipStreamSynchronize(ilu_.stream()), "sync");
                 dev::copy(st_, n, (const double*)T_, V2_);
                 status = eta <= p_.gmres_tol ? 1 : 0;
                 break;
@@ -667,6 +669,7 @@ class ReflectiveNewton {
     int n_ = 0;
     std::unique_ptr<DevMatrix> dH_;
     std::unique_ptr<Solver> amg_;
+    IluSystem ilu_;  // the ILU(0) inner solvers' system, refactored each Newton step
     std::unique_ptr<DevBuffers> B_;
     hipStream_t st_ = nullptr;
     double *X_ = nullptr, *Xo_ = nullptr, *G_ = nullptr, *D_ = nullptr, *J_ = nullptr, *V1_ = nullptr,

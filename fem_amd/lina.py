@@ -402,6 +402,7 @@ def _bind_krylov(L):
     L.lina_ilu_create.restype = C.c_void_p
     L.lina_ilu_destroy.argtypes = [C.c_void_p]
     L.lina_ilu_ok.argtypes = [C.c_void_p]
+    L.lina_ilu_refactor.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp]
     L.lina_ilu_factor_nnz.argtypes = [C.c_void_p, C.c_char]
     L.lina_ilu_factor_copy.argtypes = [C.c_void_p, C.c_char, _ip, _ip, _dp]
     L.lina_ilu_apply.argtypes = [C.c_void_p, _dp, _dp]
@@ -437,6 +438,13 @@ class ILU:
     @property
     def ok(self) -> bool:
         return bool(lib().lina_ilu_ok(self.h))
+
+    def refactor(self, rowptr, col, val):
+        """lina_ilu_refactor: ILU(0) of a new matrix on the device (same factors as the host)."""
+        rp, ci, v = (np.ascontiguousarray(rowptr, np.int32), np.ascontiguousarray(col, np.int32),
+                     np.ascontiguousarray(val, np.float64))
+        _check(lib().lina_ilu_refactor(self.h, len(rp) - 1, _i(rp), _i(ci), _d(v)))
+        return self.ok
 
     def factor(self, which: str):
         nnz = lib().lina_ilu_factor_nnz(self.h, which.encode())

@@ -143,6 +143,13 @@ typedef struct lina_ilu lina_ilu;
 lina_ilu* lina_ilu_create(int n, const int* rowptr, const int* col, const double* val, int upload);
 void lina_ilu_destroy(lina_ilu* h);
 int lina_ilu_ok(const lina_ilu* h); /* 0: zero pivot, as the reference's return flag */
+/* incomplete_LU_factorization (linear_algebra.c:1952-1982) of a new matrix on
+ * the device, bit-identical to the host factorisation (the CSR's rows sorted
+ * by column, as everywhere in this API).  While the pattern
+ * (rowptr, col) stays the same as the previous refactor's, only the values
+ * are uploaded and the numeric factorisation runs (the reflective-Newton
+ * steps' use).  lina_ilu_ok reports a zero pivot. */
+int lina_ilu_refactor(lina_ilu* h, int n, const int* rowptr, const int* col, const double* val);
 int lina_ilu_factor_nnz(const lina_ilu* h, char which);
 int lina_ilu_factor_copy(const lina_ilu* h, char which, int* rowptr, int* col, double* val);
 /* PCG_default(A, L, U) (PCGS.c:29-34) with factors given by the caller (L:

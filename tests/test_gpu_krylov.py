@@ -97,3 +97,106 @@ def test_bicgstab_amg_matches_reference(lina_mod, ref, name, s, depth):
     say("  iters ref", r["iters"], "gpu", g["iters"], "rel", rel(g["x"], r["x"]))
     assert abs(g["iters"] - r["iters"]) <= 1
     assert rel(g["x"], r["x"]) < 1e-10
+
+
+def _sorted_rows(rp, ci):
+    return all(np.all(np.diff(ci[rp[i]:rp[i + 1]]) > 0) for i in range(len(rp) - 1))
+
+
+@pytest.mark.parametrize("name,s", _systems() + [("damage129", P.damage_system(129)[0])],
+                         ids=lambda v: v if isinstance(v, str) else "")
+def test_ilu_refactor_device_matches_reference(lina_mod, ref, name, s):
+    """lina_ilu_refactor runs ILU(0) on the device (rows sorted, so the device
+    path is taken): the factors are bit-identical to the reference's
+    incomplete_LU_factorization on a first pattern, on new values of the same
+    pattern (values-only upload), and back on the first values."""
+    say("ilu refactor", name)
+    assert _sorted_rows(s.rowptr, s.col)
+    rng = np.random.default_rng(11)
+    val2 = s.val * (1.0 + 0.2 * rng.uniform(-1, 1, len(s.val)))
+    il = lina_mod.ILU(s.rowptr, s.col, s.val)
+    for vals in (s.val, val2, s.val):
+        assert il.refactor(s.rowptr, s.col, vals)
+        M = ref.matrix(s.rowptr, s.col, vals)
+        ok, Lr, Ur = ref.ilu(M)
+        ref.free(M)
+        assert ok
+        for which, R in (("L", Lr), ("U", Ur)):
+            rp, ci, v = il.factor(which)
+            assert np.array_equal(rp, R.rowptr) and np.array_equal(ci, R.col)
+            assert np.array_equal(v, R.val), f"{which} factor differs"
+    # the device factors drive the triangular solves as the host ones do
+    z = np.cos(np.arange(s.n) * 0.3)
+    w = il.apply(z)
+    host = lina_mod.ILU(s.rowptr, s.col, s.val)
+    assert np.array_equal(w, host.apply(z))
+    il.close()
+    host.close()
+
+
+def test_ilu_refactor_zero_pivot(lina_mod, ref):
+    """A zero pivot is reported as by the reference (lina_ilu_ok 0), and a
+    later refactor with good values of the same pattern recovers."""
+    s = P.poisson(9)
+    rp, ci, v = s.rowptr.copy(), s.col.copy(), s.val.copy()
+    v0 = v.copy()
+    for p in range(rp[3], rp[4]):
+        if ci[p] == 3:
+            v0[p] = 0.0  # zero diagonal in row 3, which has upper entries
+    il = lina_mod.ILU(rp, ci, v)
+    assert il.refactor(rp, ci, v)
+    M = ref.matrix(rp, ci, v0)
+    ok_ref, _, _ = ref.ilu(M)
+    ref.free(M)
+    assert not ok_ref
+    assert not il.refactor(rp, ci, v0)
+    assert il.refactor(rp, ci, v)
+    host = lina_mod.ILU(rp, ci, v)
+    for which in ("L", "U"):
+        for x, y in zip(il.factor(which), host.factor(which)):
+            assert np.array_equal(x, y)
+    il.close()
+    host.close()
+
+
+def test_ilu_gmres_after_pattern_change(lina_mod):
+    """GMRES on an ILU system whose matrix pattern changed by refactor (the
+    reflective-Newton steps' use): the re-upload drops the old Krylov basis,
+    and the solve equals a fresh system's bit for bit."""
+    s1 = P.poisson(33)
+    b = np.sin(np.arange(s1.n) * 0.1) + 1.0
+    # second pattern: the couplings (i, i+1) of every 7th row removed
+    # symmetrically, their magnitude moved onto both diagonals
+    rp, ci, v = s1.rowptr, s1.col, s1.val.copy()
+    drop = set()
+    for i in range(0, s1.n - 1, 7):
+        for p in range(rp[i], rp[i + 1]):
+            if ci[p] == i + 1:
+                drop.add((i, i + 1))
+                drop.add((i + 1, i))
+    rows, cols, vals = [], [], []
+    for i in range(s1.n):
+        for p in range(rp[i], rp[i + 1]):
+            if (i, ci[p]) not in drop:
+                rows.append(i)
+                cols.append(ci[p])
+                vals.append(v[p])
+    import scipy.sparse as sp
+    A2 = sp.csr_matrix((vals, (rows, cols)), shape=(s1.n, s1.n))
+    A2.sort_indices()
+    A2 = A2 + sp.diags(np.asarray(abs(s1.val).max()) * np.isin(np.arange(s1.n), [i for i, _ in drop]))
+    A2 = sp.csr_matrix(A2)
+    A2.sort_indices()
+    rp2, ci2, v2 = A2.indptr.astype(np.int32), A2.indices.astype(np.int32), A2.data.astype(np.float64)
+    assert len(ci2) < len(s1.col)
+    il = lina_mod.ILU(s1.rowptr, s1.col, s1.val)
+    out1 = il.gmres(b, m=30, tol=1e-8, max_iter=500)
+    assert out1["eta"] <= 1e-8
+    assert il.refactor(rp2, ci2, v2)
+    out2 = il.gmres(b, m=30, tol=1e-8, max_iter=500)
+    fresh = lina_mod.ILU(rp2, ci2, v2)
+    ref2 = fresh.gmres(b, m=30, tol=1e-8, max_iter=500)
+    assert out2["iters"] == ref2["iters"]
+    assert np.array_equal(out2["x"], ref2["x"])
+    il.close()
+    fresh.close()
