@@ -1102,29 +1102,39 @@ __global__ __launch_bounds__(TPB) void k_coarse(int nd, const int* __restrict__ 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Coarse solve with the symmetric inverse held as its upper triangle of
-// CT x CT FP32 tiles (tile t = (ti[t], tj[t]), ti <= tj, row-major, zero
-// padded), products and sums in FP64.  k_csym_tiles: one workgroup per stored
+// CT x CT tiles of TT values (tile t = (ti[t], tj[t]), ti <= tj, row-major,
+// zero padded), products and sums in FP64.  k_csym_tiles: one workgroup per stored
 // tile (I, J) writes the tile's row sums against bd_J into part[I][J] and,
 // off the diagonal, its column sums against bd_I into part[J][I];
 // k_csym_reduce sums part[I][0..nct) in a fixed order into the dense rows of
 // block I.  Every part[I][K] has exactly one writer.
+template <typename TT>
 __global__ __launch_bounds__(TPB) void k_csym_tiles(int nct, const int* __restrict__ ti, const int* __restrict__ tj,
-                                                    const float* __restrict__ tiles, const double* __restrict__ bd,
+                                                    const TT* __restrict__ tiles, const double* __restrict__ bd,
                                                     double* __restrict__ part) {
-    __shared__ float T[CT][CT + 1];
+    __shared__ TT T[CT][CT + 1];
     __shared__ double bi[CT], bj[CT];
     const int t = blockIdx.x;
     const int I = ti[t], J = tj[t];
-    const f32x4* src = reinterpret_cast<const f32x4*>(tiles + (size_t)t * CT * CT);
+    const TT* src = tiles + (size_t)t * CT * CT;
+    if constexpr (sizeof(TT) == 4) {
+        const f32x4* s4 = reinterpret_cast<const f32x4*>(src);
 #pragma unroll
-    for (int u = 0; u < CT * CT / 4 / TPB; u++) {
-        const int q = threadIdx.x + u * TPB;
-        const f32x4 v = __builtin_nontemporal_load(src + q);
-        const int r = (4 * q) / CT, c = (4 * q) % CT;
-        T[r][c] = v.x;
-        T[r][c + 1] = v.y;
-        T[r][c + 2] = v.z;
-        T[r][c + 3] = v.w;
+        for (int u = 0; u < CT * CT / 4 / TPB; u++) {
+            const int q = threadIdx.x + u * TPB;
+            const f32x4 v = __builtin_nontemporal_load(s4 + q);
+            const int r = (4 * q) / CT, c = (4 * q) % CT;
+            T[r][c] = v.x;
+            T[r][c + 1] = v.y;
+            T[r][c + 2] = v.z;
+            T[r][c + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < CT * CT / TPB; u++) {
+            const int q = threadIdx.x + u * TPB;
+            T[q / CT][q % CT] = __builtin_nontemporal_load(src + q);
+        }
     }
     if (threadIdx.x < CT) bi[threadIdx.x] = bd[I * CT + threadIdx.x];
     else if (threadIdx.x < 2 * CT) bj[threadIdx.x - CT] = bd[J * CT + threadIdx.x - CT];
@@ -1170,14 +1180,15 @@ __global__ __launch_bounds__(TPB) void k_csym_reduce(int nd, int nct, const doub
 
 // tiles of the upper triangle: entry (row, col) of tile t takes
 // inv[min(row, col)][max(row, col)], so that the FP32 copy is exactly symmetric
+template <typename TT>
 __global__ void k_inv_tiles(int nd, int ntiles, const int* __restrict__ ti, const int* __restrict__ tj,
-                            const double* __restrict__ inv, float* __restrict__ tiles) {
+                            const double* __restrict__ inv, TT* __restrict__ tiles) {
     const size_t total = (size_t)ntiles * CT * CT;
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (size_t)gridDim.x * blockDim.x) {
         const int t = (int)(k / (CT * CT)), e = (int)(k % (CT * CT));
         const int row = ti[t] * CT + e / CT, col = tj[t] * CT + e % CT;
-        float v = 0.0f;
-        if (row < nd && col < nd) v = (float)inv[(size_t)std::min(row, col) * nd + std::max(row, col)];
+        TT v = (TT)0;
+        if (row < nd && col < nd) v = (TT)inv[(size_t)std::min(row, col) * nd + std::max(row, col)];
         tiles[k] = v;
     }
 }
@@ -1583,29 +1594,32 @@ template void coarse_solve<double>(hipStream_t, int, const int*, const double*, 
 template void coarse_solve<float>(hipStream_t, int, const int*, const double*, int, const int*, const double*,
                                   const float*, float*, double*, bool);
 
-template <typename XT>
-void coarse_solve_sym(hipStream_t st, int nd, int nct, int ntiles, const int* ti, const int* tj, const float* tiles,
+template <typename XT, typename TT>
+void coarse_solve_sym(hipStream_t st, int nd, int nct, int ntiles, const int* ti, const int* tj, const TT* tiles,
                       double* part, const int* didx, int nt, const int* tidx, const double* tdiag, const XT* b, XT* x,
                       double* bd, bool gathered) {
     if (nd > 0 && !gathered)
         hipLaunchKernelGGL((k_coarse_gather<XT>), dim3(std::min((nd + TPB - 1) / TPB, 1024)), dim3(TPB), 0, st, nd,
                            didx, b, bd);
-    if (ntiles > 0) hipLaunchKernelGGL(k_csym_tiles, dim3(ntiles), dim3(TPB), 0, st, nct, ti, tj, tiles, bd, part);
+    if (ntiles > 0)
+        hipLaunchKernelGGL((k_csym_tiles<TT>), dim3(ntiles), dim3(TPB), 0, st, nct, ti, tj, tiles, bd, part);
     const int g = std::max(std::max(nct, (nt + TPB - 1) / TPB), 1);
     hipLaunchKernelGGL((k_csym_reduce<XT>), dim3(g), dim3(TPB), 0, st, nd, nct, part, didx, nt, tidx, tdiag, b, x);
 }
-template void coarse_solve_sym<double>(hipStream_t, int, int, int, const int*, const int*, const float*, double*,
-                                       const int*, int, const int*, const double*, const double*, double*, double*,
-                                       bool);
-template void coarse_solve_sym<float>(hipStream_t, int, int, int, const int*, const int*, const float*, double*,
-                                      const int*, int, const int*, const double*, const float*, float*, double*,
-                                      bool);
+template void coarse_solve_sym<double, double>(hipStream_t, int, int, int, const int*, const int*, const double*,
+                                               double*, const int*, int, const int*, const double*, const double*,
+                                               double*, double*, bool);
+template void coarse_solve_sym<float, double>(hipStream_t, int, int, int, const int*, const int*, const double*,
+                                              double*, const int*, int, const int*, const double*, const float*,
+                                              float*, double*, bool);
 
+template <typename TT>
 void inverse_to_tiles(hipStream_t st, int nd, int ntiles, const int* ti, const int* tj, const double* inv,
-                      float* tiles) {
+                      TT* tiles) {
     if (ntiles <= 0) return;
-    hipLaunchKernelGGL(k_inv_tiles, dim3(4096), dim3(TPB), 0, st, nd, ntiles, ti, tj, inv, tiles);
+    hipLaunchKernelGGL((k_inv_tiles<TT>), dim3(4096), dim3(TPB), 0, st, nd, ntiles, ti, tj, inv, tiles);
 }
+template void inverse_to_tiles<double>(hipStream_t, int, int, const int*, const int*, const double*, double*);
 
 void trsv(hipStream_t st, const int* rp, const int* ci, const double* v, int nlv, const int* lv_off,
           const int* lv_rows, const double* z, double* w, int upper) {

@@ -215,18 +215,19 @@ void combine(hipStream_t st, int n, int m, const double* V, size_t ld, const dou
 template <typename XT>
 void coarse_solve(hipStream_t st, int nd, const int* didx, const double* inv, int nt, const int* tidx,
                   const double* tdiag, const XT* b, XT* x, double* bd, bool gathered = false);
-// the same with the symmetric inverse's upper triangle as FP32 tiles of CT x
-// CT (tile t = (ti[t], tj[t]), ti[t] <= tj[t], nct tiles a side, zero
-// padded), FP64 products and sums; part: nct * nct * CT doubles of scratch,
-// bd: nct * CT doubles, zero beyond nd
+// the same with the symmetric inverse's upper triangle as tiles of CT x CT
+// values of type TT (FP32 or FP64; tile t = (ti[t], tj[t]), ti[t] <= tj[t],
+// nct tiles a side, zero padded), FP64 products and sums; part: nct * nct *
+// CT doubles of scratch, bd: nct * CT doubles, zero beyond nd
 constexpr int CT = 64;
-template <typename XT>
-void coarse_solve_sym(hipStream_t st, int nd, int nct, int ntiles, const int* ti, const int* tj, const float* tiles,
+template <typename XT, typename TT>
+void coarse_solve_sym(hipStream_t st, int nd, int nct, int ntiles, const int* ti, const int* tj, const TT* tiles,
                       double* part, const int* didx, int nt, const int* tidx, const double* tdiag, const XT* b, XT* x,
                       double* bd, bool gathered);
-// tiles = the upper triangle of inv (nd x nd, FP64) in FP32, symmetrised
+// tiles = the upper triangle of inv (nd x nd, FP64) as TT, symmetrised
+template <typename TT>
 void inverse_to_tiles(hipStream_t st, int nd, int ntiles, const int* ti, const int* tj, const double* inv,
-                      float* tiles);
+                      TT* tiles);
 
 }  // namespace dev
 }  // namespace lina

@@ -474,7 +474,9 @@ class ReflectiveNewton {
                 break;
             }
             case LINA_RN_BICGSTAB: {
+                const double tb = now_s();
                 amg_->recompute_matrix(M);
+                t_build_ = now_s() - tb;
                 double res = 0;
                 iter = amg_->bicgstab_dev(V2_, bs_, p_.bi_eps, p_.bi_max_iter, &res);
                 hip_check(hipStreamSynchronize(amg_->stream()), "sync");
@@ -482,7 +484,9 @@ class ReflectiveNewton {
                 break;
             }
             case LINA_RN_AMG: {
+                const double tb = now_s();
                 amg_->recompute_matrix(M);
+                t_build_ = now_s() - tb;
                 iter = amg_->amg_solve_dev(bs_, V2_);
                 hip_check(hipStreamSynchronize(amg_->stream()), "sync");
                 status = iter < p_.amg.amg_max_iter ? 1 : 0;

@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -583,7 +584,7 @@ void Solver::upload() {
             ccover_ = nd_ + nt_ == n0;
         }
     }
-    inv32_ = nullptr;
+    ctiles_ = nullptr;
     nct_ = ntiles_ = 0;
     tidx_ = dupload(lu.trivial_idx, B, st);
     tdiag_ = dupload(lu.trivial_diag, B, st);
@@ -605,20 +606,18 @@ void Solver::upload() {
     scal_ = dalloc<double>(dev::S_COUNT, B);
     dev::fill(st, dev::S_COUNT, scal_, 0.0);
     hip_check(hipStreamSynchronize(st), "upload sync");
-    ensure_inv32();
+    ensure_ctiles();
     if (prm_.autotune) autotune();
 }
 
-void Solver::set_params(const lina_amg_params& p) {
-    prm_ = p;
-    ensure_inv32();
-}
+void Solver::set_params(const lina_amg_params& p) { prm_ = p; }
 
-// FP32 copy of the coarse inverse for mixed-precision V-cycles: its upper
-// triangle in CT x CT tiles (made when the hierarchy is uploaded or the
-// precision changes, never inside a solve)
-void Solver::ensure_inv32() {
-    if (inv32_ || cbatched_ || nd_ == 0 || !uploaded() || prm_.precision != LINA_PREC_MIXED) return;
+// The coarse inverse's upper triangle as FP64 tiles of CT x CT, read by
+// both precisions' V-cycles (half the bytes of the full inverse, exactly
+// symmetric).  FP32 tiles were measured: 35 % more CG iterations on the
+// high-contrast adjoint system, none on the elasticity benchmark.
+void Solver::ensure_ctiles() {
+    if (ctiles_ || cbatched_ || nd_ == 0 || !uploaded()) return;
     nct_ = (nd_ + dev::CT - 1) / dev::CT;
     std::vector<int> ti, tj;
     for (int I = 0; I < nct_; I++)
@@ -629,9 +628,9 @@ void Solver::ensure_inv32() {
     ntiles_ = (int)ti.size();
     cti_ = dupload(ti, *bufs_, stream_);
     ctj_ = dupload(tj, *bufs_, stream_);
-    inv32_ = dalloc<float>((size_t)ntiles_ * dev::CT * dev::CT, *bufs_);
     cpart_ = dalloc<double>((size_t)nct_ * nct_ * dev::CT, *bufs_);
-    dev::inverse_to_tiles(stream_, nd_, ntiles_, cti_, ctj_, inv_, inv32_);
+    ctiles_ = dalloc<double>((size_t)ntiles_ * dev::CT * dev::CT, *bufs_);
+    dev::inverse_to_tiles<double>(stream_, nd_, ntiles_, cti_, ctj_, inv_, ctiles_);
     hip_check(hipStreamSynchronize(stream_), "sync");
 }
 
@@ -862,9 +861,9 @@ void Solver::vcycle(int l, const T* f, T* x, int xinit) {
         if (xinit == X_ZERO && !ccover_) dev::fill<T>(st, L_[0].n, x, (T)0);
         if (cbatched_)
             dev::bcoarse_solve(st, nd_, crblk_, coff_, cmoff_, didx_, inv_, nt_, tidx_, tdiag_, f, x);
-        else if (std::is_same<T, float>::value && inv32_)
-            dev::coarse_solve_sym(st, nd_, nct_, ntiles_, cti_, ctj_, inv32_, cpart_, didx_, nt_, tidx_, tdiag_, f, x,
-                                  cbd_, fused_gather());
+        else if (ctiles_)
+            dev::coarse_solve_sym(st, nd_, nct_, ntiles_, cti_, ctj_, (const double*)ctiles_, cpart_, didx_, nt_,
+                                  tidx_, tdiag_, f, x, cbd_, fused_gather());
         else
             dev::coarse_solve(st, nd_, didx_, inv_, nt_, tidx_, tdiag_, f, x, cbd_, fused_gather());
         return;

@@ -108,7 +108,7 @@ class Solver {
     void recompute(const std::vector<double>& val);
     // AMG_recompute_coarse_matrices with a new finest matrix of any pattern
     void recompute_matrix(const Csr& M);
-    void set_params(const lina_amg_params& p);
+    void set_params(const lina_amg_params& p);  // takes effect at the next solve
     const lina_amg_params& params() const { return prm_; }
     const HostHierarchy& host() const { return H_; }
     const std::vector<int>& colours(int l) const { return colours_.at(l); }
@@ -161,7 +161,7 @@ class Solver {
 
   private:
     bool fused_gather() const;
-    void ensure_inv32();
+    void ensure_ctiles();
     template <typename VT, typename XT>
     int tune_spmv(dev::DMat<VT>& M, const XT* x, XT* y);
     template <typename VT, typename XT>
@@ -189,7 +189,7 @@ class Solver {
     double* cbd_ = nullptr;
     int* dinv_ = nullptr;     // dense index of each coarsest-level row (-1: not dense)
     bool ccover_ = false;     // dense and trivial rows cover the coarsest level
-    float* inv32_ = nullptr;  // FP32 upper-triangle tiles of the coarse inverse (mixed precision)
+    double* ctiles_ = nullptr;  // upper-triangle FP64 tiles of the coarse inverse
     int nct_ = 0, ntiles_ = 0;  // tiles a side, stored tiles
     int *cti_ = nullptr, *ctj_ = nullptr;  // tile coordinates
     double* cpart_ = nullptr;   // nct_ x nct_ x CT tile partials
