@@ -12,7 +12,8 @@
 // The Krylov iteration is pcg per system: dots are segmented reductions over
 // row chunks that never straddle two systems, every step length is a
 // per-system device scalar, and a system that has converged keeps its
-// iterate (its step length is zero) while the others go on.  The iteration
+// iterate while the others go on; the V-cycle's row kernels skip the row
+// blocks of converged systems.  The iteration
 // body contains no host synchronisation, so `graph_iters` iterations are
 // captured once into a hipGraph and replayed; the host reads the number of
 // still-active systems between replays.
@@ -170,7 +171,12 @@ class BatchSolver {
         HostHierarchy U = join(H);
         for (double sg : U.signs)
             if (sg < 0) throw std::invalid_argument("batch: systems need positive diagonals");
+        // the systems' row offsets on every level (the union is block-diagonal)
+        std::vector<std::vector<int>> seg(U.lv.size(), std::vector<int>(ns + 1, 0));
+        for (size_t l = 0; l < U.lv.size(); l++)
+            for (int k = 0; k < ns; k++) seg[l][k + 1] = seg[l][k] + H[k].lv[l].A.n;
         S_.set_hierarchy(std::move(U), p);
+        S_.set_segments(std::move(seg));
         S_.upload();
         // row chunks inside systems
         std::vector<int> crow{0}, cinst, ichunk{0};
@@ -191,6 +197,7 @@ class BatchSolver {
         for (double** v : {&R_, &P_, &Q_, &Z_}) *v = alloc(n);
         part_ = alloc(std::max(nchunk_, 4096));
         for (double** v : {&s_, &sold_, &pq_, &rr_, &hb_, &active_, &iters_}) *v = alloc(ns_);
+        S_.set_active(active_);  // the V-cycle skips the row blocks of converged systems
         count_ = alloc(1);
         hip_check(hipHostMalloc((void**)&host_, 2 * sizeof(double)), "hipHostMalloc");
         hip_check(hipStreamSynchronize(st_), "batch upload");
@@ -218,11 +225,11 @@ class BatchSolver {
         dev::seg_sum(st, ns_, ichunk_, part_, hb_, nullptr);
         sqrt_inplace(hb_);
         DevLevel& top = S_.finest();
+        dev::fill<double>(st, ns_, active_, 1.0);  // before any row kernel (they skip inactive systems)
+        dev::fill<double>(st, ns_, iters_, 0.0);
         dev::spmv<double, double>(st, top.A, X, Bv, R_, dev::MODE_RES, nullptr, nullptr);
         dev::seg_dot(st, nchunk_, crow_, R_, R_, part_);
         dev::seg_sum(st, ns_, ichunk_, part_, rr_, nullptr);
-        dev::fill<double>(st, ns_, active_, 1.0);
-        dev::fill<double>(st, ns_, iters_, 0.0);
         dev::bpcg_check(st, ns_, rr_, hb_, tol, max_iter, active_, iters_, 0);
         dev::fill<double>(st, ns_, s_, 1.0);
         dev::fill<double>(st, n, P_, 0.0);
@@ -288,7 +295,7 @@ class BatchSolver {
         S_.vcycle_once(R_, Z_);
         dev::seg_dot(st, nchunk_, crow_, R_, Z_, part_);
         dev::seg_sum(st, ns_, ichunk_, part_, s_, sold_);
-        dev::bpcg_p(st, nchunk_, crow_, cinst_, P_, Z_, s_, sold_);
+        dev::bpcg_p(st, nchunk_, crow_, cinst_, P_, Z_, s_, sold_, active_);
         DevLevel& top = S_.finest();
         dev::spmv<double, double>(st, top.A, P_, nullptr, Q_, dev::MODE_OUTER, nullptr, nullptr);
         dev::seg_dot(st, nchunk_, crow_, P_, Q_, part_);

@@ -195,6 +195,16 @@ __device__ __forceinline__ int grid_block(int b, int G, int k, int nblk) {
     return idx < nblk ? idx : -1;
 }
 
+// a batch's row block is live while one of its systems is active (always,
+// until the flags are attached: the autotuner runs during upload)
+__device__ __forceinline__ bool block_live(const int2* __restrict__ bspec, const double* __restrict__ active, int bb) {
+    if (!bspec || !active) return true;
+    const int2 sp = bspec[bb];
+    for (int k = sp.x; k <= sp.y; k++)
+        if (active[k] != 0.0) return true;
+    return false;
+}
+
 // One row block bb.  The first row of each lane group's reduction (all rows
 // of a block when it has at most TPB/LPR rows) has its row bounds and base
 // entry loaded together with the streamed entries, off the critical path.
@@ -257,11 +267,13 @@ __global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int2* __restrict__
                                               const int2* __restrict__ cbase, const VT* __restrict__ v,
                                               const XT* __restrict__ x, const XT* __restrict__ b,
                                               XT* __restrict__ y, double* __restrict__ part,
-                                              const XT* __restrict__ w, const Epi<VT, XT> epi) {
+                                              const XT* __restrict__ w, const Epi<VT, XT> epi,
+                                              const int2* __restrict__ bspec, const double* __restrict__ active) {
     __shared__ XT prod[block_nnz<VT>()];
     for (int it = 0;; it++) {
         const int bb = grid_block(blockIdx.x, gridDim.x, it, nblk);
         if (bb < 0) break;
+        if (!block_live(bspec, active, bb)) continue;
         spmv_block<VT, XT, MODE, LPR>(bb, bk, rp, ci, cs, cbase, v, x, b, y, part, w, epi, prod);
         __syncthreads();  // prod is restaged by the next block
     }
@@ -519,10 +531,12 @@ __global__ __launch_bounds__(TPB) void k_mcsor(int nblk, int blk0, const int2* _
                                                const uint16_t* __restrict__ cs, const int2* __restrict__ cbase,
                                                const VT* __restrict__ v, const int* __restrict__ dpos,
                                                const int* __restrict__ perm, const XT* __restrict__ f,
-                                               XT* __restrict__ x, XT omega) {
+                                               XT* __restrict__ x, XT omega, const int2* __restrict__ bspec,
+                                               const double* __restrict__ active) {
     constexpr int BN = block_nnz<VT>();
     __shared__ XT prod[BN];
     const int bb = blk0 + xcd_block(blockIdx.x, nblk);
+    if (!block_live(bspec, active, bb)) return;
     const int2 s0 = bk[bb], s1 = bk[bb + 1];
     const int r0 = s0.x, r1 = s1.x, k0 = s0.y, k1 = s1.y;
     if (k1 - k0 > BN) {
@@ -633,10 +647,12 @@ __global__ __launch_bounds__(TPB) void k_jacobi(int nblk, const int2* __restrict
                                                 const int2* __restrict__ cbase, const VT* __restrict__ v,
                                                 const int* __restrict__ dpos, const VT* __restrict__ l1,
                                                 const XT* __restrict__ f, const XT* __restrict__ xo,
-                                                XT* __restrict__ x, int nlim) {
+                                                XT* __restrict__ x, int nlim, const int2* __restrict__ bspec,
+                                                const double* __restrict__ active) {
     constexpr int BN = block_nnz<VT>();
     __shared__ XT prod[BN];
     const int bb = xcd_block(blockIdx.x, nblk);
+    if (!block_live(bspec, active, bb)) return;
     const int2 s0 = bk[bb], s1 = bk[bb + 1];
     const int r0 = s0.x, r1 = s1.x, k0 = s0.y, k1 = s1.y;
     if (k1 - k0 > BN) {
@@ -1019,15 +1035,16 @@ __global__ __launch_bounds__(TPB) void k_seg_sum(int ninst, const int* __restric
 __global__ __launch_bounds__(TPB) void k_bpcg_p(int nchunk, const int* __restrict__ crow,
                                                 const int* __restrict__ cinst, double* __restrict__ P,
                                                 const double* __restrict__ Z, const double* __restrict__ s,
-                                                const double* __restrict__ sold) {
+                                                const double* __restrict__ sold, const double* __restrict__ active) {
     for (int c = blockIdx.x; c < nchunk; c += gridDim.x) {
         const int k = cinst[c];
+        if (active[k] == 0.0) continue;  // an inactive system keeps its P
         const double beta = s[k] / sold[k];
         for (int i = crow[c] + threadIdx.x; i < crow[c + 1]; i += TPB) P[i] = Z[i] + beta * P[i];
     }
 }
 
-// X += a P, R -= a Q with a = s/pq for active systems, 0 otherwise
+// X += a P, R -= a Q with a = s/pq for the active systems
 __global__ __launch_bounds__(TPB) void k_bpcg_xr(int nchunk, const int* __restrict__ crow,
                                                  const int* __restrict__ cinst, double* __restrict__ X,
                                                  double* __restrict__ R, const double* __restrict__ P,
@@ -1035,7 +1052,8 @@ __global__ __launch_bounds__(TPB) void k_bpcg_xr(int nchunk, const int* __restri
                                                  const double* __restrict__ pq, const double* __restrict__ active) {
     for (int c = blockIdx.x; c < nchunk; c += gridDim.x) {
         const int k = cinst[c];
-        const double a = active[k] != 0.0 ? s[k] / pq[k] : 0.0;
+        if (active[k] == 0.0) continue;  // its X and R stay as they are
+        const double a = s[k] / pq[k];
         for (int i = crow[c] + threadIdx.x; i < crow[c + 1]; i += TPB) {
             X[i] += a * P[i];
             R[i] -= a * Q[i];
@@ -1201,12 +1219,13 @@ __global__ __launch_bounds__(TPB) void k_bcoarse(int nd, const int* __restrict__
                                                  const size_t* __restrict__ moff, const int* __restrict__ didx,
                                                  const double* __restrict__ inv, int nt, const int* __restrict__ tidx,
                                                  const double* __restrict__ tdiag, const XT* __restrict__ b,
-                                                 XT* __restrict__ x) {
+                                                 XT* __restrict__ x, const double* __restrict__ active) {
     const int wave = (blockIdx.x * TPB + threadIdx.x) / WAVE;
     const int lane = threadIdx.x % WAVE;
     const int nw = gridDim.x * TPB / WAVE;
     for (int g = wave; g < nd; g += nw) {
         const int bb = rblk[g];
+        if (active && active[bb] == 0.0) continue;  // a converged system (block bb) of a batch
         const int o = off[bb], m = off[bb + 1] - o;
         const double* a = inv + moff[bb] + (size_t)(g - o) * m;
         double s = 0.0;
@@ -1334,7 +1353,7 @@ int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int
     const Epi<VT, XT> e = epi ? *epi : Epi<VT, XT>{};
 #define LINA_SPMV_MODE(M)                                                                                       \
     LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, M, L>), g, t, 0, st, A.nblk, A.bk, A.rp, A.ci, \
-                                                A.cs, A.cb, A.v, x, b, y, part, w, e))
+                                                A.cs, A.cb, A.v, x, b, y, part, w, e, A.bspec, A.active))
     switch (mode) {
         case MODE_SET: LINA_SPMV_MODE(MODE_SET); break;
         case MODE_OUTER: LINA_SPMV_MODE(MODE_OUTER); break;
@@ -1452,8 +1471,10 @@ void seg_sum(hipStream_t st, int ninst, const int* ichunk, const double* part, d
     if (ninst > 0) hipLaunchKernelGGL(k_seg_sum, dim3(std::min(ninst, 65535)), dim3(TPB), 0, st, ninst, ichunk, part, out, old);
 }
 void bpcg_p(hipStream_t st, int nchunk, const int* crow, const int* cinst, double* P, const double* Z, const double* s,
-            const double* sold) {
-    if (nchunk > 0) hipLaunchKernelGGL(k_bpcg_p, dim3(std::min(nchunk, 65535)), dim3(TPB), 0, st, nchunk, crow, cinst, P, Z, s, sold);
+            const double* sold, const double* active) {
+    if (nchunk > 0)
+        hipLaunchKernelGGL(k_bpcg_p, dim3(std::min(nchunk, 65535)), dim3(TPB), 0, st, nchunk, crow, cinst, P, Z, s, sold,
+                           active);
 }
 void bpcg_xr(hipStream_t st, int nchunk, const int* crow, const int* cinst, double* X, double* R, const double* P,
              const double* Q, const double* s, const double* pq, const double* active) {
@@ -1526,7 +1547,8 @@ void mcsor_colour(hipStream_t st, const DMat<VT>& Ac, int blk0, int nblk, const 
                   XT omega) {
     if (nblk <= 0) return;
     LINA_LPR_DISPATCH(Ac.lpr, hipLaunchKernelGGL((k_mcsor<VT, XT, L>), dim3(nblk), dim3(TPB), 0, st, nblk, blk0,
-                                                 Ac.bk, Ac.rp, Ac.ci, Ac.cs, Ac.cb, Ac.v, Ac.dpos, perm, f, x, omega))
+                                                 Ac.bk, Ac.rp, Ac.ci, Ac.cs, Ac.cb, Ac.v, Ac.dpos, perm, f, x, omega, Ac.bspec,
+                                                 Ac.active))
 }
 template void mcsor_colour<double, double>(hipStream_t, const DMat<double>&, int, int, const int*, const double*,
                                            double*, double);
@@ -1551,10 +1573,12 @@ void jacobi(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, const XT* 
     dim3 g(A.nblk), t(TPB);
     if (kind == JAC_REF) {
         LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_REF, L>), g, t, 0, st, A.nblk, A.bk, A.rp,
-                                                    A.ci, A.cs, A.cb, A.v, A.dpos, A.l1, f, xo, x, nlim))
+                                                    A.ci, A.cs, A.cb, A.v, A.dpos, A.l1, f, xo, x, nlim, A.bspec,
+                                                    A.active))
     } else {
         LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_L1, L>), g, t, 0, st, A.nblk, A.bk, A.rp,
-                                                    A.ci, A.cs, A.cb, A.v, A.dpos, A.l1, f, xo, x, nlim))
+                                                    A.ci, A.cs, A.cb, A.v, A.dpos, A.l1, f, xo, x, nlim, A.bspec,
+                                                    A.active))
     }
 }
 template void jacobi<double, double>(hipStream_t, const DMat<double>&, int, const double*, const double*, double*,
@@ -1648,17 +1672,20 @@ void combine(hipStream_t st, int n, int m, const double* V, size_t ld, const dou
 // piv: one device double of scratch
 template <typename XT>
 void bcoarse_solve(hipStream_t st, int nd, const int* rblk, const int* off, const size_t* moff, const int* didx,
-                   const double* inv, int nt, const int* tidx, const double* tdiag, const XT* b, XT* x) {
+                   const double* inv, int nt, const int* tidx, const double* tdiag, const XT* b, XT* x,
+                   const double* active) {
     const int waves_per_block = TPB / WAVE;
     int g = std::max(1, std::min((nd + waves_per_block - 1) / waves_per_block, 8192));
     g = std::max(g, std::min((nt + TPB - 1) / TPB, 4096));
     hipLaunchKernelGGL((k_bcoarse<XT>), dim3(g), dim3(TPB), 0, st, nd, rblk, off, moff, didx, inv, nt, tidx, tdiag,
-                       b, x);
+                       b, x, active);
 }
 template void bcoarse_solve<double>(hipStream_t, int, const int*, const int*, const size_t*, const int*,
-                                    const double*, int, const int*, const double*, const double*, double*);
+                                    const double*, int, const int*, const double*, const double*, double*,
+                                    const double*);
 template void bcoarse_solve<float>(hipStream_t, int, const int*, const int*, const size_t*, const int*,
-                                   const double*, int, const int*, const double*, const float*, float*);
+                                   const double*, int, const int*, const double*, const float*, float*,
+                                   const double*);
 
 void batched_gauss_jordan_inverse(hipStream_t st, int nd, int max_block, const int* rblk, const int* off,
                                   const size_t* moff, double* M, double* piv) {

@@ -71,6 +71,10 @@ struct DMat {
     uint16_t* cs = nullptr;
     int2* cb = nullptr;
     double idx_bytes = 0;  // column index bytes the row kernels read (host-side accounting)
+    // batches: the first and last system of each row block's rows, and the
+    // systems' active flags; a block whose systems are all inactive is skipped
+    int2* bspec = nullptr;
+    const double* active = nullptr;
     int lpr = 1;         // lanes reducing one row from LDS: 1 (sequential), 4, 8, 16, 32
     int* dpos = nullptr; // position of each row's diagonal entry in ci/v (-1: none)
     VT* l1 = nullptr;    // sum_j |a_ij| of each row (l1-Jacobi), summed in column order
@@ -146,7 +150,7 @@ void scale_by(hipStream_t st, int n, double* x, const double* scal, int slot, in
 void seg_dot(hipStream_t st, int nchunk, const int* crow, const double* a, const double* b, double* part);
 void seg_sum(hipStream_t st, int ninst, const int* ichunk, const double* part, double* out, double* old);
 void bpcg_p(hipStream_t st, int nchunk, const int* crow, const int* cinst, double* P, const double* Z, const double* s,
-            const double* sold);
+            const double* sold, const double* active);
 void bpcg_xr(hipStream_t st, int nchunk, const int* crow, const int* cinst, double* X, double* R, const double* P,
              const double* Q, const double* s, const double* pq, const double* active);
 void bpcg_check(hipStream_t st, int ninst, const double* rr, const double* hb, double tol, int max_iter,
@@ -194,9 +198,12 @@ void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* piv);
 // block b: rows [off[b], off[b+1]), matrix at M + moff[b]); piv: one per block
 void batched_gauss_jordan_inverse(hipStream_t st, int nd, int max_block, const int* rblk, const int* off,
                                   const size_t* moff, double* M, double* piv);
+// (active != nullptr: the dense rows of inactive systems -- block index =
+// system index -- are skipped)
 template <typename XT>
 void bcoarse_solve(hipStream_t st, int nd, const int* rblk, const int* off, const size_t* moff, const int* didx,
-                   const double* inv, int nt, const int* tidx, const double* tdiag, const XT* b, XT* x);
+                   const double* inv, int nt, const int* tidx, const double* tdiag, const XT* b, XT* x,
+                   const double* active = nullptr);
 // level-scheduled ILU(0) triangular solve (upper=0: L, diagonal last; 1: U, diagonal first)
 void trsv(hipStream_t st, const int* rp, const int* ci, const double* v, int nlv, const int* lv_off,
           const int* lv_rows, const double* z, double* w, int upper);

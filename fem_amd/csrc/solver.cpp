@@ -182,11 +182,32 @@ static long column_codes(const Csr& A, const std::vector<int>& starts_with_end, 
 // share: a copy of A with other values uploaded before; its row pointers and
 // column indices are reused, and its 16-bit offsets too when its schedule
 // (share_starts) is the same
+// first and last system (seg: a batch's row offsets per system) of each row
+// block's rows; row r of A is row row_ids[r] of the level (ascending in a block)
+static std::vector<int2> block_systems(const std::vector<int>& starts_with_end, const std::vector<int>& seg,
+                                       const int* row_ids) {
+    const int nblk = (int)starts_with_end.size() - 1;
+    std::vector<int2> bs(std::max(nblk, 1), make_int2(0, 0));
+    auto sys = [&](int r) {
+        const int i = row_ids ? row_ids[r] : r;
+        return (int)(std::upper_bound(seg.begin(), seg.end(), i) - seg.begin()) - 1;
+    };
+    for (int b = 0; b < nblk; b++) {
+        const int r0 = starts_with_end[b], r1 = starts_with_end[b + 1] - 1;
+        const int s0 = sys(r0), s1 = sys(r1);
+        bs[b] = make_int2(std::min(s0, s1), std::max(s0, s1));
+    }
+    return bs;
+}
+
 template <typename VT, typename OT = double>
 static void upload_mat(const Csr& A, const std::vector<int>& starts_with_end, dev::DMat<VT>& D, DevBuffers& B,
                        hipStream_t st, const dev::DMat<OT>* share = nullptr,
-                       const std::vector<int>* share_starts = nullptr, const int* row_ids = nullptr) {
+                       const std::vector<int>* share_starts = nullptr, const int* row_ids = nullptr,
+                       const std::vector<int>* seg = nullptr) {
     validate(A, starts_with_end, dev::block_nnz<VT>());
+    D.bspec = seg ? dupload(block_systems(starts_with_end, *seg, row_ids), B, st) : nullptr;
+    D.active = nullptr;
     D.n = A.n;
     D.ncols = A.ncols;
     D.nnz = A.nnz();
@@ -252,11 +273,12 @@ static std::vector<int> schedule(const Csr& A) {
 
 // upload A as FP64 + FP32 copies sharing the index arrays (each with the row
 // blocks of its own value size); returns the larger block count
-int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st) {
+int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st,
+                const std::vector<int>* seg) {
     Csr A = drop_outside_zeros(A0);
     const std::vector<int> s64 = schedule<double>(A), s32 = schedule<float>(A);
-    upload_mat(A, s64, D, B, st);
-    upload_mat(A, s32, F, B, st, &D, &s64);
+    upload_mat<double, double>(A, s64, D, B, st, nullptr, nullptr, nullptr, seg);
+    upload_mat(A, s32, F, B, st, &D, &s64, nullptr, seg);
     return std::max(D.nblk, F.nblk);
 }
 
@@ -438,7 +460,8 @@ void Solver::upload() {
         DevLevel& d = L_[l];
         d.n = h.A.n;
         maxn = std::max(maxn, d.n);
-        maxblk = std::max(maxblk, upload_pair(h.A, d.A, d.Af, B, st));
+        const std::vector<int>* segl = seg_off_.empty() ? nullptr : &seg_off_[l];
+        maxblk = std::max(maxblk, upload_pair(h.A, d.A, d.Af, B, st, segl));
         d.x = dalloc<double>(d.n, B);
         d.f = dalloc<double>(d.n, B);
         d.r = dalloc<double>(d.n, B);
@@ -449,8 +472,8 @@ void Solver::upload() {
         d.xof = dalloc<float>(d.n, B);
         d.nlim = (d.n / std::max(1, prm_.dof)) * std::max(1, prm_.dof);
         if (l == 0) continue;
-        maxblk = std::max(maxblk, upload_pair(h.P, d.P, d.Pf, B, st));
-        maxblk = std::max(maxblk, upload_pair(h.R, d.R, d.Rf, B, st));
+        maxblk = std::max(maxblk, upload_pair(h.P, d.P, d.Pf, B, st, segl));
+        maxblk = std::max(maxblk, upload_pair(h.R, d.R, d.Rf, B, st, seg_off_.empty() ? nullptr : &seg_off_[l - 1]));
         // colour-permuted copy for the multicolour sweeps
         const std::vector<int>& col = colours_[l];
         int nc = ncolours_[l];
@@ -491,8 +514,8 @@ void Solver::upload() {
         };
         const std::vector<int> c64 = colour_schedule(dev::block_nnz<double>(), d.colour_blk);
         const std::vector<int> c32 = colour_schedule(dev::block_nnz<float>(), d.colour_blk32);
-        upload_mat<double, double>(Ac, c64, d.Ac, B, st, nullptr, nullptr, perm.data());
-        upload_mat(Ac, c32, d.Acf, B, st, &d.Ac, &c64, perm.data());
+        upload_mat<double, double>(Ac, c64, d.Ac, B, st, nullptr, nullptr, perm.data(), segl);
+        upload_mat(Ac, c32, d.Acf, B, st, &d.Ac, &c64, perm.data(), segl);
         d.perm = dupload(perm, B, st);
         d.ncolours = nc;
         // level schedules of the exact lexicographic sweeps
@@ -611,6 +634,20 @@ void Solver::upload() {
 }
 
 void Solver::set_params(const lina_amg_params& p) { prm_ = p; }
+
+void Solver::set_segments(std::vector<std::vector<int>> off) {
+    if (!off.empty() && off.size() != H_.lv.size()) throw std::invalid_argument("segments: one offset list per level");
+    seg_off_ = std::move(off);
+}
+
+void Solver::set_active(const double* active) {
+    if (seg_off_.empty()) throw std::logic_error("set_active without segments");
+    for (DevLevel& d : L_) {
+        for (auto* m : {&d.A, &d.P, &d.R, &d.Ac}) m->active = m->bspec ? active : nullptr;
+        for (auto* m : {&d.Af, &d.Pf, &d.Rf, &d.Acf}) m->active = m->bspec ? active : nullptr;
+    }
+    seg_active_ = active;
+}
 
 // The coarse inverse's upper triangle as FP64 tiles of CT x CT, read by
 // both precisions' V-cycles (half the bytes of the full inverse, exactly
@@ -860,7 +897,7 @@ void Solver::vcycle(int l, const T* f, T* x, int xinit) {
         // the dense-row/trivial-row split need not cover every row
         if (xinit == X_ZERO && !ccover_) dev::fill<T>(st, L_[0].n, x, (T)0);
         if (cbatched_)
-            dev::bcoarse_solve(st, nd_, crblk_, coff_, cmoff_, didx_, inv_, nt_, tidx_, tdiag_, f, x);
+            dev::bcoarse_solve(st, nd_, crblk_, coff_, cmoff_, didx_, inv_, nt_, tidx_, tdiag_, f, x, seg_active_);
         else if (ctiles_)
             dev::coarse_solve_sym(st, nd_, nct_, ntiles_, cti_, ctj_, (const double*)ctiles_, cpart_, didx_, nt_,
                                   tidx_, tdiag_, f, x, cbd_, fused_gather());

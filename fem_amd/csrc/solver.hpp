@@ -48,7 +48,8 @@ struct DevLevel {
     float *xf = nullptr, *ff = nullptr, *rf = nullptr, *xof = nullptr;
 };
 
-int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st);
+int upload_pair(const Csr& A0, dev::DMat<double>& D, dev::DMat<float>& F, DevBuffers& B, hipStream_t st,
+                const std::vector<int>* seg = nullptr);
 void build_blocks(const Csr& A, int begin, int end, int max_nnz, std::vector<int>& starts,
                   int max_rows = dev::BLOCK_ROWS);
 std::vector<int2> block_pairs(const Csr& A, const std::vector<int>& starts_with_end);
@@ -109,6 +110,11 @@ class Solver {
     // AMG_recompute_coarse_matrices with a new finest matrix of any pattern
     void recompute_matrix(const Csr& M);
     void set_params(const lina_amg_params& p);  // takes effect at the next solve
+    // a batch's row offsets of its systems on every level (before upload), and
+    // the systems' active flags (device, after upload): row blocks whose
+    // systems are all inactive are skipped by the V-cycle's row kernels
+    void set_segments(std::vector<std::vector<int>> off);
+    void set_active(const double* active);
     const lina_amg_params& params() const { return prm_; }
     const HostHierarchy& host() const { return H_; }
     const std::vector<int>& colours(int l) const { return colours_.at(l); }
@@ -167,6 +173,8 @@ class Solver {
     template <typename VT, typename XT>
     int tune_mcsor(DevLevel& d, dev::DMat<VT>& Ac, const XT* f, XT* x);
     std::vector<Tuned> tuned_;
+    std::vector<std::vector<int>> seg_off_;
+    const double* seg_active_ = nullptr;
     lina_amg_params prm_{};
     HostHierarchy H_;
     std::vector<std::vector<int>> colours_;
