@@ -274,7 +274,14 @@ class Oracle:
         L.orc_pcg_amg.argtypes = [_vp, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                   C.c_int, C.c_int, _dp, _ip]
         L.orc_colouring.argtypes = [C.c_int, _ip, _ip, _dp, _ip]
+        L.orc_set_coarse_smoother.argtypes = [_vp, C.c_int, C.c_int]
         L.orc_spmv.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp]
+        L.orc_ilu0.argtypes = [C.c_int, _ip, _ip, _dp, _ip, _ip, _dp, _ip, _ip, _dp]
+        L.orc_gmres_ilu.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, _dp]
+        L.orc_pcg_ilu.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int, _dp]
+        L.orc_ilu_apply.argtypes = [C.c_int, _ip, _ip, _dp, _ip, _ip, _dp, _dp, _dp]
+        L.orc_bicgstab_amg.argtypes = [_vp, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double,
+                                       C.c_int, C.c_double, C.c_int, _dp]
 
     def amg_setup(self, rowptr, col, val, dof=1, depth=4, smiter=2, aggr_smiter=3, coarsening=1,
                   eps_minus=0.3, eps_plus=0.5, matrix_tol=0.1):
@@ -319,8 +326,13 @@ class Oracle:
 
     def pcg_amg(self, rowptr, col, val, b, x0=None, dof=1, depth=4, smiter=2, aggr_smiter=3, coarsening=1,
                 smoother="sor", alpha=1.6, amg_eps=1e-2, amg_max_iter=100, zero_init=False, tol=1e-10,
-                max_iter=1000, stop="reference"):
+                max_iter=1000, stop="reference", coarse_sweeps=2):
+        """smoother "hybrid": MC-SSOR on the finest level, l1-Jacobi with
+        coarse_sweeps sweeps below (the harness's refh_hybrid_smoother)."""
         h = self.amg_setup(rowptr, col, val, dof, depth, smiter, aggr_smiter, coarsening)
+        if smoother == "hybrid":
+            self.L.orc_set_coarse_smoother(h, SMOOTHERS["l1_jacobi"], coarse_sweeps)
+            smoother = "mc_ssor"
         n = len(rowptr) - 1
         X = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
         B = np.array(b, dtype=np.float64)
@@ -348,3 +360,66 @@ class Oracle:
                         dptr(np.ascontiguousarray(val, dtype=np.float64)),
                         dptr(np.ascontiguousarray(x, dtype=np.float64)), dptr(y))
         return y
+
+    def ilu0(self, rowptr, col, val):
+        """(ok, L, U) as CSR; the factors are meaningful only when ok."""
+        rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+        ci = np.ascontiguousarray(col, dtype=np.int32)
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        n, cap = len(rp) - 1, len(ci) + len(rp)
+        lrp, urp = np.zeros(n + 1, np.int32), np.zeros(n + 1, np.int32)
+        lci, uci = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        lv, uv = np.zeros(cap), np.zeros(cap)
+        ok = self.L.orc_ilu0(n, iptr(rp), iptr(ci), dptr(v), iptr(lrp), iptr(lci), dptr(lv), iptr(urp), iptr(uci),
+                             dptr(uv))
+        if not ok:
+            return 0, None, None
+        return 1, CSR(lrp, lci[:lrp[n]], lv[:lrp[n]]), CSR(urp, uci[:urp[n]], uv[:urp[n]])
+
+    def gmres_ilu(self, rowptr, col, val, b, x0=None, m=30, tol=1e-8, max_iter=1000):
+        rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+        ci = np.ascontiguousarray(col, dtype=np.int32)
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        n = len(rp) - 1
+        X = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+        B = np.ascontiguousarray(b, dtype=np.float64)
+        eta = C.c_double()
+        it = self.L.orc_gmres_ilu(n, iptr(rp), iptr(ci), dptr(v), dptr(X), dptr(B), m, tol, max_iter, C.byref(eta))
+        return {"iters": it, "x": X, "eta": eta.value}
+
+    def bicgstab_amg(self, rowptr, col, val, b, x0=None, dof=1, depth=4, smoother="sor", alpha=1.0, eps=1e-8,
+                     max_iter=1000, amg_eps=1e-4, amg_max_iter=20):
+        """BiCGStab with the AMG preconditioner set up as BI_set_AMG_preconditioner does."""
+        rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+        ci = np.ascontiguousarray(col, dtype=np.int32)
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        n = len(rp) - 1
+        h = self.amg_setup(rp, ci, v, dof, depth, 2, 3, 1, matrix_tol=0.1)
+        X = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+        B = np.ascontiguousarray(b, dtype=np.float64)
+        res = C.c_double()
+        try:
+            it = self.L.orc_bicgstab_amg(h, n, iptr(rp), iptr(ci), dptr(v), dptr(X), dptr(B), SMOOTHERS[smoother],
+                                         alpha, amg_eps, amg_max_iter, eps, max_iter, C.byref(res))
+        finally:
+            self.amg_free(h)
+        return {"iters": it, "x": X, "res": res.value}
+
+    def ilu_apply(self, L: CSR, U: CSR, z):
+        """w = U^-1 L^-1 z with the factors of ilu0."""
+        n = L.n
+        w = np.zeros(n)
+        self.L.orc_ilu_apply(n, iptr(L.rowptr), iptr(L.col), dptr(L.val), iptr(U.rowptr), iptr(U.col), dptr(U.val),
+                             dptr(np.ascontiguousarray(z, dtype=np.float64)), dptr(w))
+        return w
+
+    def pcg_ilu(self, rowptr, col, val, b, x0=None, tol=1e-10, max_iter=1000):
+        rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+        ci = np.ascontiguousarray(col, dtype=np.int32)
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        n = len(rp) - 1
+        X = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+        B = np.array(b, dtype=np.float64)
+        res = C.c_double()
+        it = self.L.orc_pcg_ilu(n, iptr(rp), iptr(ci), dptr(v), dptr(X), dptr(B), tol, max_iter, C.byref(res))
+        return {"iters": it, "x": X, "b_out": B, "res": res.value}

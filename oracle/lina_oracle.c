@@ -1378,3 +1378,424 @@ void orc_set_coarse_smoother(orc_amg* h, int sm, int sweeps) {
     h->coarse_sm = sm;
     h->coarse_sweeps = sweeps;
 }
+
+/* ---------------------------------------------------------------- ILU(0) */
+
+/* sum of l[k] * u[k] over the columns k <= lim held by both sorted lists, in
+ * increasing k from 0 (row_times_row, linear_algebra.c:1795-1819) */
+static double pair_dot(const int* lc, const double* lv, int ln, const int* uc, const double* uv, int un, int lim) {
+    int a, b = 0;
+    double s = 0;
+    for (a = 0; a < ln && lc[a] <= lim; a++) {
+        while (b < un && uc[b] < lc[a]) b++;
+        if (b < un && uc[b] == lc[a]) s += lv[a] * uv[b];
+    }
+    return s;
+}
+
+/* incomplete_LU_factorization (linear_algebra.c:1958-1984).  The reference
+ * seeds L and U with sparse_zero (linear_algebra.c:501-508): every row holds
+ * its diagonal (0.0) before the sweep, and insert_sparse adds to an entry that
+ * exists, so L's diagonal is 0.0 + (a_ii - sum).  U is built by columns (its
+ * rows hold U^T) and transposed at the end.  A zero pivot returns 0 and leaves
+ * the outputs undefined. */
+int orc_ilu0(int n, const int* rp, const int* ci, const double* v, int* lrp, int* lci, double* lv, int* urp,
+             int* uci, double* uv) {
+    int i, j, p, nl = 0;
+    int *cp = (int*)calloc(n + 1, sizeof(int)), *fill = (int*)malloc((n + 1) * sizeof(int));
+    int *cr, *pos;
+    double* cv;
+    /* U^T rows: column j of U lists (i, u_ij), i < j, by increasing i */
+    for (i = 0; i < n; i++)
+        for (p = rp[i]; p < rp[i + 1]; p++)
+            if (ci[p] > i) cp[ci[p] + 1]++;
+    for (j = 0; j < n; j++) cp[j + 1] += cp[j];
+    memcpy(fill, cp, (n + 1) * sizeof(int));
+    cr = (int*)malloc((cp[n] > 0 ? cp[n] : 1) * sizeof(int));
+    cv = (double*)malloc((cp[n] > 0 ? cp[n] : 1) * sizeof(double));
+    for (i = 0; i < n; i++) {
+        int l0 = nl;
+        double dv = 0.0, s;
+        lrp[i] = nl;
+        for (p = rp[i]; p < rp[i + 1]; p++) {
+            j = ci[p];
+            if (j <= i) {
+                s = v[p];
+                s -= pair_dot(lci + l0, lv + l0, nl - l0, cr + cp[j], cv + cp[j], fill[j] - cp[j], j - 1);
+                if (j < i) {
+                    lci[nl] = j;
+                    lv[nl] = s;
+                    nl++;
+                } else
+                    dv = 0.0 + s;
+            } else {
+                s = v[p];
+                s -= pair_dot(lci + l0, lv + l0, nl - l0, cr + cp[j], cv + cp[j], fill[j] - cp[j], i - 1);
+                if (dv == 0) {
+                    free(cp); free(fill); free(cr); free(cv);
+                    return 0;
+                }
+                s /= dv;
+                cr[fill[j]] = i;
+                cv[fill[j]] = s;
+                fill[j]++;
+            }
+        }
+        lci[nl] = i;
+        lv[nl] = dv;
+        nl++;
+    }
+    lrp[n] = nl;
+    /* transpose: U row i = (i, 1.0) then (j, u_ij) by increasing j */
+    pos = (int*)malloc((n + 1) * sizeof(int));
+    urp[0] = 0;
+    for (i = 0; i < n; i++) {
+        int up = 0;
+        for (p = rp[i]; p < rp[i + 1]; p++) up += ci[p] > i;
+        urp[i + 1] = urp[i] + 1 + up;
+        uci[urp[i]] = i;
+        uv[urp[i]] = 0.0 + 1.0;
+        pos[i] = urp[i] + 1;
+    }
+    for (j = 0; j < n; j++)
+        for (p = cp[j]; p < cp[j + 1]; p++) {
+            uci[pos[cr[p]]] = j;
+            uv[pos[cr[p]]] = cv[p];
+            pos[cr[p]]++;
+        }
+    free(cp); free(fill); free(cr); free(cv); free(pos);
+    return 1;
+}
+
+typedef struct {
+    int n;
+    const int *lrp, *lci, *urp, *uci;
+    const double *lv, *uv;
+    double* y;
+} IluFactors;
+
+/* PCG_preconlr (PCGS.c:102-108): w = U^-1 L^-1 z by PCG_preconl (forward,
+ * L's diagonal last in each row, PCGS.c:62-78) and PCG_preconr (backward,
+ * U's diagonal first, PCGS.c:80-100) */
+static void ilu_apply(IluFactors* F, const double* z, double* w) {
+    int i, p;
+    double s;
+    for (i = 0; i < F->n; i++) {
+        int last = F->lrp[i + 1] - 1;
+        s = z[i];
+        for (p = F->lrp[i]; p < last; p++) s -= F->lv[p] * F->y[F->lci[p]];
+        s /= F->lv[last];
+        F->y[i] = s;
+    }
+    for (i = F->n - 1; i >= 0; i--) {
+        int first = F->urp[i];
+        s = F->y[i];
+        for (p = first + 1; p < F->urp[i + 1]; p++) s -= F->uv[p] * w[F->uci[p]];
+        s /= F->uv[first];
+        w[i] = s;
+    }
+}
+
+/* ----------------------------------------------------------- GMRES(m)+ILU */
+
+/* the plane rotation that zeroes t2 below t1 (gmres_qrfact, gmres.c:521-527
+ * and 539-545); returns the rotated diagonal c t1 - s t2 */
+static double givens(double t1, double t2, double* c, double* s) {
+    double t;
+    if (t2 == 0.0) {
+        *c = 1.0;
+        *s = 0.0;
+    } else if (fabs(t2) >= fabs(t1)) {
+        t = t1 / t2;
+        *s = -1.0 / sqrt(1.0 + t * t);
+        *c = -*s * t;
+    } else {
+        t = t2 / t1;
+        *c = 1.0 / sqrt(1.0 + t * t);
+        *s = -*c * t;
+    }
+    return *c * t1 - *s * t2;
+}
+
+/* gmres_qrfact with ijob 2 (gmres.c:530-548), or ijob 1 for k == 1: column
+ * k-1 of the Hessenberg matrix (column-major, leading dimension ld) is rotated
+ * by the k-1 earlier rotations, then a new one zeroes its subdiagonal.
+ * Returns nonzero when the new diagonal is zero (never for k == 1, as the
+ * reference's info = k there is 0). */
+static int qr_add_column(int k, int ld, double* a, double* q) {
+    int r;
+    double* col = a + (size_t)ld * (k - 1);
+    double t1, t2;
+    for (r = 0; r < k - 1; r++) {
+        double c = q[2 * r], s = q[2 * r + 1];
+        t1 = col[r];
+        t2 = col[r + 1];
+        col[r] = c * t1 - s * t2;
+        col[r + 1] = s * t1 + c * t2;
+    }
+    col[k - 1] = givens(col[k - 1], col[k], &q[2 * (k - 1)], &q[2 * (k - 1) + 1]);
+    return col[k - 1] == 0.0 && k > 1;
+}
+
+/* gmres_qrsolv (gmres.c:550-574): rotate the right-hand side, then solve the
+ * triangle column by column from the last */
+static void qr_solve(int k, int ld, const double* a, const double* q, double* z) {
+    int r, i;
+    for (r = 0; r < k; r++) {
+        double c = q[2 * r], s = q[2 * r + 1], t1 = z[r], t2 = z[r + 1];
+        z[r] = c * t1 - s * t2;
+        z[r + 1] = s * t1 + c * t2;
+    }
+    for (r = k - 1; r >= 0; r--) {
+        double t;
+        z[r] /= a[r + (size_t)ld * r];
+        t = -z[r];
+        for (i = 0; i < r; i++) z[i] += t * a[i + (size_t)ld * r];
+    }
+}
+
+static double norm2(const double* x, int n) {
+    int i;
+    double s = 0.0;
+    for (i = 0; i < n; i++) s += x[i] * x[i];
+    return sqrt(s);
+}
+
+/* FEM.c's GMRES path (FEM.c:591-611, 730-745): ILU(0) factors of A, then the
+ * ZIB gmres (gmres.c:308-502) on A M^-1 y = b with preconr = PCG_preconlr,
+ * preconl = identity, matvec = PCG_mult, CheckOnRestart, from y = x; on return
+ * x = M^-1 y.  Parameter checks as itlin_parcheck_and_print (utils.c:345-365):
+ * tol <= 0 is an error (-1 returned), tol is clamped to [1e-15, 0.1];
+ * max_iter <= 0 means 100.  Returns info.iter (Arnoldi steps); *eta is
+ * info.precision; -2 for a zero pivot in ILU(0), -3 when gmres_qrfact fails
+ * (x is then M^-1 of the iterate reached at the last restart). */
+int orc_gmres_ilu(int n, const int* rp, const int* ci, const double* v, double* x, const double* b, int m,
+                  double tol, int max_iter, double* eta_out) {
+    int nnz = rp[n], i, j, l, iter = 0, fail = 0;
+    IluFactors F;
+    int *lrp, *lci, *urp, *uci;
+    double *lv, *uv, *V, *h, *z, *Hi, *r, *y0, *w, *q, *y;
+    double beta, beta0 = 0, eta = 0, s;
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    const int ld = m + 1;
+    if (tol <= 0 || n <= 0 || m <= 0) return -1;
+    if (tol > 1.0e-1) tol = 1.0e-1;
+    else if (tol < 1.0e-15) tol = 1.0e-15;
+    if (max_iter <= 0) max_iter = 100;
+    lrp = (int*)malloc((n + 1) * sizeof(int));
+    urp = (int*)malloc((n + 1) * sizeof(int));
+    lci = (int*)malloc((nnz + n) * sizeof(int));
+    uci = (int*)malloc((nnz + n) * sizeof(int));
+    lv = (double*)malloc((nnz + n) * sizeof(double));
+    uv = (double*)malloc((nnz + n) * sizeof(double));
+    if (!orc_ilu0(n, rp, ci, v, lrp, lci, lv, urp, uci, uv)) {
+        free(lrp); free(urp); free(lci); free(uci); free(lv); free(uv);
+        return -2;
+    }
+    F.n = n; F.lrp = lrp; F.lci = lci; F.lv = lv; F.urp = urp; F.uci = uci; F.uv = uv;
+    F.y = (double*)malloc(N * sizeof(double));
+    V = (double*)calloc((size_t)(m + 2) * N, sizeof(double));
+    h = (double*)calloc(m + 1, sizeof(double));
+    z = (double*)calloc(m + 1, sizeof(double));
+    Hi = (double*)calloc((size_t)(m + 1) * (m + 1), sizeof(double));
+    q = (double*)calloc(2 * (m + 1), sizeof(double));
+    r = (double*)malloc(N * sizeof(double));
+    y0 = (double*)malloc(N * sizeof(double));
+    w = (double*)malloc(N * sizeof(double));
+    y = (double*)malloc(N * sizeof(double));
+    memcpy(y, x, n * sizeof(double));
+    for (;;) {
+        /* restart: r = b - A M^-1 y */
+        memcpy(y0, y, n * sizeof(double));
+        ilu_apply(&F, y, V);
+        orc_spmv(n, rp, ci, v, V, w);
+        for (j = 0; j < n; j++) r[j] = b[j] - w[j];
+        beta = norm2(r, n);
+        if (iter == 0) {
+            beta0 = beta;
+            eta = beta0 == 0.0 ? 0.0 : 1.0;
+        } else
+            eta = beta / beta0;
+        for (j = 0; j < n; j++) V[j] = r[j] / beta;
+        for (i = 1; i <= m && eta > tol; i++) {
+            double *vi = V + (size_t)(i - 1) * N, *vn = V + (size_t)i * N, nv;
+            /* Arnoldi step, classical Gram-Schmidt against r = A M^-1 v_{i-1} */
+            ilu_apply(&F, vi, vn);
+            orc_spmv(n, rp, ci, v, vn, w);
+            memcpy(r, w, n * sizeof(double));
+            for (l = 0; l < i; l++) {
+                const double* vl = V + (size_t)l * N;
+                s = 0.0;
+                for (j = 0; j < n; j++) s += vl[j] * r[j];
+                h[l] = s;
+            }
+            for (j = 0; j < n; j++) {
+                s = 0.0;
+                for (l = 0; l < i; l++) s += V[(size_t)l * N + j] * h[l];
+                vn[j] = r[j] - s;
+            }
+            nv = norm2(vn, n);
+            for (j = 0; j < n; j++) vn[j] /= nv;
+            for (l = 0; l < i; l++) Hi[(size_t)ld * (i - 1) + l] = h[l];
+            Hi[(size_t)ld * (i - 1) + i] = nv;
+            if (qr_add_column(i, ld, Hi, q)) {
+                fail = 1;
+                break;
+            }
+            if (i == m) {
+                /* least-squares update at the cycle's end */
+                for (l = 0; l < i + 1; l++) z[l] = 0.0;
+                z[0] = beta;
+                qr_solve(i, ld, Hi, q, z);
+                for (j = 0; j < n; j++) {
+                    s = 0.0;
+                    for (l = 0; l < i; l++) s += V[(size_t)l * N + j] * z[l];
+                    y[j] = y0[j] + s;
+                }
+            }
+            iter++;
+        }
+        if (fail || !(eta > tol && iter < max_iter)) break;
+    }
+    ilu_apply(&F, y, x);
+    if (eta_out) *eta_out = eta;
+    free(lrp); free(urp); free(lci); free(uci); free(lv); free(uv); free(F.y);
+    free(V); free(h); free(z); free(Hi); free(q); free(r); free(y0); free(w); free(y);
+    return fail ? -3 : iter;
+}
+
+/* -------------------------------------------------------- BiCGStab + AMG */
+
+/* BiCGStab_solve (BiCGStab.c:115-198) with Precon = BI_AMG_precon
+ * (BiCGStab.c:79-81: AMG_solve warm-started from the previous PP / PS) and
+ * Mat_mult = the caller's matrix (rp, ci, v).  The hierarchy h is built as
+ * BI_set_AMG_preconditioner does (BiCGStab.c:98-113: smiter 2, aggressive
+ * coarsening, matrix tolerance 0.1) and AMG_solve stops at amg_eps / amg_max_iter
+ * (1e-4 / 20 there).  Returns the iterations; *res = |R| / |R0| at the end
+ * (0 when R0 = 0). */
+int orc_bicgstab_amg(orc_amg* hh, int n, const int* rp, const int* ci, const double* v, double* X, const double* b,
+                     int sm, double w, double amg_eps, int amg_max_iter, double eps, int max_iter, double* res) {
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    double *S = (double*)calloc(N, sizeof(double)), *PS = (double*)calloc(N, sizeof(double));
+    double *PP = (double*)calloc(N, sizeof(double)), *APP = (double*)calloc(N, sizeof(double));
+    double *APS = (double*)calloc(N, sizeof(double)), *R = (double*)calloc(N, sizeof(double));
+    double *R0 = (double*)malloc(N * sizeof(double)), *P = (double*)malloc(N * sizeof(double));
+    double hdot, t, s, r = 0, r0, alpha, beta, omega;
+    int i, iter = 0;
+    /* R = -(A X - b) */
+    orc_spmv(n, rp, ci, v, X, R);
+    for (i = 0; i < n; i++) R[i] += -1. * b[i];
+    for (i = 0; i < n; i++) R[i] *= -1;
+    memcpy(R0, R, n * sizeof(double));
+    memcpy(P, R, n * sizeof(double));
+    r0 = sqrt(dot(R, R, n));
+    if (r0 != 0) {
+        do {
+            orc_amg_solve(hh, P, PP, sm, w, amg_eps, amg_max_iter);
+            orc_spmv(n, rp, ci, v, PP, APP);
+            hdot = dot(R, R0, n);
+            t = dot(APP, R0, n);
+            if (t == 0) break;
+            alpha = hdot / t;
+            memcpy(S, R, n * sizeof(double));
+            for (i = 0; i < n; i++) S[i] += -alpha * APP[i];
+            orc_amg_solve(hh, S, PS, sm, w, amg_eps, amg_max_iter);
+            orc_spmv(n, rp, ci, v, PS, APS);
+            s = dot(APS, APS, n);
+            if (s == 0) break;
+            omega = dot(APS, S, n) / s;
+            for (i = 0; i < n; i++) X[i] += alpha * PP[i];
+            for (i = 0; i < n; i++) X[i] += omega * PS[i];
+            memcpy(R, S, n * sizeof(double));
+            for (i = 0; i < n; i++) R[i] += -omega * APS[i];
+            beta = dot(R, R0, n) * alpha / (hdot * omega);
+            for (i = 0; i < n; i++) P[i] += -omega * APP[i];
+            for (i = 0; i < n; i++) P[i] *= beta;
+            for (i = 0; i < n; i++) P[i] += 1. * R[i];
+            r = sqrt(dot(R, R, n));
+            iter++;
+        } while (r / r0 > eps && iter <= max_iter);
+    }
+    *res = r0 != 0 ? r / r0 : 0;
+    free(S); free(PS); free(PP); free(APP); free(APS); free(R); free(R0); free(P);
+    return iter;
+}
+
+/* pcg (PCGS.c:163-233) as PCG_default sets it up (PCGS.c:29-34): operator
+ * PCG_mult on A, preconditioner PCG_preconlr with the ILU(0) factors of A.
+ * Returns iterations (-2 for a zero pivot); *res = get_pcg_residuum.  B and X
+ * are scaled by 1/|B| and back in place, as the reference does (and left
+ * scaled when the preconditioned initial residual is zero). */
+int orc_pcg_ilu(int n, const int* rp, const int* ci, const double* v, double* X, double* B, double tol, int max_iter,
+                double* res) {
+    int nnz = rp[n], i, it = 0;
+    IluFactors F;
+    int *lrp, *lci, *urp, *uci;
+    double *lv, *uv, *R, *P, *W, *Q, *Z;
+    double hn, s = 0, s_old, a, b, r, r0;
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    lrp = (int*)malloc((n + 1) * sizeof(int));
+    urp = (int*)malloc((n + 1) * sizeof(int));
+    lci = (int*)malloc((nnz + n) * sizeof(int));
+    uci = (int*)malloc((nnz + n) * sizeof(int));
+    lv = (double*)malloc((nnz + n) * sizeof(double));
+    uv = (double*)malloc((nnz + n) * sizeof(double));
+    if (!orc_ilu0(n, rp, ci, v, lrp, lci, lv, urp, uci, uv)) {
+        free(lrp); free(urp); free(lci); free(uci); free(lv); free(uv);
+        return -2;
+    }
+    F.n = n; F.lrp = lrp; F.lci = lci; F.lv = lv; F.urp = urp; F.uci = uci; F.uv = uv;
+    F.y = (double*)malloc(N * sizeof(double));
+    W = (double*)calloc(N, sizeof(double));
+    Q = (double*)calloc(N, sizeof(double));
+    Z = (double*)calloc(N, sizeof(double));
+    P = (double*)calloc(N, sizeof(double));
+    R = (double*)malloc(N * sizeof(double));
+    hn = sqrt(dot(B, B, n));
+    if (hn == 0) hn = 1;
+    for (i = 0; i < n; i++) B[i] *= (double)1 / hn;
+    for (i = 0; i < n; i++) X[i] *= (double)1 / hn;
+    memcpy(R, B, n * sizeof(double));
+    orc_spmv(n, rp, ci, v, X, Q);
+    for (i = 0; i < n; i++) R[i] += -1. * Q[i];
+    ilu_apply(&F, R, W);
+    r0 = sqrt(dot(W, W, n));
+    if (r0 != 0) {
+        do {
+            ilu_apply(&F, R, Z);
+            s_old = s;
+            s = dot(R, Z, n);
+            if (it > 0) {
+                b = s / s_old;
+                for (i = 0; i < n; i++) P[i] *= b;
+                for (i = 0; i < n; i++) P[i] += 1 * Z[i];
+            } else
+                memcpy(P, Z, n * sizeof(double));
+            orc_spmv(n, rp, ci, v, P, Q);
+            a = s / dot(P, Q, n);
+            for (i = 0; i < n; i++) X[i] += a * P[i];
+            for (i = 0; i < n; i++) R[i] += -a * Q[i];
+            r = sqrt(dot(R, R, n));
+            it++;
+        } while (r / r0 > tol && it < max_iter);
+        ilu_apply(&F, R, W);
+        r = sqrt(dot(W, W, n));
+        for (i = 0; i < n; i++) B[i] *= hn;
+        for (i = 0; i < n; i++) X[i] *= hn;
+        *res = r / r0;
+    } else
+        *res = 0;
+    free(lrp); free(urp); free(lci); free(uci); free(lv); free(uv); free(F.y);
+    free(W); free(Q); free(Z); free(P); free(R);
+    return it;
+}
+
+/* PCG_preconlr (PCGS.c:102-108) with given factors: w = U^-1 L^-1 z */
+void orc_ilu_apply(int n, const int* lrp, const int* lci, const double* lv, const int* urp, const int* uci,
+                   const double* uv, const double* z, double* w) {
+    IluFactors F;
+    F.n = n; F.lrp = lrp; F.lci = lci; F.lv = lv; F.urp = urp; F.uci = uci; F.uv = uv;
+    F.y = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    ilu_apply(&F, z, w);
+    free(F.y);
+}

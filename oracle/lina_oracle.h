@@ -46,6 +46,33 @@ int orc_colouring(int n, const int* rp, const int* ci, const double* v, int* col
 /* y = A x with sequential row sums (PCG_mult, PCGS.c:36-59) */
 void orc_spmv(int n, const int* rp, const int* ci, const double* v, const double* x, double* y);
 
+/* incomplete_LU_factorization (linear_algebra.c:1958-1984): L (lower, its
+ * diagonal last in each row) and U (unit diagonal first, then the strict
+ * upper part) on A's pattern; lci/lv and uci/uv hold nnz(A) + n entries.
+ * Returns 1, or 0 at a zero pivot. */
+int orc_ilu0(int n, const int* rp, const int* ci, const double* v, int* lrp, int* lci, double* lv, int* urp,
+             int* uci, double* uv);
+
+/* FEM.c's GMRES(m) path: ILU(0), gmres (gmres.c:308-502) right-preconditioned,
+ * CheckOnRestart, x = M^-1 y on return.  Returns Arnoldi steps (negative on
+ * error), *eta = residual reduction at the last restart. */
+int orc_gmres_ilu(int n, const int* rp, const int* ci, const double* v, double* x, const double* b, int m,
+                  double tol, int max_iter, double* eta);
+
+/* BiCGStab_solve (BiCGStab.c:115-198) preconditioned by AMG_solve on h,
+ * operator (rp, ci, v).  Returns iterations; *res = |R|/|R0|. */
+int orc_bicgstab_amg(orc_amg* h, int n, const int* rp, const int* ci, const double* v, double* X, const double* b,
+                     int sm, double w, double amg_eps, int amg_max_iter, double eps, int max_iter, double* res);
+
+/* pcg (PCGS.c:163-233) with PCG_default's ILU(0) preconditioner; *res =
+ * get_pcg_residuum.  Returns iterations, -2 for a zero pivot. */
+int orc_pcg_ilu(int n, const int* rp, const int* ci, const double* v, double* X, double* B, double tol, int max_iter,
+                double* res);
+
+/* PCG_preconlr (PCGS.c:102-108): w = U^-1 L^-1 z with orc_ilu0's factors */
+void orc_ilu_apply(int n, const int* lrp, const int* lci, const double* lv, const int* urp, const int* uci,
+                   const double* uv, const double* z, double* w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,7 +10,14 @@ inputs and outputs as compressed .npz next to this script:
 * AMG_solve results for each smoother from a zero initial guess;
 * pcg with the reference preconditioner (PCG_set_AMG_preconditioner) and
   with the symmetric multicolour V-cycle, including the in-place rounding of
-  B and get_pcg_residuum.
+  B and get_pcg_residuum;
+* krylov.npz: ILU(0) factors (incomplete_LU_factorization), pcg with
+  PCG_default's ILU(0) preconditioner, FEM.c's
+  GMRES(30)+ILU(0) path, pcg with one V-cycle of the level-dependent
+  smoother (MC-SSOR finest, l1-Jacobi below) and BiCGStab with the
+  BI_set_AMG_preconditioner hierarchy, on three small systems.
+
+    python tests/golden/make_golden.py krylov    # krylov.npz only
 
 The reference is only run here; the fixtures are data (inputs and outputs).
 """
@@ -38,8 +45,46 @@ CASES = [
 SMOOTHERS = [("sor", 1.6), ("jacobi", 1.0), ("mc_sor", 1.2), ("l1_jacobi", 1.0), ("mc_ssor", 1.0), ("ssor", 1.0)]
 
 
+KRYLOV = [
+    # name, system builder, right-hand side seed
+    ("poisson17", lambda: P.poisson(17), 1),
+    ("damage17", lambda: P.damage_system(17)[0], 2),
+    ("elast17", lambda: P.degraded_elasticity(17), 3),
+]
+
+
+def krylov(ref):
+    out = {}
+    for name, build, seed in KRYLOV:
+        s = build()
+        b = np.random.default_rng(seed).standard_normal(s.n)
+        out.update({f"{name}_rowptr": s.rowptr, f"{name}_col": s.col, f"{name}_val": s.val, f"{name}_b": b,
+                    f"{name}_dof": s.dof})
+        M = ref.matrix(s.rowptr, s.col, s.val)
+        ok, L, U = ref.ilu(M)
+        assert ok
+        for w, F in (("L", L), ("U", U)):
+            out.update({f"{name}_ilu{w}_rp": F.rowptr, f"{name}_ilu{w}_ci": F.col, f"{name}_ilu{w}_v": F.val})
+        c = ref.pcg_ilu(M, b, tol=1e-10, max_iter=2000)
+        out.update({f"{name}_pcgilu_iters": c["iters"], f"{name}_pcgilu_x": c["x"], f"{name}_pcgilu_res": c["res"]})
+        g = ref.gmres_ilu(M, b, m=30, tol=1e-8, max_iter=600)
+        out.update({f"{name}_gmres_iters": g["iters"], f"{name}_gmres_x": g["x"], f"{name}_gmres_eta": g["eta"]})
+        h = ref.pcg_amg(M, b, dof=s.dof, depth=4, smiter=1, aggr_smiter=1, smoother="hybrid", alpha=1.0,
+                        amg_eps=1e-2, amg_max_iter=1, zero_init=True, tol=1e-8, max_iter=2000, coarse_sweeps=1)
+        out.update({f"{name}_pcghyb_iters": h["iters"], f"{name}_pcghyb_x": h["x"], f"{name}_pcghyb_res": h["res"]})
+        r = ref.bicgstab_amg(M, b, dof=s.dof, depth=4, smoother="sor", alpha=1.0, eps=1e-8, max_iter=300)
+        out.update({f"{name}_bicg_iters": r["iters"], f"{name}_bicg_x": r["x"]})
+        ref.free(M)
+    path = os.path.join(HERE, "krylov.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path) // 1024} KiB)")
+
+
 def main():
     ref = Ref()
+    if sys.argv[1:] == ["krylov"]:
+        krylov(ref)
+        return
     for name, build, depth, coarsening in CASES:
         s = build()
         out = {"rowptr": s.rowptr, "col": s.col, "val": s.val, "b": s.b, "dof": s.dof, "depth": depth,
@@ -70,6 +115,7 @@ def main():
         path = os.path.join(HERE, f"{name}.npz")
         np.savez_compressed(path, **out)
         print(f"wrote {path} ({os.path.getsize(path) // 1024} KiB)")
+    krylov(ref)
 
 
 if __name__ == "__main__":

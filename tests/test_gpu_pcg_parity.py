@@ -1,14 +1,16 @@
 """GPU parity of the FP64 AMG-PCG path against the reference LinA.
 
-The reference (oracle/_ref, compiled from LinA's own sources) and the HIP
-path solve the identical CSR system; iteration counts must agree to +-1 and
-iterates to 1e-12 relative, the hierarchy being
-bit-identical (tests/test_setup_cpu.py).
+The oracle (oracle/liboracle.so, the C restatement pinned bit for bit to the
+reference compiled from LinA's own sources by tests/test_oracle_cpu.py and
+tests/test_oracle_krylov_cpu.py) and the HIP path solve the identical CSR
+system; iteration counts must agree to +-1 and iterates to 1e-12 relative,
+the hierarchy being bit-identical (tests/test_setup_cpu.py).
 """
 import numpy as np
 import pytest
 
 from fem_amd import problems as P
+from oracle.bind import Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -81,7 +83,6 @@ def test_matrix_handle_spmv(gpu, case):
     workgroup-wide path), a 1x1 matrix, and column layouts that take each
     path of the 16-bit column codes (one base, two bases, 32-bit fallback:
     elast257's blocked [ux; uy] ordering and the synthetic clusters)."""
-    from oracle.bind import Oracle
     say("matrix spmv", case)
     rng = np.random.default_rng(7)
     if case in ("elast65", "elast257"):
@@ -112,24 +113,22 @@ def test_matrix_handle_spmv(gpu, case):
     m.close()
 
 
-def _ref_solve(ref, s, depth, **kw):
-    M = ref.matrix(s.rowptr, s.col, s.val)
-    out = ref.pcg_amg(M, s.b, dof=s.dof, depth=depth, **kw)
-    ref.free(M)
-    return out
+def _ref_solve(s, depth, **kw):
+    """pcg of the oracle (pinned bit for bit to the compiled reference)"""
+    return Oracle().pcg_amg(s.rowptr, s.col, s.val, s.b, dof=s.dof, depth=depth, **kw)
 
 
 @pytest.mark.parametrize("m", [17, 33, 65])
-def test_pcg_reference_preconditioner_poisson(gpu, ref, m):
+def test_pcg_reference_preconditioner_poisson(gpu, m):
     """PCG_set_AMG_preconditioner: lexicographic SOR(1.6), AMG solve to 1e-2."""
     s = P.poisson(m)
     depth = 4
     say("pcg reference-mode poisson", m)
-    r = _ref_solve(ref, s, depth, smoother="sor", alpha=1.6, smiter=2, aggr_smiter=3, amg_eps=1e-2,
+    r = _ref_solve(s, depth, smoother="sor", alpha=1.6, smiter=2, aggr_smiter=3, amg_eps=1e-2,
                    amg_max_iter=100, tol=1e-10, max_iter=500)
     amg = gpu.AMG(s.rowptr, s.col, s.val, gpu.params("reference", 1, depth))
     g = amg.pcg(s.b, tol=1e-10, max_iter=500)
-    say("  iters ref", r["iters"], "gpu", g["iters"], "vcycles ref", r["vcycles"].sum(), "gpu", g["vcycles"])
+    say("  iters oracle", r["iters"], "gpu", g["iters"], "vcycles oracle", r["vcycles"], "gpu", g["vcycles"])
     assert abs(g["iters"] - r["iters"]) <= 1
     assert rel(g["x"], r["x"]) < 1e-12
     assert np.array_equal(g["b_out"], r["b_out"]) or rel(g["b_out"], r["b_out"]) < 1e-15
@@ -138,34 +137,34 @@ def test_pcg_reference_preconditioner_poisson(gpu, ref, m):
 
 
 @pytest.mark.parametrize("kind,m", [("poisson", 33), ("elasticity", 17), ("elasticity", 33)])
-def test_pcg_vcycle_mc_ssor(gpu, ref, kind, m):
+def test_pcg_vcycle_mc_ssor(gpu, kind, m):
     """One symmetric multicolour-SOR V(1,1) per iteration; the reference runs
     the same cycle with the smoother plugged into AMG_set_smoother."""
     s = P.poisson(m) if kind == "poisson" else P.degraded_elasticity(m)
     depth = 4
     say("pcg vcycle-mode", kind, m)
-    r = _ref_solve(ref, s, depth, smoother="mc_ssor", alpha=1.0, smiter=1, aggr_smiter=1, amg_eps=1e-2,
+    r = _ref_solve(s, depth, smoother="mc_ssor", alpha=1.0, smiter=1, aggr_smiter=1, amg_eps=1e-2,
                    amg_max_iter=1, zero_init=True, tol=1e-8, max_iter=2000)
     amg = gpu.AMG(s.rowptr, s.col, s.val, gpu.params("vcycle", s.dof, depth, coarse_smoother=-1))
     g = amg.pcg(s.b, tol=1e-8, max_iter=2000)
-    say("  iters ref", r["iters"], "gpu", g["iters"], "rel", rel(g["x"], r["x"]))
+    say("  iters oracle", r["iters"], "gpu", g["iters"], "rel", rel(g["x"], r["x"]))
     assert abs(g["iters"] - r["iters"]) <= 1
     assert rel(g["x"], r["x"]) < 1e-12
 
 
 @pytest.mark.parametrize("kind,m", [("poisson", 65), ("elasticity", 33), ("elasticity", 65)])
-def test_pcg_vcycle_default(gpu, ref, kind, m):
+def test_pcg_vcycle_default(gpu, kind, m):
     """The default V-cycle: MC-SSOR on the finest level, one l1-Jacobi sweep
     below (multi-lane rows on the dense coarse levels)."""
     s = P.poisson(m) if kind == "poisson" else P.degraded_elasticity(m)
     depth = 5
     say("pcg vcycle default", kind, m)
-    r = _ref_solve(ref, s, depth, smoother="hybrid", alpha=1.0, smiter=1, aggr_smiter=1, amg_eps=1e-2,
+    r = _ref_solve(s, depth, smoother="hybrid", alpha=1.0, smiter=1, aggr_smiter=1, amg_eps=1e-2,
                    amg_max_iter=1, zero_init=True, tol=1e-8, max_iter=3000, coarse_sweeps=1)
     for prec in ("fp64", "mixed"):
         amg = gpu.AMG(s.rowptr, s.col, s.val, gpu.params("vcycle", s.dof, depth, precision=prec))
         g = amg.pcg(s.b, tol=1e-8, max_iter=3000)
-        say("  ", prec, "iters ref", r["iters"], "gpu", g["iters"], "rel", rel(g["x"], r["x"]))
+        say("  ", prec, "iters oracle", r["iters"], "gpu", g["iters"], "rel", rel(g["x"], r["x"]))
         if prec == "fp64":
             assert abs(g["iters"] - r["iters"]) <= 1
             assert rel(g["x"], r["x"]) < 1e-12
@@ -179,20 +178,19 @@ def test_pcg_vcycle_default(gpu, ref, kind, m):
 
 
 @pytest.mark.parametrize("smoother", ["mc_sor", "l1_jacobi", "ssor"])
-def test_amg_solve_other_smoothers(gpu, ref, smoother):
+def test_amg_solve_other_smoothers(gpu, smoother):
     s = P.poisson(33)
     depth = 4
     say("amg_solve", smoother)
-    M = ref.matrix(s.rowptr, s.col, s.val)
-    I = ref.amg_setup(M, dof=1, depth=depth)
+    orc = Oracle()
+    I = orc.amg_setup(s.rowptr, s.col, s.val, dof=1, depth=depth)
     x0 = np.zeros(s.n)
-    k_ref, x_ref = ref.amg_solve(I, s.b, x0, smoother=smoother, alpha=1.2, eps=1e-6, max_iter=30)
-    ref.amg_free(I)
-    ref.free(M)
+    k_ref, x_ref = orc.amg_solve(I, s.b, x0, smoother=smoother, alpha=1.2, eps=1e-6, max_iter=30)
+    orc.amg_free(I)
     amg = gpu.AMG(s.rowptr, s.col, s.val, gpu.params("reference", 1, depth, smoother=smoother, omega=1.2,
                                                       amg_eps=1e-6, amg_max_iter=30))
     k, x = amg.amg_solve(s.b, x0)
-    say("  vcycles ref", k_ref, "gpu", k)
+    say("  vcycles oracle", k_ref, "gpu", k)
     assert k == k_ref
     assert rel(x, x_ref) < 1e-12
 
