@@ -46,8 +46,22 @@ class IluSystem {
     int levels_l() const { return nlv_l_; }
     int levels_u() const { return nlv_u_; }
 
+    // a factor packed by level for dev::tri_solve_pair
+    struct TriDev {
+        dev::TriPack p;
+        int nov = 0;
+        int *vmap = nullptr, *dmap = nullptr, *ovmap = nullptr;
+        void* rec = nullptr;
+        double* ovval = nullptr;
+    };
+
   private:
     void sync_factors();
+    void pack_values();
+    bool packed_ = false;          // tf_/tb_ hold the level packs (else the k_trsv sweeps)
+    TriDev tf_, tb_;
+    int* permL_ = nullptr;         // row of each forward phase position
+    double *vZp_ = nullptr, *vYu_ = nullptr;
     Csr A_, L_, U_;
     int ilu_ok_ = 0;
     bool device_ready_ = false;    // refactor's symbolic structure is uploaded for A_'s pattern

@@ -778,6 +778,191 @@ __global__ __launch_bounds__(1024) void k_trsv(const int* __restrict__ rp, const
     }
 }
 
+// The same two sweeps with the factors packed by level (TriPack).  One row
+// per thread; each level's row records are loaded D = 2 levels ahead into
+// registers and stay in flight across the barriers, which fence LDS only: the
+// results a level reads come from the LDS ring, so a level costs an LDS round
+// trip, the row's arithmetic and the barrier, and the record stream (about
+// 72 bytes a row) is bounded by what one compute unit can load -- measured on
+// the 1.05M-row damage factors, 2,049 levels a sweep: 1.35 us a level, one
+// preconditioner application 5.5 ms (11.3 ms a sweep with k_trsv).  Deeper
+// prefetch (4 levels) measured no faster.  The arithmetic is k_trsv's:
+// s = in; s -= v_k x_k in the factor's row order; s /= diag.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// the loaded values are only stored here (nothing computed from them), so
+// that the loads stay in flight until the row is solved
+template <int MAXP>
+struct TriRow {
+    TriRec<MAXP> r;
+    double in;
+    bool ok;
+};
+
+// row t of the level ending at e (rows at or past e: ok = false, loads
+// clamped to row 0); only stored, nothing computed from the loaded values,
+// so that the loads stay in flight until the row is solved
+template <int MAXP>
+__device__ __forceinline__ void tri_load(TriRow<MAXP>& r, const TriPack& a, const double* in, int t, int e) {
+    r.ok = t < e;
+    const int tt = r.ok ? t : 0;
+    r.r = static_cast<const TriRec<MAXP>*>(a.rec)[tt];
+    r.in = in[tt];
+}
+
+// a level whose sources all lie in the ring: unconditional LDS reads (padding
+// slots hold value 0 and read the zero slot TRI_RING, s - 0 * 0 = s)
+template <int MAXP>
+__device__ __forceinline__ void tri_row_lds(const TriRow<MAXP>& row, const TriPack& a, double* ring, double* out,
+                                            int t) {
+    if (!row.ok) return;
+    const TriRec<MAXP>& r = row.r;
+    double s = row.in;
+#pragma unroll
+    for (int k = 0; k < MAXP; k++) s -= r.val[k] * ring[r.src[k]];
+    if (r.cnt > MAXP) {
+        const int o = a.ovp[t] - MAXP;
+        for (int k = MAXP; k < r.cnt; k++) s -= a.ovval[o + k] * ring[a.ovsrc[o + k]];
+    }
+    s /= r.diag;
+    ring[t & (TRI_RING - 1)] = s;
+    out[r.orow] = s;
+}
+
+// a level that also reads results through global memory
+template <int MAXP>
+__device__ __forceinline__ void tri_row_any(const TriRow<MAXP>& row, const TriPack& a, double* ring, double* out,
+                                            int t) {
+    if (!row.ok) return;
+    const TriRec<MAXP>& r = row.r;
+    double s = row.in;
+#pragma unroll
+    for (int k = 0; k < MAXP; k++)
+        if (k < r.cnt) {
+            const int q = r.src[k];
+            const double x = q >= 0 ? ring[q] : out[-q - 1];
+            s -= r.val[k] * x;
+        }
+    if (r.cnt > MAXP) {
+        const int o = a.ovp[t] - MAXP;
+        for (int k = MAXP; k < r.cnt; k++) {
+            const int q = a.ovsrc[o + k];
+            const double x = q >= 0 ? ring[q] : out[-q - 1];
+            s -= a.ovval[o + k] * x;
+        }
+    }
+    s /= r.diag;
+    ring[t & (TRI_RING - 1)] = s;
+    out[r.orow] = s;
+}
+
+// Level offsets are read from LDS (staged TRI_OFFCAP at a time): a global
+// read of them would wait behind the row prefetches in flight.
+constexpr int TRI_OFFCAP = 16384;
+
+template <int MAXP, int D>
+__device__ __forceinline__ void tri_phase(const TriPack& a, const double* in, double* out, double* ring, int* loff) {
+    constexpr int CH = ((TRI_OFFCAP - 1 - D) / D) * D;  // levels per staged window
+    constexpr int M = 0x7fffffff;
+    const int tid = threadIdx.x;
+    for (int q = tid; q < TRI_OFFCAP && q <= a.nlv; q += TRI_NT) loff[q] = a.off[q];
+    __syncthreads();
+    TriRow<MAXP> st[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        // stage d's loads issued before stage d+1's, as in the loop
+        __builtin_amdgcn_sched_barrier(0);
+        int b = 0, e = 0;
+        if (d < a.nlv) {
+            b = loff[d] & M;
+            e = loff[d + 1] & M;
+        }
+        tri_load(st[d], a, in, b + tid, e);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    int c0 = 0;
+    for (int L0 = 0; L0 < a.nlv; L0 += D) {
+        if (L0 - c0 >= CH) {
+            // the previous level's barrier ended every read of the old window
+            c0 = L0;
+            for (int q = tid; q < TRI_OFFCAP && c0 + q <= a.nlv; q += TRI_NT) loff[q] = a.off[c0 + q];
+            lds_barrier();
+        }
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            // keeps the scheduler from hoisting a later stage's address
+            // arithmetic (and so its load waits) above this stage
+            __builtin_amdgcn_sched_barrier(0);
+            const int L = L0 + d;
+            if (L < a.nlv) {
+                const int v0 = loff[L - c0];
+                const int b = v0 & M, e = loff[L + 1 - c0] & M;
+                const bool glob = v0 < 0;
+                if (glob)
+                    tri_row_any(st[d], a, ring, out, b + tid);
+                else
+                    tri_row_lds(st[d], a, ring, out, b + tid);
+                // rows past the first TRI_NT of a wide level: loaded here
+                for (int t = b + TRI_NT + tid; t < e; t += TRI_NT) {
+                    TriRow<MAXP> r;
+                    tri_load(r, a, in, t, e);
+                    if (glob)
+                        tri_row_any(r, a, ring, out, t);
+                    else
+                        tri_row_lds(r, a, ring, out, t);
+                }
+                const int L2 = L + D;
+                int b2 = 0, e2 = 0;
+                if (L2 < a.nlv) {
+                    b2 = loff[L2 - c0] & M;
+                    e2 = loff[L2 + 1 - c0] & M;
+                }
+                tri_load(st[d], a, in, b2 + tid, e2);
+                if (L + 1 < a.nlv && loff[L + 1 - c0] < 0)
+                    __syncthreads();
+                else
+                    lds_barrier();
+            }
+        }
+    }
+}
+
+template <int MAXP, int D>
+__global__ __launch_bounds__(TRI_NT) void k_tri_pair(TriPack fw, TriPack bw, const double* __restrict__ zp,
+                                                      double* y, double* w) {
+    __shared__ double ring[TRI_RING + 1];
+    __shared__ int loff[TRI_OFFCAP];
+    if (threadIdx.x == 0) ring[TRI_RING] = 0.0;
+    tri_phase<MAXP, D>(fw, zp, y, ring, loff);
+    __syncthreads();
+    tri_phase<MAXP, D>(bw, y, w, ring, loff);
+}
+
+__global__ void k_gather_perm(int n, const int* __restrict__ perm, const double* __restrict__ z,
+                              double* __restrict__ zp) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) zp[t] = z[perm[t]];
+}
+
+template <int MAXP>
+__global__ void k_tri_values(int n, const int* __restrict__ vmap, const int* __restrict__ dmap, int nov,
+                             const int* __restrict__ ovmap, const double* __restrict__ v, TriRec<MAXP>* __restrict__ rec,
+                             double* __restrict__ ovval) {
+    const int s = gridDim.x * blockDim.x;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += s) {
+        rec[t].diag = v[dmap[t]];
+#pragma unroll
+        for (int k = 0; k < MAXP; k++) {
+            const int q = vmap[k * n + t];
+            rec[t].val[k] = q >= 0 ? v[q] : 0.0;
+        }
+    }
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nov; q += s) ovval[q] = v[ovmap[q]];
+}
+
 // ILU(0) numeric factorisation (incomplete_LU_factorization,
 // linear_algebra.c:1952-1982, with row_times_row :1795-1819) of A with rows
 // sorted by column, into the fixed patterns L (row i: the lower columns of A's
@@ -1648,6 +1833,28 @@ template void inverse_to_tiles<double>(hipStream_t, int, int, const int*, const 
 void trsv(hipStream_t st, const int* rp, const int* ci, const double* v, int nlv, const int* lv_off,
           const int* lv_rows, const double* z, double* w, int upper) {
     hipLaunchKernelGGL(k_trsv, dim3(1), dim3(1024), 0, st, rp, ci, v, nlv, lv_off, lv_rows, z, w, upper);
+}
+
+void tri_solve_pair(hipStream_t st, const TriPack& fw, const TriPack& bw, const int* perm, const double* z,
+                    double* zp, double* y, double* w) {
+    if (fw.n <= 0) return;
+    hipLaunchKernelGGL(k_gather_perm, dim3(std::min(1024, (fw.n + 255) / 256)), dim3(256), 0, st, fw.n, perm, z, zp);
+    const int maxp = std::max(fw.maxp, bw.maxp);
+    if (maxp > 4)
+        hipLaunchKernelGGL((k_tri_pair<8, 2>), dim3(1), dim3(TRI_NT), 0, st, fw, bw, zp, y, w);
+    else
+        hipLaunchKernelGGL((k_tri_pair<4, 2>), dim3(1), dim3(TRI_NT), 0, st, fw, bw, zp, y, w);
+}
+
+void tri_pack_values(hipStream_t st, int n, int maxp, const int* vmap, const int* dmap, int nov, const int* ovmap,
+                     const double* v, void* rec, double* ovval) {
+    const int m = std::max(n, nov);
+    if (m <= 0) return;
+    const dim3 g(std::min(4096, (m + 255) / 256)), b(256);
+    if (maxp <= 4)
+        hipLaunchKernelGGL(k_tri_values<4>, g, b, 0, st, n, vmap, dmap, nov, ovmap, v, (TriRec<4>*)rec, ovval);
+    else
+        hipLaunchKernelGGL(k_tri_values<8>, g, b, 0, st, n, vmap, dmap, nov, ovmap, v, (TriRec<8>*)rec, ovval);
 }
 
 void ilu0_numeric(hipStream_t st, int nlv, const int* lv_off, const int* lv_rows, const int* rpA, const int* ciA,

@@ -207,6 +207,44 @@ void bcoarse_solve(hipStream_t st, int nd, const int* rblk, const int* off, cons
 // level-scheduled ILU(0) triangular solve (upper=0: L, diagonal last; 1: U, diagonal first)
 void trsv(hipStream_t st, const int* rp, const int* ci, const double* v, int nlv, const int* lv_off,
           const int* lv_rows, const double* z, double* w, int upper);
+// ILU(0) preconditioner w = U^-1 L^-1 z in one workgroup, both factors packed
+// by level: phase order t (level by level), per row t its right-hand side
+// in[t] and one record rec[t] (TriRec<maxp>, 64 or 112 bytes) holding the
+// diagonal, the first maxp off-diagonal values and their sources in the
+// factor's row order, the output index and the off-diagonal count; entries
+// past maxp come from ovval/ovsrc[ovp[t] ...].  A source >= 0 is a slot of
+// the LDS ring of the last TRI_RING results (slot = t & (TRI_RING-1); slot
+// TRI_RING holds 0 and pads the unused value slots, whose values are 0);
+// -(g+1) reads out[g] in global memory (full[L]: level L reads so, and a
+// full barrier precedes it; the flag is bit 31 of off[L]).  Row t's result
+// goes to out[orow].
+constexpr int TRI_NT = 1024;
+constexpr int TRI_RING = 8192;
+template <int MAXP>
+struct alignas(16) TriRec {
+    double diag;
+    double val[MAXP];
+    int src[MAXP];
+    int orow;
+    int cnt;
+};
+static_assert(sizeof(TriRec<4>) == 64 && sizeof(TriRec<8>) == 112, "record layout");
+struct TriPack {
+    int n = 0, nlv = 0, maxp = 0;
+    const int* off = nullptr;     // nlv + 1 level offsets, bit 31: full[L]
+    const void* rec = nullptr;    // n records TriRec<maxp>
+    const int* ovp = nullptr;     // n: first overflow entry of rows with cnt > maxp
+    const double* ovval = nullptr;
+    const int* ovsrc = nullptr;
+};
+// forward phase fw: in = zp (z in L's phase order, gathered by perm), out = y
+// (bw's phase order); backward phase bw: in = y, out = w (natural order)
+void tri_solve_pair(hipStream_t st, const TriPack& fw, const TriPack& bw, const int* perm, const double* z,
+                    double* zp, double* y, double* w);
+// the packed values from the factor's CSR values: rec[t].val[k] =
+// v[vmap[k*n + t]] (vmap < 0: 0), rec[t].diag = v[dmap[t]], ovval[q] = v[ovmap[q]]
+void tri_pack_values(hipStream_t st, int n, int maxp, const int* vmap, const int* dmap, int nov, const int* ovmap,
+                     const double* v, void* rec, double* ovval);
 // level-scheduled ILU(0) numeric factorisation into fixed L/U patterns (see
 // k_ilu0); A's rows sorted by column; *fail = 1 on a zero pivot
 void ilu0_numeric(hipStream_t st, int nlv, const int* lv_off, const int* lv_rows, const int* rpA, const int* ciA,

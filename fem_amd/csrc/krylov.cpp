@@ -69,6 +69,102 @@ static void tri_levels(const Csr& T, bool upper, std::vector<int>& off, std::vec
     for (int i = 0; i < n; i++) rows[fill[lev[i]]++] = i;
 }
 
+// The entries of a triangular factor's row that its level schedule counted
+// (all but the diagonal -- the last entry of an L row, the first of a U row --
+// and, unless pattern_only, the zero-valued ones); -1 for a factor with an
+// entry on the wrong side of the diagonal or an empty row, else the largest
+// count of a row.
+static int tri_max_count(const Csr& T, bool upper, bool pattern_only) {
+    int maxc = 0;
+    for (int i = 0; i < T.n; i++) {
+        if (T.rp[i + 1] <= T.rp[i]) return -1;
+        const int d = upper ? T.rp[i] : T.rp[i + 1] - 1;
+        int c = 0;
+        for (int p = T.rp[i]; p < T.rp[i + 1]; p++) {
+            if (p == d) continue;
+            const int j = T.ci[p];
+            if (upper ? j <= i : j >= i) return -1;
+            c += pattern_only || T.v[p] != 0.0;
+        }
+        maxc = std::max(maxc, c);
+    }
+    return maxc;
+}
+
+// A triangular factor packed by level for dev::tri_solve_pair (records of
+// MAXP value slots): rows in the level schedule's order (off, rows), the
+// counted entries in row order, which all lie in earlier levels.  A
+// dependency whose result is still in the LDS ring when its reader's level
+// ends (fewer than TRI_RING phase positions back) is read there, any other
+// from outpos in global memory.  The values are filled by pack_values.
+template <int MAXP>
+static void tri_pack(const Csr& T, bool upper, bool pattern_only, const std::vector<int>& off,
+                     const std::vector<int>& rows, const std::vector<int>& outpos, DevBuffers& B, hipStream_t st,
+                     IluSystem::TriDev& D) {
+    const int n = T.n, nlv = (int)off.size() - 1;
+    std::vector<int> pos(n), lev(n);
+    for (int L = 0; L < nlv; L++)
+        for (int t = off[L]; t < off[L + 1]; t++) {
+            pos[rows[t]] = t;
+            lev[rows[t]] = L;
+        }
+    const size_t N = (size_t)n;
+    std::vector<int> full(std::max(nlv, 1), 0), dmap(n), vmap(MAXP * N, -1), ovp(n, 0), ovmap, ovsrc;
+    std::vector<dev::TriRec<MAXP>> rec(n);
+    for (int t = 0; t < n; t++) {
+        const int i = rows[t], L = lev[i], e = off[L + 1];
+        const int d = upper ? T.rp[i] : T.rp[i + 1] - 1;
+        dev::TriRec<MAXP>& r = rec[t];
+        r.diag = 0.0;
+        for (int k = 0; k < MAXP; k++) {
+            r.val[k] = 0.0;
+            r.src[k] = dev::TRI_RING;  // the ring's zero slot
+        }
+        r.orow = outpos[i];
+        dmap[t] = d;
+        ovp[t] = (int)ovmap.size();
+        int k = 0;
+        for (int p = T.rp[i]; p < T.rp[i + 1]; p++) {
+            if (p == d || !(pattern_only || T.v[p] != 0.0)) continue;
+            const int j = T.ci[p];
+            int sj;
+            if (e - pos[j] <= dev::TRI_RING) {
+                sj = pos[j] & (dev::TRI_RING - 1);
+            } else {
+                sj = -(outpos[j] + 1);
+                full[L] = 1;
+            }
+            if (k < MAXP) {
+                vmap[k * N + t] = p;
+                r.src[k] = sj;
+            } else {
+                ovmap.push_back(p);
+                ovsrc.push_back(sj);
+            }
+            k++;
+        }
+        r.cnt = k;
+    }
+    std::vector<int> offf(off);
+    for (int L = 0; L < nlv; L++)
+        if (full[L]) offf[L] |= (int)0x80000000u;
+    dev::TriPack& P = D.p;
+    P.n = n;
+    P.nlv = nlv;
+    P.maxp = MAXP;
+    P.off = kupload(offf, B, st);
+    D.rec = kupload(rec, B, st);
+    P.rec = D.rec;
+    P.ovp = kupload(ovp, B, st);
+    P.ovsrc = kupload(ovsrc, B, st);
+    D.nov = (int)ovmap.size();
+    D.vmap = kupload(vmap, B, st);
+    D.dmap = kupload(dmap, B, st);
+    D.ovmap = kupload(ovmap, B, st);
+    D.ovval = kalloc<double>(ovmap.size(), B);
+    P.ovval = D.ovval;
+}
+
 static dev::DMat<double> upload_csr(const Csr& A, DevBuffers& B, hipStream_t st) {
     dev::DMat<double> D;
     D.n = A.n;
@@ -163,6 +259,7 @@ void IluSystem::refactor(const Csr& A) {
     hip_check(hipMemcpyAsync(fail_, host_, sizeof(int), hipMemcpyHostToDevice, stream_), "h2d");
     dev::ilu0_numeric(stream_, nlv_l_, lvl_off_, lvl_rows_, dA_.rp, dA_.ci, dA_.v, dL_.rp, dL_.ci, dL_.v, dU_.rp,
                       dU_.ci, dU_.v, fail_);
+    pack_values();
     int fail = 0;
     hip_check(hipMemcpyAsync(&host_[1], fail_, sizeof(int), hipMemcpyDeviceToHost, stream_), "d2h");
     hip_check(hipStreamSynchronize(stream_), "sync");
@@ -213,16 +310,39 @@ void IluSystem::upload(bool pattern_levels) {
     dL_ = upload_csr(L_, B, st);
     dU_ = upload_csr(U_, B, st);
     std::vector<int> off, rows;
+    std::vector<int> offu, rowsu;
     tri_levels(L_, false, off, rows, pattern_levels);
     nlv_l_ = (int)off.size() - 1;
     lvl_off_ = kupload(off, B, st);
     lvl_rows_ = kupload(rows, B, st);
-    tri_levels(U_, true, off, rows, pattern_levels);
-    nlv_u_ = (int)off.size() - 1;
-    lvu_off_ = kupload(off, B, st);
-    lvu_rows_ = kupload(rows, B, st);
+    tri_levels(U_, true, offu, rowsu, pattern_levels);
+    nlv_u_ = (int)offu.size() - 1;
+    lvu_off_ = kupload(offu, B, st);
+    lvu_rows_ = kupload(rowsu, B, st);
     const int n = n_;
     for (double** p : {&vX_, &vB_, &vR_, &vP_, &vQ_, &vZ_, &vW_, &vY_, &vT_}) *p = kalloc<double>(n, B);
+    // level packs: the forward sweep writes y in the backward sweep's order
+    std::vector<int> posu(n), natural(n);
+    for (int t = 0; t < n; t++) posu[rowsu[t]] = t;
+    for (int i = 0; i < n; i++) natural[i] = i;
+    tf_ = TriDev();
+    tb_ = TriDev();
+    const int cl = tri_max_count(L_, false, pattern_levels), cu = tri_max_count(U_, true, pattern_levels);
+    packed_ = n > 0 && cl >= 0 && cu >= 0;
+    if (packed_) {
+        // one record width for both sweeps (one kernel runs them)
+        if (std::max(cl, cu) <= 4) {
+            tri_pack<4>(L_, false, pattern_levels, off, rows, posu, B, st, tf_);
+            tri_pack<4>(U_, true, pattern_levels, offu, rowsu, natural, B, st, tb_);
+        } else {
+            tri_pack<8>(L_, false, pattern_levels, off, rows, posu, B, st, tf_);
+            tri_pack<8>(U_, true, pattern_levels, offu, rowsu, natural, B, st, tb_);
+        }
+        permL_ = lvl_rows_;
+        vZp_ = kalloc<double>(n, B);
+        vYu_ = kalloc<double>(n, B);
+        pack_values();
+    }
     part_ = kalloc<double>(std::max(4096, dA_.nblk + 16) * 40, B);
     scal_ = kalloc<double>(64, B);
     fail_ = kalloc<int>(1, B);
@@ -230,8 +350,18 @@ void IluSystem::upload(bool pattern_levels) {
     hip_check(hipStreamSynchronize(st), "upload sync");
 }
 
+void IluSystem::pack_values() {
+    if (!packed_) return;
+    dev::tri_pack_values(stream_, n_, tf_.p.maxp, tf_.vmap, tf_.dmap, tf_.nov, tf_.ovmap, dL_.v, tf_.rec, tf_.ovval);
+    dev::tri_pack_values(stream_, n_, tb_.p.maxp, tb_.vmap, tb_.dmap, tb_.nov, tb_.ovmap, dU_.v, tb_.rec, tb_.ovval);
+}
+
 void IluSystem::apply(const double* z, double* w) {
     // PCG_preconlr: y = L^-1 z, w = U^-1 y
+    if (packed_) {
+        dev::tri_solve_pair(stream_, tf_.p, tb_.p, permL_, z, vZp_, vYu_, w);
+        return;
+    }
     dev::trsv(stream_, dL_.rp, dL_.ci, dL_.v, nlv_l_, lvl_off_, lvl_rows_, z, vT_, 0);
     dev::trsv(stream_, dU_.rp, dU_.ci, dU_.v, nlv_u_, lvu_off_, lvu_rows_, vT_, w, 1);
 }

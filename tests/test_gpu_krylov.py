@@ -50,7 +50,51 @@ def test_ilu_apply_matches_reference(lina_mod, orc, name, s):
     il = lina_mod.ILU(s.rowptr, s.col, s.val)
     w = il.apply(z)
     il.close()
-    assert rel(w, w_ref) < 1e-13
+    # the sweeps perform PCG_preconl/r's operations in their order: bit for bit
+    assert np.array_equal(w, w_ref), rel(w, w_ref)
+
+
+def _ilu_stress(n=20000, seed=5):
+    """A pattern for the level-packed sweeps' edge paths: rows 0..2999 have
+    no lower neighbours (a first level of 3000 rows, wider than one
+    workgroup), later rows couple to 0..12 random earlier rows (rows with more
+    off-diagonal entries than the register slots hold), some of them more than
+    the LDS ring's 8192 rows back (results read through global memory);
+    symmetric pattern, diagonally dominant."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for i in range(3000, n):
+        k = int(rng.integers(0, 13))
+        js = rng.integers(0, i, size=k)
+        if i > 12000 and k:
+            js[0] = int(rng.integers(0, i - 9000))
+        for j in set(int(j) for j in js):
+            rows += [i, j]
+            cols += [j, i]
+    vals = -rng.uniform(0.1, 1.0, len(rows))
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    A = A + sp.diags(np.asarray(abs(A).sum(axis=1)).ravel() + 1.0)
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+
+
+def test_ilu_apply_wide_levels_long_rows_far_sources(lina_mod, orc):
+    rp, ci, v = _ilu_stress()
+    n = len(rp) - 1
+    say("ilu apply stress", n, "nnz", len(ci))
+    ok, Lo, Uo = orc.ilu0(rp, ci, v)
+    assert ok
+    z = np.sin(np.arange(n) * 0.37)
+    w_ref = orc.ilu_apply(Lo, Uo, z)
+    il = lina_mod.ILU(rp, ci, v)  # host factorisation, value-based levels
+    w1 = il.apply(z)
+    assert il.refactor(rp, ci, v)  # device factorisation, pattern levels
+    w2 = il.apply(z)
+    il.close()
+    assert np.array_equal(w1, w_ref), rel(w1, w_ref)
+    assert np.array_equal(w2, w_ref), rel(w2, w_ref)
 
 
 @pytest.mark.parametrize("name,s", _systems()[:2], ids=lambda v: v if isinstance(v, str) else "")
