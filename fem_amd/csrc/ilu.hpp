@@ -58,6 +58,12 @@ class IluSystem {
   private:
     void sync_factors();
     void pack_values();
+    bool build_plan();
+    // device factorisation plan (dev::ilu0_planned), built with the symbolic structure
+    bool plan_ = false;
+    int4 *pm1_ = nullptr, *pm2_ = nullptr;
+    int *pmask_ = nullptr, *ppu_ = nullptr;
+    std::vector<int> hoff_l_, hrows_l_;  // L's level schedule (host copy)
     bool packed_ = false;          // tf_/tb_ hold the level packs (else the k_trsv sweeps)
     TriDev tf_, tb_;
     int* permL_ = nullptr;         // row of each forward phase position

@@ -1030,6 +1030,60 @@ __global__ __launch_bounds__(1024) void k_ilu0(int nlv, const int* __restrict__ 
     }
 }
 
+// k_ilu0 without the searches: row t of the level order (one thread, levels
+// in order) takes its products' U entries from the plan, keeps its L values
+// in registers, and performs k_ilu0's operations in k_ilu0's order (so the
+// factors are bit-identical): per A entry m in column order, s = sum over
+// the mask's slots q ascending of L(i,k_q) U(k_q,j); m < nl: L = a - s;
+// the diagonal: L(i,i) = 0 + (a - s); above it: U = (a - s) / L(i,i).
+__global__ __launch_bounds__(1024) void k_ilu0_planned(int nlv, const int* __restrict__ lv_off,
+                                                       const int4* __restrict__ m1, const int4* __restrict__ m2,
+                                                       const int* __restrict__ mask, const int* __restrict__ pu,
+                                                       const double* __restrict__ vA, double* vL, double* vU,
+                                                       int* __restrict__ fail) {
+    for (int L = 0; L < nlv; L++) {
+        const int e = lv_off[L + 1];
+        for (int t = lv_off[L] + threadIdx.x; t < e; t += blockDim.x) {
+            const int4 a = m1[t], b = m2[t];
+            const int a0 = a.x, na = a.y, l0 = a.z, u0 = a.w, nl = b.x, hasd = b.y;
+            int r = b.z, pw = u0 + 1;
+            double Lr[ILU_MAXL];
+#pragma unroll
+            for (int q = 0; q < ILU_MAXL; q++) Lr[q] = 0.0;
+            double dg = 0.0;
+            vU[u0] = 1.0;
+#pragma unroll
+            for (int m = 0; m < ILU_MAXA; m++) {
+                if (m < na) {
+                    const double av = vA[a0 + m];
+                    const int mk = mask[a0 + m];
+                    double s = 0.0;
+#pragma unroll
+                    for (int q = 0; q < ILU_MAXL; q++)
+                        if ((mk >> q) & 1) {
+                            s += Lr[q] * vU[pu[r]];
+                            r++;
+                        }
+                    if (m < ILU_MAXL && m < nl) {
+                        Lr[m < ILU_MAXL ? m : 0] = av - s;
+                    } else if (hasd && m == nl) {
+                        dg = 0.0 + (av - s);
+                    } else {
+                        if (dg != 0.0) vU[pw] = (av - s) / dg;
+                        else *fail = 1;
+                        pw++;
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < ILU_MAXL; q++)
+                if (q < nl) vL[l0 + q] = Lr[q];
+            vL[l0 + nl] = dg;
+        }
+        __syncthreads();
+    }
+}
+
 // h_l = V_l . r for l < m (classical Gram-Schmidt dots), block partials
 // part[l * nblk + block]
 __global__ __launch_bounds__(TPB) void k_multidot(int n, int m, const double* __restrict__ V, size_t ld,
@@ -1855,6 +1909,11 @@ void tri_pack_values(hipStream_t st, int n, int maxp, const int* vmap, const int
         hipLaunchKernelGGL(k_tri_values<4>, g, b, 0, st, n, vmap, dmap, nov, ovmap, v, (TriRec<4>*)rec, ovval);
     else
         hipLaunchKernelGGL(k_tri_values<8>, g, b, 0, st, n, vmap, dmap, nov, ovmap, v, (TriRec<8>*)rec, ovval);
+}
+
+void ilu0_planned(hipStream_t st, int nlv, const int* lv_off, const int4* m1, const int4* m2, const int* mask,
+                  const int* pu, const double* vA, double* vL, double* vU, int* fail) {
+    hipLaunchKernelGGL(k_ilu0_planned, dim3(1), dim3(1024), 0, st, nlv, lv_off, m1, m2, mask, pu, vA, vL, vU, fail);
 }
 
 void ilu0_numeric(hipStream_t st, int nlv, const int* lv_off, const int* lv_rows, const int* rpA, const int* ciA,

@@ -245,6 +245,15 @@ void tri_solve_pair(hipStream_t st, const TriPack& fw, const TriPack& bw, const 
 // v[vmap[k*n + t]] (vmap < 0: 0), rec[t].diag = v[dmap[t]], ovval[q] = v[ovmap[q]]
 void tri_pack_values(hipStream_t st, int n, int maxp, const int* vmap, const int* dmap, int nov, const int* ovmap,
                      const double* v, void* rec, double* ovval);
+// The same factorisation from a symbolic plan (rows in level order t):
+// m1[t] = {first A entry a0, A entries na, first L entry l0, U diagonal u0},
+// m2[t] = {lower entries nl, A has the diagonal, first pair}, and per A entry
+// p a bit mask mask[p] of the row's L slots q whose U(k_q, j) exists, their U
+// entry indices listed in pu (slot order, entry after entry).  At most
+// ILU_MAXA entries a row and ILU_MAXL lower ones.
+constexpr int ILU_MAXA = 16, ILU_MAXL = 8;
+void ilu0_planned(hipStream_t st, int nlv, const int* lv_off, const int4* m1, const int4* m2, const int* mask,
+                  const int* pu, const double* vA, double* vL, double* vU, int* fail);
 // level-scheduled ILU(0) numeric factorisation into fixed L/U patterns (see
 // k_ilu0); A's rows sorted by column; *fail = 1 on a zero pivot
 void ilu0_numeric(hipStream_t st, int nlv, const int* lv_off, const int* lv_rows, const int* rpA, const int* ciA,

@@ -249,6 +249,7 @@ void IluSystem::refactor(const Csr& A) {
         U_ = std::move(U);
         n_ = A.n;
         upload(true);
+        plan_ = build_plan();
         device_ready_ = true;
     } else {
         // A_ keeps the pattern; its values are not refreshed (only dA_'s are)
@@ -257,8 +258,11 @@ void IluSystem::refactor(const Csr& A) {
     }
     host_[0] = 0.0;
     hip_check(hipMemcpyAsync(fail_, host_, sizeof(int), hipMemcpyHostToDevice, stream_), "h2d");
-    dev::ilu0_numeric(stream_, nlv_l_, lvl_off_, lvl_rows_, dA_.rp, dA_.ci, dA_.v, dL_.rp, dL_.ci, dL_.v, dU_.rp,
-                      dU_.ci, dU_.v, fail_);
+    if (plan_)
+        dev::ilu0_planned(stream_, nlv_l_, lvl_off_, pm1_, pm2_, pmask_, ppu_, dA_.v, dL_.v, dU_.v, fail_);
+    else
+        dev::ilu0_numeric(stream_, nlv_l_, lvl_off_, lvl_rows_, dA_.rp, dA_.ci, dA_.v, dL_.rp, dL_.ci, dL_.v,
+                          dU_.rp, dU_.ci, dU_.v, fail_);
     pack_values();
     int fail = 0;
     hip_check(hipMemcpyAsync(&host_[1], fail_, sizeof(int), hipMemcpyDeviceToHost, stream_), "d2h");
@@ -266,6 +270,50 @@ void IluSystem::refactor(const Csr& A) {
     std::memcpy(&fail, &host_[1], sizeof(int));
     ilu_ok_ = fail ? 0 : 1;
     host_factors_ = false;
+}
+
+// the symbolic plan of dev::ilu0_planned for A_ and the patterns L_, U_ (rows
+// sorted; L's level schedule hoff_l_/hrows_l_): false when a row has more
+// entries than the kernel's register slots (the searching kernel runs then)
+bool IluSystem::build_plan() {
+    const int n = n_;
+    std::vector<int4> m1(n), m2(n);
+    std::vector<int> mask(A_.nnz(), 0), pu;
+    for (int t = 0; t < n; t++) {
+        const int i = hrows_l_[t];
+        const int a0 = A_.rp[i], na = A_.rp[i + 1] - a0;
+        int nl = 0, hasd = 0;
+        for (int m = 0; m < na; m++) {
+            nl += A_.ci[a0 + m] < i;
+            hasd |= A_.ci[a0 + m] == i;
+        }
+        if (na > dev::ILU_MAXA || nl > dev::ILU_MAXL) return false;
+        m1[t] = make_int4(a0, na, L_.rp[i], U_.rp[i]);
+        m2[t] = make_int4(nl, hasd, (int)pu.size(), 0);
+        for (int m = 0; m < na; m++) {
+            const int j = A_.ci[a0 + m], lim = std::min(m, nl);
+            int mk = 0;
+            for (int q = 0; q < lim; q++) {
+                const int k = A_.ci[a0 + q];
+                // U row k: the diagonal first, then the upper columns ascending
+                const int* b = U_.ci.data() + U_.rp[k] + 1;
+                const int* e = U_.ci.data() + U_.rp[k + 1];
+                const int* f = std::lower_bound(b, e, j);
+                if (f != e && *f == j) {
+                    mk |= 1 << q;
+                    pu.push_back((int)(f - U_.ci.data()));
+                }
+            }
+            mask[a0 + m] = mk;
+        }
+    }
+    DevBuffers& B = *bufs_;
+    pm1_ = kupload(m1, B, stream_);
+    pm2_ = kupload(m2, B, stream_);
+    pmask_ = kupload(mask, B, stream_);
+    ppu_ = kupload(pu, B, stream_);
+    hip_check(hipStreamSynchronize(stream_), "plan upload");
+    return true;
 }
 
 void IluSystem::sync_factors() {
@@ -312,6 +360,9 @@ void IluSystem::upload(bool pattern_levels) {
     std::vector<int> off, rows;
     std::vector<int> offu, rowsu;
     tri_levels(L_, false, off, rows, pattern_levels);
+    hoff_l_ = off;
+    hrows_l_ = rows;
+    plan_ = false;
     nlv_l_ = (int)off.size() - 1;
     lvl_off_ = kupload(off, B, st);
     lvl_rows_ = kupload(rows, B, st);

@@ -585,7 +585,12 @@ class ReflectiveNewton {
     void iterate(const Csr& M) {
         const int n = n_;
         dev::copy(st_, n, (const double*)X_, Xo_);
-        DevMatrix dM(M);
+        // M keeps its pattern over most Newton steps: values-only re-upload
+        if (!dM_ || !dM_->update_values(M)) {
+            dM_.reset();
+            dM_.reset(new DevMatrix(M));
+        }
+        DevMatrix& dM = *dM_;
         int status = subspace(M, dM);
         // B = D g
         dev::copy(st_, n, (const double*)G_, Bv_);
@@ -672,6 +677,7 @@ class ReflectiveNewton {
     uint64_t seed_;
     int n_ = 0;
     std::unique_ptr<DevMatrix> dH_;
+    std::unique_ptr<DevMatrix> dM_;  // the scaled Hessian of the last Newton step
     std::unique_ptr<Solver> amg_;
     IluSystem ilu_;  // the ILU(0) inner solvers' system, refactored each Newton step
     std::unique_ptr<DevBuffers> B_;

@@ -1317,11 +1317,23 @@ DevMatrix::DevMatrix(const Csr& A) : n_(A.n), nnz_(A.nnz()), bufs_(new DevBuffer
     hip_check(hipEventCreate(&ev0_), "event");
     hip_check(hipEventCreate(&ev1_), "event");
     upload_pair(A, A_, Af_, *bufs_, stream_);
+    prp_ = A.rp;
+    pci_ = A.ci;
     x_ = dalloc<double>(n_, *bufs_);
     y_ = dalloc<double>(n_, *bufs_);
     xf_ = dalloc<float>(n_, *bufs_);
     yf_ = dalloc<float>(n_, *bufs_);
     hip_check(hipStreamSynchronize(stream_), "upload");
+}
+
+bool DevMatrix::update_values(const Csr& A0) {
+    if (A0.n != n_ || A0.rp != prp_ || A0.ci != pci_) return false;
+    const Csr A = drop_outside_zeros(A0);
+    if (A.nnz() != A_.nnz) return false;
+    hip_check(hipMemcpyAsync(A_.v, A.v.data(), A.v.size() * sizeof(double), hipMemcpyHostToDevice, stream_), "h2d");
+    dev::convert<double, float>(stream_, A_.nnz, A_.v, Af_.v);
+    hip_check(hipStreamSynchronize(stream_), "update");
+    return true;
 }
 
 DevMatrix::~DevMatrix() {

@@ -63,6 +63,11 @@ class DevMatrix {
   public:
     explicit DevMatrix(const Csr& A);
     ~DevMatrix();
+    // new values of the same pattern: uploads the FP64 values and converts
+    // the FP32 copy on the device, keeping the row blocks and column codes
+    // (the rows' l1 norms, which no product reads, keep the first values);
+    // returns false, changing nothing, for another pattern
+    bool update_values(const Csr& A);
     int n() const { return n_; }
     long nnz() const { return nnz_; }
     void apply(const double* x, double* y);  // enqueue on stream(), FP64
@@ -86,6 +91,7 @@ class DevMatrix {
     std::unique_ptr<DevBuffers> bufs_;
     dev::DMat<double> A_;
     dev::DMat<float> Af_;
+    std::vector<int> prp_, pci_;  // the uploaded pattern
     double *x_ = nullptr, *y_ = nullptr;
     float *xf_ = nullptr, *yf_ = nullptr;
 };
