@@ -123,12 +123,16 @@ def workload_damage(args, lina, problems):
     for h_scale in (1.0,):
         s, dold, ones = problems.damage_system(m, h_scale=h_scale)
         log(f"damage h_scale={h_scale}: n={s.n} nnz={s.nnz}")
-        # the BiCGStab variant's AMG: the reference's AMG_solve settings in
-        # FP32 (one V-cycle from the warm start instead made the inner solves
-        # fail and the Newton iteration stall)
+        # the BiCGStab variant's AMG: the reference's AMG_solve settings (eps
+        # 1e-4, at most 20 V-cycles from the warm start) in FP32 with
+        # multicolour SOR sweeps (one V-cycle from the warm start instead made
+        # the inner solves fail and the Newton iteration stall; the exact
+        # lexicographic SOR's 2,049-level sweeps took 209 s in all)
         for solver, prec in (("pcg", "fp64"), ("gmres", "fp64"), ("bicgstab", "mixed")):
-            prm = lina.rn_params(solver, 1, 6, delta=float(np.sqrt(s.n) / 2),
-                                 amg={"precision": 1 if prec == "mixed" else 0})
+            amg = {"precision": 1 if prec == "mixed" else 0}
+            if solver == "bicgstab":
+                amg["smoother"] = 2  # MC-SOR
+            prm = lina.rn_params(solver, 1, 6, delta=float(np.sqrt(s.n) / 2), amg=amg)
             t0 = time.perf_counter()
             rn = lina.ReflectiveNewton(s.rowptr, s.col, s.val, s.b, dold, ones, prm)
             t1 = time.perf_counter()

@@ -454,6 +454,7 @@ void Solver::upload() {
     hipStream_t st = stream_;
     const int depth = (int)H_.lv.size();
     L_.assign(depth, DevLevel());
+    up_.assign(depth, UpLevel());
     int maxn = 1, maxblk = 1;
     for (int l = 0; l < depth; l++) {
         const HostLevel& h = H_.lv[l];
@@ -471,6 +472,8 @@ void Solver::upload() {
         d.rf = dalloc<float>(d.n, B);
         d.xof = dalloc<float>(d.n, B);
         d.nlim = (d.n / std::max(1, prm_.dof)) * std::max(1, prm_.dof);
+        up_[l].rp = h.A.rp;
+        up_[l].ci = h.A.ci;
         if (l == 0) continue;
         maxblk = std::max(maxblk, upload_pair(h.P, d.P, d.Pf, B, st, segl));
         maxblk = std::max(maxblk, upload_pair(h.R, d.R, d.Rf, B, st, seg_off_.empty() ? nullptr : &seg_off_[l - 1]));
@@ -518,16 +521,22 @@ void Solver::upload() {
         upload_mat(Ac, c32, d.Acf, B, st, &d.Ac, &c64, perm.data(), segl);
         d.perm = dupload(perm, B, st);
         d.ncolours = nc;
+        up_[l].colour = col;
+        up_[l].perm = perm;
         // level schedules of the exact lexicographic sweeps
         std::vector<int> off, rows;
         level_schedule(h.A, false, off, rows);
         d.nlv_f = (int)off.size() - 1;
         d.lvf_off = dupload(off, B, st);
         d.lvf_rows = dupload(rows, B, st);
+        up_[l].lvf_off = off;
+        up_[l].lvf_rows = rows;
         level_schedule(h.A, true, off, rows);
         d.nlv_b = (int)off.size() - 1;
         d.lvb_off = dupload(off, B, st);
         d.lvb_rows = dupload(rows, B, st);
+        up_[l].lvb_off = off;
+        up_[l].lvb_rows = rows;
     }
     // finest-level extras
     signs_ = dupload(H_.signs, B, st);
@@ -578,9 +587,11 @@ void Solver::upload() {
         double* piv = dalloc<double>(nb, B);
         dev::batched_gauss_jordan_inverse(st, nd_, maxb, crblk_, coff_, cmoff_, inv_, piv);
     } else {
-        double* piv = dalloc<double>(1, B);
-        dev::gauss_jordan_inverse(st, nd_, inv_, piv);
+        cpiv_ = dalloc<double>(1, B);
+        dev::gauss_jordan_inverse(st, nd_, inv_, cpiv_);
     }
+    up_dense_ = lu.dense_idx;
+    up_trivial_ = lu.trivial_idx;
     // gathered right-hand side, padded with zeros to whole tiles of the FP32 inverse
     const int nbd = std::max((nd_ + dev::CT - 1) / dev::CT * dev::CT, dev::CT);
     cbd_ = dalloc<double>(nbd, B);
@@ -671,14 +682,110 @@ void Solver::ensure_ctiles() {
     hip_check(hipStreamSynchronize(stream_), "sync");
 }
 
+// host l1 norms of a matrix's rows as upload_mat forms them (VT arithmetic)
+template <typename VT>
+static std::vector<VT> row_l1(const Csr& A) {
+    std::vector<VT> l1(A.n, (VT)0);
+    for (int r = 0; r < A.n; r++) {
+        VT s = 0;
+        for (int p = A.rp[r]; p < A.rp[r + 1]; p++) s += std::fabs((VT)A.v[p]);
+        l1[r] = s;
+    }
+    return l1;
+}
+
+void Solver::refresh_after_recompute() {
+    static const bool timing = getenv("LINA_RN_TIMING") != nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (size_t l = 1; l < H_.lv.size(); l++) colours_[l] = greedy_colouring(H_.lv[l].A, &ncolours_[l]);
+    const auto t1 = std::chrono::steady_clock::now();
+    bool vals = false;
+    if (uploaded()) {
+        vals = refresh_values();
+        if (!vals) upload();
+    }
+    if (timing) {
+        const auto t2 = std::chrono::steady_clock::now();
+        auto s = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
+        fprintf(stderr, "[amg] recompute: colouring %.3fs, %s upload %.3fs\n", s(t0, t1), vals ? "values" : "full",
+                s(t1, t2));
+    }
+}
+
+// The values-only re-upload after a recompute: every level's A (FP64, FP32,
+// the colour-permuted copies and their row l1 norms) and the coarse inverse
+// and tiles, when each level's pattern, colouring and lexicographic level
+// schedules and the coarse LU's row split are those on the device (P, R and
+// the autotuned lane counts stay).  False, with nothing changed, otherwise.
+bool Solver::refresh_values() {
+    const int depth = (int)H_.lv.size();
+    if ((int)up_.size() != depth || (int)L_.size() != depth || cbatched_ || !seg_off_.empty()) return false;
+    for (int l = 0; l < depth; l++) {
+        const Csr& A = H_.lv[l].A;
+        const UpLevel& u = up_[l];
+        if (A.rp != u.rp || A.ci != u.ci || A.nnz() != L_[l].A.nnz) return false;
+        if (l == 0) continue;
+        if (colours_[l] != u.colour) return false;
+        std::vector<int> off, rows;
+        level_schedule(A, false, off, rows);
+        if (off != u.lvf_off || rows != u.lvf_rows) return false;
+        level_schedule(A, true, off, rows);
+        if (off != u.lvb_off || rows != u.lvb_rows) return false;
+    }
+    const CoarseLU& lu = H_.lu;
+    if (lu.dense_idx != up_dense_ || lu.trivial_idx != up_trivial_ || !cpiv_) return false;
+    hipStream_t st = stream_;
+    std::vector<std::vector<double>> keep;  // host staging alive until the copies are done
+    std::vector<std::vector<float>> keepf;
+    auto put = [&](double* d, std::vector<double> h) {
+        keep.push_back(std::move(h));
+        hip_check(hipMemcpyAsync(d, keep.back().data(), keep.back().size() * sizeof(double), hipMemcpyHostToDevice,
+                                 st), "h2d");
+    };
+    auto putf = [&](float* d, std::vector<float> h) {
+        keepf.push_back(std::move(h));
+        hip_check(hipMemcpyAsync(d, keepf.back().data(), keepf.back().size() * sizeof(float), hipMemcpyHostToDevice,
+                                 st), "h2d");
+    };
+    for (int l = 0; l < depth; l++) {
+        const Csr& A = H_.lv[l].A;
+        DevLevel& d = L_[l];
+        put(d.A.v, A.v);
+        dev::convert<double, float>(st, d.A.nnz, d.A.v, d.Af.v);
+        put(d.A.l1, row_l1<double>(A));
+        putf(d.Af.l1, row_l1<float>(A));
+        if (l == 0) continue;
+        const std::vector<int>& perm = up_[l].perm;
+        Csr Ac;
+        Ac.n = A.n;
+        Ac.rp.assign(A.n + 1, 0);
+        Ac.v.reserve(A.nnz());
+        for (int r = 0; r < A.n; r++) {
+            const int i = perm[r];
+            Ac.v.insert(Ac.v.end(), A.v.begin() + A.rp[i], A.v.begin() + A.rp[i + 1]);
+            Ac.rp[r + 1] = (int)Ac.v.size();
+        }
+        put(d.Ac.l1, row_l1<double>(Ac));
+        putf(d.Acf.l1, row_l1<float>(Ac));
+        const int nnz = (int)Ac.v.size();
+        put(d.Ac.v, std::move(Ac.v));
+        dev::convert<double, float>(st, nnz, d.Ac.v, d.Acf.v);
+    }
+    put(inv_, lu.dense);
+    dev::gauss_jordan_inverse(st, nd_, inv_, cpiv_);
+    put(tdiag_, lu.trivial_diag);
+    if (ctiles_) dev::inverse_to_tiles<double>(st, nd_, ntiles_, cti_, ctj_, inv_, ctiles_);
+    hip_check(hipStreamSynchronize(st), "refresh sync");
+    return true;
+}
+
 void Solver::recompute(const std::vector<double>& val) {
     Csr A = H_.lv.back().A;  // pattern of the finest (sign-adjusted) matrix
     if ((int)val.size() != A.nnz()) throw std::invalid_argument("recompute: value count differs from the pattern");
     for (int i = 0; i < A.n; i++)
         for (int p = A.rp[i]; p < A.rp[i + 1]; p++) A.v[p] = val[p] * (H_.signs[i] < 0 ? -1.0 : 1.0);
     amg_recompute(H_, A);
-    for (size_t l = 1; l < H_.lv.size(); l++) colours_[l] = greedy_colouring(H_.lv[l].A, &ncolours_[l]);
-    if (uploaded()) upload();
+    refresh_after_recompute();
 }
 
 void Solver::recompute_matrix(const Csr& M) {
@@ -688,8 +795,7 @@ void Solver::recompute_matrix(const Csr& M) {
         if (H_.signs[i] < 0)
             for (int p = A.rp[i]; p < A.rp[i + 1]; p++) A.v[p] = -A.v[p];
     amg_recompute(H_, A);
-    for (size_t l = 1; l < H_.lv.size(); l++) colours_[l] = greedy_colouring(H_.lv[l].A, &ncolours_[l]);
-    if (uploaded()) upload();
+    refresh_after_recompute();
 }
 
 void Solver::set_operator(DevMatrix* op) {

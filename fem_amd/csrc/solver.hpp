@@ -172,6 +172,16 @@ class Solver {
                              const double* scale, int blocked, const double* u, const double* lam, double* grad);
 
   private:
+    // recompute: colour the new levels and re-upload them (values only when
+    // every pattern-derived structure matches what is on the device)
+    void refresh_after_recompute();
+    bool refresh_values();
+    struct UpLevel {  // host copy of what upload() derived from a level's pattern
+        std::vector<int> rp, ci, colour, perm, lvf_off, lvf_rows, lvb_off, lvb_rows;
+    };
+    std::vector<UpLevel> up_;
+    std::vector<int> up_dense_, up_trivial_;
+    double* cpiv_ = nullptr;
     bool fused_gather() const;
     void ensure_ctiles();
     template <typename VT, typename XT>

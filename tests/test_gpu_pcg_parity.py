@@ -207,3 +207,37 @@ def test_mixed_precision_reaches_fp64_accuracy(gpu):
     for g in (g64, g32):
         assert np.linalg.norm(s.b - A @ g["x"]) / np.linalg.norm(s.b) < 1e-9
     assert rel(g32["x"], g64["x"]) < 1e-6
+
+
+@pytest.mark.parametrize("kind,prec", [("reference", "fp64"), ("vcycle", "mixed")])
+def test_recompute_values_only_upload(gpu, kind, prec):
+    """AMG_recompute_coarse_matrices on an uploaded hierarchy: new finest
+    values of the same pattern re-upload the levels' values only (every
+    level's pattern, colouring and level schedules unchanged).  The pcg
+    afterwards is bit for bit that of a hierarchy that reached the same values
+    through full re-uploads (an intermediate recompute with zeroed entries
+    changes the coarse patterns, so both of its re-uploads are full)."""
+    s1 = P.damage_system(65, seed_h=21)[0]
+    s2 = P.damage_system(65, seed_h=22)[0]
+    assert np.array_equal(s1.rowptr, s2.rowptr) and np.array_equal(s1.col, s2.col)
+    mid = s2.val.copy()
+    off = np.nonzero(s2.col != np.repeat(np.arange(s2.n), np.diff(s2.rowptr)))[0]
+    mid[off[::3]] = 0.0
+    prm = gpu.params(kind, 1, 4, precision=prec, autotune=0)
+    say("recompute values-only", kind, prec)
+    a = gpu.AMG(s1.rowptr, s1.col, s1.val, prm)
+    a.recompute(s2.val)
+    b = gpu.AMG(s1.rowptr, s1.col, s1.val, prm)
+    b.recompute(mid)
+    b.recompute(s2.val)
+    for l in range(a.depth):
+        assert np.array_equal(a.level(l)["A"][2], b.level(l)["A"][2]), l
+    ra = a.pcg(s2.b, tol=1e-10, max_iter=500)
+    rb = b.pcg(s2.b, tol=1e-10, max_iter=500)
+    say("  iters", ra["iters"], rb["iters"])
+    assert ra["iters"] == rb["iters"]
+    assert np.array_equal(ra["x"], rb["x"])
+    A = s2.scipy()
+    assert np.linalg.norm(s2.b - A @ ra["x"]) <= 1e-8 * np.linalg.norm(s2.b)
+    a.close()
+    b.close()
