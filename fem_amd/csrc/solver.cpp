@@ -454,7 +454,7 @@ void Solver::upload() {
     hipStream_t st = stream_;
     const int depth = (int)H_.lv.size();
     L_.assign(depth, DevLevel());
-    up_.assign(depth, UpLevel());
+    up_.assign(track_ ? depth : 0, UpLevel());
     int maxn = 1, maxblk = 1;
     for (int l = 0; l < depth; l++) {
         const HostLevel& h = H_.lv[l];
@@ -472,8 +472,10 @@ void Solver::upload() {
         d.rf = dalloc<float>(d.n, B);
         d.xof = dalloc<float>(d.n, B);
         d.nlim = (d.n / std::max(1, prm_.dof)) * std::max(1, prm_.dof);
-        up_[l].rp = h.A.rp;
-        up_[l].ci = h.A.ci;
+        if (track_) {
+            up_[l].rp = h.A.rp;
+            up_[l].ci = h.A.ci;
+        }
         if (l == 0) continue;
         maxblk = std::max(maxblk, upload_pair(h.P, d.P, d.Pf, B, st, segl));
         maxblk = std::max(maxblk, upload_pair(h.R, d.R, d.Rf, B, st, seg_off_.empty() ? nullptr : &seg_off_[l - 1]));
@@ -521,22 +523,28 @@ void Solver::upload() {
         upload_mat(Ac, c32, d.Acf, B, st, &d.Ac, &c64, perm.data(), segl);
         d.perm = dupload(perm, B, st);
         d.ncolours = nc;
-        up_[l].colour = col;
-        up_[l].perm = perm;
+        if (track_) {
+            up_[l].colour = col;
+            up_[l].perm = perm;
+        }
         // level schedules of the exact lexicographic sweeps
         std::vector<int> off, rows;
         level_schedule(h.A, false, off, rows);
         d.nlv_f = (int)off.size() - 1;
         d.lvf_off = dupload(off, B, st);
         d.lvf_rows = dupload(rows, B, st);
-        up_[l].lvf_off = off;
-        up_[l].lvf_rows = rows;
+        if (track_) {
+            up_[l].lvf_off = off;
+            up_[l].lvf_rows = rows;
+        }
         level_schedule(h.A, true, off, rows);
         d.nlv_b = (int)off.size() - 1;
         d.lvb_off = dupload(off, B, st);
         d.lvb_rows = dupload(rows, B, st);
-        up_[l].lvb_off = off;
-        up_[l].lvb_rows = rows;
+        if (track_) {
+            up_[l].lvb_off = off;
+            up_[l].lvb_rows = rows;
+        }
     }
     // finest-level extras
     signs_ = dupload(H_.signs, B, st);
@@ -702,7 +710,10 @@ void Solver::refresh_after_recompute() {
     bool vals = false;
     if (uploaded()) {
         vals = refresh_values();
-        if (!vals) upload();
+        if (!vals) {
+            track_ = true;  // from now on upload() keeps what a values-only refresh compares
+            upload();
+        }
     }
     if (timing) {
         const auto t2 = std::chrono::steady_clock::now();

@@ -179,7 +179,8 @@ class Solver {
     struct UpLevel {  // host copy of what upload() derived from a level's pattern
         std::vector<int> rp, ci, colour, perm, lvf_off, lvf_rows, lvb_off, lvb_rows;
     };
-    std::vector<UpLevel> up_;
+    std::vector<UpLevel> up_;  // kept only once a recompute has happened (track_)
+    bool track_ = false;
     std::vector<int> up_dense_, up_trivial_;
     double* cpiv_ = nullptr;
     bool fused_gather() const;

@@ -211,12 +211,14 @@ def test_mixed_precision_reaches_fp64_accuracy(gpu):
 
 @pytest.mark.parametrize("kind,prec", [("reference", "fp64"), ("vcycle", "mixed")])
 def test_recompute_values_only_upload(gpu, kind, prec):
-    """AMG_recompute_coarse_matrices on an uploaded hierarchy: new finest
-    values of the same pattern re-upload the levels' values only (every
-    level's pattern, colouring and level schedules unchanged).  The pcg
-    afterwards is bit for bit that of a hierarchy that reached the same values
-    through full re-uploads (an intermediate recompute with zeroed entries
-    changes the coarse patterns, so both of its re-uploads are full)."""
+    """AMG_recompute_coarse_matrices on an uploaded hierarchy: after a first
+    recompute (a full re-upload, from which the hierarchy keeps what a
+    values-only one compares), new finest values of the same pattern
+    re-upload the levels' values only (every level's pattern, colouring and
+    level schedules unchanged).  The pcg afterwards is bit for bit that of a
+    hierarchy that reached the same values through full re-uploads (an
+    intermediate recompute with zeroed entries changes the coarse patterns,
+    so both of its re-uploads are full)."""
     s1 = P.damage_system(65, seed_h=21)[0]
     s2 = P.damage_system(65, seed_h=22)[0]
     assert np.array_equal(s1.rowptr, s2.rowptr) and np.array_equal(s1.col, s2.col)
@@ -226,6 +228,7 @@ def test_recompute_values_only_upload(gpu, kind, prec):
     prm = gpu.params(kind, 1, 4, precision=prec, autotune=0)
     say("recompute values-only", kind, prec)
     a = gpu.AMG(s1.rowptr, s1.col, s1.val, prm)
+    a.recompute(s1.val)
     a.recompute(s2.val)
     b = gpu.AMG(s1.rowptr, s1.col, s1.val, prm)
     b.recompute(mid)
