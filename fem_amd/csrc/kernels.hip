@@ -30,6 +30,9 @@
 
 #include "kernels.hpp"
 
+#include <stdexcept>
+#include <string>
+
 namespace lina {
 namespace dev {
 
@@ -796,29 +799,74 @@ __device__ __forceinline__ void lds_barrier() {
 
 // the loaded values are only stored here (nothing computed from them), so
 // that the loads stay in flight until the row is solved
-template <int MAXP>
+// SOR mode (lexicographic SOR sweep, SorPack): a row also carries
+// a_t = (1 - omega) x_t (old x) and the products of its entries whose
+// values are the sweep's old ones, computed by k_sor_gather (zero padded);
+// the result is a_t + (omega s) / diag.
+struct SorArgs {
+    const double* a = nullptr;
+    const double* prod = nullptr;
+    double omega = 0;
+    const int* ocnt = nullptr;       // old entries per position (past MAXP: prodov[ooff[t] ...])
+    const int* ooff = nullptr;
+    const double* prodov = nullptr;
+    int old_first = 0;               // the old entries' products are already in the right-hand side
+};
+
+template <int MAXP, bool SOR>
 struct TriRow {
     TriRec<MAXP> r;
     double in;
+    double a;
+    double prod[SOR ? MAXP : 1];
+    int oc;
     bool ok;
 };
 
 // row t of the level ending at e (rows at or past e: ok = false, loads
 // clamped to row 0); only stored, nothing computed from the loaded values,
 // so that the loads stay in flight until the row is solved
-template <int MAXP>
-__device__ __forceinline__ void tri_load(TriRow<MAXP>& r, const TriPack& a, const double* in, int t, int e) {
+template <int MAXP, bool SOR>
+__device__ __forceinline__ void tri_load(TriRow<MAXP, SOR>& r, const TriPack& a, const SorArgs& so, const double* in,
+                                         int t, int e) {
     r.ok = t < e;
     const int tt = r.ok ? t : 0;
     r.r = static_cast<const TriRec<MAXP>*>(a.rec)[tt];
     r.in = in[tt];
+    if (SOR) {
+        r.a = so.a[tt];
+        r.oc = so.ocnt[tt];
+#pragma unroll
+        for (int k = 0; k < MAXP; k++) r.prod[k] = so.prod[k * a.n + tt];
+    }
+}
+
+template <int MAXP, bool SOR>
+__device__ __forceinline__ void tri_finish(const TriRow<MAXP, SOR>& row, const SorArgs& so, double s, double* ring,
+                                           double* out, int t) {
+    double x;
+    if (SOR) {
+        if (!so.old_first) {
+#pragma unroll
+            for (int k = 0; k < MAXP; k++) s -= row.prod[k];
+            if (row.oc > MAXP) {
+                const double* q = so.prodov + so.ooff[t];
+                for (int k = 0; k < row.oc - MAXP; k++) s -= q[k];
+            }
+        }
+        x = row.a + (so.omega * s) / row.r.diag;
+    } else {
+        x = s / row.r.diag;
+    }
+    ring[t & (TRI_RING - 1)] = x;
+    out[row.r.orow] = x;
 }
 
 // a level whose sources all lie in the ring: unconditional LDS reads (padding
 // slots hold value 0 and read the zero slot TRI_RING, s - 0 * 0 = s)
-template <int MAXP>
-__device__ __forceinline__ void tri_row_lds(const TriRow<MAXP>& row, const TriPack& a, double* ring, double* out,
-                                            int t) {
+template <int MAXP, bool SOR>
+__device__ __forceinline__ void tri_row_lds(const TriRow<MAXP, SOR>& row, const TriPack& a, const SorArgs& so,
+                                            double* ring, double* out, int t) {
     if (!row.ok) return;
     const TriRec<MAXP>& r = row.r;
     double s = row.in;
@@ -828,15 +876,13 @@ __device__ __forceinline__ void tri_row_lds(const TriRow<MAXP>& row, const TriPa
         const int o = a.ovp[t] - MAXP;
         for (int k = MAXP; k < r.cnt; k++) s -= a.ovval[o + k] * ring[a.ovsrc[o + k]];
     }
-    s /= r.diag;
-    ring[t & (TRI_RING - 1)] = s;
-    out[r.orow] = s;
+    tri_finish(row, so, s, ring, out, t);
 }
 
 // a level that also reads results through global memory
-template <int MAXP>
-__device__ __forceinline__ void tri_row_any(const TriRow<MAXP>& row, const TriPack& a, double* ring, double* out,
-                                            int t) {
+template <int MAXP, bool SOR>
+__device__ __forceinline__ void tri_row_any(const TriRow<MAXP, SOR>& row, const TriPack& a, const SorArgs& so,
+                                            double* ring, double* out, int t) {
     if (!row.ok) return;
     const TriRec<MAXP>& r = row.r;
     double s = row.in;
@@ -855,23 +901,22 @@ __device__ __forceinline__ void tri_row_any(const TriRow<MAXP>& row, const TriPa
             s -= a.ovval[o + k] * x;
         }
     }
-    s /= r.diag;
-    ring[t & (TRI_RING - 1)] = s;
-    out[r.orow] = s;
+    tri_finish(row, so, s, ring, out, t);
 }
 
 // Level offsets are read from LDS (staged TRI_OFFCAP at a time): a global
 // read of them would wait behind the row prefetches in flight.
 constexpr int TRI_OFFCAP = 16384;
 
-template <int MAXP, int D>
-__device__ __forceinline__ void tri_phase(const TriPack& a, const double* in, double* out, double* ring, int* loff) {
+template <int MAXP, int D, bool SOR>
+__device__ __forceinline__ void tri_phase(const TriPack& a, const SorArgs& so, const double* in, double* out,
+                                          double* ring, int* loff) {
     constexpr int CH = ((TRI_OFFCAP - 1 - D) / D) * D;  // levels per staged window
     constexpr int M = 0x7fffffff;
     const int tid = threadIdx.x;
     for (int q = tid; q < TRI_OFFCAP && q <= a.nlv; q += TRI_NT) loff[q] = a.off[q];
     __syncthreads();
-    TriRow<MAXP> st[D];
+    TriRow<MAXP, SOR> st[D];
 #pragma unroll
     for (int d = 0; d < D; d++) {
         // stage d's loads issued before stage d+1's, as in the loop
@@ -881,7 +926,7 @@ __device__ __forceinline__ void tri_phase(const TriPack& a, const double* in, do
             b = loff[d] & M;
             e = loff[d + 1] & M;
         }
-        tri_load(st[d], a, in, b + tid, e);
+        tri_load(st[d], a, so, in, b + tid, e);
     }
     __builtin_amdgcn_sched_barrier(0);
     int c0 = 0;
@@ -903,17 +948,17 @@ __device__ __forceinline__ void tri_phase(const TriPack& a, const double* in, do
                 const int b = v0 & M, e = loff[L + 1 - c0] & M;
                 const bool glob = v0 < 0;
                 if (glob)
-                    tri_row_any(st[d], a, ring, out, b + tid);
+                    tri_row_any(st[d], a, so, ring, out, b + tid);
                 else
-                    tri_row_lds(st[d], a, ring, out, b + tid);
+                    tri_row_lds(st[d], a, so, ring, out, b + tid);
                 // rows past the first TRI_NT of a wide level: loaded here
                 for (int t = b + TRI_NT + tid; t < e; t += TRI_NT) {
-                    TriRow<MAXP> r;
-                    tri_load(r, a, in, t, e);
+                    TriRow<MAXP, SOR> r;
+                    tri_load(r, a, so, in, t, e);
                     if (glob)
-                        tri_row_any(r, a, ring, out, t);
+                        tri_row_any(r, a, so, ring, out, t);
                     else
-                        tri_row_lds(r, a, ring, out, t);
+                        tri_row_lds(r, a, so, ring, out, t);
                 }
                 const int L2 = L + D;
                 int b2 = 0, e2 = 0;
@@ -921,7 +966,7 @@ __device__ __forceinline__ void tri_phase(const TriPack& a, const double* in, do
                     b2 = loff[L2 - c0] & M;
                     e2 = loff[L2 + 1 - c0] & M;
                 }
-                tri_load(st[d], a, in, b2 + tid, e2);
+                tri_load(st[d], a, so, in, b2 + tid, e2);
                 if (L + 1 < a.nlv && loff[L + 1 - c0] < 0)
                     __syncthreads();
                 else
@@ -937,9 +982,48 @@ __global__ __launch_bounds__(TRI_NT) void k_tri_pair(TriPack fw, TriPack bw, con
     __shared__ double ring[TRI_RING + 1];
     __shared__ int loff[TRI_OFFCAP];
     if (threadIdx.x == 0) ring[TRI_RING] = 0.0;
-    tri_phase<MAXP, D>(fw, zp, y, ring, loff);
+    const SorArgs none;
+    tri_phase<MAXP, D, false>(fw, none, zp, y, ring, loff);
     __syncthreads();
-    tri_phase<MAXP, D>(bw, y, w, ring, loff);
+    tri_phase<MAXP, D, false>(bw, none, y, w, ring, loff);
+}
+
+// one exact lexicographic SOR sweep: in = the gathered right-hand side (f,
+// or f minus the old entries' products when those come first in the rows),
+// out = x (natural order)
+template <int MAXP, int D>
+__global__ __launch_bounds__(TRI_NT) void k_sor_sweep(TriPack p, SorArgs so, const double* __restrict__ in,
+                                                       double* x) {
+    __shared__ double ring[TRI_RING + 1];
+    __shared__ int loff[TRI_OFFCAP];
+    if (threadIdx.x == 0) ring[TRI_RING] = 0.0;
+    tri_phase<MAXP, D, true>(p, so, in, x, ring, loff);
+}
+
+// before a sweep, per phase position t (row i): a_t = (1 - omega) x_i and,
+// for the row's entries whose values are the sweep's old ones (x before it),
+// old_first: in_t = f_i - sum of their products in row order; else in_t = f_i
+// and prod[k n + t] = their products (k < ocnt_t)
+__global__ void k_sor_gather(int n, int maxp, const int* __restrict__ row, const int* __restrict__ ocnt,
+                             const double* __restrict__ oval, const int* __restrict__ ocol,
+                             const int* __restrict__ ooff, const double* __restrict__ oovv,
+                             const int* __restrict__ oovc, int old_first, const double* __restrict__ f,
+                             const double* __restrict__ x, double omega, double* __restrict__ in,
+                             double* __restrict__ a, double* __restrict__ prod, double* __restrict__ prodov) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const int i = row[t], c = ocnt[t];
+        a[t] = (1.0 - omega) * x[i];
+        double s = f[i];
+        for (int k = 0; k < c; k++) {
+            const bool slot = k < maxp;
+            const int o = slot ? k * n + t : ooff[t] + k - maxp;
+            const double q = (slot ? oval[o] : oovv[o]) * x[slot ? ocol[o] : oovc[o]];
+            if (old_first) s -= q;
+            else if (slot) prod[o] = q;
+            else prodov[o] = q;
+        }
+        in[t] = s;
+    }
 }
 
 __global__ void k_gather_perm(int n, const int* __restrict__ perm, const double* __restrict__ z,
@@ -1898,6 +1982,23 @@ void tri_solve_pair(hipStream_t st, const TriPack& fw, const TriPack& bw, const 
         hipLaunchKernelGGL((k_tri_pair<8, 2>), dim3(1), dim3(TRI_NT), 0, st, fw, bw, zp, y, w);
     else
         hipLaunchKernelGGL((k_tri_pair<4, 2>), dim3(1), dim3(TRI_NT), 0, st, fw, bw, zp, y, w);
+}
+
+void sor_packed(hipStream_t st, const SorPack& s, const double* f, double* x, double omega, int sweeps) {
+    const int n = s.p.n;
+    if (n <= 0) return;
+    const SorArgs so{s.a, s.prod, omega, s.ocnt, s.ooff, s.prodov, s.old_first};
+    for (int k = 0; k < sweeps; k++) {
+        hipLaunchKernelGGL(k_sor_gather, dim3(std::min(1024, (n + 255) / 256)), dim3(256), 0, st, n, s.p.maxp, s.row,
+                           s.ocnt, s.oval, s.ocol, s.ooff, s.oovv, s.oovc, s.old_first, f, (const double*)x, omega,
+                           s.in, s.a, s.prod, s.prodov);
+        if (s.p.maxp > 4)
+            hipLaunchKernelGGL((k_sor_sweep<8, 2>), dim3(1), dim3(TRI_NT), 0, st, s.p, so, (const double*)s.in, x);
+        else
+            hipLaunchKernelGGL((k_sor_sweep<4, 2>), dim3(1), dim3(TRI_NT), 0, st, s.p, so, (const double*)s.in, x);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) throw std::runtime_error(std::string("k_sor_sweep launch: ") + hipGetErrorString(e));
+    }
 }
 
 void tri_pack_values(hipStream_t st, int n, int maxp, const int* vmap, const int* dmap, int nov, const int* ovmap,

@@ -237,6 +237,30 @@ struct TriPack {
     const double* ovval = nullptr;
     const int* ovsrc = nullptr;
 };
+// One exact lexicographic SOR sweep (k_sor_lex's arithmetic) over a matrix
+// packed by the sweep's level schedule: p holds, per phase position, the
+// diagonal and the entries whose values are the sweep's new ones (earlier
+// rows in sweep order), with their sources as in TriPack and orow = the
+// natural row; the entries whose values are the old ones (x before the
+// sweep) are listed per position: ocnt of them, the first p.maxp in
+// oval / ocol (slot-major), the rest from oovv / oovc[ooff[t] ...]; they
+// enter as products computed before the sweep (prod slot-major, then prodov
+// like oovv).  old_first: they come first in the rows (the backward sweep),
+// else after the new ones.  New entries past p.maxp use p's overflow arrays.
+struct SorPack {
+    TriPack p;
+    const int* row = nullptr;
+    const int* ocnt = nullptr;
+    const double* oval = nullptr;
+    const int* ocol = nullptr;
+    const int* ooff = nullptr;
+    const double* oovv = nullptr;
+    const int* oovc = nullptr;
+    int old_first = 0;
+    // scratch: n, n, maxp * n (zero when old_first), as many as oovv
+    double *in = nullptr, *a = nullptr, *prod = nullptr, *prodov = nullptr;
+};
+void sor_packed(hipStream_t st, const SorPack& s, const double* f, double* x, double omega, int sweeps);
 // forward phase fw: in = zp (z in L's phase order, gathered by perm), out = y
 // (bw's phase order); backward phase bw: in = y, out = w (natural order)
 void tri_solve_pair(hipStream_t st, const TriPack& fw, const TriPack& bw, const int* perm, const double* z,

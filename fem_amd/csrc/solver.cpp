@@ -545,6 +545,7 @@ void Solver::upload() {
             up_[l].lvb_off = off;
             up_[l].lvb_rows = rows;
         }
+        if (lex_smoothing()) build_sor_packs(l, B, false);
     }
     // finest-level extras
     signs_ = dupload(H_.signs, B, st);
@@ -690,6 +691,173 @@ void Solver::ensure_ctiles() {
     hip_check(hipStreamSynchronize(stream_), "sync");
 }
 
+// whether any level sweeps with the exact lexicographic SOR / SSOR
+bool Solver::lex_smoothing() const {
+    auto lex = [](int sm) { return sm == LINA_SMOOTHER_SOR || sm == LINA_SMOOTHER_SSOR; };
+    return lex(prm_.smoother) || lex(prm_.coarse_smoother);
+}
+
+// The level-packed forms (dev::SorPack) of level l's exact lexicographic
+// sweeps, from the host matrix and the level schedules: new entries (earlier
+// rows in the sweep's order, nonzero, as the schedule counts them) in the
+// records (past 4 or 8 in overflow arrays), old entries (all, zero or not)
+// in the slot-major lists and their overflow.  Not built (the level keeps
+// k_sor_lex) when a row lacks its diagonal.  reuse: the device arrays of the
+// level's packs are rewritten in place when their sizes still fit.
+void Solver::build_sor_packs(int l, DevBuffers& B, bool reuse) {
+    DevLevel& d = L_[l];
+    reuse = reuse && d.sor_packed;
+    d.sor_packed = false;
+    const Csr& A = H_.lv[l].A;
+    const int n = A.n;
+    if (n == 0) return;
+    hipStream_t st = stream_;
+    std::vector<int> off, rows;
+    int maxc = 0;
+    for (int i = 0; i < n; i++) {
+        int nl = 0, nu = 0, dg = 0;
+        for (int p = A.rp[i]; p < A.rp[i + 1]; p++) {
+            const int j = A.ci[p];
+            if (j == i) dg++;
+            else if (j < i) nl++;
+            else nu++;
+        }
+        if (dg != 1) return;
+        maxc = std::max(maxc, std::max(nl, nu));
+    }
+    const int maxp = maxc <= 4 ? 4 : 8;
+    for (int back = 0; back < 2; back++) {
+        level_schedule(A, back != 0, off, rows);
+        const int nlv = (int)off.size() - 1;
+        std::vector<int> pos(n), lev(n);
+        for (int L = 0; L < nlv; L++)
+            for (int t = off[L]; t < off[L + 1]; t++) {
+                pos[rows[t]] = t;
+                lev[rows[t]] = L;
+            }
+        const size_t N = (size_t)n;
+        std::vector<int> full(std::max(nlv, 1), 0), ocnt(n), ocol(maxp * N, 0), ovp(n, 0), ovsrc, ooff(n, 0), oovc;
+        std::vector<double> oval(maxp * N, 0.0), ovval, oovv;
+        std::vector<char> rec(N * (maxp == 4 ? sizeof(dev::TriRec<4>) : sizeof(dev::TriRec<8>)));
+        for (int t = 0; t < n; t++) {
+            const int i = rows[t], L = lev[i], e = off[L + 1];
+            double diag = 0;
+            double val[8];
+            int src[8];
+            for (int k = 0; k < 8; k++) {
+                val[k] = 0.0;
+                src[k] = dev::TRI_RING;
+            }
+            int kn = 0, ko = 0;
+            ovp[t] = (int)ovval.size();
+            ooff[t] = (int)oovv.size();
+            for (int p = A.rp[i]; p < A.rp[i + 1]; p++) {
+                const int j = A.ci[p];
+                if (j == i) {
+                    diag = A.v[p];
+                } else if ((j < i) != (back != 0)) {  // new value
+                    if (A.v[p] == 0.0) continue;
+                    int sj;
+                    if (e - pos[j] <= dev::TRI_RING) {
+                        sj = pos[j] & (dev::TRI_RING - 1);
+                    } else {
+                        sj = -(j + 1);
+                        full[L] = 1;
+                    }
+                    if (kn < maxp) {
+                        val[kn] = A.v[p];
+                        src[kn] = sj;
+                    } else {
+                        ovval.push_back(A.v[p]);
+                        ovsrc.push_back(sj);
+                    }
+                    kn++;
+                } else {  // old value
+                    if (ko < maxp) {
+                        oval[ko * N + t] = A.v[p];
+                        ocol[ko * N + t] = j;
+                    } else {
+                        oovv.push_back(A.v[p]);
+                        oovc.push_back(j);
+                    }
+                    ko++;
+                }
+            }
+            ocnt[t] = ko;
+            if (maxp == 4) {
+                dev::TriRec<4>& r = reinterpret_cast<dev::TriRec<4>*>(rec.data())[t];
+                r.diag = diag;
+                for (int k = 0; k < 4; k++) {
+                    r.val[k] = val[k];
+                    r.src[k] = src[k];
+                }
+                r.orow = i;
+                r.cnt = kn;
+            } else {
+                dev::TriRec<8>& r = reinterpret_cast<dev::TriRec<8>*>(rec.data())[t];
+                r.diag = diag;
+                for (int k = 0; k < 8; k++) {
+                    r.val[k] = val[k];
+                    r.src[k] = src[k];
+                }
+                r.orow = i;
+                r.cnt = kn;
+            }
+        }
+        for (int L = 0; L < nlv; L++)
+            if (full[L]) off[L] |= (int)0x80000000u;
+        dev::SorPack& S = back ? d.sorb : d.sorf;
+        std::vector<int>& used = back ? d.sor_sizes_b : d.sor_sizes_f;
+        const std::vector<int> sizes = {(int)ovval.size(), (int)oovv.size()};
+        if (reuse && used == sizes) {
+            // same pattern and schedules (checked by the caller) and overflow sizes
+            auto to = [&](const void* dst, const void* src, size_t bytes) {
+                if (bytes)
+                    hip_check(hipMemcpyAsync(const_cast<void*>(dst), src, bytes, hipMemcpyHostToDevice, st), "h2d");
+            };
+            to(S.p.off, off.data(), off.size() * sizeof(int));
+            to(S.p.rec, rec.data(), rec.size());
+            to(S.p.ovp, ovp.data(), ovp.size() * sizeof(int));
+            to(S.p.ovval, ovval.data(), ovval.size() * sizeof(double));
+            to(S.p.ovsrc, ovsrc.data(), ovsrc.size() * sizeof(int));
+            to(S.row, rows.data(), rows.size() * sizeof(int));
+            to(S.ocnt, ocnt.data(), ocnt.size() * sizeof(int));
+            to(S.oval, oval.data(), oval.size() * sizeof(double));
+            to(S.ocol, ocol.data(), ocol.size() * sizeof(int));
+            to(S.ooff, ooff.data(), ooff.size() * sizeof(int));
+            to(S.oovv, oovv.data(), oovv.size() * sizeof(double));
+            to(S.oovc, oovc.data(), oovc.size() * sizeof(int));
+            hip_check(hipStreamSynchronize(st), "sync");  // the host arrays die here
+            continue;
+        }
+        used = sizes;
+        S = dev::SorPack();
+        S.p.n = n;
+        S.p.nlv = nlv;
+        S.p.maxp = maxp;
+        S.p.off = dupload(off, B, st);
+        S.p.rec = dupload(rec, B, st);
+        S.p.ovp = dupload(ovp, B, st);
+        S.p.ovval = dupload(ovval, B, st);
+        S.p.ovsrc = dupload(ovsrc, B, st);
+        S.row = dupload(rows, B, st);
+        S.ocnt = dupload(ocnt, B, st);
+        S.oval = dupload(oval, B, st);
+        S.ocol = dupload(ocol, B, st);
+        S.ooff = dupload(ooff, B, st);
+        S.oovv = dupload(oovv, B, st);
+        S.oovc = dupload(oovc, B, st);
+        S.old_first = back;
+        S.in = dalloc<double>(n, B);
+        S.a = dalloc<double>(n, B);
+        S.prod = dalloc<double>(maxp * N, B);
+        S.prodov = dalloc<double>(std::max<size_t>(oovv.size(), 1), B);
+        dev::fill<double>(st, (int)(maxp * N), S.prod, 0.0);
+        hip_check(hipStreamSynchronize(st), "sync");  // the host arrays die here
+    }
+    d.sor_packed = true;
+}
+
 // host l1 norms of a matrix's rows as upload_mat forms them (VT arithmetic)
 template <typename VT>
 static std::vector<VT> row_l1(const Csr& A) {
@@ -781,6 +949,10 @@ bool Solver::refresh_values() {
         const int nnz = (int)Ac.v.size();
         put(d.Ac.v, std::move(Ac.v));
         dev::convert<double, float>(st, nnz, d.Ac.v, d.Acf.v);
+        if (d.sor_packed) {
+            hip_check(hipStreamSynchronize(st), "sync");
+            build_sor_packs(l, *bufs_, true);  // the packs hold values: rewritten in place
+        }
     }
     put(inv_, lu.dense);
     dev::gauss_jordan_inverse(st, nd_, inv_, cpiv_);
@@ -947,7 +1119,9 @@ void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post, int xinit) {
         case LINA_SMOOTHER_SOR:
         case LINA_SMOOTHER_SSOR: {
             bool back = (smoother == LINA_SMOOTHER_SSOR) && post;
-            if (back)
+            if (std::is_same<T, double>::value && d.sor_packed)
+                dev::sor_packed(st, back ? d.sorb : d.sorf, (const double*)f, (double*)x, (double)omega, sweeps);
+            else if (back)
                 dev::sor_lex(st, pickA<T>(d), d.nlv_b, d.lvb_off, d.lvb_rows, f, x, LevelVecs<T>::xo(d), omega,
                              sweeps, 1);
             else

@@ -43,6 +43,10 @@ struct DevLevel {
     int nlv_f = 0, nlv_b = 0;
     int *lvf_off = nullptr, *lvf_rows = nullptr, *lvb_off = nullptr, *lvb_rows = nullptr;
     int nlim = 0;  // rows touched by AMG_parallel_Jacobi_smoother: (n/dof)*dof
+    // level-packed exact lexicographic sweeps (FP64), forward and backward
+    bool sor_packed = false;
+    dev::SorPack sorf, sorb;
+    std::vector<int> sor_sizes_f, sor_sizes_b;  // overflow sizes the packs were allocated for
     // work vectors
     double *x = nullptr, *f = nullptr, *r = nullptr, *xo = nullptr;
     float *xf = nullptr, *ff = nullptr, *rf = nullptr, *xof = nullptr;
@@ -176,6 +180,8 @@ class Solver {
     // every pattern-derived structure matches what is on the device)
     void refresh_after_recompute();
     bool refresh_values();
+    bool lex_smoothing() const;
+    void build_sor_packs(int l, DevBuffers& B, bool reuse);
     struct UpLevel {  // host copy of what upload() derived from a level's pattern
         std::vector<int> rp, ci, colour, perm, lvf_off, lvf_rows, lvb_off, lvb_rows;
     };
