@@ -1980,8 +1980,10 @@ void tri_solve_pair(hipStream_t st, const TriPack& fw, const TriPack& bw, const 
     const int maxp = std::max(fw.maxp, bw.maxp);
     if (maxp > 4)
         hipLaunchKernelGGL((k_tri_pair<8, 2>), dim3(1), dim3(TRI_NT), 0, st, fw, bw, zp, y, w);
-    else
+    else if (maxp == 4)
         hipLaunchKernelGGL((k_tri_pair<4, 2>), dim3(1), dim3(TRI_NT), 0, st, fw, bw, zp, y, w);
+    else
+        hipLaunchKernelGGL((k_tri_pair<3, 2>), dim3(1), dim3(TRI_NT), 0, st, fw, bw, zp, y, w);
 }
 
 void sor_packed(hipStream_t st, const SorPack& s, const double* f, double* x, double omega, int sweeps) {
@@ -2006,7 +2008,9 @@ void tri_pack_values(hipStream_t st, int n, int maxp, const int* vmap, const int
     const int m = std::max(n, nov);
     if (m <= 0) return;
     const dim3 g(std::min(4096, (m + 255) / 256)), b(256);
-    if (maxp <= 4)
+    if (maxp == 3)
+        hipLaunchKernelGGL(k_tri_values<3>, g, b, 0, st, n, vmap, dmap, nov, ovmap, v, (TriRec<3>*)rec, ovval);
+    else if (maxp <= 4)
         hipLaunchKernelGGL(k_tri_values<4>, g, b, 0, st, n, vmap, dmap, nov, ovmap, v, (TriRec<4>*)rec, ovval);
     else
         hipLaunchKernelGGL(k_tri_values<8>, g, b, 0, st, n, vmap, dmap, nov, ovmap, v, (TriRec<8>*)rec, ovval);

@@ -209,7 +209,7 @@ void trsv(hipStream_t st, const int* rp, const int* ci, const double* v, int nlv
           const int* lv_rows, const double* z, double* w, int upper);
 // ILU(0) preconditioner w = U^-1 L^-1 z in one workgroup, both factors packed
 // by level: phase order t (level by level), per row t its right-hand side
-// in[t] and one record rec[t] (TriRec<maxp>, 64 or 112 bytes) holding the
+// in[t] and one record rec[t] (TriRec<maxp>, 48, 64 or 112 bytes) holding the
 // diagonal, the first maxp off-diagonal values and their sources in the
 // factor's row order, the output index and the off-diagonal count; entries
 // past maxp come from ovval/ovsrc[ovp[t] ...].  A source >= 0 is a slot of
@@ -228,7 +228,18 @@ struct alignas(16) TriRec {
     int orow;
     int cnt;
 };
-static_assert(sizeof(TriRec<4>) == 64 && sizeof(TriRec<8>) == 112, "record layout");
+// rows of at most 3 off-diagonal entries, all sources in the ring (the
+// 7-point stencils of the 2-D P1 factors): 48 bytes, 16-bit sources
+template <>
+struct alignas(16) TriRec<3> {
+    double diag;
+    double val[3];
+    int orow;
+    short src[3];
+    short cnt;
+    int pad;
+};
+static_assert(sizeof(TriRec<3>) == 48 && sizeof(TriRec<4>) == 64 && sizeof(TriRec<8>) == 112, "record layout");
 struct TriPack {
     int n = 0, nlv = 0, maxp = 0;
     const int* off = nullptr;     // nlv + 1 level offsets, bit 31: full[L]

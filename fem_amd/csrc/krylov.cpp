@@ -98,7 +98,7 @@ static int tri_max_count(const Csr& T, bool upper, bool pattern_only) {
 // ends (fewer than TRI_RING phase positions back) is read there, any other
 // from outpos in global memory.  The values are filled by pack_values.
 template <int MAXP>
-static void tri_pack(const Csr& T, bool upper, bool pattern_only, const std::vector<int>& off,
+static bool tri_pack(const Csr& T, bool upper, bool pattern_only, const std::vector<int>& off,
                      const std::vector<int>& rows, const std::vector<int>& outpos, DevBuffers& B, hipStream_t st,
                      IluSystem::TriDev& D) {
     const int n = T.n, nlv = (int)off.size() - 1;
@@ -131,6 +131,7 @@ static void tri_pack(const Csr& T, bool upper, bool pattern_only, const std::vec
             if (e - pos[j] <= dev::TRI_RING) {
                 sj = pos[j] & (dev::TRI_RING - 1);
             } else {
+                if (MAXP == 3) return false;  // 16-bit sources hold ring slots only
                 sj = -(outpos[j] + 1);
                 full[L] = 1;
             }
@@ -163,6 +164,7 @@ static void tri_pack(const Csr& T, bool upper, bool pattern_only, const std::vec
     D.ovmap = kupload(ovmap, B, st);
     D.ovval = kalloc<double>(ovmap.size(), B);
     P.ovval = D.ovval;
+    return true;
 }
 
 static dev::DMat<double> upload_csr(const Csr& A, DevBuffers& B, hipStream_t st) {
@@ -381,8 +383,14 @@ void IluSystem::upload(bool pattern_levels) {
     const int cl = tri_max_count(L_, false, pattern_levels), cu = tri_max_count(U_, true, pattern_levels);
     packed_ = n > 0 && cl >= 0 && cu >= 0;
     if (packed_) {
-        // one record width for both sweeps (one kernel runs them)
-        if (std::max(cl, cu) <= 4) {
+        // one record width for both sweeps (one kernel runs them); 48-byte
+        // records when every row fits 3 entries whose sources are in the ring
+        bool done = false;
+        if (std::max(cl, cu) <= 3)
+            done = tri_pack<3>(L_, false, pattern_levels, off, rows, posu, B, st, tf_) &&
+                   tri_pack<3>(U_, true, pattern_levels, offu, rowsu, natural, B, st, tb_);
+        if (done) {
+        } else if (std::max(cl, cu) <= 4) {
             tri_pack<4>(L_, false, pattern_levels, off, rows, posu, B, st, tf_);
             tri_pack<4>(U_, true, pattern_levels, offu, rowsu, natural, B, st, tb_);
         } else {
