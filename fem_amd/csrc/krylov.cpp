@@ -14,6 +14,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstdio>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -167,7 +170,8 @@ static bool tri_pack(const Csr& T, bool upper, bool pattern_only, const std::vec
     return true;
 }
 
-static dev::DMat<double> upload_csr(const Csr& A, DevBuffers& B, hipStream_t st) {
+// blocks: also the LDS row blocks of the SpMV (the factors need only the CSR)
+static dev::DMat<double> upload_csr(const Csr& A, DevBuffers& B, hipStream_t st, bool blocks = true) {
     dev::DMat<double> D;
     D.n = A.n;
     D.ncols = A.ncols;
@@ -175,6 +179,7 @@ static dev::DMat<double> upload_csr(const Csr& A, DevBuffers& B, hipStream_t st)
     D.rp = kupload(A.rp, B, st);
     D.ci = kupload(A.ci, B, st);
     D.v = kupload(A.v, B, st);
+    if (!blocks) return D;
     // LDS row blocks for the SpMV (same builder rule as the AMG levels)
     std::vector<int> starts;
     build_blocks(A, 0, A.n, dev::block_nnz<double>(), starts);
@@ -246,13 +251,21 @@ void IluSystem::refactor(const Csr& A) {
             build(A, true);
             return;
         }
+        static const bool timing = getenv("LINA_RN_TIMING") != nullptr;
+        const auto t0 = std::chrono::steady_clock::now();
         A_ = A;
         L_ = std::move(L);
         U_ = std::move(U);
         n_ = A.n;
         upload(true);
+        const auto t1 = std::chrono::steady_clock::now();
         plan_ = build_plan();
         device_ready_ = true;
+        if (timing) {
+            auto sec = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
+            fprintf(stderr, "[ilu] new pattern: upload %.3fs, plan %.3fs\n", sec(t0, t1),
+                    sec(t1, std::chrono::steady_clock::now()));
+        }
     } else {
         // A_ keeps the pattern; its values are not refreshed (only dA_'s are)
         hip_check(hipMemcpyAsync(dA_.v, A.v.data(), A.v.size() * sizeof(double), hipMemcpyHostToDevice, stream_),
@@ -280,7 +293,18 @@ void IluSystem::refactor(const Csr& A) {
 bool IluSystem::build_plan() {
     const int n = n_;
     std::vector<int4> m1(n), m2(n);
-    std::vector<int> mask(A_.nnz(), 0), pu;
+    std::vector<int> mask(A_.nnz(), 0), npair(n + 1, 0);
+    bool fits = true;
+    // U row k: the diagonal first, then the upper columns ascending; the
+    // entry of column j, or -1
+    auto u_entry = [&](int k, int j) {
+        const int* b = U_.ci.data() + U_.rp[k] + 1;
+        const int* e = U_.ci.data() + U_.rp[k + 1];
+        const int* f = std::lower_bound(b, e, j);
+        return (f != e && *f == j) ? (int)(f - U_.ci.data()) : -1;
+    };
+    // pass 1 (rows independent): counts, masks, row descriptors
+#pragma omp parallel for schedule(dynamic, 4096) reduction(&& : fits)
     for (int t = 0; t < n; t++) {
         const int i = hrows_l_[t];
         const int a0 = A_.rp[i], na = A_.rp[i + 1] - a0;
@@ -289,24 +313,38 @@ bool IluSystem::build_plan() {
             nl += A_.ci[a0 + m] < i;
             hasd |= A_.ci[a0 + m] == i;
         }
-        if (na > dev::ILU_MAXA || nl > dev::ILU_MAXL) return false;
+        if (na > dev::ILU_MAXA || nl > dev::ILU_MAXL) {
+            fits = false;
+            continue;
+        }
         m1[t] = make_int4(a0, na, L_.rp[i], U_.rp[i]);
-        m2[t] = make_int4(nl, hasd, (int)pu.size(), 0);
+        int cnt = 0;
         for (int m = 0; m < na; m++) {
             const int j = A_.ci[a0 + m], lim = std::min(m, nl);
             int mk = 0;
-            for (int q = 0; q < lim; q++) {
-                const int k = A_.ci[a0 + q];
-                // U row k: the diagonal first, then the upper columns ascending
-                const int* b = U_.ci.data() + U_.rp[k] + 1;
-                const int* e = U_.ci.data() + U_.rp[k + 1];
-                const int* f = std::lower_bound(b, e, j);
-                if (f != e && *f == j) {
+            for (int q = 0; q < lim; q++)
+                if (u_entry(A_.ci[a0 + q], j) >= 0) {
                     mk |= 1 << q;
-                    pu.push_back((int)(f - U_.ci.data()));
+                    cnt++;
                 }
-            }
             mask[a0 + m] = mk;
+        }
+        m2[t] = make_int4(nl, hasd, 0, 0);
+        npair[t + 1] = cnt;
+    }
+    if (!fits) return false;
+    for (int t = 0; t < n; t++) npair[t + 1] += npair[t];
+    std::vector<int> pu(npair[n]);
+    // pass 2: the pairs' U entries, slot order, entry after entry
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int t = 0; t < n; t++) {
+        const int a0 = m1[t].x, na = m1[t].y, nl = m2[t].x;
+        int r = npair[t];
+        m2[t].z = r;
+        for (int m = 0; m < na; m++) {
+            const int mk = mask[a0 + m], j = A_.ci[a0 + m];
+            for (int q = 0; q < std::min(m, nl); q++)
+                if ((mk >> q) & 1) pu[r++] = u_entry(A_.ci[a0 + q], j);
         }
     }
     DevBuffers& B = *bufs_;
@@ -357,8 +395,8 @@ void IluSystem::upload(bool pattern_levels) {
     DevBuffers& B = *bufs_;
     hipStream_t st = stream_;
     if (A_.n > 0) dA_ = upload_csr(A_, B, st);
-    dL_ = upload_csr(L_, B, st);
-    dU_ = upload_csr(U_, B, st);
+    dL_ = upload_csr(L_, B, st, false);
+    dU_ = upload_csr(U_, B, st, false);
     std::vector<int> off, rows;
     std::vector<int> offu, rowsu;
     tri_levels(L_, false, off, rows, pattern_levels);
