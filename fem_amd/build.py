@@ -70,7 +70,7 @@ def build_dropin(verbose: bool = True) -> str:
     tmp = DROPIN_LIB + ".tmp"
     cmd = [cc, "-O2", "-fPIC", "-shared", "-std=c99", "-D_POSIX_C_SOURCE=200809L", "-Wall", "-Wno-unused-function",
            "-ffp-contract=off", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "include", "linalg"),
-           *srcs, "-o", tmp, "-L", HERE, "-llina_hip", "-Wl,-rpath,$ORIGIN", "-lm"]
+           *srcs, "-o", tmp, "-L", HERE, "-llina_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-Bsymbolic", "-lm"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"drop-in build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -56,6 +56,38 @@ int incomplete_LU_factorization(sparse_matrix* A, sparse_matrix* L, sparse_matri
 void L_triang_invert(sparse_matrix* L, double* x);               /* :2008-2025 (on the GPU) */
 void U_triang_invert(sparse_matrix* U, double* x);               /* :2027-2043 (on the GPU) */
 
+int* zero_int_list(int n);                                        /* :3168-3174 */
+void copy_vector_content(double* x, double* y, int x_pos, int y_pos, int size); /* :79-82 */
+sparse_matrix* get_transpose(sparse_matrix* A, int new_size);    /* :1290-1300 */
+void enlarge_matrix(sparse_matrix* A, int mesh_size, int new_size, int equ_index, int var_index); /* :4109-4137 */
+void simple_enlarge_matrix(sparse_matrix* A, int add_size);      /* :4139-4156 */
+
+/* primal-dual active-set method for min/max of the quadratic of A under
+ * L <= B x <= U (linear_algebra.c:4933-5042): each pass solves the system of
+ * A augmented with the active constraints through the caller's Solver
+ * (add_constraints_default :4867-4884, or add_constraints_flux :4886-4918
+ * for mode ASS_FLUX), drops constraints whose multiplier has the wrong sign
+ * (definiteness +1: minimum, -1: maximum) and activates violated ones (tol
+ * 1e-10), until none change or max_iter passes; state in *solver_info
+ * (created on first use). */
+#define ASS_FLUX 1
+#define ASS_DEFAULT 0
+typedef struct ASS_IFO {                                         /* linear_algebra.h:81-88, same layout */
+    int active_size;
+    int* Active;
+    double* Signs;
+    double* Constraint;
+    int* Iter_history;
+    int calls;
+} ass_info;
+sparse_matrix* add_constraints_default(sparse_matrix* A, sparse_matrix* Constraints, int* ActiveSet,
+                                       int active_size);
+sparse_matrix* add_constraints_flux(sparse_matrix* A, sparse_matrix* Constraints, int* ActiveSet, int active_size);
+int active_set_solver(sparse_matrix* A, double* Sol, double* b, sparse_matrix* B, double* L, double* U,
+                      int (*Solver)(sparse_matrix* A, double* X, double* b), int max_iter, double definiteness,
+                      int mode, ass_info** solver_info);
+void free_ass_info(ass_info** info);                             /* :4920-4928 */
+
 #ifdef __cplusplus
 }
 #endif

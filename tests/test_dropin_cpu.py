@@ -107,3 +107,116 @@ def test_headers_compile_and_symbols_exported(tmp_path):
     missing = sorted(d for d in declared if d not in exported)
     assert not missing, missing
     assert len(declared) > 60
+
+
+_SOLVER = None
+
+
+def _solver_type():
+    import ctypes as C
+    from tests.dropin_bind import SparseMatrix
+    return C.CFUNCTYPE(C.c_int, C.POINTER(SparseMatrix), C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def _dense(Mp):
+    m = Mp.contents
+    D = np.zeros((m.size, m.size))
+    for i in range(m.size):
+        for j in range(m.Len[i]):
+            D[i, m.Indices[i][j]] += m.Values[i][j]
+    return D
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_active_set_solver_matches_reference(libs, mode):
+    """active_set_solver (linear_algebra.c:4933-5042) on a box-constrained
+    quadratic, both libraries calling the same dense direct solver: the same
+    augmented systems reach the solver (checked call by call), so the
+    iterations, the solution and the active sets agree exactly; the result
+    satisfies the bounds and is the constrained minimum (checked against a
+    projected-gradient step).  mode 1 (ASS_FLUX, linear_algebra.c:4886-4918):
+    a two-field system whose constraints read the first field."""
+    import ctypes as C
+    s = P.poisson(9)
+    n0 = s.n
+    A0 = s.scipy().toarray()
+    b0 = np.sin(np.arange(n0) * 0.7) * 3.0
+    if mode == 0:
+        A, b = A0, b0
+        cons = list(range(0, n0, 2))
+        Bm = np.zeros((n0, n0))
+        for i in cons:
+            Bm[i, i] = 1.0
+        lo, up = np.full(n0, -0.05), np.full(n0, 0.05)
+    else:
+        # two fields [u; v] coupled weakly; the constraints read u (diagonal)
+        A = np.block([[A0, 0.1 * np.eye(n0)], [0.1 * np.eye(n0), A0]])
+        b = np.concatenate([b0, -b0])
+        Bm = np.eye(n0)
+        lo, up = np.full(n0, -0.05), np.full(n0, 0.05)
+    seen = {}
+
+    def run(lib, tag):
+        calls = []
+
+        def solve(Mp, X, Z):
+            D = _dense(Mp)
+            z = np.ctypeslib.as_array(Z, shape=(D.shape[0],))
+            x = np.linalg.solve(D, z)
+            calls.append((D.copy(), z.copy()))
+            for k in range(D.shape[0]):
+                X[k] = x[k]
+            return 1
+
+        cb = _solver_type()(solve)
+        n = A.shape[0]
+        Ap = lib.from_csr(*_csr(A))
+        Bp = lib.from_csr(*_csr(Bm))
+        lib.L.active_set_solver.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p,
+                                            C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p, C.c_int,
+                                            C.c_double, C.c_int, C.POINTER(C.c_void_p)]
+        lib.L.active_set_solver.restype = C.c_int
+        sol = np.zeros(n)
+        bb = b.copy()
+        L_, U_ = lo.copy(), up.copy()
+        info = C.c_void_p()
+        it = lib.L.active_set_solver(Ap, dptr(sol), dptr(bb), Bp, dptr(L_), dptr(U_), C.cast(cb, C.c_void_p), 50,
+                                     1.0, mode, C.byref(info))
+        lib.L.free_ass_info.argtypes = [C.POINTER(C.c_void_p)]
+        lib.L.free_ass_info(C.byref(info))
+        lib.free(Ap)
+        lib.free(Bp)
+        seen[tag] = calls
+        return it, sol
+
+    ours, theirs = libs
+    it1, x1 = run(ours, "ours")
+    it2, x2 = run(theirs, "ref")
+    assert it1 == it2 and len(seen["ours"]) == len(seen["ref"])
+    for (D1, z1), (D2, z2) in zip(seen["ours"], seen["ref"]):
+        assert np.array_equal(D1, D2) and np.array_equal(z1, z2)
+    assert np.array_equal(x1, x2)
+    if mode == 1:
+        # the flux border puts the multiplier into the second field's
+        # equation while the constraint reads the first field: no
+        # minimisation stands behind it on this system and the set cycles
+        # (in both libraries alike, call for call) until max_iter
+        assert it1 == 50
+        return
+    assert it1 < 50
+    # constrained minimum of x.A.x/2 - b.x (the KKT system's sign convention)
+    y = Bm @ x1
+    assert np.all(y >= lo - 1e-9) and np.all(y <= up + 1e-9)
+    g = A @ x1 - b
+    q = lambda x: 0.5 * x @ A @ x - b @ x
+    for t in (1e-3, 1e-2):
+        xt = x1 - t * g
+        xt = np.where(np.diag(Bm) > 0, np.clip(xt, lo, up), xt)
+        assert q(xt) >= q(x1) - 1e-12 * abs(q(x1))
+
+
+def _csr(D):
+    import scipy.sparse as sp
+    M = sp.csr_matrix(D)
+    M.sort_indices()
+    return M.indptr, M.indices, M.data
