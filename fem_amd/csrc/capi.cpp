@@ -81,6 +81,7 @@ void check_params(const lina_amg_params* p) {
     if (p->coarse_smoother >= 0 && p->coarse_sweeps < 0) throw std::invalid_argument("negative sweeps");
     if (p->precision != LINA_PREC_FP64 && p->precision != LINA_PREC_MIXED)
         throw std::invalid_argument("unknown precision");
+    if (p->cycle != LINA_CYCLE_V && p->cycle != LINA_CYCLE_FULL) throw std::invalid_argument("unknown cycle");
 }
 
 }  // namespace

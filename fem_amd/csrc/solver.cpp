@@ -1241,6 +1241,45 @@ void Solver::vcycle(int l, const T* f, T* x, int xinit) {
     }
 }
 
+// AMG_full_multigrid / FMG_recursive (Multigrid.c:1582-1611) with the
+// recursion unrolled: the right-hand side restricted to every level
+// (sparse_mult of each restriction, the reference's CF), the coarsest level
+// solved, then on each finer level x = P x_coarse (sparse_mult, replacing
+// the level's iterate) followed by one V-cycle.  The reference also
+// restricts the incoming iterate down the levels (Coarse_X = R X), but the
+// coarsest solve overwrites it and every finer level's iterate is replaced
+// by the prolongation, so it never reaches the result and is not formed.
+// Each level's f/x vectors hold CF[l] and the level's iterate: the V-cycle
+// on level l writes only the vectors of the levels below, which the pass
+// no longer reads.
+template <typename T>
+void Solver::fmg(const T* f, T* x) {
+    hipStream_t st = stream_;
+    const int top = (int)L_.size() - 1;
+    for (int l = top; l >= 1; l--) {
+        DevLevel& d = L_[l];
+        const T* fl = l == top ? f : LevelVecs<T>::f(d);
+        // the coarsest level's gathered right-hand side comes from the
+        // restriction's epilogue when the coarse solve expects it there
+        dev::Epi<T, T> e;
+        const bool epi = l == 1 && fused_gather();
+        if (epi) {
+            e.bd = cbd_;
+            e.dinv = dinv_;
+        }
+        dev::spmv<T, T>(st, pickR<T>(d), fl, nullptr, LevelVecs<T>::f(L_[l - 1]),
+                        epi ? dev::MODE_SETX : dev::MODE_SET, nullptr, nullptr, epi ? &e : nullptr);
+    }
+    vcycle<T>(0, LevelVecs<T>::f(L_[0]), LevelVecs<T>::x(L_[0]), X_ZERO);
+    for (int l = 1; l <= top; l++) {
+        DevLevel& d = L_[l];
+        T* xl = l == top ? x : LevelVecs<T>::x(d);
+        const T* fl = l == top ? f : LevelVecs<T>::f(d);
+        dev::spmv<T, T>(st, pickP<T>(d), LevelVecs<T>::x(L_[l - 1]), nullptr, xl, dev::MODE_SET, nullptr, nullptr);
+        vcycle<T>(l, fl, xl, X_GUESS);
+    }
+}
+
 // |f - A x| on the finest level into scal[slot] (FP64 or the FP32 copy)
 template <typename T>
 double Solver::residual_norm(const T* f, const T* x, T* r, int slot) {
@@ -1254,7 +1293,8 @@ double Solver::residual_norm(const T* f, const T* x, T* r, int slot) {
 // AMG_solve (Multigrid.c:1863-1931) with F, X device vectors of the finest
 // size.  The diagnostic ILU residuals of the reference (res0, res) do not
 // influence the result except through `res0 != 0`, which holds exactly when
-// the first residual r0 is nonzero; they are not computed.
+// the first residual r0 is nonzero; they are not computed.  LINA_CYCLE_FULL
+// (the FULL_AMG case): one full-multigrid pass, counted as one cycle.
 int Solver::amg_solve_dev(const double* F, double* X) {
     hipStream_t st = stream_;
     DevLevel& top = L_.back();
@@ -1270,7 +1310,11 @@ int Solver::amg_solve_dev(const double* F, double* X) {
         double r0 = residual_norm<double>(Ft, X, rtop_, dev::S_T0);
         if (r0 == 0.0) return 0;
         double r = r0;
-        while (std::fabs(r / r0) > prm_.amg_eps && it < prm_.amg_max_iter) {
+        if (prm_.cycle == LINA_CYCLE_FULL) {
+            fmg<double>(Ft, X);
+            it = 1;
+        }
+        else while (std::fabs(r / r0) > prm_.amg_eps && it < prm_.amg_max_iter) {
             vcycle<double>((int)L_.size() - 1, Ft, X);
             it++;
             if (it < prm_.amg_max_iter) r = residual_norm<double>(Ft, X, rtop_, dev::S_T0);
@@ -1283,7 +1327,11 @@ int Solver::amg_solve_dev(const double* F, double* X) {
         double r0 = residual_norm<float>(ff, xf, top.rf, dev::S_T0);
         if (r0 != 0.0) {
             double r = r0;
-            while (std::fabs(r / r0) > prm_.amg_eps && it < prm_.amg_max_iter) {
+            if (prm_.cycle == LINA_CYCLE_FULL) {
+                fmg<float>(ff, xf);
+                it = 1;
+            }
+            else while (std::fabs(r / r0) > prm_.amg_eps && it < prm_.amg_max_iter) {
                 vcycle<float>((int)L_.size() - 1, ff, xf);
                 it++;
                 if (it < prm_.amg_max_iter) r = residual_norm<float>(ff, xf, top.rf, dev::S_T0);
@@ -1365,7 +1413,9 @@ int Solver::vcycle_once(const double* F, double* X, const double* w, int* np, bo
 
 // the preconditioner runs without host round trips (pcg's iterations can then
 // be queued ahead of the stop test)
-bool Solver::precon_async() const { return prm_.zero_init && prm_.amg_max_iter == 1; }
+bool Solver::precon_async() const {
+    return prm_.zero_init && prm_.amg_max_iter == 1 && prm_.cycle == LINA_CYCLE_V;
+}
 
 // pcg_xr can write the next preconditioner entry (vcycle_once's fused path;
 // not with make_positive_diag signs, whose signed right-hand side precon_out
@@ -1694,5 +1744,7 @@ template void Solver::smooth<double>(int, const double*, double*, int, bool, int
 template void Solver::smooth<float>(int, const float*, float*, int, bool, int);
 template void Solver::vcycle<double>(int, const double*, double*, int);
 template void Solver::vcycle<float>(int, const float*, float*, int);
+template void Solver::fmg<double>(const double*, double*);
+template void Solver::fmg<float>(const float*, float*);
 
 }  // namespace lina

@@ -155,6 +155,8 @@ class Solver {
     template <typename T>
     void vcycle(int l, const T* f, T* x, int xinit = X_GUESS);
     template <typename T>
+    void fmg(const T* f, T* x);
+    template <typename T>
     double residual_norm(const T* f, const T* x, T* r, int slot);
     double read_scalar(int slot);
 

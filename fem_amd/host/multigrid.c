@@ -80,8 +80,8 @@ amg_system_info* AMG_setup(sparse_matrix* A, int dof, int depth, int smoothing_i
     amg_side* sd;
     dropin_csr c;
     int l;
-    if (cycle_structure != STAND_ALONE)
-        fprintf(stderr, "liblinalg_hip: only the STAND_ALONE V-cycle is built; using it\n");
+    if (cycle_structure != STAND_ALONE && cycle_structure != FULL_AMG)
+        fprintf(stderr, "liblinalg_hip: unknown cycle structure %d; using STAND_ALONE\n", cycle_structure);
     if (coarsest_solver || Get_iter)
         fprintf(stderr, "liblinalg_hip: caller-defined coarse solvers and cycle counts are not used\n");
     sd = (amg_side*)calloc(1, sizeof(amg_side));
@@ -96,6 +96,7 @@ amg_system_info* AMG_setup(sparse_matrix* A, int dof, int depth, int smoothing_i
     sd->p.omega = g_alpha;
     sd->p.amg_eps = g_eps;
     sd->p.amg_max_iter = g_max_iter;
+    sd->p.cycle = cycle_structure == FULL_AMG ? LINA_CYCLE_FULL : LINA_CYCLE_V;
     c = dropin_to_csr(A);
     I = (amg_system_info*)calloc(1, sizeof(amg_system_info));
     I->device = lina_amg_create(c.n, c.rp, c.ci, c.v, &sd->p, 1);
@@ -116,7 +117,7 @@ amg_system_info* AMG_setup(sparse_matrix* A, int dof, int depth, int smoothing_i
         I->Fine_sizes[l] = f;
         I->Coarse_sizes[l] = co;
     }
-    I->AMG_cycle_structure = STAND_ALONE;
+    I->AMG_cycle_structure = sd->p.cycle == LINA_CYCLE_FULL ? FULL_AMG : STAND_ALONE;
     I->AMG_prolongation_method = prolongation_method;
     I->AMG_smoothing_iterations = smoothing_iterations;
     I->AMG_aggr_smoothing_iterations = aggr_smoothing_iterations;

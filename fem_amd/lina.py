@@ -27,7 +27,7 @@ class Params(C.Structure):
                 ("eps_plus", C.c_double), ("matrix_tol", C.c_double), ("smoother", C.c_int), ("omega", C.c_double),
                 ("amg_eps", C.c_double), ("amg_max_iter", C.c_int), ("zero_init", C.c_int),
                 ("precision", C.c_int), ("coarse_smoother", C.c_int), ("coarse_sweeps", C.c_int),
-                ("autotune", C.c_int)]
+                ("autotune", C.c_int), ("cycle", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -176,6 +176,10 @@ def params(kind: str = "reference", dof: int = 1, depth: int = 4, **kw) -> Param
             v = SMOOTHER[v]
         if k == "precision" and isinstance(v, str):
             v = PREC[v]
+        if k == "cycle" and isinstance(v, str):
+            v = {"v": 0, "full": 2}[v]
+        if k not in (f for f, _ in Params._fields_):
+            raise KeyError(f"unknown AMG parameter {k!r}")
         setattr(p, k, v)
     return p
 

@@ -46,6 +46,12 @@ extern "C" {
 #define LINA_PREC_MIXED 1         /* FP32 hierarchy + smoothers; FP64 outer Krylov,
                                      dots/norms and coarsest solve */
 
+#define LINA_CYCLE_V 0            /* STAND_ALONE (Multigrid.h:35): V-cycles until amg_eps / amg_max_iter */
+#define LINA_CYCLE_FULL 2         /* FULL_AMG (Multigrid.h:38): every AMG_solve (a Krylov
+                                     method's preconditioner calls included) is one
+                                     full-multigrid pass (AMG_full_multigrid,
+                                     Multigrid.c:1582-1611) */
+
 #define LINA_STOP_REFERENCE 0     /* pcg's test |R_k| / |M^-1 R_0| <= tol (PCGS.c:182,213) */
 #define LINA_STOP_RELRES 1        /* |R_k| <= tol with B scaled to unit norm, i.e. the
                                      relative residual |b-Ax|/|b| of the CG recurrence */
@@ -70,6 +76,7 @@ typedef struct lina_amg_params {
     int coarse_sweeps;             /* sweeps per call there (when coarse_smoother >= 0) */
     int autotune;                  /* 1: time the row-partition candidates of every level's kernels at upload
                                     * and keep the fastest (changes summation order of multi-lane rows) */
+    int cycle;                     /* LINA_CYCLE_*: AMG_setup's cycle_structure (AMG_solve's cycle) */
 } lina_amg_params;
 
 typedef struct lina_stats {

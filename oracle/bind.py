@@ -78,6 +78,7 @@ class Ref:
         L.refh_amg_setup.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
         L.refh_amg_setup.restype = _vp
         L.refh_amg_free.argtypes = [_vp]
+        L.refh_set_cycle.argtypes = [C.c_int]
         L.refh_amg_sizes.argtypes = [_vp, _ip, _ip]
         for f in ("refh_amg_A", "refh_amg_P", "refh_amg_R", "refh_amg_L", "refh_amg_U"):
             getattr(L, f).argtypes = [_vp, C.c_int]
@@ -126,9 +127,14 @@ class Ref:
         self.L.refh_free(M)
 
     # -- AMG hierarchy
-    def amg_setup(self, M, dof=1, depth=4, smiter=2, aggr_smiter=3, coarsening=1, matrix_tol=None):
-        return self.L.refh_amg_setup(M, dof, depth, smiter, aggr_smiter, coarsening,
-                                     0 if matrix_tol is None else 1, matrix_tol or 0.0)
+    def amg_setup(self, M, dof=1, depth=4, smiter=2, aggr_smiter=3, coarsening=1, matrix_tol=None, cycle="v"):
+        """cycle "v" (STAND_ALONE) or "full" (FULL_AMG, Multigrid.h:38)"""
+        self.L.refh_set_cycle({"v": 0, "full": 2}[cycle])
+        try:
+            return self.L.refh_amg_setup(M, dof, depth, smiter, aggr_smiter, coarsening,
+                                         0 if matrix_tol is None else 1, matrix_tol or 0.0)
+        finally:
+            self.L.refh_set_cycle(0)
 
     def amg_levels(self, I, depth):
         fine = np.zeros(depth, np.int32)
@@ -271,6 +277,7 @@ class Oracle:
         L.orc_amg_labels.argtypes = [_vp, C.c_int, _ip]
         L.orc_amg_recompute.argtypes = [_vp, _dp]
         L.orc_amg_solve.argtypes = [_vp, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int]
+        L.orc_amg_solve_fmg.argtypes = [_vp, _dp, _dp, C.c_int, C.c_double]
         L.orc_pcg_amg.argtypes = [_vp, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                   C.c_int, C.c_int, _dp, _ip]
         L.orc_colouring.argtypes = [C.c_int, _ip, _ip, _dp, _ip]
@@ -318,10 +325,15 @@ class Oracle:
             out.append(d)
         return out
 
-    def amg_solve(self, h, F, X, smoother="sor", alpha=1.6, eps=1e-2, max_iter=100):
+    def amg_solve(self, h, F, X, smoother="sor", alpha=1.6, eps=1e-2, max_iter=100, cycle="v"):
+        """cycle "v" (STAND_ALONE) or "full" (FULL_AMG: one full-multigrid
+        pass; eps and max_iter unused)"""
         F = np.array(F, dtype=np.float64)
         X = np.array(X, dtype=np.float64)
-        k = self.L.orc_amg_solve(h, dptr(F), dptr(X), SMOOTHERS[smoother], alpha, eps, max_iter)
+        if cycle == "full":
+            k = self.L.orc_amg_solve_fmg(h, dptr(F), dptr(X), SMOOTHERS[smoother], alpha)
+        else:
+            k = self.L.orc_amg_solve(h, dptr(F), dptr(X), SMOOTHERS[smoother], alpha, eps, max_iter)
         return k, X
 
     def pcg_amg(self, rowptr, col, val, b, x0=None, dof=1, depth=4, smiter=2, aggr_smiter=3, coarsening=1,

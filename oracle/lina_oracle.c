@@ -1307,6 +1307,67 @@ int orc_amg_solve(orc_amg* h, const double* Fin, double* X, int sm, double w, do
     return it;
 }
 
+/* sparse_mult (linear_algebra.c:134-147) of a transfer matrix, y = M x with
+ * x of length nx; P's seed entries past the coarse range are zero and skipped
+ * (the reference reads past the coarse vector there and multiplies by 0) */
+static double* transfer(const Mat* M, const double* x, int nx) {
+    double* y = (double*)malloc((M->n > 0 ? M->n : 1) * sizeof(double));
+    int i, p;
+    for (i = 0; i < M->n; i++) {
+        double s = 0;
+        for (p = M->rp[i]; p < M->rp[i + 1]; p++)
+            if (M->ci.a[p] < nx) s += M->v.a[p] * x[M->ci.a[p]];
+        y[i] = s;
+    }
+    return y;
+}
+
+/* FMG_recursive (Multigrid.c:1582-1593): the iterate restricted, the level
+ * below solved the same way, its result prolongated (replacing the level's
+ * iterate) and one V-cycle on CF[l] */
+static void fmg(orc_amg* h, double** CF, double* X, int l, int sm, double w) {
+    double *Xc, *Xf;
+    if (l == 0) {
+        coarse_solve(h, CF[0], X);
+        return;
+    }
+    Xc = transfer(&h->R[l], X, h->A[l].n);
+    fmg(h, CF, Xc, l - 1, sm, w);
+    Xf = transfer(&h->P[l], Xc, h->A[l - 1].n);
+    vcycle(h, l, CF[l], Xf, sm, w);
+    memcpy(X, Xf, h->A[l].n * sizeof(double));
+    free(Xc);
+    free(Xf);
+}
+
+/* AMG_solve (Multigrid.c:1863-1931), FULL_AMG: after the sign flip and
+ * AMG_fix_isolated as in the STAND_ALONE case, one AMG_full_multigrid
+ * (Multigrid.c:1595-1611) when r0 != 0: CF[depth-1] = F, CF[l-1] = R_l CF[l],
+ * FMG_recursive from the finest level; returns the cycle count (1, or 0
+ * when r0 == 0) */
+int orc_amg_solve_fmg(orc_amg* h, const double* Fin, double* X, int sm, double w) {
+    int d = h->depth - 1, n = h->A[d].n, i, p, l;
+    const Mat* A = &h->A[d];
+    double** CF;
+    double r0;
+    CF = (double**)malloc(h->depth * sizeof(double*));
+    CF[d] = (double*)malloc((n > 0 ? n : 1) * sizeof(double));
+    for (i = 0; i < n; i++) CF[d][i] = 0 + h->signs[i] * Fin[i];
+    for (i = 0; i < n; i++)
+        if (h->label[d][i] == ISOLATED)
+            for (p = A->rp[i]; p < A->rp[i + 1]; p++)
+                if (A->v.a[p] != 0) {
+                    X[i] = CF[d][i] / A->v.a[p];
+                    break;
+                }
+    r0 = residual_norm(A, X, CF[d]);
+    for (l = d; l > 0; l--) CF[l - 1] = transfer(&h->R[l], CF[l], h->A[l].n);
+    if (r0 != 0) fmg(h, CF, X, d, sm, w);
+    for (l = 0; l <= d; l++) free(CF[l]);
+    free(CF);
+    return r0 != 0;
+}
+
 /* pcg (PCGS.c:163-233) with an AMG_solve preconditioner (PCG_AMG_precon) */
 int orc_pcg_amg(orc_amg* h, double* X, double* B, double tol, int max_iter, int sm, double w, double amg_eps,
                 int amg_max_iter, int zero_init, int stop, double* res, int* vcycles) {

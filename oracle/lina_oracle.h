@@ -33,6 +33,7 @@ void orc_set_coarse_smoother(orc_amg* h, int sm, int sweeps);
 
 /* AMG_solve (Multigrid.c:1863-1931); smoother ids as lina_hip.h */
 int orc_amg_solve(orc_amg* h, const double* F, double* X, int smoother, double omega, double eps, int max_iter);
+int orc_amg_solve_fmg(orc_amg* h, const double* F, double* X, int smoother, double omega);
 
 /* pcg (PCGS.c:163-233) preconditioned by AMG_solve; stop 0: reference test
  * r/r0 <= tol, 1: |R| <= tol.  Returns iterations; *res = pcg residuum,

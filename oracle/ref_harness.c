@@ -278,12 +278,16 @@ static void set_smoother(int id) {
 
 /* ------------------------------------------------------------- AMG setup */
 
+/* cycle_structure of the next refh_amg_setup (STAND_ALONE or FULL_AMG) */
+static int h_cycle = STAND_ALONE;
+void refh_set_cycle(int cycle) { h_cycle = cycle; }
+
 /* AMG_setup with the reference's own arguments (Multigrid.c:1613-1754) */
 amg_system_info* refh_amg_setup(sparse_matrix* A, int dof, int depth, int smiter, int aggr_smiter,
                                 int coarsening, int matrix_tol_set, double matrix_tol) {
     AMG_print_info(NO);
     if (matrix_tol_set) AMG_set_matrix_tolerance(matrix_tol);
-    return AMG_setup(A, dof, depth, smiter, aggr_smiter, coarsening, STAND_ALONE, DIRECT, NULL, NULL);
+    return AMG_setup(A, dof, depth, smiter, aggr_smiter, coarsening, h_cycle, DIRECT, NULL, NULL);
 }
 
 void refh_amg_free(amg_system_info* I) {

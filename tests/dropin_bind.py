@@ -63,6 +63,10 @@ class LinA:
             "AMG_set_SOR_relaxation_coeff": ([C.c_double], None),
             "AMG_set_eps": ([C.c_double], None),
             "AMG_set_max_iter": ([C.c_int], None),
+            "AMG_setup": ([_SM, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                           C.c_void_p], C.c_void_p),
+            "AMG_solve": ([C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p], C.c_int),
+            "AMG_free_data": ([C.c_void_p], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

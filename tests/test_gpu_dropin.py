@@ -120,6 +120,33 @@ def test_pcg_host_callback_preconditioner(libs):
     assert abs(i1 - i2) <= 1 and rel(x1, x2) < 1e-11
 
 
+def test_amg_solve_full_multigrid_matches_reference(libs):
+    """AMG_setup with FULL_AMG (Multigrid.h:38) and AMG_solve: one
+    full-multigrid pass in both libraries, the same result to 1e-12."""
+    s = P.degraded_elasticity(17)
+
+    def run(lib):
+        A = lib.from_csr(s.rowptr, s.col, s.val)
+        lib.L.AMG_print_info(0)
+        # the smoother and AMG_solve's controls are library globals: set them
+        lib.L.AMG_set_smoother.argtypes = [C.c_void_p]
+        lib.L.AMG_set_smoother(C.cast(lib.L.AMG_SOR_smoother, C.c_void_p))
+        lib.L.AMG_set_SOR_relaxation_coeff(1.3)
+        lib.L.AMG_set_eps(1e-2)
+        lib.L.AMG_set_max_iter(100)
+        I = lib.L.AMG_setup(A, 2, 4, 2, 3, 1, 2, 1, None, None)
+        x = np.sin(np.arange(s.n) * 0.1)
+        f = s.b.copy()
+        it = lib.L.AMG_solve(I, dptr(f), dptr(x), None)
+        lib.L.AMG_free_data(I)
+        lib.free(A)
+        return it, x
+
+    (i1, x1), (i2, x2) = run(libs[0]), run(libs[1])
+    say("full multigrid iters", i1, i2, "rel", rel(x1, x2))
+    assert i1 == i2 == 1 and rel(x1, x2) < 1e-12
+
+
 def test_bicgstab_amg_matches_reference(libs):
     s = P.poisson(33)
 

@@ -195,6 +195,32 @@ def test_amg_solve_other_smoothers(gpu, smoother):
     assert rel(x, x_ref) < 1e-12
 
 
+@pytest.mark.parametrize("smoother", ["sor", "ssor", "mc_sor", "l1_jacobi"])
+def test_amg_solve_full_multigrid(gpu, smoother):
+    """AMG_solve with the FULL_AMG cycle (one full-multigrid pass): the
+    device pass against the oracle's, FP64 to 1e-12, on a scalar and a
+    two-field system, from a nonzero guess (which the pass replaces); the
+    mixed-precision pass within FP32 accuracy of the FP64 one."""
+    for s, depth in ((P.poisson(33), 4), (P.degraded_elasticity(25), 4)):
+        say("full multigrid", smoother, s.n)
+        orc = Oracle()
+        I = orc.amg_setup(s.rowptr, s.col, s.val, dof=s.dof, depth=depth)
+        x0 = np.cos(np.arange(s.n) * 0.3)
+        k_ref, x_ref = orc.amg_solve(I, s.b, x0, smoother=smoother, alpha=1.2, cycle="full")
+        orc.amg_free(I)
+        prm = gpu.params("reference", s.dof, depth, smoother=smoother, omega=1.2, cycle="full")
+        amg = gpu.AMG(s.rowptr, s.col, s.val, prm)
+        k, x = amg.amg_solve(s.b, x0)
+        amg.set_params(gpu.params("reference", s.dof, depth, smoother=smoother, omega=1.2, cycle="full",
+                                  precision="mixed"))
+        km, xm = amg.amg_solve(s.b, x0)
+        amg.close()
+        say("  oracle", k_ref, "gpu", k, "rel", rel(x, x_ref), "mixed rel", rel(xm, x_ref))
+        assert k == k_ref == 1 and km == 1
+        assert rel(x, x_ref) < 1e-12
+        assert rel(xm, x_ref) < 1e-5
+
+
 def test_mixed_precision_reaches_fp64_accuracy(gpu):
     s = P.degraded_elasticity(33)
     say("mixed precision elasticity 33")
