@@ -88,6 +88,14 @@ int active_set_solver(sparse_matrix* A, double* Sol, double* b, sparse_matrix* B
                       int mode, ass_info** solver_info);
 void free_ass_info(ass_info** info);                             /* :4920-4928 */
 
+/* sparse-matrix files: binary rows (int n, Len[n], indices, values), "(j|v)"
+ * text lines, MatrixMarket coordinate output (<Name>.mtx) */
+int write_sparse(sparse_matrix* A, char* path, char* name);               /* linear_algebra.c:832-844 */
+sparse_matrix* read_sparse(char* path, char* name);                       /* :846-868 */
+sparse_matrix* read_sparse_textfile(char* path, char* name);              /* :870-916 */
+void write_sparse_textfile(char* path, char* name, sparse_matrix* A);     /* :983-1002 */
+void print_to_matrix_market_format(sparse_matrix* A, char* Name);        /* :1023-1056 */
+
 #ifdef __cplusplus
 }
 #endif

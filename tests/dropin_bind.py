@@ -32,6 +32,11 @@ class LinA:
         self.L = L
         sig = {
             "sparse_zero": ([C.c_int], _SM),
+            "write_sparse": ([_SM, C.c_char_p, C.c_char_p], C.c_int),
+            "read_sparse": ([C.c_char_p, C.c_char_p], _SM),
+            "read_sparse_textfile": ([C.c_char_p, C.c_char_p], _SM),
+            "write_sparse_textfile": ([C.c_char_p, C.c_char_p, _SM], None),
+            "print_to_matrix_market_format": ([_SM, C.c_char_p], None),
             "insert_sparse": ([_SM, C.c_double, C.c_int, C.c_int], C.c_int),
             "set_sparse": ([_SM, C.c_double, C.c_int, C.c_int], None),
             "get_matrix_element": ([_SM, C.c_int, C.c_int], C.c_double),

@@ -220,3 +220,66 @@ def _csr(D):
     M = sp.csr_matrix(D)
     M.sort_indices()
     return M.indptr, M.indices, M.data
+
+
+def test_sparse_files_match_reference(libs, tmp_path):
+    """write_sparse / read_sparse (binary), write_sparse_textfile /
+    read_sparse_textfile ("(j|v)" lines) and print_to_matrix_market_format
+    (linear_algebra.c:832-1056): the drop-in writes byte-identical files, each
+    library reads the other's to the same row lists, the binary layout and
+    the MatrixMarket file decode independently (numpy, scipy) to the matrix,
+    and a hand-written text file with the format's corner cases (tokens
+    without '|', a missing ')', repeated columns, a last line without '\\n')
+    reads the same in both."""
+    import scipy.io
+    ours, theirs = libs
+    s = _system()
+    d = str(tmp_path).encode()
+    mats = []
+    for tag, lib in (("ours", ours), ("ref", theirs)):
+        M = lib.from_csr(s.rowptr, s.col, s.val)
+        lib.L.set_sparse(M, 0.0, 4, 4)                    # an explicit zero: not in the .mtx
+        assert lib.L.write_sparse(M, d, f"{tag}.bin".encode()) == 1
+        lib.L.write_sparse_textfile(d, f"{tag}.txt".encode(), M)
+        lib.L.print_to_matrix_market_format(M, os.path.join(str(tmp_path), tag).encode())
+        mats.append(LinA.to_lists(M))
+    assert mats[0] == mats[1]
+    for ext in ("bin", "txt", "mtx"):
+        assert (tmp_path / f"ours.{ext}").read_bytes() == (tmp_path / f"ref.{ext}").read_bytes(), ext
+    # independent decoding: binary layout, MatrixMarket
+    raw = (tmp_path / "ours.bin").read_bytes()
+    n = int(np.frombuffer(raw[:4], np.int32)[0])
+    lens = np.frombuffer(raw[4:4 + 4 * n], np.int32)
+    nnz = int(lens.sum())
+    idx = np.frombuffer(raw[4 + 4 * n:4 + 4 * n + 4 * nnz], np.int32)
+    val = np.frombuffer(raw[4 + 4 * n + 4 * nnz:], np.float64)
+    assert n == s.n and len(val) == nnz
+    assert list(idx) == [c for r in mats[0] for c in r[0]] and list(val) == [v for r in mats[0] for v in r[1]]
+    D = np.zeros((n, n))
+    for i, (cs, vs) in enumerate(mats[0]):
+        for c, v in zip(cs, vs):
+            D[i, c] += v
+    mm = scipy.io.mmread(str(tmp_path / "ours.mtx")).toarray()
+    assert mm.shape == D.shape and np.allclose(mm, D, rtol=1e-5, atol=0)
+    # cross reads
+    for ext, fn in (("bin", "read_sparse"), ("txt", "read_sparse_textfile")):
+        got = []
+        for lib, src in ((ours, "ref"), (theirs, "ours")):
+            R = getattr(lib.L, fn)(d, f"{src}.{ext}".encode())
+            got.append(LinA.to_lists(R))
+            lib.free(R)
+        assert got[0] == got[1]
+        if ext == "bin":
+            assert got[0] == mats[0]
+    # corner cases of the text format
+    (tmp_path / "odd.txt").write_text("(0|1.5)  (2|-2.25)   junk (0|0.5)\n(1|3.25  (1|1e-3)\n\n(2|7.0)\n(0|9.0)")
+    got = []
+    for lib in (ours, theirs):
+        R = lib.L.read_sparse_textfile(d, b"odd.txt")
+        got.append(LinA.to_lists(R))
+        lib.free(R)
+    assert got[0] == got[1] and len(got[0]) == 4
+    # sparse_zero's (i, 0.0) seeds plus the entries; "3.25" without ')' reads as 3.2
+    assert got[0][0] == ([0, 2], [2.0, -2.25]) and got[0][1] == ([1], [0.0 + 3.2 + 1e-3])
+    assert got[0][2] == ([2], [0.0]) and got[0][3] == ([2, 3], [7.0, 0.0])
+    assert not ours.L.read_sparse(d, b"missing.bin")
