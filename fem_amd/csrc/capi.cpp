@@ -431,6 +431,39 @@ int lina_matrix_spmv_device(lina_matrix* h, const double* x, double* y) {
     });
 }
 
+int lina_matrix_power(lina_matrix* h, double* v, int max_iter, double eps, double* eig) {
+    return guard([&]() {
+        if (!h || !v || !eig || max_iter < 1) throw std::invalid_argument("bad arguments");
+        const int n = h->m.n();
+        const double q = h->m.matrix_norm();
+        hipStream_t st = h->m.stream();
+        double *dv = nullptr, *tmp = nullptr, *part = nullptr, *scal = nullptr, *host = nullptr;
+        auto release = [&]() {
+            if (dv) (void)hipFree(dv);
+            if (tmp) (void)hipFree(tmp);
+            if (part) (void)hipFree(part);
+            if (scal) (void)hipFree(scal);
+            if (host) (void)hipHostFree(host);
+        };
+        try {
+            lina::hip_check(hipMalloc((void**)&dv, n * sizeof(double)), "hipMalloc");
+            lina::hip_check(hipMalloc((void**)&tmp, n * sizeof(double)), "hipMalloc");
+            lina::hip_check(hipMalloc((void**)&part, ((size_t)n + 8192) * sizeof(double)), "hipMalloc");
+            lina::hip_check(hipMalloc((void**)&scal, 16 * sizeof(double)), "hipMalloc");
+            lina::hip_check(hipHostMalloc((void**)&host, 16 * sizeof(double)), "hipHostMalloc");
+            lina::hip_check(hipMemcpyAsync(dv, v, n * sizeof(double), hipMemcpyHostToDevice, st), "h2d");
+            const int it = lina::power_iteration(st, h->m, q, dv, tmp, part, scal, host, max_iter, eps, eig);
+            lina::hip_check(hipMemcpyAsync(v, dv, n * sizeof(double), hipMemcpyDeviceToHost, st), "d2h");
+            lina::hip_check(hipStreamSynchronize(st), "sync");
+            release();
+            return it;
+        } catch (...) {
+            release();
+            throw;
+        }
+    });
+}
+
 double lina_matrix_time_spmv(lina_matrix* h, int precision, int reps) {
     double out = -1.0;
     int rc = guard([&]() {

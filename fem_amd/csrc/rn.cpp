@@ -35,6 +35,42 @@
 
 namespace lina {
 
+// power_method (linear_algebra.c:4802-4822) with A's device SpMV: v (device,
+// length n) holds the start vector and ends as the last normalised iterate;
+// *eig enters as the estimate the first convergence test compares with (the
+// reference reads its caller's variable there) and leaves as the last
+// estimate.  q = matrix_norm(A) (linear_algebra.c:3441-3450).  Each
+// iteration: Prev = v, v = A Prev, eig = v.Prev, v /= |v|, stop when
+// |eig - prev| / q <= eps.  tmp: n doubles, part: block partials of the
+// dots, scal: 1 device double, host: 1 pinned double.  Returns the
+// iteration count, 0 when max_iter is reached (the reference's return).
+int power_iteration(hipStream_t st, DevMatrix& A, double q, double* v, double* tmp, double* part, double* scal,
+                    double* host, int max_iter, double eps, double* eig) {
+    const int n = A.n();
+    auto dot = [&](const double* a, const double* b) {
+        int np = dev::dot_part<double>(st, n, a, b, part);
+        dev::finalize(st, np, part, scal, 0, 0);
+        hip_check(hipMemcpyAsync(host, scal, sizeof(double), hipMemcpyDeviceToHost, st), "d2h");
+        hip_check(hipStreamSynchronize(st), "sync");
+        return host[0];
+    };
+    int i = 0;
+    double r;
+    do {
+        const double prev = *eig;
+        dev::copy(st, n, (const double*)v, tmp);
+        if (A.stream() != st) hip_check(hipStreamSynchronize(st), "sync");
+        A.launch(tmp, nullptr, v, dev::MODE_SET, nullptr, nullptr);
+        if (A.stream() != st) hip_check(hipStreamSynchronize(A.stream()), "sync");
+        *eig = dot(v, tmp);
+        const double s = 1. / std::sqrt(dot(v, v));
+        dev::scale(st, n, v, s);
+        r = std::fabs(*eig - prev) / q;
+        i++;
+    } while (r > eps && i < max_iter);
+    return i == max_iter ? 0 : i;
+}
+
 namespace {
 
 // ------------------------------------------------------------ host math
@@ -505,27 +541,14 @@ class ReflectiveNewton {
         return status;
     }
 
-    // power_method (linear_algebra.c:4802-4822) on A; the first eigenvalue
-    // estimate starts at 0 (the reference reads an uninitialised variable)
+    // power_method on A; the first eigenvalue estimate starts at 0 (the
+    // reference reads an uninitialised variable there)
     double power(DevMatrix& A, const Csr& Ah, double* v, int max_iter, double eps) {
         double q = 0;
         for (double a : Ah.v) q += std::fabs(a);
         q = q / Ah.n;  // matrix_norm (:3441-3450)
-        double eig = 0, prev;
-        int i = 0;
-        double r;
-        do {
-            prev = eig;
-            dev::copy(st_, n_, (const double*)v, T_);
-            sync_with(A);
-            A.launch(T_, nullptr, v, dev::MODE_SET, nullptr, nullptr);
-            hip_check(hipStreamSynchronize(A.stream()), "sync");
-            eig = dot(v, T_);
-            double s = 1. / norm(v);
-            dev::scale(st_, n_, v, s);
-            r = std::fabs(eig - prev) / q;
-            i++;
-        } while (r > eps && i < max_iter);
+        double eig = 0;
+        power_iteration(st_, A, q, v, T_, part_, scal_, host_, max_iter, eps, &eig);
         return eig;
     }
     void random_unit(double* v) {

@@ -1677,6 +1677,15 @@ bool DevMatrix::update_values(const Csr& A0) {
     return true;
 }
 
+double DevMatrix::matrix_norm() const {
+    std::vector<double> v(A_.nnz);
+    hip_check(hipMemcpyAsync(v.data(), A_.v, v.size() * sizeof(double), hipMemcpyDeviceToHost, stream_), "d2h");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    double s = 0;
+    for (double a : v) s += std::fabs(a);
+    return s / n_;
+}
+
 DevMatrix::~DevMatrix() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     bufs_.reset();

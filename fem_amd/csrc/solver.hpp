@@ -85,6 +85,8 @@ class DevMatrix {
     void spmv_host(int precision, const double* x, double* y);
     double time_spmv(int precision, int reps);
     double bytes(int precision) const;
+    // matrix_norm (linear_algebra.c:3441-3450): sum of |a_ij| in row order / n
+    double matrix_norm() const;
     hipStream_t stream() const { return stream_; }
 
   private:
@@ -99,6 +101,10 @@ class DevMatrix {
     double *x_ = nullptr, *y_ = nullptr;
     float *xf_ = nullptr, *yf_ = nullptr;
 };
+
+// power_method (linear_algebra.c:4802-4822) on the device (rn.cpp)
+int power_iteration(hipStream_t st, DevMatrix& A, double q, double* v, double* tmp, double* part, double* scal,
+                    double* host, int max_iter, double eps, double* eig);
 
 class Solver {
   public:

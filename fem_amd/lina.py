@@ -116,6 +116,7 @@ def lib():
     L.lina_matrix_create.restype = C.c_void_p
     L.lina_matrix_destroy.argtypes = [C.c_void_p]
     L.lina_matrix_spmv.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    L.lina_matrix_power.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, _dp]
     L.lina_matrix_spmv_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.lina_matrix_time_spmv.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.lina_matrix_time_spmv.restype = C.c_double
@@ -345,6 +346,15 @@ class Matrix:
 
     def __del__(self):
         self.close()
+
+    def power(self, v, max_iter=1000, eps=1e-6, eig0=0.0):
+        """power_method from the start vector v: (iterations or 0 at max_iter,
+        eigenvalue estimate, last normalised iterate)"""
+        v = np.array(v, dtype=np.float64)
+        eig = np.array([eig0], dtype=np.float64)
+        it = lib().lina_matrix_power(self.h, _d(v), max_iter, eps, _d(eig))
+        _check(it)
+        return it, float(eig[0]), v
 
     def spmv(self, x, precision="fp64"):
         x = np.ascontiguousarray(x, dtype=np.float64)

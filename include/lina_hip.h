@@ -221,6 +221,15 @@ int lina_matrix_spmv(lina_matrix* h, int precision, const double* x, double* y);
 /* device pointers (FP64); returns after the product is complete */
 int lina_matrix_spmv_device(lina_matrix* h, const double* x, double* y);
 double lina_matrix_time_spmv(lina_matrix* h, int precision, int reps);
+/* power_method (LinA/linear_algebra.c:4802-4822), the eigenvalue estimate of
+ * the reflective Newton's negative-curvature search (get_neg_curvature_direction,
+ * :4849-4865): power iterations v <- A v / |A v| from the start vector v
+ * (host, length n; ends as the last normalised iterate), the estimate
+ * v_new.v_old / |v_old|^2 with |v_old| = 1, until |change| / matrix_norm(A) <= eps.
+ * *eig: on entry the estimate the first test compares with (the reference
+ * reads its caller's variable), on exit the last estimate.  Returns the
+ * iteration count, 0 when max_iter is reached, negative on error. */
+int lina_matrix_power(lina_matrix* h, double* v, int max_iter, double eps, double* eig);
 /* the algorithmic bytes of one product, counted as for lina_spmv_bytes */
 double lina_matrix_spmv_bytes(const lina_matrix* h, int precision);
 /* pcg (PCGS.c:163-233) with the operator op (PCG_set_mat_mult with a device

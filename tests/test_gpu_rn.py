@@ -101,3 +101,54 @@ def test_reflective_newton_gmres_local_solver(gpu, ref, h_scale):
     assert rel < 1e-6
     q_ref = lbfgsb_min(s, dold, ones)
     assert g["q"] <= q_ref + 1e-7 * abs(q_ref)
+
+
+@pytest.mark.parametrize("case", ["damage", "shifted"])
+def test_power_method_matches_reference(gpu, ref, case):
+    """power_method (linear_algebra.c:4802-4822), the eigenvalue search of
+    the negative-curvature branch, on the device (lina_matrix_power, the
+    routine the reflective Newton calls) against the compiled reference's
+    power_method from the same start vector (the reference draws it with
+    srand(time(NULL)); here it is fixed): the same iteration count (up to
+    the rounding of the stopping test) and estimate, the same iterate.  "shifted" is get_neg_curvature_direction's second matrix,
+    A - s I with s the largest eigenvalue, whose dominant eigenvalue plus s
+    is A's smallest.  Run to a tight tolerance, the estimates agree with
+    scipy's eigenvalues."""
+    import ctypes as C
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import eigsh
+    from tests.dropin_bind import LinA, dptr
+    s = P.damage_system(17, h_scale=1.0)[0]
+    A = s.scipy().tocsr()
+    lam = eigsh(A, k=1, which="LA", return_eigenvectors=False)[0]
+    lmin = eigsh(A, k=1, which="SA", return_eigenvectors=False)[0]
+    if case == "shifted":
+        A = (A - lam * sp.identity(s.n, format="csr")).tocsr()
+        A.sort_indices()
+    rng = np.random.default_rng(3)
+    v0 = rng.random(s.n)
+    v0 /= np.linalg.norm(v0)
+    R = LinA(ref.L)
+    R.L.power_method.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int, C.c_double,
+                                 C.c_void_p, C.c_double]
+    R.L.power_method.restype = C.c_int
+    M = R.from_csr(A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data)
+    dM = gpu.Matrix(A.indptr, A.indices, A.data)
+    for eps, max_iter in ((1e-6, 1000), (1e-13, 20000)):
+        vr = v0.copy()
+        er = C.c_double(0.0)
+        it_r = R.L.power_method(M, C.byref(er), dptr(vr), max_iter, eps, None, 0.0)
+        it, e, v = dM.power(v0, max_iter=max_iter, eps=eps)
+        say(case, "eps", eps, "iters ref", it_r, "gpu", it, "eig", er.value, e)
+        # near a tight threshold the estimate changes by ~1e-16 per iteration:
+        # the stopping iteration may move by a few among 18k
+        assert it > 0 and abs(it - it_r) <= max(1, it_r // 2000)
+        assert abs(e - er.value) <= 1e-9 * abs(er.value)
+        if it == it_r:
+            assert np.linalg.norm(v - vr) <= 1e-9
+    if case == "damage":
+        assert abs(e - lam) <= 1e-6 * abs(lam)
+    else:
+        assert abs(e + lam - lmin) <= 1e-6 * abs(lam)
+    dM.close()
+    R.free(M)
