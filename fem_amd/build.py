@@ -21,7 +21,8 @@ LIB = os.path.join(HERE, "liblina_hip.so")
 # the LinA drop-in: C host code with LinA's entry points over liblina_hip's C ABI
 HOST = os.path.join(HERE, "host")
 DROPIN_LIB = os.path.join(HERE, "liblinalg_hip.so")
-DROPIN_SOURCES = ["linalg.c", "pcgs.c", "multigrid.c", "bicgstab.c", "reflective_newton.c", "sparse_io.c"]
+DROPIN_SOURCES = ["linalg.c", "pcgs.c", "multigrid.c", "bicgstab.c", "reflective_newton.c", "sparse_io.c",
+                  "eigen.c"]
 
 SOURCES = ["amg_setup.cpp", "solver.cpp", "capi.cpp", "kernels.hip", "krylov.cpp", "rn.cpp", "batch.cpp",
            "dropin.cpp"]

@@ -88,6 +88,16 @@ int active_set_solver(sparse_matrix* A, double* Sol, double* b, sparse_matrix* B
                       int mode, ass_info** solver_info);
 void free_ass_info(ass_info** info);                             /* :4920-4928 */
 
+/* eigenvalue searches (power iterations on the GPU) */
+double matrix_norm(sparse_matrix* A);                                     /* linear_algebra.c:3441-3450 */
+void set_random_normal_vector(double* X, int n);                          /* :4824-4829 */
+int power_method(sparse_matrix* A, double* max_eig, double* Max_vec, int max_iter, double eps,
+                 sparse_matrix* Original, double eps_neg);                /* :4802-4822 */
+int get_most_negative_eigenvector(sparse_matrix* A, double* eigenvalue, double* Eigenvector, int max_iter,
+                                  double eps);                            /* :4831-4847 */
+int get_neg_curvature_direction(sparse_matrix* A, double* curvature, double* Direction, int max_iter, double eps,
+                                double neg_eps);                          /* :4849-4865 */
+
 /* sparse-matrix files: binary rows (int n, Len[n], indices, values), "(j|v)"
  * text lines, MatrixMarket coordinate output (<Name>.mtx) */
 int write_sparse(sparse_matrix* A, char* path, char* name);               /* linear_algebra.c:832-844 */

@@ -33,6 +33,10 @@ class LinA:
         sig = {
             "sparse_zero": ([C.c_int], _SM),
             "write_sparse": ([_SM, C.c_char_p, C.c_char_p], C.c_int),
+            "matrix_norm": ([_SM], C.c_double),
+            "power_method": ([_SM, _dp, _dp, C.c_int, C.c_double, _SM, C.c_double], C.c_int),
+            "get_most_negative_eigenvector": ([_SM, _dp, _dp, C.c_int, C.c_double], C.c_int),
+            "get_neg_curvature_direction": ([_SM, _dp, _dp, C.c_int, C.c_double, C.c_double], C.c_int),
             "read_sparse": ([C.c_char_p, C.c_char_p], _SM),
             "read_sparse_textfile": ([C.c_char_p, C.c_char_p], _SM),
             "write_sparse_textfile": ([C.c_char_p, C.c_char_p, _SM], None),

@@ -204,3 +204,50 @@ def test_reflective_newton_matches_reference(libs):
     (i1, x1), (i2, x2) = run(libs[0]), run(libs[1])
     say("reflective newton iters", i1, i2, "rel", rel(x1, x2))
     assert i1 == i2 and rel(x1, x2) < 1e-9
+
+
+def test_eigenvalue_searches_match_reference(libs):
+    """power_method, get_most_negative_eigenvector and
+    get_neg_curvature_direction (linear_algebra.c:4802-4865), which Crack
+    calls directly (crack.c:8995, 10528, 10648): power_method from the same
+    start vector in both libraries (iterations within the rounding of the
+    stopping test, estimate and iterate to 1e-9); the two-stage searches
+    (start vectors drawn from the time, as in the reference) reach scipy's
+    smallest eigenvalue, of a positive definite and of an indefinite matrix,
+    in both libraries; matrix_norm bit-identical."""
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import eigsh
+    s = P.damage_system(17, h_scale=1.0)[0]
+    A0 = s.scipy().tocsr()
+    lmax = eigsh(A0, k=1, which="LA", return_eigenvectors=False)[0]
+    lmin = eigsh(A0, k=1, which="SA", return_eigenvectors=False)[0]
+    c = 0.5 * (lmin + eigsh(A0, k=2, which="SA", return_eigenvectors=False).max())
+    Ai = (A0 - c * sp.identity(s.n, format="csr")).tocsr()
+    Ai.sort_indices()
+    v0 = np.random.default_rng(7).random(s.n)
+    v0 /= np.linalg.norm(v0)
+    out = []
+    for lib in libs:
+        M = lib.from_csr(A0.indptr.astype(np.int32), A0.indices.astype(np.int32), A0.data)
+        Mi = lib.from_csr(Ai.indptr.astype(np.int32), Ai.indices.astype(np.int32), Ai.data)
+        v = v0.copy()
+        e = np.zeros(1)
+        it = lib.L.power_method(M, dptr(e), dptr(v), 1000, 1e-8, None, 0.0)
+        d = np.zeros(s.n)
+        lo = np.zeros(1)
+        lib.L.get_most_negative_eigenvector(M, dptr(lo), dptr(d), 100000, 1e-12)
+        cu = np.zeros(1)
+        lib.L.get_neg_curvature_direction(Mi, dptr(cu), dptr(d), 100000, 1e-12, 1e-10)
+        out.append((it, e[0], v, lo[0], cu[0], lib.L.matrix_norm(M)))
+        lib.free(M)
+        lib.free(Mi)
+    (i1, e1, v1, lo1, cu1, q1), (i2, e2, v2, lo2, cu2, q2) = out
+    say("power iters", i1, i2, "eig", e1, e2, "lowest", lo1, lo2, lmin, "curvature", cu1, cu2, lmin - c)
+    assert q1 == q2
+    assert i1 > 0 and abs(i1 - i2) <= 1 and abs(e1 - e2) <= 1e-9 * abs(e2)
+    if i1 == i2:
+        assert np.linalg.norm(v1 - v2) <= 1e-9
+    for lo in (lo1, lo2):
+        assert abs(lo - lmin) <= 1e-6 * lmax
+    for cu in (cu1, cu2):
+        assert cu < 0 and abs(cu - (lmin - c)) <= 1e-6 * lmax
