@@ -21,11 +21,10 @@ LIB = os.path.join(HERE, "liblina_hip.so")
 # the LinA drop-in: C host code with LinA's entry points over liblina_hip's C ABI
 HOST = os.path.join(HERE, "host")
 DROPIN_LIB = os.path.join(HERE, "liblinalg_hip.so")
-DROPIN_SOURCES = ["linalg.c", "pcgs.c", "multigrid.c", "bicgstab.c", "reflective_newton.c", "sparse_io.c",
+DROPIN_SOURCES = ["linalg.c", "active_set.c", "pcgs.c", "multigrid.c", "bicgstab.c", "reflective_newton.c", "sparse_io.c",
                   "eigen.c"]
 
-SOURCES = ["amg_setup.cpp", "solver.cpp", "capi.cpp", "kernels.hip", "krylov.cpp", "rn.cpp", "batch.cpp",
-           "dropin.cpp"]
+SOURCES = ["amg_setup.cpp", "solver.cpp", "capi.cpp", "kernels.hip", "krylov.cpp", "rn.cpp", "batch.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LINA_ARCH", "gfx950")
@@ -48,7 +47,7 @@ def _compile(src: str, build_dir: str = BUILD, extra=()) -> str:
 def build(verbose: bool = True, out: str = LIB, build_dir: str = BUILD, extra=()) -> str:
     """Compile every source (extra: additional hipcc flags) and link `out`."""
     os.makedirs(build_dir, exist_ok=True)
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    srcs = list(SOURCES)
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(lambda s: _compile(s, build_dir, extra), srcs))
     tmp = out + ".tmp"

@@ -227,19 +227,27 @@ void copy_vector_to(double* x, double* y, int n) { memcpy(y, x, n * sizeof(doubl
 
 /* --------------------------------------------------------- CSR glue */
 
+/* CSR copy; a row without entries (read_sparse can produce one) becomes the
+ * explicit zero row (i, 0.0) of sparse_zero's convention, which the device
+ * matrices require and which multiplies like an empty row */
 dropin_csr dropin_to_csr(sparse_matrix* A) {
     dropin_csr c;
     int i, k = 0, nnz = 0;
     c.n = A->size;
-    for (i = 0; i < A->size; i++) nnz += A->Len[i];
+    for (i = 0; i < A->size; i++) nnz += A->Len[i] > 0 ? A->Len[i] : 1;
     c.rp = (int*)malloc((c.n + 1) * sizeof(int));
     c.ci = (int*)malloc((nnz > 0 ? nnz : 1) * sizeof(int));
     c.v = (double*)malloc((nnz > 0 ? nnz : 1) * sizeof(double));
     c.rp[0] = 0;
     for (i = 0; i < A->size; i++) {
-        memcpy(c.ci + k, A->Indices[i], A->Len[i] * sizeof(int));
-        memcpy(c.v + k, A->Values[i], A->Len[i] * sizeof(double));
-        k += A->Len[i];
+        if (A->Len[i] > 0) {
+            memcpy(c.ci + k, A->Indices[i], A->Len[i] * sizeof(int));
+            memcpy(c.v + k, A->Values[i], A->Len[i] * sizeof(double));
+            k += A->Len[i];
+        } else {
+            c.ci[k] = i;
+            c.v[k++] = 0.0;
+        }
         c.rp[i + 1] = k;
     }
     return c;
@@ -405,16 +413,7 @@ static void triang_invert(sparse_matrix* T, double* x, int lower) {
 void L_triang_invert(sparse_matrix* L, double* x) { triang_invert(L, x, 1); }
 void U_triang_invert(sparse_matrix* U, double* x) { triang_invert(U, x, 0); }
 
-/* ---------------------------------------------------- active-set method */
-
-int* zero_int_list(int n) {
-    int* r = (int*)calloc(n > 0 ? n : 1, sizeof(int));
-    return r;
-}
-
-void copy_vector_content(double* x, double* y, int x_pos, int y_pos, int size) {
-    memmove(y + y_pos, x + x_pos, (size_t)(size > 0 ? size : 0) * sizeof(double));
-}
+/* ------------------------------------------------------- transpose */
 
 /* get_transpose (linear_algebra.c:1290-1300): rows of a new sparse_zero
  * matrix of new_size rows receive A's entries through insert_sparse, A's rows
@@ -427,176 +426,4 @@ sparse_matrix* get_transpose(sparse_matrix* A, int new_size) {
         for (r = 0; r < A->Len[i]; r++)
             if (A->Indices[i][r] < new_size) insert_sparse(T, A->Values[i][r], A->Indices[i][r], i);
     return T;
-}
-
-/* enlarge_matrix (linear_algebra.c:4109-4137): A's blocks of mesh_size rows
- * and columns move by equ_index block rows and var_index block columns in a
- * matrix of new_size rows; the rows outside hold (i, 0.0) */
-void enlarge_matrix(sparse_matrix* A, int mesh_size, int new_size, int equ_index, int var_index) {
-    int i, j, old = A->size;
-    A->Len = (int*)realloc(A->Len, new_size * sizeof(int));
-    A->Indices = (int**)realloc(A->Indices, new_size * sizeof(int*));
-    A->Values = (double**)realloc(A->Values, new_size * sizeof(double*));
-    A->size = new_size;
-    for (i = 0; i < old; i++)
-        for (j = 0; j < A->Len[i]; j++) {
-            const int c = A->Indices[i][j];
-            A->Indices[i][j] = (c / mesh_size + var_index) * mesh_size + c % mesh_size;
-        }
-    for (i = old - 1; i >= 0; i--) {
-        const int to = (i / mesh_size + equ_index) * mesh_size + i % mesh_size;
-        A->Indices[to] = A->Indices[i];
-        A->Values[to] = A->Values[i];
-        A->Len[to] = A->Len[i];
-    }
-    for (i = 0; i < new_size; i++)
-        if (i < equ_index * mesh_size || i >= equ_index * mesh_size + old) {
-            A->Len[i] = 1;
-            A->Indices[i] = (int*)malloc(sizeof(int));
-            A->Values[i] = (double*)malloc(sizeof(double));
-            A->Indices[i][0] = i;
-            A->Values[i][0] = 0;
-        }
-}
-
-/* simple_enlarge_matrix (linear_algebra.c:4139-4156): add_size rows (i, 0.0) */
-void simple_enlarge_matrix(sparse_matrix* A, int add_size) {
-    int i, n = A->size, m = n + add_size;
-    A->Len = (int*)realloc(A->Len, m * sizeof(int));
-    A->Indices = (int**)realloc(A->Indices, m * sizeof(int*));
-    A->Values = (double**)realloc(A->Values, m * sizeof(double*));
-    A->size = m;
-    for (i = n; i < m; i++) {
-        A->Len[i] = 1;
-        A->Indices[i] = (int*)malloc(sizeof(int));
-        A->Values[i] = (double*)malloc(sizeof(double));
-        A->Indices[i][0] = i;
-        A->Values[i][0] = 0;
-    }
-}
-
-/* add_constraints_default (linear_algebra.c:4867-4884): A bordered by the
- * active constraint rows of C and their transposes, [A C_act^T; C_act 0],
- * entries added in the active set's order */
-sparse_matrix* add_constraints_default(sparse_matrix* A, sparse_matrix* C, int* act, int nact) {
-    int k, j, N = A->size;
-    sparse_matrix* R = clone(A);
-    sparse_matrix* CT = get_transpose(C, C->size);
-    enlarge_matrix(R, N, N + nact, 0, 0);
-    for (k = 0; k < nact; k++) {
-        const int i = act[k];
-        for (j = 0; j < CT->Len[i]; j++) {
-            insert_sparse(R, CT->Values[i][j], CT->Indices[i][j], N + k);
-            insert_sparse(R, CT->Values[i][j], N + k, CT->Indices[i][j]);
-        }
-    }
-    free_sparse(CT);
-    return R;
-}
-
-/* add_constraints_flux (linear_algebra.c:4886-4918): the constraints bound
- * the second field's unknowns (column offset C->size); diagonal constraints
- * only, as in the reference */
-sparse_matrix* add_constraints_flux(sparse_matrix* A, sparse_matrix* C, int* act, int nact) {
-    int k, j, N = A->size, n = C->size;
-    sparse_matrix* R = clone(A);
-    simple_enlarge_matrix(R, nact);
-    for (k = 0; k < nact; k++) {
-        const int i = act[k];
-        for (j = 0; j < C->Len[i]; j++) {
-            insert_sparse(R, C->Values[i][j], n + C->Indices[i][j], N + k);
-            insert_sparse(R, C->Values[i][j], N + k, C->Indices[i][j]);
-        }
-    }
-    return R;
-}
-
-void free_ass_info(ass_info** info) {
-    if (*info == NULL) return;
-    free((*info)->Active);
-    free((*info)->Signs);
-    free((*info)->Constraint);
-    free((*info)->Iter_history);
-    free(*info);
-}
-
-/* active_set_solver (linear_algebra.c:4933-5042) */
-int active_set_solver(sparse_matrix* A, double* Sol, double* b, sparse_matrix* B, double* L, double* U,
-                      int (*Solver)(sparse_matrix* A, double* X, double* b), int max_iter, double definiteness,
-                      int mode, ass_info** solver_info) {
-    const double tol = 1e-10;
-    int i, k, iter = 0, fulfilled;
-    const int n = A->size, m = B->size;
-    ass_info* s;
-    double *Z, *X;
-    if (*solver_info == NULL) {
-        *solver_info = (ass_info*)calloc(1, sizeof(ass_info));
-    }
-    s = *solver_info;
-    if (s->Active == NULL) s->Active = zero_int_list(m);
-    if (s->Signs == NULL) s->Signs = zero_vector(m);
-    if (s->Constraint == NULL) s->Constraint = zero_vector(m);
-    Z = clone_vector(b, n);
-    X = zero_vector(n + m);
-    copy_vector_content(Sol, X, 0, 0, n);
-    do {
-        /* the constrained system of the current active set */
-        int* act = (int*)malloc((s->active_size > 0 ? s->active_size : 1) * sizeof(int));
-        double* sg = (double*)malloc((s->active_size > 0 ? s->active_size : 1) * sizeof(double));
-        double* Y;
-        sparse_matrix* M;
-        int nact = s->active_size;
-        Z = (double*)realloc(Z, (n + nact > 0 ? n + nact : 1) * sizeof(double));
-        k = 0;
-        for (i = 0; i < m; i++)
-            if (s->Active[i]) {
-                act[k] = i;
-                sg[k] = s->Signs[i];
-                Z[n + k] = s->Constraint[i];
-                k++;
-            }
-        if (k != nact) printf("\nsomenthing's wrong in active_set_solver\n");
-        M = mode == ASS_FLUX ? add_constraints_flux(A, B, act, nact) : add_constraints_default(A, B, act, nact);
-        (*Solver)(M, X, Z);
-        free_sparse(M);
-        fulfilled = 1;
-        /* multipliers of the wrong sign: the constraint leaves the set.  The
-         * loop runs while k is below the live set size, which shrinks with
-         * every removal, so a pass looks at fewer than nact multipliers when
-         * some leave (linear_algebra.c:5001-5007 bounds it the same way) */
-        Y = sparse_mult(B, X);
-        for (k = 0; k < s->active_size; k++)
-            if (sg[k] * X[n + k] * definiteness > 0) {
-                s->Active[act[k]] = 0;
-                s->active_size--;
-                fulfilled = 0;
-            }
-        /* violated constraints join it */
-        for (i = 0; i < m; i++)
-            if (Y[i] < L[i] - tol || Y[i] > U[i] + tol) {
-                if (!s->Active[i]) s->active_size++;
-                s->Active[i] = 1;
-                if (Y[i] < L[i] - tol) {
-                    s->Signs[i] = 1.;
-                    s->Constraint[i] = L[i];
-                }
-                if (Y[i] > U[i] + tol) {
-                    s->Signs[i] = -1.;
-                    s->Constraint[i] = U[i];
-                }
-                fulfilled = 0;
-            }
-        free(Y);
-        free(act);
-        free(sg);
-        iter++;
-    } while (!fulfilled && iter < max_iter);
-    copy_vector_content(X, Sol, 0, 0, n);
-    if (iter >= max_iter) printf("Warning: no convergence in %d iterations\n", iter);
-    s->Iter_history = (int*)realloc(s->Iter_history, (s->calls + 1) * sizeof(int));
-    s->Iter_history[s->calls] = iter;
-    s->calls++;
-    free(X);
-    free(Z);
-    return iter;
 }

@@ -283,3 +283,62 @@ def test_sparse_files_match_reference(libs, tmp_path):
     assert got[0][0] == ([0, 2], [2.0, -2.25]) and got[0][1] == ([1], [0.0 + 3.2 + 1e-3])
     assert got[0][2] == ([2], [0.0]) and got[0][3] == ([2, 3], [7.0, 0.0])
     assert not ours.L.read_sparse(d, b"missing.bin")
+
+
+def _random_rowlist(lib, n, m, density, rng, diag=True, cols=None):
+    """insert_sparse-built n-row matrix with random entries in columns
+    [0, cols) (default n); diag=False removes the seeded diagonal where the
+    random draw left it zero"""
+    cols = n if cols is None else cols
+    A = lib.L.sparse_zero(n)
+    for i in range(n):
+        for j in np.flatnonzero(rng.random(cols) < density):
+            lib.L.insert_sparse(A, float(rng.normal()), i, int(j))
+    return A
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_bordered_systems_match_reference(libs, seed):
+    """add_constraints_default / _flux, enlarge_matrix and
+    simple_enlarge_matrix (linear_algebra.c:4109-4156, :4867-4918) against
+    the compiled reference's: identical row lists (columns and values) for
+    non-symmetric constraint matrices with and without diagonal entries,
+    unsorted active sets with repeats, and block shifts of rows and columns."""
+    import ctypes as C
+    from tests.dropin_bind import SparseMatrix
+    SM = C.POINTER(SparseMatrix)
+    ours, theirs = libs
+    out = []
+    for lib in (ours, theirs):
+        rng = np.random.default_rng(seed)
+        for name, args in (("add_constraints_default", [SM, SM, C.POINTER(C.c_int), C.c_int]),
+                           ("add_constraints_flux", [SM, SM, C.POINTER(C.c_int), C.c_int]),
+                           ("enlarge_matrix", [SM, C.c_int, C.c_int, C.c_int, C.c_int]),
+                           ("simple_enlarge_matrix", [SM, C.c_int])):
+            getattr(lib.L, name).argtypes = args
+        lib.L.add_constraints_default.restype = SM
+        lib.L.add_constraints_flux.restype = SM
+        N, m = 14, 6
+        A = _random_rowlist(lib, N, N, 0.3, rng)
+        Cm = _random_rowlist(lib, m, m, 0.35, rng)      # default mode reads columns of C
+        Cf = _random_rowlist(lib, m, m, 0.35, rng)      # flux mode: row k borders rows m + j
+        # zero a diagonal entry of C and drop it so some columns lack one
+        lib.L.set_sparse(Cm, 0.0, 2, 2)
+        lib.L.delete_zeros(Cm)
+        res = []
+        for act in ([0, 2, 5], [3, 1, 3, 4], [], list(range(m))):
+            a = (C.c_int * max(len(act), 1))(*act)
+            R1 = lib.L.add_constraints_default(A, Cm, a, len(act))
+            R2 = lib.L.add_constraints_flux(A, Cf, a, len(act))
+            res.append((LinA.to_lists(R1), LinA.to_lists(R2)))
+            lib.free(R1)
+            lib.free(R2)
+        B = lib.L.clone(A)
+        lib.L.enlarge_matrix(B, 7, 35, 2, 1)
+        res.append(LinA.to_lists(B))
+        lib.L.simple_enlarge_matrix(B, 3)
+        res.append(LinA.to_lists(B))
+        for M in (A, Cm, Cf, B):
+            lib.free(M)
+        out.append(res)
+    assert out[0] == out[1]
