@@ -401,6 +401,12 @@ lina_batch* lina_batch_create(int nsys, const int* row_off, const int* rowptr, c
     lina_batch* h = nullptr;
     int rc = batch_guard([&]() {
         if (nsys < 1 || !row_off || !rowptr || !col || !val || !p) throw std::invalid_argument("null argument");
+        lina::check_params(p);
+        // the batch preconditions every CG iteration with one V-cycle from a
+        // zero guess; other cycle settings would be silently ignored
+        if (p->cycle != LINA_CYCLE_V || p->amg_max_iter != 1 || p->zero_init != 1)
+            throw std::invalid_argument("a batch preconditions with one V-cycle from a zero guess: cycle must be "
+                                        "LINA_CYCLE_V, amg_max_iter 1 and zero_init 1");
         const int n = row_off[nsys];
         lina::Csr A;
         A.n = A.ncols = n;

@@ -73,6 +73,9 @@ Csr make_csr(int n, const int* rowptr, const int* col, const double* val) {
     return A;
 }
 
+}  // namespace
+
+namespace lina {
 void check_params(const lina_amg_params* p) {
     if (!p) throw std::invalid_argument("null params");
     if (p->dof < 1 || p->depth < 2) throw std::invalid_argument("dof >= 1 and depth >= 2 required");
@@ -83,8 +86,7 @@ void check_params(const lina_amg_params* p) {
         throw std::invalid_argument("unknown precision");
     if (p->cycle != LINA_CYCLE_V && p->cycle != LINA_CYCLE_FULL) throw std::invalid_argument("unknown cycle");
 }
-
-}  // namespace
+}  // namespace lina
 
 extern "C" {
 
@@ -132,7 +134,7 @@ lina_solver* lina_amg_create(int n, const int* rowptr, const int* col, const dou
                              int upload) {
     lina_solver* h = nullptr;
     int rc = guard([&]() {
-        check_params(p);
+        lina::check_params(p);
         Csr A = make_csr(n, rowptr, col, val);
         h = new lina_solver();
         h->s.setup_host(A, *p);
@@ -153,7 +155,7 @@ void lina_amg_destroy(lina_solver* s) { delete s; }
 
 int lina_amg_set_params(lina_solver* s, const lina_amg_params* p) {
     return guard([&]() {
-        check_params(p);
+        lina::check_params(p);
         const lina_amg_params& o = s->s.params();
         if (p->dof != o.dof || p->depth != o.depth || p->coarsening != o.coarsening ||
             p->eps_minus != o.eps_minus || p->eps_plus != o.eps_plus || p->matrix_tol != o.matrix_tol)

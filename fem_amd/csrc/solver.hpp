@@ -14,6 +14,10 @@
 
 namespace lina {
 
+// argument checks of an lina_amg_params block (throws std::invalid_argument)
+void check_params(const lina_amg_params* p);
+
+
 struct HipError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };

@@ -262,7 +262,10 @@ int lina_elasticity_adjoint(lina_solver* s, int n_nodes, const double* xy, int n
  * kernel of the V-cycle serves all systems.  pcg per system with the
  * relative-residual stop |b_k - A_k x_k| <= tol |b_k|; graph_iters > 0
  * captures that many iterations into a hipGraph and replays it.
- * p->autotune = 1 picks each level's row partition by timing.
+ * p->autotune = 1 picks each level's row partition by timing.  Every CG
+ * iteration is preconditioned by one V-cycle from a zero guess: p->cycle
+ * must be LINA_CYCLE_V, p->amg_max_iter 1 and p->zero_init 1 (the vcycle
+ * defaults), else lina_batch_create returns NULL (LINA_EARG message).
  * ------------------------------------------------------------------ */
 typedef struct lina_batch_stats {
     int iterations_launched; /* iterations run (graph replays x graph_iters) */

@@ -251,3 +251,33 @@ def test_eigenvalue_searches_match_reference(libs):
         assert abs(lo - lmin) <= 1e-6 * lmax
     for cu in (cu1, cu2):
         assert cu < 0 and abs(cu - (lmin - c)) <= 1e-6 * lmax
+
+
+def test_power_method_with_empty_rows_matches_reference(libs):
+    """A row list with empty rows (Len[i] == 0, as read_sparse produces from
+    a file that holds such rows) is a valid operand of the reference's
+    power_method: the rows act as zero rows.  The drop-in's device copy
+    gives them the explicit (i, 0.0) entry, so the iteration count, the
+    estimate and the iterate agree with the reference's."""
+    s = P.damage_system(17, h_scale=1.0)[0]
+    empty = [0, 5, s.n // 2, s.n - 1]
+    v0 = np.random.default_rng(11).random(s.n)
+    v0 /= np.linalg.norm(v0)
+    out = []
+    for lib in libs:
+        M = lib.from_csr(s.rowptr, s.col, s.val)
+        for i in empty:
+            M.contents.Len[i] = 0     # rows emptied in place (their arrays stay allocated)
+        v = v0.copy()
+        e = np.zeros(1)
+        it = lib.L.power_method(M, dptr(e), dptr(v), 1000, 1e-8, None, 0.0)
+        for i in empty:
+            M.contents.Len[i] = 1
+        lib.free(M)
+        out.append((it, e[0], v))
+    (i1, e1, v1), (i2, e2, v2) = out
+    say("power iters (empty rows)", i1, i2, "eig", e1, e2)
+    assert i1 > 0 and abs(i1 - i2) <= 1 and abs(e1 - e2) <= 1e-9 * abs(e2)
+    if i1 == i2:
+        assert np.linalg.norm(v1 - v2) <= 1e-9
+    assert all(abs(v1[i]) <= 1e-300 for i in empty)

@@ -34,3 +34,28 @@ def test_ilu_reproduces_pattern_of_A():
     assert np.allclose(LU[mask], A[mask], rtol=1e-12, atol=1e-12 * np.abs(A).max())
     assert np.allclose(np.diag(U), 1.0)
     assert np.all(np.triu(L, 1) == 0) and np.all(np.tril(U, -1) == 0)
+
+
+@pytest.mark.parametrize("field,value", [("cycle", 2), ("amg_max_iter", 100), ("zero_init", 0), ("smoother", 9)])
+def test_batch_rejects_unsupported_cycle_settings(field, value):
+    """lina_batch_create validates its parameters before touching a device:
+    the batch preconditions with one V-cycle from a zero guess, so a full
+    multigrid cycle, several V-cycles or a warm start are refused instead of
+    silently ignored."""
+    import ctypes as C
+    sp_ = [P.poisson(9), P.poisson(11)]
+    row_off, rp, ci, v, _ = P.block_diagonal(sp_)
+    prm = lina.params("vcycle", 1, 3)
+    setattr(prm, field, value)
+    L = lina.lib()
+    L.lina_batch_create.restype = C.c_void_p
+    ro = np.ascontiguousarray(row_off, dtype=np.int32)
+    rp = np.ascontiguousarray(rp, dtype=np.int32)
+    ci = np.ascontiguousarray(ci, dtype=np.int32)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    h = L.lina_batch_create(len(sp_), ro.ctypes.data_as(C.POINTER(C.c_int)), rp.ctypes.data_as(C.POINTER(C.c_int)),
+                            ci.ctypes.data_as(C.POINTER(C.c_int)), v.ctypes.data_as(C.POINTER(C.c_double)),
+                            C.byref(prm))
+    assert not h
+    msg = L.lina_last_error().decode()
+    assert ("V-cycle" in msg) if field != "smoother" else ("smoother" in msg)
