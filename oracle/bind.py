@@ -171,9 +171,10 @@ class Ref:
     # -- solvers
     def pcg_amg(self, M, b, x0=None, dof=1, depth=4, smiter=2, aggr_smiter=3, coarsening=1, smoother="sor",
                 alpha=1.6, amg_eps=1e-2, amg_max_iter=100, zero_init=False, tol=1e-10, max_iter=1000, keep=False,
-                coarse_sweeps=2):
+                coarse_sweeps=2, cycle="v"):
         n = self.L.refh_size(M)
         self.L.refh_set_hybrid(coarse_sweeps)
+        self.L.refh_set_cycle({"v": 0, "full": 2}[cycle])
         X = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
         B = np.array(b, dtype=np.float64)
         res = C.c_double()
@@ -187,6 +188,7 @@ class Ref:
                                  amg_eps, amg_max_iter, int(zero_init), dptr(X), dptr(B), tol, max_iter, C.byref(res),
                                  iptr(calls), cap, C.byref(ncalls), C.byref(ts), C.byref(tv),
                                  C.byref(kept) if keep else None)
+        self.L.refh_set_cycle(0)
         out = {"iters": it, "x": X, "b_out": B, "res": res.value, "vcycles": calls[:min(ncalls.value, cap)].copy(),
                "t_setup": ts.value, "t_solve": tv.value}
         if keep:
@@ -282,6 +284,7 @@ class Oracle:
                                   C.c_int, C.c_int, _dp, _ip]
         L.orc_colouring.argtypes = [C.c_int, _ip, _ip, _dp, _ip]
         L.orc_set_coarse_smoother.argtypes = [_vp, C.c_int, C.c_int]
+        L.orc_set_cycle.argtypes = [_vp, C.c_int]
         L.orc_spmv.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp]
         L.orc_ilu0.argtypes = [C.c_int, _ip, _ip, _dp, _ip, _ip, _dp, _ip, _ip, _dp]
         L.orc_gmres_ilu.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, _dp]
@@ -338,10 +341,12 @@ class Oracle:
 
     def pcg_amg(self, rowptr, col, val, b, x0=None, dof=1, depth=4, smiter=2, aggr_smiter=3, coarsening=1,
                 smoother="sor", alpha=1.6, amg_eps=1e-2, amg_max_iter=100, zero_init=False, tol=1e-10,
-                max_iter=1000, stop="reference", coarse_sweeps=2):
+                max_iter=1000, stop="reference", coarse_sweeps=2, cycle="v"):
         """smoother "hybrid": MC-SSOR on the finest level, l1-Jacobi with
-        coarse_sweeps sweeps below (the harness's refh_hybrid_smoother)."""
+        coarse_sweeps sweeps below (the harness's refh_hybrid_smoother);
+        cycle "full": every preconditioner call is one full-multigrid pass."""
         h = self.amg_setup(rowptr, col, val, dof, depth, smiter, aggr_smiter, coarsening)
+        self.L.orc_set_cycle(h, {"v": 0, "full": 2}[cycle])
         if smoother == "hybrid":
             self.L.orc_set_coarse_smoother(h, SMOOTHERS["l1_jacobi"], coarse_sweeps)
             smoother = "mc_ssor"

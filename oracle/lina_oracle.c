@@ -806,6 +806,8 @@ struct orc_amg {
     int* ncolour;
     /* smoother of the levels below the finest (-1: the same as the finest) */
     int coarse_sm, coarse_sweeps;
+    /* cycle structure of the preconditioner calls (0 STAND_ALONE, 2 FULL_AMG) */
+    int cycle;
 };
 
 /* LU_factorzation (linear_algebra.c:2909-2942): unit-lower L, U by bordering,
@@ -1151,25 +1153,53 @@ static void sor_row(const Mat* A, const double* F, double* X, int i, double w) {
     X[i] = (1. - w) * X[i] + w * sum / A->v.a[d];
 }
 
-/* smoother ids: 0 SOR, 1 parallel Jacobi, 2 MC-SOR, 3 l1-Jacobi, 4 MC-SSOR, 5 SSOR */
-static void smooth(orc_amg* h, int l, const double* F, double* X, int sweeps, double w, int sm, int post) {
-    const Mat* A = &h->A[l];
+/* planted mistakes (tests only; 0 = none): 1 = the multicolour sweep reads
+ * every neighbour value from the start of the sweep (a stale gather),
+ * 2 = the symmetric multicolour sweep keeps the forward colour order when
+ * post-smoothing.  orc_plant(0) restores the restatement. */
+static int planted = 0;
+void orc_plant(int which) { planted = which; }
+
+/* smoother ids: 0 SOR, 1 parallel Jacobi, 2 MC-SOR, 3 l1-Jacobi, 4 MC-SSOR, 5 SSOR.
+ * SOR / SSOR: AMG_SOR_smoother's row update (Multigrid.c:1345-1365) over the
+ * rows in order, the symmetric form's post-smoothing call in reverse order;
+ * MC-SOR / MC-SSOR: the same row update colour by colour (rows of a colour
+ * in natural order; the symmetric form's post call takes the colours in
+ * reverse); parallel Jacobi: AMG_parallel_Jacobi_smoother (Multigrid.c:
+ * 1367-1400, undamped, rows beyond (n/dof)*dof untouched); l1-Jacobi:
+ * x += (f - A x) / sum_j |a_ij|. */
+static void smooth_csr(const Mat* A, const int* colour, int nc, int dof, const double* F, double* X, int sweeps,
+                       double w, int sm, int post) {
     int n = A->n, k, i, c, p;
     for (k = 0; k < sweeps; k++) {
         if (sm == 0 || sm == 5) {
             int back = sm == 5 && post;
             for (i = 0; i < n; i++) sor_row(A, F, X, back ? n - 1 - i : i, w);
         } else if (sm == 2 || sm == 4) {
-            int back = sm == 4 && post, nc = h->ncolour[l];
+            int back = sm == 4 && post && planted != 2;
+            double* S = planted == 1 ? (double*)malloc((n > 0 ? n : 1) * sizeof(double)) : NULL;
+            if (S) memcpy(S, X, n * sizeof(double));
             for (c = 0; c < nc; c++) {
                 int cc = back ? nc - 1 - c : c;
                 for (i = 0; i < n; i++)
-                    if (h->colour[l][i] == cc) sor_row(A, F, X, i, w);
+                    if (colour[i] == cc) {
+                        if (S) { /* planted: gathers from the sweep's start */
+                            double sum = F[i];
+                            int d = -1;
+                            for (p = A->rp[i]; p < A->rp[i + 1]; p++) {
+                                if (A->ci.a[p] != i) sum -= A->v.a[p] * S[A->ci.a[p]];
+                                else d = p;
+                            }
+                            X[i] = (1. - w) * X[i] + w * sum / A->v.a[d];
+                        } else {
+                            sor_row(A, F, X, i, w);
+                        }
+                    }
             }
+            free(S);
         } else {
-            /* AMG_parallel_Jacobi_smoother (Multigrid.c:1367-1400) or l1-Jacobi */
             double* O = (double*)malloc((n > 0 ? n : 1) * sizeof(double));
-            int lim = sm == 1 ? (n / h->dof) * h->dof : n;
+            int lim = sm == 1 ? (n / dof) * dof : n;
             memcpy(O, X, n * sizeof(double));
             for (i = 0; i < lim; i++) {
                 double sum = F[i], d = 0;
@@ -1188,6 +1218,21 @@ static void smooth(orc_amg* h, int l, const double* F, double* X, int sweeps, do
             free(O);
         }
     }
+}
+
+static void smooth(orc_amg* h, int l, const double* F, double* X, int sweeps, double w, int sm, int post) {
+    smooth_csr(&h->A[l], h->colour[l], h->ncolour[l], h->dof, F, X, sweeps, w, sm, post);
+}
+
+void orc_smooth(int n, const int* rp, const int* ci, const double* v, const int* colour, int ncolour, int dof,
+                const double* F, double* X, int sweeps, double w, int sm, int post) {
+    Mat A;
+    A.n = A.ncols = n;
+    A.rp = (int*)rp;
+    A.ci.a = (int*)ci;
+    A.v.a = (double*)v;
+    A.ci.n = A.ci.cap = A.v.n = A.v.cap = rp[n];
+    smooth_csr(&A, colour, ncolour, dof, F, X, sweeps, w, sm, post);
 }
 
 /* AMG_Direct_solver (Multigrid.c:1450-1455): Sol = F, L_triang_invert,
@@ -1345,6 +1390,10 @@ static void fmg(orc_amg* h, double** CF, double* X, int l, int sm, double w) {
  * (Multigrid.c:1595-1611) when r0 != 0: CF[depth-1] = F, CF[l-1] = R_l CF[l],
  * FMG_recursive from the finest level; returns the cycle count (1, or 0
  * when r0 == 0) */
+/* the cycle structure AMG_solve applies when it preconditions pcg (AMG_setup's
+ * cycle argument, Multigrid.c:1902-1915): 0 STAND_ALONE, 2 FULL_AMG */
+void orc_set_cycle(orc_amg* h, int cycle) { h->cycle = cycle; }
+
 int orc_amg_solve_fmg(orc_amg* h, const double* Fin, double* X, int sm, double w) {
     int d = h->depth - 1, n = h->A[d].n, i, p, l;
     const Mat* A = &h->A[d];
@@ -1378,7 +1427,8 @@ int orc_pcg_amg(orc_amg* h, double* X, double* B, double tol, int max_iter, int 
 #define PRECON(RR, ZZ)                                                       \
     do {                                                                     \
         if (zero_init) memset((ZZ), 0, n * sizeof(double));                  \
-        vc += orc_amg_solve(h, (RR), (ZZ), sm, w, amg_eps, amg_max_iter);    \
+        vc += h->cycle == 2 ? orc_amg_solve_fmg(h, (RR), (ZZ), sm, w)        \
+                            : orc_amg_solve(h, (RR), (ZZ), sm, w, amg_eps, amg_max_iter); \
     } while (0)
     hn = sqrt(dot(B, B, n));
     if (hn == 0) hn = 1;

@@ -34,12 +34,26 @@ void orc_set_coarse_smoother(orc_amg* h, int sm, int sweeps);
 /* AMG_solve (Multigrid.c:1863-1931); smoother ids as lina_hip.h */
 int orc_amg_solve(orc_amg* h, const double* F, double* X, int smoother, double omega, double eps, int max_iter);
 int orc_amg_solve_fmg(orc_amg* h, const double* F, double* X, int smoother, double omega);
+/* cycle structure of the preconditioner inside orc_pcg_amg: 0 STAND_ALONE
+ * (orc_amg_solve), 2 FULL_AMG (one orc_amg_solve_fmg pass per call) */
+void orc_set_cycle(orc_amg* h, int cycle);
 
 /* pcg (PCGS.c:163-233) preconditioned by AMG_solve; stop 0: reference test
  * r/r0 <= tol, 1: |R| <= tol.  Returns iterations; *res = pcg residuum,
  * *vcycles = V-cycles summed over all preconditioner calls. */
 int orc_pcg_amg(orc_amg* h, double* X, double* B, double tol, int max_iter, int smoother, double omega,
                 double amg_eps, int amg_max_iter, int zero_init, int stop, double* res, int* vcycles);
+
+/* one smoother call of the V-cycle on a CSR matrix: `sweeps` sweeps of
+ * smoother sm (ids as lina_hip.h; colour/ncolour: the colouring of the
+ * multicolour sweeps, NULL otherwise; dof: the parallel Jacobi's row block;
+ * post = 1: the post-smoothing call, which reverses the symmetric sweeps) */
+void orc_smooth(int n, const int* rp, const int* ci, const double* v, const int* colour, int ncolour, int dof,
+                const double* F, double* X, int sweeps, double w, int sm, int post);
+/* planted mistakes for the tests that show the checks catch them (0 = off):
+ * 1 stale gathers in the multicolour sweep, 2 forward colour order in the
+ * symmetric sweep's post-smoothing call */
+void orc_plant(int which);
 
 /* greedy colouring of the symmetrised nonzero pattern (natural order) */
 int orc_colouring(int n, const int* rp, const int* ci, const double* v, int* colour);

@@ -363,7 +363,7 @@ int refh_pcg_amg(sparse_matrix* A, int dof, int depth, int smiter, int aggr_smit
     clock_gettime(CLOCK_MONOTONIC, &t0);
     PCG_default(A, NULL, NULL);
     AMG_print_info(NO);
-    h_amg = AMG_setup(A, dof, depth, smiter, aggr_smiter, coarsening, STAND_ALONE, DIRECT, NULL, NULL);
+    h_amg = AMG_setup(A, dof, depth, smiter, aggr_smiter, coarsening, h_cycle, DIRECT, NULL, NULL);
     clock_gettime(CLOCK_MONOTONIC, &t1);
     set_smoother(smoother);
     hyb_finest = h_amg->Coarse_A[h_amg->depth - 1];

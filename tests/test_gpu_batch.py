@@ -72,3 +72,42 @@ def test_autotuned_partitions_converge(gpu, specimens):
         x = out["x"][row_off[k]:row_off[k + 1]]
         rr = np.linalg.norm(s.b - s.scipy() @ x) / np.linalg.norm(s.b)
         assert rr <= 1.01e-8
+
+
+def test_converged_specimens_stay_frozen_in_shared_row_blocks(gpu):
+    """Small specimens (several share one row block of the union's kernels)
+    interleaved with larger, slower ones, so row blocks hold converged and
+    active systems at once.  A converged specimen's iterate must not move
+    while its neighbours iterate on: the specimen's X after the full solve
+    equals, bit for bit, its X from a solve stopped at exactly its own
+    iteration count.  Each specimen also matches a single-system pcg of the
+    same parameters (iterations within 1, solutions to the tolerance)."""
+    small = P.batch_specimens(9, seed=P.SEED_BATCH + 7001, side_min=9, side_max=12)
+    large = P.batch_specimens(3, seed=P.SEED_BATCH + 9001, side_min=36, side_max=44)
+    sp = []
+    for k in range(9):
+        sp.append(small[k])
+        if k % 3 == 2:
+            sp.append(large[k // 3])
+    row_off, rp, ci, v, b = P.block_diagonal(sp)
+    prm = gpu.params("vcycle", 2, 4, precision="fp64")
+    bt = gpu.Batch(row_off, rp, ci, v, prm)
+    full = bt.pcg(b, tol=1e-10, max_iter=3000)
+    say("iterations", full["iters"].tolist(), "sizes", [s.n for s in sp])
+    assert full["active"] == 0
+    assert full["iters"].max() > 2 * full["iters"].min()
+    early = sorted(set(int(i) for i in full["iters"]))[:3]
+    for k_stop in early:
+        part = bt.pcg(b, tol=1e-10, max_iter=k_stop)
+        for k in np.flatnonzero(full["iters"] == k_stop):
+            lo, hi = row_off[k], row_off[k + 1]
+            say("  specimen", int(k), "stopped at", k_stop, "frozen:", np.array_equal(part["x"][lo:hi], full["x"][lo:hi]))
+            assert part["iters"][k] == k_stop
+            assert np.array_equal(part["x"][lo:hi], full["x"][lo:hi])
+    for k, s in enumerate(sp):
+        single = gpu.AMG(s.rowptr, s.col, s.val, prm).pcg(s.b, tol=1e-10, max_iter=3000, stop="relres")
+        x = full["x"][row_off[k]:row_off[k + 1]]
+        d = np.linalg.norm(x - single["x"]) / np.linalg.norm(single["x"])
+        say("  specimen", k, s.n, "iters", int(full["iters"][k]), "single", single["iters"], "rel", d)
+        assert abs(int(full["iters"][k]) - single["iters"]) <= 1
+        assert d <= 1e-8

@@ -270,3 +270,52 @@ def test_recompute_values_only_upload(gpu, kind, prec):
     assert np.linalg.norm(s2.b - A @ ra["x"]) <= 1e-8 * np.linalg.norm(s2.b)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("case", ["poisson17", "poisson33", "poisson33_std", "elast17", "elast33"])
+def test_amg_solve_reference_jacobi_matches_golden(gpu, case):
+    """The reference's undamped AMG_parallel_Jacobi_smoother
+    (Multigrid.c:1367-1400) as the V-cycle smoother of AMG_solve on the
+    device, against the compiled reference's own result (golden vectors:
+    AMG_solve from a zero guess, eps 1e-6, at most 6 V-cycles): equal
+    V-cycle counts, iterates to 1e-12."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", f"{case}.npz"))
+    dof, depth, coarsening = int(g["dof"]), int(g["depth"]), int(g["coarsening"])
+    say("reference jacobi", case, "dof", dof, "depth", depth, "coarsening", coarsening)
+    prm = gpu.params("reference", dof, depth, coarsening=coarsening, smoother="jacobi", omega=1.0, amg_eps=1e-6,
+                     amg_max_iter=6)
+    amg = gpu.AMG(g["rowptr"], g["col"], g["val"], prm)
+    k, x = amg.amg_solve(g["b"], np.zeros(len(g["b"])))
+    amg.close()
+    say("  vcycles golden", int(g["amgsolve_jacobi_k"]), "gpu", k, "rel", rel(x, g["amgsolve_jacobi_x"]))
+    assert k == int(g["amgsolve_jacobi_k"])
+    assert rel(x, g["amgsolve_jacobi_x"]) < 1e-12
+
+
+@pytest.mark.parametrize("case,sm,alpha", [("poisson33", "sor", 1.2), ("poisson33", "mc_ssor", 1.0),
+                                            ("elast25", "ssor", 1.0), ("elast25", "mc_ssor", 1.0)])
+def test_pcg_full_multigrid_preconditioner(gpu, case, sm, alpha):
+    """pcg preconditioned by AMG_solve of a FULL_AMG hierarchy (one
+    full-multigrid pass per preconditioner call, Multigrid.c:1902-1915)
+    against the oracle's FULL_AMG pcg (pinned bit for bit to the compiled
+    reference on the same cases): FP64 iteration counts within 1 and
+    iterates to 1e-12; the mixed-precision pass reaches the same tolerance
+    with at most a quarter more iterations."""
+    s, depth = {"poisson33": (P.poisson(33), 4), "elast25": (P.degraded_elasticity(25), 4)}[case]
+    say("pcg full multigrid", case, sm)
+    r = Oracle().pcg_amg(s.rowptr, s.col, s.val, s.b, dof=s.dof, depth=depth, smoother=sm, alpha=alpha, tol=1e-10,
+                         max_iter=300, cycle="full")
+    A = s.scipy()
+    for prec in ("fp64", "mixed"):
+        prm = gpu.params("reference", s.dof, depth, smoother=sm, omega=alpha, cycle="full", precision=prec)
+        amg = gpu.AMG(s.rowptr, s.col, s.val, prm)
+        g = amg.pcg(s.b, tol=1e-10, max_iter=300)
+        amg.close()
+        say("  ", prec, "iters oracle", r["iters"], "gpu", g["iters"], "rel", rel(g["x"], r["x"]))
+        if prec == "fp64":
+            assert abs(g["iters"] - r["iters"]) <= 1
+            assert rel(g["x"], r["x"]) < 1e-12
+        else:
+            assert g["iters"] <= int(1.25 * r["iters"]) + 2
+            assert np.linalg.norm(s.b - A @ g["x"]) <= 1e-8 * np.linalg.norm(s.b)

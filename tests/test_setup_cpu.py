@@ -14,7 +14,8 @@ from fem_amd import problems as P
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 # the hierarchy fixtures (krylov.npz holds the Krylov drivers' vectors)
-GOLDEN = sorted(p for p in glob.glob(os.path.join(ROOT, "tests", "golden", "*.npz")) if not p.endswith("krylov.npz"))
+GOLDEN = sorted(p for p in glob.glob(os.path.join(ROOT, "tests", "golden", "*.npz")) if not p.endswith("krylov.npz")
+          and not os.path.basename(p).startswith("direct_"))
 
 
 @pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
