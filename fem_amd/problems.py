@@ -431,3 +431,30 @@ def block_diagonal(systems: list):
         val[nnz_off[k]:nnz_off[k + 1]] = s.val
     b = np.concatenate([s.b for s in systems])
     return row_off, rowptr, col, val, b
+
+
+# ---------------------------------------------------------------- sketches
+
+SEED_SKETCH = 0x5EED00A0
+
+
+def sketch_indices(n: int, count: int, seed: int = SEED_SKETCH) -> np.ndarray:
+    """`count` distinct indices of [0, n), sorted: the first `count` distinct
+    values of floor(u_k * n) along the SplitMix64 stream."""
+    draws = np.minimum((uniform(seed, 4 * count) * n).astype(np.int64), n - 1)
+    _, first = np.unique(draws, return_index=True)
+    if len(first) < count:
+        raise ValueError("sketch_indices: too few distinct draws")
+    return np.sort(draws[np.sort(first)[:count]])
+
+
+def gaussian_sketch(x: np.ndarray, rows: int = 64, seed: int = SEED_SKETCH) -> np.ndarray:
+    """S x for the seeded Gaussian S of `rows` x len(x) entries N(0, 1/rows),
+    S's row r being normal(seed + 1 + r, len(x)) / sqrt(rows): E|S e|^2 =
+    |e|^2, so |S x - S y| estimates |x - y| (relative spread ~ sqrt(2/rows)).
+    x may hold several vectors as columns (n, k): S x is then (rows, k)."""
+    x = np.asarray(x, dtype=np.float64)
+    out = np.empty((rows,) + x.shape[1:])
+    for r in range(rows):
+        out[r] = normal(seed + 1 + r, x.shape[0]) @ x
+    return out / math.sqrt(rows)
