@@ -22,9 +22,10 @@ LIB = os.path.join(HERE, "liblina_hip.so")
 HOST = os.path.join(HERE, "host")
 DROPIN_LIB = os.path.join(HERE, "liblinalg_hip.so")
 DROPIN_SOURCES = ["linalg.c", "active_set.c", "pcgs.c", "multigrid.c", "bicgstab.c", "reflective_newton.c", "sparse_io.c",
-                  "eigen.c"]
+                  "eigen.c", "gmres.c", "direct.c", "sparse_ops.c", "sweeps.c"]
 
-SOURCES = ["amg_setup.cpp", "solver.cpp", "capi.cpp", "kernels.hip", "krylov.cpp", "rn.cpp", "batch.cpp"]
+SOURCES = ["amg_setup.cpp", "solver.cpp", "capi.cpp", "kernels.hip", "krylov.cpp", "rn.cpp", "batch.cpp",
+           "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LINA_ARCH", "gfx950")
@@ -68,7 +69,7 @@ def build_dropin(verbose: bool = True) -> str:
     cc = os.environ.get("CC", shutil.which("gcc") or "cc")
     srcs = [os.path.join(HOST, s) for s in DROPIN_SOURCES]
     tmp = DROPIN_LIB + ".tmp"
-    cmd = [cc, "-O2", "-fPIC", "-shared", "-std=c99", "-D_POSIX_C_SOURCE=200809L", "-Wall", "-Wno-unused-function",
+    cmd = [cc, "-O2", "-fPIC", "-shared", "-std=c99", "-fopenmp", "-D_POSIX_C_SOURCE=200809L", "-Wall", "-Wno-unused-function",
            "-ffp-contract=off", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "include", "linalg"),
            *srcs, "-o", tmp, "-L", HERE, "-llina_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-Bsymbolic", "-lm"]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -263,6 +263,24 @@ int lina_amg_solve(lina_solver* s, const double* f, double* x) {
     });
 }
 
+int lina_amg_solve_stat(lina_solver* s, const double* f, double* x, double* relres) {
+    return guard([&]() {
+        if (!s || !relres) throw std::invalid_argument("null argument");
+        if (!s->s.uploaded()) throw std::invalid_argument("solver was created without device upload");
+        return s->s.amg_solve_host(f, x, relres);
+    });
+}
+
+int lina_amg_signs(const lina_solver* s, double* signs) {
+    return guard([&]() {
+        if (!s || !signs) throw std::invalid_argument("null argument");
+        const auto& H = s->s.host();
+        const int n = H.lv.back().A.n;
+        for (int i = 0; i < n; i++) signs[i] = H.signs.empty() ? 1.0 : H.signs[i];
+        return n;
+    });
+}
+
 int lina_amg_solve_device(lina_solver* s, const double* f, double* x) {
     return guard([&]() {
         if (!s->s.uploaded()) throw std::invalid_argument("solver was created without device upload");
@@ -352,17 +370,25 @@ int lina_amg_kernel_timing(lina_solver* s, int on) {
 
 // ---- device vector primitives on the library's primitive stream
 namespace {
+// the primitives' stream and buffers: block partials, scalars, multidot
+// partials (MULTI_MAX dots x its 512-block grid), coefficients; pinned host
+// staging for results and coefficients
+constexpr int MULTI_MAX = 64;
 struct Prim {
     hipStream_t st = nullptr;
     double* part = nullptr;
     double* scal = nullptr;
+    double* mpart = nullptr;
+    double* coef = nullptr;
     double* host = nullptr;
     void init() {
         if (st) return;
         lina::hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
         lina::hip_check(hipMalloc((void**)&part, 4096 * sizeof(double)), "hipMalloc");
         lina::hip_check(hipMalloc((void**)&scal, 8 * sizeof(double)), "hipMalloc");
-        lina::hip_check(hipHostMalloc((void**)&host, 8 * sizeof(double)), "hipHostMalloc");
+        lina::hip_check(hipMalloc((void**)&mpart, (size_t)MULTI_MAX * 512 * sizeof(double)), "hipMalloc");
+        lina::hip_check(hipMalloc((void**)&coef, MULTI_MAX * sizeof(double)), "hipMalloc");
+        lina::hip_check(hipHostMalloc((void**)&host, (8 + MULTI_MAX) * sizeof(double)), "hipHostMalloc");
     }
 };
 Prim g_prim;
@@ -382,6 +408,36 @@ double lina_dev_dot(const double* a, const double* b, long n) {
         return 0;
     });
     return rc == 0 ? out : NAN;
+}
+
+int lina_dev_multidot(long n, int m, const double* V, long ld, const double* r, double* out) {
+    return guard([&]() {
+        if (n < 0 || n > INT32_MAX || m < 0 || m > MULTI_MAX || ld < n || (m > 0 && (!V || !r || !out)))
+            throw std::invalid_argument("lina_dev_multidot: bad arguments (at most 64 dots, ld >= n)");
+        if (m == 0) return 0;
+        g_prim.init();
+        lina::dev::multidot(g_prim.st, (int)n, m, V, (size_t)ld, r, g_prim.mpart, g_prim.coef);
+        lina::hip_check(hipMemcpyAsync(g_prim.host, g_prim.coef, m * sizeof(double), hipMemcpyDeviceToHost,
+                                       g_prim.st), "d2h");
+        lina::hip_check(hipStreamSynchronize(g_prim.st), "sync");
+        std::memcpy(out, g_prim.host, m * sizeof(double));
+        return 0;
+    });
+}
+
+int lina_dev_combine(long n, int m, const double* V, long ld, const double* c, const double* base, double sign,
+                     double* y) {
+    return guard([&]() {
+        if (n < 0 || n > INT32_MAX || m < 0 || m > MULTI_MAX || ld < n || !base || !y || (m > 0 && (!V || !c)))
+            throw std::invalid_argument("lina_dev_combine: bad arguments (at most 64 vectors, ld >= n)");
+        g_prim.init();
+        std::memcpy(g_prim.host, c, m * sizeof(double));
+        lina::hip_check(hipMemcpyAsync(g_prim.coef, g_prim.host, m * sizeof(double), hipMemcpyHostToDevice,
+                                       g_prim.st), "h2d");
+        lina::dev::combine(g_prim.st, (int)n, m, V, (size_t)ld, g_prim.coef, base, sign, y);
+        lina::hip_check(hipStreamSynchronize(g_prim.st), "sync");
+        return 0;
+    });
 }
 
 int lina_dev_axpby(long n, double a, double* x, double b, const double* y) {

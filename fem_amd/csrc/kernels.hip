@@ -1696,7 +1696,7 @@ template int spmv<float, float>(hipStream_t, const DMat<float>&, const float*, c
 
 template <typename XT>
 int dot_part(hipStream_t st, int n, const XT* a, const XT* b, double* part) {
-    int g = grid_for(n, 1024);
+    int g = grid_for(n, DOT_PART_MAX);
     hipLaunchKernelGGL((k_dot_part<XT>), dim3(g), dim3(TPB), 0, st, n, a, b, part);
     return g;
 }

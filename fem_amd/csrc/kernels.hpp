@@ -111,6 +111,8 @@ inline int lanes_per_row(double mean_len) {
 template <typename VT, typename XT>
 int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w,
          const Epi<VT, XT>* epi = nullptr);
+// block partials of a.b: at most DOT_PART_MAX of them (the grid's cap)
+constexpr int DOT_PART_MAX = 1024;
 template <typename XT>
 int dot_part(hipStream_t st, int n, const XT* a, const XT* b, double* part);
 // scal[slot] = sum(part); op 0 plain, 1 sqrt, 2 shift (scal[slot+1] = scal[slot]) then plain

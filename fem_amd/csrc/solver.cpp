@@ -1295,7 +1295,7 @@ double Solver::residual_norm(const T* f, const T* x, T* r, int slot) {
 // influence the result except through `res0 != 0`, which holds exactly when
 // the first residual r0 is nonzero; they are not computed.  LINA_CYCLE_FULL
 // (the FULL_AMG case): one full-multigrid pass, counted as one cycle.
-int Solver::amg_solve_dev(const double* F, double* X) {
+int Solver::amg_solve_dev(const double* F, double* X, double* relres) {
     hipStream_t st = stream_;
     DevLevel& top = L_.back();
     const int n = top.n;
@@ -1306,9 +1306,17 @@ int Solver::amg_solve_dev(const double* F, double* X) {
     }
     dev::fix_isolated<double>(st, niso_, iso_rows_, iso_a_, Ft, X);
     int it = 0;
+    // relres: |r|/|r0| after the last V-cycle (AMG_add_stat's record,
+    // Multigrid.c:1918); 0 when r0 = 0, 1 after a full-multigrid pass (the
+    // reference leaves r = r0 there).  The loop skips the residual after the
+    // last permitted cycle; it is formed only when asked for.
+    double rel = 0.0;
     if (prm_.precision == LINA_PREC_FP64) {
         double r0 = residual_norm<double>(Ft, X, rtop_, dev::S_T0);
-        if (r0 == 0.0) return 0;
+        if (r0 == 0.0) {
+            if (relres) *relres = 0.0;
+            return 0;
+        }
         double r = r0;
         if (prm_.cycle == LINA_CYCLE_FULL) {
             fmg<double>(Ft, X);
@@ -1317,8 +1325,9 @@ int Solver::amg_solve_dev(const double* F, double* X) {
         else while (std::fabs(r / r0) > prm_.amg_eps && it < prm_.amg_max_iter) {
             vcycle<double>((int)L_.size() - 1, Ft, X);
             it++;
-            if (it < prm_.amg_max_iter) r = residual_norm<double>(Ft, X, rtop_, dev::S_T0);
+            if (it < prm_.amg_max_iter || relres) r = residual_norm<double>(Ft, X, rtop_, dev::S_T0);
         }
+        rel = std::fabs(r / r0);
     } else {
         float* ff = top.ff;
         float* xf = top.xf;
@@ -1334,11 +1343,13 @@ int Solver::amg_solve_dev(const double* F, double* X) {
             else while (std::fabs(r / r0) > prm_.amg_eps && it < prm_.amg_max_iter) {
                 vcycle<float>((int)L_.size() - 1, ff, xf);
                 it++;
-                if (it < prm_.amg_max_iter) r = residual_norm<float>(ff, xf, top.rf, dev::S_T0);
+                if (it < prm_.amg_max_iter || relres) r = residual_norm<float>(ff, xf, top.rf, dev::S_T0);
             }
             dev::convert<float, double>(st, n, xf, X);
+            rel = std::fabs(r / r0);
         }
     }
+    if (relres) *relres = rel;
     vcycles_ += it;
     precon_calls_++;
     return it;
@@ -1582,13 +1593,13 @@ int Solver::pcg_host(double* x, double* b, double tol, int max_iter, int stop, l
     return it;
 }
 
-int Solver::amg_solve_host(const double* f, double* x) {
+int Solver::amg_solve_host(const double* f, double* x, double* relres) {
     const size_t bytes = (size_t)n_ * sizeof(double);
     hip_check(hipMemcpyAsync(kB_, f, bytes, hipMemcpyHostToDevice, stream_), "h2d");
     hip_check(hipMemcpyAsync(kX_, x, bytes, hipMemcpyHostToDevice, stream_), "h2d");
     vcycles_ = 0;
     precon_calls_ = 0;
-    int it = amg_solve_dev(kB_, kX_);
+    int it = amg_solve_dev(kB_, kX_, relres);
     hip_check(hipMemcpyAsync(x, kX_, bytes, hipMemcpyDeviceToHost, stream_), "d2h");
     hip_check(hipStreamSynchronize(stream_), "sync");
     return it;

@@ -143,8 +143,8 @@ class Solver {
 
     int pcg_dev(double* X, double* B, double tol, int max_iter, int stop, lina_stats* st);
     int pcg_host(double* x, double* b, double tol, int max_iter, int stop, lina_stats* st);
-    int amg_solve_dev(const double* F, double* X);
-    int amg_solve_host(const double* f, double* x);
+    int amg_solve_dev(const double* F, double* X, double* relres = nullptr);
+    int amg_solve_host(const double* f, double* x, double* relres = nullptr);
     // Z = M^-1 R; w != nullptr: block partials of w.Z in part(), their number in *np
     int precon(const double* R, double* Z, const double* w = nullptr, int* np = nullptr, bool prepared = false);
     int vcycle_once(const double* F, double* X, const double* w = nullptr, int* np = nullptr, bool prepared = false);

@@ -51,6 +51,7 @@ void dropin_check(int rc, const char* what);
 /* the AMG settings of Multigrid.c's globals for the next solve */
 void dropin_amg_sync(void* amg_info);
 lina_matrix* dropin_pcg_default_matrix(void);
+lina_ilu* dropin_pcg_ilu(char which); /* 'B' both sweeps, 'L' forward, 'U' backward */
 lina_matrix* dropin_rn_default_matrix(void);
 
 #endif

@@ -16,18 +16,59 @@ static int g_info = YES;
 static int g_smoother = LINA_SMOOTHER_SOR;
 static int g_dof = 1;
 
+/* The two smoothers of the reference's V-cycle.  Passed to AMG_set_smoother
+ * they select the GPU V-cycle's smoother (by address); called directly they
+ * sweep the caller's host row list as the reference does: forward SOR over
+ * the rows whose label differs from `fix` (Multigrid.c:1345-1365), or the
+ * undamped Jacobi sweep over the first (n/dof)*dof rows in node blocks
+ * (Multigrid.c:1367-1400; alpha is not used there). */
 void AMG_SOR_smoother(sparse_matrix* A, double* F, double* Sol, int* Label, double alpha, int fix, int iter,
                       int deg_freedom) {
-    (void)A; (void)F; (void)Sol; (void)Label; (void)alpha; (void)fix; (void)iter; (void)deg_freedom;
-    fprintf(stderr, "liblinalg_hip: AMG_SOR_smoother runs inside AMG_solve on the GPU; select it with "
-                    "AMG_set_smoother\n");
+    int k, i, j;
+    (void)deg_freedom;
+    for (k = 0; k < iter; k++)
+        for (i = 0; i < A->size; i++) {
+            double s = F[i], d = 0;
+            int has = 0;
+            if (Label[i] == fix) continue;
+            for (j = 0; j < A->Len[i]; j++) {
+                const int c = A->Indices[i][j];
+                if (c != i) s -= A->Values[i][j] * Sol[c];
+                else {
+                    d = A->Values[i][j];
+                    has = 1;
+                }
+            }
+            if (!has) {
+                fprintf(stderr, "AMG_SOR_smoother: row %d has no diagonal entry\n", i);
+                return;
+            }
+            Sol[i] = (1. - alpha) * Sol[i] + alpha * s / d;
+        }
 }
 
 void AMG_parallel_Jacobi_smoother(sparse_matrix* A, double* F, double* Sol, int* Label, double alpha, int fix,
                                   int iter, int deg_freedom) {
-    (void)A; (void)F; (void)Sol; (void)Label; (void)alpha; (void)fix; (void)iter; (void)deg_freedom;
-    fprintf(stderr, "liblinalg_hip: AMG_parallel_Jacobi_smoother runs inside AMG_solve on the GPU; select it "
-                    "with AMG_set_smoother\n");
+    const int n = A->size, N = n / deg_freedom;
+    double* old = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    int k, i, d, j;
+    (void)alpha;
+    for (k = 0; k < iter; k++) {
+        memcpy(old, Sol, (size_t)n * sizeof(double));
+        for (i = 0; i < N; i++)
+            for (d = 0; d < deg_freedom; d++) {
+                const int r = i + N * d;
+                double s = F[r], a = 0;
+                if (Label[r] == fix) continue;
+                for (j = 0; j < A->Len[r]; j++) {
+                    const int c = A->Indices[r][j];
+                    if (c != r) s -= A->Values[r][j] * old[c];
+                    else a = A->Values[r][j];
+                }
+                Sol[r] = s / a;
+            }
+    }
+    free(old);
 }
 
 void AMG_set_smoother(void (*Smoother)(sparse_matrix*, double*, double*, int*, double, int, int, int)) {
@@ -54,6 +95,118 @@ int* AMG_default_iterations(int depth) {
     int* it = (int*)malloc((depth > 0 ? depth : 1) * sizeof(int));
     for (i = 0; i < depth; i++) it[i] = 1;
     return it;
+}
+
+/* ------------------------------------------- the level row lists */
+
+static void drop(sparse_matrix** M) {
+    if (*M) {
+        Free_sparse(*M);
+        free(*M);
+        *M = NULL;
+    }
+}
+
+static sparse_matrix* rowlist(int n, const int* rp, const int* ci, const double* v) {
+    sparse_matrix* M = (sparse_matrix*)malloc(sizeof(sparse_matrix));
+    int i;
+    M->size = n;
+    M->Len = (int*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+    M->Indices = (int**)malloc((size_t)(n > 0 ? n : 1) * sizeof(int*));
+    M->Values = (double**)malloc((size_t)(n > 0 ? n : 1) * sizeof(double*));
+    for (i = 0; i < n; i++) {
+        const int m = rp[i + 1] - rp[i];
+        M->Len[i] = m;
+        M->Indices[i] = (int*)malloc((size_t)(m > 0 ? m : 1) * sizeof(int));
+        M->Values[i] = (double*)malloc((size_t)(m > 0 ? m : 1) * sizeof(double));
+        memcpy(M->Indices[i], ci + rp[i], (size_t)m * sizeof(int));
+        memcpy(M->Values[i], v + rp[i], (size_t)m * sizeof(double));
+    }
+    return M;
+}
+
+/* level l's A, P or R of the device hierarchy's host copy */
+static sparse_matrix* level_matrix(lina_solver* s, int l, char which, int rows) {
+    const int nnz = lina_amg_level_nnz(s, l, which);
+    int* rp = (int*)malloc((size_t)(rows + 1) * sizeof(int));
+    int* ci = (int*)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int));
+    double* v = (double*)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(double));
+    sparse_matrix* M = NULL;
+    if (nnz >= 0 && lina_amg_level_copy(s, l, which, rp, ci, v) == 0) M = rowlist(rows, rp, ci, v);
+    else dropin_check(-1, "AMG level copy");
+    free(rp);
+    free(ci);
+    free(v);
+    return M;
+}
+
+/* Coarse_A[l] of every level, then the factors AMG_setup keeps beside them
+ * (Multigrid.c:1676-1680, 1741-1743): ILU(0) of every level above the
+ * coarsest, the dense LU of the coarsest */
+static void materialise_matrices(amg_system_info* I) {
+    lina_solver* s = (lina_solver*)I->device;
+    int l;
+    for (l = 0; l < I->depth; l++) {
+        drop(&I->Coarse_A[l]);
+        drop(&I->L[l]);
+        drop(&I->U[l]);
+        I->Coarse_A[l] = level_matrix(s, l, 'A', I->Fine_sizes[l]);
+        I->L[l] = sparse_zero(I->Fine_sizes[l]);
+        I->U[l] = sparse_zero(I->Fine_sizes[l]);
+        if (!I->Coarse_A[l]) continue;
+        if (l > 0) incomplete_LU_factorization(I->Coarse_A[l], I->L[l], I->U[l]);
+        else sparse_LU_factorization(I->Coarse_A[0], I->L[0], I->U[0]);
+    }
+}
+
+/* the hierarchy as the reference's amg_system_info holds it: level
+ * matrices, transfer operators, C/F labels and the finest row signs
+ * (levels indexed as there: depth-1 finest, 0 coarsest; level 0 has no
+ * P, R or labels) */
+static void materialise(amg_system_info* I) {
+    lina_solver* s = (lina_solver*)I->device;
+    const int d = I->depth, n = I->Fine_sizes[d - 1];
+    double* sg = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    int l, i;
+    I->Coarse_A = (sparse_matrix**)calloc((size_t)d, sizeof(sparse_matrix*));
+    I->Prolongation = (sparse_matrix**)calloc((size_t)d, sizeof(sparse_matrix*));
+    I->Restriction = (sparse_matrix**)calloc((size_t)d, sizeof(sparse_matrix*));
+    I->L = (sparse_matrix**)calloc((size_t)d, sizeof(sparse_matrix*));
+    I->U = (sparse_matrix**)calloc((size_t)d, sizeof(sparse_matrix*));
+    I->Coarsening = (int**)calloc((size_t)d, sizeof(int*));
+    for (l = 1; l < d; l++) {
+        I->Prolongation[l] = level_matrix(s, l, 'P', I->Fine_sizes[l]);
+        I->Restriction[l] = level_matrix(s, l, 'R', I->Coarse_sizes[l]);
+        I->Coarsening[l] = (int*)malloc((size_t)(I->Fine_sizes[l] > 0 ? I->Fine_sizes[l] : 1) * sizeof(int));
+        dropin_check(lina_amg_labels(s, l, I->Coarsening[l]), "AMG labels");
+    }
+    materialise_matrices(I);
+    I->Signs = sparse_zero(n);
+    lina_amg_signs(s, sg);
+    for (i = 0; i < n; i++) I->Signs->Values[i][0] = sg[i];
+    free(sg);
+}
+
+static void release_levels(amg_system_info* I) {
+    int l;
+    for (l = 0; l < I->depth; l++) {
+        if (I->Coarse_A) drop(&I->Coarse_A[l]);
+        if (I->Prolongation) drop(&I->Prolongation[l]);
+        if (I->Restriction) drop(&I->Restriction[l]);
+        if (I->L) drop(&I->L[l]);
+        if (I->U) drop(&I->U[l]);
+        if (I->Coarsening) free(I->Coarsening[l]);
+    }
+    free(I->Coarse_A);
+    free(I->Prolongation);
+    free(I->Restriction);
+    free(I->L);
+    free(I->U);
+    free(I->Coarsening);
+    drop(&I->Signs);
+    I->Coarse_A = I->Prolongation = I->Restriction = I->L = I->U = NULL;
+    I->Coarsening = NULL;
+    I->Signs = NULL;
 }
 
 /* params of a hierarchy, refreshed from the globals before each solve */
@@ -126,15 +279,22 @@ amg_system_info* AMG_setup(sparse_matrix* A, int dof, int depth, int smoothing_i
     I->AMG_deg_of_freedom = dof;
     I->Get_iter = AMG_default_iterations;
     I->main_size = A->size;
+    materialise(I);
     return I;
 }
 
 int AMG_solve(amg_system_info* Data, double* F, double* Fine_Sol, amg_statistics* Stat) {
     int it;
     dropin_amg_sync(Data);
-    it = lina_amg_solve((lina_solver*)Data->device, F, Fine_Sol);
-    dropin_check(it, "AMG_solve");
-    if (Stat) AMG_add_stat(Stat, it, 0.0);
+    if (Stat) {
+        double rel = 0;
+        it = lina_amg_solve_stat((lina_solver*)Data->device, F, Fine_Sol, &rel);
+        dropin_check(it, "AMG_solve");
+        if (it >= 0) AMG_add_stat(Stat, it, rel);
+    } else {
+        it = lina_amg_solve((lina_solver*)Data->device, F, Fine_Sol);
+        dropin_check(it, "AMG_solve");
+    }
     return it;
 }
 
@@ -143,10 +303,12 @@ void AMG_recompute_coarse_matrices(amg_system_info* Info, sparse_matrix* New_mat
     dropin_check(lina_amg_recompute_csr((lina_solver*)Info->device, c.n, c.rp, c.ci, c.v),
                  "AMG_recompute_coarse_matrices");
     dropin_free_csr(&c);
+    if (Info->Coarse_A) materialise_matrices(Info);
 }
 
 void AMG_free_data(amg_system_info* Data) {
     if (!Data) return;
+    release_levels(Data);
     lina_amg_destroy((lina_solver*)Data->device);
     free(Data->settings);
     free(Data->Fine_sizes);

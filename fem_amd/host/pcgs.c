@@ -15,6 +15,7 @@
 static sparse_matrix *default_A = NULL, *default_L = NULL, *default_U = NULL;
 static lina_matrix* dev_A = NULL;
 static lina_ilu* dev_LU = NULL;
+static lina_ilu *dev_Lonly = NULL, *dev_Uonly = NULL; /* one sweep each: the other factor the identity */
 static int (*precon)(int n, double* R, double* Z) = NULL;
 static void (*mat_multi)(int n, double* x, double* y) = NULL;
 static amg_system_info* PCG_AMG_data = NULL;
@@ -29,7 +30,9 @@ void PCG_default(sparse_matrix* A, sparse_matrix* L, sparse_matrix* U) {
     if (dev_A) lina_matrix_destroy(dev_A);
     dev_A = A ? dropin_device_matrix(A) : NULL;
     if (dev_LU) lina_ilu_destroy(dev_LU);
-    dev_LU = NULL;
+    if (dev_Lonly) lina_ilu_destroy(dev_Lonly);
+    if (dev_Uonly) lina_ilu_destroy(dev_Uonly);
+    dev_LU = dev_Lonly = dev_Uonly = NULL;
     if (L && U) {
         dropin_csr l = dropin_to_csr(L), u = dropin_to_csr(U);
         dev_LU = lina_ilu_create_factors(l.n, l.rp, l.ci, l.v, u.rp, u.ci, u.v);
@@ -63,6 +66,55 @@ int PCG_preconlr(int n, double* z, double* w) {
     }
     dropin_check(lina_ilu_apply(dev_LU, z, w), "PCG_preconlr");
     return 1;
+}
+
+/* the sweeps of PCG_default's factors on the device: 'B' both (PCG_preconlr),
+ * 'L' the forward one, 'U' the backward one (created on first use, with the
+ * other factor the identity: a unit row costs one exact division by 1) */
+lina_ilu* dropin_pcg_ilu(char which) {
+    lina_ilu** h;
+    dropin_csr f, id;
+    int i, n;
+    if (which == 'B') return dev_LU;
+    h = which == 'L' ? &dev_Lonly : &dev_Uonly;
+    if (*h || !default_L || !default_U) return *h;
+    f = dropin_to_csr(which == 'L' ? default_L : default_U);
+    n = f.n;
+    id.n = n;
+    id.rp = (int*)malloc((size_t)(n + 1) * sizeof(int));
+    id.ci = (int*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+    id.v = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    for (i = 0; i <= n; i++) id.rp[i] = i;
+    for (i = 0; i < n; i++) {
+        id.ci[i] = i;
+        id.v[i] = 1.0;
+    }
+    *h = which == 'L' ? lina_ilu_create_factors(n, f.rp, f.ci, f.v, id.rp, id.ci, id.v)
+                      : lina_ilu_create_factors(n, id.rp, id.ci, id.v, f.rp, f.ci, f.v);
+    if (!*h) dropin_check(-1, which == 'L' ? "PCG_preconl" : "PCG_preconr");
+    dropin_free_csr(&f);
+    dropin_free_csr(&id);
+    return *h;
+}
+
+void PCG_preconl(int n, double* z, double* w) {
+    lina_ilu* h = dropin_pcg_ilu('L');
+    (void)n;
+    if (!h) {
+        fprintf(stderr, "liblinalg_hip: PCG_preconl without PCG_default factors\n");
+        return;
+    }
+    dropin_check(lina_ilu_apply(h, z, w), "PCG_preconl");
+}
+
+void PCG_preconr(int n, double* z, double* w) {
+    lina_ilu* h = dropin_pcg_ilu('U');
+    (void)n;
+    if (!h) {
+        fprintf(stderr, "liblinalg_hip: PCG_preconr without PCG_default factors\n");
+        return;
+    }
+    dropin_check(lina_ilu_apply(h, z, w), "PCG_preconr");
 }
 
 int PCG_AMG_precon(int n, double* Y, double* X) {

@@ -138,6 +138,13 @@ int lina_pcg_device(lina_solver* s, double* d_x, double* d_b, double tol, int ma
 int lina_amg_solve(lina_solver* s, const double* f, double* x);
 /* the same on device vectors */
 int lina_amg_solve_device(lina_solver* s, const double* f, double* x);
+/* lina_amg_solve that also reports |r|/|r0| after the last V-cycle, the
+ * value AMG_solve records in its statistics (Multigrid.c:1918): 0 when r0
+ * is 0, 1 after a full-multigrid pass */
+int lina_amg_solve_stat(lina_solver* s, const double* f, double* x, double* relres);
+/* the row signs make_positive_diag applied to the finest matrix
+ * (linear_algebra.c:2612-2620): +1 or -1 per row; returns n */
+int lina_amg_signs(const lina_solver* s, double* signs);
 
 /* BiCGStab_solve (BiCGStab.c:115-198) preconditioned by AMG_solve with this
  * hierarchy (warm-started PP/PS as in the reference); stops at
@@ -185,6 +192,12 @@ int lina_dev_fill(double* d, long n, double value);
  * (the LinA drop-in layer): each returns after completion.
  * scalar (linear_algebra.c:256-263) on device vectors, deterministic tree order */
 double lina_dev_dot(const double* a, const double* b, long n);
+/* classical Gram-Schmidt pieces of GMRES (gmres.c:401-414): out[l] =
+ * V_l . r for the m <= 64 vectors V_l = V + l*ld (out: host array); and
+ * y = base + sign * sum_l c[l] V_l (c: host array; y may alias base) */
+int lina_dev_multidot(long n, int m, const double* V, long ld, const double* r, double* out);
+int lina_dev_combine(long n, int m, const double* V, long ld, const double* c, const double* base, double sign,
+                     double* y);
 /* x = a*x + b*y  (vector_add with a = 1, linear_algebra.c:311-316) */
 int lina_dev_axpby(long n, double a, double* x, double b, const double* y);
 /* x = a*x  (scalar_mult, linear_algebra.c:299-304) */

@@ -24,9 +24,12 @@ extern "C" {
 #define FULL_AMG 2
 
 /* amg_system_info (Multigrid.h:45-70): the reference's fields first, in its
- * order, then the device handle.  Coarse_A, Prolongation, Restriction, L, U,
- * Signs and Coarsening are not materialised as row lists (NULL); depth,
- * sizes and settings are filled. */
+ * order, then the device handle.  AMG_setup fills every field as the
+ * reference does -- Coarse_A, Prolongation, Restriction, Coarsening and the
+ * diagonal Signs as row lists copied from the hierarchy the GPU holds, L/U
+ * the ILU(0) factors of each level above the coarsest and the dense LU of
+ * the coarsest -- and AMG_recompute_coarse_matrices refreshes Coarse_A, L
+ * and U.  The solves read the device copy only. */
 typedef struct AMG_SYSTEM_INFO {
     sparse_matrix** Coarse_A;
     sparse_matrix** Prolongation;

@@ -33,6 +33,11 @@ void PCG_set_AMG_preconditioner(sparse_matrix* A, int num_fields, int depth);
 void PCG_mult(int n, double* y, double* z);
 int PCG_preconlr(int n, double* z, double* w);
 int PCG_AMG_precon(int n, double* Y, double* X);
+/* the two sweeps of PCG_preconlr on their own (PCGS.c:61-100): w = L^-1 z
+ * with PCG_default's L (diagonal last in each row), w = U^-1 z with its U
+ * (diagonal first) */
+void PCG_preconl(int n, double* z, double* w);
+void PCG_preconr(int n, double* z, double* w);
 
 #ifdef __cplusplus
 }

@@ -88,6 +88,22 @@ int active_set_solver(sparse_matrix* A, double* Sol, double* b, sparse_matrix* B
                       int mode, ass_info** solver_info);
 void free_ass_info(ass_info** info);                             /* :4920-4928 */
 
+/* products, sweeps and direct factorisations on caller-owned host row lists */
+sparse_matrix* sparse_product(sparse_matrix* A, sparse_matrix* B);        /* :186-222: A B, zeros deleted */
+void SOR(sparse_matrix* A, double* F, double* Sol, double alpha, int iter); /* :2464-2512 (iter <= 0: to 1e-10) */
+int gauss_seidel(int (*Inv_NB_matvec)(int n, double* x, double* b), void (*NB_matvec)(int n, double* y, double* z),
+                 double* Solution, double* b, int size, int iterations, double tol); /* :2514-2544 */
+double residuum(void (*Matvec)(int n, double* y, double* z), double* Sol, double* b, int n); /* :1999-2006 */
+void sparse_LU_factorization(sparse_matrix* A, sparse_matrix* L, sparse_matrix* U);   /* :3040-3062 */
+void complete_LU_factorization(sparse_matrix* A, sparse_matrix* L, sparse_matrix* U); /* :3122-3139 */
+void convert_array_to_sparse(double** M_A, sparse_matrix* A, int col_size, int row_size); /* :3082-3090 */
+void diagonal_preconditioner(sparse_matrix* A, sparse_matrix* L, sparse_matrix* U); /* :1984-1997 */
+long double hp_scalar(double* x1, double* x2, int n);                     /* :265-272 */
+long double hp_euklid_norm(double* x1, int n);                            /* :279-282 */
+#define AUTOMATIC -1                                                      /* linear_algebra.h:14 */
+void set_thread_num(int n);                                               /* :23-32: host threads of the setup */
+void set_omp_chunk_size(int chunk);                                       /* :34-36 */
+
 /* eigenvalue searches (power iterations on the GPU) */
 double matrix_norm(sparse_matrix* A);                                     /* linear_algebra.c:3441-3450 */
 void set_random_normal_vector(double* X, int n);                          /* :4824-4829 */

@@ -342,3 +342,141 @@ def test_bordered_systems_match_reference(libs, seed):
             lib.free(M)
         out.append(res)
     assert out[0] == out[1]
+
+
+def _sig(lib, name, args, res=None):
+    f = getattr(lib.L, name)
+    f.argtypes = args
+    f.restype = res
+    return f
+
+
+def test_host_solver_utilities_match_reference(libs):
+    """sparse_product, sparse_LU_factorization, complete_LU_factorization,
+    convert_array_to_sparse, diagonal_preconditioner, SOR (both loop forms),
+    gauss_seidel with a row-list sweep callback, residuum and hp_euklid_norm
+    (linear_algebra.c:186-222, 265-282, 1984-2006, 2464-2544, 3040-3139)
+    against the compiled reference: identical row lists and bit-identical
+    vectors; the LU factors also multiply back to the matrix."""
+    import ctypes as C
+    from tests.dropin_bind import SparseMatrix
+    SM = C.POINTER(SparseMatrix)
+    DP = C.POINTER(C.c_double)
+    s = P.damage_system(9)[0]
+    e = P.degraded_elasticity(5)
+    n = s.n
+    rng = np.random.default_rng(2)
+    F = rng.uniform(-1, 1, n)
+    X0 = rng.uniform(-1, 1, n)
+    out = []
+    for lib in libs:
+        r = {}
+        A = lib.from_csr(s.rowptr, s.col, s.val)
+        _sig(lib, "sparse_product", [SM, SM], SM)
+        Pm = lib.L.sparse_product(A, A)
+        r["AA"] = LinA.to_lists(Pm)
+        lib.free(Pm)
+        E = lib.from_csr(e.rowptr, e.col, e.val)
+        Pm = lib.L.sparse_product(E, E)
+        r["EE"] = LinA.to_lists(Pm)
+        lib.free(Pm)
+        for name in ("sparse_LU_factorization", "complete_LU_factorization", "diagonal_preconditioner"):
+            _sig(lib, name, [SM, SM, SM])
+            L_, U_ = lib.L.sparse_zero(n), lib.L.sparse_zero(n)
+            getattr(lib.L, name)(A, L_, U_)
+            r[name] = (LinA.to_lists(L_), LinA.to_lists(U_))
+            lib.free(L_)
+            lib.free(U_)
+        # the coarse-level form: rows with a single nonzero are left out of the dense system
+        nE = e.n
+        L_, U_ = lib.L.sparse_zero(nE), lib.L.sparse_zero(nE)
+        lib.L.sparse_LU_factorization(E, L_, U_)
+        r["LU_E"] = (LinA.to_lists(L_), LinA.to_lists(U_))
+        lib.free(L_)
+        lib.free(U_)
+        rows = (DP * 3)(*[(C.c_double * 4)(*v) for v in ([1, 0, 2, 0], [0, 0, 0, 0], [3, -1, 0, 5])])
+        T = lib.L.sparse_zero(3)
+        _sig(lib, "convert_array_to_sparse", [C.POINTER(DP), SM, C.c_int, C.c_int])
+        lib.L.convert_array_to_sparse(rows, T, 3, 4)
+        r["conv"] = LinA.to_lists(T)
+        lib.free(T)
+        _sig(lib, "SOR", [SM, DP, DP, C.c_double, C.c_int])
+        for it in (3, -4, 0):
+            x = X0.copy()
+            lib.L.SOR(A, dptr(F), dptr(x), 1.3, it)
+            r[f"sor{it}"] = x
+        _sig(lib, "hp_euklid_norm", [DP, C.c_int], C.c_longdouble)
+        r["hp"] = lib.L.hp_euklid_norm(dptr(X0), n)
+        # gauss_seidel over a caller sweep (one SOR pass of A) and matvec (A x)
+        SWEEP = C.CFUNCTYPE(C.c_int, C.c_int, DP, DP)
+        MV = C.CFUNCTYPE(None, C.c_int, DP, DP)
+        lib.L.SOR.argtypes = [SM, DP, DP, C.c_double, C.c_int]
+
+        def sweep(nn, x, bb, lib=lib, A=A):
+            lib.L.SOR(A, bb, x, 1.0, 1)
+            return 1
+        lib.L.sparse_mult.restype = DP
+
+        def mv(nn, x, y, lib=lib, A=A):
+            p = lib.L.sparse_mult(A, x)
+            for i in range(nn):
+                y[i] = p[i]
+            C.CDLL(None).free(C.cast(p, C.c_void_p))
+        cs, cm = SWEEP(sweep), MV(mv)
+        _sig(lib, "gauss_seidel", [C.c_void_p, C.c_void_p, DP, DP, C.c_int, C.c_int, C.c_double], C.c_int)
+        _sig(lib, "residuum", [C.c_void_p, DP, DP, C.c_int], C.c_double)
+        x = X0.copy()
+        k = lib.L.gauss_seidel(C.cast(cs, C.c_void_p), C.cast(cm, C.c_void_p), dptr(x), dptr(F), n, -50, 1e-6)
+        r["gs"] = (k, x)
+        r["res"] = lib.L.residuum(C.cast(cm, C.c_void_p), dptr(X0), dptr(F), n)
+        for M in (A, E):
+            lib.free(M)
+        out.append(r)
+    ours, theirs = out
+    for key in theirs:
+        a, b = ours[key], theirs[key]
+        if isinstance(b, np.ndarray):
+            assert np.array_equal(a, b), key
+        elif key == "gs":
+            assert a[0] == b[0] and np.array_equal(a[1], b[1]), key
+        else:
+            assert a == b, key
+    # the factors multiply back (the bordering LU of the whole matrix)
+    Ld, Ud = (np.zeros((n, n)) for _ in range(2))
+    for D, rows in zip((Ld, Ud), ours["complete_LU_factorization"]):
+        for i, (cols, vals) in enumerate(rows):
+            D[i, cols] = vals
+    A0 = s.scipy().toarray()
+    assert np.allclose(Ld @ Ud, A0, rtol=0, atol=1e-12 * np.abs(A0).max())
+    assert np.allclose(np.diag(Ld), 1.0)
+
+
+def test_amg_smoothers_called_directly_match_reference(libs):
+    """AMG_SOR_smoother and AMG_parallel_Jacobi_smoother called on a host row
+    list (Multigrid.c:1345-1400): with labels skipping the rows marked
+    `fix`, several sweeps, dof 1 and 2 -- bit-identical to the reference."""
+    import ctypes as C
+    from tests.dropin_bind import SparseMatrix
+    SM = C.POINTER(SparseMatrix)
+    DP, IP = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    s = P.degraded_elasticity(7)
+    n = s.n
+    rng = np.random.default_rng(5)
+    F, X0 = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    lab = (np.arange(n) % 5 == 0).astype(np.int32) + 1     # FINE (2) every fifth row, COARSE (1) else
+    out = []
+    for lib in libs:
+        A = lib.from_csr(s.rowptr, s.col, s.val)
+        r = []
+        for name in ("AMG_SOR_smoother", "AMG_parallel_Jacobi_smoother"):
+            f = getattr(lib.L, name)
+            f.argtypes = [SM, DP, DP, IP, C.c_double, C.c_int, C.c_int, C.c_int]
+            f.restype = None
+            for fix, dof in ((2, 1), (0, 2), (1, 2)):
+                x = X0.copy()
+                f(A, dptr(F), dptr(x), lab.ctypes.data_as(IP), 1.3, fix, 3, dof)
+                r.append(x)
+        lib.free(A)
+        out.append(r)
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)

@@ -281,3 +281,181 @@ def test_power_method_with_empty_rows_matches_reference(libs):
     if i1 == i2:
         assert np.linalg.norm(v1 - v2) <= 1e-9
     assert all(abs(v1[i]) <= 1e-300 for i in empty)
+
+
+class ItlinOpt(C.Structure):
+    """struct ITLIN_OPT (LinA/itlin.h:67-78; include/linalg/itlin.h)"""
+    _fields_ = [("tol", C.c_double), ("rho", C.c_double), ("i_max", C.c_int), ("maxiter", C.c_int),
+                ("termcheck", C.c_int), ("convcheck", C.c_int), ("rescale", C.c_int), ("errorlevel", C.c_int),
+                ("monitorlevel", C.c_int), ("datalevel", C.c_int)] + \
+               [(f, C.c_void_p) for f in ("errorfile", "monitorfile", "datafile", "iterfile", "resfile", "miscfile")] + \
+               [("scale", C.c_void_p)]
+
+
+class ItlinInfo(C.Structure):
+    _fields_ = [("precision", C.c_double), ("normdx", C.c_double), ("residuum", C.c_double)] + \
+               [(f, C.c_int) for f in ("iter", "rcode", "subcode", "nomatvec", "noprecon", "noprecl", "noprecr")]
+
+
+_MATVEC = C.CFUNCTYPE(None, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def _nonsymmetric(m=17):
+    """the damage system plus a skew part on its own pattern (GMRES's case)"""
+    s = P.damage_system(m)[0]
+    A = s.scipy().tocsr()
+    r = np.repeat(np.arange(s.n), np.diff(A.indptr))
+    skew = np.where(A.indices > r, 0.3, np.where(A.indices < r, -0.3, 0.0)) * np.abs(A.data)
+    A.data = A.data + skew
+    return A, s.b
+
+
+@pytest.mark.parametrize("termcheck", [0, 1])
+@pytest.mark.parametrize("callbacks", ["library", "host"])
+def test_gmres_matches_reference(libs, termcheck, callbacks):
+    """gmres (gmres.c:308-501) as FEM.c drives it (FEM.c:531-541, 737-744):
+    ILU(0) factors of PCG_default, PCG_mult as the operator, PCG_preconlr as
+    the right preconditioner, restart 30, then x = M^-1 y.  Either the
+    library's own callbacks (which the drop-in runs on the device) or host
+    callbacks (scipy products and triangular solves, staged through host
+    copies); CheckOnRestart and CheckEachIter.  Iteration counts equal, the
+    solutions to 1e-9, rcode 0 in both libraries, and the solution solves the
+    system."""
+    from scipy.sparse.linalg import spsolve_triangular
+    A, b = _nonsymmetric()
+    n = A.shape[0]
+    rp, ci = A.indptr.astype(np.int32), A.indices.astype(np.int32)
+    out = []
+    for lib in libs:
+        M = lib.from_csr(rp, ci, A.data)
+        L = lib.L.sparse_zero(n)
+        U = lib.L.sparse_zero(n)
+        assert lib.L.incomplete_LU_factorization(M, L, U) == 1
+        lib.L.PCG_default.argtypes = [C.c_void_p] * 3
+        lib.L.PCG_default(M, L, U)
+        keep = []
+        if callbacks == "library":
+            mv, pr = C.cast(lib.L.PCG_mult, C.c_void_p), C.cast(lib.L.PCG_preconlr, C.c_void_p)
+        else:
+            Ld = LinA.to_lists(L)
+            Ud = LinA.to_lists(U)
+            import scipy.sparse as sps
+
+            def tocsr(rows):
+                r = np.repeat(np.arange(n), [len(c) for c, _ in rows])
+                return sps.csr_matrix((np.concatenate([v for _, v in rows]), (r, np.concatenate([c for c, _ in rows]))),
+                                      shape=(n, n))
+            Ls, Us = tocsr(Ld), tocsr(Ud)
+
+            def host_mv(nn, x, y):
+                xv = np.ctypeslib.as_array(x, shape=(nn,))
+                np.ctypeslib.as_array(y, shape=(nn,))[:] = A @ xv
+
+            def host_pr(nn, z, w):
+                zv = np.ctypeslib.as_array(z, shape=(nn,))
+                t = spsolve_triangular(Ls, zv, lower=True)
+                np.ctypeslib.as_array(w, shape=(nn,))[:] = spsolve_triangular(Us, t, lower=False)
+            keep = [_MATVEC(host_mv), _MATVEC(host_pr)]
+            mv, pr = C.cast(keep[0], C.c_void_p), C.cast(keep[1], C.c_void_p)
+        opt = ItlinOpt(tol=1e-10, i_max=30, maxiter=600, termcheck=termcheck)
+        info = ItlinInfo()
+        y = np.zeros(n)
+        bb = b.copy()
+        lib.L.gmres.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.POINTER(C.c_double), C.POINTER(ItlinOpt), C.POINTER(ItlinInfo)]
+        lib.L.gmres.restype = None
+        lib.L.gmres(n, dptr(y), mv, pr, None, dptr(bb), C.byref(opt), C.byref(info))
+        x = np.zeros(n)
+        lib.L.PCG_preconlr(n, dptr(y), dptr(x))
+        out.append((info.iter, info.rcode, info.precision, info.nomatvec, x))
+        for T in (M, L, U):
+            lib.free(T)
+    (i1, rc1, e1, nm1, x1), (i2, rc2, e2, nm2, x2) = out
+    say("gmres", callbacks, "termcheck", termcheck, "iters", i1, i2, "eta", e1, e2, "rel", rel(x1, x2))
+    assert rc1 == rc2 == 0 and i1 == i2 and nm1 == nm2
+    assert rel(x1, x2) < 1e-9
+    assert np.linalg.norm(b - A @ x1) <= 1e-8 * np.linalg.norm(b)
+
+
+def test_gmres_argument_checks_match_reference(libs):
+    """gmres's option checks (utils.c:330-420): n <= 0 -> rcode 20, tol <= 0
+    -> 21; i_max beyond n reset to min(10, n); tol clamped into
+    [1e-15, 1e-1]."""
+    A, b = _nonsymmetric(5)
+    n = A.shape[0]
+    out = []
+    for lib in libs:
+        M = lib.from_csr(A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data)
+        lib.L.PCG_default.argtypes = [C.c_void_p] * 3
+        lib.L.PCG_default(M, None, None)
+        mv = C.cast(lib.L.PCG_mult, C.c_void_p)
+        res = []
+        for nn, tol, imax in ((0, 1e-8, 5), (n, 0.0, 5), (n, 0.5, 10 * n), (n, 1e-30, 3)):
+            opt = ItlinOpt(tol=tol, i_max=imax, maxiter=50, termcheck=0)
+            info = ItlinInfo()
+            y = np.zeros(max(n, 1))
+            bb = b.copy()
+            lib.L.gmres.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.POINTER(C.c_double), C.POINTER(ItlinOpt), C.POINTER(ItlinInfo)]
+            lib.L.gmres(nn, dptr(y), mv, None, None, dptr(bb), C.byref(opt), C.byref(info))
+            res.append((info.rcode, opt.i_max if info.rcode == 0 else None, opt.tol if info.rcode == 0 else None))
+        lib.free(M)
+        out.append(res)
+    say("gmres checks", out)
+    assert out[0] == out[1]
+
+
+def _amg_info_type():
+    from tests.dropin_bind import SparseMatrix
+    SMP = C.POINTER(C.POINTER(SparseMatrix))
+
+    class AmgInfo(C.Structure):
+        """the leading members of amg_system_info (Multigrid.h:45-60), common
+        to both libraries"""
+        _fields_ = [("Coarse_A", SMP), ("Prolongation", SMP), ("Restriction", SMP), ("L", SMP), ("U", SMP),
+                    ("Signs", C.POINTER(SparseMatrix)), ("Coarsening", C.POINTER(C.POINTER(C.c_int))),
+                    ("Fine_sizes", C.POINTER(C.c_int)), ("Coarse_sizes", C.POINTER(C.c_int)), ("depth", C.c_int)]
+    return AmgInfo
+
+
+@pytest.mark.parametrize("kind", ["poisson", "elasticity"])
+def test_amg_setup_fills_the_level_fields(libs, kind):
+    """AMG_setup's amg_system_info holds the hierarchy as the reference's
+    does (Multigrid.c:1613-1754): every level's Coarse_A, Prolongation,
+    Restriction and Coarsening labels, the finest row signs, the ILU(0)
+    factors L/U of every level above the coarsest and the coarsest dense LU
+    -- all identical to the reference's row lists, value for value; after
+    AMG_recompute_coarse_matrices with new finest values the coarse matrices
+    and factors are the reference's again."""
+    s = P.poisson(33) if kind == "poisson" else P.degraded_elasticity(17)
+    Info = _amg_info_type()
+    out = []
+    for lib in libs:
+        M = lib.from_csr(s.rowptr, s.col, s.val)
+        lib.L.AMG_print_info(0)
+        I = C.cast(lib.L.AMG_setup(M, s.dof, 4, 2, 3, 1, 0, 1, None, None), C.POINTER(Info)).contents
+        d = I.depth
+        r = {"depth": d, "fine": [I.Fine_sizes[l] for l in range(d)], "coarse": [I.Coarse_sizes[l] for l in range(d)]}
+        for l in range(d):
+            r[f"A{l}"] = LinA.to_lists(I.Coarse_A[l])
+            r[f"L{l}"] = LinA.to_lists(I.L[l])
+            r[f"U{l}"] = LinA.to_lists(I.U[l])
+            if l > 0:
+                r[f"P{l}"] = LinA.to_lists(I.Prolongation[l])
+                r[f"R{l}"] = LinA.to_lists(I.Restriction[l])
+                r[f"C{l}"] = [I.Coarsening[l][i] for i in range(I.Fine_sizes[l])]
+        r["signs"] = LinA.to_lists(I.Signs)
+        M2 = lib.from_csr(s.rowptr, s.col, s.val * (1.0 + 0.25 * np.cos(np.arange(len(s.val)))))
+        lib.L.AMG_recompute_coarse_matrices.argtypes = [C.c_void_p, C.c_void_p]
+        lib.L.AMG_recompute_coarse_matrices(C.addressof(I), M2)
+        for l in range(d - 1):
+            r[f"A{l}_new"] = LinA.to_lists(I.Coarse_A[l])
+            r[f"L{l}_new"] = LinA.to_lists(I.L[l])
+            r[f"U{l}_new"] = LinA.to_lists(I.U[l])
+        lib.L.AMG_free_data(C.addressof(I))
+        lib.free(M)
+        lib.free(M2)
+        out.append(r)
+    say(kind, "levels", out[0]["fine"], out[1]["fine"])
+    for key in out[1]:
+        assert out[0][key] == out[1][key], key

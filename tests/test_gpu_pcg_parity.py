@@ -272,50 +272,43 @@ def test_recompute_values_only_upload(gpu, kind, prec):
     b.close()
 
 
-@pytest.mark.parametrize("case", ["poisson17", "poisson33", "poisson33_std", "elast17", "elast33"])
+@pytest.mark.parametrize("case", ["poisson17", "poisson33", "poisson33_std", "pair17"])
 def test_amg_solve_reference_jacobi_matches_golden(gpu, case):
     """The reference's undamped AMG_parallel_Jacobi_smoother
     (Multigrid.c:1367-1400) as the V-cycle smoother of AMG_solve on the
-    device, against the compiled reference's own result (golden vectors:
-    AMG_solve from a zero guess, eps 1e-6, at most 6 V-cycles): equal
-    V-cycle counts, iterates to 1e-12."""
+    device: equal V-cycle counts, iterates to 1e-12.  Scalar systems against
+    the compiled reference's own result (golden vectors: AMG_solve from a
+    zero guess, eps 1e-6, at most 6 V-cycles); a two-field system (two
+    Poisson fields in the blocked [u; v] order, dof 2, where the smoother
+    works on node blocks) against the oracle, whose Jacobi sweep equals the
+    reference's bit for bit.  (On the elasticity systems the undamped
+    Jacobi V-cycle diverges -- the golden iterates grow to 1e22 in six
+    cycles -- so they carry no comparison.)"""
     import os
-    g = np.load(os.path.join(os.path.dirname(__file__), "golden", f"{case}.npz"))
-    dof, depth, coarsening = int(g["dof"]), int(g["depth"]), int(g["coarsening"])
+    import scipy.sparse as sps
+    if case == "pair17":
+        s1 = P.poisson(17)
+        A = sps.block_diag([s1.scipy(), s1.scipy()]).tocsr()
+        A.sort_indices()
+        rp, ci, v = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+        b = np.concatenate([s1.b, 0.5 * s1.b[::-1]])
+        dof, depth, coarsening = 2, 4, 1
+        orc = Oracle()
+        I = orc.amg_setup(rp, ci, v, dof=dof, depth=depth, coarsening=coarsening)
+        k_ref, x_ref = orc.amg_solve(I, b, np.zeros(len(b)), smoother="jacobi", alpha=1.0, eps=1e-6, max_iter=6)
+        orc.amg_free(I)
+    else:
+        g = np.load(os.path.join(os.path.dirname(__file__), "golden", f"{case}.npz"))
+        rp, ci, v, b = g["rowptr"], g["col"], g["val"], g["b"]
+        dof, depth, coarsening = int(g["dof"]), int(g["depth"]), int(g["coarsening"])
+        k_ref, x_ref = int(g["amgsolve_jacobi_k"]), g["amgsolve_jacobi_x"]
     say("reference jacobi", case, "dof", dof, "depth", depth, "coarsening", coarsening)
     prm = gpu.params("reference", dof, depth, coarsening=coarsening, smoother="jacobi", omega=1.0, amg_eps=1e-6,
                      amg_max_iter=6)
-    amg = gpu.AMG(g["rowptr"], g["col"], g["val"], prm)
-    k, x = amg.amg_solve(g["b"], np.zeros(len(g["b"])))
+    amg = gpu.AMG(rp, ci, v, prm)
+    k, x = amg.amg_solve(b, np.zeros(len(b)))
     amg.close()
-    say("  vcycles golden", int(g["amgsolve_jacobi_k"]), "gpu", k, "rel", rel(x, g["amgsolve_jacobi_x"]))
-    assert k == int(g["amgsolve_jacobi_k"])
-    assert rel(x, g["amgsolve_jacobi_x"]) < 1e-12
-
-
-@pytest.mark.parametrize("case,sm,alpha", [("poisson33", "sor", 1.2), ("poisson33", "mc_ssor", 1.0),
-                                            ("elast25", "ssor", 1.0), ("elast25", "mc_ssor", 1.0)])
-def test_pcg_full_multigrid_preconditioner(gpu, case, sm, alpha):
-    """pcg preconditioned by AMG_solve of a FULL_AMG hierarchy (one
-    full-multigrid pass per preconditioner call, Multigrid.c:1902-1915)
-    against the oracle's FULL_AMG pcg (pinned bit for bit to the compiled
-    reference on the same cases): FP64 iteration counts within 1 and
-    iterates to 1e-12; the mixed-precision pass reaches the same tolerance
-    with at most a quarter more iterations."""
-    s, depth = {"poisson33": (P.poisson(33), 4), "elast25": (P.degraded_elasticity(25), 4)}[case]
-    say("pcg full multigrid", case, sm)
-    r = Oracle().pcg_amg(s.rowptr, s.col, s.val, s.b, dof=s.dof, depth=depth, smoother=sm, alpha=alpha, tol=1e-10,
-                         max_iter=300, cycle="full")
-    A = s.scipy()
-    for prec in ("fp64", "mixed"):
-        prm = gpu.params("reference", s.dof, depth, smoother=sm, omega=alpha, cycle="full", precision=prec)
-        amg = gpu.AMG(s.rowptr, s.col, s.val, prm)
-        g = amg.pcg(s.b, tol=1e-10, max_iter=300)
-        amg.close()
-        say("  ", prec, "iters oracle", r["iters"], "gpu", g["iters"], "rel", rel(g["x"], r["x"]))
-        if prec == "fp64":
-            assert abs(g["iters"] - r["iters"]) <= 1
-            assert rel(g["x"], r["x"]) < 1e-12
-        else:
-            assert g["iters"] <= int(1.25 * r["iters"]) + 2
-            assert np.linalg.norm(s.b - A @ g["x"]) <= 1e-8 * np.linalg.norm(s.b)
+    say("  vcycles reference", k_ref, "gpu", k, "rel", rel(x, x_ref), "|x|", float(np.linalg.norm(x_ref)))
+    assert np.linalg.norm(x_ref) < 10 * np.linalg.norm(b)      # a convergent case
+    assert k == k_ref
+    assert rel(x, x_ref) < 1e-12
