@@ -37,32 +37,46 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sample_m: int, depth: int, tol: float) -> dict:
-    """The reference LinA (oracle/_ref) on a bounded member of the same
-    workload family, in a fresh single-threaded process."""
-    code = f"""
-import json, os, sys, time
-sys.path.insert(0, {ROOT!r})
-import numpy as np
-from fem_amd import problems as P
-from oracle.bind import Ref
-ref = Ref()
-ref.L.refh_init(2)
-s = P.degraded_elasticity({sample_m})
-M = ref.matrix(s.rowptr, s.col, s.val)
-out = ref.pcg_amg(M, s.b, dof=2, depth={depth}, smiter=1, aggr_smiter=1, smoother="ssor", alpha=1.0,
-                  amg_eps=1e-2, amg_max_iter=1, zero_init=True, tol={tol}, max_iter=5000)
-A = s.scipy()
-rr = float(np.linalg.norm(s.b - A @ out["x"]) / np.linalg.norm(s.b))
-print(json.dumps({{"n": s.n, "nnz": s.nnz, "iters": out["iters"], "t_setup": out["t_setup"],
-                  "t_solve": out["t_solve"], "relres": rr}}))
-"""
-    env = dict(os.environ, OMP_NUM_THREADS="1")
-    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=900)
-    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
-    if r.returncode != 0 or not lines:
-        raise RuntimeError(f"cpu baseline failed: {r.stderr[-2000:]}")
-    return json.loads(lines[-1])
+def cpu_baseline(s, amg, gpu_iters: int) -> dict:
+    """The reference LinA (oracle/_ref, -O3 for x86-64-v3 where built) timed
+    on the full 2.1M-DOF system on this machine's host cores: its pcg with one
+    symmetric lexicographic-SOR V(1,1) per iteration (the reference's SOR
+    arithmetic, AMG_set_smoother) on the same hierarchy (handed over from the
+    library's host setup, bit-identical to AMG_setup; the reference's own
+    AMG_setup takes ~14 min here), the coarsest level solved with the dense
+    inverse of its dense rows.  Timed: pcg with 2 and with 5 iterations from
+    x = 0; the difference / 3 is the cost of one iteration, at 1 thread and at
+    the host's core share.  value = that cost x the GPU solve's iteration
+    count."""
+    from oracle.bind import REF_FAST_SO, REF_SO, Ref
+    path = REF_FAST_SO if os.path.exists(REF_FAST_SO) else REF_SO
+    ref = Ref(path)
+    ref.L.refh_init(2)
+    t0 = time.time()
+    lv = [amg.level(l) for l in range(amg.depth)]
+    h, keep = ref.assemble(lv, 2, amg.signs())
+    M = ref.matrix(s.rowptr, s.col, s.val)
+    t_asm = time.time() - t0
+    log(f"cpu baseline: reference hierarchy assembled in {t_asm:.1f}s ({os.path.basename(path)})")
+    per = {}
+    cores = min(16, os.cpu_count() or 1)
+    for th in sorted({1, cores}):
+        ref.L.refh_set_threads(th)
+        r2 = ref.pcg_assembled(h, M, s.b, max_iter=2)
+        r5 = ref.pcg_assembled(h, M, s.b, max_iter=5)
+        per[th] = (r5["t_solve"] - r2["t_solve"]) / 3
+        log(f"cpu baseline: {th} thread(s): pcg 2 its {r2['t_solve']:.2f}s, 5 its {r5['t_solve']:.2f}s -> "
+            f"{per[th]:.3f}s per iteration")
+    ref.free(M)
+    best = min(per, key=per.get)
+    return {"value": per[best] * gpu_iters, "unit": "s per solve (measured per-iteration cost x GPU iterations)",
+            "cores": best, "kind": "reference",
+            "sample": (f"reference pcg + AMG_solve (PCGS.c, Multigrid.c; {os.path.basename(path)}) on the full "
+                       f"{s.n}-DOF system and hierarchy, one symmetric lexicographic-SOR V(1,1) per iteration: "
+                       f"{per[1]:.3f}s per iteration at 1 thread, {per[cores]:.3f}s at {cores}; x {gpu_iters} "
+                       f"iterations (the GPU FP64 solve's count); AMG_setup excluded (the reference's takes ~14 min "
+                       f"on one core of the build container)"),
+            "per_iteration_s": {str(k): v for k, v in per.items()}, "assemble_s": t_asm}
 
 
 def copy_gbs(lina, n=1 << 27, reps=5):
@@ -387,7 +401,6 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--tol", type=float, default=1e-10, help="relative residual target of the timed solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU baseline")
-    ap.add_argument("--cpu-sample", type=int, default=129, help="side of the reference CPU sample")
     ap.add_argument("--coarse-sweeps", type=int, default=1, help="l1-Jacobi sweeps on the levels below the finest")
     ap.add_argument("--kernels-only", action="store_true",
                     help="no solves: a 1 GiB device copy and a few finest-level SpMVs, for counter runs")
@@ -498,16 +511,8 @@ def main():
     if not args.no_cpu:
         t0 = time.time()
         try:
-            c = cpu_baseline(args.cpu_sample, 6, args.tol)
-            scale = s.n / c["n"] * res["fp64"]["iters"] / max(c["iters"], 1)
-            cpu = {"value": c["t_solve"] * scale, "unit": "s per solve (scaled)", "cores": 1, "kind": "reference",
-                   "sample": (f"reference LinA (oracle/_ref, gcc -O2, 1 thread) AMG_setup + pcg with one symmetric "
-                              f"lexicographic-SOR V(1,1) per iteration (AMG_set_smoother), {args.cpu_sample}x"
-                              f"{args.cpu_sample} member of the workload ({c['n']} DOF): {c['iters']} iters in "
-                              f"{c['t_solve']:.2f}s solve ({c['t_setup']:.2f}s setup, excluded); value scaled by "
-                              f"DOF ratio and GPU/CPU iteration ratio to the 2.1M-DOF solve"),
-                   "sample_solve_s": c["t_solve"], "sample_iters": c["iters"], "sample_relres": c["relres"]}
-            log(f"cpu baseline {cpu['value']:.1f}s (sample {c['t_solve']:.2f}s, {time.time() - t0:.0f}s wall)")
+            cpu = cpu_baseline(s, amg, res["fp64"]["iters"])
+            log(f"cpu baseline {cpu['value']:.1f}s ({time.time() - t0:.0f}s wall)")
         except Exception as e:  # the baseline never blocks the GPU line
             log(f"cpu baseline skipped: {e}")
 

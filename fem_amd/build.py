@@ -25,7 +25,7 @@ DROPIN_SOURCES = ["linalg.c", "active_set.c", "pcgs.c", "multigrid.c", "bicgstab
                   "eigen.c", "gmres.c", "direct.c", "sparse_ops.c", "sweeps.c"]
 
 SOURCES = ["amg_setup.cpp", "solver.cpp", "capi.cpp", "kernels.hip", "krylov.cpp", "rn.cpp", "batch.cpp",
-           "noalloc.hip"]
+           "noalloc.hip", "rap.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LINA_ARCH", "gfx950")

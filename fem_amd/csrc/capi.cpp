@@ -180,7 +180,8 @@ int lina_amg_level_info(const lina_solver* s, int level, int* fine, int* coarse,
 }
 
 static const Csr& pick(const lina_solver* s, int level, char which) {
-    const auto& H = s->s.host();
+    // values as the device holds them (a device recompute leaves the host copy behind until read)
+    const auto& H = const_cast<lina_solver*>(s)->s.host_synced();
     const auto& L = H.lv.at(level);
     switch (which) {
         case 'A': return L.A;

@@ -1202,6 +1202,65 @@ __global__ void k_combine(int n, int m, const double* __restrict__ V, size_t ld,
     }
 }
 
+// ------------------------------------------------- Galerkin products
+
+__global__ void k_plan_sum(int ne, const int64_t* __restrict__ ptr, const int* __restrict__ ia,
+                           const int* __restrict__ ib, const uint8_t* __restrict__ seed,
+                           const double* __restrict__ va, const double* __restrict__ vb, double* __restrict__ out) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
+        int64_t t = ptr[e];
+        const int64_t t1 = ptr[e + 1];
+        double s;
+        if (seed[e] || t == t1) {
+            s = 0.0;
+        } else {
+            s = va[ia[t]] * vb[ib[t]];
+            t++;
+        }
+        for (; t < t1; t++) s += va[ia[t]] * vb[ib[t]];
+        out[e] = s;
+    }
+}
+
+__global__ void k_plan_scatter(int ne, const double* __restrict__ val, const int* __restrict__ pos,
+                               const uint8_t* __restrict__ lone, double* __restrict__ dst, int* __restrict__ flag) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
+        const double v = val[e];
+        const int q = pos[e];
+        if (q >= 0) {
+            if (v != 0.0) dst[q] = v;
+            else if (lone[e]) dst[q] = 0.0;
+            else atomicOr(flag, 1);
+        } else if (v != 0.0) {
+            atomicOr(flag, 1);
+        }
+    }
+}
+
+template <typename VT>
+__global__ void k_row_l1(int n, const int* __restrict__ rp, const double* __restrict__ v, VT* __restrict__ out) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        VT s = 0;
+        for (int p = rp[r]; p < rp[r + 1]; p++) s += fabs((VT)v[p]);
+        out[r] = s;
+    }
+}
+
+template <typename T>
+__global__ void k_gather(int n, const int* __restrict__ idx, const T* __restrict__ src, T* __restrict__ dst) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int k = idx[i];
+        dst[i] = k >= 0 ? src[k] : (T)0;
+    }
+}
+
+__global__ void k_scatter(int n, const int* __restrict__ idx, const double* __restrict__ src, double* __restrict__ dst) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int k = idx[i];
+        if (k >= 0) dst[k] = src[i];
+    }
+}
+
 // ------------------------------------------------- reflective Newton
 //
 // Box-constraint bookkeeping of reflective_newton.c over the bounded index
@@ -1742,6 +1801,7 @@ void fill(hipStream_t st, int n, T* x, T val) {
 }
 template void fill<double>(hipStream_t, int, double*, double);
 template void fill<float>(hipStream_t, int, float*, float);
+template void fill<int>(hipStream_t, int, int*, int);
 
 template <typename TI, typename TO>
 void convert(hipStream_t st, int n, const TI* a, TO* b) {
@@ -2077,6 +2137,34 @@ void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* piv) {
         hipLaunchKernelGGL(k_gj_update, dim3(g), dim3(TPB), 0, st, N, k, M, piv);
         hipLaunchKernelGGL(k_gj_pivot, dim3(gp), dim3(TPB), 0, st, N, k, M, (const double*)piv);
     }
+}
+
+void plan_sum(hipStream_t st, int ne, const int64_t* ptr, const int* ia, const int* ib, const uint8_t* seed,
+              const double* va, const double* vb, double* out) {
+    if (ne > 0) hipLaunchKernelGGL(k_plan_sum, dim3(grid_for(ne)), dim3(TPB), 0, st, ne, ptr, ia, ib, seed, va, vb, out);
+}
+
+void plan_scatter(hipStream_t st, int ne, const double* val, const int* pos, const uint8_t* lone, double* dst,
+                  int* flag) {
+    if (ne > 0) hipLaunchKernelGGL(k_plan_scatter, dim3(grid_for(ne)), dim3(TPB), 0, st, ne, val, pos, lone, dst, flag);
+}
+
+template <typename VT>
+void row_l1(hipStream_t st, int n, const int* rp, const double* v, VT* out) {
+    if (n > 0) hipLaunchKernelGGL((k_row_l1<VT>), dim3(grid_for(n)), dim3(TPB), 0, st, n, rp, v, out);
+}
+template void row_l1<double>(hipStream_t, int, const int*, const double*, double*);
+template void row_l1<float>(hipStream_t, int, const int*, const double*, float*);
+
+template <typename T>
+void gather(hipStream_t st, int n, const int* idx, const T* src, T* dst) {
+    if (n > 0) hipLaunchKernelGGL((k_gather<T>), dim3(grid_for(n)), dim3(TPB), 0, st, n, idx, src, dst);
+}
+template void gather<double>(hipStream_t, int, const int*, const double*, double*);
+template void gather<float>(hipStream_t, int, const int*, const float*, float*);
+
+void scatter(hipStream_t st, int n, const int* idx, const double* src, double* dst) {
+    if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(grid_for(n)), dim3(TPB), 0, st, n, idx, src, dst);
 }
 
 }  // namespace dev

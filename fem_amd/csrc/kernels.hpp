@@ -320,5 +320,24 @@ template <typename TT>
 void inverse_to_tiles(hipStream_t st, int nd, int ntiles, const int* ti, const int* tj, const double* inv,
                       TT* tiles);
 
+// Galerkin products from a plan (rap.cpp): out[e] = the products va[ia[t]] *
+// vb[ib[t]], t in [ptr[e], ptr[e+1]), summed in that order, from +0.0 when
+// seed[e], else from the first product
+void plan_sum(hipStream_t st, int ne, const int64_t* ptr, const int* ia, const int* ib, const uint8_t* seed,
+              const double* va, const double* vb, double* out);
+// the coarse entries into the level's CSR values: dst[pos[e]] = val[e];
+// *flag = 1 when an entry's zero / nonzero state differs from the pattern
+// (pos < 0: not in it; lone: a zero there is the pattern's single (i, 0.0))
+void plan_scatter(hipStream_t st, int ne, const double* val, const int* pos, const uint8_t* lone, double* dst,
+                  int* flag);
+// out[r] = sum_p |(VT) v[p]| over row r, in VT arithmetic and column order
+template <typename VT>
+void row_l1(hipStream_t st, int n, const int* rp, const double* v, VT* out);
+// dst[i] = idx[i] >= 0 ? src[idx[i]] : 0
+template <typename T>
+void gather(hipStream_t st, int n, const int* idx, const T* src, T* dst);
+// dst[idx[i]] = src[i] where idx[i] >= 0
+void scatter(hipStream_t st, int n, const int* idx, const double* src, double* dst);
+
 }  // namespace dev
 }  // namespace lina

@@ -54,22 +54,6 @@ void hip_check(hipError_t e, const char* what) {
 
 // ------------------------------------------------------------ memory
 
-template <typename T>
-T* dalloc(size_t n, DevBuffers& owner) {
-    if (n == 0) n = 1;
-    void* p = nullptr;
-    hip_check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
-    owner.ptrs.push_back(p);
-    return (T*)p;
-}
-
-template <typename T>
-T* dupload(const std::vector<T>& h, DevBuffers& owner, hipStream_t st) {
-    T* d = dalloc<T>(h.size(), owner);
-    if (!h.empty()) hip_check(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st), "upload");
-    return d;
-}
-
 DevBuffers::~DevBuffers() {
     for (void* p : ptrs) (void)hipFree(p);
 }
@@ -448,6 +432,8 @@ void Solver::upload() {
     // a re-upload (recompute) frees every earlier buffer, including the
     // BiCGStab work vectors allocated on first use
     hip_check(hipStreamSynchronize(stream_), "sync");
+    rap_.reset();  // the device products' plans belong to the patterns uploaded before
+    host_stale_ = false;
     bufs_.reset(new DevBuffers());
     for (double** p : {&bS_, &bPS_, &bPP_, &bAPP_, &bAPS_, &bR_, &bR0_, &bP_}) *p = nullptr;
     DevBuffers& B = *bufs_;
@@ -877,11 +863,13 @@ void Solver::refresh_after_recompute() {
     const auto t1 = std::chrono::steady_clock::now();
     bool vals = false;
     if (uploaded()) {
-        vals = refresh_values();
+        vals = refresh_values() && refresh_isolated();
         if (!vals) {
             track_ = true;  // from now on upload() keeps what a values-only refresh compares
             upload();
         }
+    } else {
+        refresh_isolated();
     }
     if (timing) {
         const auto t2 = std::chrono::steady_clock::now();
@@ -967,18 +955,75 @@ void Solver::recompute(const std::vector<double>& val) {
     if ((int)val.size() != A.nnz()) throw std::invalid_argument("recompute: value count differs from the pattern");
     for (int i = 0; i < A.n; i++)
         for (int p = A.rp[i]; p < A.rp[i + 1]; p++) A.v[p] = val[p] * (H_.signs[i] < 0 ? -1.0 : 1.0);
+    static const bool timing = getenv("LINA_RN_TIMING") != nullptr;
+    if (device_rap_ok()) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const bool built = rap_ != nullptr;
+        const bool ok = recompute_device(A);
+        if (timing) {
+            const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            fprintf(stderr, "[amg] recompute on the device: %s %.3fs%s\n", ok ? "done" : "pattern changed", s,
+                    built ? "" : " (plan built)");
+        }
+        if (ok) {
+            H_.lv.back().A = std::move(A);
+            host_stale_ = true;
+            if (!refresh_isolated()) {
+                sync_host_levels();
+                upload();
+            }
+            return;
+        }
+    }
+    sync_host_levels();
     amg_recompute(H_, A);
     refresh_after_recompute();
 }
 
 void Solver::recompute_matrix(const Csr& M) {
     if (M.n != n_) throw std::invalid_argument("recompute: matrix size differs from the hierarchy");
+    const Csr& F = H_.lv.back().A;
+    if (M.rp == F.rp && M.ci == F.ci) {  // the finest pattern holds: the values-only path
+        recompute(M.v);
+        return;
+    }
     Csr A = M;
     for (int i = 0; i < A.n; i++)
         if (H_.signs[i] < 0)
             for (int p = A.rp[i]; p < A.rp[i + 1]; p++) A.v[p] = -A.v[p];
+    sync_host_levels();
     amg_recompute(H_, A);
     refresh_after_recompute();
+}
+
+// AMG_fix_isolated (Multigrid.c:1301-1319) divides by the first nonzero of
+// each isolated finest row: new finest values change the divisors
+bool Solver::refresh_isolated() {
+    const Csr& A = H_.lv.back().A;
+    bool zero = false;
+    for (size_t k = 0; k < H_.iso_rows.size(); k++) {
+        const int i = H_.iso_rows[k];
+        double a = 0.0;
+        for (int p = A.rp[i]; p < A.rp[i + 1]; p++)
+            if (A.v[p] != 0.0) {
+                a = A.v[p];
+                break;
+            }
+        H_.iso_a[k] = a;
+        zero = zero || a == 0.0;
+    }
+    if (!uploaded() || niso_ == 0) return true;
+    if (zero && isa_ok_) return false;  // the fused entry/exit kernels need nonzero divisors
+    hip_check(hipMemcpyAsync(iso_a_, H_.iso_a.data(), (size_t)niso_ * sizeof(double), hipMemcpyHostToDevice, stream_),
+              "h2d");
+    if (isa_) {
+        std::vector<double> isa(n_, 0.0);
+        for (int k = 0; k < niso_; k++) isa[H_.iso_rows[k]] = H_.iso_a[k];
+        hip_check(hipMemcpyAsync(isa_, isa.data(), (size_t)n_ * sizeof(double), hipMemcpyHostToDevice, stream_),
+                  "h2d");
+    }
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    return true;
 }
 
 void Solver::set_operator(DevMatrix* op) {

@@ -33,6 +33,49 @@ struct DevBuffers {
     ~DevBuffers();
 };
 
+// device arrays owned by a DevBuffers (freed with it)
+template <typename T>
+T* dalloc(size_t n, DevBuffers& owner) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+    owner.ptrs.push_back(p);
+    return (T*)p;
+}
+
+template <typename T>
+T* dupload(const std::vector<T>& h, DevBuffers& owner, hipStream_t st) {
+    T* d = dalloc<T>(h.size(), owner);
+    if (!h.empty()) hip_check(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st), "upload");
+    return d;
+}
+
+// The plans of the device Galerkin products (rap.cpp) on the device: per
+// fine level l >= 1 the product lists of A P and R (A P), P's and R's
+// values at the plans' positions, the coarse entries' CSR positions; the
+// colour-permuted copies' source positions; the coarsest level's dense and
+// trivial-row maps; a pattern-change flag.
+struct DeviceRap {
+    struct Lv {
+        int nap = 0, nac = 0;
+        int64_t *ap_ptr = nullptr, *ac_ptr = nullptr;
+        int *ap_a = nullptr, *ap_p = nullptr, *ac_r = nullptr, *ac_ap = nullptr, *pos = nullptr;
+        uint8_t *ap_seed = nullptr, *ac_seed = nullptr, *lone = nullptr;
+        double *pv = nullptr, *rv = nullptr, *ap_v = nullptr, *ac_v = nullptr;
+    };
+    std::vector<Lv> lv;
+    std::vector<int*> acmap;
+    int* dense_map = nullptr;
+    int* tdiag_map = nullptr;
+    int* flag = nullptr;
+    int* host_flag = nullptr;  // pinned
+    double bytes = 0;
+    DevBuffers bufs;
+    ~DeviceRap() {
+        if (host_flag) (void)hipHostFree(host_flag);
+    }
+};
+
 struct DevLevel {
     int n = 0;
     dev::DMat<double> A, P, R, Ac;
@@ -137,6 +180,8 @@ class Solver {
     void set_active(const double* active);
     const lina_amg_params& params() const { return prm_; }
     const HostHierarchy& host() const { return H_; }
+    // the host hierarchy with the values of a device recompute copied back
+    const HostHierarchy& host_synced();
     const std::vector<int>& colours(int l) const { return colours_.at(l); }
     bool uploaded() const { return bufs_ != nullptr; }
     int n() const { return n_; }
@@ -188,6 +233,19 @@ class Solver {
                              const double* scale, int blocked, const double* u, const double* lam, double* grad);
 
   private:
+    // recompute on the device (rap.cpp): the Galerkin products from a plan
+    // of each level's products, then every value-derived array; false (the
+    // caller takes the host path) when it does not apply or a level's
+    // pattern changes
+    bool device_rap_ok() const;
+    bool recompute_device(const Csr& A);
+    void build_rap();
+    void sync_host_levels();
+    // the isolated rows' divisors (AMG_fix_isolated) from the finest values;
+    // false when they need a full upload (a zero divisor appears or goes)
+    bool refresh_isolated();
+    std::unique_ptr<DeviceRap> rap_;
+    bool host_stale_ = false;
     // recompute: colour the new levels and re-upload them (values only when
     // every pattern-derived structure matches what is on the device)
     void refresh_after_recompute();

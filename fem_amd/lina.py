@@ -95,6 +95,7 @@ def lib():
     L.lina_amg_level_copy.argtypes = [C.c_void_p, C.c_int, C.c_char, _ip, _ip, _dp]
     L.lina_amg_labels.argtypes = [C.c_void_p, C.c_int, _ip]
     L.lina_amg_colours.argtypes = [C.c_void_p, C.c_int, _ip]
+    L.lina_amg_signs.argtypes = [C.c_void_p, _dp]
     L.lina_amg_recompute.argtypes = [C.c_void_p, _dp]
     L.lina_pcg.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, C.c_int, C.POINTER(Stats)]
     L.lina_pcg_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int,
@@ -237,6 +238,12 @@ class AMG:
             col = np.zeros(fine.value, np.int32)
             out["ncolours"] = _check(lib().lina_amg_colours(self.h, l, _i(col)))
             out["colours"] = col
+        return out
+
+    def signs(self):
+        """the finest rows' signs applied by make_positive_diag (+1 / -1)"""
+        out = np.zeros(self.n)
+        _check(lib().lina_amg_signs(self.h, _d(out)))
         return out
 
     def recompute(self, val):

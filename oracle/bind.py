@@ -15,6 +15,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_SO = os.path.join(HERE, "_ref", "libreflina.so")
+REF_FAST_SO = os.path.join(HERE, "_ref", "libreflina_fast.so")  # -O3 build, timings only
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 
 _dp = C.POINTER(C.c_double)
@@ -105,6 +106,12 @@ class Ref:
                                              C.c_double, C.c_double, _dp]
         L.refh_set_hybrid.argtypes = [C.c_int]
         L.refh_gmres_ilu.argtypes = [_vp, _dp, _dp, C.c_int, C.c_double, C.c_int, _dp]
+        L.refh_amg_assemble.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp),
+                                        C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_ip), _dp, C.c_int, _ip, _dp, _dp]
+        L.refh_amg_assemble.restype = _vp
+        L.refh_set_threads.argtypes = [C.c_int]
+        L.refh_pcg_assembled.argtypes = [_vp, _vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, _dp, _dp,
+                                         C.c_double, C.c_int, _dp]
         L.refh_init(1)
 
     # -- matrices
@@ -125,6 +132,56 @@ class Ref:
 
     def free(self, M):
         self.L.refh_free(M)
+
+    # -- an assembled hierarchy (timings of the full-size benchmark system)
+    def assemble(self, levels, dof, signs, smiter=1, aggr_smiter=1, coarsening=1):
+        """amg_system_info from level dicts {'A','P','R': (rowptr, col, val),
+        'labels'} (index 0 coarsest), the finest row signs; the coarsest
+        level is solved with the dense inverse of its dense rows.  Returns
+        (handle, keep) -- keep holds what the handle points into."""
+        import numpy as np
+        depth = len(levels)
+        keep = []
+
+        def mat(t):
+            return self.matrix(*t)
+        A = (_vp * depth)(*[mat(lv["A"]) for lv in levels])
+        P = (_vp * depth)(*[mat(lv["P"]) if "P" in lv else None for lv in levels])
+        R = (_vp * depth)(*[mat(lv["R"]) if "R" in lv else None for lv in levels])
+        labs = [np.ascontiguousarray(lv["labels"], np.int32) if "labels" in lv else None for lv in levels]
+        keep.extend(labs)
+        Lp = (_ip * depth)(*[iptr(a) if a is not None else None for a in labs])
+        rp, ci, v = levels[0]["A"]
+        n0 = len(rp) - 1
+        nzc = np.array([np.count_nonzero(v[rp[i]:rp[i + 1]]) for i in range(n0)])
+        didx = np.flatnonzero(nzc > 1).astype(np.int32)
+        pos = -np.ones(n0, np.int64)
+        pos[didx] = np.arange(len(didx))
+        D = np.zeros((len(didx), len(didx)))
+        tdiag = np.zeros(n0)
+        for i in range(n0):
+            cols, vals = ci[rp[i]:rp[i + 1]], v[rp[i]:rp[i + 1]]
+            if pos[i] >= 0:
+                m = pos[cols] >= 0
+                D[pos[i], pos[cols[m]]] = vals[m]
+            else:
+                tdiag[i] = vals[cols == i][0] if np.any(cols == i) else 0.0
+        inv = np.ascontiguousarray(np.linalg.inv(D)) if len(didx) else np.zeros(1)
+        sg = np.ascontiguousarray(signs, np.float64)
+        keep.extend([didx, inv, tdiag, sg])
+        h = self.L.refh_amg_assemble(depth, dof, smiter, aggr_smiter, coarsening, A, P, R, Lp, dptr(sg), len(didx),
+                                     iptr(didx), dptr(inv), dptr(tdiag))
+        return h, (keep, A)
+
+    def pcg_assembled(self, h, M, b, smoother="ssor", alpha=1.0, amg_eps=1e-2, amg_max_iter=1, zero_init=True,
+                      tol=1e-10, max_iter=5):
+        import numpy as np
+        X = np.zeros(len(b))
+        B = np.array(b, dtype=np.float64)
+        t = C.c_double()
+        it = self.L.refh_pcg_assembled(h, M, SMOOTHERS[smoother], alpha, amg_eps, amg_max_iter, int(zero_init),
+                                       dptr(X), dptr(B), tol, max_iter, C.byref(t))
+        return {"iters": it, "x": X, "t_solve": t.value}
 
     # -- AMG hierarchy
     def amg_setup(self, M, dof=1, depth=4, smiter=2, aggr_smiter=3, coarsening=1, matrix_tol=None, cycle="v"):

@@ -312,3 +312,55 @@ def test_amg_solve_reference_jacobi_matches_golden(gpu, case):
     assert np.linalg.norm(x_ref) < 10 * np.linalg.norm(b)      # a convergent case
     assert k == k_ref
     assert rel(x, x_ref) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["poisson33", "damage65", "elasticity33"])
+def test_device_recompute_matches_oracle(gpu, kind, capfd, monkeypatch):
+    """AMG_recompute_coarse_matrices on the device (the Galerkin products
+    R (A P) of every level from a per-pattern plan in LinA's summation
+    order): every level's matrix equals the oracle's recompute
+    (orc_amg_recompute, pinned to the reference) bit for bit, over two
+    successive sets of new finest values; the device path is the one taken
+    (its progress line), and the pcg with the recomputed hierarchy equals the
+    oracle's pcg on the new system."""
+    monkeypatch.setenv("LINA_RN_TIMING", "1")
+    base = {"poisson33": lambda: P.poisson(33), "damage65": lambda: P.damage_system(65)[0],
+            "elasticity33": lambda: P.degraded_elasticity(33)}[kind]()
+    depth = 4
+    orc = Oracle()
+    h = orc.amg_setup(base.rowptr, base.col, base.val, dof=base.dof, depth=depth)
+    amg = gpu.AMG(base.rowptr, base.col, base.val, gpu.params("reference", base.dof, depth))
+    rng = np.random.default_rng(8)
+    for trial in range(2):
+        v = base.val * (1.0 + 0.4 * rng.random(len(base.val)))
+        # keep the matrix symmetric: scale (i,j) and (j,i) alike
+        A = base.scipy().tocsr()
+        A.data = v
+        A = ((A + A.T) * 0.5).tocsr()
+        A.sort_indices()
+        assert np.array_equal(A.indptr, base.rowptr) and np.array_equal(A.indices, base.col)
+        v = np.ascontiguousarray(A.data)
+        capfd.readouterr()
+        amg.recompute(v)
+        err = capfd.readouterr().err
+        say("  ", kind, "trial", trial, [l for l in err.splitlines() if "recompute" in l])
+        assert "recompute on the device: done" in err
+        orc.L.orc_amg_recompute(h, v.ctypes.data_as(__import__("ctypes").POINTER(__import__("ctypes").c_double)))
+        ol = orc.amg_levels(h, depth)
+        for l in range(depth):
+            g = amg.level(l)["A"]
+            o = ol[l]["A"]
+            assert np.array_equal(g[0], o.rowptr) and np.array_equal(g[1], o.col), (trial, l)
+            assert np.array_equal(g[2], o.val), (trial, l)
+    import ctypes as C
+    # pcg preconditioned by the recomputed hierarchies (P and R of the first setup)
+    xo, bo = np.zeros(base.n), base.b.copy()
+    res, vc = C.c_double(), C.c_int()
+    dp = C.POINTER(C.c_double)
+    it = orc.L.orc_pcg_amg(h, xo.ctypes.data_as(dp), bo.ctypes.data_as(dp), 1e-10, 500, 0, 1.6, 1e-2, 100, 0, 0,
+                           C.byref(res), C.byref(vc))
+    g = amg.pcg(base.b, tol=1e-10, max_iter=500)
+    amg.close()
+    orc.amg_free(h)
+    say("  pcg iters oracle", it, "gpu", g["iters"], "rel", rel(g["x"], xo))
+    assert abs(g["iters"] - it) <= 1 and rel(g["x"], xo) < 1e-12

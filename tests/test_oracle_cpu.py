@@ -263,3 +263,26 @@ def test_oracle_pcg_full_multigrid_preconditioner_against_reference(orc, ref, ca
     assert r["iters"] == o["iters"] and 0 < o["iters"] < 300
     assert np.array_equal(r["x"], o["x"]) and r["res"] == o["res"]
     assert np.linalg.norm(s.b - s.scipy() @ o["x"]) <= 1e-8 * np.linalg.norm(s.b)
+
+
+def test_assembled_hierarchy_runs_the_reference_pcg(ref):
+    """The CPU baseline's timing path (oracle/ref_harness.c
+    refh_amg_assemble + refh_pcg_assembled): the reference's pcg and V-cycle
+    on a hierarchy handed over from liblina_hip's host setup, the coarsest
+    level solved with a dense inverse.  On a 65x65-node member of the
+    headline family it takes the reference's own AMG_setup + pcg iterations
+    (the same symmetric SOR V(1,1)) and iterates to 1e-10: the timed work is
+    the reference's algorithm on the reference's hierarchy."""
+    from fem_amd import lina
+    s = P.degraded_elasticity(65)
+    depth = 5
+    amg = lina.AMG(s.rowptr, s.col, s.val, lina.params("vcycle", 2, depth), upload=False)
+    lv = [amg.level(l) for l in range(amg.depth)]
+    h, keep = ref.assemble(lv, 2, amg.signs())
+    M = ref.matrix(s.rowptr, s.col, s.val)
+    got = ref.pcg_assembled(h, M, s.b, tol=1e-8, max_iter=2000)
+    want = ref.pcg_amg(M, s.b, dof=2, depth=depth, smiter=1, aggr_smiter=1, smoother="ssor", alpha=1.0, amg_eps=1e-2,
+                       amg_max_iter=1, zero_init=True, tol=1e-8, max_iter=2000)
+    ref.free(M)
+    assert 0 < want["iters"] < 2000 and got["iters"] == want["iters"]
+    assert np.linalg.norm(got["x"] - want["x"]) <= 1e-10 * np.linalg.norm(want["x"])
