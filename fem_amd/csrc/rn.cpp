@@ -841,4 +841,28 @@ int lina_rn_set_trust_region(lina_rn* h, double delta) {
     });
 }
 
+int lina_circle_minimum(double ax, double ay, double axy, double bx, double by, double delta, double* theta) {
+    return rn_guard([&]() {
+        if (!theta || !(delta > 0)) throw std::invalid_argument("lina_circle_minimum: bad arguments");
+        lina::RootSearch rs;
+        rs.ax = ax;
+        rs.ay = ay;
+        rs.axy = axy;
+        rs.bx = bx;
+        rs.by = by;
+        rs.delta = delta;
+        std::vector<double> roots = rs.roots();
+        if (roots.empty()) throw lina::NumericError("no extremum on the trust-region circle");
+        size_t m = 0;
+        double mv = rs.q(roots[0]);
+        for (size_t i = 1; i < roots.size(); i++)
+            if (rs.q(roots[i]) < mv) {
+                m = i;
+                mv = rs.q(roots[i]);
+            }
+        *theta = roots[m];
+        return (int)roots.size();
+    });
+}
+
 }  // extern "C"

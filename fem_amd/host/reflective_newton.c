@@ -5,6 +5,7 @@
 #include "../../include/linalg/reflective_newton.h"
 
 #include <string.h>
+#include <time.h>
 
 #include "dropin.h"
 
@@ -177,4 +178,87 @@ int reflective_newton(double* X) {
         if (it == RN_max_iter) printf("Reflective Newton: no convergence after %d iterations\n", it);
     }
     return it;
+}
+
+/* reflective_newton.c:394-450 on the host: the coefficients of q(t) by
+ * scalar and sparse_bilinear, the angle by liblina_hip's search over
+ * [0, 2 pi] (lina_circle_minimum). */
+void subspace_minimum(sparse_matrix* A, double* b, double* V1, double* V2, double* X) {
+    const int n = A->size;
+    double t = 0;
+    const double bx = scalar(V1, b, n), by = scalar(V2, b, n);
+    const double ax = sparse_bilinear(V1, A, V1), ay = sparse_bilinear(V2, A, V2);
+    const double axy = sparse_bilinear(V1, A, V2) + sparse_bilinear(V2, A, V1);
+    if (lina_circle_minimum(ax, ay, axy, bx, by, RN_Delta, &t) <= 0) {
+        int i;
+        printf("subspace_minimum: no extrema found (%s)\n", lina_last_error());
+        for (i = 0; i < n; i++) X[i] = NAN;
+        return;
+    }
+    copy_vector_to(V1, X, n);
+    scalar_mult(RN_Delta * cos(t), X, n);
+    vector_add(X, V2, RN_Delta * sin(t), n);
+}
+
+int* get_all_indices(int size) { return get_ID_index_map(size); } /* reflective_newton.c:64-69 */
+
+/* reflective_newton.c:376-392 writes to a fixed directory of the author's
+ * machine; here the file "func" in the working directory: n lines
+ * "x<TAB>F(x)" for x evenly spaced over [min, max] */
+void RN_print_function(double (*F)(double x), int n, double min, double max) {
+    int i;
+    FILE* f = fopen("func", "w");
+    if (!f) {
+        printf("could not open file func\n");
+        return;
+    }
+    for (i = 0; i < n; i++) {
+        const double x = min + (max - min) * ((double)i / (n - 1));
+        fprintf(f, "%f\t%e\n", x, F(x));
+    }
+    fclose(f);
+}
+
+/* reflective_newton.c:719-757: n steps of a random search on the
+ * functional of set_RN_functional: one component moved by +-0.002 inside
+ * its bounds per step (every component must be bounded); a lower value is
+ * kept with probability 1 - exp((curr - prev) / T), a higher one never; T
+ * grows by the factor 1 + 10/n per step.  The random sequence starts from
+ * the time like the reference's (whose step counter starts uninitialised;
+ * here at 0); the result goes to print_vector as there. */
+int simulated_annealing_min(double* X, int n) {
+    int i = 0;
+    const double dx = 2. / 1000;
+    const double z = 10. / n;
+    double T = 1e-4, prev;
+    if (!RN_Functional || RN_bound_num != RN_dim) {
+        printf("simulated_annealing_min: needs set_RN_functional and a bound on every component\n");
+        return -1;
+    }
+    srand((unsigned)time(NULL));
+    prev = RN_Functional(X, RN_dim);
+    printf("start value: %f\n", prev);
+    do {
+        int j, sign;
+        double curr, a, moved;
+        do {
+            j = rand() % RN_dim;
+            sign = 1 - 2 * (rand() % 2);
+            moved = X[j] + sign * dx;
+        } while (moved < RN_lower[j] || moved > RN_upper[j]);
+        X[j] = moved;
+        curr = RN_Functional(X, RN_dim);
+        a = exp((curr - prev) / T);
+        if (a < 1) {
+            if ((double)rand() / RAND_MAX < a) X[j] -= sign * dx;
+            else prev = curr;
+        } else {
+            X[j] -= sign * dx;
+        }
+        i++;
+        T *= 1. + z;
+    } while (i < n);
+    print_vector(X, RN_dim);
+    printf("final value: %f\n", prev);
+    return i;
 }

@@ -365,6 +365,13 @@ int lina_rn_set_trust_region(lina_rn* h, double delta);
  * host; mci/mv need room for rowptr[n] + n entries; returns nnz(M) */
 int lina_rn_scaled_hessian(int n, const int* rowptr, const int* col, const double* val, const double* D,
                            const double* G, const double* J, int* mrp, int* mci, double* mv);
+/* The angle search of subspace_minimum (reflective_newton.c:394-450): the
+ * stationary points of q(t) = c (c ax + s axy + bx) + s (s ay + by),
+ * c = delta cos t, s = delta sin t, found on 1000 intervals of [0, 2 pi]
+ * (GMRES_Newton.c:402-439); *theta receives the one of least q.  Returns
+ * the number of stationary points, negative (lina_last_error) when there
+ * is none. */
+int lina_circle_minimum(double ax, double ay, double axy, double bx, double by, double delta, double* theta);
 
 #ifdef __cplusplus
 }

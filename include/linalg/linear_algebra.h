@@ -11,6 +11,10 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+/* included by LinA/include/linear_algebra.h:10-11 as well */
+#include "File_stuff.h"
+#include "red_black_tree.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -23,6 +27,41 @@ typedef struct SPARSE {
     int* Len;
     int size;
 } sparse_matrix;
+
+/* the other containers of LinA/linear_algebra.h:24-80, same members in the same order */
+typedef struct MATRIX_ELEMENT {
+    double val;
+    int* Ind;
+} matrix_element;
+typedef struct SPARSE3D {       /* B_abc: row a holds Len[a] entries (Indices1, Indices2) = (b, c) */
+    double** Values;
+    int** Indices1;
+    int** Indices2;
+    int* Len;
+    int size;
+} sparse_matrix3D;
+typedef struct SPARSE_GENERAL {
+    matrix_element* Elements;
+    int size;
+    int rank;
+    int* Dimension;
+} sparse_general;
+typedef struct MAP_SPARSE {
+    int* Indices;
+    sparse_matrix* Pattern;
+} map_sparse;
+typedef struct PRODUCT_PATTERN { /* set_product_pattern: per entry of A B its value-address pairs */
+    double**** Pairs;
+    int** Indices;
+    int** Size_2;
+    int* Size_1;
+    int Size_0;
+} product_pattern;
+typedef struct UMFPACK_MATRIX_INFO {
+    int* Ind;
+    int* Col;
+    double* Val;
+} umfpack_matrix_info;
 
 sparse_matrix* sparse_zero(int length);                          /* linear_algebra.c:501-508: (i, 0.0) per row */
 int get_position(int* list, int size, int val, int* Flag);       /* :1515-1537 */
@@ -104,6 +143,80 @@ long double hp_euklid_norm(double* x1, int n);                            /* :27
 void set_thread_num(int n);                                               /* :23-32: host threads of the setup */
 void set_omp_chunk_size(int chunk);                                       /* :34-36 */
 
+/* row-list algebra, restrictions and formats (host) */
+void scalar_sparse_mult(double a, sparse_matrix* A);                      /* :318-323 */
+void sparse_add(sparse_matrix* A, sparse_matrix* B, double factor);       /* :325-334 */
+void add_subsparse_to_sparse(sparse_matrix* A, sparse_matrix* B, int i_o, int j_o); /* :1209-1219 */
+double* sparse_row_sum(sparse_matrix* A);                                 /* :3271-3278 */
+double* get_col_sums(sparse_matrix* A, int col_num);                      /* :3464-3471 */
+void normalize_row_sum(sparse_matrix* A, double* Row_Sums);               /* :455-464 */
+sparse_matrix* sparse_diagonal(double* d, int n);                         /* :484-499 */
+sparse_matrix* sparse_identity(int length);                               /* :774-781 */
+sparse_matrix* get_1D_Laplace(double* F, int n);                          /* :4192-4207 */
+int find_position_in_row(sparse_matrix* A, int row, int col);             /* :1552-1560 */
+void remove_element_at(sparse_matrix* A, int i, int pos);                 /* :1161-1179 */
+void remove_zero(sparse_matrix* A, int i);                                /* :1905-1910 */
+void remove_zeros(sparse_matrix* A);                                      /* :1912-1915 */
+void reset_row(sparse_matrix* A, int i);                                  /* :1245-1253 */
+sparse_matrix* restrict_matrix_rows(sparse_matrix* A, int* List, int list_size, int* index_map); /* :3242-3269 */
+double* restrict_matrix_cols(sparse_matrix* A, double* Sol, int* index_map); /* :3280-3299 */
+void convert_sparse_to_UMFPACK(sparse_matrix* A, int col_num, int** Col_start, int** Indices, double** Values,
+                               int* Size);                                /* :3182-3212 */
+void free_UMFPACK_matrix_info(umfpack_matrix_info* A);                    /* :5515-5519 */
+void free_sparse_index_map(map_sparse** Map);                             /* :696-701 */
+sparse_general* new_sparse_general(int rank, int* Dim);                   /* :5224-5231 */
+void sparse_general_append(sparse_general* A, double val, int rank, ...); /* :5241-5256 */
+product_pattern set_product_pattern(sparse_matrix* A, sparse_matrix* B, sparse_matrix* B_transposed,
+                                    sparse_matrix** AB);                  /* :5078-5116 */
+void fast_sparse_product(sparse_matrix* P, sparse_matrix* A, sparse_matrix* BT, product_pattern* Pattern); /* :5134-5172 */
+sparse_matrix* get_test_matrix(int n, int random_rot);                    /* :3885-3911 */
+void print_sparse(sparse_matrix* A);                                      /* :918-935 (file "sparse_matrix") */
+
+/* third-order sparse tensors and nodal rank-4 tensors (host) */
+sparse_matrix3D* sparse_zero3D(int n);                                    /* :510-525 */
+sparse_matrix* sparse3D_vB(sparse_matrix3D* B, double* X, int new_size);  /* :527-535 */
+sparse_matrix* sparse3D_Bv(sparse_matrix3D* B, double* X);                /* :551-561 */
+sparse_matrix* sparse3D_Bv_symmetric(sparse_matrix3D* B, double* X);      /* :537-549 */
+double* sparse3D_Bvv(sparse_matrix3D* B, double* X1, double* X2);         /* :573-588 */
+double* sparse3D_vvB(sparse_matrix3D* B, double* X1, double* X2, int size); /* :590-601 */
+double* sparse3D_bilinear(sparse_matrix3D* B, double* X1, double* X2);    /* :679-686 */
+void sparse3D_scalar_mult(sparse_matrix3D* B, double a);                  /* :640-647 */
+void sparse3D_add(sparse_matrix3D* A, sparse_matrix3D* B);                /* :649-677 */
+void sparse3D_merge(sparse_matrix3D* B);                                  /* :703-741 */
+sparse_matrix3D* sparse3D_transpose13(sparse_matrix3D* B, int new_size);  /* :603-638 */
+sparse_matrix3D* get_transpose3D_13(sparse_matrix3D* B, int new_size);    /* :1343-1380 */
+double* rank4_product(double* A, double* B, int dim, int mesh_size);      /* :91-110 */
+double* rank4_tensor_product(double* A, double* B, double* C, double* D, int dim, int mesh_size); /* :112-126 */
+void add_rank4_tensor_product(double* X, double* A, double* B, double* C, double* D, int dim,
+                              int mesh_size);                             /* :128-132 */
+
+/* vector, index-list and sorting utilities (host) */
+#ifndef __cplusplus
+int min(int a, int b); /* :13-19 (C only, as in the reference) */
+int max(int a, int b);
+#endif
+int* clone_list(int* x, int n);                                           /* :84-89 */
+double* generate_vector(int size, double val);                            /* :4044-4049 */
+int* get_ID_index_map(int size);                                          /* :2622-2627 */
+double get_max_element_abs(double* X, int size);                          /* :2642-2649 */
+double get_mean(double* List, int size);                                  /* :4166-4172 */
+int in_list(int* List, int size, int n);                                  /* :1821-1827 */
+double* join_vectors(double* V1, int n1, double* V2, int n2);             /* :3420-3426 */
+void map_back(double* Solution, double* Restricted, int n_sol, int* index_map); /* :3398-3405 */
+double* restrict_vector(double* Vector, int* index_map, int n_sol, int n_res);  /* :3370-3380 */
+double* restrict_vector_by(int i_min, int i_max, double* Vector);         /* :3382-3389 */
+double vec_dist(double* x1, double* x2, int n);                           /* :288-297 */
+double vector_contraction(double* x, int n);                              /* :344-349 */
+void vector_normalize(double* x, int n);                                  /* :306-309 */
+void vector_pseudo_mult(double* x, double* y, int n);                     /* :351-354 */
+void vector_pseudo_div(double* x, double* y, int n);                      /* :356-359 */
+void vector_shift(double* x, int n, double a);                            /* :361-364 */
+void print_vector(double* V, int n);                                      /* :1058-1067 (file "vector") */
+void print_scalar_data(char* Name, double* V, int n);                     /* :1069-1081 */
+void int_sort(int* List, int size);                                       /* :1635-1653 */
+void join_lists(int** List, int* size, int* L1, int* L2, int s1, int s2); /* :1655-1687 */
+void divide_and_conquer_sort(int* List, void** Values, int size, int (*Compare)(void* X1, void* X2)); /* :1591-1622 */
+
 /* eigenvalue searches (power iterations on the GPU) */
 double matrix_norm(sparse_matrix* A);                                     /* linear_algebra.c:3441-3450 */
 void set_random_normal_vector(double* X, int n);                          /* :4824-4829 */
@@ -121,6 +234,16 @@ sparse_matrix* read_sparse(char* path, char* name);                       /* :84
 sparse_matrix* read_sparse_textfile(char* path, char* name);              /* :870-916 */
 void write_sparse_textfile(char* path, char* name, sparse_matrix* A);     /* :983-1002 */
 void print_to_matrix_market_format(sparse_matrix* A, char* Name);        /* :1023-1056 */
+
+/* further utilities of linear_algebra.c (slimer.c's callers) */
+void insert_vector(double* Vector, double* vector, int start, int size);  /* :3391-3396 */
+int test_vector(double* X, int n, char* message);                         /* :1881-1888 */
+int test_negative(double* X, int n);                                      /* :1890-1897 */
+void remove_negative(double* X, int n);                                   /* :1899-1902 */
+void vector_permutation(double* F, int* Map, int n);                      /* :2857-2862 */
+void sparse_row_permutation(sparse_matrix* A, int* Map);                  /* :2836-2855 */
+double get_diag_dominance(sparse_matrix* A);                              /* :2413-2429 */
+sparse_matrix* Lap_2D_Dirichlet(double a, double b, int dom_size);        /* :4312-4339 */
 
 #ifdef __cplusplus
 }

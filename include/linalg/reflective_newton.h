@@ -64,6 +64,19 @@ void RN_set_kernel(double* K, int n);
 void RN_init_statistics(void);
 void print_RN_statistics(FILE* file);
 int reflective_newton(double* X);
+/* reflective_newton.c:394-450: X = c V1 + s V2, c = Delta cos t,
+ * s = Delta sin t, for the t of least q = X.A.X + X.b on the circle of the
+ * current trust radius Delta (RN_set_trust_region, RN_set_params), found
+ * among the stationary points of q(t).  When there is none the reference
+ * exits; here X is filled with NaN and lina_last_error() says why. */
+void subspace_minimum(sparse_matrix* A, double* b, double* V1, double* V2, double* X);
+/* reflective_newton.c:64-69: the list 0, 1, ..., size-1 */
+int* get_all_indices(int size);
+/* :376-392: n samples of F over [min, max] into the file "func" */
+void RN_print_function(double (*F)(double x), int n, double min, double max);
+/* :719-757: random search on set_RN_functional's functional within the
+ * bounds of set_RN_constraints; -1 unless every component is bounded */
+int simulated_annealing_min(double* X, int n);
 
 #ifdef __cplusplus
 }

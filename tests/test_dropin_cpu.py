@@ -90,8 +90,8 @@ def test_incomplete_lu_matches_reference(libs):
 def test_headers_compile_and_symbols_exported(tmp_path):
     inc = os.path.join(ROOT, "include", "linalg")
     src = tmp_path / "t.c"
-    src.write_text('#include "linear_algebra.h"\n#include "PCGS.h"\n#include "Multigrid.h"\n'
-                   '#include "BiCGStab.h"\n#include "reflective_newton.h"\nint main(void){return 0;}\n')
+    heads = sorted(h for h in os.listdir(inc) if h.endswith(".h"))
+    src.write_text("".join(f'#include "{h}"\n' for h in heads) + "int main(void){return 0;}\n")
     r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I", inc,
                         "-I", os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
@@ -480,3 +480,672 @@ def test_amg_smoothers_called_directly_match_reference(libs):
         out.append(r)
     for a, b in zip(*out):
         assert np.array_equal(a, b)
+
+
+def test_list_and_vector_utilities_match_reference(libs, tmp_path):
+    """The vector, index-list and sorting utilities the callers use
+    (linear_algebra.c:84-364, 1058-1081, 1591-1687, 2622-2649, 3370-3426,
+    4044-4172) against the compiled reference: equal results, including
+    divide_and_conquer_sort's order among elements that do not compare
+    greater (a comparator on the first key of pairs with many ties)."""
+    import ctypes as C
+    DP, IP = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    rng = np.random.default_rng(12)
+    x = rng.normal(size=37)
+    y = rng.uniform(0.5, 2.0, size=37)
+    il = rng.integers(0, 20, size=25).astype(np.int32)
+    il2 = rng.integers(10, 40, size=17).astype(np.int32)
+    imap = np.where(rng.random(37) < 0.6, -1, 0).astype(np.int32)
+    imap[imap == 0] = np.arange(np.count_nonzero(imap == 0))
+    keys = rng.integers(0, 4, size=60).astype(np.int32)
+    CMP = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)
+
+    def cmp(a, b):
+        return 1 if C.cast(a, IP)[0] > C.cast(b, IP)[0] else 0
+    ccmp = CMP(cmp)
+    out = []
+    for k, lib in enumerate(libs):
+        L = lib.L
+        r = {}
+
+        def f(name, args, res=None):
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+            return fn
+        r["clone_list"] = np.ctypeslib.as_array(f("clone_list", [IP, C.c_int], IP)(il.ctypes.data_as(IP), 25), (25,)).copy()
+        r["gen"] = np.ctypeslib.as_array(f("generate_vector", [C.c_int, C.c_double], DP)(9, 2.5), (9,)).copy()
+        r["id"] = np.ctypeslib.as_array(f("get_ID_index_map", [C.c_int], IP)(11), (11,)).copy()
+        r["maxabs"] = f("get_max_element_abs", [DP, C.c_int], C.c_double)(dptr(x), 37)
+        r["mean"] = f("get_mean", [DP, C.c_int], C.c_double)(dptr(x), 37)
+        r["in_list"] = [f("in_list", [IP, C.c_int, C.c_int], C.c_int)(il.ctypes.data_as(IP), 25, v) for v in (3, 99)]
+        r["join_vec"] = np.ctypeslib.as_array(f("join_vectors", [DP, C.c_int, DP, C.c_int], DP)(dptr(x), 37, dptr(y), 5),
+                                              (42,)).copy()
+        sol = np.zeros(37)
+        nres = int(imap.max()) + 1
+        res_ = np.ctypeslib.as_array(f("restrict_vector", [DP, IP, C.c_int, C.c_int], DP)(
+            dptr(x), imap.ctypes.data_as(IP), 37, nres), (nres,)).copy()
+        r["restrict"] = res_
+        f("map_back", [DP, DP, C.c_int, IP])(dptr(sol), dptr(res_), 37, imap.ctypes.data_as(IP))
+        r["map_back"] = sol
+        r["by"] = np.ctypeslib.as_array(f("restrict_vector_by", [C.c_int, C.c_int, DP], DP)(3, 11, dptr(x)), (8,)).copy()
+        r["dist"] = f("vec_dist", [DP, DP, C.c_int], C.c_double)(dptr(x), dptr(y), 37)
+        r["contr"] = f("vector_contraction", [DP, C.c_int], C.c_double)(dptr(x), 37)
+        for name in ("vector_normalize",):
+            z = x.copy()
+            f(name, [DP, C.c_int])(dptr(z), 37)
+            r[name] = z
+        for name in ("vector_pseudo_mult", "vector_pseudo_div"):
+            z = x.copy()
+            f(name, [DP, DP, C.c_int])(dptr(z), dptr(y), 37)
+            r[name] = z
+        z = x.copy()
+        f("vector_shift", [DP, C.c_int, C.c_double])(dptr(z), 37, -0.25)
+        r["shift"] = z
+        z = il.copy()
+        f("int_sort", [IP, C.c_int])(z.ctypes.data_as(IP), 25)
+        r["int_sort"] = z
+        lst, sz = IP(), C.c_int()
+        f("join_lists", [C.POINTER(IP), IP, IP, IP, C.c_int, C.c_int])(C.byref(lst), C.byref(sz), il.ctypes.data_as(IP),
+                                                                       il2.ctypes.data_as(IP), 25, 17)
+        r["join_lists"] = [lst[i] for i in range(sz.value)]
+        ptrs = (C.c_void_p * 60)(*[keys.ctypes.data + 4 * i for i in range(60)])
+        order = np.arange(60, dtype=np.int32)
+        f("divide_and_conquer_sort", [IP, C.POINTER(C.c_void_p), C.c_int, C.c_void_p])(
+            order.ctypes.data_as(IP), ptrs, 60, C.cast(ccmp, C.c_void_p))
+        r["dc_sort"] = order
+        fn = str(tmp_path / f"data{k}").encode()
+        f("print_scalar_data", [C.c_char_p, DP, C.c_int])(fn, dptr(x), 37)
+        r["file"] = open(fn, "rb").read()
+        out.append(r)
+    for key in out[1]:
+        a, b = out[0][key], out[1][key]
+        assert np.array_equal(a, b) if isinstance(b, np.ndarray) else a == b, key
+    assert np.all(np.diff(keys[out[0]["dc_sort"]]) >= 0)
+
+
+def test_row_list_algebra_matches_reference(libs, tmp_path, monkeypatch):
+    """Row-list algebra, restrictions, formats and fixed-pattern products
+    (linear_algebra.c:318-499, 696-781, 918-935, 1161-1253, 1552-1560,
+    1905-1915, 3182-3299, 3464-3471, 3885-3911, 4192-4207, 5078-5256)
+    against the compiled reference: identical row lists, vectors, UMFPACK
+    arrays, files and fast_sparse_product refills."""
+    import ctypes as C
+    from tests.dropin_bind import SparseMatrix
+    SM = C.POINTER(SparseMatrix)
+    DP, IP = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    monkeypatch.chdir(tmp_path)
+    s = P.damage_system(7)[0]
+    n = s.n
+    rng = np.random.default_rng(21)
+    sol = rng.normal(size=n)
+    lst = np.array(sorted(rng.choice(n, 9, replace=False)), np.int32)
+    out = []
+    for k, lib in enumerate(libs):
+        L = lib.L
+        r = {}
+
+        def f(name, args, res=None):
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+            return fn
+        A = lib.from_csr(s.rowptr, s.col, s.val)
+        B = lib.from_csr(s.rowptr, s.col, np.sin(np.arange(len(s.val))))
+        f("scalar_sparse_mult", [C.c_double, SM])(0.75, A)
+        f("sparse_add", [SM, SM, C.c_double])(A, B, -0.5)
+        r["add"] = LinA.to_lists(A)
+        Bs = lib.from_csr(np.array([0, 2, 3], np.int32), np.array([0, 1, 1], np.int32), np.array([1.5, 0.0, -2.0]))
+        f("add_subsparse_to_sparse", [SM, SM, C.c_int, C.c_int])(A, Bs, 3, 4)
+        r["addsub"] = LinA.to_lists(A)
+        r["rowsum"] = np.ctypeslib.as_array(f("sparse_row_sum", [SM], DP)(A), (n,)).copy()
+        r["colsum"] = np.ctypeslib.as_array(f("get_col_sums", [SM, C.c_int], DP)(A, n), (n,)).copy()
+        C2 = L.clone(A)
+        f("normalize_row_sum", [SM, DP])(C2, None)
+        r["norm"] = LinA.to_lists(C2)
+        f("normalize_row_sum", [SM, DP])(C2, dptr(sol))
+        r["norm2"] = LinA.to_lists(C2)
+        r["diag"] = LinA.to_lists(f("sparse_diagonal", [DP, C.c_int], SM)(dptr(sol), n))
+        r["ident"] = LinA.to_lists(f("sparse_identity", [C.c_int], SM)(5))
+        F = np.linspace(-1, 1, 9)
+        r["lap"] = LinA.to_lists(f("get_1D_Laplace", [DP, C.c_int], SM)(dptr(F), 9))
+        r["lapF"] = F
+        r["find"] = f("find_position_in_row", [SM, C.c_int, C.c_int], C.c_int)(A, 5, int(s.col[s.rowptr[5] + 1]))
+        D = L.clone(A)
+        f("remove_element_at", [SM, C.c_int, C.c_int])(D, 4, 1)
+        f("reset_row", [SM, C.c_int])(D, 6)
+        for i in (2, 9):
+            L.set_sparse(D, 0.0, i, i)
+        f("remove_zeros", [SM])(D)
+        r["edit"] = LinA.to_lists(D)
+        imap = np.zeros(n, np.int32)
+        Rr = f("restrict_matrix_rows", [SM, IP, C.c_int, IP], SM)(A, lst.ctypes.data_as(IP), len(lst),
+                                                                  imap.ctypes.data_as(IP))
+        r["rrows"] = LinA.to_lists(Rr)
+        r["imap"] = imap.copy()
+        rhs = np.ctypeslib.as_array(f("restrict_matrix_cols", [SM, DP, IP], DP)(Rr, dptr(sol), imap.ctypes.data_as(IP)),
+                                    (n - len(lst),)).copy()
+        r["rcols"] = LinA.to_lists(Rr)
+        r["rhs"] = rhs
+        cs, ind, val, size = IP(), IP(), DP(), C.c_int()
+        f("convert_sparse_to_UMFPACK", [SM, C.c_int, C.POINTER(IP), C.POINTER(IP), C.POINTER(DP), IP])(
+            A, n, C.byref(cs), C.byref(ind), C.byref(val), C.byref(size))   # (LinA writes out of bounds for fewer)
+        r["umf"] = ([cs[i] for i in range(n + 1)], [ind[i] for i in range(size.value)],
+                    [val[i] for i in range(size.value)])
+        class PP(C.Structure):
+            _fields_ = [("Pairs", C.c_void_p), ("Indices", C.POINTER(IP)), ("Size_2", C.POINTER(IP)), ("Size_1", IP),
+                        ("Size_0", C.c_int)]
+        AB = SM()
+        pp = f("set_product_pattern", [SM, SM, SM, C.POINTER(SM)], PP)(A, B, None, C.byref(AB))
+        r["pp_sizes"] = [[pp.Size_2[i][j] for j in range(pp.Size_1[i])] for i in range(pp.Size_0)]
+        r["AB"] = LinA.to_lists(AB)
+        for i in range(n):                         # new values at the same addresses
+            for j in range(A.contents.Len[i]):
+                A.contents.Values[i][j] = A.contents.Values[i][j] * 1.25 - 0.1
+        f("fast_sparse_product", [SM, SM, SM, C.POINTER(PP)])(AB, A, B, C.byref(pp))
+        r["fast"] = LinA.to_lists(AB)
+        if k == 0:  # (LinA's get_test_matrix writes through print_vector to a fixed path: it crashes elsewhere)
+            T = f("get_test_matrix", [C.c_int, C.c_int], SM)(8, 6)
+            M8 = np.zeros((8, 8))
+            for i, (cols, vals) in enumerate(LinA.to_lists(T)):
+                M8[i, cols] = vals
+            want = np.arange(1, 9) / 8.0
+            want[4] = -want[4]
+            assert np.allclose(np.sort(np.linalg.eigvalsh(0.5 * (M8 + M8.T))), np.sort(want), atol=1e-14)
+            assert np.allclose(M8, M8.T, atol=1e-14) and np.count_nonzero(M8 - np.diag(np.diag(M8))) > 0
+            f("print_sparse", [SM])(Bs)   # (LinA's also writes to a fixed path of the author's machine)
+            assert open("sparse_matrix").read() == "(0|1.500000000000000e+00)  (1|0.000000000000000e+00)\n" \
+                                                   "(1|-2.000000000000000e+00)\n"
+        out.append(r)
+    for key in out[1]:
+        a, b = out[0][key], out[1][key]
+        assert np.array_equal(a, b) if isinstance(b, np.ndarray) else a == b, key
+
+
+def test_sparse3d_and_rank4_match_reference(libs):
+    """Third-order sparse tensors and nodal rank-4 tensors
+    (linear_algebra.c:91-133, 510-760, 1343-1380): a tensor built from random
+    triples with repeats and zeros, its contractions, transposes, merge,
+    scaling and sums; rank-4 products of random nodal fields -- identical to
+    the reference's."""
+    import ctypes as C
+    DP, IP = C.POINTER(C.c_double), C.POINTER(C.c_int)
+
+    class S3(C.Structure):
+        _fields_ = [("Values", C.POINTER(DP)), ("Indices1", C.POINTER(IP)), ("Indices2", C.POINTER(IP)),
+                    ("Len", IP), ("size", C.c_int)]
+    from tests.dropin_bind import SparseMatrix
+    SM = C.POINTER(SparseMatrix)
+    S3P = C.POINTER(S3)
+    rng = np.random.default_rng(31)
+    n = 9
+    trip = [(int(a), int(b), int(c), float(v)) for a, b, c, v in
+            zip(rng.integers(0, n, 70), rng.integers(0, n, 70), rng.integers(0, n, 70), rng.normal(size=70))]
+    trip += trip[:10]                                  # repeats
+    trip += [(1, 2, 3, 0.0), (4, 4, 4, 0.0)]           # zeros
+    x1, x2 = rng.normal(size=n), rng.normal(size=n)
+    dim, ms = 2, 5
+    T = [rng.normal(size=dim ** 4 * ms) for _ in range(2)]
+    V = [rng.normal(size=dim * ms) for _ in range(4)]
+
+    def lists3(p):
+        b = p.contents
+        return [[(b.Indices1[i][j], b.Indices2[i][j], b.Values[i][j]) for j in range(b.Len[i])] for i in range(b.size)]
+    out = []
+    for lib in libs:
+        L = lib.L
+
+        def f(name, args, res=None):
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+            return fn
+        r = {}
+        zero3 = f("sparse_zero3D", [C.c_int], S3P)
+
+        libc = C.CDLL(None)
+        libc.realloc.restype = C.c_void_p
+        libc.realloc.argtypes = [C.c_void_p, C.c_size_t]
+
+        def build():
+            B = zero3(n)
+            b = B.contents
+            for a, i, j, v in trip:                    # append triples the way FEM2D's assembly does (realloc)
+                m = b.Len[a] + 1
+                b.Values[a] = C.cast(libc.realloc(C.cast(b.Values[a], C.c_void_p), 8 * m), DP)
+                b.Indices1[a] = C.cast(libc.realloc(C.cast(b.Indices1[a], C.c_void_p), 4 * m), IP)
+                b.Indices2[a] = C.cast(libc.realloc(C.cast(b.Indices2[a], C.c_void_p), 4 * m), IP)
+                b.Values[a][m - 1] = v
+                b.Indices1[a][m - 1] = i
+                b.Indices2[a][m - 1] = j
+                b.Len[a] = m
+            return B
+        B = build()
+        r["vB"] = LinA.to_lists(f("sparse3D_vB", [S3P, DP, C.c_int], SM)(B, dptr(x1), n))
+        r["Bv"] = LinA.to_lists(f("sparse3D_Bv", [S3P, DP], SM)(B, dptr(x1)))
+        r["Bvs"] = LinA.to_lists(f("sparse3D_Bv_symmetric", [S3P, DP], SM)(B, dptr(x1)))
+        r["Bvv"] = np.ctypeslib.as_array(f("sparse3D_Bvv", [S3P, DP, DP], DP)(B, dptr(x1), dptr(x2)), (n,)).copy()
+        r["vvB"] = np.ctypeslib.as_array(f("sparse3D_vvB", [S3P, DP, DP, C.c_int], DP)(B, dptr(x1), dptr(x2), n),
+                                         (n,)).copy()
+        r["bil"] = np.ctypeslib.as_array(f("sparse3D_bilinear", [S3P, DP, DP], DP)(B, dptr(x1), dptr(x2)), (n,)).copy()
+        r["t13"] = lists3(f("sparse3D_transpose13", [S3P, C.c_int], S3P)(B, n + 2))
+        r["T13"] = lists3(f("get_transpose3D_13", [S3P, C.c_int], S3P)(B, n + 2))
+        B2 = build()
+        f("sparse3D_scalar_mult", [S3P, C.c_double])(B2, -1.5)
+        f("sparse3D_add", [S3P, S3P])(B2, B)
+        r["add"] = lists3(B2)
+        f("sparse3D_merge", [S3P])(B2)
+        r["merge"] = lists3(B2)
+        r["r4"] = np.ctypeslib.as_array(f("rank4_product", [DP, DP, C.c_int, C.c_int], DP)(dptr(T[0]), dptr(T[1]), dim,
+                                                                                          ms), (dim ** 4 * ms,)).copy()
+        r["r4t"] = np.ctypeslib.as_array(f("rank4_tensor_product", [DP] * 4 + [C.c_int, C.c_int], DP)(
+            *[dptr(v) for v in V], dim, ms), (dim ** 4 * ms,)).copy()
+        X = T[0].copy()
+        f("add_rank4_tensor_product", [DP] * 5 + [C.c_int, C.c_int])(dptr(X), *[dptr(v) for v in V], dim, ms)
+        r["ar4"] = X
+        out.append(r)
+    for key in out[1]:
+        a, b = out[0][key], out[1][key]
+        assert np.array_equal(a, b) if isinstance(b, np.ndarray) else a == b, key
+
+
+def _rb_types():
+    import ctypes as C
+
+    class Node(C.Structure):
+        pass
+    Node._fields_ = [("key", C.c_void_p), ("info", C.c_void_p), ("red", C.c_int), ("left", C.POINTER(Node)),
+                     ("right", C.POINTER(Node)), ("parent", C.POINTER(Node))]
+    CMP = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)
+    DEST = C.CFUNCTYPE(None, C.c_void_p)
+    PK = C.CFUNCTYPE(None, C.c_void_p)
+
+    class Tree(C.Structure):
+        _fields_ = [("Compare", CMP), ("DestroyKey", DEST), ("DestroyInfo", DEST), ("PrintKey", PK),
+                    ("PrintInfo", DEST), ("root", C.POINTER(Node)), ("nil", C.POINTER(Node))]
+
+    class SNode(C.Structure):
+        pass
+    SNode._fields_ = [("info", C.c_void_p), ("next", C.POINTER(SNode))]
+
+    class Stack(C.Structure):
+        _fields_ = [("top", C.POINTER(SNode)), ("tail", C.POINTER(SNode))]
+    return Node, Tree, SNode, Stack, CMP, DEST, PK
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_rb_tree_and_stack_match_reference(libs, seed):
+    """The red-black tree and stack of red_black_tree.c/stack.c as FEM2D and
+    K2_tree use them: keys with many ties and a tolerance comparison (as
+    K2_tree.c:29-50), insertions, deletions of nodes picked in key order, the
+    shape and colours the callers walk from tree->root->left, the
+    destroy-callback sequences, exact queries, largest, enumeration ranges,
+    predecessor/successor walks and counts -- all identical to the
+    reference's; with a brute-force check of the ordering."""
+    import ctypes as C
+    Node, Tree, SNode, Stack, CMP, DEST, PK = _rb_types()
+    NP, TP, SP = C.POINTER(Node), C.POINTER(Tree), C.POINTER(Stack)
+    rng = np.random.default_rng(seed)
+    nkeys = 400
+    vals = (rng.integers(0, 70, nkeys) + rng.normal(0, 1e-12, nkeys)).astype(np.float64)
+    keys = (C.c_double * nkeys)(*vals)
+    base = C.addressof(keys)
+    picks = rng.random(150)                        # the same deletions for both libraries
+
+    def kidx(p):
+        return (p - base) // 8
+
+    def cmp(a, b):
+        x = C.c_double.from_address(a).value
+        y = C.c_double.from_address(b).value
+        return 0 if abs(x - y) < 1e-10 else (1 if x > y else -1)
+    results = []
+    for lib in libs:
+        L = lib.L
+        events = []
+        cb_cmp = CMP(cmp)
+        cb_dk = DEST(lambda p: events.append(("k", kidx(p))))
+        cb_di = DEST(lambda p: events.append(("i", p)))
+        cb_pk = PK(lambda p: events.append(("pk", kidx(p))))
+        cb_pi = DEST(lambda p: events.append(("pi", p)))
+        L.RBTreeCreate.argtypes = [CMP, DEST, DEST, PK, DEST]
+        L.RBTreeCreate.restype = TP
+        L.RBTreeInsert.argtypes = [TP, C.c_void_p, C.c_void_p]
+        L.RBTreeInsert.restype = NP
+        for nm in ("TreeSuccessor", "TreePredecessor"):
+            getattr(L, nm).argtypes = [TP, NP]
+            getattr(L, nm).restype = NP
+        L.RBDelete.argtypes = [TP, NP]
+        L.RBExactQuery.argtypes = [TP, C.c_void_p]
+        L.RBExactQuery.restype = NP
+        L.RBLargest.argtypes = [TP]
+        L.RBLargest.restype = NP
+        L.RBEnumerate.argtypes = [TP, C.c_void_p, C.c_void_p]
+        L.RBEnumerate.restype = SP
+        L.RBTreeCount.argtypes = [TP]
+        L.RBTreePrint.argtypes = [TP]
+        L.RBTreeDestroy.argtypes = [TP]
+        L.StackDestroy.argtypes = [SP, DEST]
+        t = L.RBTreeCreate(cb_cmp, cb_dk, cb_di, cb_pk, cb_pi)
+        T = t.contents
+        nil = C.addressof(T.nil.contents)
+
+        def addr(p):
+            return C.cast(p, C.c_void_p).value
+
+        def shape(p, out):                 # preorder with colours, nil as None
+            if addr(p) == nil:
+                out.append(None)
+                return out
+            nd = p.contents
+            out.append((kidx(nd.key), nd.info, nd.red))
+            shape(nd.left, out)
+            shape(nd.right, out)
+            return out
+
+        def inorder():
+            seq, p = [], L.RBLargest(t)
+            while p and addr(p) != nil:
+                seq.append(p.contents.info)
+                p = L.TreePredecessor(t, p)
+            return seq[::-1]
+        r = {}
+        nodes = {}
+        for i in range(nkeys):
+            nodes[i + 1] = L.RBTreeInsert(t, base + 8 * i, i + 1)
+            assert nodes[i + 1].contents.info == i + 1
+        r["shape0"] = shape(T.root.contents.left, [])
+        order = inorder()
+        r["order0"] = order
+        # the in-order sequence sorts the keys stably (ties in insertion order)
+        assert order == [int(i) + 1 for i in np.argsort(np.round(vals).astype(int), kind="stable")]
+        succ, p = [], T.root.contents.left
+        while p.contents.left and addr(p.contents.left) != nil:
+            p = p.contents.left
+        while addr(p) != nil:
+            succ.append(p.contents.info)
+            p = L.TreeSuccessor(t, p)
+        assert succ == order
+        # deletions by position in key order; survivors keep their nodes
+        for step in range(150):
+            cur = inorder()
+            victim = cur[int(picks[step] * len(cur))]
+            L.RBDelete(t, nodes.pop(victim))
+            if step % 25 == 0:
+                r[f"shape_del{step}"] = shape(T.root.contents.left, [])
+        for info, nd in nodes.items():
+            assert nd.contents.info == info
+        r["count"] = L.RBTreeCount(t)
+        r["order1"] = inorder()
+        q = C.c_double(33.0)
+        hit = L.RBExactQuery(t, C.addressof(q))
+        r["query"] = hit.contents.info if hit else None
+        q2 = C.c_double(1000.0)
+        r["miss"] = bool(L.RBExactQuery(t, C.addressof(q2)))
+        lo, hi = C.c_double(10.0), C.c_double(20.0)
+        s = L.RBEnumerate(t, C.addressof(lo), C.addressof(hi))
+        e, nd = [], s.contents.top
+        while nd:                                  # the stack holds the nodes
+            e.append(Node.from_address(nd.contents.info).info)
+            nd = nd.contents.next
+        r["enum"] = e
+        L.StackDestroy(s, DEST(lambda p: None))
+        L.RBTreePrint(t)
+        # reinsert, delete the largest repeatedly (FEM2D.c:6291-6318)
+        for i in range(40):
+            nodes[1000 + i] = L.RBTreeInsert(t, base + 8 * i, 1000 + i)
+        for _ in range(60):
+            L.RBDelete(t, L.RBLargest(t))
+        r["shape_last"] = shape(T.root.contents.left, [])
+        L.RBTreeDestroy(t)
+        r["events"] = events
+        e2 = L.RBTreeCreate(cb_cmp, cb_dk, cb_di, cb_pk, cb_pi)
+        r["empty_largest"] = bool(L.RBLargest(e2))
+        r["empty_count"] = L.RBTreeCount(e2)
+        L.RBTreeDestroy(e2)
+        results.append(r)
+    for key in results[1]:
+        assert results[0][key] == results[1][key], key
+    assert results[0]["enum"] and results[0]["query"] is not None and not results[0]["miss"]
+    ek = [round(vals[i - 1]) for i in results[0]["enum"]]
+    assert ek == sorted(ek) and 10 <= ek[0] and ek[-1] <= 20          # smallest on top, within [low, high]
+    assert round(vals[results[0]["query"] - 1]) == 33
+
+
+def test_stack_matches_reference(libs):
+    import ctypes as C
+    Node, Tree, SNode, Stack, CMP, DEST, PK = _rb_types()
+    SP = C.POINTER(Stack)
+    out = []
+    for lib in libs:
+        L = lib.L
+        L.StackCreate.restype = SP
+        L.StackPush.argtypes = [SP, C.c_void_p]
+        L.StackPop.argtypes = [SP]
+        L.StackPop.restype = C.c_void_p
+        L.StackNotEmpty.argtypes = [SP]
+        L.StackJoin.argtypes = [SP, SP]
+        L.StackJoin.restype = SP
+        L.StackDestroy.argtypes = [SP, DEST]
+        seen = []
+        a, b = L.StackCreate(), L.StackCreate()
+        r = [bool(L.StackNotEmpty(a)), L.StackPop(a)]
+        for i in range(1, 6):
+            L.StackPush(a, i)
+            L.StackPush(b, 10 * i)
+        r += [bool(L.StackNotEmpty(a)), L.StackPop(a), L.StackPop(a)]
+        j = L.StackJoin(a, b)
+        nd, walk = j.contents.top, []
+        while nd:
+            walk.append(nd.contents.info)
+            nd = nd.contents.next
+        r += [walk, j.contents.tail.contents.info]
+        e = L.StackCreate()
+        j2 = L.StackJoin(e, j)                     # an empty first stack hands back the second
+        r.append(C.addressof(j2.contents) == C.addressof(j.contents))
+        L.StackDestroy(j2, DEST(lambda p: seen.append(p)))
+        r.append(seen)
+        out.append(r)
+    assert out[0] == out[1]
+    assert out[0][5] == [3, 2, 1, 50, 40, 30, 20, 10]
+
+
+def test_file_helpers_match_reference(libs, tmp_path):
+    """File_stuff.c as crack.c/FEM.c use it: split, readline/readLine,
+    open_file/Open_file, append_data, write_lists/read_lists."""
+    import ctypes as C
+    txt = "alpha  beta\tgamma\n\nlast-line-no-newline"
+    src = tmp_path / "in.txt"
+    src.write_text("a = 1\nbb\tcc  dd\n\n  x  \n" + "z" * 300 + "\ntail")
+    lists = [np.arange(7, dtype=np.int32) * k - 3 for k in (1, 2, 5)]
+    data = np.linspace(-1, 1, 12)
+    out = []
+    for n, lib in enumerate(libs):
+        L = lib.L
+        libc = C.CDLL(None)
+        libc.fopen.restype = C.c_void_p
+        libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+        libc.fclose.argtypes = [C.c_void_p]
+        L.split.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_int)]
+        L.split.restype = C.POINTER(C.c_char_p)
+        r = {}
+        for s, d in ((txt, " \t\n"), ("1,2,,3,", ","), ("", " "), (",,,", ",")):
+            cnt = C.c_int(-1)
+            p = L.split(s.encode(), d.encode(), C.byref(cnt))
+            r[(s, d)] = [p[i] for i in range(cnt.value)] if cnt.value else (cnt.value, bool(p))
+        L.readline.argtypes = [C.c_void_p, C.c_char_p]
+        L.readLine.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        f = libc.fopen(str(src).encode(), b"r")
+        buf = C.create_string_buffer(1024)
+        lines = []
+        for _ in range(7):
+            rc = L.readLine(f, buf, 1024)
+            lines.append((rc, buf.value))
+        libc.fclose(f)
+        f = libc.fopen(str(src).encode(), b"r")
+        for _ in range(4):
+            rc = L.readline(f, buf)
+            lines.append((rc, buf.value))
+        libc.fclose(f)
+        r["lines"] = lines
+        L.open_file.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p]
+        L.open_file.restype = C.c_void_p
+        L.Open_file.argtypes = [C.c_char_p, C.c_char_p]
+        L.Open_file.restype = C.c_void_p
+        r["missing"] = (L.open_file(str(tmp_path).encode(), b"nope.txt", b"r"), L.Open_file(b"/nonexistent/x", b"r"))
+        fo = L.open_file(str(tmp_path).encode(), f"data{n}.txt".encode(), b"w")
+        L.append_data.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_int, C.c_double]
+        L.append_data(fo, dptr(data), 4, 3, C.c_double(0.25))
+        L.append_data(fo, dptr(data), 6, 2, C.c_double(1.5))
+        libc.fclose(fo)
+        r["data"] = (tmp_path / f"data{n}.txt").read_text()
+        fo = L.Open_file(str(tmp_path / f"abs{n}.txt").encode(), b"w")
+        fo2 = L.open_file(b"/ignored", str(tmp_path / f"abs2{n}.txt").encode(), b"w")
+        r["opened"] = (bool(fo), bool(fo2))
+        libc.fclose(fo)
+        libc.fclose(fo2)
+        IP = C.POINTER(C.c_int)
+        arr = (IP * 3)(*[l.ctypes.data_as(IP) for l in lists])
+        L.write_lists.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(IP), C.c_int, C.c_int]
+        L.write_lists(str(tmp_path).encode(), f"lists{n}".encode(), arr, 3, 7)
+        r["lists_file"] = (tmp_path / f"lists{n}").read_text()
+        L.read_lists.argtypes = [C.c_char_p, C.c_char_p, IP, IP]
+        L.read_lists.restype = C.POINTER(IP)
+        num, size = C.c_int(0), C.c_int(0)
+        R = L.read_lists(str(tmp_path).encode(), f"lists{n}".encode(), C.byref(num), C.byref(size))
+        r["read"] = (num.value, size.value, [[R[i][j] for j in range(size.value)] for i in range(num.value)])
+        out.append(r)
+    assert out[0] == out[1]
+    assert out[0]["read"][2] == [list(l) for l in lists]
+
+
+def test_subspace_minimum_and_misc_match_reference(libs):
+    """subspace_minimum (reflective_newton.c:394-450) on the trust circle of
+    a definite and an indefinite matrix, against the reference and against
+    a dense scan of q over the circle; get_all_indices, min, max."""
+    import ctypes as C
+    from tests.dropin_bind import SparseMatrix
+    SM = C.POINTER(SparseMatrix)
+    rng = np.random.default_rng(5)
+    n = 12
+    M = rng.normal(size=(n, n))
+    cases = [(M @ M.T + n * np.eye(n), 0.7), (M + M.T, 2.5)]
+    for A, delta in cases:
+        rp = np.arange(0, n * n + 1, n, dtype=np.int32)
+        ci = np.tile(np.arange(n, dtype=np.int32), n)
+        b = rng.normal(size=n)
+        V1 = rng.normal(size=n)
+        V1 /= np.linalg.norm(V1)
+        V2 = rng.normal(size=n)
+        V2 -= V1 * (V1 @ V2)
+        V2 /= np.linalg.norm(V2)
+        xs = []
+        for lib in libs:
+            L = lib.L
+            Ap = lib.from_csr(rp, ci, A.ravel())
+            L.RN_set_trust_region.argtypes = [C.c_double]
+            L.RN_set_trust_region(delta)
+            L.subspace_minimum.argtypes = [SM] + [C.POINTER(C.c_double)] * 4
+            X = np.zeros(n)
+            L.subspace_minimum(Ap, dptr(b), dptr(V1), dptr(V2), dptr(X))
+            xs.append(X)
+            lib.free(Ap)
+        assert np.allclose(xs[0], xs[1], rtol=0, atol=1e-12 * delta), (xs[0] - xs[1])
+        th = np.linspace(0, 2 * np.pi, 200001)
+        P = delta * (np.outer(np.cos(th), V1) + np.outer(np.sin(th), V2))
+        q = np.einsum("ij,jk,ik->i", P, A, P) + P @ b
+        X = xs[0]
+        assert abs(np.linalg.norm(X) - delta) < 1e-12 * delta
+        assert X @ A @ X + X @ b <= q.min() + 1e-9 * (1 + abs(q.min()))
+    for lib in libs:
+        L = lib.L
+        L.get_all_indices.restype = C.POINTER(C.c_int)
+        p = L.get_all_indices(9)
+        assert [p[i] for i in range(9)] == list(range(9))
+        assert (L.min(3, -2), L.max(3, -2), L.max(4, 4), L.min(4, 4)) == (-2, 3, 4, 4)
+
+
+def test_link_every_caller_symbol_against_both_libraries(tmp_path):
+    """tests/dropin_link.c takes the address of every LinA function the Crack
+    and FEM2D sources call and of every function of LinA's solver headers:
+    it compiles against the drop-in's headers and links against
+    liblinalg_hip.so with no undefined symbol, and compiles against LinA's
+    own headers and links against the library built from LinA's sources."""
+    src = os.path.join(ROOT, "tests", "dropin_link.c")
+    n_entries = open(src).read().count("(entry)&")
+    exe = tmp_path / "ours"
+    r = subprocess.run(["gcc", "-std=gnu99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include", "linalg"),
+                        "-I", os.path.join(ROOT, "include"), src, "-o", str(exe), "-L", os.path.dirname(DROPIN_SO),
+                        "-llinalg_hip", "-Wl,-rpath," + os.path.dirname(DROPIN_SO), "-Wl,--no-undefined"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and int(out.stdout) == n_entries > 150
+    ref_inc = "/root/reference/LinA/include"
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libreflina.so")
+    if not (os.path.isdir(ref_inc) and os.path.exists(ref_so)):
+        pytest.skip("LinA sources not present: the reference half of the link check needs them")
+    exe = tmp_path / "ref"
+    r = subprocess.run(["gcc", "-std=gnu89", "-w", "-fopenmp", "-DLINA_REFERENCE_HEADERS", "-idirafter",
+                        "/opt/conda/include", "-I", ref_inc, src, "-o", str(exe), ref_so,
+                        "-Wl,-rpath," + os.path.dirname(ref_so)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and int(out.stdout) == n_entries
+
+
+def test_slimer_utilities_match_reference(libs):
+    """The vector/matrix utilities FEM2D/slimer.c calls (linear_algebra.c
+    :1881-1902, 2413-2429, 2836-2862, 3391-3396, 4312-4339), against the
+    reference."""
+    import ctypes as C
+    from tests.dropin_bind import SparseMatrix
+    SM = C.POINTER(SparseMatrix)
+    DP = C.POINTER(C.c_double)
+    IP = C.POINTER(C.c_int)
+    rng = np.random.default_rng(4)
+    s = _system()
+    perm = rng.permutation(s.n).astype(np.int32)
+    x = rng.normal(size=40)
+    p40 = rng.permutation(40).astype(np.int32)
+    out = []
+    for lib in libs:
+        L = lib.L
+        r = {}
+        A = lib.from_csr(s.rowptr, s.col, s.val)
+        L.get_diag_dominance.argtypes = [SM]
+        L.get_diag_dominance.restype = C.c_double
+        r["dd"] = L.get_diag_dominance(A)
+        L.sparse_row_permutation.argtypes = [SM, IP]
+        L.sparse_row_permutation(A, perm.ctypes.data_as(IP))
+        r["perm"] = LinA.to_lists(A)
+        r["dd2"] = L.get_diag_dominance(A)
+        L.Lap_2D_Dirichlet.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.Lap_2D_Dirichlet.restype = SM
+        r["lap"] = LinA.to_lists(L.Lap_2D_Dirichlet(0.7, 1.3, 6))
+        v = x.copy()
+        L.vector_permutation.argtypes = [DP, IP, C.c_int]
+        L.vector_permutation(dptr(v), p40.ctypes.data_as(IP), 40)
+        r["vp"] = v
+        w = np.zeros(50)
+        L.insert_vector.argtypes = [DP, DP, C.c_int, C.c_int]
+        L.insert_vector(dptr(w), dptr(x), 7, 30)
+        r["iv"] = w
+        y = x.copy()
+        L.test_negative.argtypes = [DP, C.c_int]
+        L.remove_negative.argtypes = [DP, C.c_int]
+        L.remove_negative.restype = None
+        r["neg"] = (L.test_negative(dptr(y), 40), L.remove_negative(dptr(y), 40), L.test_negative(dptr(y), 40))
+        r["rn"] = y
+        z = x.copy()
+        L.test_vector.argtypes = [DP, C.c_int, C.c_char_p]
+        r["nan0"] = L.test_vector(dptr(z), 40, b"no nan")
+        z[13] = np.nan
+        r["nan1"] = L.test_vector(dptr(z), 40, b"nan found")
+        lib.free(A)
+        out.append(r)
+    for key in out[1]:
+        a, b = out[0][key], out[1][key]
+        assert np.array_equal(a, b) if isinstance(b, np.ndarray) else a == b, key
+    assert out[0]["nan1"] == 1 and out[0]["nan0"] == 0
+    assert np.array_equal(out[0]["vp"], x[p40]) and np.array_equal(out[0]["iv"][7:37], x[:30])
