@@ -79,6 +79,47 @@ def cpu_baseline(s, amg, gpu_iters: int) -> dict:
             "per_iteration_s": {str(k): v for k, v in per.items()}, "assemble_s": t_asm}
 
 
+def spmv_alg_bytes(nnz: int, rows: int, cols: int, prec: str) -> int:
+    """Algorithmic bytes of one CSR SpMV: values + int32 columns + row
+    pointers + x gathered once + y written once (FP64: 12 nnz + 4 (rows+1) +
+    8 cols + 8 rows; FP32 values and vectors: 8 nnz + 4 (rows+1) + 4 cols +
+    4 rows).  The library stores 16-bit column codes where they fit, so it
+    moves fewer bytes than this."""
+    if prec == "fp64":
+        return 12 * nnz + 4 * (rows + 1) + 8 * cols + 8 * rows
+    return 8 * nnz + 4 * (rows + 1) + 4 * cols + 4 * rows
+
+
+def iteration_bytes(amg, precision: str, coarse_sweeps: int = 1) -> dict:
+    """Algorithmic bytes of one AMG-PCG iteration from the actual level sizes:
+    the outer FP64 SpMV, two dots (16 n each), the fused x/r update with the
+    residual norm (48 n) and p = z + beta p (24 n); per level above the
+    coarsest its smoothing sweeps (a sweep = SpMV + b and the diagonal
+    scaling read; two on the finest level -- the forward and the reverse
+    multicolour SOR pass -- and 2 x coarse_sweeps l1-Jacobi sweeps below),
+    one residual (SpMV + b), the restriction R and the prolongation P; the
+    coarsest level's dense inverse applied once (its FP64 upper triangle,
+    4 nd^2, + 16 nd).  The hierarchy's arithmetic is FP32 in the mixed mode."""
+    top = amg.depth - 1
+    L = [amg.level_sizes(l) for l in range(amg.depth)]
+    n0, nnz0 = L[top]["fine"], L[top]["nnz_A"]
+    outer = spmv_alg_bytes(nnz0, n0, n0, "fp64") + 2 * 16 * n0 + 48 * n0 + 24 * n0
+    vp = "fp64" if precision == "fp64" else "fp32"
+    vb = 8 if vp == "fp64" else 4
+    per_level = []
+    for l in range(top, 0, -1):
+        d = L[l]
+        n, nc = d["fine"], d["coarse"]
+        sweeps = 2 if l == top else 2 * coarse_sweeps
+        a = spmv_alg_bytes(d["nnz_A"], n, n, vp)
+        per_level.append(int(sweeps * (a + 2 * vb * n) + (a + vb * n) + spmv_alg_bytes(d["nnz_R"], nc, n, vp) +
+                             spmv_alg_bytes(d["nnz_P"], n, nc, vp)))
+    nd = L[0]["fine"]
+    coarse = 4 * nd * nd + 16 * nd
+    return {"outer": int(outer), "levels": per_level, "coarse": int(coarse),
+            "total": int(outer + sum(per_level) + coarse)}
+
+
 def copy_gbs(lina, n=1 << 27, reps=5):
     """hipMemcpy device-to-device of n doubles (read n*8 + write n*8 bytes):
     the attainable streaming rate beside the 8 TB/s peak."""
@@ -494,11 +535,26 @@ def main():
 
     # finest-level SpMV against the HBM roofline: the outer-loop A*p launches
     # of the timed mixed solves, each bracketed by HIP events on the library
-    # stream; beside it the same kernel back to back
+    # stream; beside it the same kernel back to back.  Algorithmic bytes:
+    # 12 nnz + 4 (n+1) + 16 n (what the kernel stores and moves instead,
+    # with 16-bit column codes, is stored_bytes_per_launch)
     lvl = amg.depth - 1
-    spmv_bytes = amg.spmv_bytes(lvl, "fp64")
+    stored_bytes = amg.spmv_bytes(lvl, "fp64")
+    spmv_bytes = spmv_alg_bytes(s.nnz, s.n, s.n, "fp64")
     spmv_ms = res["mixed"]["spmv_ms"]
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    # the whole iteration against the same roofline
+    for name in ("mixed", "fp64"):
+        ib = iteration_bytes(amg, name, args.coarse_sweeps)
+        ms_it = res[name]["device_ms"] / max(res[name]["iters"], 1)
+        res[name]["iteration"] = {"bytes": ib["total"], "ms": ms_it,
+                                  "achieved_gbs": ib["total"] / (ms_it * 1e-3) / 1e9,
+                                  "frac": ib["total"] / (ms_it * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                  "outer_bytes": ib["outer"], "level_bytes": ib["levels"],
+                                  "coarse_bytes": ib["coarse"]}
+        log(f"{name} iteration: {ib['total'] / 1e9:.3f} GB algorithmic, {ms_it:.3f} ms -> "
+            f"{res[name]['iteration']['achieved_gbs']:.0f} GB/s ({res[name]['iteration']['frac']:.2f} of peak); "
+            f"levels {[round(b / 1e6, 1) for b in ib['levels']]} MB, coarse {ib['coarse'] / 1e6:.1f} MB")
     b2b_ms = amg.time_spmv(lvl, "fp64", reps=100)
     spmv32_ms = amg.time_spmv(lvl, "mixed", reps=100)
     spmv32_gbs = amg.spmv_bytes(lvl, "mixed") / (spmv32_ms * 1e-3) / 1e9
@@ -543,9 +599,13 @@ def main():
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": f"k_spmv<double, double, 3, {lpr64}> (finest-level A*p of the outer CG)",
-                     "bytes_per_launch": spmv_bytes, "avg_ms": spmv_ms, "launches": res["mixed"]["spmv_launches"],
+                     "bytes_per_launch": spmv_bytes, "stored_bytes_per_launch": stored_bytes, "avg_ms": spmv_ms,
+                     "launches": res["mixed"]["spmv_launches"],
                      "back_to_back_ms": b2b_ms, "back_to_back_gbs": spmv_bytes / (b2b_ms * 1e-3) / 1e9,
                      "copy_gbs": cgbs},
+        "iteration_roofline": {"mixed": res["mixed"]["iteration"]["frac"], "fp64": res["fp64"]["iteration"]["frac"],
+                               "bytes": "outer SpMV + 2 dots + fused updates + per level (sweeps, residual, R, P) "
+                                        "+ coarse inverse, from the actual level sizes"},
         "spmv_fp32": {"achieved": spmv32_gbs, "avg_ms": spmv32_ms, "kernel": "k_spmv<float, float, 3, 1>"},
         "cpu_baseline": cpu,
     }

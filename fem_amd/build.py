@@ -22,10 +22,11 @@ LIB = os.path.join(HERE, "liblina_hip.so")
 HOST = os.path.join(HERE, "host")
 DROPIN_LIB = os.path.join(HERE, "liblinalg_hip.so")
 DROPIN_SOURCES = ["linalg.c", "active_set.c", "pcgs.c", "multigrid.c", "bicgstab.c", "reflective_newton.c", "sparse_io.c",
-                  "eigen.c", "gmres.c", "direct.c", "sparse_ops.c", "sweeps.c", "lists.c", "sparse3d.c", "rbtree.c", "file_stuff.c"]
+                  "eigen.c", "gmres.c", "direct.c", "sparse_ops.c", "sweeps.c", "lists.c", "sparse3d.c", "rbtree.c", "file_stuff.c",
+                  "spd_solver.c", "checkpoint.c"]
 
 SOURCES = ["amg_setup.cpp", "solver.cpp", "capi.cpp", "kernels.hip", "krylov.cpp", "rn.cpp", "batch.cpp",
-           "noalloc.hip", "rap.cpp"]
+           "noalloc.hip", "rap.cpp", "assembly.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LINA_ARCH", "gfx950")

@@ -234,6 +234,14 @@ int lina_amg_recompute(lina_solver* s, const double* val) {
     });
 }
 
+int lina_amg_recompute_device(lina_solver* s, const double* d_val) {
+    return guard([&]() {
+        if (!s || !d_val) throw std::invalid_argument("null argument");
+        s->s.recompute_device_values(d_val);
+        return 0;
+    });
+}
+
 int lina_amg_recompute_csr(lina_solver* s, int n, const int* rowptr, const int* col, const double* val) {
     return guard([&]() {
         if (!s) throw std::invalid_argument("null solver");

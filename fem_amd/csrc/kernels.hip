@@ -24,6 +24,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include <algorithm>
 #include <cstdint>
 #include <cstdlib>
@@ -1222,6 +1224,42 @@ __global__ void k_plan_sum(int ne, const int64_t* __restrict__ ptr, const int* _
     }
 }
 
+// symmetric Dirichlet treatment keeping the pattern, one row per thread:
+// a constrained row becomes the unit row with b[r] = u[r]; any other row
+// subtracts sum over its constrained columns c (column order, from +0.0) of
+// a_rc u_c from b[r] and zeroes those entries
+__global__ void k_dirichlet_rows(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                 const uint8_t* __restrict__ dir, const double* __restrict__ u,
+                                 double* __restrict__ val, double* __restrict__ b) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        const int p0 = rp[r], p1 = rp[r + 1];
+        if (dir[r]) {
+            for (int p = p0; p < p1; p++) val[p] = ci[p] == r ? 1.0 : 0.0;
+            if (b) b[r] = u[r];
+            continue;
+        }
+        double s = 0.0;
+        bool any = false;
+        for (int p = p0; p < p1; p++) {
+            const int c = ci[p];
+            if (dir[c]) {
+                s += val[p] * u[c];
+                val[p] = 0.0;
+                any = true;
+            }
+        }
+        if (b && any) b[r] = b[r] - s;
+    }
+}
+
+// v = -v over the listed rows (the setup's diagonal signs)
+__global__ void k_negate_rows(int m, const int* __restrict__ rows, const int* __restrict__ rp, double* __restrict__ v) {
+    for (int k = blockIdx.x; k < m; k += gridDim.x) {
+        const int r = rows[k];
+        for (int p = rp[r] + threadIdx.x; p < rp[r + 1]; p += blockDim.x) v[p] = -v[p];
+    }
+}
+
 __global__ void k_plan_scatter(int ne, const double* __restrict__ val, const int* __restrict__ pos,
                                const uint8_t* __restrict__ lone, double* __restrict__ dst, int* __restrict__ flag) {
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
@@ -1698,6 +1736,132 @@ __global__ __launch_bounds__(TPB) void k_gj_pivot(int N, int k, double* __restri
     }
 }
 
+// ------------------------------- blocked Gauss-Jordan (FP64 MFMA update)
+//
+// The same inversion by blocks of GJB = 64 pivots (no pivoting).  For the
+// block K = [k0, k0 + nk) and the others I, J:
+//   D = M[K,K]^-1                        k_bgj_diag (one workgroup, in LDS)
+//   M[K,J] = D M[K,J]                    k_bgj_panels (also copied to Brow)
+//   Acol = M[I,K]; M[I,K] = -M[I,K] D    k_bgj_panels
+//   M[I,J] -= Acol M[K,J]                k_bgj_gemm (64 x 64 tiles, FP64 MFMA)
+//   M[K,K] = D
+// With GJB = 1 these are the scalar steps above.
+constexpr int GJB = 64;
+
+__global__ __launch_bounds__(1024) void k_bgj_diag(int N, int k0, int nk, double* __restrict__ M,
+                                                   double* __restrict__ Dinv) {
+    __shared__ double D[GJB][GJB + 1];
+    __shared__ double piv;
+    const int tid = threadIdx.x;
+    for (int t = tid; t < GJB * GJB; t += blockDim.x) {
+        const int i = t / GJB, j = t % GJB;
+        D[i][j] = (i < nk && j < nk) ? M[(size_t)(k0 + i) * N + k0 + j] : 0.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < nk; k++) {
+        if (tid == 0) piv = 1.0 / D[k][k];
+        __syncthreads();
+        const double p = piv;
+        for (int t = tid; t < nk * nk; t += blockDim.x) {
+            const int i = t / nk, j = t % nk;
+            if (i != k && j != k) D[i][j] -= D[i][k] * p * D[k][j];
+        }
+        __syncthreads();
+        for (int t = tid; t < nk; t += blockDim.x) {
+            if (t == k) continue;
+            D[k][t] *= p;
+            D[t][k] *= -p;
+        }
+        __syncthreads();
+        if (tid == 0) D[k][k] = p;
+        __syncthreads();
+    }
+    for (int t = tid; t < GJB * GJB; t += blockDim.x) {
+        const int i = t / GJB, j = t % GJB;
+        Dinv[t] = D[i][j];  // zero outside nk x nk
+        if (i < nk && j < nk) M[(size_t)(k0 + i) * N + k0 + j] = D[i][j];
+    }
+}
+
+// one thread per index t outside the block (64 threads a workgroup): the
+// new row panel's column t (also into Brow, 64 x N, rows beyond nk zero) and
+// the new column panel's row t (the old one into Acol, N x 64, columns
+// beyond nk zero); each thread's 64 inputs are staged in LDS
+__global__ __launch_bounds__(64) void k_bgj_panels(int N, int k0, int nk, double* __restrict__ M,
+                                                   const double* __restrict__ Dinv, double* __restrict__ Acol,
+                                                   double* __restrict__ Brow) {
+    __shared__ double D[GJB * GJB];
+    __shared__ double V[GJB][GJB + 1];  // [thread][m]
+    for (int t = threadIdx.x; t < GJB * GJB; t += blockDim.x) D[t] = Dinv[t];
+    __syncthreads();
+    const int x = threadIdx.x;
+    for (int base = blockIdx.x * blockDim.x; base < N; base += gridDim.x * blockDim.x) {
+        const int t = base + x;
+        const bool live = t < N && !(t >= k0 && t < k0 + nk);
+        // row panel: column t of M[K, :]
+        for (int m = 0; m < GJB; m++) V[x][m] = (live && m < nk) ? M[(size_t)(k0 + m) * N + t] : 0.0;
+        for (int r = 0; r < GJB; r++) {
+            double s = 0.0;
+            for (int m = 0; m < nk; m++) s += D[r * GJB + m] * V[x][m];
+            if (t < N) Brow[(size_t)r * N + t] = (live && r < nk) ? s : 0.0;
+            if (live && r < nk) M[(size_t)(k0 + r) * N + t] = s;
+        }
+        // column panel: row t of M[:, K]
+        for (int m = 0; m < GJB; m++) V[x][m] = (live && m < nk) ? M[(size_t)t * N + k0 + m] : 0.0;
+        if (t < N)
+            for (int m = 0; m < GJB; m++) Acol[(size_t)t * GJB + m] = V[x][m];
+        if (live)
+            for (int c = 0; c < nk; c++) {
+                double s = 0.0;
+                for (int m = 0; m < nk; m++) s += V[x][m] * D[m * GJB + c];
+                M[(size_t)t * N + k0 + c] = -s;
+            }
+    }
+}
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// M[I,J] -= Acol[I,:] Brow[:,J] over the 64 x 64 tiles outside block row and
+// column kb.  4 waves a tile, each a 32 x 32 quadrant of 2 x 2 MFMA blocks
+// (v_mfma_f64_16x16x4f64: lane l holds A[l%16][l/16], B[l/16][l%16] and
+// D[4 (l/16) + v][l%16], v = 0..3).
+__global__ __launch_bounds__(256) void k_bgj_gemm(int N, int kb, int nt, double* __restrict__ M,
+                                                  const double* __restrict__ Acol, const double* __restrict__ Brow) {
+    __shared__ double As[GJB][GJB + 1];  // [row in tile][k]
+    __shared__ double Bs[GJB][GJB + 1];  // [k][column in tile]
+    const int tile = blockIdx.x;
+    int ti = tile / (nt - 1), tj = tile % (nt - 1);
+    if (ti >= kb) ti++;  // skip the pivot block's row and column of tiles
+    if (tj >= kb) tj++;
+    const int i0 = ti * GJB, j0 = tj * GJB;
+    const int tid = threadIdx.x;
+    for (int t = tid; t < GJB * GJB; t += 256) {
+        const int r = t / GJB, c = t % GJB;
+        As[r][c] = (i0 + r < N) ? Acol[(size_t)(i0 + r) * GJB + c] : 0.0;
+        Bs[r][c] = (j0 + c < N) ? Brow[(size_t)r * N + j0 + c] : 0.0;
+    }
+    __syncthreads();
+    const int w = tid / 64, l = tid % 64;
+    const int qi = (w / 2) * 32, qj = (w % 2) * 32;
+    f64x4 acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = (f64x4){0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < GJB; k += 4) {
+        double av[2], bv[2];
+        for (int a = 0; a < 2; a++) av[a] = As[qi + a * 16 + l % 16][k + l / 16];
+        for (int b = 0; b < 2; b++) bv[b] = Bs[k + l / 16][qj + b * 16 + l % 16];
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int v = 0; v < 4; v++) {
+                const int i = i0 + qi + a * 16 + 4 * (l / 16) + v;
+                const int j = j0 + qj + b * 16 + l % 16;
+                if (i < N && j < N) M[(size_t)i * N + j] -= acc[a][b][v];
+            }
+}
+
 // ---------------------------------------------------------- launchers
 
 static inline int grid_for(int n, int cap = 4096) {
@@ -2129,13 +2293,32 @@ void batched_gauss_jordan_inverse(hipStream_t st, int nd, int max_block, const i
     }
 }
 
-void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* piv) {
+size_t gauss_jordan_work(int N) { return (size_t)GJB * GJB + 2 * (size_t)GJB * (size_t)std::max(N, 1); }
+
+void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* work) {
     if (N <= 0) return;
-    int g = std::min(N, 4096);
-    int gp = (N + TPB - 1) / TPB;
-    for (int k = 0; k < N; k++) {
-        hipLaunchKernelGGL(k_gj_update, dim3(g), dim3(TPB), 0, st, N, k, M, piv);
-        hipLaunchKernelGGL(k_gj_pivot, dim3(gp), dim3(TPB), 0, st, N, k, M, (const double*)piv);
+    static const char* e = getenv("LINA_GJ");
+    if (e && std::strcmp(e, "scalar") == 0) {  // the unblocked steps, for comparisons
+        int g = std::min(N, 4096);
+        int gp = (N + TPB - 1) / TPB;
+        for (int k = 0; k < N; k++) {
+            hipLaunchKernelGGL(k_gj_update, dim3(g), dim3(TPB), 0, st, N, k, M, work);
+            hipLaunchKernelGGL(k_gj_pivot, dim3(gp), dim3(TPB), 0, st, N, k, M, (const double*)work);
+        }
+        return;
+    }
+    double* Dinv = work;
+    double* Acol = work + GJB * GJB;
+    double* Brow = Acol + (size_t)GJB * N;
+    const int nt = (N + GJB - 1) / GJB;
+    const int gp = std::min((N + 63) / 64, 4096);
+    for (int kb = 0; kb < nt; kb++) {
+        const int k0 = kb * GJB, nk = std::min(GJB, N - k0);
+        hipLaunchKernelGGL(k_bgj_diag, dim3(1), dim3(1024), 0, st, N, k0, nk, M, Dinv);
+        hipLaunchKernelGGL(k_bgj_panels, dim3(gp), dim3(64), 0, st, N, k0, nk, M, (const double*)Dinv, Acol, Brow);
+        if (nt > 1)
+            hipLaunchKernelGGL(k_bgj_gemm, dim3((nt - 1) * (nt - 1)), dim3(256), 0, st, N, kb, nt, M,
+                               (const double*)Acol, (const double*)Brow);
     }
 }
 
@@ -2147,6 +2330,15 @@ void plan_sum(hipStream_t st, int ne, const int64_t* ptr, const int* ia, const i
 void plan_scatter(hipStream_t st, int ne, const double* val, const int* pos, const uint8_t* lone, double* dst,
                   int* flag) {
     if (ne > 0) hipLaunchKernelGGL(k_plan_scatter, dim3(grid_for(ne)), dim3(TPB), 0, st, ne, val, pos, lone, dst, flag);
+}
+
+void dirichlet_rows(hipStream_t st, int n, const int* rp, const int* ci, const uint8_t* dir, const double* u,
+                    double* val, double* b) {
+    if (n > 0) hipLaunchKernelGGL(k_dirichlet_rows, dim3(grid_for(n)), dim3(TPB), 0, st, n, rp, ci, dir, u, val, b);
+}
+
+void negate_rows(hipStream_t st, int m, const int* rows, const int* rp, double* v) {
+    if (m > 0) hipLaunchKernelGGL(k_negate_rows, dim3(std::min(m, 4096)), dim3(64), 0, st, m, rows, rp, v);
 }
 
 template <typename VT>

@@ -195,7 +195,10 @@ void jacobi(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, const XT* 
 // the first sweep from x = 0, elementwise (no matrix row is read)
 template <typename VT, typename XT>
 void jacobi0(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, XT* x, int nlim);
-void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* piv);
+// in-place inverse of the N x N row-major matrix M (no pivoting), blocked
+// by 64 pivots with an FP64 MFMA update; work: gauss_jordan_work(N) doubles
+size_t gauss_jordan_work(int N);
+void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* work);
 // block-diagonal dense inverse of a batch (block of dense row g: rblk[g];
 // block b: rows [off[b], off[b+1]), matrix at M + moff[b]); piv: one per block
 void batched_gauss_jordan_inverse(hipStream_t st, int nd, int max_block, const int* rblk, const int* off,
@@ -330,6 +333,15 @@ void plan_sum(hipStream_t st, int ne, const int64_t* ptr, const int* ia, const i
 // (pos < 0: not in it; lone: a zero there is the pattern's single (i, 0.0))
 void plan_scatter(hipStream_t st, int ne, const double* val, const int* pos, const uint8_t* lone, double* dst,
                   int* flag);
+// symmetric Dirichlet treatment of a CSR system keeping its pattern (dir[i]:
+// DOF i constrained to u[i]): constrained rows become unit rows with
+// b[i] = u[i]; every other row r gets b[r] -= sum over its constrained
+// columns c (column order, from +0.0) of a_rc u_c, and those a_rc = 0.  b
+// may be null (matrix only)
+void dirichlet_rows(hipStream_t st, int n, const int* rp, const int* ci, const uint8_t* dir, const double* u,
+                    double* val, double* b);
+// val = -val on the m listed rows
+void negate_rows(hipStream_t st, int m, const int* rows, const int* rp, double* v);
 // out[r] = sum_p |(VT) v[p]| over row r, in VT arithmetic and column order
 template <typename VT>
 void row_l1(hipStream_t st, int n, const int* rp, const double* v, VT* out);

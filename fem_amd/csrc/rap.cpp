@@ -246,13 +246,82 @@ void Solver::build_rap() {
 
 bool Solver::recompute_device(const Csr& A) {
     if (!rap_) build_rap();
+    DevLevel& top = L_.back();
+    if (A.nnz() != top.A.nnz) return false;
+    hip_check(hipMemcpyAsync(top.A.v, A.v.data(), (size_t)A.nnz() * sizeof(double), hipMemcpyHostToDevice, stream_),
+              "h2d");
+    return recompute_device_core();
+}
+
+void Solver::recompute_device_values(const double* d_val) {
+    if (!uploaded()) throw std::invalid_argument("recompute: the hierarchy is not on the device");
+    DevLevel& top = L_.back();
+    const size_t bytes = (size_t)top.A.nnz * sizeof(double);
+    hipStream_t st = stream_;
+    const bool timing = getenv("LINA_RN_TIMING") != nullptr;  // read per call: recomputes are rare
+    if (device_rap_ok()) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const bool built = rap_ != nullptr;
+        if (!rap_) build_rap();
+        hip_check(hipMemcpyAsync(top.A.v, d_val, bytes, hipMemcpyDeviceToDevice, st), "d2d");
+        if (has_neg_signs_) {
+            if (!neg_rows_) {
+                std::vector<int> rows;
+                for (int i = 0; i < n_; i++)
+                    if (H_.signs[i] < 0) rows.push_back(i);
+                nneg_ = (int)rows.size();
+                neg_rows_ = dupload(rows, *bufs_, st);
+            }
+            dev::negate_rows(st, nneg_, neg_rows_, top.A.rp, top.A.v);
+        }
+        const bool ok = recompute_device_core();
+        if (timing) {
+            const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            fprintf(stderr, "[amg] recompute on the device from device values: %s %.3fs%s\n",
+                    ok ? "done" : "pattern changed", sec, built ? "" : " (plan built)");
+        }
+        if (ok) {
+            host_stale_ = true;
+            host_top_stale_ = true;
+            if (!refresh_isolated()) {
+                sync_host_levels();
+                upload();
+            }
+            return;
+        }
+    }
+    // the host path from the caller's values
+    std::vector<double> v(top.A.nnz);
+    hip_check(hipMemcpyAsync(v.data(), d_val, bytes, hipMemcpyDeviceToHost, st), "d2h");
+    hip_check(hipStreamSynchronize(st), "sync");
+    recompute(v);
+}
+
+// the isolated rows' finest entries from the device into the host copy
+void Solver::fetch_isolated_rows() {
+    const Csr& A = H_.lv.back().A;
+    if (!iso_pos_) {
+        std::vector<int> pos;
+        for (int i : H_.iso_rows)
+            for (int p = A.rp[i]; p < A.rp[i + 1]; p++) pos.push_back(p);
+        niso_pos_ = (int)pos.size();
+        iso_pos_ = dupload(pos, *bufs_, stream_);
+        iso_val_ = dalloc<double>(pos.size(), *bufs_);
+        iso_pos_host_ = std::move(pos);
+    }
+    std::vector<double> v(niso_pos_);
+    dev::gather<double>(stream_, niso_pos_, iso_pos_, L_.back().A.v, iso_val_);
+    hip_check(hipMemcpyAsync(v.data(), iso_val_, (size_t)niso_pos_ * sizeof(double), hipMemcpyDeviceToHost, stream_),
+              "d2h");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    Csr& Am = H_.lv.back().A;
+    for (int k = 0; k < niso_pos_; k++) Am.v[iso_pos_host_[k]] = v[k];
+}
+
+bool Solver::recompute_device_core() {
     DeviceRap& R = *rap_;
     hipStream_t st = stream_;
     const int depth = (int)L_.size();
-    DevLevel& top = L_.back();
-    if (A.nnz() != top.A.nnz) return false;
-    hip_check(hipMemcpyAsync(top.A.v, A.v.data(), (size_t)A.nnz() * sizeof(double), hipMemcpyHostToDevice, st),
-              "h2d");
     dev::fill<int>(st, 1, R.flag, 0);
     for (int l = depth - 1; l >= 1; l--) {
         const DeviceRap::Lv& d = R.lv[l];
@@ -289,6 +358,13 @@ bool Solver::recompute_device(const Csr& A) {
 // the device's coarse values back into the host hierarchy (and the coarse
 // dense system rebuilt from them)
 void Solver::sync_host_levels() {
+    if (host_top_stale_) {
+        Csr& A = H_.lv.back().A;
+        hip_check(hipMemcpyAsync(A.v.data(), L_.back().A.v, (size_t)A.nnz() * sizeof(double), hipMemcpyDeviceToHost,
+                                 stream_), "d2h");
+        hip_check(hipStreamSynchronize(stream_), "sync");
+        host_top_stale_ = false;
+    }
     if (!host_stale_) return;
     const int depth = (int)H_.lv.size();
     for (int l = 0; l < depth - 1; l++) {

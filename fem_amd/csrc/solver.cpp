@@ -432,8 +432,15 @@ void Solver::upload() {
     // a re-upload (recompute) frees every earlier buffer, including the
     // BiCGStab work vectors allocated on first use
     hip_check(hipStreamSynchronize(stream_), "sync");
+    if (bufs_ && (host_stale_ || host_top_stale_)) sync_host_levels();  // values of a device recompute
     rap_.reset();  // the device products' plans belong to the patterns uploaded before
     host_stale_ = false;
+    host_top_stale_ = false;
+    neg_rows_ = nullptr;
+    iso_pos_ = nullptr;
+    iso_val_ = nullptr;
+    niso_pos_ = 0;
+    iso_pos_host_.clear();
     bufs_.reset(new DevBuffers());
     for (double** p : {&bS_, &bPS_, &bPP_, &bAPP_, &bAPS_, &bR_, &bR0_, &bP_}) *p = nullptr;
     DevBuffers& B = *bufs_;
@@ -582,7 +589,7 @@ void Solver::upload() {
         double* piv = dalloc<double>(nb, B);
         dev::batched_gauss_jordan_inverse(st, nd_, maxb, crblk_, coff_, cmoff_, inv_, piv);
     } else {
-        cpiv_ = dalloc<double>(1, B);
+        cpiv_ = dalloc<double>(dev::gauss_jordan_work(nd_), B);
         dev::gauss_jordan_inverse(st, nd_, inv_, cpiv_);
     }
     up_dense_ = lu.dense_idx;
@@ -857,7 +864,7 @@ static std::vector<VT> row_l1(const Csr& A) {
 }
 
 void Solver::refresh_after_recompute() {
-    static const bool timing = getenv("LINA_RN_TIMING") != nullptr;
+    const bool timing = getenv("LINA_RN_TIMING") != nullptr;  // read per call: recomputes are rare
     const auto t0 = std::chrono::steady_clock::now();
     for (size_t l = 1; l < H_.lv.size(); l++) colours_[l] = greedy_colouring(H_.lv[l].A, &ncolours_[l]);
     const auto t1 = std::chrono::steady_clock::now();
@@ -955,7 +962,7 @@ void Solver::recompute(const std::vector<double>& val) {
     if ((int)val.size() != A.nnz()) throw std::invalid_argument("recompute: value count differs from the pattern");
     for (int i = 0; i < A.n; i++)
         for (int p = A.rp[i]; p < A.rp[i + 1]; p++) A.v[p] = val[p] * (H_.signs[i] < 0 ? -1.0 : 1.0);
-    static const bool timing = getenv("LINA_RN_TIMING") != nullptr;
+    const bool timing = getenv("LINA_RN_TIMING") != nullptr;  // read per call: recomputes are rare
     if (device_rap_ok()) {
         const auto t0 = std::chrono::steady_clock::now();
         const bool built = rap_ != nullptr;
@@ -999,6 +1006,7 @@ void Solver::recompute_matrix(const Csr& M) {
 // AMG_fix_isolated (Multigrid.c:1301-1319) divides by the first nonzero of
 // each isolated finest row: new finest values change the divisors
 bool Solver::refresh_isolated() {
+    if (host_top_stale_ && !H_.iso_rows.empty()) fetch_isolated_rows();
     const Csr& A = H_.lv.back().A;
     bool zero = false;
     for (size_t k = 0; k < H_.iso_rows.size(); k++) {

@@ -170,6 +170,10 @@ class Solver {
     const std::vector<Tuned>& tuned() const { return tuned_; }
     void upload();
     void recompute(const std::vector<double>& val);
+    // the same from finest values already on the device (d_val: nnz values
+    // in the setup's pattern, before its diagonal signs); the host copy of
+    // the hierarchy catches up when read (host_synced)
+    void recompute_device_values(const double* d_val);
     // AMG_recompute_coarse_matrices with a new finest matrix of any pattern
     void recompute_matrix(const Csr& M);
     void set_params(const lina_amg_params& p);  // takes effect at the next solve
@@ -239,6 +243,8 @@ class Solver {
     // pattern changes
     bool device_rap_ok() const;
     bool recompute_device(const Csr& A);
+    bool recompute_device_core();
+    void fetch_isolated_rows();
     void build_rap();
     void sync_host_levels();
     // the isolated rows' divisors (AMG_fix_isolated) from the finest values;
@@ -246,6 +252,13 @@ class Solver {
     bool refresh_isolated();
     std::unique_ptr<DeviceRap> rap_;
     bool host_stale_ = false;
+    bool host_top_stale_ = false;  // the finest host values lag the device's
+    int* neg_rows_ = nullptr;      // rows with a negative setup sign (device)
+    int nneg_ = 0;
+    int* iso_pos_ = nullptr;       // entry positions of the isolated rows (device)
+    double* iso_val_ = nullptr;
+    int niso_pos_ = 0;
+    std::vector<int> iso_pos_host_;
     // recompute: colour the new levels and re-upload them (values only when
     // every pattern-derived structure matches what is on the device)
     void refresh_after_recompute();

@@ -141,6 +141,17 @@ def lib():
     L.lina_batch_pcg_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int, _dp, _dp,
                                         C.POINTER(BatchStats)]
     L.lina_batch_tuned.argtypes = [C.c_void_p, _ip, C.c_char_p, _ip, _ip, C.c_int]
+    L.lina_amg_recompute_device.argtypes = [C.c_void_p, C.c_void_p]
+    L.lina_assembly_create.argtypes = [C.c_int, _ip, _ip, _ip, _dp, _ip, _ip]
+    L.lina_assembly_create.restype = C.c_void_p
+    L.lina_assembly_destroy.argtypes = [C.c_void_p]
+    L.lina_assembly_nnz.argtypes = [C.c_void_p]
+    L.lina_assembly_terms.argtypes = [C.c_void_p]
+    L.lina_assembly_terms.restype = C.c_long
+    L.lina_assembly_set_dirichlet.argtypes = [C.c_void_p, C.c_void_p]
+    L.lina_assembly_values.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.lina_dirichlet_symmetric.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]
     _lib = L
     return L
 
@@ -217,6 +228,15 @@ class AMG:
     def depth(self) -> int:
         return lib().lina_amg_depth(self.h)
 
+    def level_sizes(self, l: int) -> dict:
+        """rows of level l and of the next coarser one, and the nonzeros of
+        its A, P and R (P, R: 0 on the coarsest level)"""
+        fine, coarse, na, np_, nr = (C.c_int() for _ in range(5))
+        _check(lib().lina_amg_level_info(self.h, l, C.byref(fine), C.byref(coarse), C.byref(na), C.byref(np_),
+                                         C.byref(nr)))
+        return {"fine": fine.value, "coarse": coarse.value, "nnz_A": na.value, "nnz_P": np_.value,
+                "nnz_R": nr.value}
+
     def level(self, l: int) -> dict:
         fine, coarse, na, np_, nr = (C.c_int() for _ in range(5))
         _check(lib().lina_amg_level_info(self.h, l, C.byref(fine), C.byref(coarse), C.byref(na), C.byref(np_),
@@ -239,6 +259,11 @@ class AMG:
             out["ncolours"] = _check(lib().lina_amg_colours(self.h, l, _i(col)))
             out["colours"] = col
         return out
+
+    def recompute_device(self, d_val: int):
+        """AMG_recompute_coarse_matrices from finest values in device memory
+        (a DeviceVector's ptr)"""
+        _check(lib().lina_amg_recompute_device(self.h, C.c_void_p(d_val)))
 
     def signs(self):
         """the finest rows' signs applied by make_positive_diag (+1 / -1)"""
@@ -628,3 +653,50 @@ class Batch:
         l32 = np.zeros(cap, np.int32)
         m = _check(lib().lina_batch_tuned(self.h, _i(lv), wh, _i(l64), _i(l32), cap))
         return [(int(lv[i]), wh.raw[i:i + 1].decode(), int(l64[i]), int(l32[i])) for i in range(min(m, cap))]
+
+
+class Assembly:
+    """The degraded stiffness's values on the device from a third-order
+    tensor and nodal rank-4 moduli (lina_assembly handle): the contraction of
+    sparse3D_times_rank4_2D_right_use_map, optionally with the symmetric
+    Dirichlet treatment."""
+
+    def __init__(self, n, bptr, bi1, bi2, bval, rowptr, col):
+        self.n = int(n)
+        self._keep = [np.ascontiguousarray(a, dtype=t) for a, t in
+                      ((bptr, np.int32), (bi1, np.int32), (bi2, np.int32), (bval, np.float64),
+                       (rowptr, np.int32), (col, np.int32))]
+        bp, b1, b2, bv, rp, ci = self._keep
+        self.h = lib().lina_assembly_create(self.n, _i(bp), _i(b1), _i(b2), _d(bv), _i(rp), _i(ci))
+        if not self.h:
+            raise LinaError(lib().lina_last_error().decode())
+        self.nnz = _check(lib().lina_assembly_nnz(self.h))
+        self.terms = int(lib().lina_assembly_terms(self.h))
+
+    def set_dirichlet(self, flags):
+        if flags is None:
+            _check(lib().lina_assembly_set_dirichlet(self.h, None))
+            return
+        f = np.ascontiguousarray(flags, dtype=np.uint8)
+        assert f.size == 2 * self.n
+        _check(lib().lina_assembly_set_dirichlet(self.h, f.ctypes.data))
+
+    def values(self, d_C: int, d_val: int, d_udir: int = 0, d_b: int = 0):
+        """device pointers: moduli (16 n), values out (nnz), Dirichlet values
+        (2 n) and right-hand side (2 n, in/out) when constrained DOFs are set"""
+        _check(lib().lina_assembly_values(self.h, C.c_void_p(d_C), C.c_void_p(d_val),
+                                          C.c_void_p(d_udir or None), C.c_void_p(d_b or None)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lina_assembly_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def dirichlet_symmetric(n: int, d_rowptr: int, d_col: int, d_flags: int, d_u: int, d_val: int, d_b: int = 0):
+    """lina_dirichlet_symmetric on device arrays (pointers)"""
+    _check(lib().lina_dirichlet_symmetric(n, C.c_void_p(d_rowptr), C.c_void_p(d_col), C.c_void_p(d_flags),
+                                          C.c_void_p(d_u), C.c_void_p(d_val), C.c_void_p(d_b or None)))

@@ -220,6 +220,67 @@ def apply_dirichlet(A: sp.csr_matrix, b: np.ndarray, dofs: np.ndarray, values: n
     return A2, b
 
 
+def strain_gradient_tensor(mesh: Mesh):
+    """The third-order tensor of Crack's elastic Hessian for P1 elements
+    (Strain_grad, FEM2D.c:6068-6112): row i (node), triples
+    (n b + j, n d + k, sum over the elements e holding i, j and k of
+    grad_b phi_j grad_d phi_k |e| / 3), b, d in {0, 1}.  Contracted with nodal
+    moduli C^i_abcd it gives K_(a j),(c k) = sum_i sum_bd C^i_abcd B^i_(b j),(d k)
+    -- with the same C0 at every node, the P1 elasticity stiffness.
+    Returns (bptr, bi1, bi2, bval): the triples of a row sorted by (bi1, bi2)."""
+    g, area = mesh.grads()
+    t = mesh.tri
+    n = mesh.n_nodes
+    ne = len(t)
+    # per element: (i, j, k) over its 3 x 3 x 3 node triples, (b, d) over 2 x 2
+    I = np.broadcast_to(t[:, :, None, None, None, None], (ne, 3, 3, 3, 2, 2))
+    J = np.broadcast_to(t[:, None, :, None, None, None], (ne, 3, 3, 3, 2, 2))
+    K = np.broadcast_to(t[:, None, None, :, None, None], (ne, 3, 3, 3, 2, 2))
+    Bd = np.broadcast_to(np.arange(2)[None, None, None, None, :, None], (ne, 3, 3, 3, 2, 2))
+    Dd = np.broadcast_to(np.arange(2)[None, None, None, None, None, :], (ne, 3, 3, 3, 2, 2))
+    val = (g[:, None, :, None, :, None] * g[:, None, None, :, None, :] *
+           (area / 3.0)[:, None, None, None, None, None])
+    val = np.broadcast_to(val, (ne, 3, 3, 3, 2, 2))
+    row = I.ravel().astype(np.int64)
+    c1 = (Bd * n + J).ravel().astype(np.int64)
+    c2 = (Dd * n + K).ravel().astype(np.int64)
+    key = (row * (2 * n) + c1) * (2 * n) + c2
+    uk, inv = np.unique(key, return_inverse=True)
+    acc = np.zeros(len(uk))
+    np.add.at(acc, inv, val.ravel())
+    rows = uk // (4 * n * n)
+    rest = uk % (4 * n * n)
+    bi1 = (rest // (2 * n)).astype(np.int32)
+    bi2 = (rest % (2 * n)).astype(np.int32)
+    bptr = np.zeros(n + 1, np.int32)
+    np.add.at(bptr, rows + 1, 1)
+    return np.cumsum(bptr).astype(np.int32), bi1, bi2, acc
+
+
+def nodal_moduli(n: int, scale: np.ndarray | float = 1.0, E: float = 210e3, nu: float = 0.3) -> np.ndarray:
+    """Plane-strain isotropic moduli per node, C^i_abcd = scale_i (lambda d_ab d_cd +
+    mu (d_ac d_bd + d_ad d_bc)), component (a,b,c,d) of node i at
+    n (8a + 4b + 2c + d) + i (Crack's Rank4_modulus layout)."""
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    sc = np.broadcast_to(np.asarray(scale, dtype=np.float64), (n,))
+    C = np.empty(16 * n)
+    for a in range(2):
+        for b in range(2):
+            for c in range(2):
+                for d in range(2):
+                    v = lam * (a == b) * (c == d) + mu * ((a == c) * (b == d) + (a == d) * (b == c))
+                    C[n * (8 * a + 4 * b + 2 * c + d): n * (8 * a + 4 * b + 2 * c + d + 1)] = v * sc
+    return C
+
+
+def elastic_pattern(mesh: Mesh):
+    """(rowptr, col) of the blocked [ux; uy] P1 elasticity matrix"""
+    K = plane_strain_stiffness(mesh, 1.0, 0.3)
+    K.sort_indices()
+    return K.indptr.astype(np.int32), K.indices.astype(np.int32)
+
+
 def _to_system(A: sp.csr_matrix, b: np.ndarray, dof: int, meta: dict) -> System:
     A.sort_indices()
     return System(A.indptr.astype(np.int32), A.indices.astype(np.int32),

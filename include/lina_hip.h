@@ -124,6 +124,14 @@ int lina_amg_colours(const lina_solver* s, int level, int* colours);
 int lina_amg_recompute(lina_solver* s, const double* val);
 /* the same with a new finest matrix of any pattern (P and R kept) */
 int lina_amg_recompute_csr(lina_solver* s, int n, const int* rowptr, const int* col, const double* val);
+/* lina_amg_recompute from finest values already in device memory (d_val:
+ * the setup pattern's nnz values): the Galerkin products run on the device
+ * from a per-pattern plan in LinA's summation order (bit-identical to the
+ * host products) while every level's pattern holds, else the host path
+ * takes over.  The host copy of the hierarchy (lina_amg_level_copy) is
+ * brought up to date when read.  For the staggered loop's elastic step
+ * (the new damage field's stiffness from lina_assembly_values). */
+int lina_amg_recompute_device(lina_solver* s, const double* d_val);
 
 /* pcg (PCGS.c:163-233) preconditioned by this hierarchy; x (initial guess,
  * overwritten) and b follow the reference's in/out semantics, including the
@@ -365,6 +373,43 @@ int lina_rn_set_trust_region(lina_rn* h, double delta);
  * host; mci/mv need room for rowptr[n] + n entries; returns nnz(M) */
 int lina_rn_scaled_hessian(int n, const int* rowptr, const int* col, const double* val, const double* D,
                            const double* G, const double* J, int* mrp, int* mci, double* mv);
+/* ---------------------------------------------------------- re-assembly
+ *
+ * The degraded stiffness of Crack's elastic step on the device: the
+ * contraction of the third-order tensor Strain_grad with the nodal rank-4
+ * moduli, sparse3D_times_rank4_2D_right_use_map (FEM2D/FEM2D.c:8534-8600,
+ * map: :8504-8532).  n nodes; the tensor as rows k = 0..n-1 of triples
+ * q in [bptr[k], bptr[k+1]) with indices bi1[q] = n b + j, bi2[q] = n d + k'
+ * (b, d in {0, 1}) and values bval[q]; the 2n x 2n target pattern rowptr /
+ * col (sorted columns) must hold every entry (n a + j, n c + k').  Each
+ * entry's value is the sum, from +0.0 and in the reference's order (node,
+ * triple, then (a,c) = (0,0), (0,1), (1,1), (1,0)), of
+ * C[n (8a + 4b + 2c + d) + k] * bval[q]: bit-identical to the reference.
+ * Host arrays at creation (copied); NULL on error. */
+typedef struct lina_assembly lina_assembly;
+lina_assembly* lina_assembly_create(int n, const int* bptr, const int* bi1, const int* bi2, const double* bval,
+                                    const int* rowptr, const int* col);
+void lina_assembly_destroy(lina_assembly* h);
+int lina_assembly_nnz(const lina_assembly* h);
+/* the number of products one assembly sums (4 per tensor triple) */
+long lina_assembly_terms(const lina_assembly* h);
+/* constrained DOFs (host, 2n flags; NULL: none) for lina_assembly_values */
+int lina_assembly_set_dirichlet(lina_assembly* h, const unsigned char* is_dirichlet);
+/* val (device, nnz) = the contraction with the moduli C (device, 16 n);
+ * then, with constrained DOFs set, the symmetric Dirichlet treatment of
+ * lina_dirichlet_symmetric with the values u_dir (device, 2n) on val and on
+ * b (device, 2n, may be NULL).  Returns after completion. */
+int lina_assembly_values(lina_assembly* h, const double* C, double* val, const double* u_dir, double* b);
+/* Symmetric Dirichlet treatment of a CSR system keeping its pattern (device
+ * arrays): a constrained row i becomes the unit row with b[i] = u[i]; every
+ * other row r gets b[r] -= sum over its constrained columns c, in column
+ * order from +0.0, of a_rc u[c], and those a_rc become 0.  The matrix stays
+ * symmetric where it was: the SPD-preserving replacement of Crack's
+ * Set_dirichlet_BC (Crack/crack.c:6445-6459), which only resets the rows.
+ * b may be NULL. */
+int lina_dirichlet_symmetric(int n, const int* rowptr, const int* col, const unsigned char* is_dirichlet,
+                             const double* u, double* val, double* b);
+
 /* The angle search of subspace_minimum (reflective_newton.c:394-450): the
  * stationary points of q(t) = c (c ax + s axy + bx) + s (s ay + by),
  * c = delta cos t, s = delta sin t, found on 1000 intervals of [0, 2 pi]

@@ -1910,3 +1910,64 @@ void orc_ilu_apply(int n, const int* lrp, const int* lci, const double* lv, cons
     ilu_apply(&F, z, w);
     free(F.y);
 }
+
+/* ------------------------------------------------------ re-assembly (f1)
+ *
+ * FEM2D.c:8504-8532 (get_map) fixes for every node k, triple m of its row
+ * (Indices1 = n b + j, Indices2 = n d + k') and (a, c) in the order (0,0),
+ * (0,1), (1,1), (1,0) the target (n a + j, n c + k'); :8534-8600 (use_map)
+ * starts from a zero matrix of that pattern and adds C[l] * p in the same
+ * order, l = n (4b + d) + k, then l += 2n, l += 8n, l -= 2n.  Restated
+ * with the index l written out as n (8a + 4b + 2c + d) + k. */
+static int row_position(const int* rowptr, const int* col, int r, int c) {
+    int lo = rowptr[r], hi = rowptr[r + 1] - 1;
+    while (lo <= hi) {
+        int mid = (lo + hi) / 2;
+        if (col[mid] == c) return mid;
+        if (col[mid] < c) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    return -1;
+}
+
+int orc_rank4_assembly(int n, const int* bptr, const int* bi1, const int* bi2, const double* bval, const double* C,
+                       const int* rowptr, const int* col, double* val) {
+    static const int ac[4][2] = {{0, 0}, {0, 1}, {1, 1}, {1, 0}};
+    int k, q, s2, e;
+    for (e = 0; e < rowptr[2 * n]; e++) val[e] = 0.0;
+    for (k = 0; k < n; k++)
+        for (q = bptr[k]; q < bptr[k + 1]; q++) {
+            const double p = bval[q];
+            const int b = bi1[q] < n ? 0 : 1, d = bi2[q] < n ? 0 : 1;
+            const int j = bi1[q] % n, kk = bi2[q] % n;
+            for (s2 = 0; s2 < 4; s2++) {
+                const int a = ac[s2][0], c = ac[s2][1];
+                const int l = planted == 3 ? n * (8 * a + 4 * d + 2 * c + b) + k : n * (8 * a + 4 * b + 2 * c + d) + k;
+                const int pos = row_position(rowptr, col, n * a + j, n * c + kk);
+                if (pos < 0) return -1;
+                val[pos] += C[l] * p;
+            }
+        }
+    return 0;
+}
+
+void orc_dirichlet_symmetric(int n, const int* rowptr, const int* col, const unsigned char* dir, const double* u,
+                             double* val, double* b) {
+    int r, p;
+    for (r = 0; r < n; r++) {
+        double s = 0.0;
+        int any = 0;
+        if (dir[r]) {
+            for (p = rowptr[r]; p < rowptr[r + 1]; p++) val[p] = col[p] == r ? 1.0 : 0.0;
+            if (b) b[r] = u[r];
+            continue;
+        }
+        for (p = rowptr[r]; p < rowptr[r + 1]; p++)
+            if (dir[col[p]]) {
+                s += val[p] * u[col[p]];
+                val[p] = 0.0;
+                any = 1;
+            }
+        if (b && any && planted != 4) b[r] = b[r] - s;
+    }
+}

@@ -52,8 +52,25 @@ void orc_smooth(int n, const int* rp, const int* ci, const double* v, const int*
                 const double* F, double* X, int sweeps, double w, int sm, int post);
 /* planted mistakes for the tests that show the checks catch them (0 = off):
  * 1 stale gathers in the multicolour sweep, 2 forward colour order in the
- * symmetric sweep's post-smoothing call */
+ * symmetric sweep's post-smoothing call, 3 and 4 see orc_rank4_assembly and
+ * orc_dirichlet_symmetric */
 void orc_plant(int which);
+
+/* sparse3D_times_rank4_2D_right_use_map (FEM2D/FEM2D.c:8534-8600) with
+ * its map (:8504-8532): val (nnz of the 2n x 2n pattern rowptr/col) =
+ * the contraction of the tensor rows (bptr, bi1, bi2, bval) with the nodal
+ * moduli C (16 n), accumulated from zero in the reference's order.
+ * Returns 0, -1 when a target entry is not in the pattern.  Planted
+ * mistake 3 swaps the tensor's b and d in the moduli index. */
+int orc_rank4_assembly(int n, const int* bptr, const int* bi1, const int* bi2, const double* bval, const double* C,
+                       const int* rowptr, const int* col, double* val);
+/* symmetric Dirichlet treatment keeping the pattern (the replacement of
+ * Crack's Set_dirichlet_BC, Crack/crack.c:6445-6459, that keeps the matrix
+ * symmetric): constrained rows -> unit rows with b = u; other rows subtract
+ * their constrained columns' a_rc u_c (column order, summed from +0.0) from
+ * b and zero them.  Planted mistake 4 leaves b unchanged in those rows. */
+void orc_dirichlet_symmetric(int n, const int* rowptr, const int* col, const unsigned char* dir, const double* u,
+                             double* val, double* b);
 
 /* greedy colouring of the symmetrised nonzero pattern (natural order) */
 int orc_colouring(int n, const int* rp, const int* ci, const double* v, int* colour);
