@@ -1,7 +1,9 @@
 #!/usr/bin/env python3
 """Headline benchmark: AMG-PCG solve time on the 2.1M-DOF degraded-elasticity
-system, mixed precision vs FP64, with the finest-level SpMV against the HBM
-roofline and the reference LinA timed on the host cores beside it.
+system, mixed precision vs FP64, with the finest-level SpMV and the whole CG
+iteration (algorithmic bytes summed over the levels) against the HBM
+roofline, the error against a committed SuperLU solution sketch, and the
+reference LinA timed on the host cores beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 1025]
 

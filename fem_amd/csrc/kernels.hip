@@ -1824,7 +1824,8 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 // M[I,J] -= Acol[I,:] Brow[:,J] over the 64 x 64 tiles outside block row and
 // column kb.  4 waves a tile, each a 32 x 32 quadrant of 2 x 2 MFMA blocks
 // (v_mfma_f64_16x16x4f64: lane l holds A[l%16][l/16], B[l/16][l%16] and
-// D[4 (l/16) + v][l%16], v = 0..3).
+// D[4 v + l/16][l%16], v = 0..3 -- one output row per 16-lane group and
+// register, unlike the FP32 16x16 form's 4 consecutive rows per group).
 __global__ __launch_bounds__(256) void k_bgj_gemm(int N, int kb, int nt, double* __restrict__ M,
                                                   const double* __restrict__ Acol, const double* __restrict__ Brow) {
     __shared__ double As[GJB][GJB + 1];  // [row in tile][k]
@@ -1856,7 +1857,7 @@ __global__ __launch_bounds__(256) void k_bgj_gemm(int N, int kb, int nt, double*
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++)
             for (int v = 0; v < 4; v++) {
-                const int i = i0 + qi + a * 16 + 4 * (l / 16) + v;
+                const int i = i0 + qi + a * 16 + 4 * v + l / 16;
                 const int j = j0 + qj + b * 16 + l % 16;
                 if (i < N && j < N) M[(size_t)i * N + j] -= acc[a][b][v];
             }
@@ -2297,7 +2298,7 @@ size_t gauss_jordan_work(int N) { return (size_t)GJB * GJB + 2 * (size_t)GJB * (
 
 void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* work) {
     if (N <= 0) return;
-    static const char* e = getenv("LINA_GJ");
+    const char* e = getenv("LINA_GJ");  // read per inversion (rare)
     if (e && std::strcmp(e, "scalar") == 0) {  // the unblocked steps, for comparisons
         int g = std::min(N, 4096);
         int gp = (N + TPB - 1) / TPB;

@@ -454,7 +454,7 @@ class ReflectiveNewton {
         dev::finalize(st_, np, part_ + 4096, scal_, 3, 0);
         hip_check(hipMemcpyAsync(host_, scal_ + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st_), "d2h");
         hip_check(hipStreamSynchronize(st_), "sync");
-        return host_[0] / 2. + host_[1];
+        return host_[0] / 2. + host_[1] + p_.q_offset;
     }
     void normalize(double* v) {
         double a = norm(v);
@@ -526,6 +526,20 @@ class ReflectiveNewton {
                 iter = amg_->amg_solve_dev(bs_, V2_);
                 hip_check(hipStreamSynchronize(amg_->stream()), "sync");
                 status = iter < p_.amg.amg_max_iter ? 1 : 0;
+                break;
+            }
+            case LINA_RN_HOST: {
+                // RN_UMF (reflective_newton.c:307-310): the caller's solver on
+                // the host copy of M, its return value the status
+                std::vector<double> hx(n), hb(n);
+                download(V2_, hx);
+                download(bs_, hb);
+                status = p_.host_solver(p_.host_ctx, n, M.rp.data(), M.ci.data(), M.v.data(), hx.data(), hb.data()) > 0
+                             ? 1
+                             : 0;
+                hip_check(hipMemcpyAsync(V2_, hx.data(), n * sizeof(double), hipMemcpyHostToDevice, st_), "h2d");
+                hip_check(hipStreamSynchronize(st_), "sync");
+                iter = 1;
                 break;
             }
             default:
@@ -771,6 +785,9 @@ void lina_rn_params_default(lina_rn_params* p, int dof, int depth) {
     p->amg.amg_eps = 1e-4;
     p->amg.amg_max_iter = 20;
     p->seed = 0x5EED0007ULL;
+    p->q_offset = 0.0;
+    p->host_solver = nullptr;
+    p->host_ctx = nullptr;
 }
 
 lina_rn* lina_rn_create(int n, const int* rowptr, const int* col, const double* val, const double* b, int nb,
@@ -779,7 +796,8 @@ lina_rn* lina_rn_create(int n, const int* rowptr, const int* col, const double* 
     int rc = rn_guard([&]() {
         if (n <= 0 || !rowptr || !col || !val || !b || !p || nb < 0 || (nb > 0 && (!idx || !lower || !upper)))
             throw std::invalid_argument("lina_rn_create: bad arguments");
-        if (p->solver < LINA_RN_BICGSTAB || p->solver > LINA_RN_GMRES) throw std::invalid_argument("unknown solver");
+        if (p->solver < LINA_RN_BICGSTAB || p->solver > LINA_RN_HOST) throw std::invalid_argument("unknown solver");
+        if (p->solver == LINA_RN_HOST && !p->host_solver) throw std::invalid_argument("LINA_RN_HOST needs host_solver");
         if (p->solver == LINA_RN_GMRES && (p->gmres_m < 1 || p->gmres_max_iter < 1))
             throw std::invalid_argument("GMRES restart length and iteration limit must be positive");
         lina::Csr H;

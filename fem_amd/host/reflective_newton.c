@@ -1,6 +1,8 @@
 /* reflective_newton and its globals (LinA/reflective_newton.c) in the LinA
  * drop-in: the method runs on the GPU through liblina_hip's lina_rn handle
- * for the reference's default quadratic q(x) = x.H.x/2 + x.b; the trust
+ * for a quadratic q(x) = x.H.x/2 + x.b + c -- the reference's default
+ * functional, or one given by callbacks that is quadratic (checked); the
+ * local solver of RN_set_local_solver is called on host copies; the trust
  * radius persists between calls like the reference's RN_Delta. */
 #include "../../include/linalg/reflective_newton.h"
 
@@ -117,19 +119,158 @@ void print_RN_statistics(FILE* file) {
     fprintf(file, "line searches: %d\n", statistics.line_search_count);
 }
 
+/* RN_UMF's local solver (reflective_newton.c:307-310) through the device
+ * method's LINA_RN_HOST slot: M arrives as CSR and goes to the caller's
+ * solver as a row list */
+static int rn_host_trampoline(void* ctx, int n, const int* rp, const int* ci, const double* v, double* x,
+                              const double* b) {
+    int (*solver)(sparse_matrix*, double*, double*) = (int (*)(sparse_matrix*, double*, double*))ctx;
+    sparse_matrix* M = sparse_zero(n);
+    double* bb = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    int i, rc;
+    for (i = 0; i < n; i++) {
+        const int m = rp[i + 1] - rp[i];
+        free(M->Values[i]);
+        free(M->Indices[i]);
+        M->Values[i] = (double*)malloc((size_t)(m > 0 ? m : 1) * sizeof(double));
+        M->Indices[i] = (int*)malloc((size_t)(m > 0 ? m : 1) * sizeof(int));
+        memcpy(M->Values[i], v + rp[i], (size_t)m * sizeof(double));
+        memcpy(M->Indices[i], ci + rp[i], (size_t)m * sizeof(int));
+        M->Len[i] = m;
+    }
+    memcpy(bb, b, (size_t)n * sizeof(double));
+    rc = solver(M, x, bb);
+    Free_sparse(M);
+    free(bb);
+    return rc;
+}
+
+/* A functional given by callbacks (set_RN_functional, RN_set_gradient,
+ * RN_set_Hesse_function, as Crack's damage step sets them, crack.c:9218-9220)
+ * runs on the device when it is the quadratic q(x) = x.H.x/2 + b.x + c: H is
+ * the Hessian callback's matrix at X, b = gradient(X) - H X, c = F(X) -
+ * X.H.X/2 - b.X.  Checked at a second point (X moved on every row that is
+ * not a unit row): the Hessian must be the same, the gradient H X' + b and
+ * the functional q(X') + c to 1e-9 relative.  Crack's damage functional is
+ * such a quadratic in the damage field (its weight g(h) = h (s + (1-s) h)
+ * and every other term are polynomials of degree <= 2); its gradient sets
+ * unit-row entries to X_i, which the unit rows of its Hessian reproduce. */
+static sparse_matrix* rn_capture_H = NULL;
+static double* rn_capture_b = NULL;
+
+static double quad_value(sparse_matrix* H, const double* b, const double* x, int n) {
+    int i, j;
+    double xhx = 0, bx = 0;
+    for (i = 0; i < n; i++) {
+        double s = 0;
+        for (j = 0; j < H->Len[i]; j++) s += H->Values[i][j] * x[H->Indices[i][j]];
+        xhx += x[i] * s;
+        bx += b[i] * x[i];
+    }
+    return xhx / 2. + bx;
+}
+
+static int capture_quadratic(double* X, int n, double* offset) {
+    sparse_matrix* H;
+    double *G, *Hx, *X2, *G2;
+    double F, F2, q2, c, gscale = 0, gerr = 0;
+    int i, j, ok = 1;
+    H = RN_compute_Hesse ? RN_compute_Hesse(X) : clone(RN_Hesse);
+    if (!H || H->size != n) {
+        fprintf(stderr, "liblinalg_hip: reflective_newton: no Hessian of size %d\n", n);
+        return -1;
+    }
+    Hx = sparse_mult(H, X);
+    if (RN_gradient) G = RN_gradient(X);
+    else {
+        G = clone_vector(Hx, n);
+        if (RN_b) vector_add(G, RN_b, 1., n);
+    }
+    rn_capture_b = (double*)malloc((size_t)n * sizeof(double));
+    for (i = 0; i < n; i++) rn_capture_b[i] = G[i] - Hx[i];
+    F = RN_Functional ? RN_Functional(X, n) : quad_value(H, rn_capture_b, X, n);
+    c = F - quad_value(H, rn_capture_b, X, n);
+    /* the second point */
+    X2 = clone_vector(X, n);
+    for (i = 0; i < n; i++) {
+        const int unit = H->Len[i] == 1 && H->Indices[i][0] == i && H->Values[i][0] == 1.0;
+        if (!unit) X2[i] += 1e-3 * (1.0 + fabs(X[i])) * sin(0.7 * i + 0.3);
+    }
+    if (RN_compute_Hesse) {
+        sparse_matrix* H2 = RN_compute_Hesse(X2);
+        double hs = 0, he = 0;
+        if (!H2 || H2->size != n) ok = 0;
+        for (i = 0; ok && i < n; i++) {
+            if (H2->Len[i] != H->Len[i]) {
+                ok = 0;
+                break;
+            }
+            for (j = 0; j < H->Len[i]; j++) {
+                if (H2->Indices[i][j] != H->Indices[i][j]) ok = 0;
+                hs = fmax(hs, fabs(H->Values[i][j]));
+                he = fmax(he, fabs(H2->Values[i][j] - H->Values[i][j]));
+            }
+        }
+        if (ok && he > 1e-12 * hs) ok = 0;
+        if (H2) {
+            free_sparse(H2);
+        }
+    }
+    if (ok && RN_gradient) {
+        double* H2x = sparse_mult(H, X2);
+        G2 = RN_gradient(X2);
+        for (i = 0; i < n; i++) {
+            const double g = H2x[i] + rn_capture_b[i];
+            gscale = fmax(gscale, fabs(g));
+            gerr = fmax(gerr, fabs(G2[i] - g));
+        }
+        if (gerr > 1e-9 * (gscale + 1e-300)) ok = 0;
+        free(G2);
+        free(H2x);
+    }
+    if (ok && RN_Functional) {
+        F2 = RN_Functional(X2, n);
+        q2 = quad_value(H, rn_capture_b, X2, n) + c;
+        if (fabs(F2 - q2) > 1e-9 * (fabs(F2) + fabs(F) + 1e-300)) ok = 0;
+    }
+    free(X2);
+    free(G);
+    free(Hx);
+    if (!ok) {
+        fprintf(stderr, "liblinalg_hip: reflective_newton: the functional of the callbacks is not a quadratic "
+                        "x.H.x/2 + b.x + c; the device method needs one\n");
+        free_sparse(H);
+        free(rn_capture_b);
+        rn_capture_b = NULL;
+        return -1;
+    }
+    rn_capture_H = H;
+    *offset = c;
+    return 0;
+}
+
 int reflective_newton(double* X) {
     lina_rn_params p;
     lina_rn_stats st;
     dropin_csr c;
     lina_rn* h;
+    sparse_matrix* Hm;
+    double* bvec;
+    double offset = 0;
     int it, dof = 2, depth = 6;
-    if (RN_Functional || RN_gradient || RN_compute_Hesse) {
-        fprintf(stderr, "liblinalg_hip: reflective_newton runs the default quadratic functional only\n");
-        return -1;
-    }
-    if (!RN_Hesse || !RN_b) {
-        fprintf(stderr, "liblinalg_hip: reflective_newton needs set_RN_Hesse and set_RN_linear\n");
-        return -1;
+    const int callbacks = RN_Functional || RN_gradient || RN_compute_Hesse;
+    if (callbacks) {
+        const int n = RN_compute_Hesse ? RN_dim : (RN_Hesse ? RN_Hesse->size : RN_dim);
+        if (capture_quadratic(X, n, &offset) < 0) return -1;
+        Hm = rn_capture_H;
+        bvec = rn_capture_b;
+    } else {
+        if (!RN_Hesse || !RN_b) {
+            fprintf(stderr, "liblinalg_hip: reflective_newton needs set_RN_Hesse and set_RN_linear\n");
+            return -1;
+        }
+        Hm = RN_Hesse;
+        bvec = RN_b;
     }
     if (RN_AMG_setup_data) {
         dof = RN_AMG_setup_data->AMG_deg_of_freedom;
@@ -147,26 +288,46 @@ int reflective_newton(double* X) {
     p.trust_upper = RN_trust_upper;
     p.radius_min = RN_trust_radius_min;
     p.radius_max = RN_trust_radius_max;
+    p.q_offset = offset;
+    it = 0;
     switch (RN_solver_type) {
         case RN_PCG: p.solver = LINA_RN_PCG; break;
         case RN_AMG: p.solver = LINA_RN_AMG; break;
         case RN_BICGSTAB: p.solver = LINA_RN_BICGSTAB; break;
         case RN_GMRES: p.solver = LINA_RN_GMRES; break;
+        case RN_UMF:
+            if (!RN_local_solver) {
+                p.solver = LINA_RN_GMRES;
+                break;
+            }
+            p.solver = LINA_RN_HOST;
+            p.host_solver = rn_host_trampoline;
+            p.host_ctx = (void*)RN_local_solver;
+            break;
         default:
             fprintf(stderr, "liblinalg_hip: reflective_newton inner solver %d is not built (use RN_PCG, RN_AMG, "
-                            "RN_BICGSTAB or RN_GMRES)\n", RN_solver_type);
-            return -1;
+                            "RN_BICGSTAB, RN_GMRES or RN_set_local_solver)\n", RN_solver_type);
+            it = -1;
     }
-    c = dropin_to_csr(RN_Hesse);
-    h = lina_rn_create(c.n, c.rp, c.ci, c.v, RN_b, RN_bound_num, RN_bound, RN_lower, RN_upper, &p);
-    dropin_free_csr(&c);
-    if (!h) {
-        dropin_check(-1, "reflective_newton setup");
-        return -1;
+    if (it == 0) {
+        c = dropin_to_csr(Hm);
+        h = lina_rn_create(c.n, c.rp, c.ci, c.v, bvec, RN_bound_num, RN_bound, RN_lower, RN_upper, &p);
+        dropin_free_csr(&c);
+        if (!h) {
+            dropin_check(-1, "reflective_newton setup");
+            it = -1;
+        } else {
+            it = lina_rn_solve(h, X, &st);
+            dropin_check(it, "reflective_newton");
+            lina_rn_destroy(h);
+        }
     }
-    it = lina_rn_solve(h, X, &st);
-    dropin_check(it, "reflective_newton");
-    lina_rn_destroy(h);
+    if (callbacks) {
+        free_sparse(rn_capture_H);
+        free(rn_capture_b);
+        rn_capture_H = NULL;
+        rn_capture_b = NULL;
+    }
     if (it >= 0) {
         RN_Delta = st.delta;
         statistics.calls++;

@@ -45,7 +45,8 @@ class RNParams(C.Structure):
                 ("trust_upper", C.c_double), ("radius_min", C.c_double), ("radius_max", C.c_double),
                 ("solver", C.c_int), ("pcg_iter", C.c_int), ("bi_eps", C.c_double), ("bi_max_iter", C.c_int),
                 ("gmres_m", C.c_int), ("gmres_max_iter", C.c_int), ("gmres_tol", C.c_double),
-                ("amg", Params), ("seed", C.c_ulonglong)]
+                ("amg", Params), ("seed", C.c_ulonglong), ("q_offset", C.c_double), ("host_solver", C.c_void_p),
+                ("host_ctx", C.c_void_p)]
 
 
 class RNStats(C.Structure):
@@ -57,7 +58,7 @@ class RNStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
-RN_SOLVER = {"bicgstab": 1, "amg": 2, "pcg": 3, "gmres": 4}
+RN_SOLVER = {"bicgstab": 1, "amg": 2, "pcg": 3, "gmres": 4, "host": 5}
 
 
 class BatchStats(C.Structure):

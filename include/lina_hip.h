@@ -325,6 +325,13 @@ int lina_batch_tuned(lina_batch* h, int* level, char* which, int* lpr64, int* lp
 #define LINA_RN_PCG 3      /* pcg with ILU(0) of M, tol = eps, <= pcg_iter */
 #define LINA_RN_GMRES 4    /* local solver in the RN_UMF slot (RN_set_local_solver): GMRES(gmres_m) with right
                             * ILU(0) as FEM.c drives it; M indefinite when it misses gmres_tol */
+#define LINA_RN_HOST 5     /* the RN_UMF slot with the caller's own solver (host_solver): M as host CSR, x (the
+                            * start -D g, overwritten with the solution) and b = -D g on the host; its return
+                            * value is the status (1 positive definite, 0 indefinite) */
+
+/* a host local solver for LINA_RN_HOST (rowptr/col/val: M in CSR, sorted rows) */
+typedef int (*lina_rn_host_solver)(void* ctx, int n, const int* rowptr, const int* col, const double* val, double* x,
+                                   const double* b);
 
 typedef struct lina_rn_params {
     int max_iter;        /* RN_max_iter (50) */
@@ -343,6 +350,10 @@ typedef struct lina_rn_params {
     double gmres_tol;    /* |r|/|r0| of the preconditioned system (1e-10) */
     lina_amg_params amg; /* hierarchy of H for the BiCGStab / AMG inner solvers */
     unsigned long long seed; /* start vectors of the negative-curvature power method */
+    double q_offset;     /* constant added to q (0): a functional x.H.x/2 + b.x + c keeps its values, which
+                          * the relative reduction |dq/q| of the outer test depends on */
+    lina_rn_host_solver host_solver; /* LINA_RN_HOST */
+    void* host_ctx;
 } lina_rn_params;
 
 typedef struct lina_rn_stats {

@@ -459,3 +459,101 @@ def test_amg_setup_fills_the_level_fields(libs, kind):
     say(kind, "levels", out[0]["fine"], out[1]["fine"])
     for key in out[1]:
         assert out[0][key] == out[1][key], key
+
+
+@pytest.mark.parametrize("local", [False, True])
+def test_reflective_newton_with_callbacks_matches_reference(libs, local):
+    """The functional, gradient and Hessian given as callbacks (set_RN_functional,
+    RN_set_gradient, RN_set_Hesse_function: Crack's damage step,
+    crack.c:9218-9220) for a quadratic with a constant term and two unit
+    rows (gradient x_i there, as Crack's Dirichlet rows); the drop-in
+    recognises the quadratic and runs the device method, the reference calls
+    the callbacks -- same iterations, iterates to 1e-9.  local=True: the
+    local solver slot (RN_set_local_solver, the RN_UMF path) with a host
+    dense solver, called on host copies by the drop-in."""
+    s, dold, ones = P.damage_system(33, h_scale=100.0)
+    n = s.n
+    unit = np.array([5, 600])
+    A = s.scipy().tolil()
+    for u in unit:
+        A.rows[u] = [int(u)]
+        A.data[u] = [1.0]
+    A = A.tocsr()
+    A.sort_indices()
+    b = s.b.copy()
+    b[unit] = 0.0
+    c0 = 3.7
+    rp, ci, va = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.copy()
+    lo, up = dold.copy(), ones.copy()
+    lo[unit], up[unit] = 0.0, 1.0
+    idx = np.arange(n, dtype=np.int32)
+    libc = C.CDLL(None)
+    libc.malloc.restype = C.c_void_p
+    libc.malloc.argtypes = [C.c_size_t]
+    from tests.dropin_bind import SparseMatrix
+    SM = C.POINTER(SparseMatrix)
+    FUNC = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double), C.c_int)
+    GRAD = C.CFUNCTYPE(C.c_void_p, C.POINTER(C.c_double))
+    HESS = C.CFUNCTYPE(C.c_void_p, C.POINTER(C.c_double))
+    SOLV = C.CFUNCTYPE(C.c_int, SM, C.POINTER(C.c_double), C.POINTER(C.c_double))
+    calls = {}
+
+    def run(lib, tag):
+        L = lib.L
+
+        def f(x, m):
+            xv = np.ctypeslib.as_array(x, shape=(n,))
+            return float(0.5 * xv @ (A @ xv) + b @ xv + c0)
+
+        def g(x):
+            xv = np.ctypeslib.as_array(x, shape=(n,))
+            gv = A @ xv + b
+            p = libc.malloc(8 * n)
+            C.memmove(p, gv.ctypes.data, 8 * n)
+            return p
+
+        def h(x):
+            calls[tag] = calls.get(tag, 0) + 1
+            return C.cast(lib.from_csr(rp, ci, va), C.c_void_p).value
+
+        def solve(M, x, rhs):
+            m = M.contents
+            D = np.zeros((m.size, m.size))
+            for i in range(m.size):
+                for j in range(m.Len[i]):
+                    D[i, m.Indices[i][j]] += m.Values[i][j]
+            xv = np.linalg.solve(D, np.ctypeslib.as_array(rhs, shape=(m.size,)))
+            np.ctypeslib.as_array(x, shape=(m.size,))[:] = xv
+            return 1
+        cbs = (FUNC(f), GRAD(g), HESS(h), SOLV(solve))
+        L.RN_print_info(0)
+        L.set_RN_functional.argtypes = [C.c_void_p]
+        L.RN_set_gradient.argtypes = [C.c_void_p]
+        L.RN_set_Hesse_function.argtypes = [C.c_void_p, C.c_int]
+        L.set_RN_functional(C.cast(cbs[0], C.c_void_p))
+        L.RN_set_gradient(C.cast(cbs[1], C.c_void_p))
+        L.RN_set_Hesse_function(C.cast(cbs[2], C.c_void_p), n)
+        L.set_RN_constraints.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+        L.set_RN_constraints(idx.ctypes.data_as(C.POINTER(C.c_int)), dptr(lo), dptr(up), n)
+        L.RN_set_params.argtypes = [C.c_int] + [C.c_double] * 10
+        L.RN_set_params(50, 1e-8, 0.1, 0.9, 0.1, float(np.sqrt(n) / 2), 1.0, 0.6, 1.1, 1e-3, 1000.0)
+        L.RN_set_local_solver.argtypes = [C.c_void_p]
+        if local:
+            L.RN_set_local_solver(C.cast(cbs[3], C.c_void_p))
+        else:
+            L.RN_set_local_solver(None)
+            L.RN_set_solver(3)
+        x = np.clip(dold.copy(), lo, up)
+        L.reflective_newton.argtypes = [C.POINTER(C.c_double)]
+        it = L.reflective_newton(dptr(x))
+        # leave the globals as the other tests expect them
+        L.set_RN_functional(None)
+        L.RN_set_gradient(None)
+        L.RN_set_Hesse_function(None, n)
+        L.RN_set_local_solver(None)
+        return it, x
+
+    (i1, x1), (i2, x2) = run(libs[0], "ours"), run(libs[1], "ref")
+    say("RN with callbacks", "local" if local else "pcg", "iters", i1, i2, "rel", rel(x1, x2), "hessian calls", calls)
+    assert i1 > 0 and i1 == i2 and rel(x1, x2) < 1e-9
+    assert np.all(x1 >= lo - 1e-12) and np.all(x1 <= up + 1e-12)

@@ -370,3 +370,26 @@ def test_device_recompute_matches_oracle(gpu, kind, capfd, monkeypatch):
     orc.amg_free(h)
     say("  pcg iters oracle", it, "gpu", g["iters"], "rel", rel(g["x"], xo))
     assert abs(g["iters"] - it) <= 1 and rel(g["x"], xo) < 1e-12
+
+
+@pytest.mark.parametrize("m,depth", [(65, 3), (129, 3)])
+def test_blocked_coarse_inverse_matches_scalar_steps(gpu, m, depth, monkeypatch):
+    """The coarsest level's dense inverse by 64-pivot blocks with the FP64
+    MFMA update against the unblocked Gauss-Jordan steps (LINA_GJ=scalar):
+    coarse systems of several blocks, pcg iterations equal and iterates to
+    1e-10 (the two differ only in rounding)."""
+    s = P.poisson(m)
+    out = []
+    for mode in ("scalar", "blocked"):
+        if mode == "scalar":
+            monkeypatch.setenv("LINA_GJ", "scalar")
+        else:
+            monkeypatch.delenv("LINA_GJ", raising=False)
+        amg = gpu.AMG(s.rowptr, s.col, s.val, gpu.params("reference", 1, depth))
+        nd = amg.level(0)["fine"]
+        g = amg.pcg(s.b, tol=1e-10, max_iter=500)
+        amg.close()
+        out.append((g["iters"], g["x"]))
+        say("  coarse inverse", mode, "coarsest rows", nd, "pcg iters", g["iters"])
+    assert nd > 64
+    assert out[0][0] == out[1][0] and rel(out[1][1], out[0][1]) < 1e-10

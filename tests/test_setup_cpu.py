@@ -86,6 +86,8 @@ def _declared_functions(header):
     names = set()
     for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text, flags=re.M):
         name = m.group(1)
+        if m.group(0).lstrip().startswith("typedef"):  # function-pointer types
+            continue
         if name not in ("if", "while", "for", "return", "sizeof", "typedef"):
             names.add(name)
     return names
