@@ -175,6 +175,8 @@ size_t plan_bytes(const ProductPlan& P) {
 bool Solver::device_rap_ok() const {
     if (!uploaded() || cbatched_ || !seg_off_.empty() || !cpiv_ || nd_ < 0) return false;
     if ((size_t)nd_ * (size_t)nd_ > (size_t)INT32_MAX) return false;
+    // the exact lexicographic SOR's packed records are rebuilt from host
+    // values: such hierarchies keep the host path
     for (const DevLevel& d : L_)
         if (d.sor_packed) return false;
     const char* e = getenv("LINA_RAP");

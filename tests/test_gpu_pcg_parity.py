@@ -314,17 +314,6 @@ def test_amg_solve_reference_jacobi_matches_golden(gpu, case):
     assert rel(x, x_ref) < 1e-12
 
 
-def _transpose_positions(rowptr, col):
-    """For a structurally symmetric CSR pattern: the position of (j, i) for
-    every entry (i, j)."""
-    rows = np.repeat(np.arange(len(rowptr) - 1), np.diff(rowptr))
-    by_ct = np.lexsort((rows, col))                 # entries ordered by (col, row) = the transpose's CSR order
-    tpos = np.empty(len(col), dtype=np.int64)
-    tpos[np.arange(len(col))] = by_ct
-    assert np.array_equal(rows[by_ct], col) and np.array_equal(col[by_ct], rows)
-    return tpos
-
-
 @pytest.mark.parametrize("kind", ["poisson33", "damage65", "elasticity33"])
 def test_device_recompute_matches_oracle(gpu, kind, capfd, monkeypatch):
     """AMG_recompute_coarse_matrices on the device (the Galerkin products
@@ -333,19 +322,23 @@ def test_device_recompute_matches_oracle(gpu, kind, capfd, monkeypatch):
     (orc_amg_recompute, pinned to the reference) bit for bit, over two
     successive sets of new finest values; the device path is the one taken
     (its progress line), and the pcg with the recomputed hierarchy equals the
-    oracle's pcg on the new system."""
+    oracle's pcg on the new system (multicolour SOR: a hierarchy with the
+    exact lexicographic SOR's packed records recomputes on the host)."""
     monkeypatch.setenv("LINA_RN_TIMING", "1")
     base = {"poisson33": lambda: P.poisson(33), "damage65": lambda: P.damage_system(65)[0],
             "elasticity33": lambda: P.degraded_elasticity(33)}[kind]()
     depth = 4
     orc = Oracle()
     h = orc.amg_setup(base.rowptr, base.col, base.val, dof=base.dof, depth=depth)
-    amg = gpu.AMG(base.rowptr, base.col, base.val, gpu.params("reference", base.dof, depth))
+    # multicolour SOR: the exact lexicographic SOR's packed records keep the host path
+    amg = gpu.AMG(base.rowptr, base.col, base.val, gpu.params("reference", base.dof, depth, smoother="mc_sor"))
     rng = np.random.default_rng(8)
-    tpos = _transpose_positions(base.rowptr, base.col)
+    rows = np.repeat(np.arange(base.n), np.diff(base.rowptr))
     for trial in range(2):
-        v = base.val * (1.0 + 0.4 * rng.random(len(base.val)))
-        v = 0.5 * (v + v[tpos])            # keep the matrix symmetric (explicit zeros stay in the pattern)
+        # S A S with a random positive diagonal S: new values of the same
+        # pattern that keep A symmetric positive definite (explicit zeros stay)
+        sc = 1.0 + 0.4 * rng.random(base.n)
+        v = base.val * sc[rows] * sc[base.col]
         capfd.readouterr()
         amg.recompute(v)
         err = capfd.readouterr().err
@@ -363,7 +356,7 @@ def test_device_recompute_matches_oracle(gpu, kind, capfd, monkeypatch):
     xo, bo = np.zeros(base.n), base.b.copy()
     res, vc = C.c_double(), C.c_int()
     dp = C.POINTER(C.c_double)
-    it = orc.L.orc_pcg_amg(h, xo.ctypes.data_as(dp), bo.ctypes.data_as(dp), 1e-10, 500, 0, 1.6, 1e-2, 100, 0, 0,
+    it = orc.L.orc_pcg_amg(h, xo.ctypes.data_as(dp), bo.ctypes.data_as(dp), 1e-10, 500, 2, 1.6, 1e-2, 100, 0, 0,
                            C.byref(res), C.byref(vc))
     g = amg.pcg(base.b, tol=1e-10, max_iter=500)
     amg.close()
