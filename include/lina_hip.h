@@ -200,6 +200,15 @@ int lina_dev_fill(double* d, long n, double value);
  * (the LinA drop-in layer): each returns after completion.
  * scalar (linear_algebra.c:256-263) on device vectors, deterministic tree order */
 double lina_dev_dot(const double* a, const double* b, long n);
+/* lina_dev_dot that allocates nothing.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (allocation-free serving loops,
+ * harnesses that forbid hipMalloc).  The same kernels and result as
+ * lina_dev_dot; the block partials and the result live in a file-scope
+ * device scratch of 8,256 bytes per device (at most 1,024 partials), held
+ * until the process ends.  n in [0, 2^31-1], else NaN and lina_last_error().
+ * Every call on one device uses that device's scratch: calls that overlap
+ * on one device race. */
+double lina_dev_dot_noalloc(const double* a, const double* b, long n);
 /* classical Gram-Schmidt pieces of GMRES (gmres.c:401-414): out[l] =
  * V_l . r for the m <= 64 vectors V_l = V + l*ld (out: host array); and
  * y = base + sign * sum_l c[l] V_l (c: host array; y may alias base) */
