@@ -167,6 +167,16 @@ int lina_amg_set_params(lina_solver* s, const lina_amg_params* p) {
 
 int lina_amg_depth(const lina_solver* s) { return (int)s->s.host().lv.size(); }
 
+int lina_amg_coarse_rows(const lina_solver* s, int* dense, int* trivial) {
+    return guard([&]() {
+        if (!s) throw std::invalid_argument("null solver");
+        if (!s->s.uploaded()) throw std::invalid_argument("the hierarchy is not on the device");
+        if (dense) *dense = s->s.coarse_dense_rows();
+        if (trivial) *trivial = s->s.coarse_trivial_rows();
+        return 0;
+    });
+}
+
 int lina_amg_level_info(const lina_solver* s, int level, int* fine, int* coarse, int* nnz_a, int* nnz_p, int* nnz_r) {
     return guard([&]() {
         const auto& L = s->s.host().lv.at(level);

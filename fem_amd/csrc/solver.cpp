@@ -1136,6 +1136,7 @@ void Solver::level_smoother(int l, int& smoother, int& sweeps) const {
         smoother = prm_.coarse_smoother;
         sweeps = prm_.coarse_sweeps;
     }
+    if (l > 0 && l < (int)L_.size() - 1 && l < 32 && ((prm_.skip_levels >> l) & 1u)) sweeps = 0;
 }
 
 // smoother kind and sweep count of level l's pre- and post-smoothing in the
@@ -1253,9 +1254,14 @@ void Solver::vcycle(int l, const T* f, T* x, int xinit) {
     DevLevel& c = L_[l - 1];
     int sm, sw;
     vcycle_smoother(l, sm, sw);
-    smooth<T>(l, f, x, sw, false, xinit);
+    // a level without smoothing (skip_levels) passes its right-hand side down
+    // and the coarse correction up: x = P V(R f), no matrix of its own read
+    const bool pass = sw == 0 && xinit == X_ZERO && l < (int)L_.size() - 1;
     T* r = LevelVecs<T>::r(d);
-    dev::spmv<T, T>(st, pickA<T>(d), x, f, r, dev::MODE_RES, nullptr, nullptr);
+    if (!pass) {
+        smooth<T>(l, f, x, sw, false, xinit);
+        dev::spmv<T, T>(st, pickA<T>(d), x, f, r, dev::MODE_RES, nullptr, nullptr);
+    }
     T* fc = LevelVecs<T>::f(c);
     T* xc = LevelVecs<T>::x(c);
     dev::Epi<T, T> e;
@@ -1282,9 +1288,13 @@ void Solver::vcycle(int l, const T* f, T* x, int xinit) {
             cinit = X_J0;
         }
     }
-    dev::spmv<T, T>(st, pickR<T>(d), r, nullptr, fc, epi ? dev::MODE_SETX : dev::MODE_SET, nullptr, nullptr,
-                    epi ? &e : nullptr);
+    dev::spmv<T, T>(st, pickR<T>(d), pass ? f : r, nullptr, fc, epi ? dev::MODE_SETX : dev::MODE_SET, nullptr,
+                    nullptr, epi ? &e : nullptr);
     vcycle<T>(l - 1, fc, xc, cinit);
+    if (pass) {
+        dev::spmv<T, T>(st, pickP<T>(d), xc, nullptr, x, dev::MODE_SET, nullptr, nullptr);
+        return;
+    }
     if (is_jacobi(sm) && sw % 2 == 1) {
         dev::spmv<T, T>(st, pickP<T>(d), xc, x, LevelVecs<T>::xo(d), dev::MODE_ADDB, nullptr, nullptr);
         smooth<T>(l, f, x, sw, true, X_SPARE);

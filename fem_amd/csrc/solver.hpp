@@ -189,6 +189,9 @@ class Solver {
     const std::vector<int>& colours(int l) const { return colours_.at(l); }
     bool uploaded() const { return bufs_ != nullptr; }
     int n() const { return n_; }
+    // rows of the coarsest level solved by the dense inverse / by their diagonal
+    int coarse_dense_rows() const { return nd_; }
+    int coarse_trivial_rows() const { return nt_; }
 
     int pcg_dev(double* X, double* B, double tol, int max_iter, int stop, lina_stats* st);
     int pcg_host(double* x, double* b, double tol, int max_iter, int stop, lina_stats* st);

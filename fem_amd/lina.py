@@ -27,7 +27,7 @@ class Params(C.Structure):
                 ("eps_plus", C.c_double), ("matrix_tol", C.c_double), ("smoother", C.c_int), ("omega", C.c_double),
                 ("amg_eps", C.c_double), ("amg_max_iter", C.c_int), ("zero_init", C.c_int),
                 ("precision", C.c_int), ("coarse_smoother", C.c_int), ("coarse_sweeps", C.c_int),
-                ("autotune", C.c_int), ("cycle", C.c_int)]
+                ("autotune", C.c_int), ("cycle", C.c_int), ("skip_levels", C.c_uint)]
 
 
 class Stats(C.Structure):
@@ -143,6 +143,7 @@ def lib():
                                         C.POINTER(BatchStats)]
     L.lina_batch_tuned.argtypes = [C.c_void_p, _ip, C.c_char_p, _ip, _ip, C.c_int]
     L.lina_amg_recompute_device.argtypes = [C.c_void_p, C.c_void_p]
+    L.lina_amg_coarse_rows.argtypes = [C.c_void_p, _ip, _ip]
     L.lina_assembly_create.argtypes = [C.c_int, _ip, _ip, _ip, _dp, _ip, _ip]
     L.lina_assembly_create.restype = C.c_void_p
     L.lina_assembly_destroy.argtypes = [C.c_void_p]
@@ -228,6 +229,12 @@ class AMG:
     @property
     def depth(self) -> int:
         return lib().lina_amg_depth(self.h)
+
+    def coarse_rows(self):
+        """(dense, trivial) rows of the coarsest level's direct solve"""
+        d, t = C.c_int(), C.c_int()
+        _check(lib().lina_amg_coarse_rows(self.h, C.byref(d), C.byref(t)))
+        return d.value, t.value
 
     def level_sizes(self, l: int) -> dict:
         """rows of level l and of the next coarser one, and the nonzeros of

@@ -77,6 +77,10 @@ typedef struct lina_amg_params {
     int autotune;                  /* 1: time the row-partition candidates of every level's kernels at upload
                                     * and keep the fastest (changes summation order of multi-lane rows) */
     int cycle;                     /* LINA_CYCLE_*: AMG_setup's cycle_structure (AMG_solve's cycle) */
+    unsigned skip_levels;          /* bit l (level l counted from the coarsest, 0 < l < depth - 1): no
+                                    * smoothing on that level in the V-cycle -- its right-hand side is
+                                    * restricted as it comes and the coarse correction prolongated back,
+                                    * which keeps the cycle symmetric and reads no A of that level (0) */
 } lina_amg_params;
 
 typedef struct lina_stats {
@@ -113,6 +117,9 @@ int lina_amg_set_params(lina_solver* s, const lina_amg_params* p); /* smoother/e
 /* hierarchy inspection: depth; per level sizes; copies of A/P/R ('A','P','R'),
  * labels, and the coarse LU factors ('L','U' on level 0) */
 int lina_amg_depth(const lina_solver* s);
+/* rows of the coarsest level solved through the dense inverse (AMG_Direct_
+ * solver's dense part) and rows solved by their diagonal alone */
+int lina_amg_coarse_rows(const lina_solver* s, int* dense, int* trivial);
 int lina_amg_level_info(const lina_solver* s, int level, int* fine, int* coarse, int* nnz_a, int* nnz_p, int* nnz_r);
 int lina_amg_level_nnz(const lina_solver* s, int level, char which);
 int lina_amg_level_copy(const lina_solver* s, int level, char which, int* rowptr, int* col, double* val);
