@@ -119,11 +119,12 @@ def iteration_bytes(amg, precision: str, coarse_sweeps: int = 1) -> dict:
     residual norm (48 n) and p = z + beta p (24 n); per level above the
     coarsest its smoothing sweeps (a sweep = SpMV + b and the diagonal
     scaling read; two on the finest level -- the forward and the reverse
-    multicolour SOR pass -- and 2 x coarse_sweeps l1-Jacobi sweeps below),
-    one residual (SpMV + b), the restriction R and the prolongation P; the
-    coarsest level's dense inverse applied once (its FP64 upper triangle,
-    4 nd^2, + 16 nd, nd its dense rows).  The hierarchy's arithmetic is FP32
-    in the mixed mode."""
+    multicolour SOR pass -- and 2 x coarse_sweeps l1-Jacobi sweeps below, of
+    which the first starts from the zero guess and reads no matrix entry:
+    f, the l1 diagonal and x, 3 words a row), one residual (SpMV + b), the
+    restriction R and the prolongation P; the coarsest level's dense inverse
+    applied once (its FP64 upper triangle, 4 nd^2, + 16 nd, nd its dense
+    rows).  The hierarchy's arithmetic is FP32 in the mixed mode."""
     top = amg.depth - 1
     L = [amg.level_sizes(l) for l in range(amg.depth)]
     n0, nnz0 = L[top]["fine"], L[top]["nnz_A"]
@@ -134,9 +135,12 @@ def iteration_bytes(amg, precision: str, coarse_sweeps: int = 1) -> dict:
     for l in range(top, 0, -1):
         d = L[l]
         n, nc = d["fine"], d["coarse"]
-        sweeps = 2 if l == top else 2 * coarse_sweeps
         a = spmv_alg_bytes(d["nnz_A"], n, n, vp)
-        per_level.append(int(sweeps * (a + 2 * vb * n) + (a + vb * n) + spmv_alg_bytes(d["nnz_R"], nc, n, vp) +
+        if l == top:
+            smooth = 2 * (a + 2 * vb * n)
+        else:
+            smooth = 3 * vb * n + (2 * coarse_sweeps - 1) * (a + 2 * vb * n)
+        per_level.append(int(smooth + (a + vb * n) + spmv_alg_bytes(d["nnz_R"], nc, n, vp) +
                              spmv_alg_bytes(d["nnz_P"], n, nc, vp)))
     nd = amg.coarse_rows()[0]
     coarse = 4 * nd * nd + 16 * nd

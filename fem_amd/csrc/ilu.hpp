@@ -85,6 +85,7 @@ class IluSystem {
     double* part_ = nullptr;
     double* scal_ = nullptr;
     double* vbasis_ = nullptr;
+    double* hcols_ = nullptr;  // a GMRES cycle's Gram-Schmidt columns on the device, (m+2) doubles each
     int mb_ = 0;
 };
 

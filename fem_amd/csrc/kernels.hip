@@ -1224,6 +1224,46 @@ __global__ void k_plan_sum(int ne, const int64_t* __restrict__ ptr, const int* _
     }
 }
 
+// The same sums with the products formed cooperatively: a workgroup takes
+// TPB consecutive entries, whose products are contiguous in the plan; all
+// lanes gather and multiply them chunk by chunk into LDS, then each lane
+// adds its own entry's products from LDS in their listed order, so every
+// sum is the sequential one (bit-identical to k_plan_sum) while the
+// dependent gathers run in parallel.
+constexpr int PLAN_CHUNK = 4096;
+__global__ __launch_bounds__(TPB) void k_plan_sum_staged(int ne, const int64_t* __restrict__ ptr,
+                                                        const int* __restrict__ ia, const int* __restrict__ ib,
+                                                        const uint8_t* __restrict__ seed,
+                                                        const double* __restrict__ va, const double* __restrict__ vb,
+                                                        double* __restrict__ out) {
+    __shared__ double prod[PLAN_CHUNK];
+    const int e0 = blockIdx.x * TPB;
+    const int e = e0 + (int)threadIdx.x;
+    const int elast = min(e0 + TPB, ne);
+    const int64_t b0 = ptr[e0], b1 = ptr[elast];
+    const bool live = e < ne;
+    const int64_t my0 = live ? ptr[e] : 0, my1 = live ? ptr[e + 1] : 0;
+    bool started = live && seed[e];
+    double s = 0.0;
+    for (int64_t c0 = b0; c0 < b1; c0 += PLAN_CHUNK) {
+        const int64_t c1 = min(c0 + (int64_t)PLAN_CHUNK, b1);
+        for (int64_t t = c0 + threadIdx.x; t < c1; t += TPB) prod[t - c0] = va[ia[t]] * vb[ib[t]];
+        __syncthreads();
+        const int64_t lo = max(my0, c0), hi = min(my1, c1);
+        for (int64_t t = lo; t < hi; t++) {
+            const double p = prod[t - c0];
+            if (started) {
+                s += p;
+            } else {
+                s = p;
+                started = true;
+            }
+        }
+        __syncthreads();
+    }
+    if (live) out[e] = s;
+}
+
 // symmetric Dirichlet treatment keeping the pattern, one row per thread:
 // a constrained row becomes the unit row with b[r] = u[r]; any other row
 // subtracts sum over its constrained columns c (column order, from +0.0) of
@@ -2325,7 +2365,9 @@ void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* work) {
 
 void plan_sum(hipStream_t st, int ne, const int64_t* ptr, const int* ia, const int* ib, const uint8_t* seed,
               const double* va, const double* vb, double* out) {
-    if (ne > 0) hipLaunchKernelGGL(k_plan_sum, dim3(grid_for(ne)), dim3(TPB), 0, st, ne, ptr, ia, ib, seed, va, vb, out);
+    if (ne > 0)
+        hipLaunchKernelGGL(k_plan_sum_staged, dim3((ne + TPB - 1) / TPB), dim3(TPB), 0, st, ne, ptr, ia, ib, seed, va,
+                           vb, out);
 }
 
 void plan_scatter(hipStream_t st, int ne, const double* val, const int* pos, const uint8_t* lone, double* dst,

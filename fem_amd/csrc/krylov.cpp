@@ -391,6 +391,7 @@ void IluSystem::upload(bool pattern_levels) {
     hip_check(hipStreamSynchronize(stream_), "sync");
     bufs_.reset(new DevBuffers());
     vbasis_ = nullptr;
+    hcols_ = nullptr;
     mb_ = 0;
     DevBuffers& B = *bufs_;
     hipStream_t st = stream_;
@@ -546,23 +547,31 @@ struct Givens {
 // restarted GMRES(m) on A M^-1 y = b (M = ILU(0)), the caller's x = M^-1 y:
 // at each restart r = b - A M^-1 y, eta = |r|/|r_first|; a full cycle of m
 // Arnoldi steps (classical Gram-Schmidt) follows when eta > tol; the
-// least-squares update is formed at the cycle's end.  Returns the number of
-// Arnoldi steps; y is updated in place.
+// least-squares update is formed at the cycle's end.  Nothing inside a cycle
+// depends on a host decision, so its Arnoldi steps run without a host round
+// trip: each step's Gram-Schmidt coefficients and norm stay in a device
+// column, and the host reads all m columns once, at the cycle's end, to apply
+// the Givens rotations in step order.  Returns the number of Arnoldi steps; y
+// is updated in place.
 int IluSystem::gmres_dev(double* Y, const double* Bv, int m, double tol, int max_iter, double* eta_out) {
     if (!ilu_ok_) throw NumericError("ILU(0): zero pivot");
+    // the multidot partials of a step (<= 512 blocks each) fit part_ up to m = 320
+    if (m < 1 || m > 256) throw std::invalid_argument("GMRES restart length m must be in 1..256");
     hipStream_t st = stream_;
     const int n = n_;
+    const int hl = m + 2;  // one column: coefficients 0..i-1, the norm at i
     if (mb_ < m + 2) {
         // the Krylov basis as one (m+2) x n block: V_l = V + l*n
         vbasis_ = kalloc<double>((size_t)(m + 2) * n, *bufs_);
+        hcols_ = kalloc<double>((size_t)hl * hl, *bufs_);
         mb_ = m + 2;
     }
     double* V = vbasis_;
     const size_t ld = (size_t)n;
-    double* coef = scal_ + 8;  // device copy of the combination coefficients
+    double* coef = hcols_ + (size_t)m * hl;  // device copy of the update's coefficients (m <= 2 hl)
     double beta0 = 0, eta = 0;
     int iter = 0;
-    std::vector<double> H((size_t)(m + 1) * m, 0.0), col(m + 2), g(m + 2), z(m + 1);
+    std::vector<double> H((size_t)(m + 1) * m, 0.0), cols((size_t)hl * m), col(m + 2), g(m + 2), z(m + 1);
     Givens rot;
     for (;;) {
         // restart: r = b - A M^-1 y, eta = |r| / |r of the first restart|
@@ -580,29 +589,36 @@ int IluSystem::gmres_dev(double* Y, const double* Bv, int m, double tol, int max
         }
         if (eta > tol) {
             dev::axpby(st, n, 0.0, v0, 1.0 / beta, vR_);
+            for (int i = 1; i <= m; i++) {
+                double* vi = V + (size_t)(i - 1) * ld;
+                double* vn = V + (size_t)i * ld;
+                double* hc = hcols_ + (size_t)(i - 1) * hl;
+                apply(vi, vn);
+                matvec(vn, vR_);
+                // classical Gram-Schmidt: all dots against the same r
+                dev::multidot(st, n, i, V, ld, vR_, part_, hc);
+                dev::combine(st, n, i, V, ld, hc, vR_, -1.0, vn);
+                // |vn| into the column, then vn / |vn|
+                const int np = dev::dot_part<double>(st, n, vn, vn, part_);
+                dev::finalize(st, np, part_, hc, i, 1);
+                dev::scale_by(st, n, vn, hc, i, 1);
+                iter++;
+            }
+            hip_check(hipMemcpyAsync(cols.data(), hcols_, cols.size() * sizeof(double), hipMemcpyDeviceToHost, st),
+                      "d2h");
+            hip_check(hipStreamSynchronize(st), "sync");
             rot.reset();
             std::fill(g.begin(), g.end(), 0.0);
             g[0] = beta;
             for (int i = 1; i <= m; i++) {
-                double* vi = V + (size_t)(i - 1) * ld;
-                double* vn = V + (size_t)i * ld;
-                apply(vi, vn);
-                matvec(vn, vR_);
-                // classical Gram-Schmidt: all dots against the same r
-                dev::multidot(st, n, i, V, ld, vR_, part_, coef);
-                hip_check(hipMemcpyAsync(col.data(), coef, i * sizeof(double), hipMemcpyDeviceToHost, st), "d2h");
-                hip_check(hipStreamSynchronize(st), "sync");
-                dev::combine(st, n, i, V, ld, coef, vR_, -1.0, vn);
-                double nv = std::sqrt(dot(vn, vn));
-                dev::axpby(st, n, 0.0, vn, 1.0 / nv, vn);
-                col[i] = nv;
+                std::copy(cols.begin() + (size_t)(i - 1) * hl, cols.begin() + (size_t)(i - 1) * hl + i + 1,
+                          col.begin());
                 rot.push(col, i - 1);
                 for (int k = 0; k <= i - 1; k++) H[(size_t)(i - 1) * (m + 1) + k] = col[k];
                 double cc = rot.c[i - 1], ss = rot.s[i - 1];
                 double t = cc * g[i - 1] + ss * g[i];
                 g[i] = -ss * g[i - 1] + cc * g[i];
                 g[i - 1] = t;
-                iter++;
             }
             // least-squares update at the end of the cycle: R z = g, y += V z
             for (int r = m - 1; r >= 0; r--) {
