@@ -69,8 +69,8 @@ def cpu_baseline(s, amg, gpu_iters: int) -> dict:
     AMG_setup takes ~14 min here), the coarsest level solved with the dense
     inverse of its dense rows.  Timed: pcg with 2 and with 5 iterations from
     x = 0; the difference / 3 is the cost of one iteration, at 1 thread and at
-    the host's core share.  value = that cost x the GPU solve's iteration
-    count."""
+    the host's core share.  value = that cost x the iteration count of the
+    GPU solve with every level smoothed (the reference's cycle shape)."""
     from oracle.bind import REF_FAST_SO, REF_SO, Ref
     path = REF_FAST_SO if os.path.exists(REF_FAST_SO) else REF_SO
     ref = Ref(path)
@@ -97,8 +97,8 @@ def cpu_baseline(s, amg, gpu_iters: int) -> dict:
             "sample": (f"reference pcg + AMG_solve (PCGS.c, Multigrid.c; {os.path.basename(path)}) on the full "
                        f"{s.n}-DOF system and hierarchy, one symmetric lexicographic-SOR V(1,1) per iteration: "
                        f"{per[1]:.3f}s per iteration at 1 thread, {per[cores]:.3f}s at {cores}; x {gpu_iters} "
-                       f"iterations (the GPU FP64 solve's count); AMG_setup excluded (the reference's takes ~14 min "
-                       f"on one core of the build container)"),
+                       f"iterations (the GPU solve's count with every level smoothed); AMG_setup excluded (the "
+                       f"reference's takes ~14 min on one core of the build container)"),
             "per_iteration_s": {str(k): v for k, v in per.items()}, "assemble_s": t_asm}
 
 
@@ -113,7 +113,7 @@ def spmv_alg_bytes(nnz: int, rows: int, cols: int, prec: str) -> int:
     return 8 * nnz + 4 * (rows + 1) + 4 * cols + 4 * rows
 
 
-def iteration_bytes(amg, precision: str, coarse_sweeps: int = 1) -> dict:
+def iteration_bytes(amg, precision: str, coarse_sweeps: int = 1, skip_mask: int = 0) -> dict:
     """Algorithmic bytes of one AMG-PCG iteration from the actual level sizes:
     the outer FP64 SpMV, two dots (16 n each), the fused x/r update with the
     residual norm (48 n) and p = z + beta p (24 n); per level above the
@@ -124,7 +124,8 @@ def iteration_bytes(amg, precision: str, coarse_sweeps: int = 1) -> dict:
     f, the l1 diagonal and x, 3 words a row), one residual (SpMV + b), the
     restriction R and the prolongation P; the coarsest level's dense inverse
     applied once (its FP64 upper triangle, 4 nd^2, + 16 nd, nd its dense
-    rows).  The hierarchy's arithmetic is FP32 in the mixed mode."""
+    rows).  A pass-through level (bit l of skip_mask) moves only R and P.
+    The hierarchy's arithmetic is FP32 in the mixed mode."""
     top = amg.depth - 1
     L = [amg.level_sizes(l) for l in range(amg.depth)]
     n0, nnz0 = L[top]["fine"], L[top]["nnz_A"]
@@ -136,6 +137,9 @@ def iteration_bytes(amg, precision: str, coarse_sweeps: int = 1) -> dict:
         d = L[l]
         n, nc = d["fine"], d["coarse"]
         a = spmv_alg_bytes(d["nnz_A"], n, n, vp)
+        if l != top and (skip_mask >> l) & 1:
+            per_level.append(int(spmv_alg_bytes(d["nnz_R"], nc, n, vp) + spmv_alg_bytes(d["nnz_P"], n, nc, vp)))
+            continue
         if l == top:
             smooth = 2 * (a + 2 * vb * n)
         else:
@@ -562,6 +566,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10, help="relative residual target of the timed solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU baseline")
     ap.add_argument("--coarse-sweeps", type=int, default=1, help="l1-Jacobi sweeps on the levels below the finest")
+    ap.add_argument("--skip-levels", default="1,2,4,5",
+                    help="levels (0 = coarsest) the V-cycle passes through without smoothing; '' for none")
     ap.add_argument("--kernels-only", action="store_true",
                     help="no solves: a 1 GiB device copy and a few finest-level SpMVs, for counter runs")
     ap.add_argument("--workload", default="elasticity",
@@ -592,8 +598,12 @@ def main():
     if args.kernels_only:
         kernels_only(s, lina)
         return
+    skip = [int(v) for v in args.skip_levels.split(",") if v.strip()]
+    if any(not 0 < l < args.depth - 1 for l in skip):
+        sys.exit(f"--skip-levels: levels must lie in 1..{args.depth - 2}")
+    mask = sum(1 << l for l in set(skip))
     t0 = time.time()
-    prm64 = lina.params("vcycle", 2, args.depth, coarse_sweeps=args.coarse_sweeps, autotune=1)
+    prm64 = lina.params("vcycle", 2, args.depth, coarse_sweeps=args.coarse_sweeps, autotune=1, skip_levels=mask)
     amg = lina.AMG(s.rowptr, s.col, s.val, prm64)
     t_setup = time.time() - t0
     sizes = [amg.level(l)["fine"] for l in range(amg.depth)] if amg.depth < 12 else []
@@ -601,7 +611,8 @@ def main():
     lpr_top = [t for t in tuned if t[0] == amg.depth - 1 and t[1] == "A"]
     lpr64 = lpr_top[0][2] if lpr_top else 1
     log(f"AMG setup + upload {t_setup:.1f}s, level sizes {sizes}, autotuned row partitions {tuned}")
-    prm32 = lina.params("vcycle", 2, args.depth, precision="mixed", coarse_sweeps=args.coarse_sweeps, autotune=1)
+    prm32 = lina.params("vcycle", 2, args.depth, precision="mixed", coarse_sweeps=args.coarse_sweeps, autotune=1,
+                        skip_levels=mask)
 
     dx = lina.DeviceVector(s.n)
     db = lina.DeviceVector(s.n)
@@ -639,6 +650,14 @@ def main():
                      "spmv_launches": nk,
                      "iters": runs[-1]["iters"], "vcycles": runs[-1]["vcycles"], "relres": rr}
         log(f"{name}: {res[name]}")
+    # the same solve with every level smoothed, beside the pass-through cycle
+    full = None
+    if mask:
+        prm_full = lina.params("vcycle", 2, args.depth, precision="mixed", coarse_sweeps=args.coarse_sweeps,
+                               autotune=1)
+        r = solve(prm_full, args.tol)
+        full = {"ms": r["wall_ms"], "iters": r["iters"]}
+        log(f"mixed with every level smoothed: {full}")
     # error against SuperLU's FP64 direct solution (committed sketch)
     oracle = None
     if have_direct:
@@ -665,7 +684,7 @@ def main():
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     # the whole iteration against the same roofline
     for name in ("mixed", "fp64"):
-        ib = iteration_bytes(amg, name, args.coarse_sweeps)
+        ib = iteration_bytes(amg, name, args.coarse_sweeps, mask)
         ms_it = res[name]["device_ms"] / max(res[name]["iters"], 1)
         res[name]["iteration"] = {"bytes": ib["total"], "ms": ms_it,
                                   "achieved_gbs": ib["total"] / (ms_it * 1e-3) / 1e9,
@@ -688,7 +707,7 @@ def main():
         t0 = time.time()
         try:
             with _CStdoutToStderr():
-                cpu = cpu_baseline(s, amg, res["fp64"]["iters"])
+                cpu = cpu_baseline(s, amg, full["iters"] if full else res["fp64"]["iters"])
             log(f"cpu baseline {cpu['value']:.1f}s ({time.time() - t0:.0f}s wall)")
         except Exception as e:  # the baseline never blocks the GPU line
             log(f"cpu baseline skipped: {e}")
@@ -710,11 +729,14 @@ def main():
         "config": {"workload": f"degraded plane-strain elasticity P1 {args.size}x{args.size} nodes, "
                                f"AMG-PCG to relres {args.tol:g}",
                    "dof": s.n, "nnz": s.nnz, "depth": args.depth, "levels": sizes,
-                   "smoother": f"multicolour SSOR on the finest level, {args.coarse_sweeps} l1-Jacobi sweeps below",
+                   "smoother": f"multicolour SSOR on the finest level, {args.coarse_sweeps} l1-Jacobi sweeps below"
+                               + (f"; levels {sorted(set(skip))} (0 = coarsest) passed through unsmoothed"
+                                  if skip else ""),
                    "tol": args.tol},
         "mixed": res["mixed"],
         "fp64": res["fp64"],
         "speedup_mixed_vs_fp64": res["fp64"]["ms"] / res["mixed"]["ms"],
+        "mixed_every_level_smoothed": full,
         "oracle": oracle,
         "setup_s": t_setup,
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

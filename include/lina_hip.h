@@ -79,8 +79,9 @@ typedef struct lina_amg_params {
     int cycle;                     /* LINA_CYCLE_*: AMG_setup's cycle_structure (AMG_solve's cycle) */
     unsigned skip_levels;          /* bit l (level l counted from the coarsest, 0 < l < depth - 1): no
                                     * smoothing on that level in the V-cycle -- its right-hand side is
-                                    * restricted as it comes and the coarse correction prolongated back,
-                                    * which keeps the cycle symmetric and reads no A of that level (0) */
+                                    * restricted as it comes and the coarse correction prolongated back
+                                    * (x = P V(R f)), reading no A of that level; the cycle stays symmetric
+                                    * whenever the full cycle is (0) */
 } lina_amg_params;
 
 typedef struct lina_stats {

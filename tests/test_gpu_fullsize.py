@@ -63,25 +63,29 @@ def test_elasticity1025_mixed_and_fp64(gpu):
     """Both precisions to relative residual 1e-10 (at 1e-8 the error of this
     ill-conditioned system, element stiffness down to 1e-6, is still
     ~5e-7), each within 1e-8 of SuperLU's FP64 direct solution (its
-    committed sketch: 64 Gaussian projections and 20,000 sampled entries)."""
+    committed sketch: 64 Gaussian projections and 20,000 sampled entries);
+    the same for the mixed solve whose cycle passes levels 1, 2, 4 and 5
+    through unsmoothed."""
     from tests import direct_oracle as D
     s = P.degraded_elasticity(1025)
     say("elasticity1025", s.n, "nnz", s.nnz)
     prm64 = gpu.params("vcycle", 2, 8, autotune=1)
     prm32 = gpu.params("vcycle", 2, 8, precision="mixed", autotune=1)
+    prm32s = gpu.params("vcycle", 2, 8, precision="mixed", autotune=1, skip_levels=0b110110)
     amg = gpu.AMG(s.rowptr, s.col, s.val, prm64)
     A = s.scipy()
     nb = np.linalg.norm(s.b)
     out = {}
-    for name, prm in (("fp64", prm64), ("mixed", prm32)):
+    names = ("fp64", "mixed", "mixed_skip")
+    for name, prm in zip(names, (prm64, prm32, prm32s)):
         amg.set_params(prm)
         out[name] = amg.pcg(s.b, tol=1e-10, max_iter=20000, stop="relres")
         rr = np.linalg.norm(s.b - A @ out[name]["x"]) / nb
         say("  ", name, "iters", out[name]["iters"], "true relres", rr)
         assert rr <= 1.01e-10
     amg.close()
-    errs, info = D.errors(s, [out["fp64"]["x"], out["mixed"]["x"]])
-    for name, e in zip(("fp64", "mixed"), errs):
+    errs, info = D.errors(s, [out[k]["x"] for k in names])
+    for name, e in zip(names, errs):
         say("  ", name, "error vs direct solve: sketch", e["sketch"], "sampled", e["sample"], "max", e["sample_max"])
         assert e["sketch"] <= 1e-8 and e["sample"] <= 1e-8
     assert out["mixed"]["iters"] <= int(1.1 * out["fp64"]["iters"]) + 2
