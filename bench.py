@@ -566,7 +566,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10, help="relative residual target of the timed solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU baseline")
     ap.add_argument("--coarse-sweeps", type=int, default=1, help="l1-Jacobi sweeps on the levels below the finest")
-    ap.add_argument("--skip-levels", default="1,2,4,5",
+    ap.add_argument("--skip-levels", default="1,2,4",
                     help="levels (0 = coarsest) the V-cycle passes through without smoothing; '' for none")
     ap.add_argument("--kernels-only", action="store_true",
                     help="no solves: a 1 GiB device copy and a few finest-level SpMVs, for counter runs")

@@ -2154,6 +2154,36 @@ template void mcsor_colour<double, double>(hipStream_t, const DMat<double>&, int
 template void mcsor_colour<float, float>(hipStream_t, const DMat<float>&, int, int, const int*, const float*, float*,
                                          float);
 
+// the first colour's sweep from a zero iterate, with the iterate's other
+// entries zeroed: every product of the sweep is a_ij * 0, so row r of the
+// first colour gets x_i = (1 - omega) * 0 + omega * f_i / a_ii (k_mcsor's
+// value, up to the sign of a zero) and every other row 0
+template <typename VT, typename XT>
+__global__ void k_mcsor_first(int n, int r1, const int* __restrict__ perm, const int* __restrict__ dpos,
+                              const VT* __restrict__ v, const XT* __restrict__ f, XT* __restrict__ x, XT omega) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        const int i = perm[r];
+        XT xi = (XT)0;
+        if (r < r1) {
+            const int dp = dpos[r];
+            const XT d = dp >= 0 ? (XT)v[dp] : (XT)0;
+            xi = ((XT)1 - omega) * (XT)0 + omega * f[i] / d;
+        }
+        x[i] = xi;
+    }
+}
+
+template <typename VT, typename XT>
+void mcsor_first(hipStream_t st, const DMat<VT>& Ac, int rows0, const int* perm, const XT* f, XT* x, XT omega) {
+    if (Ac.n == 0) return;
+    hipLaunchKernelGGL((k_mcsor_first<VT, XT>), dim3(grid_for(Ac.n)), dim3(TPB), 0, st, Ac.n, rows0, perm, Ac.dpos,
+                       Ac.v, f, x, omega);
+}
+template void mcsor_first<double, double>(hipStream_t, const DMat<double>&, int, const int*, const double*, double*,
+                                          double);
+template void mcsor_first<float, float>(hipStream_t, const DMat<float>&, int, const int*, const float*, float*,
+                                        float);
+
 template <typename VT, typename XT>
 void sor_lex(hipStream_t st, const DMat<VT>& A, int nlv, const int* lv_off, const int* lv_rows, const XT* f, XT* x,
              XT* xo, XT omega, int sweeps, int backward) {

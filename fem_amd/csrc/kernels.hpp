@@ -191,6 +191,11 @@ void axpby(hipStream_t st, int n, double a, double* x, double b, const double* y
 template <typename VT, typename XT>
 void mcsor_colour(hipStream_t st, const DMat<VT>& Ac, int blk0, int nblk, const int* perm, const XT* f, XT* x,
                   XT omega);
+// the first colour's sweep from a zero iterate (rows [0, rows0) of Ac), the
+// other rows' entries of x zeroed -- the fill and the first colour launch in
+// one elementwise pass
+template <typename VT, typename XT>
+void mcsor_first(hipStream_t st, const DMat<VT>& Ac, int rows0, const int* perm, const XT* f, XT* x, XT omega);
 template <typename VT, typename XT>
 void sor_lex(hipStream_t st, const DMat<VT>& A, int nlv, const int* lv_off, const int* lv_rows, const XT* f, XT* x,
              XT* xo, XT omega, int sweeps, int backward);

@@ -223,9 +223,11 @@ static void upload_mat(const Csr& A, const std::vector<int>& starts_with_end, de
     // a square matrix whose halves are coupled (a quarter or more of its
     // entries join a row of one half to a column of the other: the [ux; uy]
     // ordering) has its row kernels give each XCD matching slices of both
-    // halves
+    // halves (LINA_XCD_SPLIT=0 at upload: contiguous ranges only)
     D.bsplit = 0;
-    if (!row_ids && !seg && A.n == A.ncols && A.n >= 8192 && D.nblk >= 16) {
+    const char* xs = std::getenv("LINA_XCD_SPLIT");
+    const bool split_ok = !(xs && std::atoi(xs) == 0);
+    if (split_ok && !row_ids && !seg && A.n == A.ncols && A.n >= 8192 && D.nblk >= 16) {
         const int h = A.n / 2;
         long cross = 0;
         for (int r = 0; r < A.n; r++)
@@ -1184,7 +1186,13 @@ void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post, int xinit) {
     const bool jac = is_jacobi(smoother);
     if ((xinit == X_J0 || xinit == X_SPARE) && !(jac && sweeps > 0))
         throw std::logic_error("smooth: a fused first step without a Jacobi sweep");
-    if (xinit == X_ZERO && !(jac && sweeps > 0)) dev::fill<T>(st, d.n, x, (T)0);
+    const bool mc = smoother == LINA_SMOOTHER_MC_SOR || smoother == LINA_SMOOTHER_MC_SSOR;
+    // a forward multicolour sweep from zero starts with one elementwise pass
+    // (the first colour's values, zeros elsewhere); a batch's skipped row
+    // blocks keep the zero fill instead
+    const bool mc_first = mc && xinit == X_ZERO && sweeps > 0 && d.ncolours > 0 &&
+                          !(smoother == LINA_SMOOTHER_MC_SSOR && post) && pickAc<T>(d).bspec == nullptr;
+    if (xinit == X_ZERO && !(jac && sweeps > 0) && !mc_first) dev::fill<T>(st, d.n, x, (T)0);
     switch (smoother) {
         case LINA_SMOOTHER_SOR:
         case LINA_SMOOTHER_SSOR: {
@@ -1206,6 +1214,10 @@ void Solver::smooth(int l, const T* f, T* x, int sweeps, bool post, int xinit) {
             for (int s = 0; s < sweeps; s++)
                 for (int cc = 0; cc < d.ncolours; cc++) {
                     int c = back ? d.ncolours - 1 - cc : cc;
+                    if (s == 0 && cc == 0 && mc_first) {
+                        dev::mcsor_first(st, pickAc<T>(d), d.colour_rows[1], d.perm, f, x, omega);
+                        continue;
+                    }
                     int b0 = cb[c], b1 = cb[c + 1];
                     dev::mcsor_colour(st, pickAc<T>(d), b0, b1 - b0, d.perm, f, x, omega);
                 }

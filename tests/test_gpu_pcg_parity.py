@@ -177,6 +177,32 @@ def test_pcg_vcycle_default(gpu, kind, m):
             assert np.linalg.norm(s.b - A @ g["x"]) / np.linalg.norm(s.b) < 1e-6
 
 
+@pytest.mark.parametrize("case,sm,alpha", [("poisson33", "sor", 1.2), ("poisson33", "mc_ssor", 1.0),
+                                            ("elast25", "ssor", 1.0), ("elast25", "mc_ssor", 1.0)])
+def test_pcg_full_multigrid_preconditioner(gpu, case, sm, alpha):
+    """pcg preconditioned by AMG_solve of a FULL_AMG hierarchy (one
+    full-multigrid pass per preconditioner call), against the oracle's pcg
+    with the same preconditioner (pinned bit for bit to the compiled
+    reference): FP64 iterations within 1 and iterates to 1e-12; the mixed
+    hierarchy reaches the same true residual."""
+    s, depth = {"poisson33": (P.poisson(33), 4), "elast25": (P.degraded_elasticity(25), 4)}[case]
+    say("pcg full-multigrid preconditioner", case, sm)
+    r = _ref_solve(s, depth, smoother=sm, alpha=alpha, tol=1e-10, max_iter=300, cycle="full")
+    A = s.scipy()
+    for prec in ("fp64", "mixed"):
+        amg = gpu.AMG(s.rowptr, s.col, s.val, gpu.params("reference", s.dof, depth, smoother=sm, omega=alpha,
+                                                          cycle="full", precision=prec))
+        g = amg.pcg(s.b, tol=1e-10, max_iter=300)
+        amg.close()
+        say("  ", prec, "iters oracle", r["iters"], "gpu", g["iters"], "rel", rel(g["x"], r["x"]))
+        if prec == "fp64":
+            assert abs(g["iters"] - r["iters"]) <= 1
+            assert rel(g["x"], r["x"]) < 1e-12
+        else:
+            assert g["iters"] <= int(1.25 * r["iters"]) + 2
+            assert np.linalg.norm(s.b - A @ g["x"]) <= 1e-8 * np.linalg.norm(s.b)
+
+
 @pytest.mark.parametrize("smoother", ["mc_sor", "l1_jacobi", "ssor"])
 def test_amg_solve_other_smoothers(gpu, smoother):
     s = P.poisson(33)
