@@ -12,8 +12,11 @@ square (2,101,250 DOF, blocked [ux; uy]), E = 210e3, nu = 0.3, stiffness scaled
 per element by (1-d)^2 + 1e-6 with d = exp(-dist/0.01) to a seeded notch of
 length 0.3 from the left edge; bottom clamped, top u_y = 1e-3.  Preconditioner:
 the LinA AMG hierarchy (aggressive first coarsening, bit-identical to the
-reference's AMG_setup) with one symmetric multicolour-SOR V(1,1)-cycle per
-CG iteration.  A "step" is one complete AMG-PCG solve from x0 = 0 to
+reference's AMG_setup) with one symmetric V-cycle per CG iteration:
+multicolour SOR forward and backward on the finest level, one l1-Jacobi
+sweep on the levels below, levels 1, 2 and 4 (0 = coarsest) passed through
+unsmoothed (--skip-levels; the full cycle's mixed solve is reported beside
+it).  A "step" is one complete AMG-PCG solve from x0 = 0 to
 |b-Ax|/|b| <= tol, with A, b and the hierarchy resident in HBM.
 
 Prints one JSON line on stdout; progress goes to stderr.

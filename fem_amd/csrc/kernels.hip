@@ -44,21 +44,12 @@ constexpr int WAVE = 64;
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
 // L2).  Remap so that XCD x processes one contiguous range of logical blocks:
 // neighbouring rows (and the x entries they gather) then share an L2.
-__device__ __forceinline__ int xcd_block(int bid, int nblk, int bsplit = 0) {
+__device__ __forceinline__ int xcd_block(int bid, int nblk) {
     constexpr int X = 8;
     if (nblk < 2 * X) return bid;
     int q = nblk / X, r = nblk % X;
     int x = bid % X, k = bid / X;
-    const int vs = x * q + min(x, r);
-    if (bsplit <= 0 || bsplit >= nblk) return vs + k;
-    // two coupled halves (rows [0, n/2) and [n/2, n) with blocks [0, bsplit)
-    // and [bsplit, nblk), each row's columns near its own row in both
-    // halves): XCD x takes a proportional slice of each half at the same
-    // place, so the vector entries it gathers are fetched by its L2 alone
-    const int ve = vs + q + (x < r ? 1 : 0);
-    const int s1 = (int)((long long)bsplit * vs / nblk);
-    const int a = (int)((long long)bsplit * ve / nblk) - s1;
-    return k < a ? s1 + k : bsplit + (vs - s1) + (k - a);
+    return x * q + min(x, r) + k;
 }
 
 template <typename T>
@@ -198,8 +189,8 @@ __device__ __forceinline__ XT spmv_out(XT s, XT base, int r, XT* y, const Epi<VT
 // (G < nblk, G a multiple of 8) gives XCD x the contiguous block range
 // [x*chunk, (x+1)*chunk) and walks it with the XCD's G/8 workgroups, so that
 // workgroups stay resident instead of being dispatched once per block.
-__device__ __forceinline__ int grid_block(int b, int G, int k, int nblk, int bsplit) {
-    if (G >= nblk) return k == 0 ? xcd_block(b, nblk, bsplit) : -1;
+__device__ __forceinline__ int grid_block(int b, int G, int k, int nblk) {
+    if (G >= nblk) return k == 0 ? xcd_block(b, nblk) : -1;
     constexpr int X = 8;
     const int per = G / X, x = b % X, j = b / X;
     const int chunk = (nblk + X - 1) / X;
@@ -276,8 +267,7 @@ __device__ __forceinline__ void spmv_block(int bb, const int2* __restrict__ bk, 
 }
 
 template <typename VT, typename XT, int MODE, int LPR>
-__global__ __launch_bounds__(TPB) void k_spmv(int nblk, int bsplit, const int2* __restrict__ bk,
-                                              const int* __restrict__ rp,
+__global__ __launch_bounds__(TPB) void k_spmv(int nblk, const int2* __restrict__ bk, const int* __restrict__ rp,
                                               const int* __restrict__ ci, const uint16_t* __restrict__ cs,
                                               const int2* __restrict__ cbase, const VT* __restrict__ v,
                                               const XT* __restrict__ x, const XT* __restrict__ b,
@@ -286,7 +276,7 @@ __global__ __launch_bounds__(TPB) void k_spmv(int nblk, int bsplit, const int2* 
                                               const int2* __restrict__ bspec, const double* __restrict__ active) {
     __shared__ XT prod[block_nnz<VT>()];
     for (int it = 0;; it++) {
-        const int bb = grid_block(blockIdx.x, gridDim.x, it, nblk, bsplit);
+        const int bb = grid_block(blockIdx.x, gridDim.x, it, nblk);
         if (bb < 0) break;
         if (!block_live(bspec, active, bb)) continue;
         spmv_block<VT, XT, MODE, LPR>(bb, bk, rp, ci, cs, cbase, v, x, b, y, part, w, epi, prod);
@@ -657,8 +647,7 @@ __global__ __launch_bounds__(1024) void k_sor_lex(int n, const int* __restrict__
 //             beyond (AMG_parallel_Jacobi_smoother, Multigrid.c:1367-1400)
 //   JAC_L1  : x_i = xo_i + (f_i - sum_j a_ij xo_j)/sum_j |a_ij|
 template <typename VT, typename XT, int KIND, int LPR>
-__global__ __launch_bounds__(TPB) void k_jacobi(int nblk, int bsplit, const int2* __restrict__ bk,
-                                                const int* __restrict__ rp,
+__global__ __launch_bounds__(TPB) void k_jacobi(int nblk, const int2* __restrict__ bk, const int* __restrict__ rp,
                                                 const int* __restrict__ ci, const uint16_t* __restrict__ cs,
                                                 const int2* __restrict__ cbase, const VT* __restrict__ v,
                                                 const int* __restrict__ dpos, const VT* __restrict__ l1,
@@ -667,7 +656,7 @@ __global__ __launch_bounds__(TPB) void k_jacobi(int nblk, int bsplit, const int2
                                                 const double* __restrict__ active) {
     constexpr int BN = block_nnz<VT>();
     __shared__ XT prod[BN];
-    const int bb = xcd_block(blockIdx.x, nblk, bsplit);
+    const int bb = xcd_block(blockIdx.x, nblk);
     if (!block_live(bspec, active, bb)) return;
     const int2 s0 = bk[bb], s1 = bk[bb + 1];
     const int r0 = s0.x, r1 = s1.x, k0 = s0.y, k1 = s1.y;
@@ -1950,7 +1939,7 @@ int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int
     dim3 g(row_grid(A.nblk)), t(TPB);
     const Epi<VT, XT> e = epi ? *epi : Epi<VT, XT>{};
 #define LINA_SPMV_MODE(M)                                                                                       \
-    LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, M, L>), g, t, 0, st, A.nblk, A.bsplit, A.bk, A.rp, A.ci, \
+    LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_spmv<VT, XT, M, L>), g, t, 0, st, A.nblk, A.bk, A.rp, A.ci, \
                                                 A.cs, A.cb, A.v, x, b, y, part, w, e, A.bspec, A.active))
     switch (mode) {
         case MODE_SET: LINA_SPMV_MODE(MODE_SET); break;
@@ -2201,11 +2190,11 @@ void jacobi(hipStream_t st, const DMat<VT>& A, int kind, const XT* f, const XT* 
     if (A.nblk == 0) return;
     dim3 g(A.nblk), t(TPB);
     if (kind == JAC_REF) {
-        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_REF, L>), g, t, 0, st, A.nblk, A.bsplit, A.bk, A.rp,
+        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_REF, L>), g, t, 0, st, A.nblk, A.bk, A.rp,
                                                     A.ci, A.cs, A.cb, A.v, A.dpos, A.l1, f, xo, x, nlim, A.bspec,
                                                     A.active))
     } else {
-        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_L1, L>), g, t, 0, st, A.nblk, A.bsplit, A.bk, A.rp,
+        LINA_LPR_DISPATCH(A.lpr, hipLaunchKernelGGL((k_jacobi<VT, XT, JAC_L1, L>), g, t, 0, st, A.nblk, A.bk, A.rp,
                                                     A.ci, A.cs, A.cb, A.v, A.dpos, A.l1, f, xo, x, nlim, A.bspec,
                                                     A.active))
     }

@@ -76,10 +76,6 @@ struct DMat {
     int2* bspec = nullptr;
     const double* active = nullptr;
     int lpr = 1;         // lanes reducing one row from LDS: 1 (sequential), 4, 8, 16, 32
-    // > 0: the rows form two coupled halves (the [ux; uy] ordering of a
-    // two-field system) whose blocks start at bsplit; the row kernels give
-    // each XCD matching slices of both halves
-    int bsplit = 0;
     int* dpos = nullptr; // position of each row's diagonal entry in ci/v (-1: none)
     VT* l1 = nullptr;    // sum_j |a_ij| of each row (l1-Jacobi), summed in column order
 };

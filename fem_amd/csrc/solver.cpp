@@ -220,24 +220,6 @@ static void upload_mat(const Csr& A, const std::vector<int>& starts_with_end, de
         }
     }
     D.lpr = dev::lanes_per_row(A.n > 0 ? (double)A.nnz() / A.n : 0.0);
-    // a square matrix whose halves are coupled (a quarter or more of its
-    // entries join a row of one half to a column of the other: the [ux; uy]
-    // ordering) has its row kernels give each XCD matching slices of both
-    // halves (LINA_XCD_SPLIT=0 at upload: contiguous ranges only)
-    D.bsplit = 0;
-    const char* xs = std::getenv("LINA_XCD_SPLIT");
-    const bool split_ok = !(xs && std::atoi(xs) == 0);
-    if (split_ok && !row_ids && !seg && A.n == A.ncols && A.n >= 8192 && D.nblk >= 16) {
-        const int h = A.n / 2;
-        long cross = 0;
-        for (int r = 0; r < A.n; r++)
-            for (int p = A.rp[r]; p < A.rp[r + 1]; p++) cross += (r < h) != (A.ci[p] < h);
-        if (4 * cross >= A.nnz()) {
-            int b = 0;
-            while (b < D.nblk && starts_with_end[b] < h) b++;
-            if (b > 0 && b < D.nblk) D.bsplit = b;
-        }
-    }
     // diagonal positions (row r's own index: row_ids == nullptr) and l1 norms
     std::vector<int> dpos(A.n, -1);
     std::vector<VT> l1(A.n, (VT)0);
