@@ -623,6 +623,39 @@ lina_ilu* lina_ilu_create(int n, const int* rowptr, const int* col, const double
     return h;
 }
 
+lina_ilu* lina_ilu_create_ordered(int n, const int* rowptr, const int* col, const double* val, int ordering,
+                                  int upload) {
+    lina_ilu* h = nullptr;
+    int rc = guard([&]() {
+        if (ordering != LINA_ILU_NATURAL && ordering != LINA_ILU_MULTICOLOUR)
+            throw std::invalid_argument("unknown ILU ordering");
+        Csr A = make_csr(n, rowptr, col, val);
+        h = new lina_ilu();
+        h->s.set_ordering(ordering);
+        h->s.build(A, false);
+        if (upload) {
+            if (lina_device_count() < 1) throw lina::HipError("no HIP device available");
+            h->s.upload();
+        }
+        return 0;
+    });
+    if (rc < 0) {
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+int lina_ilu_permutation(const lina_ilu* h, int* perm) {
+    return guard([&]() {
+        if (!h || !perm) throw std::invalid_argument("null argument");
+        const std::vector<int>& p = h->s.permutation();
+        const int n = h->s.n();
+        for (int r = 0; r < n; r++) perm[r] = p.empty() ? r : p[r];
+        return 0;
+    });
+}
+
 void lina_ilu_destroy(lina_ilu* h) { delete h; }
 
 int lina_ilu_ok(const lina_ilu* h) { return h->s.ilu_ok() ? 1 : 0; }

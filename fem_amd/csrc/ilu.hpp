@@ -28,13 +28,24 @@ class IluSystem {
     // in each row, U: diagonal first); only the preconditioner is available
     void set_factors(const Csr& L, const Csr& U, bool upload = true);
     void upload(bool pattern_levels = false);
+    // 0: the natural order (the reference's factors); 1: multicolour order --
+    // the rows and columns permuted by a greedy colouring of A's pattern
+    // (colour by colour, natural order within a colour), so that both
+    // triangular sweeps and the factorisation run in a few wide levels on
+    // the whole GPU.  The factors, the operator and the Krylov iterations are
+    // those of P A P^T (factors exactly the reference's incomplete LU of
+    // P A P^T); every vector entry point takes and returns the caller's
+    // (natural) order.  Takes effect at the next build / refactor.
+    void set_ordering(int ordering);
+    int ordering() const { return ordering_; }
+    const std::vector<int>& permutation() const { return perm_; }  // empty in the natural order
     bool ilu_ok() const { return ilu_ok_ != 0; }
     // the factors (downloaded first when they were computed on the device)
     const Csr& L() { sync_factors(); return L_; }
     const Csr& U() { sync_factors(); return U_; }
     int n() const { return n_; }
 
-    void apply(const double* z, double* w);  // w = U^-1 L^-1 z (device)
+    void apply(const double* z, double* w);  // w = U^-1 L^-1 z (device, the caller's order)
     void matvec(const double* x, double* y);
     double dot(const double* a, const double* b);
     int pcg_dev(double* X, double* B, double tol, int max_iter, int stop, double* residuum);
@@ -56,6 +67,19 @@ class IluSystem {
     };
 
   private:
+    void build_p(const Csr& A, bool upload);
+    void refactor_p(const Csr& A);
+    void apply_p(const double* z, double* w);  // in the factors' (permuted) order
+    int pcg_core(double* X, double* B, double tol, int max_iter, int stop, double* residuum);
+    int gmres_core(double* Y, const double* B, int m, double tol, int max_iter, double* eta);
+    Csr permuted(const Csr& A);  // P A P^T (the permutation kept while A's pattern holds)
+    int ordering_ = 0;
+    std::vector<int> perm_, vpos_;  // row r of P A P^T is row perm_[r] of A; its entry k is A's entry vpos_[k]
+    Csr opat_, ppat_;               // A's pattern the permutation was built for, and P A P^T's pattern
+    int* dperm_ = nullptr;
+    double *vXo_ = nullptr, *vBo_ = nullptr, *vPz_ = nullptr, *vPw_ = nullptr;
+    bool wide_ = false;             // few, wide levels: a launch per level on the whole GPU
+    std::vector<int> hoff_u_;
     void sync_factors();
     void pack_values();
     bool build_plan();

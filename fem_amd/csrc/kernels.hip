@@ -1122,6 +1122,48 @@ __global__ __launch_bounds__(1024) void k_ilu0(int nlv, const int* __restrict__ 
 // factors are bit-identical): per A entry m in column order, s = sum over
 // the mask's slots q ascending of L(i,k_q) U(k_q,j); m < nl: L = a - s;
 // the diagonal: L(i,i) = 0 + (a - s); above it: U = (a - s) / L(i,i).
+// one row (position t of L's level schedule) of the planned factorisation
+__device__ __forceinline__ void ilu0_planned_row(int t, const int4* __restrict__ m1, const int4* __restrict__ m2,
+                                                 const int* __restrict__ mask, const int* __restrict__ pu,
+                                                 const double* __restrict__ vA, double* vL, double* vU,
+                                                 int* __restrict__ fail) {
+    const int4 a = m1[t], b = m2[t];
+    const int a0 = a.x, na = a.y, l0 = a.z, u0 = a.w, nl = b.x, hasd = b.y;
+    int r = b.z, pw = u0 + 1;
+    double Lr[ILU_MAXL];
+#pragma unroll
+    for (int q = 0; q < ILU_MAXL; q++) Lr[q] = 0.0;
+    double dg = 0.0;
+    vU[u0] = 1.0;
+#pragma unroll
+    for (int m = 0; m < ILU_MAXA; m++) {
+        if (m < na) {
+            const double av = vA[a0 + m];
+            const int mk = mask[a0 + m];
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < ILU_MAXL; q++)
+                if ((mk >> q) & 1) {
+                    s += Lr[q] * vU[pu[r]];
+                    r++;
+                }
+            if (m < ILU_MAXL && m < nl) {
+                Lr[m < ILU_MAXL ? m : 0] = av - s;
+            } else if (hasd && m == nl) {
+                dg = 0.0 + (av - s);
+            } else {
+                if (dg != 0.0) vU[pw] = (av - s) / dg;
+                else *fail = 1;
+                pw++;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < ILU_MAXL; q++)
+        if (q < nl) vL[l0 + q] = Lr[q];
+    vL[l0 + nl] = dg;
+}
+
 __global__ __launch_bounds__(1024) void k_ilu0_planned(int nlv, const int* __restrict__ lv_off,
                                                        const int4* __restrict__ m1, const int4* __restrict__ m2,
                                                        const int* __restrict__ mask, const int* __restrict__ pu,
@@ -1129,44 +1171,50 @@ __global__ __launch_bounds__(1024) void k_ilu0_planned(int nlv, const int* __res
                                                        int* __restrict__ fail) {
     for (int L = 0; L < nlv; L++) {
         const int e = lv_off[L + 1];
-        for (int t = lv_off[L] + threadIdx.x; t < e; t += blockDim.x) {
-            const int4 a = m1[t], b = m2[t];
-            const int a0 = a.x, na = a.y, l0 = a.z, u0 = a.w, nl = b.x, hasd = b.y;
-            int r = b.z, pw = u0 + 1;
-            double Lr[ILU_MAXL];
-#pragma unroll
-            for (int q = 0; q < ILU_MAXL; q++) Lr[q] = 0.0;
-            double dg = 0.0;
-            vU[u0] = 1.0;
-#pragma unroll
-            for (int m = 0; m < ILU_MAXA; m++) {
-                if (m < na) {
-                    const double av = vA[a0 + m];
-                    const int mk = mask[a0 + m];
-                    double s = 0.0;
-#pragma unroll
-                    for (int q = 0; q < ILU_MAXL; q++)
-                        if ((mk >> q) & 1) {
-                            s += Lr[q] * vU[pu[r]];
-                            r++;
-                        }
-                    if (m < ILU_MAXL && m < nl) {
-                        Lr[m < ILU_MAXL ? m : 0] = av - s;
-                    } else if (hasd && m == nl) {
-                        dg = 0.0 + (av - s);
-                    } else {
-                        if (dg != 0.0) vU[pw] = (av - s) / dg;
-                        else *fail = 1;
-                        pw++;
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < ILU_MAXL; q++)
-                if (q < nl) vL[l0 + q] = Lr[q];
-            vL[l0 + nl] = dg;
-        }
+        for (int t = lv_off[L] + threadIdx.x; t < e; t += blockDim.x) ilu0_planned_row(t, m1, m2, mask, pu, vA, vL, vU, fail);
         __syncthreads();
+    }
+}
+
+// the same for one wide level [t0, t1) on a full grid (a launch per level:
+// the rows of a level depend only on earlier levels)
+__global__ __launch_bounds__(TPB) void k_ilu0_planned_level(int t0, int t1, const int4* __restrict__ m1,
+                                                            const int4* __restrict__ m2, const int* __restrict__ mask,
+                                                            const int* __restrict__ pu, const double* __restrict__ vA,
+                                                            double* vL, double* vU, int* __restrict__ fail) {
+    for (int t = t0 + blockIdx.x * TPB + threadIdx.x; t < t1; t += gridDim.x * TPB)
+        ilu0_planned_row(t, m1, m2, mask, pu, vA, vL, vU, fail);
+}
+
+// One level [t0, t1) of a triangular sweep on a full grid, a row per
+// thread, each row's arithmetic that of PCG_preconl / PCG_preconr: forward
+// (L, diagonal last) s = z_i, s -= l_ij y_j in column order, y_i = s / l_ii;
+// backward (U, diagonal first) s = y_i, s -= u_ij w_j, w_i = s / u_ii.
+__global__ __launch_bounds__(TPB) void k_tri_level(int t0, int t1, const int* __restrict__ rows,
+                                                   const int* __restrict__ rp, const int* __restrict__ ci,
+                                                   const double* __restrict__ v, const double* __restrict__ in,
+                                                   double* out, int upper) {
+    for (int t = t0 + blockIdx.x * TPB + threadIdx.x; t < t1; t += gridDim.x * TPB) {
+        const int i = rows[t];
+        const int b = rp[i], e = rp[i + 1];
+        double s = in[i];
+        if (!upper) {
+            for (int p = b; p < e - 1; p++) s -= v[p] * out[ci[p]];
+            s /= v[e - 1];
+        } else {
+            for (int p = b + 1; p < e; p++) s -= v[p] * out[ci[p]];
+            s /= v[b];
+        }
+        out[i] = s;
+    }
+}
+
+// out[r] = in[perm[r]] (inverse == 0) or out[perm[r]] = in[r] (inverse == 1)
+__global__ void k_permute(int n, const int* __restrict__ perm, const double* __restrict__ in, double* __restrict__ out,
+                          int inverse) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        if (inverse) out[perm[r]] = in[r];
+        else out[r] = in[perm[r]];
     }
 }
 
@@ -2314,6 +2362,31 @@ void tri_pack_values(hipStream_t st, int n, int maxp, const int* vmap, const int
 void ilu0_planned(hipStream_t st, int nlv, const int* lv_off, const int4* m1, const int4* m2, const int* mask,
                   const int* pu, const double* vA, double* vL, double* vU, int* fail) {
     hipLaunchKernelGGL(k_ilu0_planned, dim3(1), dim3(1024), 0, st, nlv, lv_off, m1, m2, mask, pu, vA, vL, vU, fail);
+}
+
+void ilu0_planned_levels(hipStream_t st, const std::vector<int>& off, const int4* m1, const int4* m2,
+                         const int* mask, const int* pu, const double* vA, double* vL, double* vU, int* fail) {
+    for (size_t L = 0; L + 1 < off.size(); L++) {
+        const int cnt = off[L + 1] - off[L];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(k_ilu0_planned_level, dim3(grid_for(cnt)), dim3(TPB), 0, st, off[L], off[L + 1], m1, m2,
+                           mask, pu, vA, vL, vU, fail);
+    }
+}
+
+void tri_levels_wide(hipStream_t st, const int* rp, const int* ci, const double* v, const std::vector<int>& off,
+                     const int* rows, const double* in, double* out, int upper) {
+    for (size_t L = 0; L + 1 < off.size(); L++) {
+        const int cnt = off[L + 1] - off[L];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(k_tri_level, dim3(grid_for(cnt)), dim3(TPB), 0, st, off[L], off[L + 1], rows, rp, ci, v,
+                           in, out, upper);
+    }
+}
+
+void permute(hipStream_t st, int n, const int* perm, const double* in, double* out, int inverse) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_permute, dim3(grid_for(n)), dim3(TPB), 0, st, n, perm, in, out, inverse);
 }
 
 void ilu0_numeric(hipStream_t st, int nlv, const int* lv_off, const int* lv_rows, const int* rpA, const int* ciA,

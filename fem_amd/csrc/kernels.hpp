@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <cstdint>
 
 namespace lina {
@@ -297,6 +299,17 @@ void tri_pack_values(hipStream_t st, int n, int maxp, const int* vmap, const int
 // entry indices listed in pu (slot order, entry after entry).  At most
 // ILU_MAXA entries a row and ILU_MAXL lower ones.
 constexpr int ILU_MAXA = 16, ILU_MAXL = 8;
+// the planned factorisation one level per launch, each on a full grid (for
+// schedules of few, wide levels; off: the host copy of the level offsets)
+void ilu0_planned_levels(hipStream_t st, const std::vector<int>& off, const int4* m1, const int4* m2,
+                         const int* mask, const int* pu, const double* vA, double* vL, double* vU, int* fail);
+// a triangular sweep one level per launch (rows: the level-ordered rows of
+// the factor, off: host level offsets); the arithmetic of PCG_preconl
+// (upper = 0, diagonal last) or PCG_preconr (upper = 1, diagonal first)
+void tri_levels_wide(hipStream_t st, const int* rp, const int* ci, const double* v, const std::vector<int>& off,
+                     const int* rows, const double* in, double* out, int upper);
+// out[r] = in[perm[r]] (inverse 0), out[perm[r]] = in[r] (inverse 1)
+void permute(hipStream_t st, int n, const int* perm, const double* in, double* out, int inverse);
 void ilu0_planned(hipStream_t st, int nlv, const int* lv_off, const int4* m1, const int4* m2, const int* mask,
                   const int* pu, const double* vA, double* vL, double* vU, int* fail);
 // level-scheduled ILU(0) numeric factorisation into fixed L/U patterns (see

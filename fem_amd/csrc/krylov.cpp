@@ -202,7 +202,76 @@ IluSystem::~IluSystem() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+void IluSystem::set_ordering(int ordering) {
+    if (ordering != 0 && ordering != 1) throw std::invalid_argument("unknown ILU ordering");
+    if (ordering != ordering_) {
+        ordering_ = ordering;
+        device_ready_ = false;
+        opat_ = Csr();
+        ppat_ = Csr();
+        perm_.clear();
+        vpos_.clear();
+    }
+}
+
+// P A P^T for the multicolour order: P from the greedy colouring of A's
+// pattern, explicit zeros included (rows colour by colour, natural order
+// within a colour), each permuted row's entries sorted by their new column
+Csr IluSystem::permuted(const Csr& A) {
+    const bool same = A.n == opat_.n && A.rp == opat_.rp && A.ci == opat_.ci;
+    if (!same) {
+        const int n = A.n;
+        int nc = 0;
+        // colour the pattern, explicit zeros included: ILU(0) may fill them,
+        // and then rows of one colour would depend on each other
+        Csr pat;
+        pat.n = pat.ncols = n;
+        pat.rp = A.rp;
+        pat.ci = A.ci;
+        pat.v.assign(A.ci.size(), 1.0);
+        const std::vector<int> col = greedy_colouring(pat, &nc);
+        std::vector<int> cnt(nc + 1, 0), iperm(n);
+        for (int i = 0; i < n; i++) cnt[col[i] + 1]++;
+        for (int c = 0; c < nc; c++) cnt[c + 1] += cnt[c];
+        perm_.assign(n, 0);
+        for (int i = 0; i < n; i++) perm_[cnt[col[i]]++] = i;
+        for (int r = 0; r < n; r++) iperm[perm_[r]] = r;
+        ppat_ = Csr();
+        ppat_.n = ppat_.ncols = n;
+        ppat_.rp.assign(n + 1, 0);
+        ppat_.ci.resize(A.nnz());
+        vpos_.resize(A.nnz());
+        std::vector<std::pair<int, int>> row;
+        for (int r = 0; r < n; r++) {
+            const int i = perm_[r];
+            row.clear();
+            for (int p = A.rp[i]; p < A.rp[i + 1]; p++) row.emplace_back(iperm[A.ci[p]], p);
+            std::sort(row.begin(), row.end());
+            int k = ppat_.rp[r];
+            for (const auto& e : row) {
+                ppat_.ci[k] = e.first;
+                vpos_[k++] = e.second;
+            }
+            ppat_.rp[r + 1] = k;
+        }
+        opat_ = Csr();
+        opat_.n = opat_.ncols = n;
+        opat_.rp = A.rp;
+        opat_.ci = A.ci;
+        device_ready_ = false;
+    }
+    Csr P = ppat_;
+    P.v.resize(vpos_.size());
+    for (size_t k = 0; k < vpos_.size(); k++) P.v[k] = A.v[vpos_[k]];
+    return P;
+}
+
 void IluSystem::build(const Csr& A, bool upload) {
+    if (ordering_) build_p(permuted(A), upload);
+    else build_p(A, upload);
+}
+
+void IluSystem::build_p(const Csr& A, bool upload) {
     A_ = A;
     device_ready_ = false;
     host_factors_ = true;
@@ -243,12 +312,17 @@ static bool ilu_patterns(const Csr& A, Csr& L, Csr& U) {
 }
 
 void IluSystem::refactor(const Csr& A) {
+    if (ordering_) refactor_p(permuted(A));
+    else refactor_p(A);
+}
+
+void IluSystem::refactor_p(const Csr& A) {
     const bool same = device_ready_ && A.n == A_.n && A.rp == A_.rp && A.ci == A_.ci;
     if (!same) {
         Csr L, U;
         if (!ilu_patterns(A, L, U)) {
             device_ready_ = false;
-            build(A, true);
+            build_p(A, true);
             return;
         }
         static const bool timing = getenv("LINA_RN_TIMING") != nullptr;
@@ -273,7 +347,9 @@ void IluSystem::refactor(const Csr& A) {
     }
     host_[0] = 0.0;
     hip_check(hipMemcpyAsync(fail_, host_, sizeof(int), hipMemcpyHostToDevice, stream_), "h2d");
-    if (plan_)
+    if (plan_ && wide_)
+        dev::ilu0_planned_levels(stream_, hoff_l_, pm1_, pm2_, pmask_, ppu_, dA_.v, dL_.v, dU_.v, fail_);
+    else if (plan_)
         dev::ilu0_planned(stream_, nlv_l_, lvl_off_, pm1_, pm2_, pmask_, ppu_, dA_.v, dL_.v, dU_.v, fail_);
     else
         dev::ilu0_numeric(stream_, nlv_l_, lvl_off_, lvl_rows_, dA_.rp, dA_.ci, dA_.v, dL_.rp, dL_.ci, dL_.v,
@@ -368,6 +444,7 @@ void IluSystem::sync_factors() {
 
 void IluSystem::set_factors(const Csr& L, const Csr& U, bool up) {
     if (L.n != U.n) throw std::invalid_argument("L and U sizes differ");
+    set_ordering(0);  // factors handed over are in the caller's order
     A_ = Csr();
     device_ready_ = false;
     host_factors_ = true;
@@ -380,7 +457,12 @@ void IluSystem::set_factors(const Csr& L, const Csr& U, bool up) {
 
 // pattern_levels: level schedules from the factors' patterns (their values
 // are computed on the device afterwards)
-void IluSystem::upload(bool pattern_levels) {
+void IluSystem::upload(bool pattern_levels_in) {
+    // the multicolour order's wide sweeps read every entry of a row, zeros
+    // included, so their levels come from the patterns (a level schedule
+    // from the values lets a row read an entry that a zero factor value
+    // ties to a later row)
+    const bool pattern_levels = pattern_levels_in || ordering_ != 0;
     if (!stream_) {
         hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipHostMalloc((void**)&host_, 64 * sizeof(double)), "hipHostMalloc");
@@ -411,8 +493,18 @@ void IluSystem::upload(bool pattern_levels) {
     nlv_u_ = (int)offu.size() - 1;
     lvu_off_ = kupload(offu, B, st);
     lvu_rows_ = kupload(rowsu, B, st);
+    hoff_u_ = offu;
     const int n = n_;
     for (double** p : {&vX_, &vB_, &vR_, &vP_, &vQ_, &vZ_, &vW_, &vY_, &vT_}) *p = kalloc<double>(n, B);
+    dperm_ = nullptr;
+    vXo_ = vBo_ = vPz_ = vPw_ = nullptr;
+    if (ordering_ && (int)perm_.size() == n) {
+        dperm_ = kupload(perm_, B, st);
+        for (double** p : {&vXo_, &vBo_, &vPz_, &vPw_}) *p = kalloc<double>(n, B);
+    }
+    // a few wide levels (the multicolour order's): one launch per level on
+    // the whole GPU instead of the one-workgroup level packs
+    wide_ = pattern_levels && n >= 4096 && nlv_l_ <= 64 && nlv_u_ <= 64;
     // level packs: the forward sweep writes y in the backward sweep's order
     std::vector<int> posu(n), natural(n);
     for (int t = 0; t < n; t++) posu[rowsu[t]] = t;
@@ -420,7 +512,7 @@ void IluSystem::upload(bool pattern_levels) {
     tf_ = TriDev();
     tb_ = TriDev();
     const int cl = tri_max_count(L_, false, pattern_levels), cu = tri_max_count(U_, true, pattern_levels);
-    packed_ = n > 0 && cl >= 0 && cu >= 0;
+    packed_ = n > 0 && cl >= 0 && cu >= 0 && !wide_;
     if (packed_) {
         // one record width for both sweeps (one kernel runs them); 48-byte
         // records when every row fits 3 entries whose sources are in the ring
@@ -455,9 +547,24 @@ void IluSystem::pack_values() {
 }
 
 void IluSystem::apply(const double* z, double* w) {
+    if (!dperm_) {
+        apply_p(z, w);
+        return;
+    }
+    dev::permute(stream_, n_, dperm_, z, vPz_, 0);
+    apply_p(vPz_, vPw_);
+    dev::permute(stream_, n_, dperm_, vPw_, w, 1);
+}
+
+void IluSystem::apply_p(const double* z, double* w) {
     // PCG_preconlr: y = L^-1 z, w = U^-1 y
     if (packed_) {
         dev::tri_solve_pair(stream_, tf_.p, tb_.p, permL_, z, vZp_, vYu_, w);
+        return;
+    }
+    if (wide_) {
+        dev::tri_levels_wide(stream_, dL_.rp, dL_.ci, dL_.v, hoff_l_, lvl_rows_, z, vT_, 0);
+        dev::tri_levels_wide(stream_, dU_.rp, dU_.ci, dU_.v, hoff_u_, lvu_rows_, vT_, w, 1);
         return;
     }
     dev::trsv(stream_, dL_.rp, dL_.ci, dL_.v, nlv_l_, lvl_off_, lvl_rows_, z, vT_, 0);
@@ -476,8 +583,20 @@ void IluSystem::matvec(const double* x, double* y) {
     dev::spmv<double, double>(stream_, dA_, x, nullptr, y, dev::MODE_SET, nullptr, nullptr);
 }
 
-// pcg (PCGS.c:163-233) with precon = PCG_preconlr and mat_mult = PCG_mult
 int IluSystem::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, double* residuum) {
+    if (!dperm_) return pcg_core(X, B, tol, max_iter, stop, residuum);
+    if (!ilu_ok_) throw NumericError("ILU(0): zero pivot");
+    dev::permute(stream_, n_, dperm_, X, vXo_, 0);
+    dev::permute(stream_, n_, dperm_, B, vBo_, 0);
+    const int it = pcg_core(vXo_, vBo_, tol, max_iter, stop, residuum);
+    dev::permute(stream_, n_, dperm_, vXo_, X, 1);
+    dev::permute(stream_, n_, dperm_, vBo_, B, 1);
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    return it;
+}
+
+// pcg (PCGS.c:163-233) with precon = PCG_preconlr and mat_mult = PCG_mult
+int IluSystem::pcg_core(double* X, double* B, double tol, int max_iter, int stop, double* residuum) {
     if (!ilu_ok_) throw NumericError("ILU(0): zero pivot");
     hipStream_t st = stream_;
     const int n = n_;
@@ -488,13 +607,13 @@ int IluSystem::pcg_dev(double* X, double* B, double tol, int max_iter, int stop,
     dev::axpby(st, n, inv, X, 0.0, X);
     double *R = vR_, *P = vP_, *Q = vQ_, *Z = vZ_, *W = vW_;
     dev::spmv<double, double>(st, dA_, X, B, R, dev::MODE_RES, nullptr, nullptr);
-    apply(R, W);
+    apply_p(R, W);
     double r0 = std::sqrt(dot(W, W));
     if (r0 == 0) return 0;
     double r = r0, s = 0, s_old = 0;
     int i = 0;
     do {
-        apply(R, Z);
+        apply_p(R, Z);
         s_old = s;
         s = dot(R, Z);
         if (i > 0) {
@@ -510,7 +629,7 @@ int IluSystem::pcg_dev(double* X, double* B, double tol, int max_iter, int stop,
         r = std::sqrt(dot(R, R));
         i++;
     } while ((stop == LINA_STOP_RELRES ? r : r / r0) > tol && i < max_iter);
-    apply(R, W);
+    apply_p(R, W);
     double rw = std::sqrt(dot(W, W));
     dev::axpby(st, n, h, B, 0.0, B);
     dev::axpby(st, n, h, X, 0.0, X);
@@ -554,6 +673,17 @@ struct Givens {
 // the Givens rotations in step order.  Returns the number of Arnoldi steps; y
 // is updated in place.
 int IluSystem::gmres_dev(double* Y, const double* Bv, int m, double tol, int max_iter, double* eta_out) {
+    if (!dperm_) return gmres_core(Y, Bv, m, tol, max_iter, eta_out);
+    if (!ilu_ok_) throw NumericError("ILU(0): zero pivot");
+    dev::permute(stream_, n_, dperm_, Y, vXo_, 0);
+    dev::permute(stream_, n_, dperm_, Bv, vBo_, 0);
+    const int it = gmres_core(vXo_, vBo_, m, tol, max_iter, eta_out);
+    dev::permute(stream_, n_, dperm_, vXo_, Y, 1);
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    return it;
+}
+
+int IluSystem::gmres_core(double* Y, const double* Bv, int m, double tol, int max_iter, double* eta_out) {
     if (!ilu_ok_) throw NumericError("ILU(0): zero pivot");
     // the multidot partials of a step (<= 512 blocks each) fit part_ up to m = 320
     if (m < 1 || m > 256) throw std::invalid_argument("GMRES restart length m must be in 1..256");
@@ -576,7 +706,7 @@ int IluSystem::gmres_dev(double* Y, const double* Bv, int m, double tol, int max
     for (;;) {
         // restart: r = b - A M^-1 y, eta = |r| / |r of the first restart|
         double* v0 = V;
-        apply(Y, v0);
+        apply_p(Y, v0);
         matvec(v0, vW_);
         dev::axpby(st, n, 0.0, vR_, 1.0, Bv);
         dev::axpby(st, n, 1.0, vR_, -1.0, vW_);
@@ -593,7 +723,7 @@ int IluSystem::gmres_dev(double* Y, const double* Bv, int m, double tol, int max
                 double* vi = V + (size_t)(i - 1) * ld;
                 double* vn = V + (size_t)i * ld;
                 double* hc = hcols_ + (size_t)(i - 1) * hl;
-                apply(vi, vn);
+                apply_p(vi, vn);
                 matvec(vn, vR_);
                 // classical Gram-Schmidt: all dots against the same r
                 dev::multidot(st, n, i, V, ld, vR_, part_, hc);

@@ -334,6 +334,7 @@ class ReflectiveNewton {
         up_.assign(up, up + nb);
         for (int k : idx_)
             if (k < 0 || k >= n_) throw std::invalid_argument("bound index out of range");
+        ilu_.set_ordering(p.ilu_ordering);
         dH_.reset(new DevMatrix(H_));
         st_ = dH_->stream();
         B_.reset(new DevBuffers());
@@ -788,6 +789,7 @@ void lina_rn_params_default(lina_rn_params* p, int dof, int depth) {
     p->q_offset = 0.0;
     p->host_solver = nullptr;
     p->host_ctx = nullptr;
+    p->ilu_ordering = LINA_ILU_NATURAL;
 }
 
 lina_rn* lina_rn_create(int n, const int* rowptr, const int* col, const double* val, const double* b, int nb,
@@ -798,6 +800,8 @@ lina_rn* lina_rn_create(int n, const int* rowptr, const int* col, const double* 
             throw std::invalid_argument("lina_rn_create: bad arguments");
         if (p->solver < LINA_RN_BICGSTAB || p->solver > LINA_RN_HOST) throw std::invalid_argument("unknown solver");
         if (p->solver == LINA_RN_HOST && !p->host_solver) throw std::invalid_argument("LINA_RN_HOST needs host_solver");
+        if (p->ilu_ordering != LINA_ILU_NATURAL && p->ilu_ordering != LINA_ILU_MULTICOLOUR)
+            throw std::invalid_argument("unknown ILU ordering");
         if (p->solver == LINA_RN_GMRES && (p->gmres_m < 1 || p->gmres_max_iter < 1))
             throw std::invalid_argument("GMRES restart length and iteration limit must be positive");
         lina::Csr H;

@@ -46,7 +46,7 @@ class RNParams(C.Structure):
                 ("solver", C.c_int), ("pcg_iter", C.c_int), ("bi_eps", C.c_double), ("bi_max_iter", C.c_int),
                 ("gmres_m", C.c_int), ("gmres_max_iter", C.c_int), ("gmres_tol", C.c_double),
                 ("amg", Params), ("seed", C.c_ulonglong), ("q_offset", C.c_double), ("host_solver", C.c_void_p),
-                ("host_ctx", C.c_void_p)]
+                ("host_ctx", C.c_void_p), ("ilu_ordering", C.c_int)]
 
 
 class RNStats(C.Structure):
@@ -454,6 +454,10 @@ def _bind_krylov(L):
         return
     L.lina_ilu_create.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int]
     L.lina_ilu_create.restype = C.c_void_p
+    L.lina_ilu_create_ordered.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int]
+    L.lina_ilu_create_ordered.restype = C.c_void_p
+    L.lina_ilu_permutation.argtypes = [C.c_void_p, _ip]
+    L.lina_ilu_apply_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.lina_ilu_destroy.argtypes = [C.c_void_p]
     L.lina_ilu_ok.argtypes = [C.c_void_p]
     L.lina_ilu_refactor.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp]
@@ -466,20 +470,32 @@ def _bind_krylov(L):
     L._krylov_bound = True
 
 
+ILU_ORDER = {"natural": 0, "multicolour": 1}
+
+
 class ILU:
     """ILU(0) factors of A with device triangular solves (lina_ilu handle)."""
 
-    def __init__(self, rowptr, col, val, upload: bool = True):
+    def __init__(self, rowptr, col, val, upload: bool = True, ordering: str = "natural"):
+        """ordering "multicolour": the factors of P A P^T (P from a greedy
+        colouring of A's pattern), vectors in A's order"""
         L = lib()
         _bind_krylov(L)
         self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
         self.col = np.ascontiguousarray(col, dtype=np.int32)
         self.val = np.ascontiguousarray(val, dtype=np.float64)
         self.n = len(self.rowptr) - 1
-        h = L.lina_ilu_create(self.n, _i(self.rowptr), _i(self.col), _d(self.val), 1 if upload else 0)
+        h = L.lina_ilu_create_ordered(self.n, _i(self.rowptr), _i(self.col), _d(self.val), ILU_ORDER[ordering],
+                                      1 if upload else 0)
         if not h:
             raise LinaError(f"lina_ilu_create failed: {L.lina_last_error().decode()}")
         self.h = h
+
+    def permutation(self):
+        """perm[r] = the row of A at position r of the factors"""
+        p = np.zeros(self.n, np.int32)
+        _check(lib().lina_ilu_permutation(self.h, _i(p)))
+        return p
 
     def close(self):
         if getattr(self, "h", None):
@@ -507,6 +523,10 @@ class ILU:
         v = np.zeros(max(nnz, 1))
         lib().lina_ilu_factor_copy(self.h, which.encode(), _i(rp), _i(ci), _d(v))
         return rp, ci[:nnz], v[:nnz]
+
+    def apply_device(self, d_z: int, d_w: int):
+        """w = U^-1 L^-1 z on device vectors (DeviceVector ptrs)"""
+        _check(lib().lina_ilu_apply_device(self.h, C.c_void_p(d_z), C.c_void_p(d_w)))
 
     def apply(self, z):
         z = np.ascontiguousarray(z, dtype=np.float64)
@@ -547,6 +567,8 @@ def rn_params(solver="bicgstab", dof=1, depth=6, **kw) -> RNParams:
         if k == "amg":
             for kk, vv in v.items():
                 setattr(p.amg, kk, vv)
+        elif k == "ilu_ordering" and isinstance(v, str):
+            p.ilu_ordering = ILU_ORDER[v]
         else:
             setattr(p, k, v)
     return p

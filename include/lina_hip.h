@@ -172,6 +172,22 @@ int lina_bicgstab(lina_solver* s, double* x, const double* b, double eps, int ma
 typedef struct lina_ilu lina_ilu;
 lina_ilu* lina_ilu_create(int n, const int* rowptr, const int* col, const double* val, int upload);
 void lina_ilu_destroy(lina_ilu* h);
+/* Row orders of an ILU(0) system.  NATURAL: the reference's factors of A
+ * (level-scheduled sweeps: ~2 sqrt(n) levels on a 2-D mesh, one workgroup).
+ * MULTICOLOUR (no counterpart in the reference; for callers that can trade
+ * the natural order's factors for sweeps on the whole GPU): A's rows and
+ * columns permuted by a greedy colouring of its pattern, explicit zeros
+ * included (colour by colour,
+ * natural order within a colour); the factors are exactly the reference's
+ * incomplete LU of P A P^T, whose sweeps have as many levels as colours;
+ * pcg / gmres iterate on P A P^T.  Vectors are always passed in A's order;
+ * the factors (lina_ilu_factor_copy) are P A P^T's. */
+#define LINA_ILU_NATURAL 0
+#define LINA_ILU_MULTICOLOUR 1
+lina_ilu* lina_ilu_create_ordered(int n, const int* rowptr, const int* col, const double* val, int ordering,
+                                  int upload);
+/* perm[r] = the row of A at position r of the factors (identity when natural) */
+int lina_ilu_permutation(const lina_ilu* h, int* perm);
 int lina_ilu_ok(const lina_ilu* h); /* 0: zero pivot, as the reference's return flag */
 /* incomplete_LU_factorization (linear_algebra.c:1952-1982) of a new matrix on
  * the device, bit-identical to the host factorisation (the CSR's rows sorted
@@ -371,6 +387,7 @@ typedef struct lina_rn_params {
                           * the relative reduction |dq/q| of the outer test depends on */
     lina_rn_host_solver host_solver; /* LINA_RN_HOST */
     void* host_ctx;
+    int ilu_ordering;    /* LINA_ILU_* of the PCG / GMRES inner solvers' ILU(0) (natural) */
 } lina_rn_params;
 
 typedef struct lina_rn_stats {

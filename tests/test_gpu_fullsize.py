@@ -91,19 +91,21 @@ def test_elasticity1025_mixed_and_fp64(gpu):
     assert out["mixed"]["iters"] <= int(1.1 * out["fp64"]["iters"]) + 2
 
 
-@pytest.mark.parametrize("solver,prec", [("pcg", "fp64"), ("gmres", "fp64"), ("bicgstab", "mixed")])
-def test_damage1025_reflective_newton_kkt(gpu, solver, prec):
+@pytest.mark.parametrize("solver,prec,order", [("pcg", "fp64", "natural"), ("gmres", "fp64", "natural"),
+                                              ("bicgstab", "mixed", "natural"), ("pcg", "fp64", "multicolour")])
+def test_damage1025_reflective_newton_kkt(gpu, solver, prec, order):
     """The damage subproblem's reflective Newton with each inner solver:
     ILU(0)-PCG; GMRES(30) + ILU(0) (FEM.c:737-744); BiCGStab + AMG
     (reflective_newton.c:205-217) with the FP32 hierarchy (multicolour SOR,
-    the reference's AMG_solve settings) -- the scaling, reflections and
-    active-set tests stay FP64 in every variant."""
+    the reference's AMG_solve settings); ILU(0)-PCG in multicolour order --
+    the scaling, reflections and active-set tests stay FP64 in every
+    variant."""
     s, dold, ones = P.damage_system(1025, h_scale=1.0)
-    say("damage1025", s.n, "nnz", s.nnz, "inner", solver, prec)
+    say("damage1025", s.n, "nnz", s.nnz, "inner", solver, prec, order)
     amg = {"precision": 1 if prec == "mixed" else 0}
     if solver == "bicgstab":
         amg["smoother"] = 2  # MC-SOR
-    prm = gpu.rn_params(solver, 1, 6, delta=float(np.sqrt(s.n) / 2), amg=amg)
+    prm = gpu.rn_params(solver, 1, 6, delta=float(np.sqrt(s.n) / 2), amg=amg, ilu_ordering=order)
     if solver == "gmres":
         assert prm.gmres_m == 30
     rn = gpu.ReflectiveNewton(s.rowptr, s.col, s.val, s.b, dold, ones, prm)

@@ -59,3 +59,54 @@ def test_batch_rejects_unsupported_cycle_settings(field, value):
     assert not h
     msg = L.lina_last_error().decode()
     assert ("V-cycle" in msg) if field != "smoother" else ("smoother" in msg)
+
+
+def _permuted(rp, ci, v, perm):
+    """P A P^T: row r is row perm[r] of A, columns renumbered and sorted"""
+    n = len(rp) - 1
+    iperm = np.empty(n, np.int64)
+    iperm[perm] = np.arange(n)
+    prp = np.zeros(n + 1, np.int32)
+    pci, pv = [], []
+    for r in range(n):
+        i = perm[r]
+        c = iperm[ci[rp[i]:rp[i + 1]]]
+        o = np.argsort(c, kind="stable")
+        pci.append(c[o])
+        pv.append(v[rp[i]:rp[i + 1]][o])
+        prp[r + 1] = prp[r] + len(o)
+    return prp, np.concatenate(pci).astype(np.int32), np.concatenate(pv)
+
+
+@pytest.mark.parametrize("kind", ["poisson", "damage", "elasticity"])
+def test_multicolour_ilu_is_the_reference_ilu_of_the_permuted_matrix(ref, kind):
+    """LINA_ILU_MULTICOLOUR: the permutation sorts the rows by a proper
+    colouring of A's pattern, explicit zeros included (no two rows of one
+    colour coupled: every colour's diagonal block of P A P^T is diagonal), the factors are the compiled reference's
+    incomplete_LU_factorization of P A P^T bit for bit, and L's level
+    schedule has no more levels than there are colours."""
+    from oracle.bind import Oracle
+    s = {"poisson": lambda: P.poisson(33), "damage": lambda: P.damage_system(33)[0],
+         "elasticity": lambda: P.degraded_elasticity(17)}[kind]()
+    il = lina.ILU(s.rowptr, s.col, s.val, upload=False, ordering="multicolour")
+    perm = il.permutation()
+    ncol, col = Oracle().colouring(s.rowptr, s.col, np.ones(len(s.col)))
+    assert np.array_equal(perm, np.argsort(col, kind="stable"))
+    prp, pci, pv = _permuted(s.rowptr, s.col, s.val, perm)
+    cs = col[perm]
+    for r in range(s.n):
+        same = cs[pci[prp[r]:prp[r + 1]]] == cs[r]
+        assert np.array_equal(pci[prp[r]:prp[r + 1]][same], [r])
+    M = ref.matrix(prp, pci, pv)
+    ok, Lr, Ur = ref.ilu(M)
+    ref.free(M)
+    assert ok and il.ok
+    for w, R in (("L", Lr), ("U", Ur)):
+        rp, ci, v = il.factor(w)
+        assert np.array_equal(rp, R.rowptr) and np.array_equal(ci, R.col) and np.array_equal(v, R.val)
+    rp, ci, _ = il.factor("L")
+    lev = np.zeros(s.n, np.int64)
+    for i in range(s.n):
+        lower = ci[rp[i]:rp[i + 1] - 1]
+        lev[i] = 1 + lev[lower].max() if len(lower) else 0
+    assert lev.max() + 1 <= ncol
