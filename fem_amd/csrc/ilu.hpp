@@ -24,6 +24,10 @@ class IluSystem {
     // stays the same, so a refactorisation uploads A's values and runs the
     // numeric kernel only.  Unsorted rows fall back to build().
     void refactor(const Csr& A);
+    // the same numeric factorisation from values already in device memory
+    // (d_val: the last refactor's matrix, in its own order and pattern);
+    // false, changing nothing, when no device structure is in place
+    bool refactor_device(const double* d_val);
     // PCG_default(A, L, U) with factors computed elsewhere (L: diagonal last
     // in each row, U: diagonal first); only the preconditioner is available
     void set_factors(const Csr& L, const Csr& U, bool upload = true);
@@ -77,6 +81,8 @@ class IluSystem {
     std::vector<int> perm_, vpos_;  // row r of P A P^T is row perm_[r] of A; its entry k is A's entry vpos_[k]
     Csr opat_, ppat_;               // A's pattern the permutation was built for, and P A P^T's pattern
     int* dperm_ = nullptr;
+    int* dvpos_ = nullptr;          // vpos_ on the device (values of A into P A P^T)
+    void numeric();
     double *vXo_ = nullptr, *vBo_ = nullptr, *vPz_ = nullptr, *vPw_ = nullptr;
     bool wide_ = false;             // few, wide levels: a launch per level on the whole GPU
     std::vector<int> hoff_u_;

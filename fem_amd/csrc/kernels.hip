@@ -1393,6 +1393,89 @@ __global__ void k_scatter(int n, const int* __restrict__ idx, const double* __re
 // set idx[0..nb) with bounds lo/up (NaN: unbounded side).
 
 // RN_set_D_J (reflective_newton.c:227-248); D = 1, J = 0 beforehand
+// The reflective Newton's scaled Hessian M = D H D + diag(|g| J) on the
+// device, one row per thread, with the host assembly's rules (RN_get_M_D's
+// two products with their zero deletions, then insert_sparse of |g_i| J_i):
+// over the row's positions in F = H's pattern plus the diagonal (fh: H has
+// the entry, fhv its value):
+//   HD: off-diagonal (i,k) present iff h_ik != 0 and d_k != 0, value
+//       h_ik d_k; the diagonal always, 0.0 + (h_ii d_i or 0) (a 0.0 seed
+//       without h_ii); then the zeros deleted when the row has more than one
+//       entry, the diagonal 0.0 kept when nothing else remains;
+//   M:  off-diagonal present iff d_i != 0 (its HD value is nonzero), value
+//       d_i (h_ik d_k); the diagonal 0.0 + d_i b (b its HD value, both
+//       nonzero) or a 0.0 seed; the same zero deletion; then |g_i| J_i added
+//       to the diagonal (inserted when J_i != 0 and no diagonal is left).
+// mpos: each F position's entry in the cached pattern of M (-1: absent);
+// a position whose presence differs sets *flag (the caller assembles on the
+// host); present values go to out[mpos].
+__device__ __forceinline__ double rn_hd(int i, int k, bool has, double a, const double* __restrict__ D) {
+    const double b = D[k];
+    const double val = (has && a != 0 && b != 0) ? a * b : 0.0;
+    return k == i ? (has ? 0.0 + val : 0.0) : val;
+}
+
+__global__ void k_rn_scaled_hessian(int n, const int* __restrict__ frp, const int* __restrict__ fci,
+                                    const double* __restrict__ fhv, const uint8_t* __restrict__ fh,
+                                    const double* __restrict__ D, const double* __restrict__ G,
+                                    const double* __restrict__ J, const int* __restrict__ mpos,
+                                    double* __restrict__ out, int* __restrict__ flag) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int b0 = frp[i], b1 = frp[i + 1];
+        const double di = D[i];
+        // HD: entries present, nonzero entries, the diagonal's value
+        int hd_len = 0, hd_nz = 0;
+        double hd_diag = 0.0;
+        for (int p = b0; p < b1; p++) {
+            const int k = fci[p];
+            const bool has = fh[p] != 0;
+            const double a = has ? fhv[p] : 0.0;
+            const double v = rn_hd(i, k, has, a, D);
+            const bool pres = k == i || (has && a != 0 && D[k] != 0);
+            if (!pres) continue;
+            hd_len++;
+            hd_nz += v != 0;
+            if (k == i) hd_diag = v;
+        }
+        // after HD's deletion: the diagonal kept with value hd_diag, or 0.0
+        const bool hd_keep_diag = hd_len == 1 || hd_diag != 0 || hd_nz == 0;
+        // M before its deletion: the diagonal always, off-diagonals iff
+        // di != 0 (over HD's kept, nonzero off-diagonals)
+        const double m_diag = (hd_keep_diag && di != 0 && hd_diag != 0) ? 0.0 + di * hd_diag : 0.0;
+        int m_len = 1, m_nz = m_diag != 0;
+        for (int p = b0; p < b1; p++) {
+            const int k = fci[p];
+            if (k == i || !fh[p]) continue;
+            const double v = rn_hd(i, k, true, fhv[p], D);
+            if (!(fhv[p] != 0 && D[k] != 0) || v == 0 || di == 0) continue;
+            m_len++;
+            m_nz += di * v != 0;
+        }
+        const bool keep_diag = m_len == 1 || m_diag != 0 || m_nz == 0;
+        const double add = J[i] != 0 ? fabs(G[i]) * J[i] : 0.0;
+        bool bad = false;
+        for (int p = b0; p < b1; p++) {
+            const int k = fci[p];
+            bool pres;
+            double val;
+            if (k == i) {
+                pres = keep_diag || J[i] != 0;
+                val = keep_diag ? (J[i] != 0 ? m_diag + add : m_diag) : add;
+            } else {
+                const bool has = fh[p] != 0;
+                const double v = has ? rn_hd(i, k, true, fhv[p], D) : 0.0;
+                const bool hd_kept = has && fhv[p] != 0 && D[k] != 0 && v != 0;
+                val = di * v;
+                pres = hd_kept && di != 0 && val != 0;
+            }
+            const int q = mpos[p];
+            if (pres != (q >= 0)) bad = true;
+            else if (pres) out[q] = val;
+        }
+        if (bad) *flag = 1;
+    }
+}
+
 __global__ void k_rn_dj(int nb, const int* __restrict__ idx, const double* __restrict__ lo,
                         const double* __restrict__ up, const double* __restrict__ G, const double* __restrict__ X,
                         double* __restrict__ D, double* __restrict__ J) {
@@ -2125,6 +2208,13 @@ void bpcg_check(hipStream_t st, int ninst, const double* rr, const double* hb, d
                            iters, count);
 }
 
+void rn_scaled_hessian(hipStream_t st, int n, const int* frp, const int* fci, const double* fhv, const uint8_t* fh,
+                       const double* D, const double* G, const double* J, const int* mpos, double* out, int* flag) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_rn_scaled_hessian, dim3(grid_for(n)), dim3(TPB), 0, st, n, frp, fci, fhv, fh, D, G, J,
+                           mpos, out, flag);
+}
+
 void rn_dj(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, const double* G,
            const double* X, double* D, double* J) {
     if (nb > 0) hipLaunchKernelGGL(k_rn_dj, dim3(grid_for(nb)), dim3(TPB), 0, st, nb, idx, lo, up, G, X, D, J);
@@ -2382,6 +2472,11 @@ void tri_levels_wide(hipStream_t st, const int* rp, const int* ci, const double*
         hipLaunchKernelGGL(k_tri_level, dim3(grid_for(cnt)), dim3(TPB), 0, st, off[L], off[L + 1], rows, rp, ci, v,
                            in, out, upper);
     }
+}
+
+void permute_idx(hipStream_t st, int n, const int* idx, const double* in, double* out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_permute, dim3(grid_for(n)), dim3(TPB), 0, st, n, idx, in, out, 0);
 }
 
 void permute(hipStream_t st, int n, const int* perm, const double* in, double* out, int inverse) {

@@ -164,6 +164,12 @@ void rn_dj(hipStream_t st, int nb, const int* idx, const double* lo, const doubl
            const double* X, double* D, double* J);
 void rn_reflect(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, double* X);
 void rn_snap(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, double* X, double eps);
+// M = D H D + diag(|g| J) over F = H's pattern plus the diagonal (fh: H has
+// the position, fhv: its value), the values of the positions present in the
+// cached pattern of M written to out[mpos]; *flag set when the presence of
+// any position differs from the cache (flag is not cleared here)
+void rn_scaled_hessian(hipStream_t st, int n, const int* frp, const int* fci, const double* fhv, const uint8_t* fh,
+                       const double* D, const double* G, const double* J, const int* mpos, double* out, int* flag);
 // block partial counts of entries within 1e-30 of a bound; returns #partials
 int rn_touch(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, const double* X,
              double* part);
@@ -310,6 +316,8 @@ void tri_levels_wide(hipStream_t st, const int* rp, const int* ci, const double*
                      const int* rows, const double* in, double* out, int upper);
 // out[r] = in[perm[r]] (inverse 0), out[perm[r]] = in[r] (inverse 1)
 void permute(hipStream_t st, int n, const int* perm, const double* in, double* out, int inverse);
+// out[k] = in[idx[k]], k < n (a gather of n entries)
+void permute_idx(hipStream_t st, int n, const int* idx, const double* in, double* out);
 void ilu0_planned(hipStream_t st, int nlv, const int* lv_off, const int4* m1, const int4* m2, const int* mask,
                   const int* pu, const double* vA, double* vL, double* vU, int* fail);
 // level-scheduled ILU(0) numeric factorisation into fixed L/U patterns (see

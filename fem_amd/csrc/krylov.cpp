@@ -325,7 +325,8 @@ void IluSystem::refactor_p(const Csr& A) {
             build_p(A, true);
             return;
         }
-        static const bool timing = getenv("LINA_RN_TIMING") != nullptr;
+        const char* tenv = getenv("LINA_RN_TIMING");
+        const bool timing = tenv && atoi(tenv) != 0;
         const auto t0 = std::chrono::steady_clock::now();
         A_ = A;
         L_ = std::move(L);
@@ -345,6 +346,20 @@ void IluSystem::refactor_p(const Csr& A) {
         hip_check(hipMemcpyAsync(dA_.v, A.v.data(), A.v.size() * sizeof(double), hipMemcpyHostToDevice, stream_),
                   "h2d");
     }
+    numeric();
+}
+
+bool IluSystem::refactor_device(const double* d_val) {
+    if (!device_ready_ || !stream_) return false;
+    const int nnz = A_.nnz();
+    if (dvpos_) dev::permute_idx(stream_, nnz, dvpos_, d_val, dA_.v);
+    else hip_check(hipMemcpyAsync(dA_.v, d_val, (size_t)nnz * sizeof(double), hipMemcpyDeviceToDevice, stream_), "d2d");
+    numeric();
+    return true;
+}
+
+// the numeric factorisation of dA_'s values on the symbolic structure in place
+void IluSystem::numeric() {
     host_[0] = 0.0;
     hip_check(hipMemcpyAsync(fail_, host_, sizeof(int), hipMemcpyHostToDevice, stream_), "h2d");
     if (plan_ && wide_)
@@ -497,9 +512,11 @@ void IluSystem::upload(bool pattern_levels_in) {
     const int n = n_;
     for (double** p : {&vX_, &vB_, &vR_, &vP_, &vQ_, &vZ_, &vW_, &vY_, &vT_}) *p = kalloc<double>(n, B);
     dperm_ = nullptr;
+    dvpos_ = nullptr;
     vXo_ = vBo_ = vPz_ = vPw_ = nullptr;
     if (ordering_ && (int)perm_.size() == n) {
         dperm_ = kupload(perm_, B, st);
+        if ((int)vpos_.size() == A_.nnz() && A_.nnz() > 0) dvpos_ = kupload(vpos_, B, st);
         for (double** p : {&vXo_, &vBo_, &vPz_, &vPw_}) *p = kalloc<double>(n, B);
     }
     // a few wide levels (the multicolour order's): one launch per level on

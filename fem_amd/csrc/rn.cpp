@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <vector>
@@ -187,8 +188,8 @@ double now_s() {
 
 // LINA_RN_TIMING set: one line per Newton step with the time of its phases
 bool rn_timing() {
-    static const bool v = getenv("LINA_RN_TIMING") != nullptr;
-    return v;
+    const char* e = getenv("LINA_RN_TIMING");  // read per call: a process may switch it
+    return e && atoi(e) != 0;
 }
 
 uint64_t splitmix(uint64_t& s) {
@@ -366,6 +367,41 @@ class ReflectiveNewton {
             amg_->setup_host(H_, p_.amg);
             amg_->upload();
         }
+        if (p_.solver == LINA_RN_PCG || p_.solver == LINA_RN_GMRES) {
+            // F = H's pattern plus the diagonal, for the device assembly of M
+            F_.n = F_.ncols = n_;
+            F_.rp.assign(n_ + 1, 0);
+            std::vector<double> fhv;
+            std::vector<uint8_t> fh;
+            for (int i = 0; i < n_; i++) {
+                bool diag = false;
+                for (int q = H_.rp[i]; q < H_.rp[i + 1]; q++) {
+                    const int k = H_.ci[q];
+                    if (!diag && k >= i) {
+                        if (k > i) {
+                            F_.ci.push_back(i);
+                            fhv.push_back(0.0);
+                            fh.push_back(0);
+                        }
+                        diag = true;
+                    }
+                    F_.ci.push_back(k);
+                    fhv.push_back(H_.v[q]);
+                    fh.push_back(1);
+                }
+                if (!diag) {
+                    F_.ci.push_back(i);
+                    fhv.push_back(0.0);
+                    fh.push_back(0);
+                }
+                F_.rp[i + 1] = (int)F_.ci.size();
+            }
+            frp_ = dupload(F_.rp, *B_, st_);
+            fci_ = dupload(F_.ci, *B_, st_);
+            fhv_ = dupload(fhv, *B_, st_);
+            fh_ = dupload(fh, *B_, st_);
+            mflag_ = dalloc<int>(1, *B_);
+        }
         hip_check(hipStreamSynchronize(st_), "rn upload");
     }
 
@@ -391,16 +427,35 @@ class ReflectiveNewton {
             dev::fill<double>(st_, n, D_, 1.0);
             dev::fill<double>(st_, n, J_, 0.0);
             dev::rn_dj(st_, (int)idx_.size(), idxd_, lod_, upd_, G_, X_, D_, J_);
-            download(D_, hD);
-            download(G_, hG);
-            download(J_, hJ);
             const double t0 = now_s();
-            Csr M = scaled_hessian(H_, hD, hG, hJ);
+            // M on the device while its pattern is the cached one (the same
+            // values as the host assembly), else assembled on the host
+            bool on_dev = false;
+            if (mc_ready_ && !host_m_forced()) {
+                dev::fill<int>(st_, 1, mflag_, 0);
+                dev::rn_scaled_hessian(st_, n, frp_, fci_, fhv_, fh_, D_, G_, J_, mpos_, mv_, mflag_);
+                hip_check(hipMemcpyAsync(host_ + 8, mflag_, sizeof(int), hipMemcpyDeviceToHost, st_), "d2h");
+                hip_check(hipStreamSynchronize(st_), "sync");
+                int flag = 1;
+                std::memcpy(&flag, host_ + 8, sizeof(int));
+                on_dev = flag == 0;
+                if (on_dev) mc_stale_ = true;
+            }
+            if (!on_dev) {
+                download(D_, hD);
+                download(G_, hG);
+                download(J_, hJ);
+                Mc_ = scaled_hessian(H_, hD, hG, hJ);
+                mc_stale_ = false;
+                if (frp_) cache_pattern();
+            }
             t_m_ = now_s() - t0;
-            iterate(M);
+            iterate(on_dev);
             if (rn_timing())
-                fprintf(stderr, "[rn] step %d: M %.3fs (nnz %d), inner %.3fs (build %.3fs, %d its), rest %.3fs\n",
-                        iter, t_m_, M.nnz(), t_inner_, t_build_, last_inner_, now_s() - t0 - t_m_ - t_inner_);
+                fprintf(stderr,
+                        "[rn] step %d: M %.3fs (nnz %d, %s), inner %.3fs (build %.3fs, %d its), rest %.3fs\n", iter,
+                        t_m_, Mc_.nnz(), on_dev ? "device" : "host", t_inner_, t_build_, last_inner_,
+                        now_s() - t0 - t_m_ - t_inner_);
             q = functional(X_);
             dq = std::fabs((q - q0) / q0);
             iter++;
@@ -466,7 +521,8 @@ class ReflectiveNewton {
     // RN_get_subspace (reflective_newton.c:294-366): V1 = D g, V2 solves
     // M V2 = -V1 from the start V2 = -V1; returns POSITIVE_DEFINITE (1) or
     // INDEFINITE (0)
-    int subspace(const Csr& M, DevMatrix& dM) {
+    int subspace(bool on_dev, DevMatrix& dM) {
+        const Csr& M = Mc_;  // values current unless on_dev (host_M() refreshes them)
         const int n = n_;
         dev::copy(st_, n, (const double*)G_, V1_);
         dev::mul(st_, n, V1_, D_);
@@ -481,7 +537,7 @@ class ReflectiveNewton {
                 // one ILU system for every Newton step: while M's pattern
                 // stays the same only the numeric factorisation runs (device)
                 const double tb = now_s();
-                ilu_.refactor(M);
+                if (!on_dev || !ilu_.refactor_device(mv_)) ilu_.refactor(host_M());
                 t_build_ = now_s() - tb;
                 if (!ilu_.ilu_ok()) {
                     status = 0;
@@ -496,7 +552,7 @@ class ReflectiveNewton {
                 // RN_UMF's slot (reflective_newton.c:312-315): the local solver
                 // decides the status; FEM.c's convention y = V2, V2 = M^-1 y
                 const double tb = now_s();
-                ilu_.refactor(M);
+                if (!on_dev || !ilu_.refactor_device(mv_)) ilu_.refactor(host_M());
                 t_build_ = now_s() - tb;
                 if (!ilu_.ilu_ok()) {
                     status = 0;
@@ -551,9 +607,45 @@ class ReflectiveNewton {
         last_inner_ = iter;
         if (status == 0) {
             stats_.neg_curvature++;
-            neg_curvature(M, dM);
+            neg_curvature(host_M(), dM);
         }
         return status;
+    }
+
+    // the scaled Hessian's host copy, its values fetched after a device step
+    const Csr& host_M() {
+        if (mc_stale_) {
+            hip_check(hipMemcpyAsync(Mc_.v.data(), mv_, Mc_.v.size() * sizeof(double), hipMemcpyDeviceToHost, st_),
+                      "d2h");
+            hip_check(hipStreamSynchronize(st_), "sync");
+            mc_stale_ = false;
+        }
+        return Mc_;
+    }
+
+    // after a host assembly: each F position's entry in M (M's pattern lies
+    // in F), for the device assembly of the next steps
+    void cache_pattern() {
+        mc_ready_ = false;
+        std::vector<int> mpos(F_.ci.size(), -1);
+        for (int i = 0; i < n_; i++) {
+            int q = Mc_.rp[i];
+            for (int p = F_.rp[i]; p < F_.rp[i + 1]; p++)
+                if (q < Mc_.rp[i + 1] && Mc_.ci[q] == F_.ci[p]) mpos[p] = q++;
+            if (q != Mc_.rp[i + 1]) return;  // an entry outside F: the host path stays
+        }
+        hip_check(hipStreamSynchronize(st_), "sync");
+        mbufs_.reset(new DevBuffers());
+        mpos_ = dupload(mpos, *mbufs_, st_);
+        mv_ = dalloc<double>(std::max<size_t>(Mc_.v.size(), 1), *mbufs_);
+        hip_check(hipMemcpyAsync(mv_, Mc_.v.data(), Mc_.v.size() * sizeof(double), hipMemcpyHostToDevice, st_),
+                  "h2d");
+        hip_check(hipStreamSynchronize(st_), "sync");
+        mc_ready_ = true;
+    }
+    static bool host_m_forced() {
+        const char* e = getenv("LINA_RN_HOST_M");
+        return e && atoi(e) != 0;
     }
 
     // power_method on A; the first eigenvalue estimate starts at 0 (the
@@ -620,16 +712,21 @@ class ReflectiveNewton {
     }
 
     // RN_iterate (reflective_newton.c:536-627)
-    void iterate(const Csr& M) {
+    void iterate(bool on_dev) {
         const int n = n_;
         dev::copy(st_, n, (const double*)X_, Xo_);
         // M keeps its pattern over most Newton steps: values-only re-upload
-        if (!dM_ || !dM_->update_values(M)) {
+        // (from the device assembly, whose pattern is the one dM_ was last
+        // given by a host step)
+        if (on_dev) {
+            hip_check(hipStreamSynchronize(st_), "sync");
+            if (!dM_ || !dM_->update_values_device(mv_, (long)Mc_.nnz())) dM_.reset(new DevMatrix(host_M()));
+        } else if (!dM_ || !dM_->update_values(Mc_)) {
             dM_.reset();
-            dM_.reset(new DevMatrix(M));
+            dM_.reset(new DevMatrix(Mc_));
         }
         DevMatrix& dM = *dM_;
-        int status = subspace(M, dM);
+        int status = subspace(on_dev, dM);
         // B = D g
         dev::copy(st_, n, (const double*)G_, Bv_);
         dev::mul(st_, n, Bv_, D_);
@@ -729,6 +826,16 @@ class ReflectiveNewton {
     double t_m_ = 0, t_inner_ = 0, t_build_ = 0;
     int last_inner_ = 0;
     lina_rn_stats stats_{};
+    // the scaled Hessian (PCG / GMRES inner solvers): F = H's pattern plus the
+    // diagonal on the device; Mc_ the last host assembly (its values stale
+    // after device steps until host_M()), mpos_ / mv_ its pattern's map and
+    // device values
+    Csr F_, Mc_;
+    int *frp_ = nullptr, *fci_ = nullptr, *mflag_ = nullptr, *mpos_ = nullptr;
+    double *fhv_ = nullptr, *mv_ = nullptr;
+    uint8_t* fh_ = nullptr;
+    std::unique_ptr<DevBuffers> mbufs_;
+    bool mc_ready_ = false, mc_stale_ = false;
 };
 
 }  // namespace lina

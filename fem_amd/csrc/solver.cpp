@@ -1761,6 +1761,14 @@ bool DevMatrix::update_values(const Csr& A0) {
     return true;
 }
 
+bool DevMatrix::update_values_device(const double* d_val, long nnz) {
+    if (nnz != (long)A_.nnz || nnz_ != nnz) return false;
+    hip_check(hipMemcpyAsync(A_.v, d_val, nnz * sizeof(double), hipMemcpyDeviceToDevice, stream_), "d2d");
+    dev::convert<double, float>(stream_, A_.nnz, A_.v, Af_.v);
+    hip_check(hipStreamSynchronize(stream_), "update");
+    return true;
+}
+
 double DevMatrix::matrix_norm() const {
     std::vector<double> v(A_.nnz);
     hip_check(hipMemcpyAsync(v.data(), A_.v, v.size() * sizeof(double), hipMemcpyDeviceToHost, stream_), "d2h");

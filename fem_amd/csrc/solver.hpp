@@ -119,6 +119,9 @@ class DevMatrix {
     // (the rows' l1 norms, which no product reads, keep the first values);
     // returns false, changing nothing, for another pattern
     bool update_values(const Csr& A);
+    // the same from nnz values in device memory, in the uploaded pattern's
+    // entry order (the caller vouches for the pattern); false when nnz differs
+    bool update_values_device(const double* d_val, long nnz);
     int n() const { return n_; }
     long nnz() const { return nnz_; }
     void apply(const double* x, double* y);  // enqueue on stream(), FP64

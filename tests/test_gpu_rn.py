@@ -152,3 +152,34 @@ def test_power_method_matches_reference(gpu, ref, case):
         assert abs(e + lam - lmin) <= 1e-6 * abs(lam)
     dM.close()
     R.free(M)
+
+
+@pytest.mark.parametrize("solver,order", [("pcg", "natural"), ("pcg", "multicolour"), ("gmres", "natural")])
+def test_device_scaled_hessian_matches_host_assembly(gpu, capfd, monkeypatch, solver, order):
+    """The scaled Hessian M = D H D + diag(|g| J) assembled on the device
+    (while its pattern is the last host assembly's) against the host
+    assembly on every step (LINA_RN_HOST_M=1): the same Newton and inner
+    iteration counts and the same minimiser, bit for bit."""
+    s, dold, ones = P.damage_system(129, h_scale=100.0)
+    monkeypatch.setenv("LINA_RN_TIMING", "1")
+    out = {}
+    for host in (True, False):
+        monkeypatch.setenv("LINA_RN_HOST_M", "1" if host else "0")
+        prm = gpu.rn_params(solver, 1, 6, delta=float(np.sqrt(s.n) / 2), ilu_ordering=order)
+        rn = gpu.ReflectiveNewton(s.rowptr, s.col, s.val, s.b, dold, ones, prm)
+        capfd.readouterr()
+        out[host] = rn.solve(dold)
+        err = capfd.readouterr().err
+        rn.close()
+        steps = [l for l in err.splitlines() if l.startswith("[rn] step")]
+        dev_steps = sum("device" in l for l in steps)
+        say(solver, order, "host M" if host else "device M", "newton", out[host]["iters"], "inner",
+            out[host]["inner_iterations"], "device-assembled steps", dev_steps, "of", len(steps))
+        assert len(steps) == out[host]["iters"]
+        if host:
+            assert dev_steps == 0
+        else:
+            assert dev_steps >= len(steps) // 2
+    assert out[True]["iters"] == out[False]["iters"]
+    assert out[True]["inner_iterations"] == out[False]["inner_iterations"]
+    assert np.array_equal(out[True]["x"], out[False]["x"])
