@@ -1408,7 +1408,8 @@ __global__ void k_scatter(int n, const int* __restrict__ idx, const double* __re
 //       to the diagonal (inserted when J_i != 0 and no diagonal is left).
 // mpos: each F position's entry in the cached pattern of M (-1: absent);
 // a position whose presence differs sets *flag (the caller assembles on the
-// host); present values go to out[mpos].
+// host); present values go to out[mpos] (mpos == nullptr: neither); outF,
+// when given, receives every F position's value, 0.0 where M has no entry.
 __device__ __forceinline__ double rn_hd(int i, int k, bool has, double a, const double* __restrict__ D) {
     const double b = D[k];
     const double val = (has && a != 0 && b != 0) ? a * b : 0.0;
@@ -1419,7 +1420,7 @@ __global__ void k_rn_scaled_hessian(int n, const int* __restrict__ frp, const in
                                     const double* __restrict__ fhv, const uint8_t* __restrict__ fh,
                                     const double* __restrict__ D, const double* __restrict__ G,
                                     const double* __restrict__ J, const int* __restrict__ mpos,
-                                    double* __restrict__ out, int* __restrict__ flag) {
+                                    double* __restrict__ out, double* __restrict__ outF, int* __restrict__ flag) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int b0 = frp[i], b1 = frp[i + 1];
         const double di = D[i];
@@ -1468,6 +1469,8 @@ __global__ void k_rn_scaled_hessian(int n, const int* __restrict__ frp, const in
                 val = di * v;
                 pres = hd_kept && di != 0 && val != 0;
             }
+            if (outF) outF[p] = pres ? val : 0.0;
+            if (!mpos) continue;
             const int q = mpos[p];
             if (pres != (q >= 0)) bad = true;
             else if (pres) out[q] = val;
@@ -2209,10 +2212,11 @@ void bpcg_check(hipStream_t st, int ninst, const double* rr, const double* hb, d
 }
 
 void rn_scaled_hessian(hipStream_t st, int n, const int* frp, const int* fci, const double* fhv, const uint8_t* fh,
-                       const double* D, const double* G, const double* J, const int* mpos, double* out, int* flag) {
+                       const double* D, const double* G, const double* J, const int* mpos, double* out,
+                       double* outF, int* flag) {
     if (n > 0)
         hipLaunchKernelGGL(k_rn_scaled_hessian, dim3(grid_for(n)), dim3(TPB), 0, st, n, frp, fci, fhv, fh, D, G, J,
-                           mpos, out, flag);
+                           mpos, out, outF, flag);
 }
 
 void rn_dj(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, const double* G,

@@ -167,9 +167,12 @@ void rn_snap(hipStream_t st, int nb, const int* idx, const double* lo, const dou
 // M = D H D + diag(|g| J) over F = H's pattern plus the diagonal (fh: H has
 // the position, fhv: its value), the values of the positions present in the
 // cached pattern of M written to out[mpos]; *flag set when the presence of
-// any position differs from the cache (flag is not cleared here)
+// any position differs from the cache (flag is not cleared here; mpos ==
+// nullptr: no cache); outF (optional): every F position's value, 0.0 where
+// M has no entry
 void rn_scaled_hessian(hipStream_t st, int n, const int* frp, const int* fci, const double* fhv, const uint8_t* fh,
-                       const double* D, const double* G, const double* J, const int* mpos, double* out, int* flag);
+                       const double* D, const double* G, const double* J, const int* mpos, double* out,
+                       double* outF, int* flag);
 // block partial counts of entries within 1e-30 of a bound; returns #partials
 int rn_touch(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, const double* X,
              double* part);

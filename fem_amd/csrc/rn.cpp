@@ -401,6 +401,12 @@ class ReflectiveNewton {
             fhv_ = dupload(fhv, *B_, st_);
             fh_ = dupload(fh, *B_, st_);
             mflag_ = dalloc<int>(1, *B_);
+            // M's products on F's pattern (zeros where M has no entry): one
+            // device matrix for every Newton step, patterns changing or not
+            mvF_ = dalloc<double>(F_.ci.size(), *B_);
+            F_.v.assign(F_.ci.size(), 0.0);
+            dMF_.reset(new DevMatrix(F_));
+            F_.v.clear();
         }
         hip_check(hipStreamSynchronize(st_), "rn upload");
     }
@@ -431,15 +437,18 @@ class ReflectiveNewton {
             // M on the device while its pattern is the cached one (the same
             // values as the host assembly), else assembled on the host
             bool on_dev = false;
-            if (mc_ready_ && !host_m_forced()) {
+            if (frp_) {
+                const bool cached = mc_ready_ && !host_m_forced();
                 dev::fill<int>(st_, 1, mflag_, 0);
-                dev::rn_scaled_hessian(st_, n, frp_, fci_, fhv_, fh_, D_, G_, J_, mpos_, mv_, mflag_);
+                dev::rn_scaled_hessian(st_, n, frp_, fci_, fhv_, fh_, D_, G_, J_, cached ? mpos_ : nullptr,
+                                       cached ? mv_ : nullptr, mvF_, mflag_);
                 hip_check(hipMemcpyAsync(host_ + 8, mflag_, sizeof(int), hipMemcpyDeviceToHost, st_), "d2h");
                 hip_check(hipStreamSynchronize(st_), "sync");
                 int flag = 1;
                 std::memcpy(&flag, host_ + 8, sizeof(int));
-                on_dev = flag == 0;
+                on_dev = cached && flag == 0;
                 if (on_dev) mc_stale_ = true;
+                dMF_->update_values_device(mvF_, (long)F_.ci.size());
             }
             if (!on_dev) {
                 download(D_, hD);
@@ -715,17 +724,14 @@ class ReflectiveNewton {
     void iterate(bool on_dev) {
         const int n = n_;
         dev::copy(st_, n, (const double*)X_, Xo_);
-        // M keeps its pattern over most Newton steps: values-only re-upload
-        // (from the device assembly, whose pattern is the one dM_ was last
-        // given by a host step)
-        if (on_dev) {
-            hip_check(hipStreamSynchronize(st_), "sync");
-            if (!dM_ || !dM_->update_values_device(mv_, (long)Mc_.nnz())) dM_.reset(new DevMatrix(host_M()));
-        } else if (!dM_ || !dM_->update_values(Mc_)) {
+        // M's products: on F's pattern (PCG / GMRES inner solvers, updated by
+        // the step's device assembly), else M's own pattern, which keeps over
+        // most Newton steps (values-only re-upload)
+        if (!dMF_ && (!dM_ || !dM_->update_values(Mc_))) {
             dM_.reset();
             dM_.reset(new DevMatrix(Mc_));
         }
-        DevMatrix& dM = *dM_;
+        DevMatrix& dM = dMF_ ? *dMF_ : *dM_;
         int status = subspace(on_dev, dM);
         // B = D g
         dev::copy(st_, n, (const double*)G_, Bv_);
@@ -832,7 +838,8 @@ class ReflectiveNewton {
     // device values
     Csr F_, Mc_;
     int *frp_ = nullptr, *fci_ = nullptr, *mflag_ = nullptr, *mpos_ = nullptr;
-    double *fhv_ = nullptr, *mv_ = nullptr;
+    double *fhv_ = nullptr, *mv_ = nullptr, *mvF_ = nullptr;
+    std::unique_ptr<DevMatrix> dMF_;  // M on F's pattern (the products, line search, power method)
     uint8_t* fh_ = nullptr;
     std::unique_ptr<DevBuffers> mbufs_;
     bool mc_ready_ = false, mc_stale_ = false;
