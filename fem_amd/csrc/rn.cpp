@@ -407,6 +407,7 @@ class ReflectiveNewton {
             F_.v.assign(F_.ci.size(), 0.0);
             dMF_.reset(new DevMatrix(F_));
             F_.v.clear();
+            f_ilu_ = p_.ilu_ordering == LINA_ILU_MULTICOLOUR;
         }
         hip_check(hipStreamSynchronize(st_), "rn upload");
     }
@@ -437,8 +438,10 @@ class ReflectiveNewton {
             // M on the device while its pattern is the cached one (the same
             // values as the host assembly), else assembled on the host
             bool on_dev = false;
+            mc_current_ = false;
             if (frp_) {
-                const bool cached = mc_ready_ && !host_m_forced();
+                // the multicolour ILU factors M on F's pattern: no cache needed
+                const bool cached = !f_ilu_ && mc_ready_ && !host_m_forced();
                 dev::fill<int>(st_, 1, mflag_, 0);
                 dev::rn_scaled_hessian(st_, n, frp_, fci_, fhv_, fh_, D_, G_, J_, cached ? mpos_ : nullptr,
                                        cached ? mv_ : nullptr, mvF_, mflag_);
@@ -446,7 +449,7 @@ class ReflectiveNewton {
                 hip_check(hipStreamSynchronize(st_), "sync");
                 int flag = 1;
                 std::memcpy(&flag, host_ + 8, sizeof(int));
-                on_dev = cached && flag == 0;
+                on_dev = f_ilu_ || (cached && flag == 0);
                 if (on_dev) mc_stale_ = true;
                 dMF_->update_values_device(mvF_, (long)F_.ci.size());
             }
@@ -456,6 +459,7 @@ class ReflectiveNewton {
                 download(J_, hJ);
                 Mc_ = scaled_hessian(H_, hD, hG, hJ);
                 mc_stale_ = false;
+                mc_current_ = true;
                 if (frp_) cache_pattern();
             }
             t_m_ = now_s() - t0;
@@ -463,8 +467,9 @@ class ReflectiveNewton {
             if (rn_timing())
                 fprintf(stderr,
                         "[rn] step %d: M %.3fs (nnz %d, %s), inner %.3fs (build %.3fs, %d its), rest %.3fs\n", iter,
-                        t_m_, Mc_.nnz(), on_dev ? "device" : "host", t_inner_, t_build_, last_inner_,
-                        now_s() - t0 - t_m_ - t_inner_);
+                        t_m_, f_ilu_ ? (int)F_.ci.size() : Mc_.nnz(),
+                        f_ilu_ ? "device, F's pattern" : (on_dev ? "device" : "host"), t_inner_, t_build_,
+                        last_inner_, now_s() - t0 - t_m_ - t_inner_);
             q = functional(X_);
             dq = std::fabs((q - q0) / q0);
             iter++;
@@ -546,7 +551,7 @@ class ReflectiveNewton {
                 // one ILU system for every Newton step: while M's pattern
                 // stays the same only the numeric factorisation runs (device)
                 const double tb = now_s();
-                if (!on_dev || !ilu_.refactor_device(mv_)) ilu_.refactor(host_M());
+                refactor_ilu(on_dev);
                 t_build_ = now_s() - tb;
                 if (!ilu_.ilu_ok()) {
                     status = 0;
@@ -561,7 +566,7 @@ class ReflectiveNewton {
                 // RN_UMF's slot (reflective_newton.c:312-315): the local solver
                 // decides the status; FEM.c's convention y = V2, V2 = M^-1 y
                 const double tb = now_s();
-                if (!on_dev || !ilu_.refactor_device(mv_)) ilu_.refactor(host_M());
+                refactor_ilu(on_dev);
                 t_build_ = now_s() - tb;
                 if (!ilu_.ilu_ok()) {
                     status = 0;
@@ -621,8 +626,38 @@ class ReflectiveNewton {
         return status;
     }
 
+    // the ILU(0) of the step's M: the multicolour order factors M on F's
+    // fixed pattern (zeros where M has no entry) from the device values,
+    // built once; the natural order factors M's own pattern, from the device
+    // values while it is the cached one
+    void refactor_ilu(bool on_dev) {
+        if (f_ilu_) {
+            if (ilu_f_built_ && ilu_.refactor_device(mvF_)) return;
+            Csr Fv = F_;
+            Fv.v.resize(F_.ci.size());
+            hip_check(hipMemcpyAsync(Fv.v.data(), mvF_, Fv.v.size() * sizeof(double), hipMemcpyDeviceToHost, st_),
+                      "d2h");
+            hip_check(hipStreamSynchronize(st_), "sync");
+            ilu_.refactor(Fv);
+            ilu_f_built_ = true;
+            return;
+        }
+        if (!on_dev || !ilu_.refactor_device(mv_)) ilu_.refactor(host_M());
+    }
+
     // the scaled Hessian's host copy, its values fetched after a device step
+    // (assembled from D, G, J when the step's M lives on F's pattern only)
     const Csr& host_M() {
+        if (f_ilu_ && !mc_current_) {
+            std::vector<double> hD(n_), hG(n_), hJ(n_);
+            download(D_, hD);
+            download(G_, hG);
+            download(J_, hJ);
+            Mc_ = scaled_hessian(H_, hD, hG, hJ);
+            mc_current_ = true;
+            mc_stale_ = false;
+            return Mc_;
+        }
         if (mc_stale_) {
             hip_check(hipMemcpyAsync(Mc_.v.data(), mv_, Mc_.v.size() * sizeof(double), hipMemcpyDeviceToHost, st_),
                       "d2h");
@@ -842,7 +877,8 @@ class ReflectiveNewton {
     std::unique_ptr<DevMatrix> dMF_;  // M on F's pattern (the products, line search, power method)
     uint8_t* fh_ = nullptr;
     std::unique_ptr<DevBuffers> mbufs_;
-    bool mc_ready_ = false, mc_stale_ = false;
+    bool mc_ready_ = false, mc_stale_ = false, mc_current_ = false;
+    bool f_ilu_ = false, ilu_f_built_ = false;  // multicolour ILU on F's pattern
 };
 
 }  // namespace lina

@@ -387,7 +387,9 @@ typedef struct lina_rn_params {
                           * the relative reduction |dq/q| of the outer test depends on */
     lina_rn_host_solver host_solver; /* LINA_RN_HOST */
     void* host_ctx;
-    int ilu_ordering;    /* LINA_ILU_* of the PCG / GMRES inner solvers' ILU(0) (natural) */
+    int ilu_ordering;    /* LINA_ILU_* of the PCG / GMRES inner solvers' ILU(0) (natural); MULTICOLOUR
+                          * factors M on H's pattern plus the diagonal (M's absent entries as zeros), its
+                          * values assembled on the device every step */
 } lina_rn_params;
 
 typedef struct lina_rn_stats {
