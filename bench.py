@@ -682,8 +682,9 @@ def main():
                          "err_sampled: the relative error over 20,000 seeded entries"}
 
     # finest-level SpMV against the HBM roofline: the outer-loop A*p launches
-    # of the timed mixed solves, each bracketed by HIP events on the library
-    # stream; beside it the same kernel back to back.  Algorithmic bytes:
+    # of the timed mixed solves (every fourth), timed by HIP events that the
+    # launch records at the kernel's own start and end on the library stream;
+    # beside it the same kernel back to back.  Algorithmic bytes:
     # 12 nnz + 4 (n+1) + 16 n (what the kernel stores and moves instead,
     # with 16-bit column codes, is stored_bytes_per_launch)
     lvl = amg.depth - 1

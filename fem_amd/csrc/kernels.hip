@@ -23,6 +23,7 @@
 // single-block finalisation.
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstring>
 
@@ -2089,6 +2090,20 @@ int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int
 }
 template int spmv<double, double>(hipStream_t, const DMat<double>&, const double*, const double*, double*, int,
                                   double*, const double*, const Epi<double, double>*);
+
+// the outer CG's y = A x (MODE_OUTER, FP64) with HIP events recorded by the
+// launch itself at the kernel's start and end (hipExtLaunchKernel), so that
+// their interval is the kernel's own duration
+int spmv_outer_timed(hipStream_t st, const DMat<double>& A, const double* x, double* y, double* part,
+                     const double* w, hipEvent_t e0, hipEvent_t e1) {
+    if (A.nblk == 0) return 0;
+    dim3 g(row_grid(A.nblk)), t(TPB);
+    const Epi<double, double> e{};
+    LINA_LPR_DISPATCH(A.lpr, hipExtLaunchKernelGGL((k_spmv<double, double, MODE_OUTER, L>), g, t, 0, st, e0, e1, 0,
+                                                   A.nblk, A.bk, A.rp, A.ci, A.cs, A.cb, A.v, x,
+                                                   (const double*)nullptr, y, part, w, e, A.bspec, A.active))
+    return A.nblk;
+}
 template int spmv<float, float>(hipStream_t, const DMat<float>&, const float*, const float*, float*, int, double*,
                                 const float*, const Epi<float, float>*);
 

@@ -113,6 +113,10 @@ inline int lanes_per_row(double mean_len) {
 template <typename VT, typename XT>
 int spmv(hipStream_t st, const DMat<VT>& A, const XT* x, const XT* b, XT* y, int mode, double* part, const XT* w,
          const Epi<VT, XT>* epi = nullptr);
+// MODE_OUTER y = A x in FP64 with events recorded at the kernel's own start
+// and end (the outer CG's timed launches)
+int spmv_outer_timed(hipStream_t st, const DMat<double>& A, const double* x, double* y, double* part,
+                     const double* w, hipEvent_t e0, hipEvent_t e1);
 // block partials of a.b: at most DOT_PART_MAX of them (the grid's cap)
 constexpr int DOT_PART_MAX = 1024;
 template <typename XT>

@@ -1583,9 +1583,10 @@ int Solver::pcg_dev(double* X, double* B, double tol, int max_iter, int stop, li
             apply_A(P, Q);
             m = dev::dot_part<double>(st, n, P, Q, part_);
         } else {
-            if (sampled(k)) hip_check(hipEventRecord(evk_[k % 2][0], st), "event");
-            m = dev::spmv<double, double>(st, top.A, P, nullptr, Q, dev::MODE_OUTER, part_, P);  // Q = A P, P.Q
-            if (sampled(k)) hip_check(hipEventRecord(evk_[k % 2][1], st), "event");
+            if (sampled(k))  // the events bracket the kernel itself
+                m = dev::spmv_outer_timed(st, top.A, P, Q, part_, P, evk_[k % 2][0], evk_[k % 2][1]);
+            else
+                m = dev::spmv<double, double>(st, top.A, P, nullptr, Q, dev::MODE_OUTER, part_, P);  // Q = A P, P.Q
         }
         dev::finalize(st, m, part_, scal_, dev::S_PQ, 0);
         if (!prep) {
