@@ -1962,38 +1962,6 @@ __global__ __launch_bounds__(1024) void k_bgj_diag(int N, int k0, int nk, double
 // new row panel's column t (also into Brow, 64 x N, rows beyond nk zero) and
 // the new column panel's row t (the old one into Acol, N x 64, columns
 // beyond nk zero); each thread's 64 inputs are staged in LDS
-__global__ __launch_bounds__(64) void k_bgj_panels(int N, int k0, int nk, double* __restrict__ M,
-                                                   const double* __restrict__ Dinv, double* __restrict__ Acol,
-                                                   double* __restrict__ Brow) {
-    __shared__ double D[GJB * GJB];
-    __shared__ double V[GJB][GJB + 1];  // [thread][m]
-    for (int t = threadIdx.x; t < GJB * GJB; t += blockDim.x) D[t] = Dinv[t];
-    __syncthreads();
-    const int x = threadIdx.x;
-    for (int base = blockIdx.x * blockDim.x; base < N; base += gridDim.x * blockDim.x) {
-        const int t = base + x;
-        const bool live = t < N && !(t >= k0 && t < k0 + nk);
-        // row panel: column t of M[K, :]
-        for (int m = 0; m < GJB; m++) V[x][m] = (live && m < nk) ? M[(size_t)(k0 + m) * N + t] : 0.0;
-        for (int r = 0; r < GJB; r++) {
-            double s = 0.0;
-            for (int m = 0; m < nk; m++) s += D[r * GJB + m] * V[x][m];
-            if (t < N) Brow[(size_t)r * N + t] = (live && r < nk) ? s : 0.0;
-            if (live && r < nk) M[(size_t)(k0 + r) * N + t] = s;
-        }
-        // column panel: row t of M[:, K]
-        for (int m = 0; m < GJB; m++) V[x][m] = (live && m < nk) ? M[(size_t)t * N + k0 + m] : 0.0;
-        if (t < N)
-            for (int m = 0; m < GJB; m++) Acol[(size_t)t * GJB + m] = V[x][m];
-        if (live)
-            for (int c = 0; c < nk; c++) {
-                double s = 0.0;
-                for (int m = 0; m < nk; m++) s += V[x][m] * D[m * GJB + c];
-                M[(size_t)t * N + k0 + c] = -s;
-            }
-    }
-}
-
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 // M[I,J] -= Acol[I,:] Brow[:,J] over the 64 x 64 tiles outside block row and
@@ -2035,6 +2003,73 @@ __global__ __launch_bounds__(256) void k_bgj_gemm(int N, int kb, int nt, double*
                 const int i = i0 + qi + a * 16 + 4 * v + l / 16;
                 const int j = j0 + qj + b * 16 + l % 16;
                 if (i < N && j < N) M[(size_t)i * N + j] -= acc[a][b][v];
+            }
+}
+
+// The two panels of block step kb as 64 x 64 MFMA tile products (grid: nt
+// row-panel tiles, then nt column-panel tiles; the pivot block's own tile
+// writes zeros to its panel buffer and nothing else):
+//   row panel, columns J of tile tj:  M[K,J] = D M[K,J], copied to Brow;
+//   column panel, rows I of tile ti:  Acol = M[I,K], M[I,K] = -M[I,K] D.
+// D = Dinv on its nk x nk corner, zero elsewhere.  The tiles touch disjoint
+// parts of M (row panel: rows K, columns outside K; column panel: rows
+// outside K, columns K).
+__global__ __launch_bounds__(256) void k_bgj_panels(int N, int kb, int nt, int nk, double* __restrict__ M,
+                                                    const double* __restrict__ Dinv, double* __restrict__ Acol,
+                                                    double* __restrict__ Brow) {
+    __shared__ double As[GJB][GJB + 1];  // [row][k]
+    __shared__ double Bs[GJB][GJB + 1];  // [k][column]
+    const int tid = threadIdx.x;
+    const bool rowp = (int)blockIdx.x < nt;
+    const int tb = rowp ? (int)blockIdx.x : (int)blockIdx.x - nt;
+    const int k0 = kb * GJB, t0 = tb * GJB;
+    if (tb == kb) {
+        for (int t = tid; t < GJB * GJB; t += 256) {
+            const int r = t / GJB, c = t % GJB;
+            if (rowp && t0 + c < N) Brow[(size_t)r * N + t0 + c] = 0.0;
+            if (!rowp && t0 + r < N) Acol[(size_t)(t0 + r) * GJB + c] = 0.0;
+        }
+        return;
+    }
+    for (int t = tid; t < GJB * GJB; t += 256) {
+        const int r = t / GJB, c = t % GJB;
+        if (rowp) {
+            As[r][c] = (r < nk && c < nk) ? Dinv[r * GJB + c] : 0.0;
+            Bs[r][c] = (r < nk && t0 + c < N) ? M[(size_t)(k0 + r) * N + t0 + c] : 0.0;
+        } else {
+            const double v = (t0 + r < N && c < nk) ? M[(size_t)(t0 + r) * N + k0 + c] : 0.0;
+            As[r][c] = v;
+            if (t0 + r < N) Acol[(size_t)(t0 + r) * GJB + c] = v;
+            Bs[r][c] = (r < nk && c < nk) ? Dinv[r * GJB + c] : 0.0;
+        }
+    }
+    __syncthreads();
+    const int w = tid / 64, l = tid % 64;
+    const int qi = (w / 2) * 32, qj = (w % 2) * 32;
+    f64x4 acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = (f64x4){0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < GJB; k += 4) {
+        double av[2], bv[2];
+        for (int a = 0; a < 2; a++) av[a] = As[qi + a * 16 + l % 16][k + l / 16];
+        for (int b = 0; b < 2; b++) bv[b] = Bs[k + l / 16][qj + b * 16 + l % 16];
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int v = 0; v < 4; v++) {
+                const int r = qi + a * 16 + 4 * v + l / 16;  // row of the tile product
+                const int c = qj + b * 16 + l % 16;          // its column
+                const double x = acc[a][b][v];
+                if (rowp) {
+                    if (t0 + c < N) {
+                        Brow[(size_t)r * N + t0 + c] = r < nk ? x : 0.0;
+                        if (r < nk) M[(size_t)(k0 + r) * N + t0 + c] = x;
+                    }
+                } else if (t0 + r < N && c < nk) {
+                    M[(size_t)(t0 + r) * N + k0 + c] = -x;
+                }
             }
 }
 
@@ -2569,11 +2604,11 @@ void gauss_jordan_inverse(hipStream_t st, int N, double* M, double* work) {
     double* Acol = work + GJB * GJB;
     double* Brow = Acol + (size_t)GJB * N;
     const int nt = (N + GJB - 1) / GJB;
-    const int gp = std::min((N + 63) / 64, 4096);
     for (int kb = 0; kb < nt; kb++) {
         const int k0 = kb * GJB, nk = std::min(GJB, N - k0);
         hipLaunchKernelGGL(k_bgj_diag, dim3(1), dim3(1024), 0, st, N, k0, nk, M, Dinv);
-        hipLaunchKernelGGL(k_bgj_panels, dim3(gp), dim3(64), 0, st, N, k0, nk, M, (const double*)Dinv, Acol, Brow);
+        hipLaunchKernelGGL(k_bgj_panels, dim3(2 * nt), dim3(256), 0, st, N, kb, nt, nk, M, (const double*)Dinv, Acol,
+                           Brow);
         if (nt > 1)
             hipLaunchKernelGGL(k_bgj_gemm, dim3((nt - 1) * (nt - 1)), dim3(256), 0, st, N, kb, nt, M,
                                (const double*)Acol, (const double*)Brow);
