@@ -1979,14 +1979,24 @@ __global__ __launch_bounds__(256) void k_bgj_gemm(int N, int kb, int nt, double*
     if (tj >= kb) tj++;
     const int i0 = ti * GJB, j0 = tj * GJB;
     const int tid = threadIdx.x;
+    const int w = tid / 64, l = tid % 64;
+    const int qi = (w / 2) * 32, qj = (w % 2) * 32;
+    // the lane's 16 entries of the M tile, loaded first so that their
+    // latency overlaps the staging and the products
+    double mv[2][2][4];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int v = 0; v < 4; v++) {
+                const int i = i0 + qi + a * 16 + 4 * v + l / 16;
+                const int j = j0 + qj + b * 16 + l % 16;
+                mv[a][b][v] = (i < N && j < N) ? M[(size_t)i * N + j] : 0.0;
+            }
     for (int t = tid; t < GJB * GJB; t += 256) {
         const int r = t / GJB, c = t % GJB;
         As[r][c] = (i0 + r < N) ? Acol[(size_t)(i0 + r) * GJB + c] : 0.0;
         Bs[r][c] = (j0 + c < N) ? Brow[(size_t)r * N + j0 + c] : 0.0;
     }
     __syncthreads();
-    const int w = tid / 64, l = tid % 64;
-    const int qi = (w / 2) * 32, qj = (w % 2) * 32;
     f64x4 acc[2][2];
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++) acc[a][b] = (f64x4){0.0, 0.0, 0.0, 0.0};
@@ -2002,7 +2012,7 @@ __global__ __launch_bounds__(256) void k_bgj_gemm(int N, int kb, int nt, double*
             for (int v = 0; v < 4; v++) {
                 const int i = i0 + qi + a * 16 + 4 * v + l / 16;
                 const int j = j0 + qj + b * 16 + l % 16;
-                if (i < N && j < N) M[(size_t)i * N + j] -= acc[a][b][v];
+                if (i < N && j < N) M[(size_t)i * N + j] = mv[a][b][v] - acc[a][b][v];
             }
 }
 
