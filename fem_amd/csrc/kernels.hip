@@ -1255,30 +1255,14 @@ __global__ void k_combine(int n, int m, const double* __restrict__ V, size_t ld,
 
 // ------------------------------------------------- Galerkin products
 
-__global__ void k_plan_sum(int ne, const int64_t* __restrict__ ptr, const int* __restrict__ ia,
-                           const int* __restrict__ ib, const uint8_t* __restrict__ seed,
-                           const double* __restrict__ va, const double* __restrict__ vb, double* __restrict__ out) {
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
-        int64_t t = ptr[e];
-        const int64_t t1 = ptr[e + 1];
-        double s;
-        if (seed[e] || t == t1) {
-            s = 0.0;
-        } else {
-            s = va[ia[t]] * vb[ib[t]];
-            t++;
-        }
-        for (; t < t1; t++) s += va[ia[t]] * vb[ib[t]];
-        out[e] = s;
-    }
-}
-
-// The same sums with the products formed cooperatively: a workgroup takes
-// TPB consecutive entries, whose products are contiguous in the plan; all
-// lanes gather and multiply them chunk by chunk into LDS, then each lane
-// adds its own entry's products from LDS in their listed order, so every
-// sum is the sequential one (bit-identical to k_plan_sum) while the
-// dependent gathers run in parallel.
+// Plan sums: out[e] = the sum of entry e's listed products va[ia[t]] *
+// vb[ib[t]], t in [ptr[e], ptr[e+1]), in their listed order -- started from
+// 0.0 when seed[e] (sparse_zero's seed the reference adds onto), else from
+// the first product.  A workgroup takes TPB consecutive entries, whose
+// products are contiguous in the plan; all lanes gather and multiply them
+// chunk by chunk into LDS, then each lane adds its own entry's products from
+// LDS in their listed order (every sum sequential, as the reference's),
+// while the dependent gathers run in parallel.
 constexpr int PLAN_CHUNK = 4096;
 __global__ __launch_bounds__(TPB) void k_plan_sum_staged(int ne, const int64_t* __restrict__ ptr,
                                                         const int* __restrict__ ia, const int* __restrict__ ib,

@@ -15,7 +15,7 @@
 // zero, which leaves any nonzero sum unchanged, so every entry that LinA
 // keeps gets the value LinA computes, bit for bit; entries that sum to zero
 // are the ones delete_zeros removes.  The device sums each entry's list in
-// order (k_plan_sum), first A P, then R (A P), and scatters the coarse
+// order (k_plan_sum_staged), first A P, then R (A P), and scatters the coarse
 // entries into the level's CSR values, flagging any entry whose zero /
 // nonzero state differs from the level's pattern (then the recompute takes
 // the host path).
