@@ -1379,7 +1379,7 @@ __global__ void k_scatter(int n, const int* __restrict__ idx, const double* __re
 
 // RN_set_D_J (reflective_newton.c:227-248); D = 1, J = 0 beforehand
 // The reflective Newton's scaled Hessian M = D H D + diag(|g| J) on the
-// device, one row per thread, with the host assembly's rules (RN_get_M_D's
+// device, row by row, with the host assembly's rules (RN_get_M_D's
 // two products with their zero deletions, then insert_sparse of |g_i| J_i):
 // over the row's positions in F = H's pattern plus the diagonal (fh: H has
 // the entry, fhv its value):
@@ -1401,18 +1401,27 @@ __device__ __forceinline__ double rn_hd(int i, int k, bool has, double a, const 
     return k == i ? (has ? 0.0 + val : 0.0) : val;
 }
 
-__global__ void k_rn_scaled_hessian(int n, const int* __restrict__ frp, const int* __restrict__ fci,
-                                    const double* __restrict__ fhv, const uint8_t* __restrict__ fh,
-                                    const double* __restrict__ D, const double* __restrict__ G,
-                                    const double* __restrict__ J, const int* __restrict__ mpos,
-                                    double* __restrict__ out, double* __restrict__ outF, int* __restrict__ flag) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+// 8 lanes a row, each lane the row's entries lane, lane + 8, ...; the
+// row's counts and its diagonal gathered by xor shuffles within the 8
+constexpr int RN_LPR = 8;
+__global__ __launch_bounds__(TPB) void k_rn_scaled_hessian(int n, const int* __restrict__ frp,
+                                                           const int* __restrict__ fci,
+                                                           const double* __restrict__ fhv,
+                                                           const uint8_t* __restrict__ fh,
+                                                           const double* __restrict__ D,
+                                                           const double* __restrict__ G,
+                                                           const double* __restrict__ J,
+                                                           const int* __restrict__ mpos, double* __restrict__ out,
+                                                           double* __restrict__ outF, int* __restrict__ flag) {
+    const int lane = threadIdx.x % RN_LPR;
+    const int groups = gridDim.x * (TPB / RN_LPR);
+    for (int i = blockIdx.x * (TPB / RN_LPR) + threadIdx.x / RN_LPR; i < n; i += groups) {
         const int b0 = frp[i], b1 = frp[i + 1];
         const double di = D[i];
         // HD: entries present, nonzero entries, the diagonal's value
         int hd_len = 0, hd_nz = 0;
         double hd_diag = 0.0;
-        for (int p = b0; p < b1; p++) {
+        for (int p = b0 + lane; p < b1; p += RN_LPR) {
             const int k = fci[p];
             const bool has = fh[p] != 0;
             const double a = has ? fhv[p] : 0.0;
@@ -1423,24 +1432,29 @@ __global__ void k_rn_scaled_hessian(int n, const int* __restrict__ frp, const in
             hd_nz += v != 0;
             if (k == i) hd_diag = v;
         }
+        hd_len = group_sum<RN_LPR>(hd_len);
+        hd_nz = group_sum<RN_LPR>(hd_nz);
+        hd_diag = group_sum<RN_LPR>(hd_diag);  // one lane holds it, the others add 0.0
         // after HD's deletion: the diagonal kept with value hd_diag, or 0.0
         const bool hd_keep_diag = hd_len == 1 || hd_diag != 0 || hd_nz == 0;
         // M before its deletion: the diagonal always, off-diagonals iff
         // di != 0 (over HD's kept, nonzero off-diagonals)
         const double m_diag = (hd_keep_diag && di != 0 && hd_diag != 0) ? 0.0 + di * hd_diag : 0.0;
-        int m_len = 1, m_nz = m_diag != 0;
-        for (int p = b0; p < b1; p++) {
+        int m_off = 0, m_nz = 0;
+        for (int p = b0 + lane; p < b1; p += RN_LPR) {
             const int k = fci[p];
             if (k == i || !fh[p]) continue;
             const double v = rn_hd(i, k, true, fhv[p], D);
             if (!(fhv[p] != 0 && D[k] != 0) || v == 0 || di == 0) continue;
-            m_len++;
+            m_off++;
             m_nz += di * v != 0;
         }
-        const bool keep_diag = m_len == 1 || m_diag != 0 || m_nz == 0;
+        m_off = group_sum<RN_LPR>(m_off);
+        m_nz = group_sum<RN_LPR>(m_nz) + (m_diag != 0);
+        const bool keep_diag = m_off == 0 || m_diag != 0 || m_nz == 0;
         const double add = J[i] != 0 ? fabs(G[i]) * J[i] : 0.0;
         bool bad = false;
-        for (int p = b0; p < b1; p++) {
+        for (int p = b0 + lane; p < b1; p += RN_LPR) {
             const int k = fci[p];
             bool pres;
             double val;
@@ -2259,8 +2273,8 @@ void rn_scaled_hessian(hipStream_t st, int n, const int* frp, const int* fci, co
                        const double* D, const double* G, const double* J, const int* mpos, double* out,
                        double* outF, int* flag) {
     if (n > 0)
-        hipLaunchKernelGGL(k_rn_scaled_hessian, dim3(grid_for(n)), dim3(TPB), 0, st, n, frp, fci, fhv, fh, D, G, J,
-                           mpos, out, outF, flag);
+        hipLaunchKernelGGL(k_rn_scaled_hessian, dim3(std::min((n + TPB / RN_LPR - 1) / (TPB / RN_LPR), 16384)),
+                           dim3(TPB), 0, st, n, frp, fci, fhv, fh, D, G, J, mpos, out, outF, flag);
 }
 
 void rn_dj(hipStream_t st, int nb, const int* idx, const double* lo, const double* up, const double* G,
