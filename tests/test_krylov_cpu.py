@@ -110,3 +110,17 @@ def test_multicolour_ilu_is_the_reference_ilu_of_the_permuted_matrix(ref, kind):
         lower = ci[rp[i]:rp[i + 1] - 1]
         lev[i] = 1 + lev[lower].max() if len(lower) else 0
     assert lev.max() + 1 <= ncol
+
+
+def test_ordered_ilu_and_rn_reject_unknown_orderings():
+    """an unknown ILU ordering is an argument error (no handle, a message),
+    for the ILU system and for the reflective Newton's inner solver"""
+    s = P.poisson(9)
+    lib = lina.lib()
+    lina._bind_krylov(lib)
+    h = lib.lina_ilu_create_ordered(s.n, lina._i(s.rowptr), lina._i(s.col), lina._d(s.val), 7, 0)
+    assert not h
+    assert b"ordering" in lib.lina_last_error()
+    prm = lina.rn_params("pcg", 1, 3, ilu_ordering=7)
+    with pytest.raises(lina.LinaError, match="ordering"):
+        lina.ReflectiveNewton(s.rowptr, s.col, s.val, s.b, np.zeros(s.n), np.ones(s.n), prm)
