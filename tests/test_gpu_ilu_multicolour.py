@@ -13,7 +13,8 @@ incomplete_LU_factorization, PCG_preconlr, pcg and gmres):
 * pcg / GMRES(30) on A equal the oracle's on the permuted system.
 
 The damage step's reflective Newton with the multicolour ILU-PCG inner
-solver is checked at full size in test_gpu_fullsize.py.
+solver is checked against the reference's reflective_newton on the permuted
+system in test_gpu_rn.py, and at full size in test_gpu_fullsize.py.
 """
 import numpy as np
 import pytest

@@ -161,7 +161,8 @@ def test_device_scaled_hessian_matches_host_assembly(gpu, capfd, monkeypatch, so
     assembly on every step (LINA_RN_HOST_M=1): the same Newton and inner
     iteration counts and the same minimiser, bit for bit.  (The multicolour
     ILU factors M on the fixed superset pattern from the device values on
-    every step; test_damage1025_reflective_newton_kkt covers it.)"""
+    every step; test_multicolour_reflective_newton_is_the_reference_on_the_permuted_system
+    and test_damage1025_reflective_newton_kkt cover it.)"""
     s, dold, ones = P.damage_system(129, h_scale=100.0)
     monkeypatch.setenv("LINA_RN_TIMING", "1")
     out = {}
@@ -185,3 +186,37 @@ def test_device_scaled_hessian_matches_host_assembly(gpu, capfd, monkeypatch, so
     assert out[True]["iters"] == out[False]["iters"]
     assert out[True]["inner_iterations"] == out[False]["inner_iterations"]
     assert np.array_equal(out[True]["x"], out[False]["x"])
+
+
+@pytest.mark.parametrize("h_scale", [100.0, 1000.0])
+def test_multicolour_reflective_newton_is_the_reference_on_the_permuted_system(gpu, ref, h_scale):
+    """ILU-PCG in multicolour order inside the reflective Newton is the
+    reference's reflective_newton (ILU-PCG) run on P H P^T, P b and the
+    permuted bounds, P the stable sort of the greedy colouring of H's
+    pattern: the scaled Hessian keeps H's pattern while the iterates stay
+    interior, so its ILU(0) in that order is the reference's ILU of the
+    permuted scaled Hessian.  Same Newton iteration count, the minimiser
+    mapped back to 1e-9 relative, the same minimum."""
+    from oracle.bind import Oracle
+    from tests.test_gpu_ilu_multicolour import _permuted
+    s, dold, ones = P.damage_system(33, h_scale=h_scale)
+    _, col = Oracle().colouring(s.rowptr, s.col, np.ones(len(s.col)))
+    perm = np.argsort(col, kind="stable").astype(np.int32)
+    prp, pci, pv = _permuted(s.rowptr, s.col, s.val, perm)
+    delta = np.sqrt(s.n) / 2
+    H = ref.matrix(prp, pci, pv)
+    r = ref.reflective_newton(H, s.b[perm], dold[perm], ones[perm], dold[perm].copy(), solver=3, dof=1, depth=4,
+                              max_iter=50, eps=1e-8, delta=delta)
+    ref.free(H)
+    x_ref = np.empty(s.n)
+    x_ref[perm] = r["x"]
+    prm = gpu.rn_params("pcg", 1, 4, delta=delta, ilu_ordering="multicolour")
+    rn = gpu.ReflectiveNewton(s.rowptr, s.col, s.val, s.b, dold, ones, prm)
+    g = rn.solve(dold)
+    rn.close()
+    rel = np.linalg.norm(g["x"] - x_ref) / np.linalg.norm(x_ref)
+    say("rn multicolour", h_scale, "iters ref", r["iters"], "gpu", g["iters"], "rel", rel, "q", g["q"], r["q"])
+    assert g["iters"] == r["iters"]
+    assert rel < 1e-9
+    assert abs(g["q"] - r["q"]) <= 1e-10 * abs(r["q"])
+    assert np.all(g["x"] >= dold) and np.all(g["x"] <= 1.0)
