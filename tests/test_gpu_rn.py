@@ -83,11 +83,12 @@ def test_reflective_newton_partial_bounds(gpu, ref):
     assert rel < 1e-9
 
 
+@pytest.mark.parametrize("order", ["natural", "multicolour"])
 @pytest.mark.parametrize("h_scale", [100.0, 1000.0])
-def test_reflective_newton_gmres_local_solver(gpu, ref, h_scale):
-    """GMRES(30)+ILU(0) in the local-solver slot: no reference counterpart, so
-    the minimiser is compared with the reference's ILU-PCG variant and with
-    L-BFGS-B."""
+def test_reflective_newton_gmres_local_solver(gpu, ref, h_scale, order):
+    """GMRES(30)+ILU(0) in the local-solver slot, the ILU in either order: no
+    reference counterpart, so the minimiser is compared with the reference's
+    ILU-PCG variant and with L-BFGS-B."""
     s, dold, ones = P.damage_system(33, h_scale=h_scale)
     delta = np.sqrt(s.n) / 2
     H = ref.matrix(s.rowptr, s.col, s.val)
@@ -95,9 +96,9 @@ def test_reflective_newton_gmres_local_solver(gpu, ref, h_scale):
                               delta=delta)
     ref.free(H)
     g = gpu.ReflectiveNewton(s.rowptr, s.col, s.val, s.b, dold, ones,
-                             gpu.rn_params("gmres", 1, 4, delta=delta)).solve(dold)
+                             gpu.rn_params("gmres", 1, 4, delta=delta, ilu_ordering=order)).solve(dold)
     rel = np.linalg.norm(g["x"] - r["x"]) / np.linalg.norm(r["x"])
-    say("  gmres iters", g["iters"], "ref pcg", r["iters"], "rel", rel, "inner", g["inner_iterations"])
+    say("  gmres", order, "iters", g["iters"], "ref pcg", r["iters"], "rel", rel, "inner", g["inner_iterations"])
     assert rel < 1e-6
     q_ref = lbfgsb_min(s, dold, ones)
     assert g["q"] <= q_ref + 1e-7 * abs(q_ref)
