@@ -92,12 +92,14 @@ def test_elasticity1025_mixed_and_fp64(gpu):
 
 
 @pytest.mark.parametrize("solver,prec,order", [("pcg", "fp64", "natural"), ("gmres", "fp64", "natural"),
-                                              ("bicgstab", "mixed", "natural"), ("pcg", "fp64", "multicolour")])
+                                              ("bicgstab", "mixed", "natural"), ("pcg", "fp64", "multicolour"),
+                                              ("gmres", "fp64", "multicolour")])
 def test_damage1025_reflective_newton_kkt(gpu, solver, prec, order):
     """The damage subproblem's reflective Newton with each inner solver:
     ILU(0)-PCG; GMRES(30) + ILU(0) (FEM.c:737-744); BiCGStab + AMG
     (reflective_newton.c:205-217) with the FP32 hierarchy (multicolour SOR,
-    the reference's AMG_solve settings); ILU(0)-PCG in multicolour order --
+    the reference's AMG_solve settings); ILU(0)-PCG and GMRES(30) with the
+    ILU(0) in multicolour order --
     the scaling, reflections and active-set tests stay FP64 in every
     variant."""
     s, dold, ones = P.damage_system(1025, h_scale=1.0)
