@@ -5,7 +5,8 @@ the oracle (itself bit-identical to the compiled reference's
 incomplete_LU_factorization, PCG_preconlr, pcg and gmres):
 
 * the permutation is the stable sort of the oracle's colouring of A's
-  pattern (explicit zeros included);
+  pattern (explicit zeros included; one case stores 15 % of the damage
+  matrix's off-diagonal values as zeros);
 * the factors are exactly the reference's incomplete LU of P A P^T, from the
   host factorisation and from the device one (a launch per level);
 * the preconditioner applied to z in A's order is P^T U^-1 L^-1 P z of the
@@ -57,11 +58,14 @@ def _permuted(rp, ci, v, perm):
 
 
 def _case(name):
+    if name == "damage257_zeros":
+        from tests.test_krylov_cpu import with_explicit_zeros
+        return with_explicit_zeros(P.damage_system(257)[0])
     return {"poisson65": P.poisson(65), "damage129": P.damage_system(129)[0],
             "elast33": P.degraded_elasticity(33), "damage257": P.damage_system(257)[0]}[name]
 
 
-@pytest.mark.parametrize("name", ["poisson65", "damage129", "elast33", "damage257"])
+@pytest.mark.parametrize("name", ["poisson65", "damage129", "elast33", "damage257", "damage257_zeros"])
 def test_multicolour_factors_and_sweeps_match_reference(gpu, orc, name):
     s = _case(name)
     ncol, col = orc.colouring(s.rowptr, s.col, np.ones(len(s.col)))

@@ -78,7 +78,18 @@ def _permuted(rp, ci, v, perm):
     return prp, np.concatenate(pci).astype(np.int32), np.concatenate(pv)
 
 
-@pytest.mark.parametrize("kind", ["poisson", "damage", "elasticity"])
+def with_explicit_zeros(s, frac=0.15, seed=2):
+    """s with the same pattern and a seeded share of its off-diagonal values
+    set to stored zeros (the ILU(0) pattern keeps them; elimination fills them)"""
+    import dataclasses
+    rows = np.repeat(np.arange(s.n), np.diff(s.rowptr))
+    off = np.flatnonzero(s.col != rows)
+    v = s.val.copy()
+    v[np.random.default_rng(seed).choice(off, int(frac * len(off)), replace=False)] = 0.0
+    return dataclasses.replace(s, val=v)
+
+
+@pytest.mark.parametrize("kind", ["poisson", "damage", "elasticity", "damage_zeros"])
 def test_multicolour_ilu_is_the_reference_ilu_of_the_permuted_matrix(ref, kind):
     """LINA_ILU_MULTICOLOUR: the permutation sorts the rows by a proper
     colouring of A's pattern, explicit zeros included (no two rows of one
@@ -87,7 +98,8 @@ def test_multicolour_ilu_is_the_reference_ilu_of_the_permuted_matrix(ref, kind):
     schedule has no more levels than there are colours."""
     from oracle.bind import Oracle
     s = {"poisson": lambda: P.poisson(33), "damage": lambda: P.damage_system(33)[0],
-         "elasticity": lambda: P.degraded_elasticity(17)}[kind]()
+         "elasticity": lambda: P.degraded_elasticity(17),
+         "damage_zeros": lambda: with_explicit_zeros(P.damage_system(33)[0])}[kind]()
     il = lina.ILU(s.rowptr, s.col, s.val, upload=False, ordering="multicolour")
     perm = il.permutation()
     ncol, col = Oracle().colouring(s.rowptr, s.col, np.ones(len(s.col)))
